@@ -1,0 +1,215 @@
+/*
+ * tecs_amd.h - C ABI of the MI355X (gfx950) per-step engine for TECS cell simulations.
+ *
+ * The engine runs the per-step hot path of TECS on one AMD GPU:
+ *   grid rebuild (device counting sort)      replaces include/utils/grid.hpp:389-401
+ *   binned / naive all-pairs finders         replace include/pair_finders/binned_all_pairs.hpp:66-183,
+ *                                                    include/pair_finders/naive_all_pairs.hpp:32-104
+ *   binned / naive Gabriel finders           replace include/pair_finders/binned_gabriel.hpp:71-362,
+ *                                                    include/pair_finders/naive_gabriel.hpp:36-120
+ *   pairwise force laws (PiecewiseLinear,    replace include/forces/predefined.hpp:107-144,
+ *     bending, polarity alignment)                   include/types/polarity.hpp:131-198
+ *   Euler / Heun integrators                 replace include/integrators/euler.hpp:49-85,
+ *                                                    include/integrators/heun.hpp:143-208
+ *   NoComFixer / GlobalComFixer              replace include/com_fixers/com_fixers.hpp:11-45
+ *
+ * Everything crossing this boundary is plain C: opaque handles, POD descriptors, host pointers
+ * and sizes. No exceptions cross it; every entry point returns a status (TECS_OK == 0) and the
+ * message of the last failure on the calling thread is available from tecs_last_error().
+ * A handle may be used by one host thread at a time.
+ *
+ * Host arrays use the reference's memory layout: positions are VectorN<float,3> (12 bytes per
+ * cell, x,y,z), polarities are Polarity_<float> (8 bytes per cell, theta,phi), types are one byte
+ * per cell. All arrays are indexed by the reference's agent index i.
+ */
+#ifndef TECS_AMD_H
+#define TECS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TECS_AMD_ABI_VERSION 1
+
+/* ---- status codes ---------------------------------------------------------------------- */
+enum {
+  TECS_OK = 0,
+  TECS_ERR_INVALID = -1,     /* bad argument or descriptor */
+  TECS_ERR_DEVICE = -2,      /* a HIP call failed */
+  TECS_ERR_OOM = -3,         /* device allocation failed */
+  TECS_ERR_UNSUPPORTED = -4, /* valid request this build does not implement */
+  TECS_ERR_STATE = -5,       /* call not valid in the current state (e.g. no grid yet) */
+  TECS_ERR_CAPACITY = -6     /* caller buffer too small; the required size is reported */
+};
+
+/* ---- component selectors (the CONFIG_* choices of include/simulation_config.hpp:212-379) -- */
+enum {
+  TECS_NAIVE_ALL_PAIRS = 0,  /* pair_finders::NaiveAllPairs  */
+  TECS_BINNED_ALL_PAIRS = 1, /* pair_finders::BinnedAllPairs */
+  TECS_NAIVE_GABRIEL = 2,    /* pair_finders::NaiveGabriel   */
+  TECS_BINNED_GABRIEL = 3    /* pair_finders::BinnedGabriel  */
+};
+enum { TECS_EULER = 0, TECS_HEUN = 1 };                        /* integrators::Euler / Heun */
+enum { TECS_NO_COM_FIXER = 0, TECS_GLOBAL_COM_FIXER = 1 };     /* com_fixers::*             */
+
+/* Optional per-cell fields besides the position (CONFIG_FIELDS). */
+#define TECS_FIELD_POLARITY 1u /* Polarity_<float> field, integrated like the position */
+#define TECS_FIELD_TYPE 2u     /* read-only uint8 cell type, selects the adhesion table entry */
+
+/* PairFinder::Settings of the binned and Gabriel finders (binned_all_pairs.hpp:20-25,
+ * binned_gabriel.hpp:20-26, naive_gabriel.hpp:22-24). Naive all-pairs ignores all fields. */
+typedef struct tecs_pair_finder_settings {
+  float min_bounds[3];       /* default -20 */
+  float max_bounds[3];       /* default +20 */
+  float bin_size_scale;      /* bin edge = bin_size_scale * cutoff_distance; default 1 */
+  int32_t rebuild_every_n;   /* grid rebuild period in full steps; 0 = every step (default) */
+  float gabriel_coefficient; /* Gabriel ball diameter factor c; default 0.8 */
+} tecs_pair_finder_settings;
+
+#define TECS_MAX_TYPES 4
+
+/* The fused pairwise force, evaluated per ordered pair (i, j) in this order:
+ *   position.delta += PiecewiseLinear(disp, d, homeostatic_radius, repulsion_scale, k_adh)
+ *        with k_adh = adhesion_table[type_i * n_types + type_j] when n_types > 0,
+ *        else adhesion_scale                                   (predefined.hpp:125-144)
+ *   if bending_strength != 0:   b = polarity_i.bending_force(disp, polarity_j, d)
+ *        position.delta += bending_strength * b.vector
+ *        polarity.delta += bending_strength * b.polarity        (polarity.hpp:181-198)
+ *   if alignment_strength != 0:
+ *        polarity.delta += alignment_strength *
+ *             polarity_i.bidirectional_polarization_force(polarity_j)   (polarity.hpp:165-168)
+ * With a strength of exactly 1 the product is exact, so the term equals the reference's plain
+ * `delta += term`. Polarity terms need TECS_FIELD_POLARITY, a type table needs TECS_FIELD_TYPE.
+ * No term writes drag, so the drag/velocity correction of the reference adds +0. */
+typedef struct tecs_force_desc {
+  float homeostatic_radius;
+  float repulsion_scale;
+  float adhesion_scale;
+  int32_t n_types; /* 0 = no type table, else 1..TECS_MAX_TYPES */
+  float adhesion_table[TECS_MAX_TYPES * TECS_MAX_TYPES];
+  float bending_strength;
+  float alignment_strength;
+} tecs_force_desc;
+
+/* Everything Simulation<Config>(Settings) needs for this path (simulation.hpp:76-98, 133-182). */
+typedef struct tecs_sim_desc {
+  uint32_t agent_count;     /* active cells */
+  uint32_t capacity;        /* allocated cells, >= agent_count (GlobalComFixer divides by it) */
+  uint32_t fields;          /* TECS_FIELD_* bit mask */
+  float cutoff_distance;    /* r_max */
+  int32_t pair_finder;      /* TECS_*_ALL_PAIRS / TECS_*_GABRIEL */
+  tecs_pair_finder_settings pair_finder_settings;
+  int32_t integrator;       /* TECS_EULER / TECS_HEUN */
+  int32_t com_fixer;        /* TECS_NO_COM_FIXER / TECS_GLOBAL_COM_FIXER */
+  tecs_force_desc force;
+  int32_t track_old_velocities; /* 1 = keep Base::old_velocities (base.hpp:74) like the reference */
+} tecs_sim_desc;
+
+/* Counters of a simulation or batch since creation (or the last reset). */
+typedef struct tecs_stats {
+  uint64_t pair_interactions; /* force-functor invocations on ordered pairs, all evaluations */
+  uint64_t evaluations;       /* pair-finder evaluations (Heun: 2 per step) */
+  uint64_t rebuilds;          /* grid rebuilds */
+  uint64_t steps;             /* integrator steps */
+} tecs_stats;
+
+/* Fill the reference defaults (bounds +-20, bin scale 1, rebuild every step, c = 0.8). */
+void tecs_default_pair_finder_settings(tecs_pair_finder_settings* settings);
+/* Reference defaults for a position-only Heun simulation with NoComFixer and
+ * RelaxForce's PiecewiseLinear(0.8, 2, 1) (initializers/relax_force.hpp:27-29). */
+void tecs_default_sim_desc(tecs_sim_desc* desc);
+
+int tecs_abi_version(void);
+const char* tecs_last_error(void);
+/* Number of visible HIP devices (0 without a GPU); never initialises more than the runtime. */
+int tecs_device_count(int32_t* count);
+
+/* =========================================================================================
+ * Single simulation: the drop-in for Simulation<Config>::take_step (simulation.hpp:548-556)
+ * and the pair finder / integrator / com fixer it drives.
+ * ========================================================================================= */
+typedef struct tecs_sim tecs_sim;
+
+/* Allocates all device state on the current HIP device. `stream` is a hipStream_t or NULL
+ * (then the simulation creates its own stream at default priority). */
+int tecs_sim_create(const tecs_sim_desc* desc, void* stream, tecs_sim** out);
+int tecs_sim_destroy(tecs_sim* sim);
+
+/* Copy host state in (positions: agent_count*3 floats; polarities: agent_count*2 floats or NULL
+ * when the field is absent; types: agent_count bytes or NULL). Keeps the current grid, so a
+ * following non-full evaluation sees a stale grid exactly like the reference's Heun corrector. */
+int tecs_sim_upload(tecs_sim* sim, const float* positions, const float* polarities,
+                    const uint8_t* types);
+/* Copy state out; any pointer may be NULL. old_velocities: agent_count*3 floats. */
+int tecs_sim_download(tecs_sim* sim, float* positions, float* polarities, float* old_velocities);
+
+/* n_steps integrator steps of size dt (Integrator::integrate, euler.hpp:81-85 / heun.hpp:201-208).
+ * Asynchronous on the simulation's stream. */
+int tecs_sim_take_step(tecs_sim* sim, double dt, int32_t n_steps);
+
+/* One pair-finder evaluation on the current state into zeroed deltas, exactly as
+ * PairFinder::evaluate_force(in, out, ..., is_full_step) (binned_all_pairs.hpp:66-183):
+ * a full evaluation rebuilds the grid when due and advances the rebuild counter.
+ * dpos: agent_count*3 floats, dpol: agent_count*2 floats or NULL. Synchronous. */
+int tecs_sim_evaluate_force(tecs_sim* sim, int32_t is_full_step, float* dpos, float* dpol);
+
+/* The ordered pairs (i, j) such an evaluation would hand to the force functor, as CSR over i:
+ * row_ptr[agent_count+1], cols[n_pairs], each row in the reference's visiting order
+ * (neighbour bins x-fastest, members in ascending agent index). Rebuild semantics as
+ * tecs_sim_evaluate_force. Call with cols == NULL to obtain n_pairs only (nothing advances
+ * then); returns TECS_ERR_CAPACITY when cols_capacity < n_pairs. Synchronous. */
+int tecs_sim_pairs(tecs_sim* sim, int32_t is_full_step, int64_t* row_ptr, int32_t* cols,
+                   int64_t cols_capacity, int64_t* n_pairs);
+
+/* Rebuild the grid at the current positions (Grid::rebuild, grid.hpp:389-401) and reset the
+ * rebuild counter as a due full step would. */
+int tecs_sim_rebuild(tecs_sim* sim);
+
+int tecs_sim_synchronize(tecs_sim* sim);
+int tecs_sim_get_stats(tecs_sim* sim, tecs_stats* stats);
+int tecs_sim_reset_stats(tecs_sim* sim);
+
+/* Kernel timing with HIP events recorded on the simulation's stream around every launch of a
+ * kernel class. Enabling adds two events per timed launch. */
+enum {
+  TECS_KT_SWEEP_FULL = 0,  /* pair sweep of a full evaluation (Heun substep 1, Euler) */
+  TECS_KT_SWEEP_STALE = 1, /* pair sweep of the Heun corrector evaluation (stale grid) */
+  TECS_KT_REBUILD = 2      /* the whole grid rebuild (keys, scan, scatter, gather) */
+};
+int tecs_sim_set_kernel_timing(tecs_sim* sim, int32_t enabled);
+/* Summed duration in ms and number of timed launches of one class since creation. */
+int tecs_sim_get_kernel_timing(tecs_sim* sim, int32_t kernel_class, double* ms,
+                               int64_t* launches);
+
+/* =========================================================================================
+ * Batch: many independent simulations advanced together, one launch per kernel per substep
+ * for all of them. All members share pair finder, integrator, com fixer, fields, old-velocity
+ * tracking and rebuild period; agent counts, bounds, cutoffs, Gabriel coefficients, force
+ * parameters and dt may differ. Each member's results are those of the same simulation run
+ * alone through tecs_sim. This has no counterpart in the reference, which runs one
+ * Simulation per call of take_step (simulation.hpp:548-556).
+ * ========================================================================================= */
+typedef struct tecs_batch tecs_batch;
+int tecs_batch_create(const tecs_sim_desc* descs, int32_t count, void* stream, tecs_batch** out);
+int tecs_batch_destroy(tecs_batch* batch);
+int tecs_batch_upload(tecs_batch* batch, int32_t index, const float* positions,
+                      const float* polarities, const uint8_t* types);
+int tecs_batch_download(tecs_batch* batch, int32_t index, float* positions, float* polarities,
+                        float* old_velocities);
+/* dts: one dt per member. */
+int tecs_batch_take_step(tecs_batch* batch, const double* dts, int32_t n_steps);
+int tecs_batch_synchronize(tecs_batch* batch);
+int tecs_batch_get_stats(tecs_batch* batch, tecs_stats* stats);
+int tecs_batch_reset_stats(tecs_batch* batch);
+int tecs_batch_set_kernel_timing(tecs_batch* batch, int32_t enabled);
+int tecs_batch_get_kernel_timing(tecs_batch* batch, int32_t kernel_class, double* ms,
+                                 int64_t* launches);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* TECS_AMD_H */

@@ -1,0 +1,159 @@
+"""ctypes wrapper of the CPU oracle (oracle/tecs_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by the tests, the smoke check and the benchmark's CPU
+baseline leg as the checker. The engine never imports it.
+
+OracleSim mirrors tecs_amd.engine.GpuSimulation method for method, so a parity test runs the
+same calls against both and compares the results.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from tecs_amd import abi
+
+HERE = Path(__file__).resolve().parent
+SRC = HERE / "tecs_oracle.cpp"
+LIB_PATH = HERE / "_build" / "libtecs_oracle.so"
+
+# -ffp-contract=off: every operation of the reference expressions rounds once (no FMA);
+# no -ffast-math, no -march=native (the x86-64 baseline has no FMA instruction anyway).
+CXXFLAGS = ["-std=c++17", "-O2", "-fopenmp", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared"]
+
+_lib = None
+
+
+def build() -> Path:
+    LIB_PATH.parent.mkdir(exist_ok=True)
+    tmp = LIB_PATH.with_suffix(".so.tmp")
+    cmd = [os.environ.get("CXX", "g++"), *CXXFLAGS, "-o", str(tmp), str(SRC)]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + r.stdout.decode(errors="replace"))
+    os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        build()
+    lib = C.CDLL(str(LIB_PATH))
+    P = C.c_void_p
+    sig = {
+        "orc_last_error": (C.c_char_p, []),
+        "orc_set_threads": (C.c_int, [C.c_int]),
+        "orc_create": (C.c_int, [C.POINTER(abi.SimDesc), C.POINTER(P)]),
+        "orc_destroy": (C.c_int, [P]),
+        "orc_upload": (C.c_int, [P, P, P, P]),
+        "orc_download": (C.c_int, [P, P, P, P]),
+        "orc_take_step": (C.c_int, [P, C.c_double, C.c_int]),
+        "orc_rebuild": (C.c_int, [P]),
+        "orc_evaluate_force": (C.c_int, [P, C.c_int, P, P]),
+        "orc_pairs": (C.c_int, [P, C.c_int, P, P, C.c_int64, C.POINTER(C.c_int64)]),
+        "orc_get_stats": (C.c_int, [P, C.POINTER(abi.Stats)]),
+        "orc_forces_f64": (C.c_int, [P, P, P, P, P, P, P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError(f"oracle error {rc}: {load().orc_last_error().decode()}")
+
+
+def set_threads(n: int) -> int:
+    return int(load().orc_set_threads(int(n)))
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+class OracleSim:
+    def __init__(self, desc: abi.SimDesc):
+        self._lib = load()
+        self.desc = abi.copy_desc(desc)
+        self.n = int(desc.agent_count)
+        self.has_pol = bool(desc.fields & abi.FIELD_POLARITY)
+        self.has_type = bool(desc.fields & abi.FIELD_TYPE)
+        h = C.c_void_p()
+        _check(self._lib.orc_create(C.byref(self.desc), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.orc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, positions, polarities=None, types=None):
+        p = np.ascontiguousarray(positions, np.float32).reshape(self.n, 3)
+        q = np.ascontiguousarray(polarities, np.float32).reshape(self.n, 2) if polarities is not None else None
+        t = np.ascontiguousarray(types, np.uint8).reshape(self.n) if types is not None else None
+        _check(self._lib.orc_upload(self._h, _ptr(p), _ptr(q), _ptr(t)))
+
+    def download(self):
+        pos = np.empty((self.n, 3), np.float32)
+        pol = np.empty((self.n, 2), np.float32) if self.has_pol else None
+        ov = np.empty((self.n, 3), np.float32)
+        _check(self._lib.orc_download(self._h, _ptr(pos), _ptr(pol), _ptr(ov)))
+        return pos, pol, ov
+
+    def take_step(self, dt: float, n_steps: int = 1):
+        _check(self._lib.orc_take_step(self._h, float(dt), int(n_steps)))
+
+    def rebuild(self):
+        _check(self._lib.orc_rebuild(self._h))
+
+    def evaluate_force(self, is_full_step: bool = True):
+        dpos = np.empty((self.n, 3), np.float32)
+        dpol = np.empty((self.n, 2), np.float32) if self.has_pol else None
+        _check(self._lib.orc_evaluate_force(self._h, 1 if is_full_step else 0, _ptr(dpos), _ptr(dpol)))
+        return dpos, dpol
+
+    def pairs(self, is_full_step: bool = True):
+        count = C.c_int64(0)
+        _check(self._lib.orc_pairs(self._h, 1 if is_full_step else 0, None, None, 0, C.byref(count)))
+        row_ptr = np.empty(self.n + 1, np.int64)
+        cols = np.empty(max(count.value, 1), np.int32)
+        _check(
+            self._lib.orc_pairs(self._h, 1 if is_full_step else 0, _ptr(row_ptr), _ptr(cols), cols.size, C.byref(count))
+        )
+        return row_ptr, cols[: count.value]
+
+    def stats(self) -> dict:
+        s = abi.Stats()
+        _check(self._lib.orc_get_stats(self._h, C.byref(s)))
+        return {k: int(getattr(s, k)) for k, _ in abi.Stats._fields_}
+
+    def forces_f64(self, row_ptr, cols):
+        """FP64 forces of the current stage-0 state over the given pair set, and per component
+        the sum of absolute pair contributions."""
+        rp = np.ascontiguousarray(row_ptr, np.int64)
+        cc = np.ascontiguousarray(cols, np.int32)
+        if cc.size == 0:
+            cc = np.zeros(1, np.int32)
+        dpos = np.empty((self.n, 3), np.float64)
+        spos = np.empty((self.n, 3), np.float64)
+        dpol = np.empty((self.n, 2), np.float64)
+        spol = np.empty((self.n, 2), np.float64)
+        _check(self._lib.orc_forces_f64(self._h, _ptr(rp), _ptr(cc), _ptr(dpos), _ptr(dpol), _ptr(spos), _ptr(spol)))
+        return dpos, dpol, spos, spol

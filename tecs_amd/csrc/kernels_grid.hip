@@ -1,0 +1,231 @@
+// kernels_grid.hip - grid rebuild as a deterministic device counting sort, plus state I/O.
+//
+// Grid::rebuild (grid.hpp:389-401) computes one bin key per cell and Kokkos::BinSort orders the
+// cells by key; on the Serial backend the members of a bin end up in ascending agent index.
+// The device version reproduces that order exactly and moves the cell state into it:
+//   1. keys:    key = flatten(clamp(int((p - min) / bin)))  and  cnt[key] += 1   (atomics)
+//   2. scan:    off = exclusive_scan(cnt) over all instances' bins (+ pad bins)
+//   3. scatter: slot = off[key] + (atomic decrement of cnt[key]) - 1 -> tmp; this leaves cnt at
+//               zero for the next rebuild, so no clearing pass is needed
+//   4. gather:  each cell's final slot is off[key] + #(members of its bin with a smaller agent
+//               index); its state (position, polarity, id, old velocity) moves there.
+// Step 3's order within a bin is arbitrary, step 4 makes it canonical, so the result does not
+// depend on atomic arrival order.
+#include "tecs_internal.hpp"
+
+namespace tecs_amd {
+namespace {
+
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;
+
+__global__ __launch_bounds__(kBlock) void k_bin_keys(RebuildArgs a) {
+  const int lb = logical_block();
+  const InstanceDesc& in = a.inst[a.block_inst[lb]];
+  const int s = lb * kBlock + (int)threadIdx.x;
+  const int local = s - in.slot_begin;
+  if (local < in.n) {
+    const float4 p = a.x_old[s];
+    int c[3];
+    bin_coords_div(in, p.x, p.y, p.z, c);  // grid.hpp:394-395
+    const int key = flatten(in, c[0], c[1], c[2]);
+    a.keys[s] = key;
+    atomicAdd(&a.cnt[in.bin_begin + key], 1);
+  } else if (local == in.n) {
+    // the pad bin holds the instance's pad slots, so the global scan lands on slot_begin of
+    // the next instance
+    a.cnt[in.bin_begin + in.bin_count] = in.padded - in.n;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scatter(RebuildArgs a) {
+  const int lb = logical_block();
+  const InstanceDesc& in = a.inst[a.block_inst[lb]];
+  const int s = lb * kBlock + (int)threadIdx.x;
+  const int local = s - in.slot_begin;
+  if (local < in.n) {
+    const int g = in.bin_begin + a.keys[s];
+    const int pos = a.off[g] + atomicSub(&a.cnt[g], 1) - 1;
+    a.tmp[pos] = s;
+  } else if (local == in.n) {
+    a.cnt[in.bin_begin + in.bin_count] = 0;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather(RebuildArgs a) {
+  const int lb = logical_block();
+  const InstanceDesc& in = a.inst[a.block_inst[lb]];
+  const int t = lb * kBlock + (int)threadIdx.x;
+  if ((t - in.slot_begin) >= in.n) return;
+  const int o = a.tmp[t];
+  const int g = in.bin_begin + a.keys[o];
+  const int start = a.off[g], end = a.off[g + 1];
+  const int my_id = a.id_old[o];
+  int rank = 0;
+  for (int u = start; u < end; ++u) rank += (a.id_old[a.tmp[u]] < my_id) ? 1 : 0;
+  const int dst = start + rank;
+  a.id_new[dst] = my_id;
+  a.x_new[dst] = a.x_old[o];
+  if (a.pol_new) a.pol_new[dst] = a.pol_old[o];
+  if (a.ov_new) a.ov_new[dst] = a.ov_old[o];
+}
+
+// ------------------------------------------------------------------ exclusive scan (int)
+__device__ __forceinline__ int block_exclusive_scan(int v, int* lds_w, int* total) {
+  // inclusive scan inside the wavefront
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  __syncthreads();
+  if (lane == 63) lds_w[w] = x;
+  __syncthreads();
+  int wave_off = 0;
+  for (int k = 0; k < w; ++k) wave_off += lds_w[k];
+  *total = lds_w[0] + lds_w[1] + lds_w[2] + lds_w[3];
+  return wave_off + x - v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(const int* in, int n, int* bsum) {
+  __shared__ int lds[4];
+  const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  int v = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k)
+    if (base + k < n) v += in[base + k];
+  int total;
+  block_exclusive_scan(v, lds, &total);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+// Single workgroup: exclusive scan of the tile sums in place; writes the grand total.
+__global__ __launch_bounds__(kBlock) void k_scan_top(int* bsum, int nt, int* total_out) {
+  __shared__ int lds[4];
+  int carry = 0;
+  for (int base = 0; base < nt; base += kBlock) {
+    const int i = base + threadIdx.x;
+    const int v = i < nt ? bsum[i] : 0;
+    int total;
+    const int ex = block_exclusive_scan(v, lds, &total);
+    if (i < nt) bsum[i] = carry + ex;
+    carry += total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total_out = carry;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_apply(const int* in, int n, const int* bsum,
+                                                       int* out) {
+  __shared__ int lds[4];
+  const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  int v[kScanItems];
+  int sum = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    v[k] = (base + k < n) ? in[base + k] : 0;
+    sum += v[k];
+  }
+  int total;
+  int run = bsum[blockIdx.x] + block_exclusive_scan(sum, lds, &total);
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    if (base + k < n) out[base + k] = run;
+    run += v[k];
+  }
+}
+
+// ------------------------------------------------------------------------ state I/O
+__global__ __launch_bounds__(kBlock) void k_init_ids(const InstanceDesc* inst,
+                                                     const int* block_inst, int* id) {
+  const int lb = blockIdx.x;
+  const InstanceDesc& in = inst[block_inst[lb]];
+  const int s = lb * kBlock + (int)threadIdx.x;
+  const int local = s - in.slot_begin;
+  id[s] = local < in.n ? local : -1;
+}
+
+// Host arrays are in agent order; slot s holds agent id[s].
+__global__ __launch_bounds__(kBlock) void k_upload(const InstanceDesc* inst, int ii,
+                                                   int block_begin, const int* id,
+                                                   const float* pos, const float* pol,
+                                                   const uint8_t* type, float4* x, float2* p) {
+  const InstanceDesc& in = inst[ii];
+  const int s = (block_begin + (int)blockIdx.x) * kBlock + (int)threadIdx.x;
+  if ((s - in.slot_begin) >= in.n) return;
+  const int i = id[s];
+  float w = x[s].w;
+  if (type) w = __int_as_float((int)type[i]);
+  x[s] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], w);
+  if (pol) p[s] = make_float2(pol[2 * i], pol[2 * i + 1]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_download(const InstanceDesc* inst, int ii,
+                                                     int block_begin, const int* id,
+                                                     const float4* x, const float2* p,
+                                                     const float4* ov, float* pos, float* pol,
+                                                     float* old_vel) {
+  const InstanceDesc& in = inst[ii];
+  const int s = (block_begin + (int)blockIdx.x) * kBlock + (int)threadIdx.x;
+  if ((s - in.slot_begin) >= in.n) return;
+  const int i = id[s];
+  if (pos) {
+    const float4 v = x[s];
+    pos[3 * i] = v.x;
+    pos[3 * i + 1] = v.y;
+    pos[3 * i + 2] = v.z;
+  }
+  if (pol) {
+    const float2 v = p[s];
+    pol[2 * i] = v.x;
+    pol[2 * i + 1] = v.y;
+  }
+  if (old_vel) {
+    const float4 v = ov[s];
+    old_vel[3 * i] = v.x;
+    old_vel[3 * i + 1] = v.y;
+    old_vel[3 * i + 2] = v.z;
+  }
+}
+
+}  // namespace
+
+int scan_tiles_for(int n) { return (n + kScanTile - 1) / kScanTile; }
+
+// out[0..n] = exclusive scan of in[0..n), out[n] = total. bsum needs scan_tiles_for(n) ints.
+void launch_exclusive_scan(const int* in, int n, int* out, int* bsum, hipStream_t stream) {
+  const int nt = scan_tiles_for(n);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nt), dim3(kBlock), 0, stream, in, n, bsum);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kBlock), 0, stream, bsum, nt, out + n);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(kBlock), 0, stream, in, n, bsum, out);
+}
+
+void launch_rebuild(const RebuildArgs& a, int n_blocks, hipStream_t stream) {
+  hipLaunchKernelGGL(k_bin_keys, dim3(n_blocks), dim3(kBlock), 0, stream, a);
+  launch_exclusive_scan(a.cnt, a.total_bins, a.off, a.scan_bsum, stream);
+  hipLaunchKernelGGL(k_scatter, dim3(n_blocks), dim3(kBlock), 0, stream, a);
+  hipLaunchKernelGGL(k_gather, dim3(n_blocks), dim3(kBlock), 0, stream, a);
+}
+
+void launch_init_ids(const InstanceDesc* inst, const int* block_inst, int* id, int n_blocks,
+                     hipStream_t stream) {
+  hipLaunchKernelGGL(k_init_ids, dim3(n_blocks), dim3(kBlock), 0, stream, inst, block_inst, id);
+}
+
+void launch_upload(const InstanceDesc* inst, int ii, int block_begin, int n_blocks, const int* id,
+                   const float* pos, const float* pol, const uint8_t* type, float4* x, float2* p,
+                   hipStream_t stream) {
+  hipLaunchKernelGGL(k_upload, dim3(n_blocks), dim3(kBlock), 0, stream, inst, ii, block_begin, id,
+                     pos, pol, type, x, p);
+}
+
+void launch_download(const InstanceDesc* inst, int ii, int block_begin, int n_blocks,
+                     const int* id, const float4* x, const float2* p, const float4* ov,
+                     float* pos, float* pol, float* old_vel, hipStream_t stream) {
+  hipLaunchKernelGGL(k_download, dim3(n_blocks), dim3(kBlock), 0, stream, inst, ii, block_begin,
+                     id, x, p, ov, pos, pol, old_vel);
+}
+
+}  // namespace tecs_amd

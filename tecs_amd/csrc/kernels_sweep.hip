@@ -1,0 +1,426 @@
+// kernels_sweep.hip - pair-finder sweeps with the fused force and integrator updates.
+//
+// One thread per slot (cell i). The binned sweep walks the neighbour bins of i in the order of
+// BinnedAllPairs / BinnedGabriel (binned_all_pairs.hpp:116-171, binned_gabriel.hpp:108-227):
+// z outer, y middle, x inner. Because slots are sorted by (bin, agent index), the x-run of
+// neighbour bins in one (y, z) row is one contiguous slot range, visited front to back, which
+// is exactly the reference's sequence of (bin, member) visits. Neighbouring threads of a
+// wavefront hold neighbouring cells, so their neighbour ranges overlap and are served from
+// L1/L2. The per-cell sum is accumulated in that visiting order, so it is bitwise the serial
+// reference sum.
+//
+// Stale-grid semantics (binned_all_pairs.hpp:77-84, heun.hpp:206): the Heun corrector sweep
+// reads predicted positions while the slot order and bin offsets stay those of the last
+// rebuild; the centre bin of i is recomputed from its current position, exactly like the
+// reference.
+#include "tecs_internal.hpp"
+
+namespace tecs_amd {
+namespace {
+
+// Per-pair visit context; `Acc` carries the force accumulators of cell i.
+struct CellState {
+  float acc_pos[3];
+  float acc_pol[2];
+  unsigned pairs;
+  int64_t emit_at;
+};
+
+template <bool HAS_TYPE, bool HAS_POL, int MODE>
+__device__ __forceinline__ void visit(const SweepArgs& a, const InstanceDesc& in, CellState& cs,
+                                      float4 me, float2 mp, int m, float4 pj, const float* disp,
+                                      float d2) {
+  cs.pairs++;
+  if (MODE == kModeCount) return;
+  if (MODE == kModeEmit) {
+    a.cols[cs.emit_at++] = a.id[m];
+    return;
+  }
+  const float d = sqrtf(d2);
+  float tj = 0.0f, pj_phi = 0.0f;
+  if (HAS_POL) {
+    const float2 q = a.ps[m];
+    tj = q.x;
+    pj_phi = q.y;
+  }
+  apply_force<HAS_TYPE, HAS_POL>(in.force, disp, d, __float_as_int(me.w), __float_as_int(pj.w),
+                                 mp.x, mp.y, tj, pj_phi, cs.acc_pos, cs.acc_pol);
+}
+
+// Gabriel emptiness test of the ball with diameter c*|p_i - p_j| around the midpoint over the
+// members of the (stale) grid bins overlapping its bounding box, binned_gabriel.hpp:133-209.
+__device__ __forceinline__ bool gabriel_blocked(const InstanceDesc& in, const float4* __restrict__ xs,
+                                                const int* __restrict__ offb, int s, int m,
+                                                float4 me, const float* disp, float d2) {
+  const float mid0 = me.x - disp[0] * 0.5f;
+  const float mid1 = me.y - disp[1] * 0.5f;
+  const float mid2 = me.z - disp[2] * 0.5f;
+  const float rho2 = d2 * in.gabriel_factor;
+  const float rho = sqrtf(rho2);
+  const float mid[3] = {mid0, mid1, mid2};
+  int mn[3], mx[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    mn[d] = clampi((int)floorf((mid[d] - rho - in.minb[d]) * in.inv_bin), in.ext[d] - 1);
+    mx[d] = clampi((int)floorf((mid[d] + rho - in.minb[d]) * in.inv_bin), in.ext[d] - 1);
+  }
+  for (int b2 = mn[2]; b2 <= mx[2]; ++b2) {
+    for (int b1 = mn[1]; b1 <= mx[1]; ++b1) {
+      const int start = offb[flatten(in, mn[0], b1, b2)];
+      const int end = offb[flatten(in, mx[0], b1, b2) + 1];
+      for (int k = start; k < end; ++k) {
+        if (k == s || k == m) continue;
+        const float4 pk = xs[k];
+        if (len2(pk.x - mid0, pk.y - mid1, pk.z - mid2) < rho2) return true;
+      }
+    }
+  }
+  return false;
+}
+
+template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE>
+__device__ __forceinline__ void walk_binned(const SweepArgs& a, const InstanceDesc& in, int s,
+                                            float4 me, float2 mp, CellState& cs) {
+  const float4* __restrict__ xs = a.xs;
+  const int* __restrict__ offb = a.off + in.bin_begin;
+  int bc[3];
+  if (GABRIEL)
+    bin_coords_div(in, me.x, me.y, me.z, bc);  // binned_gabriel.hpp:101
+  else
+    bin_coords_mul(in, me.x, me.y, me.z, bc);  // binned_all_pairs.hpp:103-109
+  const int R = in.search_radius;
+  const int x_lo = max(bc[0] - R, 0);
+  const int x_hi = min(bc[0] + R, in.ext[0] - 1);
+  const float r2 = in.cutoff2;
+  for (int dz = -R; dz <= R; ++dz) {
+    const int z = bc[2] + dz;
+    if (z < 0 || z >= in.ext[2]) continue;
+    for (int dy = -R; dy <= R; ++dy) {
+      const int y = bc[1] + dy;
+      if (y < 0 || y >= in.ext[1]) continue;
+      const int start = offb[flatten(in, x_lo, y, z)];
+      const int end = offb[flatten(in, x_hi, y, z) + 1];
+      for (int m = start; m < end; ++m) {
+        if (m == s) continue;
+        const float4 pj = xs[m];
+        const float disp[3] = {me.x - pj.x, me.y - pj.y, me.z - pj.z};
+        const float d2 = len2(disp[0], disp[1], disp[2]);
+        if (d2 >= r2) continue;
+        if (GABRIEL) {
+          if (gabriel_blocked(in, xs, offb, s, m, me, disp, d2)) continue;
+        }
+        visit<HAS_TYPE, HAS_POL, MODE>(a, in, cs, me, mp, m, pj, disp, d2);
+      }
+    }
+  }
+}
+
+// Deterministic block sum of one float per thread (fixed butterfly + fixed wave order).
+__device__ __forceinline__ float block_sum_f(float v, float* lds) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = v + __shfl_xor(v, o, 64);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) lds[w] = v;
+  __syncthreads();
+  float r = 0.0f;
+  if (threadIdx.x == 0) r = ((lds[0] + lds[1]) + lds[2]) + lds[3];
+  return r;
+}
+__device__ __forceinline__ unsigned block_sum_u(unsigned v, unsigned* lds) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) lds[w] = v;
+  __syncthreads();
+  unsigned r = 0;
+  if (threadIdx.x == 0) r = lds[0] + lds[1] + lds[2] + lds[3];
+  return r;
+}
+
+// Epilogue shared by the binned and naive sweeps: turn the accumulators into the evaluation
+// result and apply the mode's integrator update.
+template <bool HAS_POL, int MODE>
+__device__ __forceinline__ void finish(const SweepArgs& a, const InstanceDesc& in, int lb, int s,
+                                       bool active, float4 me, float2 mp, CellState& cs) {
+  __shared__ float lds_f[4];
+  __shared__ unsigned lds_u[4];
+  if (MODE == kModeCount) {
+    if (active) a.row_count[a.id[s]] = (int)cs.pairs;
+    return;
+  }
+  if (MODE == kModeEmit) return;
+  // out(i) += total (the delta buffer is zero before an evaluation), then
+  // out.pos(i) += total_velocity / fmax(total_drag, eps) which is +0 because no force term
+  // writes drag (binned_all_pairs.hpp:174-180).
+  float F[3], G[2];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) F[c] = (0.0f + cs.acc_pos[c]) + 0.0f;
+  G[0] = 0.0f + cs.acc_pol[0];
+  G[1] = 0.0f + cs.acc_pol[1];
+  const float dt = in.dt;
+  if (active) {
+    if (MODE == kModeDelta) {
+      a.o_pos[s] = make_float4(F[0], F[1], F[2], 0.0f);
+      if (HAS_POL) a.o_pol[s] = make_float2(G[0], G[1]);
+    } else if (MODE == kModePred) {
+      // apply_euler_predictor, heun.hpp:146-156: predicted = current + delta * dt
+      a.o_pos[s] = make_float4(F[0], F[1], F[2], 0.0f);
+      a.o2_pos[s] = make_float4(me.x + F[0] * dt, me.y + F[1] * dt, me.z + F[2] * dt, me.w);
+      if (HAS_POL) {
+        a.o_pol[s] = make_float2(G[0], G[1]);
+        a.o2_pol[s] = make_float2(mp.x + G[0] * dt, mp.y + G[1] * dt);
+      }
+    } else if (MODE == kModeCorr) {
+      // apply_heun_corrector, heun.hpp:170-186
+      const float4 d1 = a.d1_pos[s];
+      const float s0 = d1.x + F[0], s1 = d1.y + F[1], s2 = d1.z + F[2];
+      if (a.old_vel) a.old_vel[s] = make_float4(s0 * 0.5f, s1 * 0.5f, s2 * 0.5f, 0.0f);
+      const float4 x0 = a.o2_pos[s];
+      a.o2_pos[s] = make_float4(x0.x + s0 * 0.5f * dt, x0.y + s1 * 0.5f * dt,
+                                x0.z + s2 * 0.5f * dt, x0.w);
+      if (HAS_POL) {
+        const float2 e1 = a.d1_pol[s];
+        const float2 p0 = a.o2_pol[s];
+        a.o2_pol[s] = make_float2(p0.x + (e1.x + G[0]) * 0.5f * dt,
+                                  p0.y + (e1.y + G[1]) * 0.5f * dt);
+      }
+    } else if (MODE == kModeEuler) {
+      // apply_euler, euler.hpp:52-68
+      if (a.old_vel) a.old_vel[s] = make_float4(F[0], F[1], F[2], 0.0f);
+      a.o_pos[s] = make_float4(me.x + F[0] * dt, me.y + F[1] * dt, me.z + F[2] * dt, me.w);
+      if (HAS_POL) a.o_pol[s] = make_float2(mp.x + G[0] * dt, mp.y + G[1] * dt);
+    }
+  }
+  if (MODE == kModeDelta && a.com_partial) {
+    const float f0 = active ? F[0] : 0.0f, f1 = active ? F[1] : 0.0f, f2 = active ? F[2] : 0.0f;
+    const float t0 = block_sum_f(f0, lds_f);
+    const float t1 = block_sum_f(f1, lds_f);
+    const float t2 = block_sum_f(f2, lds_f);
+    if (threadIdx.x == 0) a.com_partial[lb] = make_float4(t0, t1, t2, 0.0f);
+  }
+  const unsigned tp = block_sum_u(active ? cs.pairs : 0u, lds_u);
+  if (threadIdx.x == 0) a.block_pairs[lb] += tp;
+}
+
+template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE>
+__global__ __launch_bounds__(kBlock) void k_sweep_binned(SweepArgs a) {
+  const int lb = logical_block();
+  const int ii = a.block_inst[lb];
+  const InstanceDesc& in = a.inst[ii];
+  const int s = lb * kBlock + (int)threadIdx.x;
+  const bool active = (s - in.slot_begin) < in.n;
+  CellState cs{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f}, 0u, 0};
+  float4 me = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float2 mp = make_float2(0.0f, 0.0f);
+  if (active) {
+    me = a.xs[s];
+    if (HAS_POL) mp = a.ps[s];
+    if (MODE == kModeEmit) cs.emit_at = a.row_ptr[a.id[s]];
+    walk_binned<GABRIEL, HAS_TYPE, HAS_POL, MODE>(a, in, s, me, mp, cs);
+  }
+  finish<HAS_POL, MODE>(a, in, lb, s, active, me, mp, cs);
+}
+
+// Naive finders (naive_all_pairs.hpp:41-103, naive_gabriel.hpp:47-119): j runs over all agents
+// in ascending order; tiles of j positions are staged in LDS and shared by the workgroup.
+// Slots of a naive-finder instance are never re-sorted, so slot order is agent order.
+template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE>
+__global__ __launch_bounds__(kBlock) void k_sweep_naive(SweepArgs a) {
+  __shared__ float4 tile[kBlock];
+  const int lb = logical_block();
+  const int ii = a.block_inst[lb];
+  const InstanceDesc& in = a.inst[ii];
+  const int s = lb * kBlock + (int)threadIdx.x;
+  const bool active = (s - in.slot_begin) < in.n;
+  CellState cs{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f}, 0u, 0};
+  float4 me = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float2 mp = make_float2(0.0f, 0.0f);
+  if (active) {
+    me = a.xs[s];
+    if (HAS_POL) mp = a.ps[s];
+    if (MODE == kModeEmit) cs.emit_at = a.row_ptr[a.id[s]];
+  }
+  const int begin = in.slot_begin, n = in.n;
+  const float r2 = in.cutoff2;
+  for (int base = 0; base < n; base += kBlock) {
+    __syncthreads();
+    if (base + (int)threadIdx.x < n) tile[threadIdx.x] = a.xs[begin + base + threadIdx.x];
+    __syncthreads();
+    const int cnt = min(kBlock, n - base);
+    if (!active) continue;
+    for (int t = 0; t < cnt; ++t) {
+      const int m = begin + base + t;
+      if (m == s) continue;
+      const float4 pj = tile[t];
+      const float disp[3] = {me.x - pj.x, me.y - pj.y, me.z - pj.z};
+      const float d2 = len2(disp[0], disp[1], disp[2]);
+      if (d2 >= r2) continue;
+      if (GABRIEL) {
+        // naive_gabriel.hpp:68-80: rho^2 = (d2 * 0.25) * c^2, strict test over every k
+        const float mid0 = me.x - disp[0] * 0.5f;
+        const float mid1 = me.y - disp[1] * 0.5f;
+        const float mid2 = me.z - disp[2] * 0.5f;
+        const float rho2 = d2 * 0.25f * in.gabriel_c2;
+        bool blocked = false;
+        for (int k = begin; k < begin + n; ++k) {
+          if (k == s || k == m) continue;
+          const float4 pk = a.xs[k];
+          if (len2(pk.x - mid0, pk.y - mid1, pk.z - mid2) < rho2) {
+            blocked = true;
+            break;
+          }
+        }
+        if (blocked) continue;
+      }
+      visit<HAS_TYPE, HAS_POL, MODE>(a, in, cs, me, mp, m, pj, disp, d2);
+    }
+  }
+  finish<HAS_POL, MODE>(a, in, lb, s, active, me, mp, cs);
+}
+
+template <bool GABRIEL, bool BINNED, bool HAS_TYPE, bool HAS_POL, int MODE>
+void launch_one(const SweepArgs& a, int n_blocks, hipStream_t stream) {
+  if (BINNED)
+    hipLaunchKernelGGL((k_sweep_binned<GABRIEL, HAS_TYPE, HAS_POL, MODE>), dim3(n_blocks),
+                       dim3(kBlock), 0, stream, a);
+  else
+    hipLaunchKernelGGL((k_sweep_naive<GABRIEL, HAS_TYPE, HAS_POL, MODE>), dim3(n_blocks),
+                       dim3(kBlock), 0, stream, a);
+}
+
+template <bool GABRIEL, bool BINNED, int MODE>
+void launch_force(bool has_type, bool has_pol, const SweepArgs& a, int n_blocks,
+                  hipStream_t stream) {
+  if (MODE == kModeCount || MODE == kModeEmit) {
+    launch_one<GABRIEL, BINNED, false, false, MODE>(a, n_blocks, stream);
+    return;
+  }
+  if (has_type) {
+    if (has_pol)
+      launch_one<GABRIEL, BINNED, true, true, MODE>(a, n_blocks, stream);
+    else
+      launch_one<GABRIEL, BINNED, true, false, MODE>(a, n_blocks, stream);
+  } else {
+    if (has_pol)
+      launch_one<GABRIEL, BINNED, false, true, MODE>(a, n_blocks, stream);
+    else
+      launch_one<GABRIEL, BINNED, false, false, MODE>(a, n_blocks, stream);
+  }
+}
+
+template <bool GABRIEL, bool BINNED>
+void launch_mode(int mode, bool has_type, bool has_pol, const SweepArgs& a, int n_blocks,
+                 hipStream_t stream) {
+  switch (mode) {
+    case kModeDelta: launch_force<GABRIEL, BINNED, kModeDelta>(has_type, has_pol, a, n_blocks, stream); break;
+    case kModePred: launch_force<GABRIEL, BINNED, kModePred>(has_type, has_pol, a, n_blocks, stream); break;
+    case kModeCorr: launch_force<GABRIEL, BINNED, kModeCorr>(has_type, has_pol, a, n_blocks, stream); break;
+    case kModeEuler: launch_force<GABRIEL, BINNED, kModeEuler>(has_type, has_pol, a, n_blocks, stream); break;
+    case kModeCount: launch_force<GABRIEL, BINNED, kModeCount>(has_type, has_pol, a, n_blocks, stream); break;
+    default: launch_force<GABRIEL, BINNED, kModeEmit>(has_type, has_pol, a, n_blocks, stream); break;
+  }
+}
+
+// ------------------------------------------------------------------ COM fix and updates
+// GlobalComFixer::fix, com_fixers.hpp:35-43: sum of the position deltas over the capacity,
+// divided by the capacity; deterministic fixed-order reduction of the per-block partials.
+__global__ __launch_bounds__(kBlock) void k_com_finalize(const InstanceDesc* inst,
+                                                         const float4* partial, float4* com) {
+  __shared__ float lds[4];
+  const InstanceDesc& in = inst[blockIdx.x];
+  float v[3] = {0.0f, 0.0f, 0.0f};
+  for (int b = threadIdx.x; b < in.n_blocks; b += kBlock) {
+    const float4 p = partial[in.block_begin + b];
+    v[0] = v[0] + p.x;
+    v[1] = v[1] + p.y;
+    v[2] = v[2] + p.z;
+  }
+  const float t0 = block_sum_f(v[0], lds);
+  const float t1 = block_sum_f(v[1], lds);
+  const float t2 = block_sum_f(v[2], lds);
+  if (threadIdx.x == 0) {
+    const float n = (float)in.cap;
+    com[blockIdx.x] = in.cap == 0 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f)
+                                  : make_float4(t0 / n, t1 / n, t2 / n, 0.0f);
+  }
+}
+
+template <int KIND, bool HAS_POL>
+__global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
+  const int lb = logical_block();
+  const int ii = a.block_inst[lb];
+  const InstanceDesc& in = a.inst[ii];
+  const int s = lb * kBlock + (int)threadIdx.x;
+  if ((s - in.slot_begin) >= in.n) return;
+  const float4 com = a.com[ii];
+  const float dt = in.dt;
+  float4 d1 = a.d1_pos[s];
+  d1 = make_float4(d1.x - com.x, d1.y - com.y, d1.z - com.z, 0.0f);
+  if (KIND == kUpdatePred) {
+    // heun.hpp:150-154
+    a.d1_pos[s] = d1;
+    const float4 x = a.x[s];
+    a.xp[s] = make_float4(x.x + d1.x * dt, x.y + d1.y * dt, x.z + d1.z * dt, x.w);
+    if (HAS_POL) {
+      const float2 p = a.pol[s], e = a.d1_pol[s];
+      a.polp[s] = make_float2(p.x + e.x * dt, p.y + e.y * dt);
+    }
+  } else if (KIND == kUpdateCorr) {
+    // heun.hpp:174-183 (d1 was COM-fixed by the predictor; com here is the corrector's)
+    const float4 d1f = a.d1_pos[s];
+    float4 d3 = a.d3_pos[s];
+    d3 = make_float4(d3.x - com.x, d3.y - com.y, d3.z - com.z, 0.0f);
+    const float s0 = d1f.x + d3.x, s1 = d1f.y + d3.y, s2 = d1f.z + d3.z;
+    if (a.old_vel) a.old_vel[s] = make_float4(s0 * 0.5f, s1 * 0.5f, s2 * 0.5f, 0.0f);
+    const float4 x = a.x[s];
+    a.x[s] = make_float4(x.x + s0 * 0.5f * dt, x.y + s1 * 0.5f * dt, x.z + s2 * 0.5f * dt, x.w);
+    if (HAS_POL) {
+      const float2 p = a.pol[s], e1 = a.d1_pol[s], e3 = a.d3_pol[s];
+      a.pol[s] = make_float2(p.x + (e1.x + e3.x) * 0.5f * dt, p.y + (e1.y + e3.y) * 0.5f * dt);
+    }
+  } else {
+    // euler.hpp:56-66
+    if (a.old_vel) a.old_vel[s] = d1;
+    const float4 x = a.x[s];
+    a.x[s] = make_float4(x.x + d1.x * dt, x.y + d1.y * dt, x.z + d1.z * dt, x.w);
+    if (HAS_POL) {
+      const float2 p = a.pol[s], e = a.d1_pol[s];
+      a.pol[s] = make_float2(p.x + e.x * dt, p.y + e.y * dt);
+    }
+  }
+}
+
+}  // namespace
+
+void launch_sweep(int finder, bool has_type, bool has_pol, int mode, const SweepArgs& a,
+                  int n_blocks, hipStream_t stream) {
+  switch (finder) {
+    case kBinnedAllPairs: launch_mode<false, true>(mode, has_type, has_pol, a, n_blocks, stream); break;
+    case kBinnedGabriel: launch_mode<true, true>(mode, has_type, has_pol, a, n_blocks, stream); break;
+    case kNaiveAllPairs: launch_mode<false, false>(mode, has_type, has_pol, a, n_blocks, stream); break;
+    default: launch_mode<true, false>(mode, has_type, has_pol, a, n_blocks, stream); break;
+  }
+}
+
+void launch_com_finalize(const InstanceDesc* inst, int n_inst, const float4* partial, float4* com,
+                         hipStream_t stream) {
+  hipLaunchKernelGGL(k_com_finalize, dim3(n_inst), dim3(kBlock), 0, stream, inst, partial, com);
+}
+
+void launch_update(int kind, bool has_pol, const UpdateArgs& a, int n_blocks, hipStream_t stream) {
+  const dim3 g(n_blocks), b(kBlock);
+  if (kind == kUpdatePred) {
+    if (has_pol) hipLaunchKernelGGL((k_update<kUpdatePred, true>), g, b, 0, stream, a);
+    else hipLaunchKernelGGL((k_update<kUpdatePred, false>), g, b, 0, stream, a);
+  } else if (kind == kUpdateCorr) {
+    if (has_pol) hipLaunchKernelGGL((k_update<kUpdateCorr, true>), g, b, 0, stream, a);
+    else hipLaunchKernelGGL((k_update<kUpdateCorr, false>), g, b, 0, stream, a);
+  } else {
+    if (has_pol) hipLaunchKernelGGL((k_update<kUpdateEuler, true>), g, b, 0, stream, a);
+    else hipLaunchKernelGGL((k_update<kUpdateEuler, false>), g, b, 0, stream, a);
+  }
+}
+
+}  // namespace tecs_amd

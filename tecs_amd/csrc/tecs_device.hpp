@@ -1,0 +1,203 @@
+// tecs_device.hpp - device data layout and the per-pair arithmetic of the engine.
+//
+// Layout (HBM, per batch of simulations; a single simulation is a batch of one):
+//   * Cells live in "slots". Each instance owns a slot range [slot_begin, slot_begin + padded)
+//     with padded a multiple of kBlock, so every workgroup belongs to exactly one instance.
+//   * Between grid rebuilds the slots of an instance are ordered by (bin, agent index): the
+//     members of a bin are contiguous and ascending in agent index, which is the order
+//     Kokkos::BinSort gives on the Serial backend (grid.hpp:399-400). Walking a bin's slot range
+//     therefore visits j in the reference's order.
+//   * x4[s]   = float4(x, y, z, type bits)  - position and the read-only cell type
+//     pol[s]  = float2(theta, phi)           - polarity (when present)
+//     id[s]   = agent index of the cell in slot s (within its instance)
+//   * Bin offsets of all instances are concatenated; instance bins are followed by one pad bin
+//     that holds the instance's pad slots, so one global exclusive scan yields global slot
+//     offsets for every instance.
+//
+// All expressions below keep the reference's operand order; the library is compiled with
+// -ffp-contract=off so no multiply-add is fused and every operation rounds once, as in
+// IEEE binary32 evaluation of the reference source. Division and square root are the
+// correctly rounded sequences (hipcc's default for HIP single precision).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tecs_amd {
+
+constexpr int kBlock = 256;  // slots per workgroup (4 wavefronts of 64)
+constexpr int kMaxTypes = 4;
+
+struct ForceParams {
+  float h, krep, kadh;
+  int n_types;
+  float table[kMaxTypes * kMaxTypes];
+  float bend, align;
+};
+
+struct InstanceDesc {
+  int slot_begin;  // first global slot
+  int n;           // active cells
+  int cap;         // capacity (GlobalComFixer divisor)
+  int padded;      // slot range length, multiple of kBlock
+  int block_begin; // first logical workgroup of the instance
+  int n_blocks;    // padded / kBlock
+  int bin_begin;   // first entry of the instance in the concatenated bin arrays
+  int bin_count;   // real bins; the pad bin has local index bin_count
+  int ext[3];
+  float minb[3];
+  float bin_size, inv_bin;
+  float cutoff2;
+  float gabriel_factor;  // c*c*0.25 (binned_gabriel.hpp:42)
+  float gabriel_c2;      // c*c     (naive_gabriel.hpp:392)
+  int search_radius;
+  float dt;              // float(dt) of the current step
+  ForceParams force;
+};
+
+// Logical workgroup index with an XCD-aware remap: the hardware hands consecutive workgroup
+// ids to the 8 XCDs round-robin; remapping gives each XCD a contiguous run of logical blocks,
+// so spatially neighbouring slots (which share neighbour bins) meet in the same L2.
+__device__ __forceinline__ int logical_block() {
+  const int p = (int)blockIdx.x;
+  const int nb = (int)gridDim.x;
+  const int q = nb >> 3, r = nb & 7;
+  const int xcd = p & 7, idx = p >> 3;
+  return (xcd < r) ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
+}
+
+// ---------------------------------------------------------------- bin arithmetic (grid.hpp)
+__device__ __forceinline__ int clampi(int v, int hi) { return max(0, min(v, hi)); }
+
+// Grid::calc_bin_coords_from_point, grid.hpp:222-230: (p - min) / bin, truncation, clamp.
+__device__ __forceinline__ void bin_coords_div(const InstanceDesc& in, float x, float y, float z,
+                                               int* c) {
+  c[0] = clampi((int)((x - in.minb[0]) / in.bin_size), in.ext[0] - 1);
+  c[1] = clampi((int)((y - in.minb[1]) / in.bin_size), in.ext[1] - 1);
+  c[2] = clampi((int)((z - in.minb[2]) / in.bin_size), in.ext[2] - 1);
+}
+// binned_all_pairs.hpp:103-109: (p - min) * inv_bin, truncation, clamp.
+__device__ __forceinline__ void bin_coords_mul(const InstanceDesc& in, float x, float y, float z,
+                                               int* c) {
+  c[0] = clampi((int)((x - in.minb[0]) * in.inv_bin), in.ext[0] - 1);
+  c[1] = clampi((int)((y - in.minb[1]) * in.inv_bin), in.ext[1] - 1);
+  c[2] = clampi((int)((z - in.minb[2]) * in.inv_bin), in.ext[2] - 1);
+}
+// Grid::flatten_bin_index, grid.hpp:243-245.
+__device__ __forceinline__ int flatten(const InstanceDesc& in, int c0, int c1, int c2) {
+  return c0 + c1 * in.ext[0] + c2 * (in.ext[0] * in.ext[1]);
+}
+
+// ------------------------------------------------------------------- vector.hpp:312-343
+__device__ __forceinline__ float len2(float a, float b, float c) {
+  float r = 0.0f;
+  r = r + a * a;
+  r = r + b * b;
+  r = r + c * c;
+  return r;
+}
+// VectorN::dot, vector.hpp:312-318.
+__device__ __forceinline__ float dot3(const float* a, const float* b) {
+  float r = 0.0f;
+  r = r + a[0] * b[0];
+  r = r + a[1] * b[1];
+  r = r + a[2] * b[2];
+  return r;
+}
+
+// ---------------------------------------------------------- force laws (predefined.hpp)
+// PiecewiseLinear, predefined.hpp:107-144.
+__device__ __forceinline__ void piecewise_linear(const float* disp, float d, float h, float krep,
+                                                 float kadh, float* out) {
+  if (d == 0.0f) {
+    out[0] = out[1] = out[2] = 0.0f;
+    return;
+  }
+  const float F = krep * fmaxf(h - d, 0.0f) - kadh * fmaxf(d - h, 0.0f);
+  out[0] = disp[0] * F / d;
+  out[1] = disp[1] * F / d;
+  out[2] = disp[2] * F / d;
+}
+
+// ---------------------------------------------------------------- polarity.hpp:111-198
+// to_vector3, polarity.hpp:111-118.
+__device__ __forceinline__ void pol_to_vec3(float t, float p, float* v) {
+  v[0] = sinf(t) * cosf(p);
+  v[1] = sinf(t) * sinf(p);
+  v[2] = cosf(t);
+}
+// unidirectional_polarization_force, polarity.hpp:145-157 (self = (t0,p0), other = (t1,p1)).
+__device__ __forceinline__ void pol_unidirectional(float t0, float p0, float t1, float p1,
+                                                   float* r) {
+  r[0] = cosf(t0) * sinf(t1) * cosf(p0 - p1) - sinf(t0) * cosf(t1);
+  r[1] = 0.0f;
+  const float sin_theta = sinf(t0);
+  if (fabsf(sin_theta) > 1.1920928955078125e-07f)  // FLT_EPSILON
+    r[1] = -sinf(t1) * sinf(p0 - p1) / sin_theta;
+}
+// dot, polarity.hpp:131-136.
+__device__ __forceinline__ float pol_dot(float t0, float p0, float t1, float p1) {
+  return sinf(t0) * sinf(t1) * cosf(p0 - p1) + cosf(t0) * cosf(t1);
+}
+// Kokkos::pow(float, int) promotes to double; the quotient is narrowed to float when it
+// multiplies the float vector (vector.hpp:293-296). The squares are exact in double.
+__device__ __forceinline__ float pow2_ratio(float a, float b) {
+  const double ad = (double)a, bd = (double)b;
+  return (float)((ad * ad) / (bd * bd));
+}
+
+// One invocation of the fused pairwise force (see tecs_force_desc in tecs_amd.h), adding the
+// contributions to the accumulators in the reference's statement order.
+template <bool HAS_TYPE, bool HAS_POL>
+__device__ __forceinline__ void apply_force(const ForceParams& f, const float* disp, float d,
+                                            int type_i, int type_j, float ti, float pi, float tj,
+                                            float pj, float* acc_pos, float* acc_pol) {
+  float kadh = f.kadh;
+  if (HAS_TYPE) {
+    if (f.n_types > 0) kadh = f.table[type_i * f.n_types + type_j];
+  }
+  float v[3];
+  piecewise_linear(disp, d, f.h, f.krep, kadh, v);
+  acc_pos[0] = acc_pos[0] + v[0];
+  acc_pos[1] = acc_pos[1] + v[1];
+  acc_pos[2] = acc_pos[2] + v[2];
+  if (HAS_POL) {
+    if (f.bend != 0.0f) {
+      // bending_force, polarity.hpp:181-198
+      float tv[3], ov[3];
+      pol_to_vec3(ti, pi, tv);
+      const float prod_i = dot3(tv, disp) / d;
+      pol_to_vec3(tj, pj, ov);
+      const float prod_j = dot3(ov, disp) / d;
+      const float a = -prod_i / d;
+      const float b = pow2_ratio(prod_i, d);
+      const float c = -prod_j / d;
+      const float e = pow2_ratio(prod_j, d);
+      float bv[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) bv[k] = ((a * tv[k] + b * disp[k]) + c * ov[k]) + e * disp[k];
+      // from_vector3(displacement, distance), polarity.hpp:62-66
+      const float q0 = acosf(fminf(1.0f, fmaxf(-1.0f, disp[2] / d)));
+      const float q1 = atan2f(disp[1], disp[0]);
+      float u[2];
+      pol_unidirectional(ti, pi, q0, q1, u);
+      const float bp0 = -prod_i * u[0];
+      const float bp1 = -prod_i * u[1];
+      acc_pos[0] = acc_pos[0] + f.bend * bv[0];
+      acc_pos[1] = acc_pos[1] + f.bend * bv[1];
+      acc_pos[2] = acc_pos[2] + f.bend * bv[2];
+      acc_pol[0] = acc_pol[0] + f.bend * bp0;
+      acc_pol[1] = acc_pol[1] + f.bend * bp1;
+    }
+    if (f.align != 0.0f) {
+      // bidirectional_polarization_force, polarity.hpp:165-168
+      float u[2];
+      pol_unidirectional(ti, pi, tj, pj, u);
+      const float s = pol_dot(ti, pi, tj, pj);
+      acc_pol[0] = acc_pol[0] + f.align * (s * u[0]);
+      acc_pol[1] = acc_pol[1] + f.align * (s * u[1]);
+    }
+  }
+}
+
+}  // namespace tecs_amd

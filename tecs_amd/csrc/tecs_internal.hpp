@@ -1,0 +1,98 @@
+// tecs_internal.hpp - host-side view of the device state and the kernel launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "tecs_device.hpp"
+
+namespace tecs_amd {
+
+enum Finder { kNaiveAllPairs = 0, kBinnedAllPairs = 1, kNaiveGabriel = 2, kBinnedGabriel = 3 };
+
+// What a sweep does with the force sum F of each cell (dt and the COM fix come from the
+// instance descriptor / com buffer).
+enum Mode {
+  kModeDelta = 0,  // write F (and per-block COM partials when requested)
+  kModePred = 1,   // Heun substep 1 without COM fix: d1 = F, x~ = x + d1*dt
+  kModeCorr = 2,   // Heun substep 2 without COM fix: x += ((d1 + F)*0.5)*dt, old_vel
+  kModeEuler = 3,  // Euler without COM fix: old_vel = F, x' = x + F*dt (ping-pong buffer)
+  kModeCount = 4,  // pair export: number of pairs per agent
+  kModeEmit = 5    // pair export: the pairs, in visiting order
+};
+
+struct SweepArgs {
+  const InstanceDesc* inst;
+  const int* block_inst;
+  const float4* xs;  // positions of the evaluated stage
+  const float2* ps;  // polarities of the evaluated stage (may be null)
+  const int* off;    // concatenated bin offsets (global slots)
+  const int* id;     // slot -> agent index
+  float4* o_pos;     // Delta: F; Pred: d1; Euler: new positions
+  float2* o_pol;
+  float4* o2_pos;    // Pred: predicted positions; Corr: stage-0 positions (updated in place)
+  float2* o2_pol;
+  const float4* d1_pos;  // Corr: first evaluation
+  const float2* d1_pol;
+  float4* old_vel;       // Corr / Euler, may be null
+  float4* com_partial;   // Delta: per-block sums of F (may be null)
+  unsigned long long* block_pairs;  // per-block pair-interaction counters (accumulated)
+  int* row_count;        // Count
+  const int64_t* row_ptr;  // Emit
+  int32_t* cols;           // Emit
+};
+
+void launch_sweep(int finder, bool has_type, bool has_pol, int mode, const SweepArgs& a,
+                  int n_blocks, hipStream_t stream);
+
+struct UpdateArgs {
+  const InstanceDesc* inst;
+  const int* block_inst;
+  const float4* com;  // per instance
+  float4* x;          // stage-0 positions
+  float2* pol;
+  float4* d1_pos;     // first (or only) evaluation, COM-fixed in place
+  float2* d1_pol;
+  const float4* d3_pos;  // second evaluation (Heun corrector)
+  const float2* d3_pol;
+  float4* xp;            // predicted positions (Heun predictor)
+  float2* polp;
+  float4* old_vel;
+};
+enum UpdateKind { kUpdatePred = 0, kUpdateCorr = 1, kUpdateEuler = 2 };
+void launch_update(int kind, bool has_pol, const UpdateArgs& a, int n_blocks, hipStream_t stream);
+void launch_com_finalize(const InstanceDesc* inst, int n_inst, const float4* partial, float4* com,
+                         hipStream_t stream);
+
+struct RebuildArgs {
+  const InstanceDesc* inst;
+  const int* block_inst;
+  const float4* x_old;
+  const float2* pol_old;
+  const int* id_old;
+  const float4* ov_old;
+  float4* x_new;
+  float2* pol_new;
+  int* id_new;
+  float4* ov_new;
+  int* keys;
+  int* tmp;
+  int* cnt;
+  int* off;
+  int* scan_bsum;
+  int total_bins;
+};
+void launch_rebuild(const RebuildArgs& a, int n_blocks, hipStream_t stream);
+int scan_tiles_for(int n);
+void launch_exclusive_scan(const int* in, int n, int* out, int* bsum, hipStream_t stream);
+
+void launch_init_ids(const InstanceDesc* inst, const int* block_inst, int* id, int n_blocks,
+                     hipStream_t stream);
+void launch_upload(const InstanceDesc* inst, int ii, int block_begin, int n_blocks, const int* id,
+                   const float* pos, const float* pol, const uint8_t* type, float4* x, float2* p,
+                   hipStream_t stream);
+void launch_download(const InstanceDesc* inst, int ii, int block_begin, int n_blocks,
+                     const int* id, const float4* x, const float2* p, const float4* ov,
+                     float* pos, float* pol, float* old_vel, hipStream_t stream);
+
+}  // namespace tecs_amd

@@ -1,0 +1,188 @@
+"""numpy-facing handles over the C ABI: GpuSimulation (tecs_sim) and GpuBatch (tecs_batch).
+
+Arrays are in the reference's agent order and layout: positions (n, 3) float32, polarities
+(n, 2) float32 (theta, phi), types (n,) uint8.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from ._lib import check, load
+
+
+def _f32(a, shape):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.shape != shape:
+        raise ValueError(f"expected shape {shape}, got {a.shape}")
+    return a
+
+
+def _u8(a, n):
+    a = np.ascontiguousarray(a, dtype=np.uint8)
+    if a.shape != (n,):
+        raise ValueError(f"expected shape ({n},), got {a.shape}")
+    return a
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+class GpuSimulation:
+    """One simulation on the GPU (the drop-in for Simulation<Config>::take_step)."""
+
+    def __init__(self, desc: abi.SimDesc):
+        self._lib = load()
+        self.desc = abi.copy_desc(desc)
+        self.n = int(desc.agent_count)
+        self.has_pol = bool(desc.fields & abi.FIELD_POLARITY)
+        self.has_type = bool(desc.fields & abi.FIELD_TYPE)
+        h = C.c_void_p()
+        check(self._lib.tecs_sim_create(C.byref(self.desc), None, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.tecs_sim_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def upload(self, positions, polarities=None, types=None):
+        p = _f32(positions, (self.n, 3))
+        q = _f32(polarities, (self.n, 2)) if (polarities is not None and self.has_pol) else None
+        t = _u8(types, self.n) if (types is not None and self.has_type) else None
+        check(self._lib.tecs_sim_upload(self._h, _ptr(p), _ptr(q), _ptr(t)))
+
+    def download(self):
+        pos = np.empty((self.n, 3), np.float32)
+        pol = np.empty((self.n, 2), np.float32) if self.has_pol else None
+        ov = np.empty((self.n, 3), np.float32)
+        check(self._lib.tecs_sim_download(self._h, _ptr(pos), _ptr(pol), _ptr(ov)))
+        return pos, pol, ov
+
+    def take_step(self, dt: float, n_steps: int = 1):
+        check(self._lib.tecs_sim_take_step(self._h, float(dt), int(n_steps)))
+
+    def synchronize(self):
+        check(self._lib.tecs_sim_synchronize(self._h))
+
+    def evaluate_force(self, is_full_step: bool = True):
+        dpos = np.empty((self.n, 3), np.float32)
+        dpol = np.empty((self.n, 2), np.float32) if self.has_pol else None
+        check(self._lib.tecs_sim_evaluate_force(self._h, 1 if is_full_step else 0, _ptr(dpos), _ptr(dpol)))
+        return dpos, dpol
+
+    def pairs(self, is_full_step: bool = True):
+        """CSR (row_ptr[n+1] int64, cols int32) of the ordered pairs in visiting order."""
+        count = C.c_int64(0)
+        check(self._lib.tecs_sim_pairs(self._h, 1 if is_full_step else 0, None, None, 0, C.byref(count)))
+        row_ptr = np.empty(self.n + 1, np.int64)
+        cols = np.empty(max(count.value, 1), np.int32)
+        check(
+            self._lib.tecs_sim_pairs(
+                self._h, 1 if is_full_step else 0, _ptr(row_ptr), _ptr(cols), cols.size, C.byref(count)
+            )
+        )
+        return row_ptr, cols[: count.value]
+
+    def rebuild(self):
+        check(self._lib.tecs_sim_rebuild(self._h))
+
+    def stats(self) -> dict:
+        s = abi.Stats()
+        check(self._lib.tecs_sim_get_stats(self._h, C.byref(s)))
+        return {k: int(getattr(s, k)) for k, _ in abi.Stats._fields_}
+
+    def reset_stats(self):
+        check(self._lib.tecs_sim_reset_stats(self._h))
+
+    def set_kernel_timing(self, enabled: bool):
+        check(self._lib.tecs_sim_set_kernel_timing(self._h, 1 if enabled else 0))
+
+    def kernel_timing(self, kernel_class: int):
+        ms = C.c_double(0.0)
+        n = C.c_int64(0)
+        check(self._lib.tecs_sim_get_kernel_timing(self._h, int(kernel_class), C.byref(ms), C.byref(n)))
+        return float(ms.value), int(n.value)
+
+
+class GpuBatch:
+    """Many independent simulations advanced together (one launch per kernel per substep)."""
+
+    def __init__(self, descs):
+        self._lib = load()
+        self.descs = [abi.copy_desc(d) for d in descs]
+        arr = (abi.SimDesc * len(self.descs))(*self.descs)
+        self._arr = arr
+        h = C.c_void_p()
+        check(self._lib.tecs_batch_create(arr, len(self.descs), None, C.byref(h)))
+        self._h = h
+
+    def __len__(self):
+        return len(self.descs)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.tecs_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, k, positions, polarities=None, types=None):
+        d = self.descs[k]
+        n = int(d.agent_count)
+        p = _f32(positions, (n, 3))
+        q = _f32(polarities, (n, 2)) if (polarities is not None and d.fields & abi.FIELD_POLARITY) else None
+        t = _u8(types, n) if (types is not None and d.fields & abi.FIELD_TYPE) else None
+        check(self._lib.tecs_batch_upload(self._h, int(k), _ptr(p), _ptr(q), _ptr(t)))
+
+    def download(self, k):
+        d = self.descs[k]
+        n = int(d.agent_count)
+        pos = np.empty((n, 3), np.float32)
+        pol = np.empty((n, 2), np.float32) if d.fields & abi.FIELD_POLARITY else None
+        ov = np.empty((n, 3), np.float32)
+        check(self._lib.tecs_batch_download(self._h, int(k), _ptr(pos), _ptr(pol), _ptr(ov)))
+        return pos, pol, ov
+
+    def take_step(self, dts, n_steps: int = 1):
+        dts = np.ascontiguousarray(np.broadcast_to(np.asarray(dts, np.float64), (len(self.descs),)))
+        check(self._lib.tecs_batch_take_step(self._h, _ptr(dts), int(n_steps)))
+
+    def synchronize(self):
+        check(self._lib.tecs_batch_synchronize(self._h))
+
+    def stats(self) -> dict:
+        s = abi.Stats()
+        check(self._lib.tecs_batch_get_stats(self._h, C.byref(s)))
+        return {k: int(getattr(s, k)) for k, _ in abi.Stats._fields_}
+
+    def reset_stats(self):
+        check(self._lib.tecs_batch_reset_stats(self._h))
+
+    def set_kernel_timing(self, enabled: bool):
+        check(self._lib.tecs_batch_set_kernel_timing(self._h, 1 if enabled else 0))
+
+    def kernel_timing(self, kernel_class: int):
+        ms = C.c_double(0.0)
+        n = C.c_int64(0)
+        check(self._lib.tecs_batch_get_kernel_timing(self._h, int(kernel_class), C.byref(ms), C.byref(n)))
+        return float(ms.value), int(n.value)

@@ -1,0 +1,222 @@
+"""GPU engine vs the CPU oracle on identical inputs, through the C ABI.
+
+Bars (stated per test):
+  * pair sets: bit-exact - same CSR rows, same order (the reference's visiting order)
+  * PiecewiseLinear / type-table forces: bit-exact vs the FP32 oracle (same op sequence, same
+    summation order), and within 1e-5 of the FP64 evaluation
+  * polarity forces (sin/cos/acos/atan2 differ by ulps between GPU and host libm): within
+    1e-5 of the FP64 evaluation, measured against the per-component sum of absolute pair
+    contributions (the FP32 summation error is bounded by a multiple of that sum)
+  * trajectories: bit-exact where the forces are; tolerance stated for polarity models
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from tecs_amd import GpuBatch, GpuSimulation, abi, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+FINDERS = [abi.BINNED_ALL_PAIRS, abi.BINNED_GABRIEL, abi.NAIVE_ALL_PAIRS, abi.NAIVE_GABRIEL]
+
+
+def _with(desc, **kw):
+    d = abi.copy_desc(desc)
+    for k, v in kw.items():
+        if k in ("rebuild_every_n", "bin_size_scale", "gabriel_coefficient"):
+            setattr(d.pair_finder_settings, k, v)
+        else:
+            setattr(d, k, v)
+    return d
+
+
+def _pair(oracle_lib, desc, w, positions=None):
+    g = GpuSimulation(desc)
+    o = oracle_lib.OracleSim(desc)
+    pos = w.positions if positions is None else positions
+    for s in (g, o):
+        s.upload(pos, w.polarities, w.types)
+    return g, o
+
+
+def _workload(name, n, finder):
+    if name == "relaxation":
+        return W.relaxation(n, finder=finder)
+    if name == "epithelium":
+        return W.epithelium(n, finder=finder)
+    return W.sorting(n, finder=finder)
+
+
+def _assert_same_csr(a, b, what):
+    ra, ca = a
+    rb, cb = b
+    assert ra.shape == rb.shape, what
+    assert np.array_equal(ra, rb), f"{what}: row pointers differ"
+    assert np.array_equal(ca, cb), f"{what}: pair lists differ"
+
+
+def _force_scale_check(f32, f64, scale, rel=1e-5):
+    excess = np.abs(f32.astype(np.float64) - f64) - rel * np.maximum(scale, 1e-30)
+    return float(excess.max())
+
+
+CASES = [
+    ("relaxation", 3000),
+    ("epithelium", 4000),
+    ("sorting", 20000),
+]
+
+
+@pytest.mark.parametrize("finder", FINDERS, ids=lambda f: abi.FINDER_NAMES[f])
+@pytest.mark.parametrize("name,n", CASES)
+def test_pairs_full_step_bit_exact(gpu, oracle_lib, finder, name, n):
+    if finder in (abi.NAIVE_ALL_PAIRS, abi.NAIVE_GABRIEL):
+        n = min(n, 1500)
+    w = _workload(name, n, finder)
+    print(f"case {name} n={n} finder={abi.FINDER_NAMES[finder]}", flush=True)
+    g, o = _pair(oracle_lib, w.desc, w)
+    pg = g.pairs(True)
+    po = o.pairs(True)
+    assert pg[1].size > 0
+    _assert_same_csr(pg, po, f"{name}/{abi.FINDER_NAMES[finder]}")
+
+
+@pytest.mark.parametrize("finder", [abi.BINNED_ALL_PAIRS, abi.BINNED_GABRIEL], ids=lambda f: abi.FINDER_NAMES[f])
+def test_pairs_stale_grid_bit_exact(gpu, oracle_lib, finder):
+    """Heun corrector semantics: the grid of the last rebuild with moved positions."""
+    w = _workload("sorting", 20000, finder)
+    print(f"case stale grid finder={abi.FINDER_NAMES[finder]}", flush=True)
+    g, o = _pair(oracle_lib, w.desc, w)
+    for s in (g, o):
+        s.rebuild()
+    moved = (w.positions + W.rng(77).uniform(-0.3, 0.3, w.positions.shape)).astype(np.float32)
+    for s in (g, o):
+        s.upload(moved, w.polarities, w.types)
+    _assert_same_csr(g.pairs(False), o.pairs(False), "stale grid")
+
+
+@pytest.mark.parametrize("finder", FINDERS, ids=lambda f: abi.FINDER_NAMES[f])
+@pytest.mark.parametrize("name,n", [("relaxation", 3000), ("sorting", 20000)])
+def test_forces_bit_exact_piecewise_linear(gpu, oracle_lib, finder, name, n):
+    if finder in (abi.NAIVE_ALL_PAIRS, abi.NAIVE_GABRIEL):
+        n = min(n, 1500)
+    w = _workload(name, n, finder)
+    print(f"case forces {name} n={n} finder={abi.FINDER_NAMES[finder]}", flush=True)
+    g, o = _pair(oracle_lib, w.desc, w)
+    fg, _ = g.evaluate_force(True)
+    fo, _ = o.evaluate_force(True)
+    assert np.array_equal(fg.view(np.uint32), fo.view(np.uint32)), "forces differ bitwise"
+    rp, cols = o.pairs(False)
+    f64, _, s64, _ = o.forces_f64(rp, cols)
+    assert _force_scale_check(fg, f64, s64) <= 0.0
+
+
+@pytest.mark.parametrize("finder", [abi.BINNED_GABRIEL, abi.NAIVE_GABRIEL], ids=lambda f: abi.FINDER_NAMES[f])
+def test_forces_polarity_within_1e5_of_fp64(gpu, oracle_lib, finder):
+    n = 4000 if finder == abi.BINNED_GABRIEL else 1500
+    w = _workload("epithelium", n, finder)
+    print(f"case polarity forces n={n} finder={abi.FINDER_NAMES[finder]}", flush=True)
+    g, o = _pair(oracle_lib, w.desc, w)
+    fg, pg = g.evaluate_force(True)
+    rp, cols = o.pairs(False)
+    f64, p64, s64, sp64 = o.forces_f64(rp, cols)
+    e1 = _force_scale_check(fg, f64, s64)
+    e2 = _force_scale_check(pg, p64, sp64)
+    print(f"excess over 1e-5 bound: pos {e1:.3e} pol {e2:.3e}")
+    assert e1 <= 0.0 and e2 <= 0.0
+
+
+def _run_both(oracle_lib, w, desc, steps):
+    g, o = _pair(oracle_lib, desc, w)
+    g.take_step(w.dt, steps)
+    o.take_step(w.dt, steps)
+    return g, o
+
+
+@pytest.mark.parametrize(
+    "name,n,steps",
+    [("relaxation", 3000, 100), ("sorting", 20000, 100)],
+)
+def test_trajectory_bit_exact(gpu, oracle_lib, name, n, steps):
+    w = _workload(name, n, abi.BINNED_ALL_PAIRS)
+    print(f"case trajectory {name} n={n} steps={steps}", flush=True)
+    g, o = _run_both(oracle_lib, w, w.desc, steps)
+    xg, _, vg = g.download()
+    xo, _, vo = o.download()
+    assert np.array_equal(xg.view(np.uint32), xo.view(np.uint32)), (
+        f"positions differ after {steps} steps, max {np.abs(xg - xo).max()}"
+    )
+    assert np.array_equal(vg.view(np.uint32), vo.view(np.uint32)), "old velocities differ"
+    assert g.stats()["pair_interactions"] == o.stats()["pair_interactions"]
+
+
+def test_trajectory_gabriel_relaxation_bit_exact(gpu, oracle_lib):
+    w = _workload("relaxation", 3000, abi.BINNED_GABRIEL)
+    d = _with(w.desc, integrator=abi.HEUN)
+    print("case trajectory relaxation/BinnedGabriel Heun 50 steps", flush=True)
+    g, o = _run_both(oracle_lib, w, d, 50)
+    xg, _, _ = g.download()
+    xo, _, _ = o.download()
+    assert np.array_equal(xg.view(np.uint32), xo.view(np.uint32))
+
+
+def test_trajectory_epithelium_tolerance(gpu, oracle_lib):
+    """100 Heun steps of the polarity model. Per-step differences are libm ulps; a Gabriel
+    pair whose test sits within an ulp of the boundary can flip and move two cells by up to
+    ~dt*0.2. Bar: median |dx| <= 1e-5, 99th percentile <= 1e-3, max <= 0.05; polarities alike."""
+    w = _workload("epithelium", 4000, abi.BINNED_GABRIEL)
+    print("case trajectory epithelium n=4000 steps=100", flush=True)
+    g, o = _run_both(oracle_lib, w, w.desc, 100)
+    xg, pg, _ = g.download()
+    xo, po, _ = o.download()
+    dx = np.abs(xg - xo).max(axis=1)
+    dp = np.abs(pg - po).max(axis=1)
+    print(f"dx median {np.median(dx):.2e} p99 {np.quantile(dx, 0.99):.2e} max {dx.max():.2e}")
+    print(f"dpol median {np.median(dp):.2e} p99 {np.quantile(dp, 0.99):.2e} max {dp.max():.2e}")
+    assert np.isfinite(xg).all() and np.isfinite(pg).all()
+    assert np.median(dx) <= 1e-5 and np.quantile(dx, 0.99) <= 1e-3 and dx.max() <= 0.05
+    assert np.median(dp) <= 1e-5 and np.quantile(dp, 0.99) <= 1e-3 and dp.max() <= 0.05
+
+
+@pytest.mark.parametrize("integrator", [abi.EULER, abi.HEUN], ids=["Euler", "Heun"])
+def test_global_com_fixer(gpu, oracle_lib, integrator):
+    """GlobalComFixer divides by the capacity (com_fixers.hpp:40-42). Its reduction order is
+    backend-defined, so positions agree to a tolerance (1e-5 absolute after 20 steps)."""
+    w = _workload("relaxation", 3000, abi.BINNED_ALL_PAIRS)
+    d = _with(w.desc, integrator=integrator, com_fixer=abi.GLOBAL_COM_FIXER, capacity=3500)
+    print(f"case com fixer integrator={integrator}", flush=True)
+    g, o = _run_both(oracle_lib, w, d, 20)
+    xg, _, vg = g.download()
+    xo, _, vo = o.download()
+    assert np.abs(xg - xo).max() <= 1e-5
+    assert np.abs(vg - vo).max() <= 1e-5
+
+
+def test_rebuild_every_n_bit_exact(gpu, oracle_lib):
+    w = _workload("sorting", 20000, abi.BINNED_ALL_PAIRS)
+    d = _with(w.desc, rebuild_every_n=3)
+    print("case rebuild_every_n=3", flush=True)
+    g, o = _run_both(oracle_lib, w, d, 10)
+    xg, _, _ = g.download()
+    xo, _, _ = o.download()
+    assert np.array_equal(xg.view(np.uint32), xo.view(np.uint32))
+    assert g.stats()["rebuilds"] == o.stats()["rebuilds"] == 4
+
+
+def test_batch_member_equals_alone(gpu):
+    """Every member of a batch ends bitwise where it ends when run alone."""
+    ws = W.sweep(count=6, n_min=500, n_max=20000)
+    print("case batch of 6", flush=True)
+    b = GpuBatch([w.desc for w in ws])
+    for k, w in enumerate(ws):
+        b.upload(k, w.positions, w.polarities, w.types)
+    b.take_step([w.dt for w in ws], 20)
+    for k, w in enumerate(ws):
+        s = GpuSimulation(w.desc)
+        s.upload(w.positions, w.polarities, w.types)
+        s.take_step(w.dt, 20)
+        xa, _, va = s.download()
+        xb, _, vb = b.download(k)
+        assert np.array_equal(xa.view(np.uint32), xb.view(np.uint32)), f"member {k} differs"
+        assert np.array_equal(va.view(np.uint32), vb.view(np.uint32))
