@@ -60,6 +60,9 @@ def load():
         "orc_pairs": (C.c_int, [P, C.c_int, P, P, C.c_int64, C.POINTER(C.c_int64)]),
         "orc_get_stats": (C.c_int, [P, C.POINTER(abi.Stats)]),
         "orc_forces_f64": (C.c_int, [P, P, P, P, P, P, P]),
+        "orc_vector_op": (C.c_int, [C.c_int, P, P]),
+        "orc_polarity_op": (C.c_int, [C.c_int, P, P]),
+        "orc_com_fix": (C.c_int, [P, C.c_uint, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -157,3 +160,45 @@ class OracleSim:
         spol = np.empty((self.n, 2), np.float64)
         _check(self._lib.orc_forces_f64(self._h, _ptr(rp), _ptr(cc), _ptr(dpos), _ptr(dpol), _ptr(spos), _ptr(spol)))
         return dpos, dpol, spos, spol
+
+
+def vector_op(op: int, a, b=None) -> float:
+    """0: dot(a, b), 1: length_squared(a), 2: distance_to_squared(a, b) (float32)."""
+    buf = np.zeros(6, np.float32)
+    buf[:3] = a
+    if b is not None:
+        buf[3:] = b
+    out = np.zeros(1, np.float32)
+    _check(load().orc_vector_op(int(op), _ptr(buf), _ptr(out)))
+    return out[0]
+
+
+def polarity_op(op: int, *args) -> np.ndarray:
+    """Polarity_ arithmetic of the oracle; see orc_polarity_op in tecs_oracle.cpp."""
+    buf = np.concatenate([np.atleast_1d(np.asarray(a, np.float32)) for a in args]).astype(np.float32)
+    buf = np.concatenate([buf, np.zeros(8, np.float32)])
+    out = np.zeros(8, np.float32)
+    _check(load().orc_polarity_op(int(op), _ptr(buf), _ptr(out)))
+    return out
+
+
+def com_fix(deltas, capacity: int) -> np.ndarray:
+    d = np.zeros((capacity, 3), np.float32)
+    d[: len(deltas)] = deltas
+    out = np.zeros(3, np.float32)
+    _check(load().orc_com_fix(_ptr(d), int(capacity), _ptr(out)))
+    return out
+
+
+def force_excess(f32, f64, scale, rel: float = 1e-5) -> float:
+    """How far FP32 forces exceed the bar |F32 - F64| <= rel * (S_i + F_rms) per component.
+
+    S_i is the cell's sum of absolute pair contributions (bounds the FP32 summation error) and
+    F_rms the root-mean-square FP64 force of the field (the force scale: a cell whose pair
+    forces cancel, or whose pairs sit at the rest length, still carries the rounding error of
+    its distances, which is a fraction of the field's force scale, not of its own tiny force).
+    Returns max(|F32 - F64| - bound): <= 0 means the bar holds."""
+    f64 = np.asarray(f64, np.float64)
+    rms = np.sqrt(np.mean(f64 * f64, axis=0, keepdims=True)) if f64.size else 0.0
+    bound = rel * (np.asarray(scale, np.float64) + rms)
+    return float((np.abs(np.asarray(f32, np.float64) - f64) - bound).max()) if f64.size else 0.0

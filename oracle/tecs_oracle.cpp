@@ -839,4 +839,54 @@ int orc_forces_f64(orc_sim* h, const int64_t* row_ptr, const int32_t* cols, doub
   return TECS_OK;
 }
 
+// ------------------------------------------------------------------ unit-test hooks
+// Direct access to the restated vector / polarity / COM arithmetic for known-answer tests
+// (tests/test_vector.cpp:91-105, tests/test_polarity.cpp:28-96 of the reference).
+//   op 0: dot(a, b)                       a = in[0..3), b = in[3..6)         -> out[0]
+//   op 1: length_squared(a)                                                   -> out[0]
+//   op 2: distance_to_squared(a, b)                                           -> out[0]
+int orc_vector_op(int op, const float* in, float* out) {
+  switch (op) {
+    case 0: out[0] = dot3(in, in + 3); return TECS_OK;
+    case 1: out[0] = dot3(in, in); return TECS_OK;
+    case 2: out[0] = dist2(in, in + 3); return TECS_OK;
+    default: return fail(TECS_ERR_INVALID, "bad vector op");
+  }
+}
+//   op 0: Polarity_(vector, distance)   in = v[3], d                         -> out[0..2)
+//   op 1: Polarity_(vector)             in = v[3] (length computed)          -> out[0..2)
+//   op 2: to_vector3(p)                 in = p[2]                            -> out[0..3)
+//   op 3: p.dot(q)                      in = p[2], q[2]                      -> out[0]
+//   op 4: p.unidirectional(q)           in = p[2], q[2]                      -> out[0..2)
+//   op 5: p.bidirectional(q)            in = p[2], q[2]                      -> out[0..2)
+//   op 6: p.bending_force(disp, q, d)   in = p[2], disp[3], q[2], d          -> vec[3], pol[2]
+int orc_polarity_op(int op, const float* in, float* out) {
+  switch (op) {
+    case 0: pol_from_vec3(in, in[3], out); return TECS_OK;
+    case 1: {
+      const float len = std::sqrt(dot3(in, in));
+      pol_from_vec3(in, len, out);
+      return TECS_OK;
+    }
+    case 2: pol_to_vec3(in, out); return TECS_OK;
+    case 3: out[0] = pol_dot(in, in + 2); return TECS_OK;
+    case 4: pol_unidirectional(in, in + 2, out); return TECS_OK;
+    case 5: pol_bidirectional(in, in + 2, out); return TECS_OK;
+    case 6: pol_bending(in, in + 2, in + 5, in[7], out, out + 3); return TECS_OK;
+    default: return fail(TECS_ERR_INVALID, "bad polarity op");
+  }
+}
+// GlobalComFixer::fix on a host array of `capacity` position deltas (com_fixers.hpp:35-43).
+int orc_com_fix(const float* deltas, unsigned capacity, float* com) {
+  float t[3] = {0.0f, 0.0f, 0.0f};
+  if (capacity == 0) {
+    com[0] = com[1] = com[2] = 0.0f;
+    return TECS_OK;
+  }
+  for (size_t i = 0; i < capacity; ++i)
+    for (int c = 0; c < 3; ++c) t[c] += deltas[3 * i + c];
+  for (int c = 0; c < 3; ++c) com[c] = t[c] / (float)capacity;
+  return TECS_OK;
+}
+
 }  // extern "C"

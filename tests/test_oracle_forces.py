@@ -1,0 +1,152 @@
+"""Pins the oracle's force arithmetic.
+
+  * the reference's own known-answer tests for VectorN and Polarity_
+    (tests/test_vector.cpp:91-105, tests/test_polarity.cpp:28-96)
+  * closed forms: two-body PiecewiseLinear magnitude and direction, Newton's third law,
+    the type table, zero bending torque/force at parallel and perpendicular polarities, zero
+    alignment force for equal and for orthogonal polarities
+  * FP32 oracle forces within 1e-5 of the FP64 evaluation (per component, relative to the sum
+    of absolute pair contributions plus the field's RMS force, oracle.force_excess)
+  * the FP64 evaluator against closed forms as well (so it cannot share a mistake unseen)
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from tecs_amd import abi, workloads as W
+
+
+# ------------------------------------------------------------- reference KATs (vector.hpp)
+def test_vector_known_answers(oracle_lib):
+    o = oracle_lib
+    v1 = [2.0, 2.0, 2.0]
+    v2 = [3.0, 3.0, 3.0]
+    assert o.vector_op(0, v1, v2) == np.float32(6 * 3)  # test_vector.cpp:95-96
+    assert o.vector_op(1, v1) == np.float32(4 * 3)      # :98
+    assert o.vector_op(2, v1, v2) == np.float32(1 * 3)  # :99
+
+
+# ---------------------------------------------------------- reference KATs (polarity.hpp)
+def test_polarity_known_answers(oracle_lib):
+    o = oracle_lib
+    half_pi = np.float32(math.pi) / np.float32(2)
+    p = o.polarity_op(0, [0, 0, 1], 1.0)[:2]  # +Z, test_polarity.cpp:33-36
+    assert abs(p[0]) <= 1e-5 and abs(p[1]) <= 1e-5
+    p = o.polarity_op(1, [1, 0, 0])[:2]  # +X, :39-42
+    assert abs(p[0] - half_pi) <= 1e-5 and abs(p[1]) <= 1e-5
+    v = o.polarity_op(2, p)[:3]  # back to a vector, :45-48
+    assert np.all(np.abs(v - [1, 0, 0]) <= 1e-4)
+    pX, pY, pZ = [half_pi, 0], [half_pi, half_pi], [0, 0]  # :58-70
+    for a in (pX, pY, pZ):
+        assert abs(o.polarity_op(3, a, a)[0] - 1) < 1e-4
+    for a, b in ((pX, pY), (pX, pZ), (pY, pZ)):
+        assert abs(o.polarity_op(3, a, b)[0]) < 1e-4
+    p1, p2 = [0.5, 0.5], [0.6, 0.6]  # :77-93 (finite results)
+    assert np.isfinite(o.polarity_op(4, p1, p2)[:2]).all()
+    assert np.isfinite(o.polarity_op(5, p1, p2)[:2]).all()
+    assert np.isfinite(o.polarity_op(6, p1, [1, 0, 0], p2, 1.0)[:5]).all()
+
+
+def test_polarity_closed_forms(oracle_lib):
+    o = oracle_lib
+    rng = W.rng(11)
+    for _ in range(50):
+        t, f = rng.uniform(0.3, 2.8), rng.uniform(-3, 3)
+        # identical polarities: no alignment force; orthogonal ones: dot = 0 so none either
+        assert np.all(np.abs(o.polarity_op(5, [t, f], [t, f])[:2]) < 1e-6)
+        n = np.array([math.sin(t) * math.cos(f), math.sin(t) * math.sin(f), math.cos(t)])
+        # a unit vector orthogonal to n
+        q = np.cross(n, [0.3, -0.7, 0.64])
+        q /= np.linalg.norm(q)
+        qt, qf = math.acos(q[2]), math.atan2(q[1], q[0])
+        assert abs(o.polarity_op(3, [t, f], [qt, qf])[0]) < 1e-6
+        assert np.all(np.abs(o.polarity_op(5, [t, f], [qt, qf])[:2]) < 1e-6)
+        # bending: both polarities along the displacement (n || r) or both perpendicular to it
+        # (n . r = 0) give no position force (polarity.hpp:194-195 vanishes)
+        d = rng.uniform(0.5, 1.0)
+        disp = (n * d).astype(np.float32)
+        out = o.polarity_op(6, [t, f], disp, [t, f], d)
+        assert np.all(np.abs(out[:3]) < 1e-5)
+        out = o.polarity_op(6, [t, f], (q * d).astype(np.float32), [t, f], d)
+        assert np.all(np.abs(out[:5]) < 1e-5)
+
+
+# -------------------------------------------------------------------- PiecewiseLinear
+def _two_body(oracle_lib, sep, direction, table=None, types=None):
+    pos = np.zeros((2, 3), np.float32)
+    pos[1] = (np.asarray(direction) * sep).astype(np.float32)
+    kw = {}
+    if table is not None:
+        kw = dict(types=True, adhesion_table=table)
+    d = abi.make_desc(2, pair_finder=abi.NAIVE_ALL_PAIRS, integrator=abi.EULER, **kw)
+    s = oracle_lib.OracleSim(d)
+    s.upload(pos, None, types)
+    f, _ = s.evaluate_force(True)
+    return pos, f
+
+
+@pytest.mark.parametrize("sep", [0.3, 0.6, 0.79, 0.81, 0.9, 0.999])
+def test_piecewise_linear_two_body(oracle_lib, sep):
+    u = np.array([0.48, -0.6, 0.64])
+    pos, f = _two_body(oracle_lib, sep, u)
+    d = float(np.linalg.norm(pos[1].astype(np.float64)))
+    mag = 2.0 * max(0.8 - d, 0.0) - 1.0 * max(d - 0.8, 0.0)  # predefined.hpp:115-116
+    # force on cell 0 points along p0 - p1 = -u (repulsion for mag > 0)
+    assert np.allclose(f[0], -u * mag, rtol=0, atol=2e-6 * max(1.0, abs(mag)))
+    # Newton's third law holds exactly: disp_ji = -disp_ij and F(d) is shared
+    assert np.array_equal(f[1], -f[0])
+
+
+def test_piecewise_linear_cutoff_and_zero_distance(oracle_lib):
+    _, f = _two_body(oracle_lib, 1.0, [1.0, 0.0, 0.0])  # d == r_max: no pair
+    assert np.all(f == 0)
+    _, f = _two_body(oracle_lib, 0.0, [1.0, 0.0, 0.0])  # d == 0: zero force (predefined.hpp:134)
+    assert np.all(f == 0)
+
+
+def test_type_table(oracle_lib):
+    u = np.array([1.0, 0.0, 0.0])
+    table = [[1.0, 0.3], [0.3, 1.0]]
+    _, f_ab = _two_body(oracle_lib, 0.9, u, table, np.array([0, 1], np.uint8))
+    _, f_aa = _two_body(oracle_lib, 0.9, u, table, np.array([1, 1], np.uint8))
+    d = float(np.float32(0.9))
+    assert abs(f_ab[0, 0] - 0.3 * (d - 0.8)) < 1e-6   # adhesion pulls cell 0 toward +x
+    assert abs(f_aa[0, 0] - 1.0 * (d - 0.8)) < 1e-6
+
+
+# ---------------------------------------------------------- FP32 vs FP64 (whole systems)
+@pytest.mark.parametrize("name", ["relaxation", "epithelium", "sorting"])
+def test_fp32_forces_within_1e5_of_fp64(oracle_lib, name):
+    w = {"relaxation": W.relaxation(3000), "epithelium": W.epithelium(4000),
+         "sorting": W.sorting(20000)}[name]
+    s = oracle_lib.OracleSim(w.desc)
+    s.upload(w.positions, w.polarities, w.types)
+    f32, p32 = s.evaluate_force(True)
+    rp, cols = s.pairs(False)
+    f64, p64, s64, sp64 = s.forces_f64(rp, cols)
+    assert oracle_lib.force_excess(f32, f64, s64) <= 0.0
+    if p32 is not None:
+        assert oracle_lib.force_excess(p32, p64, sp64) <= 0.0
+    # antisymmetric pair laws: the total force vanishes up to rounding
+    assert np.abs(f64.sum(axis=0)).max() <= 1e-9 * s64.sum()
+
+
+def test_fp64_evaluator_closed_form(oracle_lib):
+    """The FP64 evaluator on a two-body system equals the analytic PiecewiseLinear force."""
+    u = np.array([0.6, 0.0, 0.8])
+    pos = np.zeros((2, 3), np.float32)
+    pos[1] = (u * 0.5).astype(np.float32)
+    d = abi.make_desc(2, pair_finder=abi.NAIVE_ALL_PAIRS, integrator=abi.EULER)
+    s = oracle_lib.OracleSim(d)
+    s.upload(pos)
+    rp, cols = s.pairs(True)
+    f64, _, s64, _ = s.forces_f64(rp, cols)
+    r = pos[0].astype(np.float64) - pos[1].astype(np.float64)
+    dist = np.linalg.norm(r)
+    h = float(np.float32(0.8))  # parameters are float32 (Scalar = float) in the reference
+    expected = r * (2.0 * (h - dist)) / dist
+    assert np.allclose(f64[0], expected, rtol=1e-12, atol=1e-14)
+    assert np.allclose(s64[0], np.abs(expected), rtol=1e-12, atol=1e-14)

@@ -1,0 +1,196 @@
+"""Pins the oracle's pair finders against independent restatements and reference fixtures.
+
+  * all-pairs sets == an independent numpy float32 brute force of the predicate
+    d2 = ((0 + dx^2) + dy^2) + dz^2 < r_max^2 (binned_all_pairs.hpp:147-151)
+  * the visiting order of each row == (neighbour-bin task index, agent index), the order of
+    the bin walk (binned_all_pairs.hpp:117-145) over Kokkos::BinSort's Serial order
+  * the stale-grid set == the closed-form predicate over the last rebuild's bins
+  * Gabriel sets == an independent numpy brute force (naive_gabriel.hpp:68-80); binned Gabriel
+    == naive Gabriel on a fresh grid; Gabriel(c=1) <= Gabriel(c=0.8) <= all-pairs
+  * the main.cpp 11-point table (tests/golden/main_cpp_gabriel.json)
+"""
+from __future__ import annotations
+
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from tecs_amd import abi, workloads as W
+
+GOLDEN = Path(__file__).parent / "golden"
+
+
+def _desc(w, finder, **kw):
+    d = abi.copy_desc(w.desc)
+    d.pair_finder = finder
+    for k, v in kw.items():
+        setattr(d.pair_finder_settings, k, v)
+    return d
+
+
+def _oracle_pairs(oracle_lib, desc, w, full=True, positions=None):
+    s = oracle_lib.OracleSim(desc)
+    s.upload(w.positions if positions is None else positions, w.polarities, w.types)
+    return s.pairs(full)
+
+
+def _rows(rp, cols):
+    return [cols[rp[i]:rp[i + 1]] for i in range(len(rp) - 1)]
+
+
+def _brute_all_pairs(pos, r):
+    p = pos.astype(np.float32)
+    r2 = np.float32(r) * np.float32(r)
+    out = []
+    for i in range(len(p)):
+        d = p[i][None, :] - p  # float32 elementwise, one rounding per operation
+        d2 = ((np.float32(0) + d[:, 0] * d[:, 0]) + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+        m = d2 < r2
+        m[i] = False
+        out.append(np.nonzero(m)[0])
+    return out
+
+
+def _brute_gabriel(pos, r, c):
+    p = pos.astype(np.float32)
+    r2 = np.float32(r) * np.float32(r)
+    c2 = np.float32(c) * np.float32(c)
+    allp = _brute_all_pairs(p, r)
+    out = []
+    for i in range(len(p)):
+        keep = []
+        for j in allp[i]:
+            d = p[i] - p[j]
+            d2 = ((np.float32(0) + d[0] * d[0]) + d[1] * d[1]) + d[2] * d[2]
+            mid = p[i] - d * np.float32(0.5)
+            rho2 = d2 * np.float32(0.25) * c2
+            o = p - mid
+            o2 = ((np.float32(0) + o[:, 0] * o[:, 0]) + o[:, 1] * o[:, 1]) + o[:, 2] * o[:, 2]
+            o2[[i, j]] = np.inf
+            if not (o2 < rho2).any():
+                keep.append(j)
+        out.append(np.asarray(keep, np.int64))
+    return out
+
+
+def _bins_div(pos, desc):
+    ps = desc.pair_finder_settings
+    b = np.float32(ps.bin_size_scale) * np.float32(desc.cutoff_distance)
+    mn = np.asarray(ps.min_bounds, np.float32)
+    ext = np.ceil((np.asarray(ps.max_bounds, np.float32) - mn) / b).astype(np.int64)
+    c = ((pos.astype(np.float32) - mn) / b).astype(np.int64)  # trunc toward zero for >= 0
+    neg = (pos.astype(np.float32) - mn) / b < 0
+    c[neg] = 0
+    return np.clip(c, 0, ext - 1), ext
+
+
+def _bins_mul(pos, desc):
+    ps = desc.pair_finder_settings
+    b = np.float32(ps.bin_size_scale) * np.float32(desc.cutoff_distance)
+    inv = np.float32(1.0) / b
+    mn = np.asarray(ps.min_bounds, np.float32)
+    ext = np.ceil((np.asarray(ps.max_bounds, np.float32) - mn) / b).astype(np.int64)
+    v = (pos.astype(np.float32) - mn) * inv
+    c = np.trunc(v).astype(np.int64)
+    return np.clip(c, 0, ext - 1), ext
+
+
+@pytest.mark.parametrize("n", [400, 2500])
+def test_all_pairs_equal_brute_force(oracle_lib, n):
+    w = W.relaxation(n)
+    expected = _brute_all_pairs(w.positions, 1.0)
+    for finder in (abi.NAIVE_ALL_PAIRS, abi.BINNED_ALL_PAIRS):
+        rows = _rows(*_oracle_pairs(oracle_lib, _desc(w, finder), w))
+        for i in range(n):
+            assert np.array_equal(np.sort(rows[i]), expected[i]), (finder, i)
+
+
+def test_all_pairs_symmetric_on_fresh_grid(oracle_lib):
+    w = W.sorting(20000)
+    rp, cols = _oracle_pairs(oracle_lib, w.desc, w)
+    i = np.repeat(np.arange(w.n), np.diff(rp))
+    a = np.sort(i * w.n + cols)
+    b = np.sort(cols.astype(np.int64) * w.n + i)
+    assert np.array_equal(a, b)
+
+
+def test_binned_visiting_order(oracle_lib):
+    """Rows follow (task index of j's bin relative to i's centre bin, agent index j)."""
+    w = W.relaxation(3000)
+    d = w.desc
+    rows = _rows(*_oracle_pairs(oracle_lib, d, w))
+    bi, _ = _bins_mul(w.positions, d)   # centre bin of i (binned_all_pairs.hpp:103-109)
+    bj, _ = _bins_div(w.positions, d)   # grid bin of j (grid.hpp:222-230)
+    side = 3
+    for i in range(w.n):
+        js = rows[i]
+        off = bj[js] - bi[i] + 1
+        task = off[:, 0] + side * (off[:, 1] + side * off[:, 2])
+        order = np.lexsort((js, task))
+        assert np.array_equal(js, js[order]), i
+
+
+def test_stale_grid_predicate(oracle_lib):
+    """(i, j) in P  <=>  d2(now) < r^2 and |mulbin(p_i now) - divbin(p_j at rebuild)|_inf <= 1."""
+    w = W.sorting(6000)
+    s = oracle_lib.OracleSim(w.desc)
+    s.upload(w.positions, None, w.types)
+    s.rebuild()
+    moved = (w.positions + W.rng(5).uniform(-0.4, 0.4, w.positions.shape)).astype(np.float32)
+    s.upload(moved, None, w.types)
+    rows = _rows(*s.pairs(False))
+    stale, _ = _bins_div(w.positions, w.desc)
+    now, _ = _bins_mul(moved, w.desc)
+    within = _brute_all_pairs(moved, 1.0)
+    n_diff = 0
+    for i in range(w.n):
+        js = within[i]
+        ok = np.abs(stale[js] - now[i]).max(axis=1) <= 1 if len(js) else np.zeros(0, bool)
+        exp = js[ok]
+        assert np.array_equal(np.sort(rows[i]), exp), i
+        n_diff += int((~ok).sum())
+    assert n_diff > 0, "the displacement should make some pairs invisible to the stale grid"
+
+
+@pytest.mark.parametrize("name", ["relaxation", "epithelium"])
+def test_gabriel_equals_brute_force(oracle_lib, name):
+    w = W.relaxation(500) if name == "relaxation" else W.epithelium(600)
+    expected = _brute_gabriel(w.positions, 1.0, 0.8)
+    for finder in (abi.NAIVE_GABRIEL, abi.BINNED_GABRIEL):
+        rows = _rows(*_oracle_pairs(oracle_lib, _desc(w, finder), w))
+        for i in range(w.n):
+            assert np.array_equal(np.sort(rows[i]), expected[i]), (finder, i)
+
+
+def test_gabriel_binned_equals_naive_and_nested(oracle_lib):
+    w = W.relaxation(3000)
+    naive = _rows(*_oracle_pairs(oracle_lib, _desc(w, abi.NAIVE_GABRIEL), w))
+    binned = _rows(*_oracle_pairs(oracle_lib, _desc(w, abi.BINNED_GABRIEL), w))
+    strict = _rows(*_oracle_pairs(oracle_lib, _desc(w, abi.BINNED_GABRIEL, gabriel_coefficient=1.0), w))
+    allp = _rows(*_oracle_pairs(oracle_lib, _desc(w, abi.BINNED_ALL_PAIRS), w))
+    fewer = 0
+    for i in range(w.n):
+        assert set(naive[i]) == set(binned[i])
+        assert set(strict[i]) <= set(binned[i]) <= set(allp[i])
+        fewer += len(binned[i]) < len(allp[i])
+    assert fewer > w.n // 2
+
+
+def test_main_cpp_eleven_point_fixture(oracle_lib):
+    g = json.loads((GOLDEN / "main_cpp_gabriel.json").read_text())
+    assert g["min_decision_margin"] > 1e-3  # float32 evaluation cannot flip a decision
+    pts = np.zeros((11, 3), np.float32)
+    pts[:, :2] = np.asarray(g["points"], np.float32)
+    expected = sorted(map(tuple, g["ordered_pairs"]))
+    for finder in (abi.NAIVE_GABRIEL, abi.BINNED_GABRIEL):
+        d = abi.make_desc(11, cutoff_distance=2.5, pair_finder=finder, integrator=abi.EULER)
+        s = oracle_lib.OracleSim(d)
+        s.upload(pts)
+        rp, cols = s.pairs(True)
+        got = sorted((i, int(j)) for i in range(11) for j in cols[rp[i]:rp[i + 1]])
+        assert got == expected
+        counts = np.diff(rp)
+        assert counts.tolist() == g["neighbour_counts"]
+        assert len(got) // 2 == g["total_links"] == 16
