@@ -74,6 +74,8 @@ def main():
     ap.add_argument("--n", type=int, default=200_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run the streaming copy kernel used to calibrate HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this engine runs each simulation on a single device; "
@@ -85,6 +87,18 @@ def main():
     if device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if args.copy_probe:
+        import ctypes
+
+        from tecs_amd import _lib
+
+        ms = ctypes.c_double(0.0)
+        nbytes = 1 << 30
+        _lib.check(_lib.load().tecs_copy_probe(nbytes, 5, ctypes.byref(ms)))
+        print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes,
+                          "launches": 6, "mean_ms": ms.value,
+                          "GB/s (read+write)": 2 * nbytes / (ms.value * 1e-3) / 1e9}))
+        return
     w = workloads.epithelium(args.n)
     sim = GpuSimulation(w.desc)
     sim.upload(w.positions, w.polarities, w.types)

@@ -208,6 +208,12 @@ int tecs_batch_set_kernel_timing(tecs_batch* batch, int32_t enabled);
 int tecs_batch_get_kernel_timing(tecs_batch* batch, int32_t kernel_class, double* ms,
                                  int64_t* launches);
 
+/* Measurement helper with no counterpart in the reference: copies `bytes` (rounded down to a
+ * multiple of 16) between two device buffers it allocates, `repeats` times, with a plain
+ * streaming copy kernel (k_copy_probe), and reports the mean kernel time. Profilers read the
+ * HBM byte counters of this kernel to calibrate them against a known traffic. */
+int tecs_copy_probe(int64_t bytes, int32_t repeats, double* mean_ms);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

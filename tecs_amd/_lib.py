@@ -47,6 +47,7 @@ PROTOTYPES = [
     ("tecs_batch_reset_stats", C.c_int, [_P]),
     ("tecs_batch_set_kernel_timing", C.c_int, [_P, _I32]),
     ("tecs_batch_get_kernel_timing", C.c_int, [_P, _I32, C.POINTER(C.c_double), _I64P]),
+    ("tecs_copy_probe", C.c_int, [C.c_int64, _I32, C.POINTER(C.c_double)]),
 ]
 
 

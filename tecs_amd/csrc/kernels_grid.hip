@@ -190,7 +190,18 @@ __global__ __launch_bounds__(kBlock) void k_download(const InstanceDesc* inst, i
   }
 }
 
+__global__ __launch_bounds__(kBlock) void k_copy_probe(const float4* __restrict__ src,
+                                                       float4* __restrict__ dst, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    dst[i] = src[i];
+}
+
 }  // namespace
+
+void launch_copy_probe(const float4* src, float4* dst, int64_t n, hipStream_t stream) {
+  hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(kBlock), 0, stream, src, dst, n);
+}
 
 int scan_tiles_for(int n) { return (n + kScanTile - 1) / kScanTile; }
 

@@ -883,4 +883,32 @@ int tecs_batch_get_kernel_timing(tecs_batch* batch, int32_t kernel_class, double
   return TECS_OK;
 }
 
+int tecs_copy_probe(int64_t bytes, int32_t repeats, double* mean_ms) {
+  if (bytes < 16 || repeats < 1 || !mean_ms) return fail(TECS_ERR_INVALID, "bad copy probe request");
+  const int64_t n = bytes / 16;
+  void *src = nullptr, *dst = nullptr;
+  hipEvent_t a = nullptr, b = nullptr;
+  int rc = TECS_OK;
+  float ms = 0.0f;
+  auto check = [&](hipError_t e) {
+    if (e != hipSuccess && rc == TECS_OK) rc = fail(TECS_ERR_DEVICE, hipGetErrorString(e));
+    return e == hipSuccess;
+  };
+  if (check(hipMalloc(&src, n * 16)) && check(hipMalloc(&dst, n * 16)) &&
+      check(hipMemset(src, 0, n * 16)) && check(hipEventCreate(&a)) && check(hipEventCreate(&b))) {
+    tecs_amd::launch_copy_probe((const float4*)src, (float4*)dst, n, nullptr);  // warm-up
+    check(hipEventRecord(a, nullptr));
+    for (int r = 0; r < repeats; ++r)
+      tecs_amd::launch_copy_probe((const float4*)src, (float4*)dst, n, nullptr);
+    check(hipEventRecord(b, nullptr));
+    if (check(hipEventSynchronize(b)) && check(hipEventElapsedTime(&ms, a, b)))
+      *mean_ms = ms / repeats;
+  }
+  if (a) hipEventDestroy(a);
+  if (b) hipEventDestroy(b);
+  if (src) hipFree(src);
+  if (dst) hipFree(dst);
+  return rc;
+}
+
 }  // extern "C"

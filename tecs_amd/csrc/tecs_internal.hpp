@@ -95,4 +95,6 @@ void launch_download(const InstanceDesc* inst, int ii, int block_begin, int n_bl
                      const int* id, const float4* x, const float2* p, const float4* ov,
                      float* pos, float* pol, float* old_vel, hipStream_t stream);
 
+void launch_copy_probe(const float4* src, float4* dst, int64_t n, hipStream_t stream);
+
 }  // namespace tecs_amd
