@@ -172,6 +172,15 @@ int tecs_sim_synchronize(tecs_sim* sim);
 int tecs_sim_get_stats(tecs_sim* sim, tecs_stats* stats);
 int tecs_sim_reset_stats(tecs_sim* sim);
 
+/* Pair-sweep kernel variant. Both produce identical results; THREAD walks one cell per thread,
+ * TEAM splits a cell over 16 lanes (binned finders with a search radius of one bin and, for
+ * Gabriel, gabriel_coefficient <= 0.95; otherwise THREAD is used). AUTO (default) picks TEAM
+ * when it applies. This tuning knob has no counterpart in the reference. */
+enum { TECS_SWEEP_AUTO = 0, TECS_SWEEP_THREAD = 1, TECS_SWEEP_TEAM = 2 };
+int tecs_sim_set_sweep_variant(tecs_sim* sim, int32_t variant);
+/* The variant the next sweep will use (TECS_SWEEP_THREAD or TECS_SWEEP_TEAM). */
+int tecs_sim_get_sweep_variant(tecs_sim* sim, int32_t* variant);
+
 /* Kernel timing with HIP events recorded on the simulation's stream around every launch of a
  * kernel class. Enabling adds two events per timed launch. */
 enum {
@@ -204,6 +213,8 @@ int tecs_batch_take_step(tecs_batch* batch, const double* dts, int32_t n_steps);
 int tecs_batch_synchronize(tecs_batch* batch);
 int tecs_batch_get_stats(tecs_batch* batch, tecs_stats* stats);
 int tecs_batch_reset_stats(tecs_batch* batch);
+int tecs_batch_set_sweep_variant(tecs_batch* batch, int32_t variant);
+int tecs_batch_get_sweep_variant(tecs_batch* batch, int32_t* variant);
 int tecs_batch_set_kernel_timing(tecs_batch* batch, int32_t enabled);
 int tecs_batch_get_kernel_timing(tecs_batch* batch, int32_t kernel_class, double* ms,
                                  int64_t* launches);

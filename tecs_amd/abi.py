@@ -38,6 +38,10 @@ FIELD_POLARITY = 1
 FIELD_TYPE = 2
 MAX_TYPES = 4
 
+SWEEP_AUTO = 0
+SWEEP_THREAD = 1
+SWEEP_TEAM = 2
+
 KT_SWEEP_FULL = 0
 KT_SWEEP_STALE = 1
 KT_REBUILD = 2

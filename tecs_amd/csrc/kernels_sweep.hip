@@ -327,12 +327,14 @@ void launch_mode(int mode, bool has_type, bool has_pol, const SweepArgs& a, int 
 // GlobalComFixer::fix, com_fixers.hpp:35-43: sum of the position deltas over the capacity,
 // divided by the capacity; deterministic fixed-order reduction of the per-block partials.
 __global__ __launch_bounds__(kBlock) void k_com_finalize(const InstanceDesc* inst,
-                                                         const float4* partial, float4* com) {
+                                                         const float4* partial, int per_block,
+                                                         float4* com) {
   __shared__ float lds[4];
   const InstanceDesc& in = inst[blockIdx.x];
   float v[3] = {0.0f, 0.0f, 0.0f};
-  for (int b = threadIdx.x; b < in.n_blocks; b += kBlock) {
-    const float4 p = partial[in.block_begin + b];
+  const int first = in.block_begin * per_block, count = in.n_blocks * per_block;
+  for (int b = threadIdx.x; b < count; b += kBlock) {
+    const float4 p = partial[first + b];
     v[0] = v[0] + p.x;
     v[1] = v[1] + p.y;
     v[2] = v[2] + p.z;
@@ -404,9 +406,10 @@ void launch_sweep(int finder, bool has_type, bool has_pol, int mode, const Sweep
   }
 }
 
-void launch_com_finalize(const InstanceDesc* inst, int n_inst, const float4* partial, float4* com,
-                         hipStream_t stream) {
-  hipLaunchKernelGGL(k_com_finalize, dim3(n_inst), dim3(kBlock), 0, stream, inst, partial, com);
+void launch_com_finalize(const InstanceDesc* inst, int n_inst, const float4* partial,
+                         int partials_per_block, float4* com, hipStream_t stream) {
+  hipLaunchKernelGGL(k_com_finalize, dim3(n_inst), dim3(kBlock), 0, stream, inst, partial,
+                     partials_per_block, com);
 }
 
 void launch_update(int kind, bool has_pol, const UpdateArgs& a, int n_blocks, hipStream_t stream) {

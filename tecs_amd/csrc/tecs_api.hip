@@ -123,6 +123,8 @@ struct Batch {
   int64_t cols_cap = 0;
   std::vector<void*> allocations;
 
+  int variant = TECS_SWEEP_AUTO;  // requested sweep variant
+  bool team_ok = false;           // the team sweep reproduces this batch exactly
   int xc = 0, ic = 0;  // current position/polarity/old-velocity buffer, current id buffer
   int step_count = 0;
   bool grid_built = false;
@@ -284,10 +286,17 @@ struct Batch {
       HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(int) * (size_t)total_bins, stream));
       HIP_TRY(hipMemsetAsync(off, 0, sizeof(int) * ((size_t)total_bins + 1), stream));
     }
-    TRY(alloc(&block_pairs, n_blocks));
-    TRY(alloc(&com_partial, n_blocks));
+    // team sweep conditions: search radius 1 and, for Gabriel, c <= 0.95 (kernels_team.hip)
+    team_ok = is_binned(finder);
+    for (int k = 0; k < count; ++k) {
+      if (h_inst[k].search_radius != 1) team_ok = false;
+      const float c = ds[k].pair_finder_settings.gabriel_coefficient;
+      if (finder == kBinnedGabriel && !(c >= 0.0f && c <= 0.95f)) team_ok = false;
+    }
+    TRY(alloc(&block_pairs, (size_t)n_blocks * kTeamLanes));
+    TRY(alloc(&com_partial, (size_t)n_blocks * kTeamLanes));
     TRY(alloc(&com, n_inst));
-    HIP_TRY(hipMemsetAsync(block_pairs, 0, sizeof(unsigned long long) * n_blocks, stream));
+    HIP_TRY(hipMemsetAsync(block_pairs, 0, sizeof(unsigned long long) * n_blocks * kTeamLanes, stream));
     HIP_TRY(hipMemsetAsync(com, 0, sizeof(float4) * n_inst, stream));
     for (int b = 0; b < 2; ++b) {
       HIP_TRY(hipMemsetAsync(x[b], 0, sizeof(float4) * S, stream));
@@ -425,10 +434,19 @@ struct Batch {
     return a;
   }
 
+  bool use_team() const {
+    if (!team_ok) return false;
+    return variant != TECS_SWEEP_THREAD;
+  }
+  int partials_per_block() const { return use_team() ? kTeamLanes : 1; }
+
   int sweep(int mode, const SweepArgs& a, int cls) {
     Pending p{};
     TRY(time_begin(&p, cls));
-    launch_sweep(finder, has_type, has_pol, mode, a, n_blocks, stream);
+    if (use_team())
+      launch_sweep_team(finder == kBinnedGabriel, has_type, has_pol, mode, a, n_blocks, stream);
+    else
+      launch_sweep(finder, has_type, has_pol, mode, a, n_blocks, stream);
     HIP_TRY(hipGetLastError());
     TRY(time_end(&p));
     if (mode != kModeCount && mode != kModeEmit) stats.evaluations++;
@@ -436,7 +454,7 @@ struct Batch {
   }
 
   int com_and_update(int kind) {
-    launch_com_finalize(d_inst, n_inst, com_partial, com, stream);
+    launch_com_finalize(d_inst, n_inst, com_partial, partials_per_block(), com, stream);
     UpdateArgs u{};
     u.inst = d_inst;
     u.block_inst = d_block_inst;
@@ -638,8 +656,8 @@ struct Batch {
   }
 
   int get_stats(tecs_stats* out) {
-    std::vector<unsigned long long> bp(n_blocks);
-    HIP_TRY(hipMemcpyAsync(bp.data(), block_pairs, sizeof(unsigned long long) * n_blocks,
+    std::vector<unsigned long long> bp((size_t)n_blocks * kTeamLanes);
+    HIP_TRY(hipMemcpyAsync(bp.data(), block_pairs, sizeof(unsigned long long) * bp.size(),
                            hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
     tecs_stats s = stats;
@@ -650,7 +668,7 @@ struct Batch {
   }
 
   int reset_stats() {
-    HIP_TRY(hipMemsetAsync(block_pairs, 0, sizeof(unsigned long long) * n_blocks, stream));
+    HIP_TRY(hipMemsetAsync(block_pairs, 0, sizeof(unsigned long long) * n_blocks * kTeamLanes, stream));
     HIP_TRY(hipStreamSynchronize(stream));
     stats = tecs_stats{};
     return TECS_OK;
@@ -791,6 +809,21 @@ int tecs_sim_reset_stats(tecs_sim* sim) {
   return sim->b.reset_stats();
 }
 
+int tecs_sim_set_sweep_variant(tecs_sim* sim, int32_t variant) {
+  GUARD(sim);
+  if (variant < TECS_SWEEP_AUTO || variant > TECS_SWEEP_TEAM)
+    return fail(TECS_ERR_INVALID, "bad sweep variant");
+  sim->b.variant = variant;
+  return TECS_OK;
+}
+
+int tecs_sim_get_sweep_variant(tecs_sim* sim, int32_t* variant) {
+  GUARD(sim);
+  if (!variant) return fail(TECS_ERR_INVALID, "null variant");
+  *variant = sim->b.use_team() ? TECS_SWEEP_TEAM : TECS_SWEEP_THREAD;
+  return TECS_OK;
+}
+
 int tecs_sim_set_kernel_timing(tecs_sim* sim, int32_t enabled) {
   GUARD(sim);
   sim->b.timing = enabled != 0;
@@ -863,6 +896,21 @@ int tecs_batch_get_stats(tecs_batch* batch, tecs_stats* stats) {
 int tecs_batch_reset_stats(tecs_batch* batch) {
   GUARD(batch);
   return batch->b.reset_stats();
+}
+
+int tecs_batch_set_sweep_variant(tecs_batch* batch, int32_t variant) {
+  GUARD(batch);
+  if (variant < TECS_SWEEP_AUTO || variant > TECS_SWEEP_TEAM)
+    return fail(TECS_ERR_INVALID, "bad sweep variant");
+  batch->b.variant = variant;
+  return TECS_OK;
+}
+
+int tecs_batch_get_sweep_variant(tecs_batch* batch, int32_t* variant) {
+  GUARD(batch);
+  if (!variant) return fail(TECS_ERR_INVALID, "null variant");
+  *variant = batch->b.use_team() ? TECS_SWEEP_TEAM : TECS_SWEEP_THREAD;
+  return TECS_OK;
 }
 
 int tecs_batch_set_kernel_timing(tecs_batch* batch, int32_t enabled) {

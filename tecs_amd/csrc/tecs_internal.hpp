@@ -61,8 +61,16 @@ struct UpdateArgs {
 };
 enum UpdateKind { kUpdatePred = 0, kUpdateCorr = 1, kUpdateEuler = 2 };
 void launch_update(int kind, bool has_pol, const UpdateArgs& a, int n_blocks, hipStream_t stream);
-void launch_com_finalize(const InstanceDesc* inst, int n_inst, const float4* partial, float4* com,
-                         hipStream_t stream);
+// partials_per_block: COM partials written per 256-slot block (1, or the team count of the team
+// sweep)
+void launch_com_finalize(const InstanceDesc* inst, int n_inst, const float4* partial,
+                         int partials_per_block, float4* com, hipStream_t stream);
+
+// Team sweep (binned finders, search radius 1): kTeamLanes lanes per cell; block_pairs and
+// com_partial need n_blocks * kTeamLanes entries.
+constexpr int kTeamLanes = 16;
+void launch_sweep_team(bool gabriel, bool has_type, bool has_pol, int mode, const SweepArgs& a,
+                       int n_blocks, hipStream_t stream);
 
 struct RebuildArgs {
   const InstanceDesc* inst;

@@ -110,6 +110,14 @@ class GpuSimulation:
     def reset_stats(self):
         check(self._lib.tecs_sim_reset_stats(self._h))
 
+    def set_sweep_variant(self, variant: int):
+        check(self._lib.tecs_sim_set_sweep_variant(self._h, int(variant)))
+
+    def sweep_variant(self) -> int:
+        v = C.c_int32(0)
+        check(self._lib.tecs_sim_get_sweep_variant(self._h, C.byref(v)))
+        return int(v.value)
+
     def set_kernel_timing(self, enabled: bool):
         check(self._lib.tecs_sim_set_kernel_timing(self._h, 1 if enabled else 0))
 
@@ -177,6 +185,14 @@ class GpuBatch:
 
     def reset_stats(self):
         check(self._lib.tecs_batch_reset_stats(self._h))
+
+    def set_sweep_variant(self, variant: int):
+        check(self._lib.tecs_batch_set_sweep_variant(self._h, int(variant)))
+
+    def sweep_variant(self) -> int:
+        v = C.c_int32(0)
+        check(self._lib.tecs_batch_get_sweep_variant(self._h, C.byref(v)))
+        return int(v.value)
 
     def set_kernel_timing(self, enabled: bool):
         check(self._lib.tecs_batch_set_kernel_timing(self._h, 1 if enabled else 0))

@@ -31,8 +31,13 @@ def _with(desc, **kw):
     return d
 
 
-def _pair(oracle_lib, desc, w, positions=None):
+VARIANTS = [abi.SWEEP_THREAD, abi.SWEEP_TEAM]
+VARIANT_IDS = ["thread", "team"]
+
+
+def _pair(oracle_lib, desc, w, positions=None, variant=abi.SWEEP_AUTO):
     g = GpuSimulation(desc)
+    g.set_sweep_variant(variant)
     o = oracle_lib.OracleSim(desc)
     pos = w.positions if positions is None else positions
     for s in (g, o):
@@ -69,26 +74,30 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("variant", VARIANTS, ids=VARIANT_IDS)
 @pytest.mark.parametrize("finder", FINDERS, ids=lambda f: abi.FINDER_NAMES[f])
 @pytest.mark.parametrize("name,n", CASES)
-def test_pairs_full_step_bit_exact(gpu, oracle_lib, finder, name, n):
+def test_pairs_full_step_bit_exact(gpu, oracle_lib, finder, name, n, variant):
     if finder in (abi.NAIVE_ALL_PAIRS, abi.NAIVE_GABRIEL):
+        if variant == abi.SWEEP_TEAM:
+            pytest.skip("naive finders have one sweep kernel")
         n = min(n, 1500)
     w = _workload(name, n, finder)
-    print(f"case {name} n={n} finder={abi.FINDER_NAMES[finder]}", flush=True)
-    g, o = _pair(oracle_lib, w.desc, w)
+    print(f"case {name} n={n} finder={abi.FINDER_NAMES[finder]} variant={variant}", flush=True)
+    g, o = _pair(oracle_lib, w.desc, w, variant=variant)
     pg = g.pairs(True)
     po = o.pairs(True)
     assert pg[1].size > 0
     _assert_same_csr(pg, po, f"{name}/{abi.FINDER_NAMES[finder]}")
 
 
+@pytest.mark.parametrize("variant", VARIANTS, ids=VARIANT_IDS)
 @pytest.mark.parametrize("finder", [abi.BINNED_ALL_PAIRS, abi.BINNED_GABRIEL], ids=lambda f: abi.FINDER_NAMES[f])
-def test_pairs_stale_grid_bit_exact(gpu, oracle_lib, finder):
+def test_pairs_stale_grid_bit_exact(gpu, oracle_lib, finder, variant):
     """Heun corrector semantics: the grid of the last rebuild with moved positions."""
     w = _workload("sorting", 20000, finder)
-    print(f"case stale grid finder={abi.FINDER_NAMES[finder]}", flush=True)
-    g, o = _pair(oracle_lib, w.desc, w)
+    print(f"case stale grid finder={abi.FINDER_NAMES[finder]} variant={variant}", flush=True)
+    g, o = _pair(oracle_lib, w.desc, w, variant=variant)
     for s in (g, o):
         s.rebuild()
     moved = (w.positions + W.rng(77).uniform(-0.3, 0.3, w.positions.shape)).astype(np.float32)
@@ -97,14 +106,17 @@ def test_pairs_stale_grid_bit_exact(gpu, oracle_lib, finder):
     _assert_same_csr(g.pairs(False), o.pairs(False), "stale grid")
 
 
+@pytest.mark.parametrize("variant", VARIANTS, ids=VARIANT_IDS)
 @pytest.mark.parametrize("finder", FINDERS, ids=lambda f: abi.FINDER_NAMES[f])
 @pytest.mark.parametrize("name,n", [("relaxation", 3000), ("sorting", 20000)])
-def test_forces_bit_exact_piecewise_linear(gpu, oracle_lib, finder, name, n):
+def test_forces_bit_exact_piecewise_linear(gpu, oracle_lib, finder, name, n, variant):
     if finder in (abi.NAIVE_ALL_PAIRS, abi.NAIVE_GABRIEL):
+        if variant == abi.SWEEP_TEAM:
+            pytest.skip("naive finders have one sweep kernel")
         n = min(n, 1500)
     w = _workload(name, n, finder)
-    print(f"case forces {name} n={n} finder={abi.FINDER_NAMES[finder]}", flush=True)
-    g, o = _pair(oracle_lib, w.desc, w)
+    print(f"case forces {name} n={n} finder={abi.FINDER_NAMES[finder]} variant={variant}", flush=True)
+    g, o = _pair(oracle_lib, w.desc, w, variant=variant)
     fg, _ = g.evaluate_force(True)
     fo, _ = o.evaluate_force(True)
     assert np.array_equal(fg.view(np.uint32), fo.view(np.uint32)), "forces differ bitwise"
@@ -113,12 +125,15 @@ def test_forces_bit_exact_piecewise_linear(gpu, oracle_lib, finder, name, n):
     assert _force_scale_check(fg, f64, s64) <= 0.0
 
 
+@pytest.mark.parametrize("variant", VARIANTS, ids=VARIANT_IDS)
 @pytest.mark.parametrize("finder", [abi.BINNED_GABRIEL, abi.NAIVE_GABRIEL], ids=lambda f: abi.FINDER_NAMES[f])
-def test_forces_polarity_within_1e5_of_fp64(gpu, oracle_lib, finder):
+def test_forces_polarity_within_1e5_of_fp64(gpu, oracle_lib, finder, variant):
+    if finder == abi.NAIVE_GABRIEL and variant == abi.SWEEP_TEAM:
+        pytest.skip("naive finders have one sweep kernel")
     n = 4000 if finder == abi.BINNED_GABRIEL else 1500
     w = _workload("epithelium", n, finder)
-    print(f"case polarity forces n={n} finder={abi.FINDER_NAMES[finder]}", flush=True)
-    g, o = _pair(oracle_lib, w.desc, w)
+    print(f"case polarity forces n={n} finder={abi.FINDER_NAMES[finder]} variant={variant}", flush=True)
+    g, o = _pair(oracle_lib, w.desc, w, variant=variant)
     fg, pg = g.evaluate_force(True)
     rp, cols = o.pairs(True)  # same positions, so the same freshly built grid
     f64, p64, s64, sp64 = o.forces_f64(rp, cols)
@@ -128,21 +143,22 @@ def test_forces_polarity_within_1e5_of_fp64(gpu, oracle_lib, finder):
     assert e1 <= 0.0 and e2 <= 0.0
 
 
-def _run_both(oracle_lib, w, desc, steps):
-    g, o = _pair(oracle_lib, desc, w)
+def _run_both(oracle_lib, w, desc, steps, variant=abi.SWEEP_AUTO):
+    g, o = _pair(oracle_lib, desc, w, variant=variant)
     g.take_step(w.dt, steps)
     o.take_step(w.dt, steps)
     return g, o
 
 
+@pytest.mark.parametrize("variant", VARIANTS, ids=VARIANT_IDS)
 @pytest.mark.parametrize(
     "name,n,steps",
     [("relaxation", 3000, 100), ("sorting", 20000, 100)],
 )
-def test_trajectory_bit_exact(gpu, oracle_lib, name, n, steps):
+def test_trajectory_bit_exact(gpu, oracle_lib, name, n, steps, variant):
     w = _workload(name, n, abi.BINNED_ALL_PAIRS)
-    print(f"case trajectory {name} n={n} steps={steps}", flush=True)
-    g, o = _run_both(oracle_lib, w, w.desc, steps)
+    print(f"case trajectory {name} n={n} steps={steps} variant={variant}", flush=True)
+    g, o = _run_both(oracle_lib, w, w.desc, steps, variant)
     xg, _, vg = g.download()
     xo, _, vo = o.download()
     assert np.array_equal(xg.view(np.uint32), xo.view(np.uint32)), (
@@ -152,11 +168,12 @@ def test_trajectory_bit_exact(gpu, oracle_lib, name, n, steps):
     assert g.stats()["pair_interactions"] == o.stats()["pair_interactions"]
 
 
-def test_trajectory_gabriel_relaxation_bit_exact(gpu, oracle_lib):
+@pytest.mark.parametrize("variant", VARIANTS, ids=VARIANT_IDS)
+def test_trajectory_gabriel_relaxation_bit_exact(gpu, oracle_lib, variant):
     w = _workload("relaxation", 3000, abi.BINNED_GABRIEL)
     d = _with(w.desc, integrator=abi.HEUN)
-    print("case trajectory relaxation/BinnedGabriel Heun 50 steps", flush=True)
-    g, o = _run_both(oracle_lib, w, d, 50)
+    print(f"case trajectory relaxation/BinnedGabriel Heun 50 steps variant={variant}", flush=True)
+    g, o = _run_both(oracle_lib, w, d, 50, variant)
     xg, _, _ = g.download()
     xo, _, _ = o.download()
     assert np.array_equal(xg.view(np.uint32), xo.view(np.uint32))
@@ -221,3 +238,44 @@ def test_batch_member_equals_alone(gpu):
         xb, _, vb = b.download(k)
         assert np.array_equal(xa.view(np.uint32), xb.view(np.uint32)), f"member {k} differs"
         assert np.array_equal(va.view(np.uint32), vb.view(np.uint32))
+
+
+@pytest.mark.parametrize("finder", [abi.BINNED_ALL_PAIRS, abi.BINNED_GABRIEL], ids=lambda f: abi.FINDER_NAMES[f])
+def test_dense_cluster_team_overflow_path(gpu, oracle_lib, finder):
+    """~110 neighbours per cell: the team sweep's LDS list overflows and the leader walks the
+    cell serially; pairs and forces stay bit-exact."""
+    g0 = W.rng(21)
+    n = 3000
+    pos = W.ball(n, 3.0, g0)
+    lo, hi = W.bounds(pos)
+    d = abi.make_desc(n, pair_finder=finder, min_bounds=lo, max_bounds=hi, integrator=abi.HEUN)
+    w = W.Workload("dense", d, pos, None, None, 0.05, 1)
+    print(f"case dense cluster finder={abi.FINDER_NAMES[finder]}", flush=True)
+    g, o = _pair(oracle_lib, d, w, variant=abi.SWEEP_TEAM)
+    assert g.sweep_variant() == abi.SWEEP_TEAM
+    _assert_same_csr(g.pairs(True), o.pairs(True), "dense")
+    fg, _ = g.evaluate_force(True)
+    fo, _ = o.evaluate_force(True)
+    assert np.array_equal(fg.view(np.uint32), fo.view(np.uint32))
+
+
+def test_team_variant_requires_search_radius_one(gpu):
+    w = _workload("sorting", 5000, abi.BINNED_ALL_PAIRS)
+    d = _with(w.desc, bin_size_scale=0.5)
+    g = GpuSimulation(d)
+    g.set_sweep_variant(abi.SWEEP_TEAM)
+    assert g.sweep_variant() == abi.SWEEP_THREAD
+
+
+@pytest.mark.parametrize("scale", [0.5, 2.0])
+@pytest.mark.parametrize("finder", [abi.BINNED_ALL_PAIRS, abi.BINNED_GABRIEL], ids=lambda f: abi.FINDER_NAMES[f])
+def test_bin_size_scale_bit_exact(gpu, oracle_lib, finder, scale):
+    """Bins of half / twice the cutoff: search radius 2 (thread sweep) and 1 (team sweep)."""
+    w = _workload("relaxation", 3000, finder)
+    d = _with(w.desc, bin_size_scale=scale)
+    print(f"case bin_size_scale={scale} finder={abi.FINDER_NAMES[finder]}", flush=True)
+    g, o = _pair(oracle_lib, d, w)
+    _assert_same_csr(g.pairs(True), o.pairs(True), f"scale {scale}")
+    fg, _ = g.evaluate_force(True)
+    fo, _ = o.evaluate_force(True)
+    assert np.array_equal(fg.view(np.uint32), fo.view(np.uint32))
