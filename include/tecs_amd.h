@@ -181,6 +181,16 @@ int tecs_sim_set_sweep_variant(tecs_sim* sim, int32_t variant);
 /* The variant the next sweep will use (TECS_SWEEP_THREAD or TECS_SWEEP_TEAM). */
 int tecs_sim_get_sweep_variant(tecs_sim* sim, int32_t* variant);
 
+/* hipGraph replay of step sequences inside tecs_sim_take_step. AUTO (default) captures and
+ * replays when a call runs at least 4 graph periods of steps (a period is
+ * 2 * max(1, rebuild_every_n) steps) and kernel timing is off; ON replays whenever a whole
+ * period fits; OFF always launches kernel by kernel. Results are identical in every mode.
+ * No counterpart in the reference. */
+enum { TECS_GRAPHS_AUTO = 0, TECS_GRAPHS_OFF = 1, TECS_GRAPHS_ON = 2 };
+int tecs_sim_set_graph_mode(tecs_sim* sim, int32_t mode);
+/* Number of graph launches so far (each runs one period of steps). */
+int tecs_sim_get_graph_launches(tecs_sim* sim, int64_t* launches);
+
 /* Kernel timing with HIP events recorded on the simulation's stream around every launch of a
  * kernel class. Enabling adds two events per timed launch. */
 enum {
@@ -215,6 +225,8 @@ int tecs_batch_get_stats(tecs_batch* batch, tecs_stats* stats);
 int tecs_batch_reset_stats(tecs_batch* batch);
 int tecs_batch_set_sweep_variant(tecs_batch* batch, int32_t variant);
 int tecs_batch_get_sweep_variant(tecs_batch* batch, int32_t* variant);
+int tecs_batch_set_graph_mode(tecs_batch* batch, int32_t mode);
+int tecs_batch_get_graph_launches(tecs_batch* batch, int64_t* launches);
 int tecs_batch_set_kernel_timing(tecs_batch* batch, int32_t enabled);
 int tecs_batch_get_kernel_timing(tecs_batch* batch, int32_t kernel_class, double* ms,
                                  int64_t* launches);

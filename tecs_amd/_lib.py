@@ -37,6 +37,8 @@ PROTOTYPES = [
     ("tecs_sim_reset_stats", C.c_int, [_P]),
     ("tecs_sim_set_sweep_variant", C.c_int, [_P, _I32]),
     ("tecs_sim_get_sweep_variant", C.c_int, [_P, C.POINTER(C.c_int32)]),
+    ("tecs_sim_set_graph_mode", C.c_int, [_P, _I32]),
+    ("tecs_sim_get_graph_launches", C.c_int, [_P, _I64P]),
     ("tecs_sim_set_kernel_timing", C.c_int, [_P, _I32]),
     ("tecs_sim_get_kernel_timing", C.c_int, [_P, _I32, C.POINTER(C.c_double), _I64P]),
     ("tecs_batch_create", C.c_int, [C.POINTER(abi.SimDesc), _I32, _P, C.POINTER(_P)]),
@@ -49,6 +51,8 @@ PROTOTYPES = [
     ("tecs_batch_reset_stats", C.c_int, [_P]),
     ("tecs_batch_set_sweep_variant", C.c_int, [_P, _I32]),
     ("tecs_batch_get_sweep_variant", C.c_int, [_P, C.POINTER(C.c_int32)]),
+    ("tecs_batch_set_graph_mode", C.c_int, [_P, _I32]),
+    ("tecs_batch_get_graph_launches", C.c_int, [_P, _I64P]),
     ("tecs_batch_set_kernel_timing", C.c_int, [_P, _I32]),
     ("tecs_batch_get_kernel_timing", C.c_int, [_P, _I32, C.POINTER(C.c_double), _I64P]),
     ("tecs_copy_probe", C.c_int, [C.c_int64, _I32, C.POINTER(C.c_double)]),

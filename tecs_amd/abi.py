@@ -42,6 +42,10 @@ SWEEP_AUTO = 0
 SWEEP_THREAD = 1
 SWEEP_TEAM = 2
 
+GRAPHS_AUTO = 0
+GRAPHS_OFF = 1
+GRAPHS_ON = 2
+
 KT_SWEEP_FULL = 0
 KT_SWEEP_STALE = 1
 KT_REBUILD = 2

@@ -140,10 +140,38 @@ struct Batch {
   double t_ms[kTimeClasses] = {0, 0, 0};
   int64_t t_launches[kTimeClasses] = {0, 0, 0};
 
+  // hipGraph replay of step sequences: the host-side state (buffer parities, rebuild phase)
+  // repeats with period graph_period(); a sequence of that many steps captured from a state
+  // can be replayed whenever the batch is in that state again. Kernels read dt from the
+  // device-side instance table, so a dt change does not invalidate a graph.
+  int graph_mode = TECS_GRAPHS_AUTO;
+  struct HostState {
+    int xc, ic, step_count;
+    bool operator==(const HostState& o) const {
+      return xc == o.xc && ic == o.ic && step_count == o.step_count;
+    }
+  };
+  struct StepGraph {
+    HostState start;
+    hipGraphExec_t exec;
+    tecs_stats delta;  // counters advanced by one replay
+  };
+  std::vector<StepGraph> graphs;
+  int64_t graph_launches = 0;
+
+  HostState host_state() const { return HostState{xc, ic, step_count}; }
+  int graph_period() const { return 2 * std::max(1, is_binned(finder) ? rebuild_every_n : 1); }
+
+  void drop_graphs() {
+    for (auto& g : graphs) hipGraphExecDestroy(g.exec);
+    graphs.clear();
+  }
+
   ~Batch() { release(); }
 
   void release() {
     if (stream) hipStreamSynchronize(stream);
+    drop_graphs();
     for (auto& p : pending) {
       hipEventDestroy(p.a);
       hipEventDestroy(p.b);
@@ -527,11 +555,78 @@ struct Batch {
     return TECS_OK;
   }
 
+  // Capture graph_period() steps from the current state (the host state advances as if they
+  // ran; the caller launches the graph once to make the device catch up).
+  int capture_graph(StepGraph* out) {
+    const HostState start = host_state();
+    const tecs_stats before = stats;
+    HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    int rc = TECS_OK;
+    const int P = graph_period();
+    for (int k = 0; k < P && rc == TECS_OK; ++k) rc = step_once();
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(stream, &g);
+    if (rc != TECS_OK) {
+      if (g) hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(TECS_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (ei != hipSuccess) return fail(TECS_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    out->start = start;
+    out->exec = exec;
+    out->delta.pair_interactions = 0;
+    out->delta.evaluations = stats.evaluations - before.evaluations;
+    out->delta.rebuilds = stats.rebuilds - before.rebuilds;
+    out->delta.steps = stats.steps - before.steps;
+    if (!(host_state() == start)) {  // not periodic from here: run it once, do not keep it
+      HIP_TRY(hipGraphLaunch(exec, stream));
+      hipGraphExecDestroy(exec);
+      out->exec = nullptr;
+    }
+    return TECS_OK;
+  }
+
   int take_step(const double* dts, int n_steps) {
     if (n_steps < 0) return fail(TECS_ERR_INVALID, "n_steps must be >= 0");
     for (int k = 0; k < n_inst; ++k) set_dt(k, dts[k]);
     TRY(sync_inst());
-    for (int s = 0; s < n_steps; ++s) TRY(step_once());
+    const int P = graph_period();
+    const bool graphs_on = graph_mode == TECS_GRAPHS_ON ||
+                           (graph_mode == TECS_GRAPHS_AUTO && n_steps >= 4 * P);
+    int done = 0;
+    if (graphs_on && !timing) {
+      // the first step after creation or a manual rebuild may start a new rebuild phase
+      while (done < n_steps && (step_count == 0 && is_binned(finder))) {
+        TRY(step_once());
+        ++done;
+      }
+      while (n_steps - done >= P) {
+        const HostState st = host_state();
+        StepGraph* g = nullptr;
+        for (auto& c : graphs)
+          if (c.start == st) g = &c;
+        if (!g) {
+          StepGraph fresh{};
+          TRY(capture_graph(&fresh));
+          done += P;
+          if (!fresh.exec) continue;  // ran once eagerly through the graph
+          graphs.push_back(fresh);
+          HIP_TRY(hipGraphLaunch(graphs.back().exec, stream));
+          ++graph_launches;
+          continue;
+        }
+        HIP_TRY(hipGraphLaunch(g->exec, stream));
+        ++graph_launches;
+        stats.evaluations += g->delta.evaluations;
+        stats.rebuilds += g->delta.rebuilds;
+        stats.steps += g->delta.steps;
+        done += P;
+      }
+    }
+    for (; done < n_steps; ++done) TRY(step_once());
     return TECS_OK;
   }
 
@@ -824,6 +919,20 @@ int tecs_sim_get_sweep_variant(tecs_sim* sim, int32_t* variant) {
   return TECS_OK;
 }
 
+int tecs_sim_set_graph_mode(tecs_sim* sim, int32_t mode) {
+  GUARD(sim);
+  if (mode < TECS_GRAPHS_AUTO || mode > TECS_GRAPHS_ON) return fail(TECS_ERR_INVALID, "bad graph mode");
+  sim->b.graph_mode = mode;
+  return TECS_OK;
+}
+
+int tecs_sim_get_graph_launches(tecs_sim* sim, int64_t* launches) {
+  GUARD(sim);
+  if (!launches) return fail(TECS_ERR_INVALID, "null output");
+  *launches = sim->b.graph_launches;
+  return TECS_OK;
+}
+
 int tecs_sim_set_kernel_timing(tecs_sim* sim, int32_t enabled) {
   GUARD(sim);
   sim->b.timing = enabled != 0;
@@ -910,6 +1019,20 @@ int tecs_batch_get_sweep_variant(tecs_batch* batch, int32_t* variant) {
   GUARD(batch);
   if (!variant) return fail(TECS_ERR_INVALID, "null variant");
   *variant = batch->b.use_team() ? TECS_SWEEP_TEAM : TECS_SWEEP_THREAD;
+  return TECS_OK;
+}
+
+int tecs_batch_set_graph_mode(tecs_batch* batch, int32_t mode) {
+  GUARD(batch);
+  if (mode < TECS_GRAPHS_AUTO || mode > TECS_GRAPHS_ON) return fail(TECS_ERR_INVALID, "bad graph mode");
+  batch->b.graph_mode = mode;
+  return TECS_OK;
+}
+
+int tecs_batch_get_graph_launches(tecs_batch* batch, int64_t* launches) {
+  GUARD(batch);
+  if (!launches) return fail(TECS_ERR_INVALID, "null output");
+  *launches = batch->b.graph_launches;
   return TECS_OK;
 }
 

@@ -118,6 +118,14 @@ class GpuSimulation:
         check(self._lib.tecs_sim_get_sweep_variant(self._h, C.byref(v)))
         return int(v.value)
 
+    def set_graph_mode(self, mode: int):
+        check(self._lib.tecs_sim_set_graph_mode(self._h, int(mode)))
+
+    def graph_launches(self) -> int:
+        n = C.c_int64(0)
+        check(self._lib.tecs_sim_get_graph_launches(self._h, C.byref(n)))
+        return int(n.value)
+
     def set_kernel_timing(self, enabled: bool):
         check(self._lib.tecs_sim_set_kernel_timing(self._h, 1 if enabled else 0))
 
@@ -193,6 +201,14 @@ class GpuBatch:
         v = C.c_int32(0)
         check(self._lib.tecs_batch_get_sweep_variant(self._h, C.byref(v)))
         return int(v.value)
+
+    def set_graph_mode(self, mode: int):
+        check(self._lib.tecs_batch_set_graph_mode(self._h, int(mode)))
+
+    def graph_launches(self) -> int:
+        n = C.c_int64(0)
+        check(self._lib.tecs_batch_get_graph_launches(self._h, C.byref(n)))
+        return int(n.value)
 
     def set_kernel_timing(self, enabled: bool):
         check(self._lib.tecs_batch_set_kernel_timing(self._h, 1 if enabled else 0))

@@ -279,3 +279,34 @@ def test_bin_size_scale_bit_exact(gpu, oracle_lib, finder, scale):
     fg, _ = g.evaluate_force(True)
     fo, _ = o.evaluate_force(True)
     assert np.array_equal(fg.view(np.uint32), fo.view(np.uint32))
+
+
+@pytest.mark.parametrize("name,rebuild_every_n", [("relaxation", 0), ("sorting", 0), ("sorting", 3), ("epithelium", 0)])
+def test_graph_replay_matches_eager(gpu, name, rebuild_every_n):
+    """Steps replayed from a captured hipGraph end bitwise where kernel-by-kernel launches
+    end; checked after every call, with eager work and synchronisations between replays."""
+    w = _workload(name, 6000, W.epithelium(10).desc.pair_finder if name == "epithelium" else abi.BINNED_ALL_PAIRS)
+    d = _with(w.desc, rebuild_every_n=rebuild_every_n)
+    print(f"case graph replay {name} rebuild_every_n={rebuild_every_n}", flush=True)
+    sims = []
+    for mode in (abi.GRAPHS_ON, abi.GRAPHS_OFF):
+        s = GpuSimulation(d)
+        s.set_graph_mode(mode)
+        s.upload(w.positions, w.polarities, w.types)
+        sims.append(s)
+    graph, eager = sims
+    other = GpuSimulation(W.relaxation(2000).desc)
+    other.upload(W.relaxation(2000).positions)
+    for call, steps in enumerate([1, 8, 13, 8, 2, 16]):
+        graph.take_step(w.dt, steps)
+        eager.take_step(w.dt, steps)
+        other.evaluate_force(True)  # unrelated eager work between replays
+        xg, pg, vg = graph.download()
+        xe, pe, ve = eager.download()
+        assert np.array_equal(xg.view(np.uint32), xe.view(np.uint32)), f"call {call}"
+        assert np.array_equal(vg.view(np.uint32), ve.view(np.uint32)), f"call {call}"
+        if pg is not None:
+            assert np.array_equal(pg.view(np.uint32), pe.view(np.uint32)), f"call {call}"
+    assert graph.graph_launches() >= 4
+    assert eager.graph_launches() == 0
+    assert graph.stats() == eager.stats()
