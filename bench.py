@@ -134,7 +134,10 @@ def main():
     dom = max(per, key=lambda k: per[k][0])
     avg_ms, b_cell = per[dom]
     achieved = (b_cell * w.n) / (avg_ms * 1e-3) / 1e9
-    kernel_names = {"full": "k_sweep_binned<Gabriel,pol,Pred>", "stale": "k_sweep_binned<Gabriel,pol,Corr>"}
+    variant = abi.SWEEP_NAMES.get(sim.sweep_variant(), "?")
+    kname = "k_sweep_binned" if variant == "thread" else f"k_sweep_team<{variant[4:]}>"
+    kernel_names = {"full": f"{kname} Gabriel polarity mode Pred (Heun substep 1)",
+                    "stale": f"{kname} Gabriel polarity mode Corr (Heun substep 2, stale grid)"}
     print(
         f"bench: sweep full {per['full'][0]*1e3:.1f} us, stale {per['stale'][0]*1e3:.1f} us, "
         f"rebuild {reb_ms/max(reb_n,1)*1e3:.1f} us per launch",
@@ -175,6 +178,8 @@ def main():
             "sweep_full_us": per["full"][0] * 1e3,
             "sweep_stale_us": per["stale"][0] * 1e3,
             "rebuild_us": reb_ms / max(reb_n, 1) * 1e3,
+            "sweep_variant": variant,
+            "autotune_ms": sim.autotune_ms(),
         },
     }
     if not args.no_cpu_baseline:

@@ -172,14 +172,24 @@ int tecs_sim_synchronize(tecs_sim* sim);
 int tecs_sim_get_stats(tecs_sim* sim, tecs_stats* stats);
 int tecs_sim_reset_stats(tecs_sim* sim);
 
-/* Pair-sweep kernel variant. Both produce identical results; THREAD walks one cell per thread,
- * TEAM splits a cell over 16 lanes (binned finders with a search radius of one bin and, for
- * Gabriel, gabriel_coefficient <= 0.95; otherwise THREAD is used). AUTO (default) picks TEAM
- * when it applies. This tuning knob has no counterpart in the reference. */
-enum { TECS_SWEEP_AUTO = 0, TECS_SWEEP_THREAD = 1, TECS_SWEEP_TEAM = 2 };
+/* Pair-sweep kernel variant. All produce identical results. THREAD walks one cell per
+ * thread; TEAM4/8/16 split a cell over 4, 8 or 16 lanes (binned finders with a search radius
+ * of one bin and, for Gabriel, gabriel_coefficient <= 0.95; otherwise THREAD is used); TEAM
+ * means TEAM8. AUTO (default) lets the autotuner time every applicable variant on the live
+ * state at the first take_step and keep the fastest. No counterpart in the reference. */
+enum {
+  TECS_SWEEP_AUTO = 0,
+  TECS_SWEEP_THREAD = 1,
+  TECS_SWEEP_TEAM = 2,
+  TECS_SWEEP_TEAM4 = 3,
+  TECS_SWEEP_TEAM8 = 4,
+  TECS_SWEEP_TEAM16 = 5
+};
 int tecs_sim_set_sweep_variant(tecs_sim* sim, int32_t variant);
-/* The variant the next sweep will use (TECS_SWEEP_THREAD or TECS_SWEEP_TEAM). */
+/* The variant the next sweep will use (THREAD, TEAM4, TEAM8 or TEAM16). */
 int tecs_sim_get_sweep_variant(tecs_sim* sim, int32_t* variant);
+/* The autotuner's best time in ms per variant, indexed by TECS_SWEEP_* (0 = not timed). */
+int tecs_sim_get_autotune(tecs_sim* sim, double* ms_per_variant /* [6] */);
 
 /* hipGraph replay of step sequences inside tecs_sim_take_step. AUTO (default) captures and
  * replays when a call runs at least 4 graph periods of steps (a period is
@@ -225,6 +235,7 @@ int tecs_batch_get_stats(tecs_batch* batch, tecs_stats* stats);
 int tecs_batch_reset_stats(tecs_batch* batch);
 int tecs_batch_set_sweep_variant(tecs_batch* batch, int32_t variant);
 int tecs_batch_get_sweep_variant(tecs_batch* batch, int32_t* variant);
+int tecs_batch_get_autotune(tecs_batch* batch, double* ms_per_variant /* [6] */);
 int tecs_batch_set_graph_mode(tecs_batch* batch, int32_t mode);
 int tecs_batch_get_graph_launches(tecs_batch* batch, int64_t* launches);
 int tecs_batch_set_kernel_timing(tecs_batch* batch, int32_t enabled);

@@ -37,6 +37,7 @@ PROTOTYPES = [
     ("tecs_sim_reset_stats", C.c_int, [_P]),
     ("tecs_sim_set_sweep_variant", C.c_int, [_P, _I32]),
     ("tecs_sim_get_sweep_variant", C.c_int, [_P, C.POINTER(C.c_int32)]),
+    ("tecs_sim_get_autotune", C.c_int, [_P, C.POINTER(C.c_double)]),
     ("tecs_sim_set_graph_mode", C.c_int, [_P, _I32]),
     ("tecs_sim_get_graph_launches", C.c_int, [_P, _I64P]),
     ("tecs_sim_set_kernel_timing", C.c_int, [_P, _I32]),
@@ -51,6 +52,7 @@ PROTOTYPES = [
     ("tecs_batch_reset_stats", C.c_int, [_P]),
     ("tecs_batch_set_sweep_variant", C.c_int, [_P, _I32]),
     ("tecs_batch_get_sweep_variant", C.c_int, [_P, C.POINTER(C.c_int32)]),
+    ("tecs_batch_get_autotune", C.c_int, [_P, C.POINTER(C.c_double)]),
     ("tecs_batch_set_graph_mode", C.c_int, [_P, _I32]),
     ("tecs_batch_get_graph_launches", C.c_int, [_P, _I64P]),
     ("tecs_batch_set_kernel_timing", C.c_int, [_P, _I32]),

@@ -41,6 +41,10 @@ MAX_TYPES = 4
 SWEEP_AUTO = 0
 SWEEP_THREAD = 1
 SWEEP_TEAM = 2
+SWEEP_TEAM4 = 3
+SWEEP_TEAM8 = 4
+SWEEP_TEAM16 = 5
+SWEEP_NAMES = {1: "thread", 3: "team4", 4: "team8", 5: "team16"}
 
 GRAPHS_AUTO = 0
 GRAPHS_OFF = 1

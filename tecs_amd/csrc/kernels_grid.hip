@@ -190,6 +190,16 @@ __global__ __launch_bounds__(kBlock) void k_download(const InstanceDesc* inst, i
   }
 }
 
+// sin/cos of both polarity angles per slot: the values sinf/cosf give where the force
+// evaluates them (to_vector3 polarity.hpp:111-118, dot :131-136, the sin theta of :153)
+__global__ __launch_bounds__(kBlock) void k_polar_trig(const float2* __restrict__ pol,
+                                                       float4* __restrict__ trig, int n) {
+  const int s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= n) return;
+  const float2 q = pol[s];
+  trig[s] = make_float4(sinf(q.x), cosf(q.x), sinf(q.y), cosf(q.y));
+}
+
 __global__ __launch_bounds__(kBlock) void k_copy_probe(const float4* __restrict__ src,
                                                        float4* __restrict__ dst, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
@@ -198,6 +208,11 @@ __global__ __launch_bounds__(kBlock) void k_copy_probe(const float4* __restrict_
 }
 
 }  // namespace
+
+void launch_polar_trig(const float2* pol, float4* trig, int total_slots, hipStream_t stream) {
+  hipLaunchKernelGGL(k_polar_trig, dim3((total_slots + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                     stream, pol, trig, total_slots);
+}
 
 void launch_copy_probe(const float4* src, float4* dst, int64_t n, hipStream_t stream) {
   hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(kBlock), 0, stream, src, dst, n);

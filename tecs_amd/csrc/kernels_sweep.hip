@@ -28,8 +28,8 @@ struct CellState {
 
 template <bool HAS_TYPE, bool HAS_POL, int MODE>
 __device__ __forceinline__ void visit(const SweepArgs& a, const InstanceDesc& in, CellState& cs,
-                                      float4 me, float2 mp, int m, float4 pj, const float* disp,
-                                      float d2) {
+                                      float4 me, const Pol& pme, int m, float4 pj,
+                                      const float* disp, float d2) {
   cs.pairs++;
   if (MODE == kModeCount) return;
   if (MODE == kModeEmit) {
@@ -37,14 +37,10 @@ __device__ __forceinline__ void visit(const SweepArgs& a, const InstanceDesc& in
     return;
   }
   const float d = sqrtf(d2);
-  float tj = 0.0f, pj_phi = 0.0f;
-  if (HAS_POL) {
-    const float2 q = a.ps[m];
-    tj = q.x;
-    pj_phi = q.y;
-  }
+  Pol po{};
+  if (HAS_POL) po = make_pol(a.ps[m], a.trig[m]);
   apply_force<HAS_TYPE, HAS_POL>(in.force, disp, d, __float_as_int(me.w), __float_as_int(pj.w),
-                                 mp.x, mp.y, tj, pj_phi, cs.acc_pos, cs.acc_pol);
+                                 pme, po, cs.acc_pos, cs.acc_pol);
 }
 
 // Gabriel emptiness test of the ball with diameter c*|p_i - p_j| around the midpoint over the
@@ -80,7 +76,7 @@ __device__ __forceinline__ bool gabriel_blocked(const InstanceDesc& in, const fl
 
 template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE>
 __device__ __forceinline__ void walk_binned(const SweepArgs& a, const InstanceDesc& in, int s,
-                                            float4 me, float2 mp, CellState& cs) {
+                                            float4 me, const Pol& pme, CellState& cs) {
   const float4* __restrict__ xs = a.xs;
   const int* __restrict__ offb = a.off + in.bin_begin;
   int bc[3];
@@ -109,7 +105,7 @@ __device__ __forceinline__ void walk_binned(const SweepArgs& a, const InstanceDe
         if (GABRIEL) {
           if (gabriel_blocked(in, xs, offb, s, m, me, disp, d2)) continue;
         }
-        visit<HAS_TYPE, HAS_POL, MODE>(a, in, cs, me, mp, m, pj, disp, d2);
+        visit<HAS_TYPE, HAS_POL, MODE>(a, in, cs, me, pme, m, pj, disp, d2);
       }
     }
   }
@@ -216,9 +212,13 @@ __global__ __launch_bounds__(kBlock) void k_sweep_binned(SweepArgs a) {
   float2 mp = make_float2(0.0f, 0.0f);
   if (active) {
     me = a.xs[s];
-    if (HAS_POL) mp = a.ps[s];
+    Pol pme{};
+    if (HAS_POL) {
+      mp = a.ps[s];
+      pme = make_pol(mp, a.trig[s]);
+    }
     if (MODE == kModeEmit) cs.emit_at = a.row_ptr[a.id[s]];
-    walk_binned<GABRIEL, HAS_TYPE, HAS_POL, MODE>(a, in, s, me, mp, cs);
+    walk_binned<GABRIEL, HAS_TYPE, HAS_POL, MODE>(a, in, s, me, pme, cs);
   }
   finish<HAS_POL, MODE>(a, in, lb, s, active, me, mp, cs);
 }
@@ -237,9 +237,13 @@ __global__ __launch_bounds__(kBlock) void k_sweep_naive(SweepArgs a) {
   CellState cs{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f}, 0u, 0};
   float4 me = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   float2 mp = make_float2(0.0f, 0.0f);
+  Pol pme{};
   if (active) {
     me = a.xs[s];
-    if (HAS_POL) mp = a.ps[s];
+    if (HAS_POL) {
+      mp = a.ps[s];
+      pme = make_pol(mp, a.trig[s]);
+    }
     if (MODE == kModeEmit) cs.emit_at = a.row_ptr[a.id[s]];
   }
   const int begin = in.slot_begin, n = in.n;
@@ -274,7 +278,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_naive(SweepArgs a) {
         }
         if (blocked) continue;
       }
-      visit<HAS_TYPE, HAS_POL, MODE>(a, in, cs, me, mp, m, pj, disp, d2);
+      visit<HAS_TYPE, HAS_POL, MODE>(a, in, cs, me, pme, m, pj, disp, d2);
     }
   }
   finish<HAS_POL, MODE>(a, in, lb, s, active, me, mp, cs);

@@ -1,39 +1,53 @@
-// kernels_team.hip - binned sweeps with a team of kTeam lanes per cell.
+// kernels_team.hip - binned sweeps with a team of T lanes per cell (T = 4, 8 or 16).
 //
-// A wavefront holds 64 / kTeam cells. The lanes of a cell's team split its work:
-//   1. rows: the (2R+1)^2 = 9 neighbour rows (x-runs of bins, z outer, y inner - the order of
-//      binned_all_pairs.hpp:117-138) are loaded one per lane and broadcast with shuffles;
+// A wavefront holds 64 / T cells. The lanes of a cell's team split its work:
+//   1. rows: the 9 neighbour rows (x-runs of bins, z outer, y inner - the order of
+//      binned_all_pairs.hpp:117-138) are looked up by the team's lanes into an LDS row table;
 //   2. candidates: the concatenated row ranges are the reference's visiting sequence; lane t
-//      takes candidates t, t+kTeam, ...; each computes d2 and the cutoff test;
+//      takes candidates t, t+T, ...; each computes d2 and the cutoff test;
 //   3. compaction: accepted candidates go to LDS in visiting order (ballot + popcount);
 //   4. Gabriel (binned_gabriel.hpp:133-209): each accepted pair is tested against the team's
-//      accepted list. Every reference blocker k of (i, j) lies within 0.9 d(i, j) of i (the ball
-//      has radius c d / 2 <= 0.475 d around the midpoint, for c <= 0.95) and inside the 27 bins
-//      around i, so it is an accepted candidate of i; the reference only counts k whose bin (of
-//      the last rebuild) lies in the ball's bounding box, which is checked with the candidate's
-//      row and x-bin. The result is the reference's blocked flag exactly;
+//      accepted list. Every reference blocker k of (i, j) lies within (c/2 + 1/2) d(i, j) of i
+//      (the ball has radius c d / 2 around the midpoint; c <= 0.95 gives < 0.98 d) and inside
+//      the 27 bins around i, so it is an accepted candidate of i; the reference only counts k
+//      whose bin (of the last rebuild) lies in the ball's bounding box, which is checked with
+//      the candidate's row and x-bin. The result is the reference's blocked flag exactly;
 //   5. forces of the surviving pairs are computed by their lanes and stored in LDS;
 //   6. the team leader sums them in visiting order, so the sum is bitwise the serial one.
 // A team whose accepted list overflows the LDS capacity (or, defensively, whose Gabriel box
 // leaves the 27 bins) hands the cell to its leader, which walks it serially like the
-// one-thread-per-cell kernel. Requires search_radius == 1 and, for
-// Gabriel, gabriel_coefficient <= 0.95 (checked on the host).
+// one-thread-per-cell kernel. Requires search_radius == 1 and, for Gabriel,
+// gabriel_coefficient <= 0.95 (checked on the host).
+//
+// T trades lane utilisation against per-team overhead: the force phase runs one round per
+// T accepted pairs, the row and candidate phases one per T rows / candidates. The host's
+// autotuner times T = 4, 8, 16 (and the thread-per-cell sweep) on the live state.
 #include "tecs_internal.hpp"
 
 namespace tecs_amd {
 namespace {
 
-constexpr int kTeam = kTeamLanes;
-constexpr int kTeamsPerBlock = kBlock / kTeam;
-constexpr int kAMax = 32;  // accepted neighbours per cell held in LDS
+template <int T>
+struct TeamShape {
+  static constexpr int kTeams = kBlock / T;          // teams (cells) per workgroup
+  static constexpr int kAMax = T >= 8 ? 32 : 24;     // accepted neighbours held in LDS per cell
+};
 
+template <int T>
 struct TeamLds {
-  float4 apos[kTeamsPerBlock][kAMax];  // accepted neighbour position (w = type bits)
-  int aslot[kTeamsPerBlock][kAMax];    // its slot
-  int arel[kTeamsPerBlock][kAMax];     // its (stale) bin relative to i's centre bin - 1, packed
-  float4 cpos[kTeamsPerBlock][kAMax];  // position contribution, w = 1 when the pair counts
-  float2 cpol[kTeamsPerBlock][kAMax];  // polarity contribution
-  int unsafe[kTeamsPerBlock];  // a bounding box left the 27 bins: leader walks serially
+  static constexpr int TB = TeamShape<T>::kTeams;
+  static constexpr int AM = TeamShape<T>::kAMax;
+  float4 apos[TB][AM];  // accepted neighbour position (w = type bits)
+  int aslot[TB][AM];    // its slot
+  int arel[TB][AM];     // its (stale) bin relative to i's centre bin - 1, packed 2 bits/dim
+  float4 cpos[TB][AM];  // position contribution, w = 1 when the pair counts
+  float2 cpol[TB][AM];  // polarity contribution
+  int row_start[TB][12];
+  int row_len[TB][12];
+  int row_b1[TB][12];   // first slot of the row's second bin (Gabriel)
+  int row_b2[TB][12];   // first slot of the row's third bin (Gabriel)
+  int row_xlo[TB][12];
+  int unsafe[TB];
 };
 
 __device__ __forceinline__ int pack_rel(int rx, int ry, int rz) { return rx | (ry << 2) | (rz << 4); }
@@ -41,7 +55,7 @@ __device__ __forceinline__ int pack_rel(int rx, int ry, int rz) { return rx | (r
 // ---------------------------------------------------------------- serial fallback (leader)
 template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE>
 __device__ void serial_cell(const SweepArgs& a, const InstanceDesc& in, int s, float4 me,
-                            float2 mp, float* acc_pos, float* acc_pol, unsigned& pairs) {
+                            const Pol& pme, float* acc_pos, float* acc_pol, unsigned& pairs) {
   int64_t emit_at = 0;
   if (MODE == kModeEmit) emit_at = a.row_ptr[a.id[s]];
   const float4* __restrict__ xs = a.xs;
@@ -101,11 +115,10 @@ __device__ void serial_cell(const SweepArgs& a, const InstanceDesc& in, int s, f
           continue;
         }
         const float d = sqrtf(d2);
-        float2 q = make_float2(0.0f, 0.0f);
-        if (HAS_POL) q = a.ps[m];
+        Pol po{};
+        if (HAS_POL) po = make_pol(a.ps[m], a.trig[m]);
         apply_force<HAS_TYPE, HAS_POL>(in.force, disp, d, __float_as_int(me.w),
-                                       __float_as_int(pj.w), mp.x, mp.y, q.x, q.y, acc_pos,
-                                       acc_pol);
+                                       __float_as_int(pj.w), pme, po, acc_pos, acc_pol);
       }
     }
   }
@@ -134,21 +147,23 @@ __device__ __forceinline__ unsigned tblock_sum_u(unsigned v, unsigned* lds) {
   return r;
 }
 
-template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE>
+template <int T, bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE>
 __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
-  __shared__ TeamLds L;
+  constexpr int TB = TeamShape<T>::kTeams;
+  constexpr int AM = TeamShape<T>::kAMax;
+  __shared__ TeamLds<T> L;
   __shared__ float lds_f[4];
   __shared__ unsigned lds_u[4];
 
-  // team-block -> slots: each 256-slot chunk (one instance) holds kTeam team-blocks
+  // team-block -> slots: each 256-slot chunk (one instance) holds T team-blocks;
+  // XCD-aware remap (see logical_block): contiguous runs of team-blocks per XCD
   const int p = (int)blockIdx.x;
   const int nb = (int)gridDim.x;
-  // XCD-aware remap (see logical_block): contiguous runs of team-blocks per XCD
   const int q = nb >> 3, r = nb & 7, xcd = p & 7, idx = p >> 3;
   const int tb = (xcd < r) ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
-  const int team = (int)threadIdx.x / kTeam;
-  const int lane = (int)threadIdx.x % kTeam;
-  const int s = tb * kTeamsPerBlock + team;
+  const int team = (int)threadIdx.x / T;
+  const int lane = (int)threadIdx.x % T;
+  const int s = tb * TB + team;
   const int chunk = s / kBlock;  // the 256-slot chunk, one instance
   const int ii = a.block_inst[chunk];
   const InstanceDesc& in = a.inst[ii];
@@ -156,70 +171,82 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
 
   float4 me = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   float2 mp = make_float2(0.0f, 0.0f);
+  Pol pme{};
   int bc[3] = {0, 0, 0};
-  // rows: lane r < 9 owns row r = (dz + 1) * 3 + (dy + 1)
-  int r_start = 0, r_len = 0, r_b1 = 0x7fffffff, r_b2 = 0x7fffffff, r_xlo = 0;
   if (active) {
     me = a.xs[s];
-    if (HAS_POL) mp = a.ps[s];
+    if (HAS_POL) {
+      mp = a.ps[s];
+      pme = make_pol(mp, a.trig[s]);
+    }
     if (GABRIEL)
       bin_coords_div(in, me.x, me.y, me.z, bc);  // binned_gabriel.hpp:101
     else
       bin_coords_mul(in, me.x, me.y, me.z, bc);  // binned_all_pairs.hpp:103-109
-    if (lane < 9) {
-      const int dz = lane / 3 - 1, dy = lane % 3 - 1;
-      const int z = bc[2] + dz, y = bc[1] + dy;
+    // row rr = (dz + 1) * 3 + (dy + 1), looked up by lane rr mod T
+    const int x_lo = max(bc[0] - 1, 0), x_hi = min(bc[0] + 1, in.ext[0] - 1);
+    const int* offb = a.off + in.bin_begin;
+    for (int rr = lane; rr < 9; rr += T) {
+      const int z = bc[2] + rr / 3 - 1, y = bc[1] + rr % 3 - 1;
+      int start = 0, len = 0, b1 = 0x7fffffff, b2 = 0x7fffffff;
       if (z >= 0 && z < in.ext[2] && y >= 0 && y < in.ext[1]) {
-        const int x_lo = max(bc[0] - 1, 0), x_hi = min(bc[0] + 1, in.ext[0] - 1);
-        const int* offb = a.off + in.bin_begin;
         const int b_lo = flatten(in, x_lo, y, z);
         const int b_hi = flatten(in, x_hi, y, z);
-        r_start = offb[b_lo];
-        r_len = offb[b_hi + 1] - r_start;
-        r_xlo = x_lo;
+        start = offb[b_lo];
+        len = offb[b_hi + 1] - start;
         if (GABRIEL) {
-          if (x_lo + 1 <= x_hi) r_b1 = offb[b_lo + 1];
-          if (x_lo + 2 <= x_hi) r_b2 = offb[b_lo + 2];
+          if (x_lo + 1 <= x_hi) b1 = offb[b_lo + 1];
+          if (x_lo + 2 <= x_hi) b2 = offb[b_lo + 2];
         }
+      }
+      L.row_start[team][rr] = start;
+      L.row_len[team][rr] = len;
+      if (GABRIEL) {
+        L.row_b1[team][rr] = b1;
+        L.row_b2[team][rr] = b2;
+        L.row_xlo[team][rr] = x_lo;
       }
     }
   }
-  // broadcast the row table inside the team
-  int row_start[9], row_pre[9], row_b1[9], row_b2[9], row_xlo[9];
+  if (lane == 0) L.unsafe[team] = 0;
+  __syncthreads();
+
+  int row_pre[9], row_start[9];
   int total = 0;
 #pragma unroll
   for (int rr = 0; rr < 9; ++rr) {
-    row_start[rr] = __shfl(r_start, rr, kTeam);
-    const int len = __shfl(r_len, rr, kTeam);
     row_pre[rr] = total;
-    total += len;
-    if (GABRIEL) {
-      row_b1[rr] = __shfl(r_b1, rr, kTeam);
-      row_b2[rr] = __shfl(r_b2, rr, kTeam);
-      row_xlo[rr] = __shfl(r_xlo, rr, kTeam);
-    }
+    row_start[rr] = L.row_start[team][rr];
+    total += L.row_len[team][rr];
   }
   if (!active) total = 0;
 
-  // candidates -> accepted list in LDS (visiting order)
+  // candidates -> accepted list in LDS, in visiting order
   int base = 0;
-  bool overflow = false;
-  const unsigned long long team_mask_shift = (unsigned long long)((threadIdx.x & 63) / kTeam) * kTeam;
-  const int n_iter = (total + kTeam - 1) / kTeam;
-  float acc_pos[3] = {0.0f, 0.0f, 0.0f}, acc_pol[2] = {0.0f, 0.0f};
+  const int team_shift = ((int)(threadIdx.x & 63) / T) * T;
+  const unsigned team_bits = (T == 32) ? 0xffffffffu : ((1u << T) - 1u);
+  const int n_iter = (total + T - 1) / T;
   for (int it = 0; it < n_iter; ++it) {
-    const int c = it * kTeam + lane;
+    const int c = it * T + lane;
     bool acc = false;
     int m = -1, rr_sel = 0;
     float4 pj = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float disp[3] = {0.0f, 0.0f, 0.0f};
     float d2 = 0.0f;
     if (c < total) {
+      // the row holding candidate c: the last row whose prefix is <= c (empty rows share
+      // their prefix with the next row)
 #pragma unroll
       for (int rr = 1; rr < 9; ++rr)
         if (c >= row_pre[rr]) rr_sel = rr;
-      // rows of zero length share a prefix with the next row; pick the last row starting at c
-      m = row_start[rr_sel] + (c - row_pre[rr_sel]);
+      int st = row_start[0], pre = row_pre[0];
+#pragma unroll
+      for (int rr = 1; rr < 9; ++rr)
+        if (rr == rr_sel) {
+          st = row_start[rr];
+          pre = row_pre[rr];
+        }
+      m = st + (c - pre);
       if (m != s) {
         pj = a.xs[m];
         disp[0] = me.x - pj.x;
@@ -230,41 +257,39 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
       }
     }
     const unsigned long long bal = __ballot(acc);
-    const unsigned tm = (unsigned)((bal >> team_mask_shift) & ((1ull << kTeam) - 1ull));
+    const unsigned tm = (unsigned)(bal >> team_shift) & team_bits;
     const int pos = base + __popc(tm & ((1u << lane) - 1u));
-    if (acc) {
-      if (pos < kAMax) {
-        if (GABRIEL) {
-          // stale bin of m relative to (bc - 1): row gives y, z; the row's bin boundaries x
-          const int bx = row_xlo[rr_sel] + (m >= row_b1[rr_sel] ? 1 : 0) + (m >= row_b2[rr_sel] ? 1 : 0);
-          const int rel = pack_rel(bx - bc[0] + 1, rr_sel % 3, rr_sel / 3);
-          L.apos[team][pos] = pj;
-          L.aslot[team][pos] = m;
-          L.arel[team][pos] = rel;
-        } else {
-          // all-pairs: the force needs nothing else, compute it here
+    if (acc && pos < AM) {
+      L.aslot[team][pos] = m;
+      if (GABRIEL) {
+        // stale bin of m relative to (bc - 1): the row gives y and z, the row's bin
+        // boundaries give x
+        const int bx = L.row_xlo[team][rr_sel] + (m >= L.row_b1[team][rr_sel] ? 1 : 0) +
+                       (m >= L.row_b2[team][rr_sel] ? 1 : 0);
+        L.apos[team][pos] = pj;
+        L.arel[team][pos] = pack_rel(bx - bc[0] + 1, rr_sel % 3, rr_sel / 3);
+      } else {
+        // all-pairs: the force needs nothing else, compute it here
+        float cp[3] = {0.0f, 0.0f, 0.0f}, cl[2] = {0.0f, 0.0f};
+        if (MODE != kModeCount && MODE != kModeEmit) {
           const float d = sqrtf(d2);
-          float cp[3] = {0.0f, 0.0f, 0.0f}, cl[2] = {0.0f, 0.0f};
-          float2 qj = make_float2(0.0f, 0.0f);
-          if (HAS_POL) qj = a.ps[m];
-          if (MODE != kModeCount && MODE != kModeEmit)
-            apply_force<HAS_TYPE, HAS_POL>(in.force, disp, d, __float_as_int(me.w),
-                                           __float_as_int(pj.w), mp.x, mp.y, qj.x, qj.y, cp, cl);
-          L.cpos[team][pos] = make_float4(cp[0], cp[1], cp[2], 1.0f);
-          L.cpol[team][pos] = make_float2(cl[0], cl[1]);
-          L.aslot[team][pos] = m;
+          Pol po{};
+          if (HAS_POL) po = make_pol(a.ps[m], a.trig[m]);
+          apply_force<HAS_TYPE, HAS_POL>(in.force, disp, d, __float_as_int(me.w),
+                                         __float_as_int(pj.w), pme, po, cp, cl);
         }
+        L.cpos[team][pos] = make_float4(cp[0], cp[1], cp[2], 1.0f);
+        L.cpol[team][pos] = make_float2(cl[0], cl[1]);
       }
     }
     base += __popc(tm);
   }
-  overflow = base > kAMax;
-  const int na = min(base, kAMax);
-  if (lane == 0) L.unsafe[team] = 0;
+  const bool overflow = base > AM;
+  const int na = min(base, AM);
   __syncthreads();
 
   if (GABRIEL && active && !overflow) {
-    for (int e = lane; e < na; e += kTeam) {
+    for (int e = lane; e < na; e += T) {
       const float4 pj = L.apos[team][e];
       const int m = L.aslot[team][e];
       const float disp[3] = {me.x - pj.x, me.y - pj.y, me.z - pj.z};
@@ -276,12 +301,14 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
       const float rho = sqrtf(rho2);
       // bounding-box bins (binned_gabriel.hpp:142-153), relative to bc - 1
       const float mid[3] = {mid0, mid1, mid2};
-      int lo = 0, hi = 0;  // packed 2-bit fields per dimension
+      int lo = 0, hi = 0;
       bool out_of_reach = false;
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        const int mn = clampi((int)floorf((mid[d] - rho - in.minb[d]) * in.inv_bin), in.ext[d] - 1) - bc[d] + 1;
-        const int mx = clampi((int)floorf((mid[d] + rho - in.minb[d]) * in.inv_bin), in.ext[d] - 1) - bc[d] + 1;
+        const int mn =
+            clampi((int)floorf((mid[d] - rho - in.minb[d]) * in.inv_bin), in.ext[d] - 1) - bc[d] + 1;
+        const int mx =
+            clampi((int)floorf((mid[d] + rho - in.minb[d]) * in.inv_bin), in.ext[d] - 1) - bc[d] + 1;
         out_of_reach = out_of_reach || mn < 0 || mx > 2;
         lo |= max(mn, 0) << (2 * d);
         hi |= min(mx, 2) << (2 * d);
@@ -307,10 +334,10 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
       float cp[3] = {0.0f, 0.0f, 0.0f}, cl[2] = {0.0f, 0.0f};
       if (!blocked && MODE != kModeCount && MODE != kModeEmit) {
         const float d = sqrtf(d2);
-        float2 qj = make_float2(0.0f, 0.0f);
-        if (HAS_POL) qj = a.ps[m];
+        Pol po{};
+        if (HAS_POL) po = make_pol(a.ps[m], a.trig[m]);
         apply_force<HAS_TYPE, HAS_POL>(in.force, disp, d, __float_as_int(me.w),
-                                       __float_as_int(pj.w), mp.x, mp.y, qj.x, qj.y, cp, cl);
+                                       __float_as_int(pj.w), pme, po, cp, cl);
       }
       L.cpos[team][e] = make_float4(cp[0], cp[1], cp[2], blocked ? 0.0f : 1.0f);
       L.cpol[team][e] = make_float2(cl[0], cl[1]);
@@ -318,12 +345,13 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
   }
   __syncthreads();
 
-  // leader: ordered sum (or the serial walk after an overflow)
+  // leader: ordered sum (or the serial walk after an overflow / unsafe box)
+  float acc_pos[3] = {0.0f, 0.0f, 0.0f}, acc_pol[2] = {0.0f, 0.0f};
   unsigned pairs = 0;
   const bool leader = lane == 0 && active;
   if (leader) {
     if (overflow || L.unsafe[team]) {
-      serial_cell<GABRIEL, HAS_TYPE, HAS_POL, MODE>(a, in, s, me, mp, acc_pos, acc_pol, pairs);
+      serial_cell<GABRIEL, HAS_TYPE, HAS_POL, MODE>(a, in, s, me, pme, acc_pos, acc_pol, pairs);
     } else {
       int64_t emit_at = 0;
       if (MODE == kModeEmit) emit_at = a.row_ptr[a.id[s]];
@@ -352,6 +380,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
   }
   if (MODE == kModeEmit) return;
 
+  // out(i) += total, then the +0 drag correction (binned_all_pairs.hpp:174-180)
   float F[3], G[2];
 #pragma unroll
   for (int c = 0; c < 3; ++c) F[c] = (0.0f + acc_pos[c]) + 0.0f;
@@ -363,6 +392,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
       a.o_pos[s] = make_float4(F[0], F[1], F[2], 0.0f);
       if (HAS_POL) a.o_pol[s] = make_float2(G[0], G[1]);
     } else if (MODE == kModePred) {
+      // heun.hpp:146-156
       a.o_pos[s] = make_float4(F[0], F[1], F[2], 0.0f);
       a.o2_pos[s] = make_float4(me.x + F[0] * dt, me.y + F[1] * dt, me.z + F[2] * dt, me.w);
       if (HAS_POL) {
@@ -370,6 +400,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
         a.o2_pol[s] = make_float2(mp.x + G[0] * dt, mp.y + G[1] * dt);
       }
     } else if (MODE == kModeCorr) {
+      // heun.hpp:170-186
       const float4 d1 = a.d1_pos[s];
       const float s0 = d1.x + F[0], s1 = d1.y + F[1], s2 = d1.z + F[2];
       if (a.old_vel) a.old_vel[s] = make_float4(s0 * 0.5f, s1 * 0.5f, s2 * 0.5f, 0.0f);
@@ -383,6 +414,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
                                   p0.y + (e1.y + G[1]) * 0.5f * dt);
       }
     } else if (MODE == kModeEuler) {
+      // euler.hpp:52-68
       if (a.old_vel) a.old_vel[s] = make_float4(F[0], F[1], F[2], 0.0f);
       a.o_pos[s] = make_float4(me.x + F[0] * dt, me.y + F[1] * dt, me.z + F[2] * dt, me.w);
       if (HAS_POL) a.o_pol[s] = make_float2(mp.x + G[0] * dt, mp.y + G[1] * dt);
@@ -398,49 +430,56 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
   if (threadIdx.x == 0) a.block_pairs[tb] += tp;
 }
 
-template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE>
-void launch_t(const SweepArgs& a, int n_team_blocks, hipStream_t stream) {
-  hipLaunchKernelGGL((k_sweep_team<GABRIEL, HAS_TYPE, HAS_POL, MODE>), dim3(n_team_blocks),
+template <int T, bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE>
+void launch_t(const SweepArgs& a, int n_blocks, hipStream_t stream) {
+  hipLaunchKernelGGL((k_sweep_team<T, GABRIEL, HAS_TYPE, HAS_POL, MODE>), dim3(n_blocks * T),
                      dim3(kBlock), 0, stream, a);
 }
 
-template <bool GABRIEL, int MODE>
+template <int T, bool GABRIEL, int MODE>
 void launch_force_t(bool has_type, bool has_pol, const SweepArgs& a, int nb, hipStream_t st) {
   if (MODE == kModeCount || MODE == kModeEmit) {
-    launch_t<GABRIEL, false, false, MODE>(a, nb, st);
+    launch_t<T, GABRIEL, false, false, MODE>(a, nb, st);
   } else if (has_type) {
-    if (has_pol) launch_t<GABRIEL, true, true, MODE>(a, nb, st);
-    else launch_t<GABRIEL, true, false, MODE>(a, nb, st);
+    if (has_pol) launch_t<T, GABRIEL, true, true, MODE>(a, nb, st);
+    else launch_t<T, GABRIEL, true, false, MODE>(a, nb, st);
   } else {
-    if (has_pol) launch_t<GABRIEL, false, true, MODE>(a, nb, st);
-    else launch_t<GABRIEL, false, false, MODE>(a, nb, st);
+    if (has_pol) launch_t<T, GABRIEL, false, true, MODE>(a, nb, st);
+    else launch_t<T, GABRIEL, false, false, MODE>(a, nb, st);
   }
 }
 
-template <bool GABRIEL>
+template <int T, bool GABRIEL>
 void launch_mode_t(int mode, bool has_type, bool has_pol, const SweepArgs& a, int nb,
                    hipStream_t st) {
   switch (mode) {
-    case kModeDelta: launch_force_t<GABRIEL, kModeDelta>(has_type, has_pol, a, nb, st); break;
-    case kModePred: launch_force_t<GABRIEL, kModePred>(has_type, has_pol, a, nb, st); break;
-    case kModeCorr: launch_force_t<GABRIEL, kModeCorr>(has_type, has_pol, a, nb, st); break;
-    case kModeEuler: launch_force_t<GABRIEL, kModeEuler>(has_type, has_pol, a, nb, st); break;
-    case kModeCount: launch_force_t<GABRIEL, kModeCount>(has_type, has_pol, a, nb, st); break;
-    default: launch_force_t<GABRIEL, kModeEmit>(has_type, has_pol, a, nb, st); break;
+    case kModeDelta: launch_force_t<T, GABRIEL, kModeDelta>(has_type, has_pol, a, nb, st); break;
+    case kModePred: launch_force_t<T, GABRIEL, kModePred>(has_type, has_pol, a, nb, st); break;
+    case kModeCorr: launch_force_t<T, GABRIEL, kModeCorr>(has_type, has_pol, a, nb, st); break;
+    case kModeEuler: launch_force_t<T, GABRIEL, kModeEuler>(has_type, has_pol, a, nb, st); break;
+    case kModeCount: launch_force_t<T, GABRIEL, kModeCount>(has_type, has_pol, a, nb, st); break;
+    default: launch_force_t<T, GABRIEL, kModeEmit>(has_type, has_pol, a, nb, st); break;
   }
+}
+
+template <int T>
+void launch_team_width(bool gabriel, bool has_type, bool has_pol, int mode, const SweepArgs& a,
+                       int n_blocks, hipStream_t stream) {
+  if (gabriel)
+    launch_mode_t<T, true>(mode, has_type, has_pol, a, n_blocks, stream);
+  else
+    launch_mode_t<T, false>(mode, has_type, has_pol, a, n_blocks, stream);
 }
 
 }  // namespace
 
-int team_blocks_for(int n_blocks) { return n_blocks * kTeam; }
-
-void launch_sweep_team(bool gabriel, bool has_type, bool has_pol, int mode, const SweepArgs& a,
-                       int n_blocks, hipStream_t stream) {
-  const int nb = team_blocks_for(n_blocks);
-  if (gabriel)
-    launch_mode_t<true>(mode, has_type, has_pol, a, nb, stream);
-  else
-    launch_mode_t<false>(mode, has_type, has_pol, a, nb, stream);
+void launch_sweep_team(int team_lanes, bool gabriel, bool has_type, bool has_pol, int mode,
+                       const SweepArgs& a, int n_blocks, hipStream_t stream) {
+  switch (team_lanes) {
+    case 4: launch_team_width<4>(gabriel, has_type, has_pol, mode, a, n_blocks, stream); break;
+    case 8: launch_team_width<8>(gabriel, has_type, has_pol, mode, a, n_blocks, stream); break;
+    default: launch_team_width<16>(gabriel, has_type, has_pol, mode, a, n_blocks, stream); break;
+  }
 }
 
 }  // namespace tecs_amd

@@ -124,7 +124,11 @@ struct Batch {
   std::vector<void*> allocations;
 
   int variant = TECS_SWEEP_AUTO;  // requested sweep variant
+  int tuned = -1;                 // concrete variant picked by the autotuner (AUTO)
   bool team_ok = false;           // the team sweep reproduces this batch exactly
+  float4* trig = nullptr;         // sin/cos of the polarity angles of the evaluated stage
+  unsigned long long* tune_pairs = nullptr;  // scratch counters for autotuning sweeps
+  double tune_ms[6] = {0, 0, 0, 0, 0, 0};
   int xc = 0, ic = 0;  // current position/polarity/old-velocity buffer, current id buffer
   int step_count = 0;
   bool grid_built = false;
@@ -321,10 +325,12 @@ struct Batch {
       const float c = ds[k].pair_finder_settings.gabriel_coefficient;
       if (finder == kBinnedGabriel && !(c >= 0.0f && c <= 0.95f)) team_ok = false;
     }
-    TRY(alloc(&block_pairs, (size_t)n_blocks * kTeamLanes));
-    TRY(alloc(&com_partial, (size_t)n_blocks * kTeamLanes));
+    TRY(alloc(&block_pairs, (size_t)n_blocks * kMaxTeamLanes));
+    TRY(alloc(&tune_pairs, (size_t)n_blocks * kMaxTeamLanes));
+    if (has_pol) TRY(alloc(&trig, S));
+    TRY(alloc(&com_partial, (size_t)n_blocks * kMaxTeamLanes));
     TRY(alloc(&com, n_inst));
-    HIP_TRY(hipMemsetAsync(block_pairs, 0, sizeof(unsigned long long) * n_blocks * kTeamLanes, stream));
+    HIP_TRY(hipMemsetAsync(block_pairs, 0, sizeof(unsigned long long) * n_blocks * kMaxTeamLanes, stream));
     HIP_TRY(hipMemsetAsync(com, 0, sizeof(float4) * n_inst, stream));
     for (int b = 0; b < 2; ++b) {
       HIP_TRY(hipMemsetAsync(x[b], 0, sizeof(float4) * S, stream));
@@ -456,25 +462,96 @@ struct Batch {
     a.block_inst = d_block_inst;
     a.xs = xs;
     a.ps = ps;
+    a.trig = trig;
     a.off = off;
     a.id = id[ic];
     a.block_pairs = block_pairs;
     return a;
   }
 
-  bool use_team() const {
-    if (!team_ok) return false;
-    return variant != TECS_SWEEP_THREAD;
+  // The concrete sweep: TECS_SWEEP_THREAD or TECS_SWEEP_TEAM4/8/16.
+  int concrete_variant() const {
+    if (!team_ok) return TECS_SWEEP_THREAD;
+    switch (variant) {
+      case TECS_SWEEP_THREAD: return TECS_SWEEP_THREAD;
+      case TECS_SWEEP_TEAM: return TECS_SWEEP_TEAM8;
+      case TECS_SWEEP_TEAM4:
+      case TECS_SWEEP_TEAM8:
+      case TECS_SWEEP_TEAM16: return variant;
+      default: return tuned >= 0 ? tuned : TECS_SWEEP_TEAM8;
+    }
   }
-  int partials_per_block() const { return use_team() ? kTeamLanes : 1; }
+  static int lanes_of(int v) {
+    return v == TECS_SWEEP_TEAM4 ? 4 : v == TECS_SWEEP_TEAM8 ? 8 : v == TECS_SWEEP_TEAM16 ? 16 : 1;
+  }
+  int partials_per_block() const { return lanes_of(concrete_variant()); }
+
+  void launch_variant(int v, int mode, const SweepArgs& a) {
+    if (v == TECS_SWEEP_THREAD)
+      launch_sweep(finder, has_type, has_pol, mode, a, n_blocks, stream);
+    else
+      launch_sweep_team(lanes_of(v), finder == kBinnedGabriel, has_type, has_pol, mode, a,
+                        n_blocks, stream);
+  }
+
+  // sin/cos of the polarity angles of the stage a sweep is about to read
+  int prepare_trig(const float2* pol_stage) {
+    if (!has_pol) return TECS_OK;
+    launch_polar_trig(pol_stage, trig, total_slots, stream);
+    HIP_TRY(hipGetLastError());
+    return TECS_OK;
+  }
+
+  // Autotuner: time every exact sweep variant on the live state (an evaluation into scratch
+  // buffers with scratch counters, so neither the state nor the statistics change) and keep
+  // the fastest. Runs once, after the first grid exists.
+  int autotune() {
+    if (tuned >= 0) return TECS_OK;
+    if (!team_ok) {
+      tuned = TECS_SWEEP_THREAD;
+      return TECS_OK;
+    }
+    TRY(prepare_trig(pol[xc]));
+    const int cands[4] = {TECS_SWEEP_TEAM4, TECS_SWEEP_TEAM8, TECS_SWEEP_TEAM16, TECS_SWEEP_THREAD};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    double best = 1e30;
+    int best_v = TECS_SWEEP_TEAM8;
+    int rc = TECS_OK;
+    for (int v : cands) {
+      SweepArgs t = base_args(x[xc], pol[xc]);
+      t.o_pos = d1;
+      t.o_pol = d1p;
+      t.block_pairs = tune_pairs;
+      float ms_min = 1e30f;
+      for (int rep = 0; rep < 4 && rc == TECS_OK; ++rep) {
+        hipEventRecord(e0, stream);
+        launch_variant(v, kModeDelta, t);
+        hipEventRecord(e1, stream);
+        hipError_t e = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) rc = fail(TECS_ERR_DEVICE, std::string("autotune: ") + hipGetErrorString(e));
+        if (rep > 0) ms_min = std::min(ms_min, ms);  // first launch warms caches
+      }
+      tune_ms[v] = ms_min;
+      if (ms_min < best) {
+        best = ms_min;
+        best_v = v;
+      }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc != TECS_OK) return rc;
+    tuned = best_v;
+    return TECS_OK;
+  }
 
   int sweep(int mode, const SweepArgs& a, int cls) {
     Pending p{};
     TRY(time_begin(&p, cls));
-    if (use_team())
-      launch_sweep_team(finder == kBinnedGabriel, has_type, has_pol, mode, a, n_blocks, stream);
-    else
-      launch_sweep(finder, has_type, has_pol, mode, a, n_blocks, stream);
+    launch_variant(concrete_variant(), mode, a);
     HIP_TRY(hipGetLastError());
     TRY(time_end(&p));
     if (mode != kModeCount && mode != kModeEmit) stats.evaluations++;
@@ -507,12 +584,14 @@ struct Batch {
     const bool com_on = com_fixer == TECS_GLOBAL_COM_FIXER;
     if (integrator == TECS_HEUN) {
       if (!com_on) {
+        TRY(prepare_trig(pol[xc]));
         SweepArgs a = base_args(x[xc], pol[xc]);
         a.o_pos = d1;
         a.o_pol = d1p;
         a.o2_pos = xp;
         a.o2_pol = polp;
         TRY(sweep(kModePred, a, kTimeSweepFull));
+        TRY(prepare_trig(polp));
         SweepArgs b = base_args(xp, polp);
         b.o2_pos = x[xc];
         b.o2_pol = pol[xc];
@@ -521,12 +600,14 @@ struct Batch {
         b.old_vel = track_ov ? ov[xc] : nullptr;
         TRY(sweep(kModeCorr, b, kTimeSweepStale));
       } else {
+        TRY(prepare_trig(pol[xc]));
         SweepArgs a = base_args(x[xc], pol[xc]);
         a.o_pos = d1;
         a.o_pol = d1p;
         a.com_partial = com_partial;
         TRY(sweep(kModeDelta, a, kTimeSweepFull));
         TRY(com_and_update(kUpdatePred));
+        TRY(prepare_trig(polp));
         SweepArgs b = base_args(xp, polp);
         b.o_pos = d3;
         b.o_pol = d3p;
@@ -535,6 +616,7 @@ struct Batch {
         TRY(com_and_update(kUpdateCorr));
       }
     } else {
+      TRY(prepare_trig(pol[xc]));
       if (!com_on) {
         SweepArgs a = base_args(x[xc], pol[xc]);
         a.o_pos = x[xc ^ 1];
@@ -593,10 +675,17 @@ struct Batch {
     if (n_steps < 0) return fail(TECS_ERR_INVALID, "n_steps must be >= 0");
     for (int k = 0; k < n_inst; ++k) set_dt(k, dts[k]);
     TRY(sync_inst());
+    int done = 0;
+    if (variant == TECS_SWEEP_AUTO && tuned < 0 && n_steps > 0) {
+      if (!grid_built && is_binned(finder)) {
+        TRY(step_once());
+        ++done;
+      }
+      TRY(autotune());
+    }
     const int P = graph_period();
     const bool graphs_on = graph_mode == TECS_GRAPHS_ON ||
                            (graph_mode == TECS_GRAPHS_AUTO && n_steps >= 4 * P);
-    int done = 0;
     if (graphs_on && !timing) {
       // the first step after creation or a manual rebuild may start a new rebuild phase
       while (done < n_steps && (step_count == 0 && is_binned(finder))) {
@@ -683,6 +772,7 @@ struct Batch {
   // One evaluation into the d1 scratch buffers, then out in agent order.
   int evaluate(bool is_full_step, float* dpos, float* dpol) {
     TRY(maybe_rebuild(is_full_step));
+    TRY(prepare_trig(pol[xc]));
     SweepArgs a = base_args(x[xc], pol[xc]);
     a.o_pos = d1;
     a.o_pol = d1p;
@@ -751,7 +841,7 @@ struct Batch {
   }
 
   int get_stats(tecs_stats* out) {
-    std::vector<unsigned long long> bp((size_t)n_blocks * kTeamLanes);
+    std::vector<unsigned long long> bp((size_t)n_blocks * kMaxTeamLanes);
     HIP_TRY(hipMemcpyAsync(bp.data(), block_pairs, sizeof(unsigned long long) * bp.size(),
                            hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
@@ -763,7 +853,7 @@ struct Batch {
   }
 
   int reset_stats() {
-    HIP_TRY(hipMemsetAsync(block_pairs, 0, sizeof(unsigned long long) * n_blocks * kTeamLanes, stream));
+    HIP_TRY(hipMemsetAsync(block_pairs, 0, sizeof(unsigned long long) * n_blocks * kMaxTeamLanes, stream));
     HIP_TRY(hipStreamSynchronize(stream));
     stats = tecs_stats{};
     return TECS_OK;
@@ -906,16 +996,24 @@ int tecs_sim_reset_stats(tecs_sim* sim) {
 
 int tecs_sim_set_sweep_variant(tecs_sim* sim, int32_t variant) {
   GUARD(sim);
-  if (variant < TECS_SWEEP_AUTO || variant > TECS_SWEEP_TEAM)
+  if (variant < TECS_SWEEP_AUTO || variant > TECS_SWEEP_TEAM16)
     return fail(TECS_ERR_INVALID, "bad sweep variant");
   sim->b.variant = variant;
+  sim->b.drop_graphs();
   return TECS_OK;
 }
 
 int tecs_sim_get_sweep_variant(tecs_sim* sim, int32_t* variant) {
   GUARD(sim);
   if (!variant) return fail(TECS_ERR_INVALID, "null variant");
-  *variant = sim->b.use_team() ? TECS_SWEEP_TEAM : TECS_SWEEP_THREAD;
+  *variant = sim->b.concrete_variant();
+  return TECS_OK;
+}
+
+int tecs_sim_get_autotune(tecs_sim* sim, double* ms_per_variant) {
+  GUARD(sim);
+  if (!ms_per_variant) return fail(TECS_ERR_INVALID, "null output");
+  for (int v = 0; v < 6; ++v) ms_per_variant[v] = sim->b.tune_ms[v];
   return TECS_OK;
 }
 
@@ -1009,16 +1107,24 @@ int tecs_batch_reset_stats(tecs_batch* batch) {
 
 int tecs_batch_set_sweep_variant(tecs_batch* batch, int32_t variant) {
   GUARD(batch);
-  if (variant < TECS_SWEEP_AUTO || variant > TECS_SWEEP_TEAM)
+  if (variant < TECS_SWEEP_AUTO || variant > TECS_SWEEP_TEAM16)
     return fail(TECS_ERR_INVALID, "bad sweep variant");
   batch->b.variant = variant;
+  batch->b.drop_graphs();
   return TECS_OK;
 }
 
 int tecs_batch_get_sweep_variant(tecs_batch* batch, int32_t* variant) {
   GUARD(batch);
   if (!variant) return fail(TECS_ERR_INVALID, "null variant");
-  *variant = batch->b.use_team() ? TECS_SWEEP_TEAM : TECS_SWEEP_THREAD;
+  *variant = batch->b.concrete_variant();
+  return TECS_OK;
+}
+
+int tecs_batch_get_autotune(tecs_batch* batch, double* ms_per_variant) {
+  GUARD(batch);
+  if (!ms_per_variant) return fail(TECS_ERR_INVALID, "null output");
+  for (int v = 0; v < 6; ++v) ms_per_variant[v] = batch->b.tune_ms[v];
   return TECS_OK;
 }
 

@@ -120,25 +120,6 @@ __device__ __forceinline__ void piecewise_linear(const float* disp, float d, flo
 }
 
 // ---------------------------------------------------------------- polarity.hpp:111-198
-// to_vector3, polarity.hpp:111-118.
-__device__ __forceinline__ void pol_to_vec3(float t, float p, float* v) {
-  v[0] = sinf(t) * cosf(p);
-  v[1] = sinf(t) * sinf(p);
-  v[2] = cosf(t);
-}
-// unidirectional_polarization_force, polarity.hpp:145-157 (self = (t0,p0), other = (t1,p1)).
-__device__ __forceinline__ void pol_unidirectional(float t0, float p0, float t1, float p1,
-                                                   float* r) {
-  r[0] = cosf(t0) * sinf(t1) * cosf(p0 - p1) - sinf(t0) * cosf(t1);
-  r[1] = 0.0f;
-  const float sin_theta = sinf(t0);
-  if (fabsf(sin_theta) > 1.1920928955078125e-07f)  // FLT_EPSILON
-    r[1] = -sinf(t1) * sinf(p0 - p1) / sin_theta;
-}
-// dot, polarity.hpp:131-136.
-__device__ __forceinline__ float pol_dot(float t0, float p0, float t1, float p1) {
-  return sinf(t0) * sinf(t1) * cosf(p0 - p1) + cosf(t0) * cosf(t1);
-}
 // Kokkos::pow(float, int) promotes to double; the quotient is narrowed to float when it
 // multiplies the float vector (vector.hpp:293-296). The squares are exact in double.
 __device__ __forceinline__ float pow2_ratio(float a, float b) {
@@ -146,12 +127,21 @@ __device__ __forceinline__ float pow2_ratio(float a, float b) {
   return (float)((ad * ad) / (bd * bd));
 }
 
+// Polarity of a cell with the sines and cosines of its angles: trig = (sin theta, cos theta,
+// sin phi, cos phi), computed once per substep per slot (k_polar_trig). The force reads them
+// where the reference evaluates sinf/cosf of the same angles, so the values are identical.
+struct Pol {
+  float t, p;    // theta, phi
+  float4 trig;   // sin t, cos t, sin p, cos p
+};
+__device__ __forceinline__ Pol make_pol(float2 q, float4 trig) { return Pol{q.x, q.y, trig}; }
+
 // One invocation of the fused pairwise force (see tecs_force_desc in tecs_amd.h), adding the
 // contributions to the accumulators in the reference's statement order.
 template <bool HAS_TYPE, bool HAS_POL>
 __device__ __forceinline__ void apply_force(const ForceParams& f, const float* disp, float d,
-                                            int type_i, int type_j, float ti, float pi, float tj,
-                                            float pj, float* acc_pos, float* acc_pol) {
+                                            int type_i, int type_j, const Pol& pi, const Pol& pj,
+                                            float* acc_pos, float* acc_pol) {
   float kadh = f.kadh;
   if (HAS_TYPE) {
     if (f.n_types > 0) kadh = f.table[type_i * f.n_types + type_j];
@@ -162,12 +152,14 @@ __device__ __forceinline__ void apply_force(const ForceParams& f, const float* d
   acc_pos[1] = acc_pos[1] + v[1];
   acc_pos[2] = acc_pos[2] + v[2];
   if (HAS_POL) {
+    const float sti = pi.trig.x, cti = pi.trig.y, spi = pi.trig.z, cpi = pi.trig.w;
+    const float stj = pj.trig.x, ctj = pj.trig.y, spj = pj.trig.z, cpj = pj.trig.w;
     if (f.bend != 0.0f) {
-      // bending_force, polarity.hpp:181-198
-      float tv[3], ov[3];
-      pol_to_vec3(ti, pi, tv);
+      // bending_force, polarity.hpp:181-198; to_vector3 (polarity.hpp:111-118) from the
+      // precomputed sines and cosines
+      const float tv[3] = {sti * cpi, sti * spi, cti};
       const float prod_i = dot3(tv, disp) / d;
-      pol_to_vec3(tj, pj, ov);
+      const float ov[3] = {stj * cpj, stj * spj, ctj};
       const float prod_j = dot3(ov, disp) / d;
       const float a = -prod_i / d;
       const float b = pow2_ratio(prod_i, d);
@@ -179,10 +171,13 @@ __device__ __forceinline__ void apply_force(const ForceParams& f, const float* d
       // from_vector3(displacement, distance), polarity.hpp:62-66
       const float q0 = acosf(fminf(1.0f, fmaxf(-1.0f, disp[2] / d)));
       const float q1 = atan2f(disp[1], disp[0]);
-      float u[2];
-      pol_unidirectional(ti, pi, q0, q1, u);
-      const float bp0 = -prod_i * u[0];
-      const float bp1 = -prod_i * u[1];
+      // unidirectional_polarization_force(q), polarity.hpp:145-157
+      const float sq0 = sinf(q0), cq0 = cosf(q0);
+      float u0 = cti * sq0 * cosf(pi.p - q1) - sti * cq0;
+      float u1 = 0.0f;
+      if (fabsf(sti) > 1.1920928955078125e-07f) u1 = -sq0 * sinf(pi.p - q1) / sti;
+      const float bp0 = -prod_i * u0;
+      const float bp1 = -prod_i * u1;
       acc_pos[0] = acc_pos[0] + f.bend * bv[0];
       acc_pos[1] = acc_pos[1] + f.bend * bv[1];
       acc_pos[2] = acc_pos[2] + f.bend * bv[2];
@@ -190,12 +185,14 @@ __device__ __forceinline__ void apply_force(const ForceParams& f, const float* d
       acc_pol[1] = acc_pol[1] + f.bend * bp1;
     }
     if (f.align != 0.0f) {
-      // bidirectional_polarization_force, polarity.hpp:165-168
-      float u[2];
-      pol_unidirectional(ti, pi, tj, pj, u);
-      const float s = pol_dot(ti, pi, tj, pj);
-      acc_pol[0] = acc_pol[0] + f.align * (s * u[0]);
-      acc_pol[1] = acc_pol[1] + f.align * (s * u[1]);
+      // bidirectional_polarization_force, polarity.hpp:165-168 (dot :131-136)
+      const float cdp = cosf(pi.p - pj.p);
+      float u0 = cti * stj * cdp - sti * ctj;
+      float u1 = 0.0f;
+      if (fabsf(sti) > 1.1920928955078125e-07f) u1 = -stj * sinf(pi.p - pj.p) / sti;
+      const float sdot = sti * stj * cdp + cti * ctj;
+      acc_pol[0] = acc_pol[0] + f.align * (sdot * u0);
+      acc_pol[1] = acc_pol[1] + f.align * (sdot * u1);
     }
   }
 }

@@ -26,6 +26,7 @@ struct SweepArgs {
   const int* block_inst;
   const float4* xs;  // positions of the evaluated stage
   const float2* ps;  // polarities of the evaluated stage (may be null)
+  const float4* trig;  // sin/cos of the polarity angles of the evaluated stage (k_polar_trig)
   const int* off;    // concatenated bin offsets (global slots)
   const int* id;     // slot -> agent index
   float4* o_pos;     // Delta: F; Pred: d1; Euler: new positions
@@ -66,11 +67,12 @@ void launch_update(int kind, bool has_pol, const UpdateArgs& a, int n_blocks, hi
 void launch_com_finalize(const InstanceDesc* inst, int n_inst, const float4* partial,
                          int partials_per_block, float4* com, hipStream_t stream);
 
-// Team sweep (binned finders, search radius 1): kTeamLanes lanes per cell; block_pairs and
-// com_partial need n_blocks * kTeamLanes entries.
-constexpr int kTeamLanes = 16;
-void launch_sweep_team(bool gabriel, bool has_type, bool has_pol, int mode, const SweepArgs& a,
-                       int n_blocks, hipStream_t stream);
+// Team sweep (binned finders, search radius 1): team_lanes (4, 8 or 16) lanes per cell, launched
+// as n_blocks * team_lanes workgroups; block_pairs and com_partial need
+// n_blocks * kMaxTeamLanes entries.
+constexpr int kMaxTeamLanes = 16;
+void launch_sweep_team(int team_lanes, bool gabriel, bool has_type, bool has_pol, int mode,
+                       const SweepArgs& a, int n_blocks, hipStream_t stream);
 
 struct RebuildArgs {
   const InstanceDesc* inst;
@@ -103,6 +105,8 @@ void launch_download(const InstanceDesc* inst, int ii, int block_begin, int n_bl
                      const int* id, const float4* x, const float2* p, const float4* ov,
                      float* pos, float* pol, float* old_vel, hipStream_t stream);
 
+// trig[s] = (sin theta, cos theta, sin phi, cos phi) of pol[s] for all slots of the batch
+void launch_polar_trig(const float2* pol, float4* trig, int total_slots, hipStream_t stream);
 void launch_copy_probe(const float4* src, float4* dst, int64_t n, hipStream_t stream);
 
 }  // namespace tecs_amd

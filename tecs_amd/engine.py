@@ -118,6 +118,11 @@ class GpuSimulation:
         check(self._lib.tecs_sim_get_sweep_variant(self._h, C.byref(v)))
         return int(v.value)
 
+    def autotune_ms(self) -> dict:
+        arr = (C.c_double * 6)()
+        check(self._lib.tecs_sim_get_autotune(self._h, arr))
+        return {abi.SWEEP_NAMES[v]: arr[v] for v in abi.SWEEP_NAMES if arr[v] > 0}
+
     def set_graph_mode(self, mode: int):
         check(self._lib.tecs_sim_set_graph_mode(self._h, int(mode)))
 
@@ -201,6 +206,11 @@ class GpuBatch:
         v = C.c_int32(0)
         check(self._lib.tecs_batch_get_sweep_variant(self._h, C.byref(v)))
         return int(v.value)
+
+    def autotune_ms(self) -> dict:
+        arr = (C.c_double * 6)()
+        check(self._lib.tecs_batch_get_autotune(self._h, arr))
+        return {abi.SWEEP_NAMES[v]: arr[v] for v in abi.SWEEP_NAMES if arr[v] > 0}
 
     def set_graph_mode(self, mode: int):
         check(self._lib.tecs_batch_set_graph_mode(self._h, int(mode)))

@@ -31,8 +31,9 @@ def _with(desc, **kw):
     return d
 
 
-VARIANTS = [abi.SWEEP_THREAD, abi.SWEEP_TEAM]
-VARIANT_IDS = ["thread", "team"]
+VARIANTS = [abi.SWEEP_THREAD, abi.SWEEP_TEAM4, abi.SWEEP_TEAM8, abi.SWEEP_TEAM16]
+VARIANT_IDS = ["thread", "team4", "team8", "team16"]
+TEAM_VARIANTS = [abi.SWEEP_TEAM4, abi.SWEEP_TEAM8, abi.SWEEP_TEAM16]
 
 
 def _pair(oracle_lib, desc, w, positions=None, variant=abi.SWEEP_AUTO):
@@ -79,7 +80,7 @@ CASES = [
 @pytest.mark.parametrize("name,n", CASES)
 def test_pairs_full_step_bit_exact(gpu, oracle_lib, finder, name, n, variant):
     if finder in (abi.NAIVE_ALL_PAIRS, abi.NAIVE_GABRIEL):
-        if variant == abi.SWEEP_TEAM:
+        if variant in TEAM_VARIANTS:
             pytest.skip("naive finders have one sweep kernel")
         n = min(n, 1500)
     w = _workload(name, n, finder)
@@ -111,7 +112,7 @@ def test_pairs_stale_grid_bit_exact(gpu, oracle_lib, finder, variant):
 @pytest.mark.parametrize("name,n", [("relaxation", 3000), ("sorting", 20000)])
 def test_forces_bit_exact_piecewise_linear(gpu, oracle_lib, finder, name, n, variant):
     if finder in (abi.NAIVE_ALL_PAIRS, abi.NAIVE_GABRIEL):
-        if variant == abi.SWEEP_TEAM:
+        if variant in TEAM_VARIANTS:
             pytest.skip("naive finders have one sweep kernel")
         n = min(n, 1500)
     w = _workload(name, n, finder)
@@ -128,7 +129,7 @@ def test_forces_bit_exact_piecewise_linear(gpu, oracle_lib, finder, name, n, var
 @pytest.mark.parametrize("variant", VARIANTS, ids=VARIANT_IDS)
 @pytest.mark.parametrize("finder", [abi.BINNED_GABRIEL, abi.NAIVE_GABRIEL], ids=lambda f: abi.FINDER_NAMES[f])
 def test_forces_polarity_within_1e5_of_fp64(gpu, oracle_lib, finder, variant):
-    if finder == abi.NAIVE_GABRIEL and variant == abi.SWEEP_TEAM:
+    if finder == abi.NAIVE_GABRIEL and variant in TEAM_VARIANTS:
         pytest.skip("naive finders have one sweep kernel")
     n = 4000 if finder == abi.BINNED_GABRIEL else 1500
     w = _workload("epithelium", n, finder)
@@ -240,8 +241,9 @@ def test_batch_member_equals_alone(gpu):
         assert np.array_equal(va.view(np.uint32), vb.view(np.uint32))
 
 
+@pytest.mark.parametrize("variant", TEAM_VARIANTS, ids=VARIANT_IDS[1:])
 @pytest.mark.parametrize("finder", [abi.BINNED_ALL_PAIRS, abi.BINNED_GABRIEL], ids=lambda f: abi.FINDER_NAMES[f])
-def test_dense_cluster_team_overflow_path(gpu, oracle_lib, finder):
+def test_dense_cluster_team_overflow_path(gpu, oracle_lib, finder, variant):
     """~110 neighbours per cell: the team sweep's LDS list overflows and the leader walks the
     cell serially; pairs and forces stay bit-exact."""
     g0 = W.rng(21)
@@ -251,8 +253,8 @@ def test_dense_cluster_team_overflow_path(gpu, oracle_lib, finder):
     d = abi.make_desc(n, pair_finder=finder, min_bounds=lo, max_bounds=hi, integrator=abi.HEUN)
     w = W.Workload("dense", d, pos, None, None, 0.05, 1)
     print(f"case dense cluster finder={abi.FINDER_NAMES[finder]}", flush=True)
-    g, o = _pair(oracle_lib, d, w, variant=abi.SWEEP_TEAM)
-    assert g.sweep_variant() == abi.SWEEP_TEAM
+    g, o = _pair(oracle_lib, d, w, variant=variant)
+    assert g.sweep_variant() == variant
     _assert_same_csr(g.pairs(True), o.pairs(True), "dense")
     fg, _ = g.evaluate_force(True)
     fo, _ = o.evaluate_force(True)
@@ -310,3 +312,20 @@ def test_graph_replay_matches_eager(gpu, name, rebuild_every_n):
     assert graph.graph_launches() >= 4
     assert eager.graph_launches() == 0
     assert graph.stats() == eager.stats()
+
+
+
+def test_autotuner_picks_an_exact_variant(gpu, oracle_lib):
+    """AUTO times every applicable sweep on the live state at the first take_step; whatever it
+    picks, the trajectory equals the oracle's bitwise and the statistics are untouched."""
+    w = _workload("sorting", 20000, abi.BINNED_ALL_PAIRS)
+    print("case autotune sorting n=20000", flush=True)
+    g, o = _run_both(oracle_lib, w, w.desc, 12)
+    assert g.sweep_variant() in (abi.SWEEP_THREAD, abi.SWEEP_TEAM4, abi.SWEEP_TEAM8, abi.SWEEP_TEAM16)
+    times = g.autotune_ms()
+    print("autotune ms", times)
+    assert set(times) == {"thread", "team4", "team8", "team16"}
+    xg, _, _ = g.download()
+    xo, _, _ = o.download()
+    assert np.array_equal(xg.view(np.uint32), xo.view(np.uint32))
+    assert g.stats() == o.stats()
