@@ -2,15 +2,20 @@
 //
 // Grid::rebuild (grid.hpp:389-401) computes one bin key per cell and Kokkos::BinSort orders the
 // cells by key; on the Serial backend the members of a bin end up in ascending agent index.
-// The device version reproduces that order exactly and moves the cell state into it:
-//   1. keys:    key = flatten(clamp(int((p - min) / bin)))  and  cnt[key] += 1   (atomics)
-//   2. scan:    off = exclusive_scan(cnt) over all instances' bins (+ pad bins)
-//   3. scatter: slot = off[key] + (atomic decrement of cnt[key]) - 1 -> tmp; this leaves cnt at
-//               zero for the next rebuild, so no clearing pass is needed
-//   4. gather:  each cell's final slot is off[key] + #(members of its bin with a smaller agent
-//               index); its state (position, polarity, id, old velocity) moves there.
-// Step 3's order within a bin is arbitrary, step 4 makes it canonical, so the result does not
-// depend on atomic arrival order.
+// The device version reproduces that order exactly and moves the cell state into it. Cells are
+// counted per segment of 2^shift consecutive bins (shift 0 = per bin for dense grids; a sparse
+// grid, e.g. a cell sheet on a sphere with ~40 bins per cell, counts segments of ~32 bins so
+// the scan runs over segments, not bins):
+//   1. keys:    key = flatten(clamp(int((p - min) / bin))), cnt[key >> shift] += 1 (atomics)
+//   2. scan:    coff = exclusive_scan(cnt) over all instances' segments (+ pad segments)
+//   3. scatter: slot = coff[seg] + (atomic decrement of cnt[seg]) - 1 -> tmp; this leaves cnt
+//               at zero for the next rebuild, so no clearing pass is needed
+//   4. gather:  each cell's final slot is coff[seg] + #(segment members before it in (key,
+//               agent index) order); its state (position, polarity, id, old velocity) moves
+//   5. fill:    (shift > 0 only) per-bin offsets off[b] = coff[seg] + #(members with key < b),
+//               one coalesced write per bin.
+// Step 3's order within a segment is arbitrary, step 4 makes it canonical, so the result does
+// not depend on atomic arrival order.
 #include "tecs_internal.hpp"
 
 namespace tecs_amd {
@@ -30,11 +35,11 @@ __global__ __launch_bounds__(kBlock) void k_bin_keys(RebuildArgs a) {
     bin_coords_div(in, p.x, p.y, p.z, c);  // grid.hpp:394-395
     const int key = flatten(in, c[0], c[1], c[2]);
     a.keys[s] = key;
-    atomicAdd(&a.cnt[in.bin_begin + key], 1);
+    atomicAdd(&a.cnt[in.cbin_begin + (key >> in.shift)], 1);
   } else if (local == in.n) {
-    // the pad bin holds the instance's pad slots, so the global scan lands on slot_begin of
-    // the next instance
-    a.cnt[in.bin_begin + in.bin_count] = in.padded - in.n;
+    // the pad segment holds the instance's pad slots, so the global scan lands on
+    // slot_begin of the next instance
+    a.cnt[in.cbin_begin + in.cbin_count - 1] = in.padded - in.n;
   }
 }
 
@@ -44,30 +49,58 @@ __global__ __launch_bounds__(kBlock) void k_scatter(RebuildArgs a) {
   const int s = lb * kBlock + (int)threadIdx.x;
   const int local = s - in.slot_begin;
   if (local < in.n) {
-    const int g = in.bin_begin + a.keys[s];
-    const int pos = a.off[g] + atomicSub(&a.cnt[g], 1) - 1;
+    const int g = in.cbin_begin + (a.keys[s] >> in.shift);
+    const int pos = a.coff[g] + atomicSub(&a.cnt[g], 1) - 1;
     a.tmp[pos] = s;
   } else if (local == in.n) {
-    a.cnt[in.bin_begin + in.bin_count] = 0;
+    a.cnt[in.cbin_begin + in.cbin_count - 1] = 0;
   }
 }
 
+// Final slot = segment start + #(segment members before it in (bin, agent index) order).
 __global__ __launch_bounds__(kBlock) void k_gather(RebuildArgs a) {
   const int lb = logical_block();
   const InstanceDesc& in = a.inst[a.block_inst[lb]];
   const int t = lb * kBlock + (int)threadIdx.x;
   if ((t - in.slot_begin) >= in.n) return;
   const int o = a.tmp[t];
-  const int g = in.bin_begin + a.keys[o];
-  const int start = a.off[g], end = a.off[g + 1];
+  const int key = a.keys[o];
+  const int g = in.cbin_begin + (key >> in.shift);
+  const int start = a.coff[g], end = a.coff[g + 1];
   const int my_id = a.id_old[o];
   int rank = 0;
-  for (int u = start; u < end; ++u) rank += (a.id_old[a.tmp[u]] < my_id) ? 1 : 0;
+  for (int u = start; u < end; ++u) {
+    const int ou = a.tmp[u];
+    const int ku = a.keys[ou];
+    rank += (ku < key || (ku == key && a.id_old[ou] < my_id)) ? 1 : 0;
+  }
   const int dst = start + rank;
   a.id_new[dst] = my_id;
   a.x_new[dst] = a.x_old[o];
   if (a.pol_new) a.pol_new[dst] = a.pol_old[o];
   if (a.ov_new) a.ov_new[dst] = a.ov_old[o];
+  if (a.fill) a.skey[dst] = key;
+}
+
+// Per-bin offsets from the segment offsets: off[b] = segment start + #(members with key < b).
+// One workgroup per 256 bins; a wavefront covers 64 consecutive bins, i.e. one segment or
+// part of one for shifts >= 6, so the segment reads are shared.
+__global__ __launch_bounds__(kBlock) void k_fill_offsets(RebuildArgs a) {
+  const int blk = (int)blockIdx.x;
+  const InstanceDesc& in = a.inst[a.fill_inst[blk]];
+  const int b = a.fill_base[blk] + (int)threadIdx.x;
+  if (b > in.bin_count) return;
+  int v;
+  if (b == in.bin_count) {
+    v = in.slot_begin + in.n;
+  } else {
+    const int g = in.cbin_begin + (b >> in.shift);
+    const int start = a.coff[g], end = a.coff[g + 1];
+    int less = 0;
+    for (int u = start; u < end; ++u) less += a.skey[u] < b ? 1 : 0;
+    v = start + less;
+  }
+  a.off[in.bin_begin + b] = v;
 }
 
 // ------------------------------------------------------------------ exclusive scan (int)
@@ -89,32 +122,49 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* lds_w, int* tota
   return wave_off + x - v;
 }
 
+// Loads 8 consecutive ints per thread; full tiles use two 16-byte loads.
+__device__ __forceinline__ void load_items(const int* in, int n, int base, int* v) {
+  if (base + kScanItems <= n && ((reinterpret_cast<uintptr_t>(in + base) & 15) == 0)) {
+    const int4 a = *reinterpret_cast<const int4*>(in + base);
+    const int4 b = *reinterpret_cast<const int4*>(in + base + 4);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+  } else {
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) v[k] = (base + k < n) ? in[base + k] : 0;
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_scan_reduce(const int* in, int n, int* bsum) {
   __shared__ int lds[4];
   const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  int items[kScanItems];
+  load_items(in, n, base, items);
   int v = 0;
 #pragma unroll
-  for (int k = 0; k < kScanItems; ++k)
-    if (base + k < n) v += in[base + k];
+  for (int k = 0; k < kScanItems; ++k) v += items[k];
   int total;
   block_exclusive_scan(v, lds, &total);
   if (threadIdx.x == 0) bsum[blockIdx.x] = total;
 }
 
-// Single workgroup: exclusive scan of the tile sums in place; writes the grand total.
+// Single workgroup: exclusive scan of the tile sums in place; writes the grand total. Each
+// thread owns a contiguous run of the tile sums, so one block scan covers them all.
 __global__ __launch_bounds__(kBlock) void k_scan_top(int* bsum, int nt, int* total_out) {
   __shared__ int lds[4];
-  int carry = 0;
-  for (int base = 0; base < nt; base += kBlock) {
-    const int i = base + threadIdx.x;
-    const int v = i < nt ? bsum[i] : 0;
-    int total;
-    const int ex = block_exclusive_scan(v, lds, &total);
-    if (i < nt) bsum[i] = carry + ex;
-    carry += total;
-    __syncthreads();
+  const int per = (nt + kBlock - 1) / kBlock;
+  const int first = (int)threadIdx.x * per;
+  const int last = min(first + per, nt);
+  int sum = 0;
+  for (int i = first; i < last; ++i) sum += bsum[i];
+  int total;
+  int run = block_exclusive_scan(sum, lds, &total);
+  for (int i = first; i < last; ++i) {
+    const int v = bsum[i];
+    bsum[i] = run;
+    run += v;
   }
-  if (threadIdx.x == 0) *total_out = carry;
+  if (threadIdx.x == 0) *total_out = total;
 }
 
 __global__ __launch_bounds__(kBlock) void k_scan_apply(const int* in, int n, const int* bsum,
@@ -122,12 +172,10 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const int* in, int n, con
   __shared__ int lds[4];
   const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
   int v[kScanItems];
+  load_items(in, n, base, v);
   int sum = 0;
 #pragma unroll
-  for (int k = 0; k < kScanItems; ++k) {
-    v[k] = (base + k < n) ? in[base + k] : 0;
-    sum += v[k];
-  }
+  for (int k = 0; k < kScanItems; ++k) sum += v[k];
   int total;
   int run = bsum[blockIdx.x] + block_exclusive_scan(sum, lds, &total);
 #pragma unroll
@@ -230,9 +278,11 @@ void launch_exclusive_scan(const int* in, int n, int* out, int* bsum, hipStream_
 
 void launch_rebuild(const RebuildArgs& a, int n_blocks, hipStream_t stream) {
   hipLaunchKernelGGL(k_bin_keys, dim3(n_blocks), dim3(kBlock), 0, stream, a);
-  launch_exclusive_scan(a.cnt, a.total_bins, a.off, a.scan_bsum, stream);
+  launch_exclusive_scan(a.cnt, a.total_cbins, a.coff, a.scan_bsum, stream);
   hipLaunchKernelGGL(k_scatter, dim3(n_blocks), dim3(kBlock), 0, stream, a);
   hipLaunchKernelGGL(k_gather, dim3(n_blocks), dim3(kBlock), 0, stream, a);
+  if (a.fill)
+    hipLaunchKernelGGL(k_fill_offsets, dim3(a.fill_blocks), dim3(kBlock), 0, stream, a);
 }
 
 void launch_init_ids(const InstanceDesc* inst, const int* block_inst, int* id, int n_blocks,

@@ -88,6 +88,9 @@ struct Batch {
   std::vector<InstanceDesc> h_inst;
   std::vector<int> h_block_inst;
   int n_inst = 0, n_blocks = 0, total_slots = 0, total_bins = 0, max_n = 0;
+  int total_cbins = 0;         // counting-sort segments of all instances
+  bool fill_bins = false;      // some instance counts multi-bin segments (sparse grid)
+  std::vector<int> h_fill_inst, h_fill_base;
   hipStream_t stream = nullptr;
   bool own_stream = false;
   bool inst_dirty = false;
@@ -108,7 +111,11 @@ struct Batch {
   int* keys = nullptr;
   int* tmp = nullptr;
   int* cnt = nullptr;
+  int* coff = nullptr;
   int* off = nullptr;
+  int* skey = nullptr;
+  int* d_fill_inst = nullptr;
+  int* d_fill_base = nullptr;
   int* bsum = nullptr;
   unsigned long long* block_pairs = nullptr;
   float4* com_partial = nullptr;
@@ -260,6 +267,14 @@ struct Batch {
         in.bin_count = 0;
       }
       in.bin_begin = (int)bins;
+      // sparse grids (many bins per cell, e.g. a sheet of cells on a sphere) count segments of
+      // 2^shift bins so the scan does not run over every empty bin
+      in.shift = 0;
+      if (is_binned(finder) && in.n > 0) {
+        const double bpc = (double)in.bin_count / (double)in.n;
+        if (bpc >= 8.0) in.shift = std::min(10, (int)std::floor(std::log2(bpc)));
+      }
+      in.cbin_count = ((in.bin_count + (1 << in.shift) - 1) >> in.shift) + 1;
       in.dt = 0.0f;
       ForceParams& f = in.force;
       f.h = d.force.homeostatic_radius;
@@ -279,6 +294,24 @@ struct Batch {
     total_slots = (int)slot;
     n_blocks = (int)blocks;
     total_bins = (int)bins;
+    fill_bins = false;
+    for (int k = 0; k < count; ++k) fill_bins = fill_bins || h_inst[k].shift > 0;
+    {
+      int64_t cb = 0;
+      for (int k = 0; k < count; ++k) {
+        // without any sparse instance the segments are the bins (shared arrays)
+        h_inst[k].cbin_begin = fill_bins ? (int)cb : h_inst[k].bin_begin;
+        cb += h_inst[k].cbin_count;
+      }
+      total_cbins = (int)cb;
+    }
+    if (fill_bins) {
+      for (int k = 0; k < count; ++k)
+        for (int b = 0; b <= h_inst[k].bin_count; b += kBlock) {
+          h_fill_inst.push_back(k);
+          h_fill_base.push_back(b);
+        }
+    }
     h_block_inst.resize(n_blocks);
     for (int k = 0; k < count; ++k)
       for (int b = 0; b < h_inst[k].n_blocks; ++b) h_block_inst[h_inst[k].block_begin + b] = k;
@@ -312,11 +345,24 @@ struct Batch {
     if (is_binned(finder)) {
       TRY(alloc(&keys, S));
       TRY(alloc(&tmp, S));
-      TRY(alloc(&cnt, (size_t)total_bins));
-      TRY(alloc(&off, (size_t)total_bins + 1));
-      TRY(alloc(&bsum, (size_t)scan_tiles_for(total_bins)));
-      HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(int) * (size_t)total_bins, stream));
-      HIP_TRY(hipMemsetAsync(off, 0, sizeof(int) * ((size_t)total_bins + 1), stream));
+      TRY(alloc(&cnt, (size_t)total_cbins));
+      TRY(alloc(&coff, (size_t)total_cbins + 1));
+      if (fill_bins) {
+        TRY(alloc(&off, (size_t)total_bins + 1));
+        TRY(alloc(&skey, S));
+        TRY(alloc(&d_fill_inst, h_fill_inst.size()));
+        TRY(alloc(&d_fill_base, h_fill_base.size()));
+        HIP_TRY(hipMemcpyAsync(d_fill_inst, h_fill_inst.data(), sizeof(int) * h_fill_inst.size(),
+                               hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipMemcpyAsync(d_fill_base, h_fill_base.data(), sizeof(int) * h_fill_base.size(),
+                               hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipMemsetAsync(off, 0, sizeof(int) * ((size_t)total_bins + 1), stream));
+      } else {
+        off = coff;
+      }
+      TRY(alloc(&bsum, (size_t)scan_tiles_for(total_cbins)));
+      HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(int) * (size_t)total_cbins, stream));
+      HIP_TRY(hipMemsetAsync(coff, 0, sizeof(int) * ((size_t)total_cbins + 1), stream));
     }
     // team sweep conditions: search radius 1 and, for Gabriel, c <= 0.95 (kernels_team.hip)
     team_ok = is_binned(finder);
@@ -428,9 +474,15 @@ struct Batch {
     r.keys = keys;
     r.tmp = tmp;
     r.cnt = cnt;
+    r.coff = coff;
     r.off = off;
+    r.skey = skey;
     r.scan_bsum = bsum;
-    r.total_bins = total_bins;
+    r.total_cbins = total_cbins;
+    r.fill = fill_bins;
+    r.fill_inst = d_fill_inst;
+    r.fill_base = d_fill_base;
+    r.fill_blocks = (int)h_fill_inst.size();
     launch_rebuild(r, n_blocks, stream);
     HIP_TRY(hipGetLastError());
     TRY(time_end(&p));

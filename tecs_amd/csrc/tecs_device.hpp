@@ -44,6 +44,9 @@ struct InstanceDesc {
   int n_blocks;    // padded / kBlock
   int bin_begin;   // first entry of the instance in the concatenated bin arrays
   int bin_count;   // real bins; the pad bin has local index bin_count
+  int shift;       // the counting sort counts segments of 2^shift consecutive bins
+  int cbin_begin;  // first entry of the instance in the concatenated segment arrays
+  int cbin_count;  // segments including the pad segment (the last one)
   int ext[3];
   float minb[3];
   float bin_size, inv_bin;

@@ -87,10 +87,17 @@ struct RebuildArgs {
   float4* ov_new;
   int* keys;
   int* tmp;
-  int* cnt;
-  int* off;
+  int* cnt;        // per-segment counts (zero between rebuilds)
+  int* coff;       // per-segment offsets (global slots); aliases off when every shift is 0
+  int* off;        // per-bin offsets (global slots)
+  int* skey;       // bin key of every slot in the new order (needed by the fill)
   int* scan_bsum;
-  int total_bins;
+  int total_cbins;  // segments of all instances (scan length)
+  // per-bin fill (only when some instance counts segments of several bins)
+  bool fill;
+  const int* fill_inst;  // fill workgroup -> instance
+  const int* fill_base;  // fill workgroup -> first local bin
+  int fill_blocks;
 };
 void launch_rebuild(const RebuildArgs& a, int n_blocks, hipStream_t stream);
 int scan_tiles_for(int n);
