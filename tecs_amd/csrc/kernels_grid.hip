@@ -24,23 +24,50 @@ namespace {
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;
 
+// Wavefront-aggregated counter update: lanes holding the same counter index in a run of
+// consecutive lanes (cells arrive in the previous bin order, so runs are common) issue one
+// atomic for the run. Returns the counter's value before the update, minus the lane's rank in
+// the run, so that with delta = -1 every lane gets a distinct position (old - 1 - rank).
+__device__ __forceinline__ int aggregated_atomic(int* counter_base, int idx, bool valid, int delta) {
+  const int lane = (int)(threadIdx.x & 63);
+  const int key = valid ? idx : -1;
+  const int prev = __shfl_up(key, 1, 64);
+  const bool head = valid && (lane == 0 || prev != key);
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long valids = __ballot(valid);
+  // my run's head: the highest head at or below this lane
+  const unsigned long long below = heads & ((lane == 63) ? ~0ull : ((2ull << lane) - 1ull));
+  const int my_head = valid ? 63 - __clzll((long long)below) : 0;
+  // run length: lanes from the head up to (not including) the next head or invalid lane
+  const unsigned long long after = (heads | ~valids) & ~((2ull << my_head) - 1ull);
+  const int next = after ? __ffsll((long long)after) - 1 : 64;
+  const int run = next - my_head;
+  int old = 0;
+  if (head) old = atomicAdd(counter_base + idx, delta * run);
+  old = __shfl(old, my_head, 64);
+  return old - (lane - my_head);
+}
+
 __global__ __launch_bounds__(kBlock) void k_bin_keys(RebuildArgs a) {
   const int lb = logical_block();
   const InstanceDesc& in = a.inst[a.block_inst[lb]];
   const int s = lb * kBlock + (int)threadIdx.x;
   const int local = s - in.slot_begin;
-  if (local < in.n) {
+  const bool valid = local < in.n;
+  int g = 0;
+  if (valid) {
     const float4 p = a.x_old[s];
     int c[3];
     bin_coords_div(in, p.x, p.y, p.z, c);  // grid.hpp:394-395
     const int key = flatten(in, c[0], c[1], c[2]);
     a.keys[s] = key;
-    atomicAdd(&a.cnt[in.cbin_begin + (key >> in.shift)], 1);
+    g = in.cbin_begin + (key >> in.shift);
   } else if (local == in.n) {
     // the pad segment holds the instance's pad slots, so the global scan lands on
     // slot_begin of the next instance
     a.cnt[in.cbin_begin + in.cbin_count - 1] = in.padded - in.n;
   }
+  aggregated_atomic(a.cnt, g, valid, 1);
 }
 
 __global__ __launch_bounds__(kBlock) void k_scatter(RebuildArgs a) {
@@ -48,12 +75,20 @@ __global__ __launch_bounds__(kBlock) void k_scatter(RebuildArgs a) {
   const InstanceDesc& in = a.inst[a.block_inst[lb]];
   const int s = lb * kBlock + (int)threadIdx.x;
   const int local = s - in.slot_begin;
-  if (local < in.n) {
-    const int g = in.cbin_begin + (a.keys[s] >> in.shift);
-    const int pos = a.coff[g] + atomicSub(&a.cnt[g], 1) - 1;
-    a.tmp[pos] = s;
+  const bool valid = local < in.n;
+  int g = 0, key = 0;
+  if (valid) {
+    key = a.keys[s];
+    g = in.cbin_begin + (key >> in.shift);
   } else if (local == in.n) {
     a.cnt[in.cbin_begin + in.cbin_count - 1] = 0;
+  }
+  const int old = aggregated_atomic(a.cnt, g, valid, -1);
+  if (valid) {
+    const int pos = a.coff[g] + old - 1;
+    a.tmp[pos] = s;
+    a.tkey[pos] = key;
+    a.tid[pos] = a.id_old[s];
   }
 }
 
@@ -63,17 +98,16 @@ __global__ __launch_bounds__(kBlock) void k_gather(RebuildArgs a) {
   const InstanceDesc& in = a.inst[a.block_inst[lb]];
   const int t = lb * kBlock + (int)threadIdx.x;
   if ((t - in.slot_begin) >= in.n) return;
-  const int o = a.tmp[t];
-  const int key = a.keys[o];
+  const int key = a.tkey[t];
+  const int my_id = a.tid[t];
   const int g = in.cbin_begin + (key >> in.shift);
   const int start = a.coff[g], end = a.coff[g + 1];
-  const int my_id = a.id_old[o];
   int rank = 0;
   for (int u = start; u < end; ++u) {
-    const int ou = a.tmp[u];
-    const int ku = a.keys[ou];
-    rank += (ku < key || (ku == key && a.id_old[ou] < my_id)) ? 1 : 0;
+    const int ku = a.tkey[u];
+    rank += (ku < key || (ku == key && a.tid[u] < my_id)) ? 1 : 0;
   }
+  const int o = a.tmp[t];
   const int dst = start + rank;
   a.id_new[dst] = my_id;
   a.x_new[dst] = a.x_old[o];
@@ -83,24 +117,30 @@ __global__ __launch_bounds__(kBlock) void k_gather(RebuildArgs a) {
 }
 
 // Per-bin offsets from the segment offsets: off[b] = segment start + #(members with key < b).
-// One workgroup per 256 bins; a wavefront covers 64 consecutive bins, i.e. one segment or
-// part of one for shifts >= 6, so the segment reads are shared.
+// One thread per 4 consecutive bins (instance bin ranges start on multiples of 64 entries, so
+// the 16-byte stores of a wavefront cover 1 KiB of consecutive, aligned offsets); the threads
+// of a segment share its two offset loads, and only non-empty segments read member keys.
 __global__ __launch_bounds__(kBlock) void k_fill_offsets(RebuildArgs a) {
   const int blk = (int)blockIdx.x;
   const InstanceDesc& in = a.inst[a.fill_inst[blk]];
-  const int b = a.fill_base[blk] + (int)threadIdx.x;
-  if (b > in.bin_count) return;
-  int v;
-  if (b == in.bin_count) {
-    v = in.slot_begin + in.n;
-  } else {
+  const int b0 = (a.fill_base[blk] + (int)threadIdx.x) * 4;  // local bin of the quad
+  if (b0 > in.bin_count) return;
+  int* out = a.off + in.bin_begin;
+  int v[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int b = b0 + q;
+    if (b >= in.bin_count) {
+      v[q] = in.slot_begin + in.n;  // the pad bin (and the alignment gap after it)
+      continue;
+    }
     const int g = in.cbin_begin + (b >> in.shift);
     const int start = a.coff[g], end = a.coff[g + 1];
     int less = 0;
     for (int u = start; u < end; ++u) less += a.skey[u] < b ? 1 : 0;
-    v = start + less;
+    v[q] = start + less;
   }
-  a.off[in.bin_begin + b] = v;
+  *reinterpret_cast<int4*>(out + b0) = make_int4(v[0], v[1], v[2], v[3]);
 }
 
 // ------------------------------------------------------------------ exclusive scan (int)

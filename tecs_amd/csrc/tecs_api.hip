@@ -110,6 +110,8 @@ struct Batch {
   float2* d3p = nullptr;
   int* keys = nullptr;
   int* tmp = nullptr;
+  int* tkey = nullptr;
+  int* tidb = nullptr;
   int* cnt = nullptr;
   int* coff = nullptr;
   int* off = nullptr;
@@ -286,7 +288,9 @@ struct Batch {
       f.align = d.force.alignment_strength;
       slot += in.padded;
       blocks += nb;
-      bins += (int64_t)in.bin_count + 1;
+      // bin ranges start on multiples of 64 entries (aligned 16-byte stores in the fill) and
+      // leave room for the fill's last 4-bin quad, which may reach bin_count + 3
+      bins += (((int64_t)in.bin_count + 4 + 63) / 64) * 64;
       max_n = std::max(max_n, in.n);
       if (slot > (int64_t)1 << 30 || bins > (int64_t)2000000000)
         return fail(TECS_ERR_INVALID, "batch too large for 32-bit slot/bin indices");
@@ -303,14 +307,18 @@ struct Batch {
         h_inst[k].cbin_begin = fill_bins ? (int)cb : h_inst[k].bin_begin;
         cb += h_inst[k].cbin_count;
       }
-      total_cbins = (int)cb;
+      // shared arrays cover the padded bin ranges (gap entries keep count 0)
+      total_cbins = fill_bins ? (int)cb : total_bins;
     }
     if (fill_bins) {
-      for (int k = 0; k < count; ++k)
-        for (int b = 0; b <= h_inst[k].bin_count; b += kBlock) {
+      // one fill thread per 4 bins (the padded range has room for the last quad)
+      for (int k = 0; k < count; ++k) {
+        const int quads = (h_inst[k].bin_count + 1 + 3) / 4;
+        for (int q = 0; q < quads; q += kBlock) {
           h_fill_inst.push_back(k);
-          h_fill_base.push_back(b);
+          h_fill_base.push_back(q);
         }
+      }
     }
     h_block_inst.resize(n_blocks);
     for (int k = 0; k < count; ++k)
@@ -345,6 +353,8 @@ struct Batch {
     if (is_binned(finder)) {
       TRY(alloc(&keys, S));
       TRY(alloc(&tmp, S));
+      TRY(alloc(&tkey, S));
+      TRY(alloc(&tidb, S));
       TRY(alloc(&cnt, (size_t)total_cbins));
       TRY(alloc(&coff, (size_t)total_cbins + 1));
       if (fill_bins) {
@@ -473,6 +483,8 @@ struct Batch {
     r.ov_new = ov[xc ^ 1];
     r.keys = keys;
     r.tmp = tmp;
+    r.tkey = tkey;
+    r.tid = tidb;
     r.cnt = cnt;
     r.coff = coff;
     r.off = off;

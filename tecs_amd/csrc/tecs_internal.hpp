@@ -87,6 +87,8 @@ struct RebuildArgs {
   float4* ov_new;
   int* keys;
   int* tmp;
+  int* tkey;       // bin key of the cell scattered to each temporary slot
+  int* tid;        // agent index of the cell scattered to each temporary slot
   int* cnt;        // per-segment counts (zero between rebuilds)
   int* coff;       // per-segment offsets (global slots); aliases off when every shift is 0
   int* off;        // per-bin offsets (global slots)
@@ -96,7 +98,7 @@ struct RebuildArgs {
   // per-bin fill (only when some instance counts segments of several bins)
   bool fill;
   const int* fill_inst;  // fill workgroup -> instance
-  const int* fill_base;  // fill workgroup -> first local bin
+  const int* fill_base;  // fill workgroup -> first local 4-bin quad
   int fill_blocks;
 };
 void launch_rebuild(const RebuildArgs& a, int n_blocks, hipStream_t stream);
