@@ -76,6 +76,14 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the streaming copy kernel used to calibrate HBM byte counters")
+    ap.add_argument("--workload", default="epithelium",
+                    choices=["epithelium", "relaxation", "sorting", "sweep"],
+                    help="epithelium (default): 200k cells, BinnedGabriel + polarity, Heun; "
+                    "relaxation: 10k cells, BinnedAllPairs, Euler; sorting: 4M cells, "
+                    "BinnedAllPairs + type table, Heun; sweep: 256 mixed-size instances, batched "
+                    "vs one simulation at a time")
+    ap.add_argument("--plain-steps", type=int, default=20,
+                    help="sweep workload: steps per instance for the one-at-a-time comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this engine runs each simulation on a single device; "
@@ -99,10 +107,33 @@ def main():
                           "launches": 6, "mean_ms": ms.value,
                           "GB/s (read+write)": 2 * nbytes / (ms.value * 1e-3) / 1e9}))
         return
-    w = workloads.epithelium(args.n)
+    if args.workload == "sweep":
+        return bench_sweep(args)
+    if args.workload == "relaxation":
+        w = workloads.relaxation(10_000 if args.n == 200_000 else args.n)
+    elif args.workload == "sorting":
+        w = workloads.sorting(4_000_000 if args.n == 200_000 else args.n)
+    else:
+        w = workloads.epithelium(args.n)
+    return bench_single(args, w)
+
+
+WORKLOAD_TEXT = {
+    "epithelium": "polar epithelium: {n} cells on a sphere with polarity, BinnedGabriel c=0.8 "
+    "r_max=1, PiecewiseLinear(0.8,2,1)+bending+alignment, Heun dt=0.05",
+    "relaxation": "passive relaxation: {n} cells uniform in a ball of radius 0.5*cbrt(N), "
+    "BinnedAllPairs r_max=1, PiecewiseLinear(0.8,2,1), Euler dt=0.1",
+    "sorting": "large single tissue: {n} cells uniform in a cube at 1.5 cells/unit^3, types A/B, "
+    "BinnedAllPairs r_max=1, PiecewiseLinear(0.8,2,tau) tau_AB=0.3, Heun dt=0.05",
+}
+
+
+def bench_single(args, w):
+    from tecs_amd import GpuSimulation, abi
+
     sim = GpuSimulation(w.desc)
     sim.upload(w.positions, w.polarities, w.types)
-    print(f"bench: epithelium n={w.n} warmup={args.warmup} steps={args.steps}", file=sys.stderr, flush=True)
+    print(f"bench: {w.name} n={w.n} warmup={args.warmup} steps={args.steps}", file=sys.stderr, flush=True)
     sim.take_step(w.dt, args.warmup)
     sim.synchronize()
     sim.reset_stats()
@@ -126,7 +157,13 @@ def main():
     stale_ms, stale_n = sim.kernel_timing(abi.KT_SWEEP_STALE)
     reb_ms, reb_n = sim.kernel_timing(abi.KT_REBUILD)
     sim.set_kernel_timing(False)
-    bpc = algorithmic_bytes_per_cell(True, False)
+    has_pol = bool(w.desc.fields & abi.FIELD_POLARITY)
+    has_type = bool(w.desc.fields & abi.FIELD_TYPE)
+    bpc = algorithmic_bytes_per_cell(has_pol, has_type)
+    if w.desc.integrator == abi.EULER:
+        # Euler sweep: read s + a, write x' (s) and the old velocity (12)
+        s_bytes = 12 + (8 if has_pol else 0)
+        bpc = {"full": 2 * s_bytes + (1 if has_type else 0) + 12, "stale": 0}
     per = {
         "full": (full_ms / max(full_n, 1), bpc["full"]),
         "stale": (stale_ms / max(stale_n, 1), bpc["stale"]),
@@ -136,8 +173,12 @@ def main():
     achieved = (b_cell * w.n) / (avg_ms * 1e-3) / 1e9
     variant = abi.SWEEP_NAMES.get(sim.sweep_variant(), "?")
     kname = "k_sweep_binned" if variant == "thread" else f"k_sweep_team<{variant[4:]}>"
-    kernel_names = {"full": f"{kname} Gabriel polarity mode Pred (Heun substep 1)",
-                    "stale": f"{kname} Gabriel polarity mode Corr (Heun substep 2, stale grid)"}
+    fname = abi.FINDER_NAMES[w.desc.pair_finder]
+    if w.desc.integrator == abi.EULER:
+        kernel_names = {"full": f"{kname} {fname} mode Euler", "stale": "-"}
+    else:
+        kernel_names = {"full": f"{kname} {fname} mode Pred (Heun substep 1)",
+                        "stale": f"{kname} {fname} mode Corr (Heun substep 2, stale grid)"}
     print(
         f"bench: sweep full {per['full'][0]*1e3:.1f} us, stale {per['stale'][0]*1e3:.1f} us, "
         f"rebuild {reb_ms/max(reb_n,1)*1e3:.1f} us per launch",
@@ -146,7 +187,8 @@ def main():
     )
 
     out = {
-        "metric": "pair-interactions/sec per Heun step",
+        "metric": "pair-interactions/sec per Heun step" if w.desc.integrator == abi.HEUN
+        else "pair-interactions/sec per Euler step",
         "value": value,
         "unit": "pair-interactions/s",
         "n_gpus": 1,
@@ -159,8 +201,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded Philox, seed 2807)",
         "config": {
-            "workload": "polar epithelium: 200k cells on a sphere with polarity, BinnedGabriel "
-            "c=0.8 r_max=1, PiecewiseLinear(0.8,2,1)+bending+alignment, Heun dt=0.05",
+            "workload": WORKLOAD_TEXT[w.name].format(n=w.n),
             "cells": w.n,
             "pair_interactions_per_step": pairs / max(args.steps, 1),
             "parallelism": "single GPU",
@@ -187,6 +228,80 @@ def main():
         cb = cpu_baseline(w, args.cpu_seconds)
         out["cpu_baseline"] = cb
         out["gpu_over_cpu"] = value / cb["value"]
+    print(json.dumps(out), flush=True)
+
+
+def bench_sweep(args):
+    """256 independent instances (sizes log-uniform in [1k, 100k], relaxation or sorting
+    geometry, tau_AB swept over [0.1, 1]), Heun dt=0.05: batched (every kernel launched once per
+    substep for all instances) vs plain (each instance stepped alone, one after another)."""
+    from tecs_amd import GpuBatch, GpuSimulation, workloads
+
+    ws = workloads.sweep(256)
+    total_cells = sum(w.n for w in ws)
+    print(f"bench: sweep of {len(ws)} instances, {total_cells} cells", file=sys.stderr, flush=True)
+    b = GpuBatch([w.desc for w in ws])
+    for k, w in enumerate(ws):
+        b.upload(k, w.positions, w.polarities, w.types)
+    dts = [w.dt for w in ws]
+    b.take_step(dts, args.warmup)
+    b.synchronize()
+    b.reset_stats()
+    t0 = time.perf_counter()
+    b.take_step(dts, args.steps)
+    b.synchronize()
+    el = time.perf_counter() - t0
+    pairs = b.stats()["pair_interactions"]
+    batched = pairs / el
+    print(f"bench: batched {args.steps} steps in {el:.3f} s", file=sys.stderr, flush=True)
+    # plain: one simulation on the GPU at a time, the same instances and state
+    plain_pairs = 0
+    plain_time = 0.0
+    for w in ws:
+        s = GpuSimulation(w.desc)
+        s.upload(w.positions, w.polarities, w.types)
+        s.take_step(w.dt, 2)
+        s.synchronize()
+        s.reset_stats()
+        t1 = time.perf_counter()
+        s.take_step(w.dt, args.plain_steps)
+        s.synchronize()
+        plain_time += time.perf_counter() - t1
+        plain_pairs += s.stats()["pair_interactions"]
+        s.close()
+    plain = plain_pairs / plain_time
+    print(f"bench: plain {args.plain_steps} steps per instance in {plain_time:.3f} s",
+          file=sys.stderr, flush=True)
+    out = {
+        "metric": "pair-interactions/sec per Heun step, batched vs plain GPU",
+        "value": batched,
+        "unit": "pair-interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * el / max(args.steps, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Philox, seed 2807)",
+        "config": {
+            "workload": "batched parameter sweep: 256 instances, cells log-uniform in [1k,100k], "
+            "relaxation (ball) or sorting (cube, A/B) geometry, tau_AB linear in [0.1,1], "
+            "BinnedAllPairs, Heun dt=0.05",
+            "cells": total_cells,
+            "instances": len(ws),
+            "plain_value": plain,
+            "batched_over_plain": batched / plain,
+            "plain_steps_per_instance": args.plain_steps,
+            "parallelism": "single GPU",
+        },
+    }
+    if not args.no_cpu_baseline:
+        print("bench: cpu baseline ...", file=sys.stderr, flush=True)
+        cb = cpu_baseline(ws[0] if ws[0].n > 20000 else max(ws, key=lambda w: w.n), args.cpu_seconds)
+        out["cpu_baseline"] = cb
+        out["gpu_over_cpu"] = batched / cb["value"]
     print(json.dumps(out), flush=True)
 
 

@@ -127,18 +127,36 @@ __global__ __launch_bounds__(kBlock) void k_fill_offsets(RebuildArgs a) {
   if (b0 > in.bin_count) return;
   int* out = a.off + in.bin_begin;
   int v[4];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int b = b0 + q;
-    if (b >= in.bin_count) {
-      v[q] = in.slot_begin + in.n;  // the pad bin (and the alignment gap after it)
-      continue;
-    }
-    const int g = in.cbin_begin + (b >> in.shift);
+  if (in.shift >= 2 && b0 + 3 < in.bin_count) {
+    // the quad lies in one segment: one pair of segment offsets, member keys only if occupied
+    const int g = in.cbin_begin + (b0 >> in.shift);
     const int start = a.coff[g], end = a.coff[g + 1];
-    int less = 0;
-    for (int u = start; u < end; ++u) less += a.skey[u] < b ? 1 : 0;
-    v[q] = start + less;
+    int c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+    for (int u = start; u < end; ++u) {
+      const int k = a.skey[u];
+      c0 += k < b0 ? 1 : 0;
+      c1 += k < b0 + 1 ? 1 : 0;
+      c2 += k < b0 + 2 ? 1 : 0;
+      c3 += k < b0 + 3 ? 1 : 0;
+    }
+    v[0] = start + c0;
+    v[1] = start + c1;
+    v[2] = start + c2;
+    v[3] = start + c3;
+  } else {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int b = b0 + q;
+      if (b >= in.bin_count) {
+        v[q] = in.slot_begin + in.n;  // the pad bin (and the alignment gap after it)
+        continue;
+      }
+      const int g = in.cbin_begin + (b >> in.shift);
+      const int start = a.coff[g], end = a.coff[g + 1];
+      int less = 0;
+      for (int u = start; u < end; ++u) less += a.skey[u] < b ? 1 : 0;
+      v[q] = start + less;
+    }
   }
   *reinterpret_cast<int4*>(out + b0) = make_int4(v[0], v[1], v[2], v[3]);
 }
