@@ -242,6 +242,37 @@ int tecs_batch_set_kernel_timing(tecs_batch* batch, int32_t enabled);
 int tecs_batch_get_kernel_timing(tecs_batch* batch, int32_t kernel_class, double* ms,
                                  int64_t* launches);
 
+/* =========================================================================================
+ * Snapshots: io::HDF5_Writer (include/io/hdf5_writer.hpp) and Simulation::write
+ * (simulation.hpp:607-632). The file layout is the reference's: <path>.h5 with one group
+ * "t<step>" per write holding "position" [n x 3] and "polarity" [n x 2] float datasets, a
+ * "static" group, and the XDMF companion <path>.xmf. libhdf5 is loaded at tecs_writer_open
+ * (TECS_HDF5_LIB, libhdf5.so, or /opt/conda/lib/libhdf5.so); TECS_ERR_UNSUPPORTED without it.
+ * ========================================================================================= */
+typedef struct tecs_writer tecs_writer;
+/* HDF5_Writer(path, settings), hdf5_writer.hpp:66-79: truncates <path>.h5 (and <path>.xmf). */
+int tecs_writer_open(const char* path, int32_t write_xmf, tecs_writer** out);
+/* Waits for pending snapshots, writes the XDMF tail and closes (~HDF5_Writer, :84-86). */
+int tecs_writer_close(tecs_writer* writer);
+/* Waits until every snapshot handed to the writer is in the file. */
+int tecs_writer_flush(tecs_writer* writer);
+/* HDF5_Writer::write (:306-321) from host arrays, synchronously. */
+int tecs_writer_write(tecs_writer* writer, double time, uint32_t step, uint32_t n,
+                      const float* positions, const float* polarities);
+/* HDF5_Writer::write_static (:335-349) for one uint8 field; a second call fails. */
+int tecs_writer_write_static(tecs_writer* writer, const char* label, uint32_t n,
+                             const uint8_t* data);
+/* Simulation::write(time): snapshot the current state as group t<k>, k = this simulation's
+ * write counter. Asynchronous: one gather kernel runs on the simulation's stream, the copy to
+ * pinned host memory runs on a side stream and a writer thread writes the file, so following
+ * steps overlap with the copy and the file write. Blocks only when 3 snapshots are in flight. */
+int tecs_sim_write(tecs_sim* sim, tecs_writer* writer, double time);
+int tecs_batch_write(tecs_batch* batch, int32_t index, tecs_writer* writer, double time);
+/* HDF5_Reader::read_dataset (include/io/hdf5_reader.hpp:66-71): a float dataset of rank 1 or
+ * 2 ("t3/position"); dims[0..1] receive its shape; out may be NULL to query the shape. */
+int tecs_read_dataset(const char* h5_path, const char* dataset, float* out, int64_t capacity,
+                      int64_t* dims);
+
 /* Measurement helper with no counterpart in the reference: copies `bytes` (rounded down to a
  * multiple of 16) between two device buffers it allocates, `repeats` times, with a plain
  * streaming copy kernel (k_copy_probe), and reports the mean kernel time. Profilers read the

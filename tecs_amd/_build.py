@@ -19,7 +19,7 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 OUT_DIR = PKG / "_lib"
 LIB = OUT_DIR / "libtecs_amd.so"
-SOURCES = ["tecs_api.hip", "kernels_sweep.hip", "kernels_team.hip", "kernels_grid.hip"]
+SOURCES = ["tecs_api.hip", "kernels_sweep.hip", "kernels_team.hip", "kernels_grid.hip", "io_snapshot.hip"]
 ARCH = os.environ.get("TECS_AMD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
@@ -71,7 +71,7 @@ def build(verbose: bool = False, jobs: int = 3) -> Path:
     if failed:
         raise RuntimeError("hipcc failed:\n" + "\n".join(failed))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
+    cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs), "-ldl", "-lpthread"]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout.decode(errors="replace"))

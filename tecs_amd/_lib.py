@@ -57,6 +57,14 @@ PROTOTYPES = [
     ("tecs_batch_get_graph_launches", C.c_int, [_P, _I64P]),
     ("tecs_batch_set_kernel_timing", C.c_int, [_P, _I32]),
     ("tecs_batch_get_kernel_timing", C.c_int, [_P, _I32, C.POINTER(C.c_double), _I64P]),
+    ("tecs_writer_open", C.c_int, [C.c_char_p, _I32, C.POINTER(_P)]),
+    ("tecs_writer_close", C.c_int, [_P]),
+    ("tecs_writer_flush", C.c_int, [_P]),
+    ("tecs_writer_write", C.c_int, [_P, C.c_double, C.c_uint32, C.c_uint32, _F32P, _F32P]),
+    ("tecs_writer_write_static", C.c_int, [_P, C.c_char_p, C.c_uint32, _P]),
+    ("tecs_sim_write", C.c_int, [_P, _P, C.c_double]),
+    ("tecs_batch_write", C.c_int, [_P, _I32, _P, C.c_double]),
+    ("tecs_read_dataset", C.c_int, [C.c_char_p, C.c_char_p, _F32P, C.c_int64, _I64P]),
     ("tecs_copy_probe", C.c_int, [C.c_int64, _I32, C.POINTER(C.c_double)]),
 ]
 

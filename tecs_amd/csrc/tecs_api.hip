@@ -11,6 +11,7 @@
 //   Euler, GlobalComFixer:[rebuild] sweep(delta) -> com -> euler update
 // [rebuild] follows the reference schedule (binned_all_pairs.hpp:77-84).
 #include "../../include/tecs_amd.h"
+#include "io_snapshot.hpp"
 #include "tecs_internal.hpp"
 
 #include <cmath>
@@ -172,6 +173,14 @@ struct Batch {
   std::vector<StepGraph> graphs;
   int64_t graph_launches = 0;
 
+  // snapshot hand-off (io_snapshot.hip): gather on `stream`, copy on `side`, write elsewhere
+  static constexpr int kSnapshotSlots = 3;
+  SnapshotSlot snap[kSnapshotSlots];
+  std::mutex snap_mu;
+  std::condition_variable snap_cv;
+  hipStream_t side = nullptr;
+  std::vector<unsigned> write_step_count;  // Simulation::current_step per member
+
   HostState host_state() const { return HostState{xc, ic, step_count}; }
   int graph_period() const { return 2 * std::max(1, is_binned(finder) ? rebuild_every_n : 1); }
 
@@ -184,6 +193,24 @@ struct Batch {
 
   void release() {
     if (stream) hipStreamSynchronize(stream);
+    {
+      // snapshots in flight still reference the slots: wait for the writer to finish them
+      std::unique_lock<std::mutex> g(snap_mu);
+      snap_cv.wait(g, [&] {
+        for (auto& sl : snap)
+          if (sl.busy) return false;
+        return true;
+      });
+    }
+    for (auto& sl : snap) {
+      if (sl.h_pos) hipHostFree(sl.h_pos);
+      if (sl.h_pol) hipHostFree(sl.h_pol);
+      if (sl.gathered) hipEventDestroy(sl.gathered);
+      if (sl.copied) hipEventDestroy(sl.copied);
+      sl = SnapshotSlot{};
+    }
+    if (side) hipStreamDestroy(side);
+    side = nullptr;
     drop_graphs();
     for (auto& p : pending) {
       hipEventDestroy(p.a);
@@ -231,6 +258,7 @@ struct Batch {
     }
     descs.assign(ds, ds + count);
     n_inst = count;
+    write_step_count.assign(count, 0u);
 
     // layout: slots padded per instance to whole workgroups; bins concatenated (+1 pad bin)
     int64_t slot = 0, blocks = 0, bins = 0;
@@ -904,6 +932,66 @@ struct Batch {
     return TECS_OK;
   }
 
+  // Simulation::write(time) (simulation.hpp:607-611) for member k, off the critical path.
+  int write_snapshot(int k, Writer* w, double time) {
+    TRY(check_index(k));
+    const InstanceDesc& in = h_inst[k];
+    const size_t n = descs[k].agent_count;
+    const size_t cap = (size_t)std::max(max_n, 1);
+    if (!side) HIP_TRY(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    SnapshotSlot* sl = nullptr;
+    {
+      std::unique_lock<std::mutex> g(snap_mu);
+      snap_cv.wait(g, [&] {
+        for (auto& c : snap)
+          if (!c.busy) return true;
+        return false;
+      });
+      for (auto& c : snap)
+        if (!c.busy) {
+          sl = &c;
+          break;
+        }
+      sl->busy = true;
+    }
+    if (!sl->h_pos) {
+      sl->mu = &snap_mu;
+      sl->cv = &snap_cv;
+      int rc = alloc(&sl->d_pos, 3 * cap);
+      if (rc == TECS_OK && has_pol) rc = alloc(&sl->d_pol, 2 * cap);
+      hipError_t e = hipSuccess;
+      if (rc == TECS_OK) e = hipHostMalloc((void**)&sl->h_pos, sizeof(float) * 3 * cap, hipHostMallocDefault);
+      if (rc == TECS_OK && e == hipSuccess && has_pol)
+        e = hipHostMalloc((void**)&sl->h_pol, sizeof(float) * 2 * cap, hipHostMallocDefault);
+      if (rc == TECS_OK && e == hipSuccess) e = hipEventCreateWithFlags(&sl->gathered, hipEventDisableTiming);
+      if (rc == TECS_OK && e == hipSuccess) e = hipEventCreateWithFlags(&sl->copied, hipEventDisableTiming);
+      if (rc != TECS_OK || e != hipSuccess) {
+        sl->release();
+        return rc != TECS_OK ? rc : fail(TECS_ERR_OOM, std::string("snapshot buffers: ") + hipGetErrorString(e));
+      }
+    }
+    sl->n = n;
+    sl->has_pol = has_pol;
+    sl->time = time;
+    sl->step = write_step_count[k]++;
+    launch_download(d_inst, k, in.block_begin, in.n_blocks, id[ic], x[xc], pol[xc], nullptr,
+                    sl->d_pos, has_pol ? sl->d_pol : nullptr, nullptr, stream);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(sl->gathered, stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(side, sl->gathered, 0);
+    if (e == hipSuccess && n)
+      e = hipMemcpyAsync(sl->h_pos, sl->d_pos, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, side);
+    if (e == hipSuccess && n && has_pol)
+      e = hipMemcpyAsync(sl->h_pol, sl->d_pol, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, side);
+    if (e == hipSuccess) e = hipEventRecord(sl->copied, side);
+    if (e != hipSuccess) {
+      sl->release();
+      return fail(TECS_ERR_DEVICE, std::string("snapshot hand-off: ") + hipGetErrorString(e));
+    }
+    w->enqueue(sl);
+    return TECS_OK;
+  }
+
   int get_stats(tecs_stats* out) {
     std::vector<unsigned long long> bp((size_t)n_blocks * kMaxTeamLanes);
     HIP_TRY(hipMemcpyAsync(bp.data(), block_pairs, sizeof(unsigned long long) * bp.size(),
@@ -935,6 +1023,9 @@ struct tecs_sim {
 };
 struct tecs_batch {
   Batch b;
+};
+struct tecs_writer {
+  tecs_amd::Writer w;
 };
 
 extern "C" {
@@ -1221,6 +1312,76 @@ int tecs_batch_get_kernel_timing(tecs_batch* batch, int32_t kernel_class, double
   if (rc) return rc;
   if (ms) *ms = batch->b.t_ms[kernel_class];
   if (launches) *launches = batch->b.t_launches[kernel_class];
+  return TECS_OK;
+}
+
+// ---------------------------------------------------------------------------- snapshots
+int tecs_writer_open(const char* path, int32_t write_xmf, tecs_writer** out) {
+  if (!path || !out) return fail(TECS_ERR_INVALID, "null argument");
+  *out = nullptr;
+  tecs_writer* w = new (std::nothrow) tecs_writer();
+  if (!w) return fail(TECS_ERR_OOM, "host allocation failed");
+  std::string err;
+  if (w->w.open(path, write_xmf != 0, &err) != 0) {
+    delete w;
+    return fail(err.find("libhdf5") != std::string::npos ? TECS_ERR_UNSUPPORTED : TECS_ERR_INVALID, err);
+  }
+  *out = w;
+  return TECS_OK;
+}
+
+int tecs_writer_close(tecs_writer* w) {
+  GUARD(w);
+  std::string err;
+  const int rc = w->w.close(&err);
+  delete w;
+  if (rc != 0) return fail(TECS_ERR_INVALID, err);
+  return TECS_OK;
+}
+
+int tecs_writer_flush(tecs_writer* w) {
+  GUARD(w);
+  std::string err;
+  if (w->w.flush(&err) != 0) return fail(TECS_ERR_INVALID, err);
+  return TECS_OK;
+}
+
+int tecs_writer_write(tecs_writer* w, double time, uint32_t step, uint32_t n, const float* positions,
+                      const float* polarities) {
+  GUARD(w);
+  if (n && !positions) return fail(TECS_ERR_INVALID, "positions required");
+  std::string err;
+  if (w->w.write_step(time, step, n, positions, polarities, &err) != 0) return fail(TECS_ERR_INVALID, err);
+  return TECS_OK;
+}
+
+int tecs_writer_write_static(tecs_writer* w, const char* label, uint32_t n, const uint8_t* data) {
+  GUARD(w);
+  if (!label || (n && !data)) return fail(TECS_ERR_INVALID, "null argument");
+  std::string err;
+  if (w->w.write_static_u8(label, n, data, &err) != 0) return fail(TECS_ERR_STATE, err);
+  return TECS_OK;
+}
+
+int tecs_sim_write(tecs_sim* sim, tecs_writer* w, double time) {
+  GUARD(sim);
+  GUARD(w);
+  return sim->b.write_snapshot(0, &w->w, time);
+}
+
+int tecs_batch_write(tecs_batch* batch, int32_t index, tecs_writer* w, double time) {
+  GUARD(batch);
+  GUARD(w);
+  return batch->b.write_snapshot(index, &w->w, time);
+}
+
+int tecs_read_dataset(const char* h5_path, const char* dataset, float* out, int64_t capacity,
+                      int64_t* dims) {
+  if (!h5_path || !dataset || !dims) return fail(TECS_ERR_INVALID, "null argument");
+  std::string err;
+  const int rc = tecs_amd::read_dataset(h5_path, dataset, out, capacity, dims, &err);
+  if (rc == -2) return fail(TECS_ERR_CAPACITY, err);
+  if (rc != 0) return fail(err.find("libhdf5") != std::string::npos ? TECS_ERR_UNSUPPORTED : TECS_ERR_INVALID, err);
   return TECS_OK;
 }
 

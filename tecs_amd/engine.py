@@ -77,6 +77,10 @@ class GpuSimulation:
     def take_step(self, dt: float, n_steps: int = 1):
         check(self._lib.tecs_sim_take_step(self._h, float(dt), int(n_steps)))
 
+    def write(self, writer, time: float):
+        """Simulation::write(time): asynchronous snapshot into an io.Writer."""
+        check(self._lib.tecs_sim_write(self._h, writer.handle, float(time)))
+
     def synchronize(self):
         check(self._lib.tecs_sim_synchronize(self._h))
 
@@ -183,6 +187,9 @@ class GpuBatch:
         ov = np.empty((n, 3), np.float32)
         check(self._lib.tecs_batch_download(self._h, int(k), _ptr(pos), _ptr(pol), _ptr(ov)))
         return pos, pol, ov
+
+    def write(self, k, writer, time: float):
+        check(self._lib.tecs_batch_write(self._h, int(k), writer.handle, float(time)))
 
     def take_step(self, dts, n_steps: int = 1):
         dts = np.ascontiguousarray(np.broadcast_to(np.asarray(dts, np.float64), (len(self.descs),)))

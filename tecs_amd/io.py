@@ -1,0 +1,62 @@
+"""HDF5 snapshots in the reference's layout (io::HDF5_Writer, include/io/hdf5_writer.hpp)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import check, load
+
+
+class Writer:
+    """HDF5_Writer(path, settings): <path>.h5 (+ <path>.xmf) with groups t<step>/position etc."""
+
+    def __init__(self, path: str, write_xmf: bool = True):
+        self._lib = load()
+        h = C.c_void_p()
+        check(self._lib.tecs_writer_open(str(path).encode(), 1 if write_xmf else 0, C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def write(self, time: float, step: int, positions, polarities=None):
+        p = np.ascontiguousarray(positions, np.float32)
+        q = None if polarities is None else np.ascontiguousarray(polarities, np.float32)
+        check(self._lib.tecs_writer_write(self._h, float(time), int(step), p.shape[0], p.ctypes.data,
+                                          None if q is None else q.ctypes.data))
+
+    def write_static(self, label: str, data):
+        d = np.ascontiguousarray(data, np.uint8)
+        check(self._lib.tecs_writer_write_static(self._h, label.encode(), d.size, d.ctypes.data))
+
+    def flush(self):
+        check(self._lib.tecs_writer_flush(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            h, self._h = self._h, None
+            check(self._lib.tecs_writer_close(h))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def read_dataset(h5_path: str, name: str) -> np.ndarray:
+    """HDF5_Reader::read_dataset (include/io/hdf5_reader.hpp:66-71) for float datasets."""
+    lib = load()
+    dims = (C.c_int64 * 2)()
+    check(lib.tecs_read_dataset(str(h5_path).encode(), name.encode(), None, 0, dims))
+    out = np.empty((dims[0], dims[1]), np.float32)
+    check(lib.tecs_read_dataset(str(h5_path).encode(), name.encode(), out.ctypes.data, out.size, dims))
+    return out

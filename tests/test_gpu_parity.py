@@ -353,3 +353,28 @@ def test_batch_mixed_sparse_and_dense_grids(gpu, oracle_lib):
         o.upload(xa, pa)
         s.upload(xa, pa)
         _assert_same_csr(s.pairs(True), o.pairs(True), f"member {k} pairs after steps")
+
+
+def test_snapshots_async_hand_off(gpu, tmp_path):
+    """Simulation::write while stepping: every group t<k> holds the state at the k-th write,
+    bitwise, although the copies and the file writes overlap the following steps."""
+    from tecs_amd import io
+
+    w = _workload("epithelium", 20000, abi.BINNED_GABRIEL)
+    s = GpuSimulation(w.desc)
+    s.upload(w.positions, w.polarities, w.types)
+    ref = GpuSimulation(w.desc)
+    ref.upload(w.positions, w.polarities, w.types)
+    print("case snapshots epithelium n=20000", flush=True)
+    expected = []
+    with io.Writer(str(tmp_path / "snap" / "epi")) as writer:
+        for k in range(6):
+            s.write(writer, k * w.dt * 5)
+            x, p, _ = ref.download()
+            expected.append((x, p))
+            s.take_step(w.dt, 5)
+            ref.take_step(w.dt, 5)
+    h5 = str(tmp_path / "snap" / "epi.h5")
+    for k, (x, p) in enumerate(expected):
+        assert np.array_equal(io.read_dataset(h5, f"t{k}/position"), x), k
+        assert np.array_equal(io.read_dataset(h5, f"t{k}/polarity"), p), k
