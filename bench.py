@@ -77,13 +77,16 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the streaming copy kernel used to calibrate HBM byte counters")
     ap.add_argument("--workload", default="epithelium",
-                    choices=["epithelium", "relaxation", "sorting", "sweep"],
+                    choices=["epithelium", "relaxation", "sorting", "sweep", "serve"],
                     help="epithelium (default): 200k cells, BinnedGabriel + polarity, Heun; "
                     "relaxation: 10k cells, BinnedAllPairs, Euler; sorting: 4M cells, "
                     "BinnedAllPairs + type table, Heun; sweep: 256 mixed-size instances, batched "
                     "vs one simulation at a time")
     ap.add_argument("--plain-steps", type=int, default=20,
                     help="sweep workload: steps per instance for the one-at-a-time comparison")
+    ap.add_argument("--requests", type=int, default=48, help="serve workload: number of requests")
+    ap.add_argument("--rate", type=float, default=8.0, help="serve workload: arrivals per second")
+    ap.add_argument("--no-plain", action="store_true", help="serve workload: skip the one-at-a-time run")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this engine runs each simulation on a single device; "
@@ -109,6 +112,8 @@ def main():
         return
     if args.workload == "sweep":
         return bench_sweep(args)
+    if args.workload == "serve":
+        return bench_serve(args)
     if args.workload == "relaxation":
         w = workloads.relaxation(10_000 if args.n == 200_000 else args.n)
     elif args.workload == "sorting":
@@ -302,6 +307,104 @@ def bench_sweep(args):
         cb = cpu_baseline(ws[0] if ws[0].n > 20000 else max(ws, key=lambda w: w.n), args.cpu_seconds)
         out["cpu_baseline"] = cb
         out["gpu_over_cpu"] = batched / cb["value"]
+    print(json.dumps(out), flush=True)
+
+
+def _percentiles(xs):
+    import numpy as np
+
+    a = np.asarray(xs, np.float64)
+    return float(np.percentile(a, 50)), float(np.percentile(a, 99))
+
+
+def bench_serve(args):
+    """Served simulation requests: Poisson arrivals, models relaxation / epithelium / sorting
+    drawn uniformly, cells log-uniform in [2k, 500k], steps uniform in [100, 5000], admission by
+    an HBM budget of 200 GB, continuous batching; compared with serving the same requests one
+    at a time on the GPU (each runs alone, in arrival order)."""
+    from tecs_amd import GpuSimulation, workloads
+    from tecs_amd.serving import Server
+
+    reqs = workloads.serve_requests(args.requests, args.rate)
+    print(f"bench: serve {len(reqs)} requests at {args.rate}/s", file=sys.stderr, flush=True)
+    srv = Server(200 * 10**9, max_chunk_steps=32)
+    t0 = time.perf_counter()
+    ids = []
+    nxt = 0
+    done = set()
+    last_report = t0
+    while len(done) < len(reqs):
+        now = time.perf_counter() - t0
+        while nxt < len(reqs) and reqs[nxt].arrival <= now:
+            r = reqs[nxt]
+            ids.append(srv.submit(r.workload.desc, r.workload.positions, r.workload.polarities,
+                                  r.workload.types, r.workload.dt, r.n_steps))
+            nxt += 1
+        running, pending = srv.advance()
+        if running == 0 and pending == 0 and nxt < len(reqs):
+            time.sleep(max(0.0, reqs[nxt].arrival - (time.perf_counter() - t0)))
+        for k, rid in enumerate(ids):
+            if k not in done and srv.info(rid)["state"] == 2:
+                done.add(k)
+        if time.perf_counter() - last_report > 10:
+            last_report = time.perf_counter()
+            print(f"bench: serve {len(done)}/{len(reqs)} done", file=sys.stderr, flush=True)
+    makespan = time.perf_counter() - t0
+    st = srv.stats()
+    lat = [srv.info(rid)["t_done"] - srv.info(rid)["t_submit"] for rid in ids]
+    p50, p99 = _percentiles(lat)
+    served = st["pair_interactions"] / makespan
+    print(f"bench: served makespan {makespan:.2f} s p50 {p50:.2f} s p99 {p99:.2f} s",
+          file=sys.stderr, flush=True)
+    out = {
+        "metric": "pair-interactions/sec per Heun step, served; p50/p99 request latency",
+        "value": served,
+        "unit": "pair-interactions/s",
+        "n_gpus": 1,
+        "steps": int(sum(r.n_steps for r in reqs)),
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Philox, seed 2807)",
+        "config": {
+            "workload": f"served requests: {len(reqs)} Poisson arrivals at {args.rate}/s, model "
+            "relaxation/epithelium/sorting uniform, cells log-uniform [2k,500k], steps uniform "
+            "[100,5000], HBM budget 200 GB, continuous batching (chunks of <= 32 steps)",
+            "requests": len(reqs),
+            "cells_total": int(sum(r.workload.n for r in reqs)),
+            "makespan_s": makespan,
+            "latency_p50_s": p50,
+            "latency_p99_s": p99,
+            "regroups": st["regroups"],
+            "parallelism": "single GPU",
+        },
+    }
+    if not args.no_plain:
+        plain_pairs = 0
+        t1 = time.perf_counter()
+        lat_plain = []
+        for r in reqs:
+            now = time.perf_counter() - t1
+            if r.arrival > now:
+                time.sleep(r.arrival - now)
+            s = GpuSimulation(r.workload.desc)
+            s.upload(r.workload.positions, r.workload.polarities, r.workload.types)
+            s.take_step(r.workload.dt, r.n_steps)
+            s.synchronize()
+            s.download()
+            plain_pairs += s.stats()["pair_interactions"]
+            lat_plain.append(time.perf_counter() - t1 - r.arrival)
+            s.close()
+        plain_span = time.perf_counter() - t1
+        q50, q99 = _percentiles(lat_plain)
+        out["config"].update({"plain_value": plain_pairs / plain_span, "plain_makespan_s": plain_span,
+                              "plain_latency_p50_s": q50, "plain_latency_p99_s": q99,
+                              "served_over_plain": served / (plain_pairs / plain_span)})
+        print(f"bench: plain makespan {plain_span:.2f} s p50 {q50:.2f} s p99 {q99:.2f} s",
+              file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 
 

@@ -273,6 +273,47 @@ int tecs_batch_write(tecs_batch* batch, int32_t index, tecs_writer* writer, doub
 int tecs_read_dataset(const char* h5_path, const char* dataset, float* out, int64_t capacity,
                       int64_t* dims);
 
+/* =========================================================================================
+ * Serving: simulation requests admitted by an HBM budget as they arrive and advanced together
+ * (continuous batching). Requests that share pair finder, integrator, COM fixer, fields and
+ * old-velocity tracking run as members of one batch; a batch is re-formed when members join or
+ * leave, with the running members' state copied on the device. Every request keeps its own
+ * arithmetic: its result is bitwise that of the same simulation run alone. Served requests must
+ * rebuild the grid every step (rebuild_every_n = 0). No counterpart in the reference.
+ * ========================================================================================= */
+typedef struct tecs_server tecs_server;
+enum { TECS_REQUEST_PENDING = 0, TECS_REQUEST_RUNNING = 1, TECS_REQUEST_DONE = 2 };
+typedef struct tecs_request_info {
+  int32_t state;        /* TECS_REQUEST_* */
+  int32_t steps_done;
+  double t_submit;      /* seconds since the server was created */
+  double t_admit;
+  double t_done;
+  uint64_t hbm_bytes;   /* the device bytes the request is charged */
+} tecs_request_info;
+typedef struct tecs_server_stats {
+  uint64_t completed, pending, running;
+  uint64_t pair_interactions;  /* over all requests, all evaluations */
+  uint64_t member_steps;       /* sum over requests of the steps run */
+  uint64_t regroups;           /* batches re-formed */
+  uint64_t hbm_bytes_in_use, hbm_budget;
+} tecs_server_stats;
+/* max_chunk_steps: steps per scheduling round at most (admission and retirement happen
+ * between rounds). */
+int tecs_server_create(uint64_t hbm_budget_bytes, int32_t max_chunk_steps, tecs_server** out);
+int tecs_server_destroy(tecs_server* server);
+/* Queue a request (inputs are copied). TECS_ERR_CAPACITY if it could never fit the budget. */
+int tecs_server_submit(tecs_server* server, const tecs_sim_desc* desc, const float* positions,
+                       const float* polarities, const uint8_t* types, double dt,
+                       int32_t n_steps, int64_t* request_id);
+/* One scheduling round: admit, re-form batches, step every batch, retire finished requests. */
+int tecs_server_advance(tecs_server* server, int32_t* running, int32_t* pending);
+int tecs_server_request_info(tecs_server* server, int64_t request_id, tecs_request_info* info);
+/* The final state of a finished request (positions n*3, polarities n*2 or NULL). */
+int tecs_server_result(tecs_server* server, int64_t request_id, float* positions,
+                       float* polarities);
+int tecs_server_get_stats(tecs_server* server, tecs_server_stats* stats);
+
 /* Measurement helper with no counterpart in the reference: copies `bytes` (rounded down to a
  * multiple of 16) between two device buffers it allocates, `repeats` times, with a plain
  * streaming copy kernel (k_copy_probe), and reports the mean kernel time. Profilers read the

@@ -65,6 +65,13 @@ PROTOTYPES = [
     ("tecs_sim_write", C.c_int, [_P, _P, C.c_double]),
     ("tecs_batch_write", C.c_int, [_P, _I32, _P, C.c_double]),
     ("tecs_read_dataset", C.c_int, [C.c_char_p, C.c_char_p, _F32P, C.c_int64, _I64P]),
+    ("tecs_server_create", C.c_int, [C.c_uint64, _I32, C.POINTER(_P)]),
+    ("tecs_server_destroy", C.c_int, [_P]),
+    ("tecs_server_submit", C.c_int, [_P, C.POINTER(abi.SimDesc), _F32P, _F32P, _P, C.c_double, _I32, _I64P]),
+    ("tecs_server_advance", C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    ("tecs_server_request_info", C.c_int, [_P, C.c_int64, C.POINTER(abi.RequestInfo)]),
+    ("tecs_server_result", C.c_int, [_P, C.c_int64, _F32P, _F32P]),
+    ("tecs_server_get_stats", C.c_int, [_P, C.POINTER(abi.ServerStats)]),
     ("tecs_copy_probe", C.c_int, [C.c_int64, _I32, C.POINTER(C.c_double)]),
 ]
 

@@ -166,3 +166,32 @@ def copy_desc(d: SimDesc) -> SimDesc:
     out = SimDesc()
     C.memmove(C.addressof(out), C.addressof(d), C.sizeof(SimDesc))
     return out
+
+
+REQUEST_PENDING = 0
+REQUEST_RUNNING = 1
+REQUEST_DONE = 2
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [
+        ("state", C.c_int32),
+        ("steps_done", C.c_int32),
+        ("t_submit", C.c_double),
+        ("t_admit", C.c_double),
+        ("t_done", C.c_double),
+        ("hbm_bytes", C.c_uint64),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("completed", C.c_uint64),
+        ("pending", C.c_uint64),
+        ("running", C.c_uint64),
+        ("pair_interactions", C.c_uint64),
+        ("member_steps", C.c_uint64),
+        ("regroups", C.c_uint64),
+        ("hbm_bytes_in_use", C.c_uint64),
+        ("hbm_budget", C.c_uint64),
+    ]

@@ -1,0 +1,1067 @@
+// batch.hpp - the device state of one or more simulations and their step schedule.
+//
+// A Batch owns the device state of one or more independent simulations that share the pair
+// finder, integrator, COM fixer and field set; each step runs one launch per kernel for all of
+// them. tecs_sim is a batch of one; tecs_batch exposes the general form; the server keeps one
+// batch per compatible group of requests.
+//
+// Step schedule (no HIP allocation, memset or host synchronisation inside a step):
+//   Heun, NoComFixer:     [rebuild] sweep(pred: d1, x~) -> sweep(corr: x, old_vel)
+//   Heun, GlobalComFixer: [rebuild] sweep(delta) -> com -> pred -> sweep(delta) -> com -> corr
+//   Euler, NoComFixer:    [rebuild] sweep(euler: x', old_vel) (ping-pong position buffers)
+//   Euler, GlobalComFixer:[rebuild] sweep(delta) -> com -> euler update
+// [rebuild] follows the reference schedule (binned_all_pairs.hpp:77-84).
+#pragma once
+
+#include "../../include/tecs_amd.h"
+#include "io_snapshot.hpp"
+#include "tecs_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace tecs_amd {
+
+extern thread_local std::string g_err;
+
+inline int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+
+#define HIP_TRY(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(e_ == hipErrorOutOfMemory ? TECS_ERR_OOM : TECS_ERR_DEVICE,          \
+                  std::string(#expr) + " failed: " + hipGetErrorString(e_));           \
+  } while (0)
+
+#define TRY(expr)            \
+  do {                       \
+    const int rc_ = (expr);  \
+    if (rc_ != TECS_OK) return rc_; \
+  } while (0)
+
+enum TimingClass { kTimeSweepFull = 0, kTimeSweepStale = 1, kTimeRebuild = 2, kTimeClasses = 3 };
+
+inline bool is_binned(int finder) { return finder == kBinnedAllPairs || finder == kBinnedGabriel; }
+
+inline int validate_desc(const tecs_sim_desc* d) {
+  if (!d) return fail(TECS_ERR_INVALID, "null descriptor");
+  if (d->capacity < d->agent_count) return fail(TECS_ERR_INVALID, "capacity < agent_count");
+  if (d->agent_count > (1u << 30)) return fail(TECS_ERR_INVALID, "agent_count too large");
+  if (d->pair_finder < 0 || d->pair_finder > 3) return fail(TECS_ERR_INVALID, "bad pair_finder");
+  if (d->integrator != TECS_EULER && d->integrator != TECS_HEUN)
+    return fail(TECS_ERR_INVALID, "bad integrator");
+  if (d->com_fixer != TECS_NO_COM_FIXER && d->com_fixer != TECS_GLOBAL_COM_FIXER)
+    return fail(TECS_ERR_INVALID, "bad com_fixer");
+  const tecs_force_desc& f = d->force;
+  if (f.n_types < 0 || f.n_types > TECS_MAX_TYPES) return fail(TECS_ERR_INVALID, "bad n_types");
+  if (f.n_types > 0 && !(d->fields & TECS_FIELD_TYPE))
+    return fail(TECS_ERR_INVALID, "type table needs TECS_FIELD_TYPE");
+  if ((f.bending_strength != 0.0f || f.alignment_strength != 0.0f) &&
+      !(d->fields & TECS_FIELD_POLARITY))
+    return fail(TECS_ERR_INVALID, "polarity forces need TECS_FIELD_POLARITY");
+  if (!(d->cutoff_distance > 0.0f) || !std::isfinite(d->cutoff_distance))
+    return fail(TECS_ERR_INVALID, "cutoff_distance must be finite and > 0");
+  if (is_binned(d->pair_finder)) {
+    const tecs_pair_finder_settings& s = d->pair_finder_settings;
+    if (!(s.bin_size_scale > 0.0f)) return fail(TECS_ERR_INVALID, "bin_size_scale must be > 0");
+    const float bin = s.bin_size_scale * d->cutoff_distance;
+    double bins = 1.0;
+    for (int k = 0; k < 3; ++k) {
+      if (!(s.max_bounds[k] > s.min_bounds[k]))
+        return fail(TECS_ERR_INVALID, "max_bounds must exceed min_bounds");
+      bins *= std::ceil((double)((s.max_bounds[k] - s.min_bounds[k]) / bin));
+    }
+    if (bins > 1.0e9) return fail(TECS_ERR_INVALID, "grid has more than 1e9 bins");
+    if (s.rebuild_every_n < 0) return fail(TECS_ERR_INVALID, "rebuild_every_n must be >= 0");
+  }
+  return TECS_OK;
+}
+
+struct Batch {
+  // homogeneous configuration
+  int finder = 0, integrator = 0, com_fixer = 0, rebuild_every_n = 0;
+  bool has_pol = false, has_type = false, track_ov = true;
+  std::vector<tecs_sim_desc> descs;
+  std::vector<InstanceDesc> h_inst;
+  std::vector<int> h_block_inst;
+  int n_inst = 0, n_blocks = 0, total_slots = 0, total_bins = 0, max_n = 0;
+  int total_cbins = 0;         // counting-sort segments of all instances
+  bool fill_bins = false;      // some instance counts multi-bin segments (sparse grid)
+  std::vector<int> h_fill_inst, h_fill_base;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  bool inst_dirty = false;
+
+  // device state
+  InstanceDesc* d_inst = nullptr;
+  int* d_block_inst = nullptr;
+  float4* x[2] = {nullptr, nullptr};
+  float2* pol[2] = {nullptr, nullptr};
+  int* id[2] = {nullptr, nullptr};
+  float4* ov[2] = {nullptr, nullptr};
+  float4* xp = nullptr;
+  float2* polp = nullptr;
+  float4* d1 = nullptr;
+  float2* d1p = nullptr;
+  float4* d3 = nullptr;
+  float2* d3p = nullptr;
+  int* keys = nullptr;
+  int* tmp = nullptr;
+  int* tkey = nullptr;
+  int* tidb = nullptr;
+  int* cnt = nullptr;
+  int* coff = nullptr;
+  int* off = nullptr;
+  int* skey = nullptr;
+  int* d_fill_inst = nullptr;
+  int* d_fill_base = nullptr;
+  int* bsum = nullptr;
+  unsigned long long* block_pairs = nullptr;
+  float4* com_partial = nullptr;
+  float4* com = nullptr;
+  float* st_pos = nullptr;
+  float* st_pol = nullptr;
+  float* st_ov = nullptr;
+  uint8_t* st_type = nullptr;
+  int* d_row_count = nullptr;
+  int64_t* d_row_ptr = nullptr;
+  int32_t* d_cols = nullptr;
+  int64_t cols_cap = 0;
+  std::vector<void*> allocations;
+
+  int variant = TECS_SWEEP_AUTO;  // requested sweep variant
+  int tuned = -1;                 // concrete variant picked by the autotuner (AUTO)
+  bool team_ok = false;           // the team sweep reproduces this batch exactly
+  float4* trig = nullptr;         // sin/cos of the polarity angles of the evaluated stage
+  unsigned long long* tune_pairs = nullptr;  // scratch counters for autotuning sweeps
+  double tune_ms[6] = {0, 0, 0, 0, 0, 0};
+  int xc = 0, ic = 0;  // current position/polarity/old-velocity buffer, current id buffer
+  int step_count = 0;
+  bool grid_built = false;
+  tecs_stats stats{};
+
+  bool timing = false;
+  struct Pending {
+    hipEvent_t a, b;
+    int cls;
+  };
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<Pending> pending;
+  double t_ms[kTimeClasses] = {0, 0, 0};
+  int64_t t_launches[kTimeClasses] = {0, 0, 0};
+
+  // hipGraph replay of step sequences: the host-side state (buffer parities, rebuild phase)
+  // repeats with period graph_period(); a sequence of that many steps captured from a state
+  // can be replayed whenever the batch is in that state again. Kernels read dt from the
+  // device-side instance table, so a dt change does not invalidate a graph.
+  int graph_mode = TECS_GRAPHS_AUTO;
+  struct HostState {
+    int xc, ic, step_count;
+    bool operator==(const HostState& o) const {
+      return xc == o.xc && ic == o.ic && step_count == o.step_count;
+    }
+  };
+  struct StepGraph {
+    HostState start;
+    hipGraphExec_t exec;
+    tecs_stats delta;  // counters advanced by one replay
+  };
+  std::vector<StepGraph> graphs;
+  int64_t graph_launches = 0;
+
+  // snapshot hand-off (io_snapshot.hip): gather on `stream`, copy on `side`, write elsewhere
+  static constexpr int kSnapshotSlots = 3;
+  SnapshotSlot snap[kSnapshotSlots];
+  std::mutex snap_mu;
+  std::condition_variable snap_cv;
+  hipStream_t side = nullptr;
+  std::vector<unsigned> write_step_count;  // Simulation::current_step per member
+
+  HostState host_state() const { return HostState{xc, ic, step_count}; }
+  int graph_period() const { return 2 * std::max(1, is_binned(finder) ? rebuild_every_n : 1); }
+
+  void drop_graphs() {
+    for (auto& g : graphs) hipGraphExecDestroy(g.exec);
+    graphs.clear();
+  }
+
+  ~Batch() { release(); }
+
+  void release() {
+    if (stream) hipStreamSynchronize(stream);
+    {
+      // snapshots in flight still reference the slots: wait for the writer to finish them
+      std::unique_lock<std::mutex> g(snap_mu);
+      snap_cv.wait(g, [&] {
+        for (auto& sl : snap)
+          if (sl.busy) return false;
+        return true;
+      });
+    }
+    for (auto& sl : snap) {
+      if (sl.h_pos) hipHostFree(sl.h_pos);
+      if (sl.h_pol) hipHostFree(sl.h_pol);
+      if (sl.gathered) hipEventDestroy(sl.gathered);
+      if (sl.copied) hipEventDestroy(sl.copied);
+      sl = SnapshotSlot{};
+    }
+    if (side) hipStreamDestroy(side);
+    side = nullptr;
+    drop_graphs();
+    for (auto& p : pending) {
+      hipEventDestroy(p.a);
+      hipEventDestroy(p.b);
+    }
+    pending.clear();
+    for (hipEvent_t e : ev_pool) hipEventDestroy(e);
+    ev_pool.clear();
+    for (void* p : allocations) hipFree(p);
+    allocations.clear();
+    if (own_stream && stream) hipStreamDestroy(stream);
+    stream = nullptr;
+  }
+
+  template <typename T>
+  int alloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    void* v = nullptr;
+    HIP_TRY(hipMalloc(&v, count * sizeof(T)));
+    allocations.push_back(v);
+    *p = static_cast<T*>(v);
+    return TECS_OK;
+  }
+
+  int init(const tecs_sim_desc* ds, int count, void* user_stream) {
+    if (count <= 0) return fail(TECS_ERR_INVALID, "batch needs at least one simulation");
+    for (int k = 0; k < count; ++k) TRY(validate_desc(&ds[k]));
+    const tecs_sim_desc& d0 = ds[0];
+    finder = d0.pair_finder;
+    integrator = d0.integrator;
+    com_fixer = d0.com_fixer;
+    rebuild_every_n = d0.pair_finder_settings.rebuild_every_n;
+    has_pol = (d0.fields & TECS_FIELD_POLARITY) != 0;
+    has_type = (d0.fields & TECS_FIELD_TYPE) != 0;
+    track_ov = d0.track_old_velocities != 0;
+    for (int k = 1; k < count; ++k) {
+      const tecs_sim_desc& d = ds[k];
+      if (d.pair_finder != finder || d.integrator != integrator || d.com_fixer != com_fixer ||
+          d.fields != d0.fields || (d.track_old_velocities != 0) != track_ov ||
+          (is_binned(finder) && d.pair_finder_settings.rebuild_every_n != rebuild_every_n))
+        return fail(TECS_ERR_INVALID,
+                    "batched simulations must share pair finder, integrator, com fixer, fields, "
+                    "old-velocity tracking and rebuild period");
+    }
+    descs.assign(ds, ds + count);
+    n_inst = count;
+    write_step_count.assign(count, 0u);
+
+    // layout: slots padded per instance to whole workgroups; bins concatenated (+1 pad bin)
+    int64_t slot = 0, blocks = 0, bins = 0;
+    h_inst.resize(count);
+    for (int k = 0; k < count; ++k) {
+      const tecs_sim_desc& d = ds[k];
+      InstanceDesc& in = h_inst[k];
+      std::memset(&in, 0, sizeof(in));
+      in.slot_begin = (int)slot;
+      in.n = (int)d.agent_count;
+      in.cap = (int)d.capacity;
+      const int nb = std::max(1, (int)((d.agent_count + kBlock - 1) / kBlock));
+      in.padded = nb * kBlock;
+      in.block_begin = (int)blocks;
+      in.n_blocks = nb;
+      const float cutoff = d.cutoff_distance;
+      in.cutoff2 = cutoff * cutoff;  // binned_all_pairs.hpp:33
+      const tecs_pair_finder_settings& ps = d.pair_finder_settings;
+      in.gabriel_factor = ps.gabriel_coefficient * ps.gabriel_coefficient * 0.25f;
+      in.gabriel_c2 = ps.gabriel_coefficient * ps.gabriel_coefficient;
+      if (is_binned(finder)) {
+        // binned_all_pairs.hpp:36-38, grid.hpp:201-215
+        in.bin_size = ps.bin_size_scale * cutoff;
+        in.inv_bin = 1.0f / in.bin_size;
+        in.search_radius = (int)std::ceil(cutoff * in.inv_bin);
+        int bc = 1;
+        for (int q = 0; q < 3; ++q) {
+          in.minb[q] = ps.min_bounds[q];
+          in.ext[q] = (int)std::ceil((ps.max_bounds[q] - ps.min_bounds[q]) / in.bin_size);
+          bc *= in.ext[q];
+        }
+        in.bin_count = bc;
+      } else {
+        in.bin_size = 1.0f;
+        in.inv_bin = 1.0f;
+        in.bin_count = 0;
+      }
+      in.bin_begin = (int)bins;
+      // sparse grids (many bins per cell, e.g. a sheet of cells on a sphere) count segments of
+      // 2^shift bins so the scan does not run over every empty bin
+      in.shift = 0;
+      if (is_binned(finder) && in.n > 0) {
+        const double bpc = (double)in.bin_count / (double)in.n;
+        if (bpc >= 8.0) in.shift = std::min(10, (int)std::floor(std::log2(bpc)));
+      }
+      in.cbin_count = ((in.bin_count + (1 << in.shift) - 1) >> in.shift) + 1;
+      in.dt = 0.0f;
+      ForceParams& f = in.force;
+      f.h = d.force.homeostatic_radius;
+      f.krep = d.force.repulsion_scale;
+      f.kadh = d.force.adhesion_scale;
+      f.n_types = d.force.n_types;
+      std::memcpy(f.table, d.force.adhesion_table, sizeof(f.table));
+      f.bend = d.force.bending_strength;
+      f.align = d.force.alignment_strength;
+      slot += in.padded;
+      blocks += nb;
+      // bin ranges start on multiples of 64 entries (aligned 16-byte stores in the fill) and
+      // leave room for the fill's last 4-bin quad, which may reach bin_count + 3
+      bins += (((int64_t)in.bin_count + 4 + 63) / 64) * 64;
+      max_n = std::max(max_n, in.n);
+      if (slot > (int64_t)1 << 30 || bins > (int64_t)2000000000)
+        return fail(TECS_ERR_INVALID, "batch too large for 32-bit slot/bin indices");
+    }
+    total_slots = (int)slot;
+    n_blocks = (int)blocks;
+    total_bins = (int)bins;
+    fill_bins = false;
+    for (int k = 0; k < count; ++k) fill_bins = fill_bins || h_inst[k].shift > 0;
+    {
+      int64_t cb = 0;
+      for (int k = 0; k < count; ++k) {
+        // without any sparse instance the segments are the bins (shared arrays)
+        h_inst[k].cbin_begin = fill_bins ? (int)cb : h_inst[k].bin_begin;
+        cb += h_inst[k].cbin_count;
+      }
+      // shared arrays cover the padded bin ranges (gap entries keep count 0)
+      total_cbins = fill_bins ? (int)cb : total_bins;
+    }
+    if (fill_bins) {
+      // one fill thread per 4 bins (the padded range has room for the last quad)
+      for (int k = 0; k < count; ++k) {
+        const int quads = (h_inst[k].bin_count + 1 + 3) / 4;
+        for (int q = 0; q < quads; q += kBlock) {
+          h_fill_inst.push_back(k);
+          h_fill_base.push_back(q);
+        }
+      }
+    }
+    h_block_inst.resize(n_blocks);
+    for (int k = 0; k < count; ++k)
+      for (int b = 0; b < h_inst[k].n_blocks; ++b) h_block_inst[h_inst[k].block_begin + b] = k;
+
+    if (user_stream) {
+      stream = static_cast<hipStream_t>(user_stream);
+    } else {
+      HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+      own_stream = true;
+    }
+
+    const size_t S = (size_t)total_slots;
+    TRY(alloc(&d_inst, n_inst));
+    TRY(alloc(&d_block_inst, n_blocks));
+    for (int b = 0; b < 2; ++b) {
+      TRY(alloc(&x[b], S));
+      TRY(alloc(&id[b], S));
+      if (has_pol) TRY(alloc(&pol[b], S));
+      if (track_ov) TRY(alloc(&ov[b], S));
+    }
+    TRY(alloc(&d1, S));
+    if (has_pol) TRY(alloc(&d1p, S));
+    if (integrator == TECS_HEUN) {
+      TRY(alloc(&xp, S));
+      if (has_pol) TRY(alloc(&polp, S));
+    }
+    if (integrator == TECS_HEUN && com_fixer == TECS_GLOBAL_COM_FIXER) {
+      TRY(alloc(&d3, S));
+      if (has_pol) TRY(alloc(&d3p, S));
+    }
+    if (is_binned(finder)) {
+      TRY(alloc(&keys, S));
+      TRY(alloc(&tmp, S));
+      TRY(alloc(&tkey, S));
+      TRY(alloc(&tidb, S));
+      TRY(alloc(&cnt, (size_t)total_cbins));
+      TRY(alloc(&coff, (size_t)total_cbins + 1));
+      if (fill_bins) {
+        TRY(alloc(&off, (size_t)total_bins + 1));
+        TRY(alloc(&skey, S));
+        TRY(alloc(&d_fill_inst, h_fill_inst.size()));
+        TRY(alloc(&d_fill_base, h_fill_base.size()));
+        HIP_TRY(hipMemcpyAsync(d_fill_inst, h_fill_inst.data(), sizeof(int) * h_fill_inst.size(),
+                               hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipMemcpyAsync(d_fill_base, h_fill_base.data(), sizeof(int) * h_fill_base.size(),
+                               hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipMemsetAsync(off, 0, sizeof(int) * ((size_t)total_bins + 1), stream));
+      } else {
+        off = coff;
+      }
+      TRY(alloc(&bsum, (size_t)scan_tiles_for(total_cbins)));
+      HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(int) * (size_t)total_cbins, stream));
+      HIP_TRY(hipMemsetAsync(coff, 0, sizeof(int) * ((size_t)total_cbins + 1), stream));
+    }
+    // team sweep conditions: search radius 1 and, for Gabriel, c <= 0.95 (kernels_team.hip)
+    team_ok = is_binned(finder);
+    for (int k = 0; k < count; ++k) {
+      if (h_inst[k].search_radius != 1) team_ok = false;
+      const float c = ds[k].pair_finder_settings.gabriel_coefficient;
+      if (finder == kBinnedGabriel && !(c >= 0.0f && c <= 0.95f)) team_ok = false;
+    }
+    TRY(alloc(&block_pairs, (size_t)n_blocks * kMaxTeamLanes));
+    TRY(alloc(&tune_pairs, (size_t)n_blocks * kMaxTeamLanes));
+    if (has_pol) TRY(alloc(&trig, S));
+    TRY(alloc(&com_partial, (size_t)n_blocks * kMaxTeamLanes));
+    TRY(alloc(&com, n_inst));
+    HIP_TRY(hipMemsetAsync(block_pairs, 0, sizeof(unsigned long long) * n_blocks * kMaxTeamLanes, stream));
+    HIP_TRY(hipMemsetAsync(com, 0, sizeof(float4) * n_inst, stream));
+    for (int b = 0; b < 2; ++b) {
+      HIP_TRY(hipMemsetAsync(x[b], 0, sizeof(float4) * S, stream));
+      if (has_pol) HIP_TRY(hipMemsetAsync(pol[b], 0, sizeof(float2) * S, stream));
+      if (track_ov) HIP_TRY(hipMemsetAsync(ov[b], 0, sizeof(float4) * S, stream));
+    }
+    const size_t M = (size_t)std::max(max_n, 1);
+    TRY(alloc(&st_pos, 3 * M));
+    TRY(alloc(&st_ov, 3 * M));
+    if (has_pol) TRY(alloc(&st_pol, 2 * M));
+    if (has_type) TRY(alloc(&st_type, M));
+    TRY(alloc(&d_row_count, M));
+    TRY(alloc(&d_row_ptr, M + 1));
+    HIP_TRY(hipMemcpyAsync(d_inst, h_inst.data(), sizeof(InstanceDesc) * n_inst,
+                           hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipMemcpyAsync(d_block_inst, h_block_inst.data(), sizeof(int) * n_blocks,
+                           hipMemcpyHostToDevice, stream));
+    for (int b = 0; b < 2; ++b) launch_init_ids(d_inst, d_block_inst, id[b], n_blocks, stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(stream));
+    return TECS_OK;
+  }
+
+  // ---------------------------------------------------------------------- timing
+  int ev_get(hipEvent_t* e) {
+    if (!ev_pool.empty()) {
+      *e = ev_pool.back();
+      ev_pool.pop_back();
+      return TECS_OK;
+    }
+    HIP_TRY(hipEventCreate(e));
+    return TECS_OK;
+  }
+  int time_begin(Pending* p, int cls) {
+    if (!timing) return TECS_OK;
+    TRY(ev_get(&p->a));
+    TRY(ev_get(&p->b));
+    p->cls = cls;
+    HIP_TRY(hipEventRecord(p->a, stream));
+    return TECS_OK;
+  }
+  int time_end(Pending* p) {
+    if (!timing) return TECS_OK;
+    HIP_TRY(hipEventRecord(p->b, stream));
+    pending.push_back(*p);
+    if (pending.size() > 4096) TRY(fold_timing());
+    return TECS_OK;
+  }
+  int fold_timing() {
+    if (pending.empty()) return TECS_OK;
+    HIP_TRY(hipEventSynchronize(pending.back().b));
+    for (auto& p : pending) {
+      float ms = 0.0f;
+      HIP_TRY(hipEventElapsedTime(&ms, p.a, p.b));
+      t_ms[p.cls] += ms;
+      t_launches[p.cls]++;
+      ev_pool.push_back(p.a);
+      ev_pool.push_back(p.b);
+    }
+    pending.clear();
+    return TECS_OK;
+  }
+
+  // ---------------------------------------------------------------------- helpers
+  int sync_inst() {
+    if (!inst_dirty) return TECS_OK;
+    HIP_TRY(hipMemcpyAsync(d_inst, h_inst.data(), sizeof(InstanceDesc) * n_inst,
+                           hipMemcpyHostToDevice, stream));
+    inst_dirty = false;
+    return TECS_OK;
+  }
+  void set_dt(int k, double dt) {
+    const float f = (float)dt;  // VectorN * Scalar narrows dt (vector.hpp:263-267)
+    if (h_inst[k].dt != f || std::signbit(h_inst[k].dt) != std::signbit(f)) {
+      h_inst[k].dt = f;
+      inst_dirty = true;
+    }
+  }
+
+  int rebuild() {
+    if (!is_binned(finder)) return TECS_OK;
+    Pending p{};
+    TRY(time_begin(&p, kTimeRebuild));
+    RebuildArgs r{};
+    r.inst = d_inst;
+    r.block_inst = d_block_inst;
+    r.x_old = x[xc];
+    r.pol_old = pol[xc];
+    r.id_old = id[ic];
+    r.ov_old = ov[xc];
+    r.x_new = x[xc ^ 1];
+    r.pol_new = pol[xc ^ 1];
+    r.id_new = id[ic ^ 1];
+    r.ov_new = ov[xc ^ 1];
+    r.keys = keys;
+    r.tmp = tmp;
+    r.tkey = tkey;
+    r.tid = tidb;
+    r.cnt = cnt;
+    r.coff = coff;
+    r.off = off;
+    r.skey = skey;
+    r.scan_bsum = bsum;
+    r.total_cbins = total_cbins;
+    r.fill = fill_bins;
+    r.fill_inst = d_fill_inst;
+    r.fill_base = d_fill_base;
+    r.fill_blocks = (int)h_fill_inst.size();
+    launch_rebuild(r, n_blocks, stream);
+    HIP_TRY(hipGetLastError());
+    TRY(time_end(&p));
+    xc ^= 1;
+    ic ^= 1;
+    grid_built = true;
+    stats.rebuilds++;
+    return TECS_OK;
+  }
+
+  // The reference rebuild schedule for a full evaluation (binned_all_pairs.hpp:77-84).
+  int maybe_rebuild(bool is_full_step) {
+    if (!is_binned(finder)) return TECS_OK;
+    if (is_full_step) {
+      if (step_count == 0 || step_count >= rebuild_every_n) {
+        TRY(rebuild());
+        step_count = 0;
+      }
+      step_count++;
+    }
+    if (!grid_built)
+      return fail(TECS_ERR_STATE, "binned pair finder evaluated before any full step built a grid");
+    return TECS_OK;
+  }
+
+  SweepArgs base_args(const float4* xs, const float2* ps) const {
+    SweepArgs a{};
+    a.inst = d_inst;
+    a.block_inst = d_block_inst;
+    a.xs = xs;
+    a.ps = ps;
+    a.trig = trig;
+    a.off = off;
+    a.id = id[ic];
+    a.block_pairs = block_pairs;
+    return a;
+  }
+
+  // The concrete sweep: TECS_SWEEP_THREAD or TECS_SWEEP_TEAM4/8/16.
+  int concrete_variant() const {
+    if (!team_ok) return TECS_SWEEP_THREAD;
+    switch (variant) {
+      case TECS_SWEEP_THREAD: return TECS_SWEEP_THREAD;
+      case TECS_SWEEP_TEAM: return TECS_SWEEP_TEAM8;
+      case TECS_SWEEP_TEAM4:
+      case TECS_SWEEP_TEAM8:
+      case TECS_SWEEP_TEAM16: return variant;
+      default: return tuned >= 0 ? tuned : TECS_SWEEP_TEAM8;
+    }
+  }
+  static int lanes_of(int v) {
+    return v == TECS_SWEEP_TEAM4 ? 4 : v == TECS_SWEEP_TEAM8 ? 8 : v == TECS_SWEEP_TEAM16 ? 16 : 1;
+  }
+  int partials_per_block() const { return lanes_of(concrete_variant()); }
+
+  void launch_variant(int v, int mode, const SweepArgs& a) {
+    if (v == TECS_SWEEP_THREAD)
+      launch_sweep(finder, has_type, has_pol, mode, a, n_blocks, stream);
+    else
+      launch_sweep_team(lanes_of(v), finder == kBinnedGabriel, has_type, has_pol, mode, a,
+                        n_blocks, stream);
+  }
+
+  // sin/cos of the polarity angles of the stage a sweep is about to read
+  int prepare_trig(const float2* pol_stage) {
+    if (!has_pol) return TECS_OK;
+    launch_polar_trig(pol_stage, trig, total_slots, stream);
+    HIP_TRY(hipGetLastError());
+    return TECS_OK;
+  }
+
+  // Autotuner: time every exact sweep variant on the live state (an evaluation into scratch
+  // buffers with scratch counters, so neither the state nor the statistics change) and keep
+  // the fastest. Runs once, after the first grid exists.
+  int autotune() {
+    if (tuned >= 0) return TECS_OK;
+    if (!team_ok) {
+      tuned = TECS_SWEEP_THREAD;
+      return TECS_OK;
+    }
+    TRY(prepare_trig(pol[xc]));
+    const int cands[4] = {TECS_SWEEP_TEAM4, TECS_SWEEP_TEAM8, TECS_SWEEP_TEAM16, TECS_SWEEP_THREAD};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    double best = 1e30;
+    int best_v = TECS_SWEEP_TEAM8;
+    int rc = TECS_OK;
+    for (int v : cands) {
+      SweepArgs t = base_args(x[xc], pol[xc]);
+      t.o_pos = d1;
+      t.o_pol = d1p;
+      t.block_pairs = tune_pairs;
+      float ms_min = 1e30f;
+      for (int rep = 0; rep < 4 && rc == TECS_OK; ++rep) {
+        hipEventRecord(e0, stream);
+        launch_variant(v, kModeDelta, t);
+        hipEventRecord(e1, stream);
+        hipError_t e = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) rc = fail(TECS_ERR_DEVICE, std::string("autotune: ") + hipGetErrorString(e));
+        if (rep > 0) ms_min = std::min(ms_min, ms);  // first launch warms caches
+      }
+      tune_ms[v] = ms_min;
+      if (ms_min < best) {
+        best = ms_min;
+        best_v = v;
+      }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc != TECS_OK) return rc;
+    tuned = best_v;
+    return TECS_OK;
+  }
+
+  int sweep(int mode, const SweepArgs& a, int cls) {
+    Pending p{};
+    TRY(time_begin(&p, cls));
+    launch_variant(concrete_variant(), mode, a);
+    HIP_TRY(hipGetLastError());
+    TRY(time_end(&p));
+    if (mode != kModeCount && mode != kModeEmit) stats.evaluations++;
+    return TECS_OK;
+  }
+
+  int com_and_update(int kind) {
+    launch_com_finalize(d_inst, n_inst, com_partial, partials_per_block(), com, stream);
+    UpdateArgs u{};
+    u.inst = d_inst;
+    u.block_inst = d_block_inst;
+    u.com = com;
+    u.x = x[xc];
+    u.pol = pol[xc];
+    u.d1_pos = d1;
+    u.d1_pol = d1p;
+    u.d3_pos = d3;
+    u.d3_pol = d3p;
+    u.xp = xp;
+    u.polp = polp;
+    u.old_vel = track_ov ? ov[xc] : nullptr;
+    launch_update(kind, has_pol, u, n_blocks, stream);
+    HIP_TRY(hipGetLastError());
+    return TECS_OK;
+  }
+
+  // One integrator step (Euler::integrate euler.hpp:81-85, Heun::integrate heun.hpp:201-208).
+  int step_once() {
+    TRY(maybe_rebuild(true));
+    const bool com_on = com_fixer == TECS_GLOBAL_COM_FIXER;
+    if (integrator == TECS_HEUN) {
+      if (!com_on) {
+        TRY(prepare_trig(pol[xc]));
+        SweepArgs a = base_args(x[xc], pol[xc]);
+        a.o_pos = d1;
+        a.o_pol = d1p;
+        a.o2_pos = xp;
+        a.o2_pol = polp;
+        TRY(sweep(kModePred, a, kTimeSweepFull));
+        TRY(prepare_trig(polp));
+        SweepArgs b = base_args(xp, polp);
+        b.o2_pos = x[xc];
+        b.o2_pol = pol[xc];
+        b.d1_pos = d1;
+        b.d1_pol = d1p;
+        b.old_vel = track_ov ? ov[xc] : nullptr;
+        TRY(sweep(kModeCorr, b, kTimeSweepStale));
+      } else {
+        TRY(prepare_trig(pol[xc]));
+        SweepArgs a = base_args(x[xc], pol[xc]);
+        a.o_pos = d1;
+        a.o_pol = d1p;
+        a.com_partial = com_partial;
+        TRY(sweep(kModeDelta, a, kTimeSweepFull));
+        TRY(com_and_update(kUpdatePred));
+        TRY(prepare_trig(polp));
+        SweepArgs b = base_args(xp, polp);
+        b.o_pos = d3;
+        b.o_pol = d3p;
+        b.com_partial = com_partial;
+        TRY(sweep(kModeDelta, b, kTimeSweepStale));
+        TRY(com_and_update(kUpdateCorr));
+      }
+    } else {
+      TRY(prepare_trig(pol[xc]));
+      if (!com_on) {
+        SweepArgs a = base_args(x[xc], pol[xc]);
+        a.o_pos = x[xc ^ 1];
+        a.o_pol = pol[xc ^ 1];
+        a.old_vel = track_ov ? ov[xc ^ 1] : nullptr;
+        TRY(sweep(kModeEuler, a, kTimeSweepFull));
+        xc ^= 1;
+      } else {
+        SweepArgs a = base_args(x[xc], pol[xc]);
+        a.o_pos = d1;
+        a.o_pol = d1p;
+        a.com_partial = com_partial;
+        TRY(sweep(kModeDelta, a, kTimeSweepFull));
+        TRY(com_and_update(kUpdateEuler));
+      }
+    }
+    stats.steps++;
+    return TECS_OK;
+  }
+
+  // Capture graph_period() steps from the current state (the host state advances as if they
+  // ran; the caller launches the graph once to make the device catch up).
+  int capture_graph(StepGraph* out) {
+    const HostState start = host_state();
+    const tecs_stats before = stats;
+    HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    int rc = TECS_OK;
+    const int P = graph_period();
+    for (int k = 0; k < P && rc == TECS_OK; ++k) rc = step_once();
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(stream, &g);
+    if (rc != TECS_OK) {
+      if (g) hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(TECS_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (ei != hipSuccess) return fail(TECS_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    out->start = start;
+    out->exec = exec;
+    out->delta.pair_interactions = 0;
+    out->delta.evaluations = stats.evaluations - before.evaluations;
+    out->delta.rebuilds = stats.rebuilds - before.rebuilds;
+    out->delta.steps = stats.steps - before.steps;
+    if (!(host_state() == start)) {  // not periodic from here: run it once, do not keep it
+      HIP_TRY(hipGraphLaunch(exec, stream));
+      hipGraphExecDestroy(exec);
+      out->exec = nullptr;
+    }
+    return TECS_OK;
+  }
+
+  int take_step(const double* dts, int n_steps) {
+    if (n_steps < 0) return fail(TECS_ERR_INVALID, "n_steps must be >= 0");
+    for (int k = 0; k < n_inst; ++k) set_dt(k, dts[k]);
+    TRY(sync_inst());
+    int done = 0;
+    if (variant == TECS_SWEEP_AUTO && tuned < 0 && n_steps > 0) {
+      if (!grid_built && is_binned(finder)) {
+        TRY(step_once());
+        ++done;
+      }
+      TRY(autotune());
+    }
+    const int P = graph_period();
+    const bool graphs_on = graph_mode == TECS_GRAPHS_ON ||
+                           (graph_mode == TECS_GRAPHS_AUTO && n_steps >= 4 * P);
+    if (graphs_on && !timing) {
+      // the first step after creation or a manual rebuild may start a new rebuild phase
+      while (done < n_steps && (step_count == 0 && is_binned(finder))) {
+        TRY(step_once());
+        ++done;
+      }
+      while (n_steps - done >= P) {
+        const HostState st = host_state();
+        StepGraph* g = nullptr;
+        for (auto& c : graphs)
+          if (c.start == st) g = &c;
+        if (!g) {
+          StepGraph fresh{};
+          TRY(capture_graph(&fresh));
+          done += P;
+          if (!fresh.exec) continue;  // ran once eagerly through the graph
+          graphs.push_back(fresh);
+          HIP_TRY(hipGraphLaunch(graphs.back().exec, stream));
+          ++graph_launches;
+          continue;
+        }
+        HIP_TRY(hipGraphLaunch(g->exec, stream));
+        ++graph_launches;
+        stats.evaluations += g->delta.evaluations;
+        stats.rebuilds += g->delta.rebuilds;
+        stats.steps += g->delta.steps;
+        done += P;
+      }
+    }
+    for (; done < n_steps; ++done) TRY(step_once());
+    return TECS_OK;
+  }
+
+  int check_index(int k) const {
+    if (k < 0 || k >= n_inst) return fail(TECS_ERR_INVALID, "simulation index out of range");
+    return TECS_OK;
+  }
+
+  int upload(int k, const float* pos, const float* p, const uint8_t* type) {
+    TRY(check_index(k));
+    if (!pos) return fail(TECS_ERR_INVALID, "positions are required");
+    const tecs_sim_desc& d = descs[k];
+    const size_t n = d.agent_count;
+    if (has_type && type && d.force.n_types > 0)
+      for (size_t i = 0; i < n; ++i)
+        if (type[i] >= (unsigned)d.force.n_types)
+          return fail(TECS_ERR_INVALID, "cell type outside the adhesion table");
+    HIP_TRY(hipMemcpyAsync(st_pos, pos, sizeof(float) * 3 * n, hipMemcpyHostToDevice, stream));
+    const bool up_pol = has_pol && p;
+    const bool up_type = has_type && type;
+    if (up_pol)
+      HIP_TRY(hipMemcpyAsync(st_pol, p, sizeof(float) * 2 * n, hipMemcpyHostToDevice, stream));
+    if (up_type) HIP_TRY(hipMemcpyAsync(st_type, type, n, hipMemcpyHostToDevice, stream));
+    const InstanceDesc& in = h_inst[k];
+    launch_upload(d_inst, k, in.block_begin, in.n_blocks, id[ic], st_pos, up_pol ? st_pol : nullptr,
+                  up_type ? st_type : nullptr, x[xc], pol[xc], stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(stream));
+    return TECS_OK;
+  }
+
+  int download(int k, float* pos, float* p, float* old_vel) {
+    TRY(check_index(k));
+    const size_t n = descs[k].agent_count;
+    const InstanceDesc& in = h_inst[k];
+    const bool want_pol = p && has_pol;
+    const bool want_ov = old_vel != nullptr;
+    if (want_ov && !track_ov) {
+      std::memset(old_vel, 0, sizeof(float) * 3 * n);
+    }
+    launch_download(d_inst, k, in.block_begin, in.n_blocks, id[ic], x[xc], pol[xc], ov[xc],
+                    pos ? st_pos : nullptr, want_pol ? st_pol : nullptr,
+                    (want_ov && track_ov) ? st_ov : nullptr, stream);
+    HIP_TRY(hipGetLastError());
+    if (pos) HIP_TRY(hipMemcpyAsync(pos, st_pos, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, stream));
+    if (want_pol)
+      HIP_TRY(hipMemcpyAsync(p, st_pol, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, stream));
+    if (want_ov && track_ov)
+      HIP_TRY(hipMemcpyAsync(old_vel, st_ov, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return TECS_OK;
+  }
+
+  // One evaluation into the d1 scratch buffers, then out in agent order.
+  int evaluate(bool is_full_step, float* dpos, float* dpol) {
+    TRY(maybe_rebuild(is_full_step));
+    TRY(prepare_trig(pol[xc]));
+    SweepArgs a = base_args(x[xc], pol[xc]);
+    a.o_pos = d1;
+    a.o_pol = d1p;
+    TRY(sweep(kModeDelta, a, is_full_step ? kTimeSweepFull : kTimeSweepStale));
+    const InstanceDesc& in = h_inst[0];
+    launch_download(d_inst, 0, in.block_begin, in.n_blocks, id[ic], d1, d1p, nullptr,
+                    dpos ? st_pos : nullptr, (dpol && has_pol) ? st_pol : nullptr, nullptr, stream);
+    HIP_TRY(hipGetLastError());
+    const size_t n = descs[0].agent_count;
+    if (dpos) HIP_TRY(hipMemcpyAsync(dpos, st_pos, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, stream));
+    if (dpol && has_pol)
+      HIP_TRY(hipMemcpyAsync(dpol, st_pol, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return TECS_OK;
+  }
+
+  int pairs(bool is_full_step, int64_t* row_ptr, int32_t* cols, int64_t cols_capacity,
+            int64_t* n_pairs) {
+    const size_t n = descs[0].agent_count;
+    if (!cols) {
+      // size query: build the grid a due full evaluation would see, advance nothing
+      if (is_binned(finder) && is_full_step && (step_count == 0 || step_count >= rebuild_every_n)) {
+        TRY(rebuild());
+        stats.rebuilds--;
+      }
+      if (is_binned(finder) && !grid_built)
+        return fail(TECS_ERR_STATE, "binned pair finder evaluated before any full step built a grid");
+    } else {
+      TRY(maybe_rebuild(is_full_step));
+    }
+    SweepArgs a = base_args(x[xc], pol[xc]);
+    a.row_count = d_row_count;
+    TRY(sweep(kModeCount, a, kTimeSweepFull));
+    std::vector<int> counts(n);
+    if (n) HIP_TRY(hipMemcpyAsync(counts.data(), d_row_count, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    std::vector<int64_t> rp(n + 1);
+    rp[0] = 0;
+    for (size_t i = 0; i < n; ++i) rp[i + 1] = rp[i] + counts[i];
+    const int64_t total = rp[n];
+    if (n_pairs) *n_pairs = total;
+    if (row_ptr) std::memcpy(row_ptr, rp.data(), sizeof(int64_t) * (n + 1));
+    if (!cols) return TECS_OK;
+    if (cols_capacity < total) return fail(TECS_ERR_CAPACITY, "cols buffer smaller than n_pairs");
+    if (total > cols_cap) {
+      void* v = nullptr;
+      HIP_TRY(hipMalloc(&v, sizeof(int32_t) * (size_t)std::max<int64_t>(total, 1)));
+      if (d_cols) {
+        hipFree(d_cols);
+        for (auto& q : allocations)
+          if (q == d_cols) q = v;
+      } else {
+        allocations.push_back(v);
+      }
+      d_cols = static_cast<int32_t*>(v);
+      cols_cap = total;
+    }
+    HIP_TRY(hipMemcpyAsync(d_row_ptr, rp.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, stream));
+    SweepArgs b = base_args(x[xc], pol[xc]);
+    b.row_ptr = d_row_ptr;
+    b.cols = d_cols;
+    TRY(sweep(kModeEmit, b, kTimeSweepFull));
+    if (total) HIP_TRY(hipMemcpyAsync(cols, d_cols, sizeof(int32_t) * total, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return TECS_OK;
+  }
+
+  // Simulation::write(time) (simulation.hpp:607-611) for member k, off the critical path.
+  int write_snapshot(int k, Writer* w, double time) {
+    TRY(check_index(k));
+    const InstanceDesc& in = h_inst[k];
+    const size_t n = descs[k].agent_count;
+    const size_t cap = (size_t)std::max(max_n, 1);
+    if (!side) HIP_TRY(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    SnapshotSlot* sl = nullptr;
+    {
+      std::unique_lock<std::mutex> g(snap_mu);
+      snap_cv.wait(g, [&] {
+        for (auto& c : snap)
+          if (!c.busy) return true;
+        return false;
+      });
+      for (auto& c : snap)
+        if (!c.busy) {
+          sl = &c;
+          break;
+        }
+      sl->busy = true;
+    }
+    if (!sl->h_pos) {
+      sl->mu = &snap_mu;
+      sl->cv = &snap_cv;
+      int rc = alloc(&sl->d_pos, 3 * cap);
+      if (rc == TECS_OK && has_pol) rc = alloc(&sl->d_pol, 2 * cap);
+      hipError_t e = hipSuccess;
+      if (rc == TECS_OK) e = hipHostMalloc((void**)&sl->h_pos, sizeof(float) * 3 * cap, hipHostMallocDefault);
+      if (rc == TECS_OK && e == hipSuccess && has_pol)
+        e = hipHostMalloc((void**)&sl->h_pol, sizeof(float) * 2 * cap, hipHostMallocDefault);
+      if (rc == TECS_OK && e == hipSuccess) e = hipEventCreateWithFlags(&sl->gathered, hipEventDisableTiming);
+      if (rc == TECS_OK && e == hipSuccess) e = hipEventCreateWithFlags(&sl->copied, hipEventDisableTiming);
+      if (rc != TECS_OK || e != hipSuccess) {
+        sl->release();
+        return rc != TECS_OK ? rc : fail(TECS_ERR_OOM, std::string("snapshot buffers: ") + hipGetErrorString(e));
+      }
+    }
+    sl->n = n;
+    sl->has_pol = has_pol;
+    sl->time = time;
+    sl->step = write_step_count[k]++;
+    launch_download(d_inst, k, in.block_begin, in.n_blocks, id[ic], x[xc], pol[xc], nullptr,
+                    sl->d_pos, has_pol ? sl->d_pol : nullptr, nullptr, stream);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(sl->gathered, stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(side, sl->gathered, 0);
+    if (e == hipSuccess && n)
+      e = hipMemcpyAsync(sl->h_pos, sl->d_pos, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, side);
+    if (e == hipSuccess && n && has_pol)
+      e = hipMemcpyAsync(sl->h_pol, sl->d_pol, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, side);
+    if (e == hipSuccess) e = hipEventRecord(sl->copied, side);
+    if (e != hipSuccess) {
+      sl->release();
+      return fail(TECS_ERR_DEVICE, std::string("snapshot hand-off: ") + hipGetErrorString(e));
+    }
+    w->enqueue(sl);
+    return TECS_OK;
+  }
+
+  // Move member ks of src (its current cell state: positions, types, polarities, ids, old
+  // velocities) into member kd of this batch, which must not have stepped yet. The grid is not
+  // carried: this batch rebuilds it at its first full step, which every member's schedule does
+  // anyway when rebuild_every_n is 0 (the server only groups such requests).
+  int copy_member_from(Batch& src, int ks, int kd) {
+    TRY(check_index(kd));
+    TRY(src.check_index(ks));
+    const InstanceDesc& a = src.h_inst[ks];
+    const InstanceDesc& b = h_inst[kd];
+    if (a.n != b.n) return fail(TECS_ERR_INVALID, "member sizes differ");
+    HIP_TRY(hipStreamSynchronize(src.stream));
+    const size_t n = (size_t)a.n;
+    HIP_TRY(hipMemcpyAsync(x[xc] + b.slot_begin, src.x[src.xc] + a.slot_begin, sizeof(float4) * n,
+                           hipMemcpyDeviceToDevice, stream));
+    HIP_TRY(hipMemcpyAsync(id[ic] + b.slot_begin, src.id[src.ic] + a.slot_begin, sizeof(int) * n,
+                           hipMemcpyDeviceToDevice, stream));
+    if (has_pol)
+      HIP_TRY(hipMemcpyAsync(pol[xc] + b.slot_begin, src.pol[src.xc] + a.slot_begin,
+                             sizeof(float2) * n, hipMemcpyDeviceToDevice, stream));
+    if (track_ov && src.track_ov)
+      HIP_TRY(hipMemcpyAsync(ov[xc] + b.slot_begin, src.ov[src.xc] + a.slot_begin,
+                             sizeof(float4) * n, hipMemcpyDeviceToDevice, stream));
+    write_step_count[kd] = src.write_step_count[ks];
+    return TECS_OK;
+  }
+
+  // Device bytes a member of this description adds to a batch (state, scratch and grid).
+  static size_t estimate_bytes(const tecs_sim_desc& d) {
+    const size_t slots = ((size_t)d.agent_count + kBlock - 1) / kBlock * kBlock + kBlock;
+    const bool pol = (d.fields & TECS_FIELD_POLARITY) != 0;
+    size_t per_slot = 2 * 16 + 2 * 4 + 16 + 16 * 2 /* x, id, d1, ov */ + 5 * 4 /* keys.. */;
+    if (pol) per_slot += 2 * 8 + 8 + 16 /* pol, d1 pol, trig */;
+    if (d.integrator == TECS_HEUN) per_slot += 16 + (pol ? 8 : 0);
+    if (d.integrator == TECS_HEUN && d.com_fixer == TECS_GLOBAL_COM_FIXER) per_slot += 16 + (pol ? 8 : 0);
+    size_t bins = 0;
+    if (is_binned(d.pair_finder)) {
+      const tecs_pair_finder_settings& s = d.pair_finder_settings;
+      const float bin = s.bin_size_scale * d.cutoff_distance;
+      double b = 1.0;
+      for (int k = 0; k < 3; ++k) b *= std::ceil((double)((s.max_bounds[k] - s.min_bounds[k]) / bin));
+      bins = (size_t)b + 64;
+    }
+    return slots * per_slot + bins * 12 + 2 * 4096;
+  }
+
+  int get_stats(tecs_stats* out) {
+    std::vector<unsigned long long> bp((size_t)n_blocks * kMaxTeamLanes);
+    HIP_TRY(hipMemcpyAsync(bp.data(), block_pairs, sizeof(unsigned long long) * bp.size(),
+                           hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    tecs_stats s = stats;
+    s.pair_interactions = 0;
+    for (auto v : bp) s.pair_interactions += v;
+    *out = s;
+    return TECS_OK;
+  }
+
+  int reset_stats() {
+    HIP_TRY(hipMemsetAsync(block_pairs, 0, sizeof(unsigned long long) * n_blocks * kMaxTeamLanes, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    stats = tecs_stats{};
+    return TECS_OK;
+  }
+};
+
+
+}  // namespace tecs_amd

@@ -157,3 +157,32 @@ def sweep(count: int = 256, n_min: int = 1_000, n_max: int = 100_000, seed_strea
         )
         out.append(Workload(f"sweep[{k}]", d, pos, None, types, 0.05, 500))
     return out
+
+
+@dataclass
+class ServeRequest:
+    arrival: float  # seconds after the start
+    model: str
+    workload: Workload
+    n_steps: int
+
+
+def serve_requests(count: int = 64, rate: float = 8.0, n_min: int = 2_000, n_max: int = 500_000,
+                   steps_min: int = 100, steps_max: int = 5_000, seed_stream: int = 4):
+    """Served requests: Poisson arrivals at `rate` per second, the model drawn uniformly from
+    relaxation / epithelium / sorting, cell counts log-uniform in [n_min, n_max] and step counts
+    uniform in [steps_min, steps_max]."""
+    g = rng(seed_stream)
+    gaps = g.exponential(1.0 / rate, count)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    models = g.integers(0, 3, count)
+    sizes = np.exp(g.uniform(math.log(n_min), math.log(n_max), count)).astype(np.int64)
+    steps = g.integers(steps_min, steps_max + 1, count)
+    out = []
+    for k in range(count):
+        n = int(sizes[k])
+        name = ("relaxation", "epithelium", "sorting")[int(models[k])]
+        stream = 10_000 + seed_stream * 1000 + k
+        w = {"relaxation": relaxation, "epithelium": epithelium, "sorting": sorting}[name](n, seed_stream=stream)
+        out.append(ServeRequest(float(arrivals[k]), name, w, int(steps[k])))
+    return out

@@ -378,3 +378,54 @@ def test_snapshots_async_hand_off(gpu, tmp_path):
     for k, (x, p) in enumerate(expected):
         assert np.array_equal(io.read_dataset(h5, f"t{k}/position"), x), k
         assert np.array_equal(io.read_dataset(h5, f"t{k}/polarity"), p), k
+
+
+def test_server_results_equal_alone(gpu):
+    """Continuous batching: requests of three models arrive over several rounds, a tight HBM
+    budget makes some wait, members join and leave batches; every result is bitwise the
+    result of the same simulation run alone."""
+    from tecs_amd.serving import Server
+
+    reqs = [
+        (W.relaxation(3000, seed_stream=50), 37, 0),
+        (W.epithelium(4000, seed_stream=51), 23, 0),
+        (W.sorting(9000, seed_stream=52), 41, 1),
+        (W.relaxation(1500, seed_stream=53), 12, 2),
+        (W.epithelium(2500, seed_stream=54), 30, 3),
+        (W.sorting(5000, seed_stream=55), 9, 3),
+    ]
+    # a budget that holds only about half of the requests at once
+    sizes = []
+    for w, _, _ in reqs:
+        srv_probe = Server(1 << 40)
+        rid = srv_probe.submit(w.desc, w.positions, w.polarities, w.types, w.dt, 1)
+        sizes.append(srv_probe.info(rid)["hbm_bytes"])
+        srv_probe.close()
+    budget = sum(sorted(sizes)[-3:])
+    srv = Server(budget, max_chunk_steps=4)
+    ids = {}
+    rounds = 0
+    while True:
+        for k, (w, steps, arrive) in enumerate(reqs):
+            if arrive == rounds and k not in ids:
+                ids[k] = srv.submit(w.desc, w.positions, w.polarities, w.types, w.dt, steps)
+        running, pending = srv.advance()
+        rounds += 1
+        if len(ids) == len(reqs) and all(srv.info(i)["state"] == 2 for i in ids.values()):
+            break
+        assert rounds < 500
+    st = srv.stats()
+    print("server stats", st, "rounds", rounds, flush=True)
+    assert st["completed"] == len(reqs) and st["regroups"] >= 4
+    assert st["hbm_bytes_in_use"] == 0
+    for k, (w, steps, _) in enumerate(reqs):
+        info = srv.info(ids[k])
+        assert info["t_submit"] <= info["t_admit"] <= info["t_done"]
+        xs, ps = srv.result(ids[k])
+        s = GpuSimulation(w.desc)
+        s.upload(w.positions, w.polarities, w.types)
+        s.take_step(w.dt, steps)
+        xa, pa, _ = s.download()
+        assert np.array_equal(xs.view(np.uint32), xa.view(np.uint32)), f"request {k}"
+        if pa is not None:
+            assert np.array_equal(ps.view(np.uint32), pa.view(np.uint32)), f"request {k}"

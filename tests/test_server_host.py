@@ -1,0 +1,37 @@
+"""Serving-loop admission checks that need no GPU (validation happens at submit)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from tecs_amd import TecsError, abi, workloads as W
+from tecs_amd.serving import Server
+
+
+def test_submit_rejects_stale_grid_requests():
+    w = W.relaxation(500)
+    d = abi.copy_desc(w.desc)
+    d.pair_finder_settings.rebuild_every_n = 3
+    s = Server(1 << 30)
+    with pytest.raises(TecsError) as e:
+        s.submit(d, w.positions, dt=w.dt, n_steps=10)
+    assert e.value.code == abi.ERR_UNSUPPORTED
+
+
+def test_submit_rejects_requests_beyond_the_budget():
+    w = W.relaxation(5000)
+    s = Server(1 << 16)  # 64 KiB
+    with pytest.raises(TecsError) as e:
+        s.submit(w.desc, w.positions, dt=w.dt, n_steps=10)
+    assert e.value.code == abi.ERR_CAPACITY
+
+
+def test_submitted_request_waits_pending():
+    w = W.relaxation(500)
+    s = Server(1 << 30)
+    rid = s.submit(w.desc, w.positions, dt=w.dt, n_steps=10)
+    info = s.info(rid)
+    assert info["state"] == abi.REQUEST_PENDING and info["steps_done"] == 0
+    assert info["hbm_bytes"] > 500 * 16
+    with pytest.raises(TecsError):
+        s.result(rid)
