@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define TECS_AMD_ABI_VERSION 1
+#define TECS_AMD_ABI_VERSION 2
 
 /* ---- status codes ---------------------------------------------------------------------- */
 enum {
@@ -297,16 +297,20 @@ typedef struct tecs_server_stats {
   uint64_t member_steps;       /* sum over requests of the steps run */
   uint64_t regroups;           /* batches re-formed */
   uint64_t hbm_bytes_in_use, hbm_budget;
+  uint64_t pool_cached_bytes;  /* freed batch buffers the server keeps for re-use */
+  double regroup_seconds;      /* host time spent re-forming batches */
 } tecs_server_stats;
-/* max_chunk_steps: steps per scheduling round at most (admission and retirement happen
- * between rounds). */
+/* max_chunk_steps: steps a group advances at most between two services (admission,
+ * retirement and re-forming happen between chunks). */
 int tecs_server_create(uint64_t hbm_budget_bytes, int32_t max_chunk_steps, tecs_server** out);
 int tecs_server_destroy(tecs_server* server);
 /* Queue a request (inputs are copied). TECS_ERR_CAPACITY if it could never fit the budget. */
 int tecs_server_submit(tecs_server* server, const tecs_sim_desc* desc, const float* positions,
                        const float* polarities, const uint8_t* types, double dt,
                        int32_t n_steps, int64_t* request_id);
-/* One scheduling round: admit, re-form batches, step every batch, retire finished requests. */
+/* Admit what fits the budget, then service every group whose chunk has ended (retire finished
+ * members, re-form the batch if members left or joined, enqueue the next chunk). Returns once
+ * at least one group was serviced, or at once when nothing is running. */
 int tecs_server_advance(tecs_server* server, int32_t* running, int32_t* pending);
 int tecs_server_request_info(tecs_server* server, int64_t request_id, tecs_request_info* info);
 /* The final state of a finished request (positions n*3, polarities n*2 or NULL). */

@@ -6,7 +6,7 @@ from __future__ import annotations
 
 import ctypes as C
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 # status codes
 OK = 0
@@ -194,4 +194,6 @@ class ServerStats(C.Structure):
         ("regroups", C.c_uint64),
         ("hbm_bytes_in_use", C.c_uint64),
         ("hbm_budget", C.c_uint64),
+        ("pool_cached_bytes", C.c_uint64),
+        ("regroup_seconds", C.c_double),
     ]

@@ -21,8 +21,10 @@
 #include <cmath>
 #include <condition_variable>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace tecs_amd {
@@ -87,6 +89,64 @@ inline int validate_desc(const tecs_sim_desc* d) {
   return TECS_OK;
 }
 
+// Device memory kept for reuse by batches that come and go: the server re-forms a group's
+// batch whenever its membership changes, and the new batch asks for buffers of about the
+// sizes the old one returns. Blocks are handed out by size class (multiples of 1/8 of the
+// request's power of two, at least 64 KiB, so at most 12.5% is wasted) and returned on
+// release; hipFree, which synchronises the whole device, runs only when the cache exceeds its
+// limit or the pool is destroyed. Callers return a block only after the stream that used it
+// is idle.
+class DevicePool {
+ public:
+  explicit DevicePool(size_t max_cached_bytes) : limit_(max_cached_bytes) {}
+  ~DevicePool() { trim(0); }
+  DevicePool(const DevicePool&) = delete;
+  DevicePool& operator=(const DevicePool&) = delete;
+
+  static size_t size_class(size_t bytes) {
+    size_t top = 1;
+    while (top <= bytes / 2) top <<= 1;  // largest power of two <= bytes
+    const size_t g = std::max<size_t>(65536, top / 8);
+    return (std::max<size_t>(bytes, 1) + g - 1) / g * g;
+  }
+  hipError_t get(size_t bytes, void** p) {
+    const size_t c = size_class(bytes);
+    auto it = free_.find(c);
+    if (it != free_.end()) {
+      *p = it->second;
+      cached_ -= c;
+      free_.erase(it);
+      return hipSuccess;
+    }
+    hipError_t e = hipMalloc(p, c);
+    if (e == hipErrorOutOfMemory && cached_ > 0) {
+      (void)hipGetLastError();
+      trim(0);
+      e = hipMalloc(p, c);
+    }
+    return e;
+  }
+  void put(void* p, size_t cls) {
+    free_.emplace(cls, p);
+    cached_ += cls;
+    if (cached_ > limit_) trim(limit_);
+  }
+  void trim(size_t keep) {
+    while (cached_ > keep && !free_.empty()) {
+      auto it = std::prev(free_.end());  // largest first
+      hipFree(it->second);
+      cached_ -= it->first;
+      free_.erase(it);
+    }
+  }
+  size_t cached_bytes() const { return cached_; }
+
+ private:
+  std::multimap<size_t, void*> free_;
+  size_t cached_ = 0;
+  size_t limit_;
+};
+
 struct Batch {
   // homogeneous configuration
   int finder = 0, integrator = 0, com_fixer = 0, rebuild_every_n = 0;
@@ -137,7 +197,8 @@ struct Batch {
   int64_t* d_row_ptr = nullptr;
   int32_t* d_cols = nullptr;
   int64_t cols_cap = 0;
-  std::vector<void*> allocations;
+  DevicePool* pool = nullptr;                          // optional source of device memory
+  std::vector<std::pair<void*, size_t>> allocations;  // (block, size class; 0 = hipMalloc)
 
   int variant = TECS_SWEEP_AUTO;  // requested sweep variant
   int tuned = -1;                 // concrete variant picked by the autotuner (AUTO)
@@ -225,7 +286,12 @@ struct Batch {
     pending.clear();
     for (hipEvent_t e : ev_pool) hipEventDestroy(e);
     ev_pool.clear();
-    for (void* p : allocations) hipFree(p);
+    for (auto& a : allocations) {
+      if (pool && a.second)
+        pool->put(a.first, a.second);
+      else
+        hipFree(a.first);
+    }
     allocations.clear();
     if (own_stream && stream) hipStreamDestroy(stream);
     stream = nullptr;
@@ -236,8 +302,13 @@ struct Batch {
     *p = nullptr;
     if (count == 0) count = 1;
     void* v = nullptr;
-    HIP_TRY(hipMalloc(&v, count * sizeof(T)));
-    allocations.push_back(v);
+    if (pool) {
+      HIP_TRY(pool->get(count * sizeof(T), &v));
+      allocations.emplace_back(v, DevicePool::size_class(count * sizeof(T)));
+    } else {
+      HIP_TRY(hipMalloc(&v, count * sizeof(T)));
+      allocations.emplace_back(v, 0);
+    }
     *p = static_cast<T*>(v);
     return TECS_OK;
   }
@@ -918,13 +989,17 @@ struct Batch {
     if (total > cols_cap) {
       void* v = nullptr;
       HIP_TRY(hipMalloc(&v, sizeof(int32_t) * (size_t)std::max<int64_t>(total, 1)));
-      if (d_cols) {
-        hipFree(d_cols);
-        for (auto& q : allocations)
-          if (q == d_cols) q = v;
-      } else {
-        allocations.push_back(v);
-      }
+      bool replaced = false;
+      for (auto& q : allocations)
+        if (q.first == d_cols && d_cols) {
+          if (pool && q.second)
+            pool->put(q.first, q.second);
+          else
+            hipFree(q.first);
+          q = {v, 0};
+          replaced = true;
+        }
+      if (!replaced) allocations.emplace_back(v, 0);
       d_cols = static_cast<int32_t*>(v);
       cols_cap = total;
     }

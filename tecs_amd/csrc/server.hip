@@ -1,26 +1,30 @@
 // server.hip - a continuous-batching serving loop for simulation requests.
 //
 // Requests (a simulation description, its initial state, dt and a number of steps) arrive at
-// any time. Each scheduling round (tecs_server_advance):
+// any time. Requests that share what a batch must share (pair finder, integrator, COM fixer,
+// fields, old-velocity tracking) form a group; each group owns a stream and one batch holding
+// its running members. tecs_server_advance:
 //   1. admission: pending requests are admitted in arrival order while their estimated device
 //      bytes fit the HBM budget (a request that does not fit waits; later, smaller requests may
-//      pass it);
-//   2. regrouping: requests are grouped by what a batch must share (pair finder, integrator, COM
-//      fixer, fields, old-velocity tracking); a group whose membership changed gets a new batch,
-//      the running members' state is copied into it device to device and new members are
-//      uploaded;
-//   3. stepping: every group advances by the same number of steps, the smallest remaining step
-//      count of its members capped at max_chunk; the groups run concurrently on their own
-//      streams;
-//   4. retirement: finished members are downloaded into their request and leave their group;
-//      their bytes return to the budget.
-// The state of a running request stays on the device between rounds. Every request rebuilds its
-// grid at every full step (rebuild_every_n = 0 is required), so regrouping changes nothing in
-// its arithmetic: a request's result is bitwise the result of the same simulation run alone.
-// This serving layer has no counterpart in the reference.
+//      pass it) and join their group's waiting list;
+//   2. every group whose stream is idle is serviced: the chunk that just ended is accounted,
+//      finished members are downloaded and leave (their bytes return to the budget), the batch
+//      is re-formed once if members left or joined (running members' state is copied device
+//      to device, new members are uploaded), and the next chunk is enqueued: the smallest
+//      remaining step count of the members, at most max_chunk steps;
+//   3. if no group was idle, the call waits until one is.
+// Groups therefore advance independently: a group of small simulations does not wait for the
+// chunk of a group of large ones, and their kernels overlap on the device. The state of a
+// running request stays on the device between chunks, and batch buffers come from a device
+// pool, so a re-form costs no hipMalloc/hipFree in the steady state. The autotuner's choice is
+// kept per group and re-used while the group's cell count stays within a factor of two.
+// Every request rebuilds its grid at every full step (rebuild_every_n = 0 is required), so
+// regrouping changes nothing in its arithmetic: a request's result is bitwise the result of the
+// same simulation run alone. This serving layer has no counterpart in the reference.
 #include "server.hpp"
 
 #include <chrono>
+#include <thread>
 
 namespace tecs_amd {
 
@@ -37,10 +41,16 @@ bool same_group(const tecs_sim_desc& a, const tecs_sim_desc& b) {
 }  // namespace
 
 Server::Server(uint64_t budget_bytes, int max_chunk)
-    : budget(budget_bytes), chunk(std::max(1, max_chunk)), t0(std::chrono::steady_clock::now()) {}
+    : budget(budget_bytes),
+      chunk(std::max(1, max_chunk)),
+      pool(budget_bytes / 4),
+      t0(std::chrono::steady_clock::now()) {}
 
 Server::~Server() {
-  for (auto& g : groups) delete g.batch;
+  for (auto& g : groups) {
+    delete g.batch;  // synchronises the group's stream first
+    if (g.stream) hipStreamDestroy(g.stream);
+  }
 }
 
 int Server::submit(const tecs_sim_desc* d, const float* pos, const float* pol, const uint8_t* types,
@@ -69,15 +79,23 @@ int Server::submit(const tecs_sim_desc* d, const float* pos, const float* pol, c
   return TECS_OK;
 }
 
+// Replace the group's batch by one holding `members` (running ones carry their device state).
+// Called only while the group's stream is idle.
 int Server::regroup(Group& g, std::vector<Request*> members) {
-  // members: the group's requests for the next batch, running ones first (they own state)
+  const auto t_start = std::chrono::steady_clock::now();
   std::unique_ptr<Batch> nb;
   if (!members.empty()) {
     std::vector<tecs_sim_desc> ds;
-    for (Request* r : members) ds.push_back(r->desc);
+    int64_t cells = 0;
+    for (Request* r : members) {
+      ds.push_back(r->desc);
+      cells += (int64_t)r->desc.agent_count;
+    }
     nb = std::make_unique<Batch>();
-    int rc = nb->init(ds.data(), (int)ds.size(), nullptr);
+    nb->pool = &pool;
+    int rc = nb->init(ds.data(), (int)ds.size(), g.stream);
     if (rc) return rc;
+    if (g.tuned >= 0 && cells <= 2 * g.tuned_cells && 2 * cells >= g.tuned_cells) nb->tuned = g.tuned;
     for (size_t k = 0; k < members.size(); ++k) {
       Request* r = members[k];
       if (r->slot >= 0) {
@@ -99,91 +117,112 @@ int Server::regroup(Group& g, std::vector<Request*> members) {
   }
   g.batch = nb.release();
   g.members = std::move(members);
+  g.retired = false;
+  regroup_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+  return TECS_OK;
+}
+
+// The chunk in flight on the group's (now idle) stream has ended: account it, retire members.
+int Server::finish_chunk(Group& g) {
+  g.busy = false;
+  if (g.batch && g.batch->tuned >= 0 && g.tuned != g.batch->tuned) {
+    g.tuned = g.batch->tuned;
+    g.tuned_cells = 0;
+    for (Request* r : g.members) g.tuned_cells += (int64_t)r->desc.agent_count;
+  }
+  for (Request* r : g.members) {
+    if (r->done) continue;
+    r->steps_done += g.chunk_steps;
+    steps_run += (uint64_t)g.chunk_steps;
+    if (r->steps_done < r->n_steps) continue;
+    const size_t n = r->desc.agent_count;
+    r->pos.assign(3 * n, 0.0f);
+    if (r->desc.fields & TECS_FIELD_POLARITY) r->pol.assign(2 * n, 0.0f);
+    int rc = g.batch->download(r->slot, r->pos.data(), r->pol.empty() ? nullptr : r->pol.data(), nullptr);
+    if (rc) return rc;
+    r->t_done = now_s(t0);
+    r->done = true;
+    used -= r->bytes;
+    completed++;
+    g.retired = true;
+  }
+  g.chunk_steps = 0;
+  return TECS_OK;
+}
+
+// Re-form the (idle) group if its membership changed, then enqueue its next chunk.
+int Server::start_chunk(Group& g) {
+  if (g.retired || !g.joining.empty()) {
+    std::vector<Request*> members;
+    for (Request* r : g.members) {
+      if (r->done)
+        r->slot = -1;
+      else
+        members.push_back(r);
+    }
+    for (Request* r : g.joining) members.push_back(r);
+    g.joining.clear();
+    TRY(regroup(g, members));
+  }
+  if (g.members.empty()) return TECS_OK;
+  int steps = chunk;
+  for (Request* r : g.members) steps = std::min(steps, r->n_steps - r->steps_done);
+  std::vector<double> dts;
+  for (Request* r : g.members) dts.push_back(r->dt);
+  if (steps > 0) TRY(g.batch->take_step(dts.data(), steps));
+  g.chunk_steps = steps;
+  g.busy = true;
   return TECS_OK;
 }
 
 int Server::advance(int32_t* running, int32_t* waiting) {
   // 1. admission
-  std::vector<Request*> admitted;
   for (auto it = pending.begin(); it != pending.end();) {
     Request* r = *it;
-    if (used + r->bytes <= budget) {
-      used += r->bytes;
-      admitted.push_back(r);
-      it = pending.erase(it);
-    } else {
+    if (used + r->bytes > budget) {
       ++it;
+      continue;
     }
-  }
-  // 2. regroup groups that gained members
-  for (Request* r : admitted) {
+    used += r->bytes;
     Group* g = nullptr;
     for (auto& c : groups)
       if (same_group(c.key, r->desc)) g = &c;
     if (!g) {
-      groups.push_back(Group{r->desc, nullptr, {}, {}});
+      groups.emplace_back();
       g = &groups.back();
+      g->key = r->desc;
+      HIP_TRY(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
     }
     g->joining.push_back(r);
+    it = pending.erase(it);
   }
-  for (auto& g : groups) {
-    if (g.joining.empty()) continue;
-    std::vector<Request*> members = g.members;
-    for (Request* r : g.joining) members.push_back(r);
-    g.joining.clear();
-    int rc = regroup(g, members);
-    if (rc) return rc;
-  }
-  // 3. stepping: all groups enqueue, then wait
-  int n_run = 0;
-  for (auto& g : groups) {
-    if (g.members.empty()) continue;
-    int steps = chunk;
-    for (Request* r : g.members) steps = std::min(steps, r->n_steps - r->steps_done);
-    std::vector<double> dts;
-    for (Request* r : g.members) dts.push_back(r->dt);
-    if (steps > 0) {
-      int rc = g.batch->take_step(dts.data(), steps);
-      if (rc) return rc;
-    }
-    g.pending_steps = steps;
-    n_run += (int)g.members.size();
-  }
-  for (auto& g : groups) {
-    if (g.members.empty()) continue;
-    const hipError_t e = hipStreamSynchronize(g.batch->stream);
-    if (e != hipSuccess) return fail(TECS_ERR_DEVICE, hipGetErrorString(e));
-    for (Request* r : g.members) r->steps_done += g.pending_steps;
-    steps_run += (uint64_t)g.pending_steps * g.members.size();
-  }
-  // 4. retirement
-  for (auto& g : groups) {
-    if (g.members.empty()) continue;
-    std::vector<Request*> keep;
-    bool changed = false;
-    for (Request* r : g.members) {
-      if (r->steps_done < r->n_steps) {
-        keep.push_back(r);
-        continue;
+  // 2./3. service idle groups; wait until at least one was idle
+  for (;;) {
+    bool serviced = false, any_busy = false;
+    for (auto& g : groups) {
+      if (g.busy) {
+        const hipError_t q = hipStreamQuery(g.stream);
+        if (q == hipErrorNotReady) {
+          any_busy = true;
+          continue;
+        }
+        if (q != hipSuccess) return fail(TECS_ERR_DEVICE, hipGetErrorString(q));
+        TRY(finish_chunk(g));
+        serviced = true;
       }
-      const size_t n = r->desc.agent_count;
-      r->pos.assign(3 * n, 0.0f);
-      if (r->desc.fields & TECS_FIELD_POLARITY) r->pol.assign(2 * n, 0.0f);
-      int rc = g.batch->download(r->slot, r->pos.data(), r->pol.empty() ? nullptr : r->pol.data(), nullptr);
-      if (rc) return rc;
-      r->t_done = now_s(t0);
-      r->done = true;
-      r->slot = -1;
-      used -= r->bytes;
-      completed++;
-      changed = true;
+      const bool had_work = g.retired || !g.joining.empty() || !g.members.empty();
+      TRY(start_chunk(g));
+      serviced = serviced || had_work;
+      any_busy = any_busy || g.busy;
     }
-    if (changed) {
-      int rc = regroup(g, keep);
-      if (rc) return rc;
-    }
+    if (serviced || !any_busy) break;
+    std::this_thread::sleep_for(std::chrono::microseconds(50));
   }
-  if (running) *running = n_run;
+  if (running) {
+    int32_t n_run = 0;
+    for (auto& g : groups) n_run += g.busy ? (int32_t)g.members.size() : 0;
+    *running = n_run;
+  }
   if (waiting) *waiting = (int32_t)pending.size();
   return TECS_OK;
 }
@@ -217,11 +256,11 @@ int Server::stats(tecs_server_stats* out) {
   out->running = 0;
   uint64_t pairs = retired_pairs;
   for (auto& g : groups) {
-    out->running += g.members.size();
+    for (Request* r : g.members) out->running += r->done ? 0 : 1;
+    out->running += g.joining.size();
     if (g.batch) {
       tecs_stats st{};
-      int rc = g.batch->get_stats(&st);
-      if (rc) return rc;
+      TRY(g.batch->get_stats(&st));  // waits for the group's chunk in flight
       pairs += st.pair_interactions;
     }
   }
@@ -230,6 +269,8 @@ int Server::stats(tecs_server_stats* out) {
   out->regroups = rebuilds;
   out->hbm_bytes_in_use = used;
   out->hbm_budget = budget;
+  out->pool_cached_bytes = pool.cached_bytes();
+  out->regroup_seconds = regroup_s;
   return TECS_OK;
 }
 

@@ -39,22 +39,31 @@ class Server {
  private:
   struct Group {
     tecs_sim_desc key;
-    Batch* batch;
-    std::vector<Request*> members;
-    std::vector<Request*> joining;
-    int pending_steps = 0;
+    hipStream_t stream = nullptr;     // the group's batches run here, one after another
+    Batch* batch = nullptr;
+    std::vector<Request*> members;    // members of `batch`, in member-index order
+    std::vector<Request*> joining;    // admitted, waiting for the next re-form
+    bool busy = false;                // a chunk is in flight on `stream`
+    bool retired = false;             // some members finished since the last re-form
+    int chunk_steps = 0;              // steps of the chunk in flight
+    int tuned = -1;                   // sweep variant the autotuner picked for this group
+    int64_t tuned_cells = 0;          // the group's cell count when it was tuned
   };
   int regroup(Group& g, std::vector<Request*> members);
+  int finish_chunk(Group& g);
+  int start_chunk(Group& g);
 
   uint64_t budget;
   uint64_t used = 0;
   int chunk;
+  DevicePool pool;
   std::chrono::steady_clock::time_point t0;
   int64_t next_id = 1;
   std::map<int64_t, std::unique_ptr<Request>> requests;
   std::list<Request*> pending;
-  std::vector<Group> groups;
+  std::list<Group> groups;  // stable addresses
   uint64_t retired_pairs = 0, steps_run = 0, rebuilds = 0, completed = 0;
+  double regroup_s = 0.0;
 };
 
 }  // namespace tecs_amd

@@ -53,7 +53,7 @@ class Server:
     def stats(self) -> dict:
         s = abi.ServerStats()
         check(self._lib.tecs_server_get_stats(self._h, C.byref(s)))
-        return {k: int(getattr(s, k)) for k, _ in abi.ServerStats._fields_}
+        return {k: getattr(s, k) for k, _ in abi.ServerStats._fields_}
 
     def close(self):
         if getattr(self, "_h", None):
