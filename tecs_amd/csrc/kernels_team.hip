@@ -39,7 +39,8 @@ struct TeamLds {
   static constexpr int AM = TeamShape<T>::kAMax;
   float4 apos[TB][AM];  // accepted neighbour position (w = type bits)
   int aslot[TB][AM];    // its slot
-  int arel[TB][AM];     // its (stale) bin relative to i's centre bin - 1, packed 2 bits/dim
+  int abit[TB][AM];     // one-hot bit rx + 3 ry + 9 rz of its (stale) bin relative to
+                        // i's centre bin - 1 (r* in 0..2)
   float4 cpos[TB][AM];  // position contribution, w = 1 when the pair counts
   float2 cpol[TB][AM];  // polarity contribution
   int row_start[TB][12];
@@ -50,7 +51,6 @@ struct TeamLds {
   int unsafe[TB];
 };
 
-__device__ __forceinline__ int pack_rel(int rx, int ry, int rz) { return rx | (ry << 2) | (rz << 4); }
 
 // ---------------------------------------------------------------- serial fallback (leader)
 template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE>
@@ -135,16 +135,21 @@ __device__ __forceinline__ float tblock_sum_f(float v, float* lds) {
   if (threadIdx.x == 0) r = ((lds[0] + lds[1]) + lds[2]) + lds[3];
   return r;
 }
-__device__ __forceinline__ unsigned tblock_sum_u(unsigned v, unsigned* lds) {
+__device__ __forceinline__ unsigned wave_sum_u(unsigned v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  const int w = threadIdx.x >> 6;
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) lds[w] = v;
-  __syncthreads();
-  unsigned r = 0;
-  if (threadIdx.x == 0) r = lds[0] + lds[1] + lds[2] + lds[3];
-  return r;
+  return v;
+}
+
+// A team's LDS rows are touched only by the team's lanes and a team never spans two
+// wavefronts (T divides 64), so the phases synchronise the wavefront, not the workgroup: the
+// fences wait for the wavefront's outstanding LDS accesses and keep the compiler from moving
+// LDS accesses across this point; the four wavefronts of a workgroup run their cells
+// independently.
+__device__ __forceinline__ void team_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
 template <int T, bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE>
@@ -153,7 +158,6 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
   constexpr int AM = TeamShape<T>::kAMax;
   __shared__ TeamLds<T> L;
   __shared__ float lds_f[4];
-  __shared__ unsigned lds_u[4];
 
   // team-block -> slots: each 256-slot chunk (one instance) holds T team-blocks;
   // XCD-aware remap (see logical_block): contiguous runs of team-blocks per XCD
@@ -209,7 +213,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
     }
   }
   if (lane == 0) L.unsafe[team] = 0;
-  __syncthreads();
+  team_sync();
 
   int row_pre[9], row_start[9];
   int total = 0;
@@ -267,7 +271,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
         const int bx = L.row_xlo[team][rr_sel] + (m >= L.row_b1[team][rr_sel] ? 1 : 0) +
                        (m >= L.row_b2[team][rr_sel] ? 1 : 0);
         L.apos[team][pos] = pj;
-        L.arel[team][pos] = pack_rel(bx - bc[0] + 1, rr_sel % 3, rr_sel / 3);
+        // row rr_sel = 3 rz + ry, so 3 ry + 9 rz = 3 rr_sel
+        L.abit[team][pos] = 1 << ((bx - bc[0] + 1) + 3 * rr_sel);
       } else {
         // all-pairs: the force needs nothing else, compute it here
         float cp[3] = {0.0f, 0.0f, 0.0f}, cl[2] = {0.0f, 0.0f};
@@ -286,7 +291,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
   }
   const bool overflow = base > AM;
   const int na = min(base, AM);
-  __syncthreads();
+  team_sync();
 
   if (GABRIEL && active && !overflow) {
     for (int e = lane; e < na; e += T) {
@@ -301,7 +306,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
       const float rho = sqrtf(rho2);
       // bounding-box bins (binned_gabriel.hpp:142-153), relative to bc - 1
       const float mid[3] = {mid0, mid1, mid2};
-      int lo = 0, hi = 0;
+      int dmask[3];  // per dimension: bits lo..hi of the box, relative to bc - 1
       bool out_of_reach = false;
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
@@ -310,26 +315,25 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
         const int mx =
             clampi((int)floorf((mid[d] + rho - in.minb[d]) * in.inv_bin), in.ext[d] - 1) - bc[d] + 1;
         out_of_reach = out_of_reach || mn < 0 || mx > 2;
-        lo |= max(mn, 0) << (2 * d);
-        hi |= min(mx, 2) << (2 * d);
+        const int lo = max(mn, 0), hi = min(mx, 2);
+        dmask[d] = lo <= hi ? ((2 << hi) - (1 << lo)) : 0;
       }
       if (out_of_reach) L.unsafe[team] = 1;
+      // the box as a set of the 27 relative bins (bit rx + 3 ry + 9 rz)
+      int m9 = 0, box = 0;
+#pragma unroll
+      for (int y = 0; y < 3; ++y)
+        if ((dmask[1] >> y) & 1) m9 |= dmask[0] << (3 * y);
+#pragma unroll
+      for (int z = 0; z < 3; ++z)
+        if ((dmask[2] >> z) & 1) box |= m9 << (9 * z);
       bool blocked = false;
       for (int k = 0; k < na; ++k) {
-        if (k == e) continue;
-        const int rel = L.arel[team][k];
-        bool inside = true;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          const int v = (rel >> (2 * d)) & 3;
-          inside = inside && v >= ((lo >> (2 * d)) & 3) && v <= ((hi >> (2 * d)) & 3);
-        }
-        if (!inside) continue;
+        const int bits = L.abit[team][k];
         const float4 pk = L.apos[team][k];
-        if (len2(pk.x - mid0, pk.y - mid1, pk.z - mid2) < rho2) {
-          blocked = true;
-          break;
-        }
+        const bool in_box = k != e && (bits & box) != 0;
+        const bool in_ball = len2(pk.x - mid0, pk.y - mid1, pk.z - mid2) < rho2;
+        blocked = blocked | (in_box & in_ball);
       }
       float cp[3] = {0.0f, 0.0f, 0.0f}, cl[2] = {0.0f, 0.0f};
       if (!blocked && MODE != kModeCount && MODE != kModeEmit) {
@@ -343,7 +347,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
       L.cpol[team][e] = make_float2(cl[0], cl[1]);
     }
   }
-  __syncthreads();
+  team_sync();
 
   // leader: ordered sum (or the serial walk after an overflow / unsafe box)
   float acc_pos[3] = {0.0f, 0.0f, 0.0f}, acc_pol[2] = {0.0f, 0.0f};
@@ -426,8 +430,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
     const float t2 = tblock_sum_f(leader ? F[2] : 0.0f, lds_f);
     if (threadIdx.x == 0) a.com_partial[tb] = make_float4(t0, t1, t2, 0.0f);
   }
-  const unsigned tp = tblock_sum_u(pairs, lds_u);
-  if (threadIdx.x == 0) a.block_pairs[tb] += tp;
+  const unsigned tp = wave_sum_u(pairs);
+  if ((threadIdx.x & 63) == 0 && tp != 0u) atomicAdd(&a.block_pairs[tb], (unsigned long long)tp);
 }
 
 template <int T, bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE>

@@ -123,16 +123,8 @@ __device__ __forceinline__ void piecewise_linear(const float* disp, float d, flo
 }
 
 // ---------------------------------------------------------------- polarity.hpp:111-198
-// Kokkos::pow(float, int) promotes to double; the quotient is narrowed to float when it
-// multiplies the float vector (vector.hpp:293-296). The squares are exact in double.
-__device__ __forceinline__ float pow2_ratio(float a, float b) {
-  const double ad = (double)a, bd = (double)b;
-  return (float)((ad * ad) / (bd * bd));
-}
-
 // Polarity of a cell with the sines and cosines of its angles: trig = (sin theta, cos theta,
-// sin phi, cos phi), computed once per substep per slot (k_polar_trig). The force reads them
-// where the reference evaluates sinf/cosf of the same angles, so the values are identical.
+// sin phi, cos phi), computed once per substep per slot (k_polar_trig).
 struct Pol {
   float t, p;    // theta, phi
   float4 trig;   // sin t, cos t, sin p, cos p
@@ -141,6 +133,23 @@ __device__ __forceinline__ Pol make_pol(float2 q, float4 trig) { return Pol{q.x,
 
 // One invocation of the fused pairwise force (see tecs_force_desc in tecs_amd.h), adding the
 // contributions to the accumulators in the reference's statement order.
+//
+// The PiecewiseLinear part is evaluated with the reference's operations in its order, so it is
+// bitwise the reference's FP32 result. The polarity terms are evaluated from the same
+// mathematical expressions with the angle functions replaced by exact identities over the
+// precomputed sines and cosines (checked against an FP64 evaluation of the reference's
+// formulas, 1e-5 relative):
+//   bending_force (polarity.hpp:181-198): prod = v.disp / d, the coefficients -prod/d and
+//     pow(prod, 2)/pow(d, 2) = (prod/d)^2 from one reciprocal of d;
+//   from_vector3(disp, d) (polarity.hpp:62-66) gives q = (acos(dz/d), atan2(dy, dx)), so
+//     sin q0 cos(p - q1) = (cos p dx + sin p dy) / d, sin q0 sin(p - q1) = (sin p dx - cos p dy) / d
+//     and cos q0 = dz / d; unidirectional_polarization_force(q) (polarity.hpp:145-157) needs
+//     only these three (sin q0 = sqrt(dx^2 + dy^2) / d >= 0 is never formed, which avoids the
+//     cancellation of acos near the poles);
+//   bidirectional_polarization_force (polarity.hpp:165-168, dot :131-136) uses
+//     cos(p_i - p_j) = cos p_i cos p_j + sin p_i sin p_j and
+//     sin(p_i - p_j) = sin p_i cos p_j - cos p_i sin p_j.
+// The guard |sin theta_i| > FLT_EPSILON (polarity.hpp:154) is kept on the same value.
 template <bool HAS_TYPE, bool HAS_POL>
 __device__ __forceinline__ void apply_force(const ForceParams& f, const float* disp, float d,
                                             int type_i, int type_j, const Pol& pi, const Pol& pj,
@@ -157,42 +166,35 @@ __device__ __forceinline__ void apply_force(const ForceParams& f, const float* d
   if (HAS_POL) {
     const float sti = pi.trig.x, cti = pi.trig.y, spi = pi.trig.z, cpi = pi.trig.w;
     const float stj = pj.trig.x, ctj = pj.trig.y, spj = pj.trig.z, cpj = pj.trig.w;
+    const bool pole = !(fabsf(sti) > 1.1920928955078125e-07f);
+    const float inv_sti = pole ? 0.0f : 1.0f / sti;
     if (f.bend != 0.0f) {
-      // bending_force, polarity.hpp:181-198; to_vector3 (polarity.hpp:111-118) from the
-      // precomputed sines and cosines
-      const float tv[3] = {sti * cpi, sti * spi, cti};
-      const float prod_i = dot3(tv, disp) / d;
+      const float inv_d = 1.0f / d;
+      const float tv[3] = {sti * cpi, sti * spi, cti};  // to_vector3, polarity.hpp:111-118
       const float ov[3] = {stj * cpj, stj * spj, ctj};
-      const float prod_j = dot3(ov, disp) / d;
-      const float a = -prod_i / d;
-      const float b = pow2_ratio(prod_i, d);
-      const float c = -prod_j / d;
-      const float e = pow2_ratio(prod_j, d);
+      const float prod_i = dot3(tv, disp) * inv_d;
+      const float prod_j = dot3(ov, disp) * inv_d;
+      const float ri = prod_i * inv_d, rj = prod_j * inv_d;
+      const float a = -ri, b = ri * ri, c = -rj, e = rj * rj;
       float bv[3];
 #pragma unroll
       for (int k = 0; k < 3; ++k) bv[k] = ((a * tv[k] + b * disp[k]) + c * ov[k]) + e * disp[k];
-      // from_vector3(displacement, distance), polarity.hpp:62-66
-      const float q0 = acosf(fminf(1.0f, fmaxf(-1.0f, disp[2] / d)));
-      const float q1 = atan2f(disp[1], disp[0]);
-      // unidirectional_polarization_force(q), polarity.hpp:145-157
-      const float sq0 = sinf(q0), cq0 = cosf(q0);
-      float u0 = cti * sq0 * cosf(pi.p - q1) - sti * cq0;
-      float u1 = 0.0f;
-      if (fabsf(sti) > 1.1920928955078125e-07f) u1 = -sq0 * sinf(pi.p - q1) / sti;
-      const float bp0 = -prod_i * u0;
-      const float bp1 = -prod_i * u1;
+      const float sc = (cpi * disp[0] + spi * disp[1]) * inv_d;  // sin q0 cos(p_i - q1)
+      const float ss = (spi * disp[0] - cpi * disp[1]) * inv_d;  // sin q0 sin(p_i - q1)
+      const float cq0 = disp[2] * inv_d;
+      const float u0 = cti * sc - sti * cq0;
+      const float u1 = pole ? 0.0f : -ss * inv_sti;
       acc_pos[0] = acc_pos[0] + f.bend * bv[0];
       acc_pos[1] = acc_pos[1] + f.bend * bv[1];
       acc_pos[2] = acc_pos[2] + f.bend * bv[2];
-      acc_pol[0] = acc_pol[0] + f.bend * bp0;
-      acc_pol[1] = acc_pol[1] + f.bend * bp1;
+      acc_pol[0] = acc_pol[0] + f.bend * (-prod_i * u0);
+      acc_pol[1] = acc_pol[1] + f.bend * (-prod_i * u1);
     }
     if (f.align != 0.0f) {
-      // bidirectional_polarization_force, polarity.hpp:165-168 (dot :131-136)
-      const float cdp = cosf(pi.p - pj.p);
-      float u0 = cti * stj * cdp - sti * ctj;
-      float u1 = 0.0f;
-      if (fabsf(sti) > 1.1920928955078125e-07f) u1 = -stj * sinf(pi.p - pj.p) / sti;
+      const float cdp = cpi * cpj + spi * spj;  // cos(p_i - p_j)
+      const float sdp = spi * cpj - cpi * spj;  // sin(p_i - p_j)
+      const float u0 = cti * stj * cdp - sti * ctj;
+      const float u1 = pole ? 0.0f : -stj * sdp * inv_sti;
       const float sdot = sti * stj * cdp + cti * ctj;
       acc_pol[0] = acc_pol[0] + f.align * (sdot * u0);
       acc_pol[1] = acc_pol[1] + f.align * (sdot * u1);
