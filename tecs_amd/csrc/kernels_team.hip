@@ -27,30 +27,36 @@
 namespace tecs_amd {
 namespace {
 
-template <int T>
-struct TeamShape {
-  static constexpr int kTeams = kBlock / T;          // teams (cells) per workgroup
-  static constexpr int kAMax = T >= 8 ? 32 : 24;     // accepted neighbours held in LDS per cell
+// Accepted neighbours held in LDS per cell: a Gabriel cell on a sheet or in a tissue has ~5-8
+// neighbours within the cutoff, an all-pairs cell in a dense tissue ~6-10; a cell with more
+// takes the serial path.
+template <bool GABRIEL>
+struct TeamCap {
+  static constexpr int kAMax = GABRIEL ? 16 : 24;
 };
 
-template <int T>
+template <int T, bool GABRIEL, bool HAS_POL>
 struct TeamLds {
-  static constexpr int TB = TeamShape<T>::kTeams;
-  static constexpr int AM = TeamShape<T>::kAMax;
-  float4 apos[TB][AM];  // accepted neighbour position (w = type bits)
+  static constexpr int TB = kBlock / T;  // teams (cells) per workgroup
+  static constexpr int AM = TeamCap<GABRIEL>::kAMax;
+  static constexpr int TG = GABRIEL ? TB : 1;
+  static constexpr int TP = HAS_POL ? TB : 1;
+  // per accepted neighbour, in visiting order:
+  //   ent: Gabriel - its position (w = type bits) until the Gabriel tests are done, then
+  //        the pair's position contribution; all-pairs - the contribution. w = 1 when the
+  //        pair counts.
+  float4 ent[TB][AM];
   int aslot[TB][AM];    // its slot
-  int abit[TB][AM];     // one-hot bit rx + 3 ry + 9 rz of its (stale) bin relative to
+  int abit[TG][AM];     // one-hot bit rx + 3 ry + 9 rz of its (stale) bin relative to
                         // i's centre bin - 1 (r* in 0..2)
-  float4 cpos[TB][AM];  // position contribution, w = 1 when the pair counts
-  float2 cpol[TB][AM];  // polarity contribution
-  int row_start[TB][12];
-  int row_len[TB][12];
-  int row_b1[TB][12];   // first slot of the row's second bin (Gabriel)
-  int row_b2[TB][12];   // first slot of the row's third bin (Gabriel)
-  int row_xlo[TB][12];
+  float2 cpol[TP][AM];  // polarity contribution
+  int row_start[TB][9];
+  int row_len[TB][9];
+  int row_b1[TG][9];    // first slot of the row's second bin (Gabriel)
+  int row_b2[TG][9];    // first slot of the row's third bin (Gabriel)
+  int row_xlo[TG][9];
   int unsafe[TB];
 };
-
 
 // ---------------------------------------------------------------- serial fallback (leader)
 template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE>
@@ -154,9 +160,9 @@ __device__ __forceinline__ void team_sync() {
 
 template <int T, bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE>
 __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
-  constexpr int TB = TeamShape<T>::kTeams;
-  constexpr int AM = TeamShape<T>::kAMax;
-  __shared__ TeamLds<T> L;
+  constexpr int TB = TeamLds<T, GABRIEL, HAS_POL>::TB;
+  constexpr int AM = TeamLds<T, GABRIEL, HAS_POL>::AM;
+  __shared__ TeamLds<T, GABRIEL, HAS_POL> L;
   __shared__ float lds_f[4];
 
   // team-block -> slots: each 256-slot chunk (one instance) holds T team-blocks;
@@ -270,7 +276,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
         // boundaries give x
         const int bx = L.row_xlo[team][rr_sel] + (m >= L.row_b1[team][rr_sel] ? 1 : 0) +
                        (m >= L.row_b2[team][rr_sel] ? 1 : 0);
-        L.apos[team][pos] = pj;
+        L.ent[team][pos] = pj;
         // row rr_sel = 3 rz + ry, so 3 ry + 9 rz = 3 rr_sel
         L.abit[team][pos] = 1 << ((bx - bc[0] + 1) + 3 * rr_sel);
       } else {
@@ -283,8 +289,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
           apply_force<HAS_TYPE, HAS_POL>(in.force, disp, d, __float_as_int(me.w),
                                          __float_as_int(pj.w), pme, po, cp, cl);
         }
-        L.cpos[team][pos] = make_float4(cp[0], cp[1], cp[2], 1.0f);
-        L.cpol[team][pos] = make_float2(cl[0], cl[1]);
+        L.ent[team][pos] = make_float4(cp[0], cp[1], cp[2], 1.0f);
+        if (HAS_POL) L.cpol[team][pos] = make_float2(cl[0], cl[1]);
       }
     }
     base += __popc(tm);
@@ -293,10 +299,12 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
   const int na = min(base, AM);
   team_sync();
 
+  // Gabriel tests of every accepted pair (bit r of `blocked_bits`: entry lane + r T)
+  unsigned blocked_bits = 0u;
   if (GABRIEL && active && !overflow) {
-    for (int e = lane; e < na; e += T) {
-      const float4 pj = L.apos[team][e];
-      const int m = L.aslot[team][e];
+    int r = 0;
+    for (int e = lane; e < na; e += T, ++r) {
+      const float4 pj = L.ent[team][e];
       const float disp[3] = {me.x - pj.x, me.y - pj.y, me.z - pj.z};
       const float d2 = len2(disp[0], disp[1], disp[2]);
       const float mid0 = me.x - disp[0] * 0.5f;
@@ -306,7 +314,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
       const float rho = sqrtf(rho2);
       // bounding-box bins (binned_gabriel.hpp:142-153), relative to bc - 1
       const float mid[3] = {mid0, mid1, mid2};
-      int dmask[3];  // per dimension: bits lo..hi of the box, relative to bc - 1
+      int dmask[3];  // per dimension: bits lo..hi of the box
       bool out_of_reach = false;
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
@@ -330,21 +338,34 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
       bool blocked = false;
       for (int k = 0; k < na; ++k) {
         const int bits = L.abit[team][k];
-        const float4 pk = L.apos[team][k];
+        const float4 pk = L.ent[team][k];
         const bool in_box = k != e && (bits & box) != 0;
         const bool in_ball = len2(pk.x - mid0, pk.y - mid1, pk.z - mid2) < rho2;
         blocked = blocked | (in_box & in_ball);
       }
+      if (blocked) blocked_bits |= 1u << r;
+    }
+  }
+  team_sync();
+
+  // forces of the surviving pairs; each lane overwrites only its own entries
+  if (GABRIEL && active && !overflow) {
+    int r = 0;
+    for (int e = lane; e < na; e += T, ++r) {
+      const float4 pj = L.ent[team][e];
+      const bool blocked = (blocked_bits >> r) & 1u;
       float cp[3] = {0.0f, 0.0f, 0.0f}, cl[2] = {0.0f, 0.0f};
       if (!blocked && MODE != kModeCount && MODE != kModeEmit) {
-        const float d = sqrtf(d2);
+        const int m = L.aslot[team][e];
+        const float disp[3] = {me.x - pj.x, me.y - pj.y, me.z - pj.z};
+        const float d = sqrtf(len2(disp[0], disp[1], disp[2]));
         Pol po{};
         if (HAS_POL) po = make_pol(a.ps[m], a.trig[m]);
         apply_force<HAS_TYPE, HAS_POL>(in.force, disp, d, __float_as_int(me.w),
                                        __float_as_int(pj.w), pme, po, cp, cl);
       }
-      L.cpos[team][e] = make_float4(cp[0], cp[1], cp[2], blocked ? 0.0f : 1.0f);
-      L.cpol[team][e] = make_float2(cl[0], cl[1]);
+      L.ent[team][e] = make_float4(cp[0], cp[1], cp[2], blocked ? 0.0f : 1.0f);
+      if (HAS_POL) L.cpol[team][e] = make_float2(cl[0], cl[1]);
     }
   }
   team_sync();
@@ -360,7 +381,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
       int64_t emit_at = 0;
       if (MODE == kModeEmit) emit_at = a.row_ptr[a.id[s]];
       for (int e = 0; e < na; ++e) {
-        const float4 c = L.cpos[team][e];
+        const float4 c = L.ent[team][e];
         if (c.w == 0.0f) continue;
         pairs++;
         if (MODE == kModeEmit) {
