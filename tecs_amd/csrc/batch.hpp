@@ -182,7 +182,6 @@ struct Batch {
   int* cnt = nullptr;
   int* coff = nullptr;
   int* off = nullptr;
-  int* skey = nullptr;
   int* d_fill_inst = nullptr;
   int* d_fill_base = nullptr;
   int* bsum = nullptr;
@@ -203,12 +202,16 @@ struct Batch {
   int variant = TECS_SWEEP_AUTO;  // requested sweep variant
   int tuned = -1;                 // concrete variant picked by the autotuner (AUTO)
   bool team_ok = false;           // the team sweep reproduces this batch exactly
-  float4* trig = nullptr;         // sin/cos of the polarity angles of the evaluated stage
+  float4* trig[2] = {nullptr, nullptr};  // polar_trig of pol[b] (kept current by every writer)
+  float4* trigp = nullptr;                // polar_trig of polp
   unsigned long long* tune_pairs = nullptr;  // scratch counters for autotuning sweeps
   double tune_ms[6] = {0, 0, 0, 0, 0, 0};
   int xc = 0, ic = 0;  // current position/polarity/old-velocity buffer, current id buffer
   int step_count = 0;
   bool grid_built = false;
+  // the last step's final sweep stored the keys of x[xc] and counted them in cnt (count_key),
+  // so the rebuild that starts the next step skips k_bin_keys
+  bool keys_ready = false;
   tecs_stats stats{};
 
   bool timing = false;
@@ -228,8 +231,9 @@ struct Batch {
   int graph_mode = TECS_GRAPHS_AUTO;
   struct HostState {
     int xc, ic, step_count;
+    bool keys_ready;
     bool operator==(const HostState& o) const {
-      return xc == o.xc && ic == o.ic && step_count == o.step_count;
+      return xc == o.xc && ic == o.ic && step_count == o.step_count && keys_ready == o.keys_ready;
     }
   };
   struct StepGraph {
@@ -248,7 +252,7 @@ struct Batch {
   hipStream_t side = nullptr;
   std::vector<unsigned> write_step_count;  // Simulation::current_step per member
 
-  HostState host_state() const { return HostState{xc, ic, step_count}; }
+  HostState host_state() const { return HostState{xc, ic, step_count, keys_ready}; }
   int graph_period() const { return 2 * std::max(1, is_binned(finder) ? rebuild_every_n : 1); }
 
   void drop_graphs() {
@@ -375,11 +379,13 @@ struct Batch {
       }
       in.bin_begin = (int)bins;
       // sparse grids (many bins per cell, e.g. a sheet of cells on a sphere) count segments of
-      // 2^shift bins so the scan does not run over every empty bin
+      // 2^shift bins so the scan does not run over every empty bin: about 8 segments per cell,
+      // which keeps the segments that run along a sheet short (the gather ranks a cell among
+      // its segment's members)
       in.shift = 0;
       if (is_binned(finder) && in.n > 0) {
         const double bpc = (double)in.bin_count / (double)in.n;
-        if (bpc >= 8.0) in.shift = std::min(10, (int)std::floor(std::log2(bpc)));
+        if (bpc >= 8.0) in.shift = std::min(10, std::max(2, (int)std::floor(std::log2(bpc)) - 3));
       }
       in.cbin_count = ((in.bin_count + (1 << in.shift) - 1) >> in.shift) + 1;
       in.dt = 0.0f;
@@ -416,10 +422,11 @@ struct Batch {
       total_cbins = fill_bins ? (int)cb : total_bins;
     }
     if (fill_bins) {
-      // one fill thread per 4 bins (the padded range has room for the last quad)
+      // one fill thread per kFillQuads quads of 4 bins (the padded range has room for the
+      // last quad)
       for (int k = 0; k < count; ++k) {
         const int quads = (h_inst[k].bin_count + 1 + 3) / 4;
-        for (int q = 0; q < quads; q += kBlock) {
+        for (int q = 0; q < quads; q += kBlock * kFillQuads) {
           h_fill_inst.push_back(k);
           h_fill_base.push_back(q);
         }
@@ -464,7 +471,6 @@ struct Batch {
       TRY(alloc(&coff, (size_t)total_cbins + 1));
       if (fill_bins) {
         TRY(alloc(&off, (size_t)total_bins + 1));
-        TRY(alloc(&skey, S));
         TRY(alloc(&d_fill_inst, h_fill_inst.size()));
         TRY(alloc(&d_fill_base, h_fill_base.size()));
         HIP_TRY(hipMemcpyAsync(d_fill_inst, h_fill_inst.data(), sizeof(int) * h_fill_inst.size(),
@@ -488,7 +494,11 @@ struct Batch {
     }
     TRY(alloc(&block_pairs, (size_t)n_blocks * kMaxTeamLanes));
     TRY(alloc(&tune_pairs, (size_t)n_blocks * kMaxTeamLanes));
-    if (has_pol) TRY(alloc(&trig, S));
+    if (has_pol) {
+      TRY(alloc(&trig[0], S));
+      TRY(alloc(&trig[1], S));
+      if (integrator == TECS_HEUN) TRY(alloc(&trigp, S));
+    }
     TRY(alloc(&com_partial, (size_t)n_blocks * kMaxTeamLanes));
     TRY(alloc(&com, n_inst));
     HIP_TRY(hipMemsetAsync(block_pairs, 0, sizeof(unsigned long long) * n_blocks * kMaxTeamLanes, stream));
@@ -497,7 +507,9 @@ struct Batch {
       HIP_TRY(hipMemsetAsync(x[b], 0, sizeof(float4) * S, stream));
       if (has_pol) HIP_TRY(hipMemsetAsync(pol[b], 0, sizeof(float2) * S, stream));
       if (track_ov) HIP_TRY(hipMemsetAsync(ov[b], 0, sizeof(float4) * S, stream));
+      if (has_pol) launch_polar_trig(pol[b], trig[b], total_slots, stream);
     }
+    HIP_TRY(hipGetLastError());
     const size_t M = (size_t)std::max(max_n, 1);
     TRY(alloc(&st_pos, 3 * M));
     TRY(alloc(&st_ov, 3 * M));
@@ -580,10 +592,12 @@ struct Batch {
     r.block_inst = d_block_inst;
     r.x_old = x[xc];
     r.pol_old = pol[xc];
+    r.trig_old = trig_of(pol[xc]);
     r.id_old = id[ic];
     r.ov_old = ov[xc];
     r.x_new = x[xc ^ 1];
     r.pol_new = pol[xc ^ 1];
+    r.trig_new = trig_of(pol[xc ^ 1]);
     r.id_new = id[ic ^ 1];
     r.ov_new = ov[xc ^ 1];
     r.keys = keys;
@@ -593,7 +607,7 @@ struct Batch {
     r.cnt = cnt;
     r.coff = coff;
     r.off = off;
-    r.skey = skey;
+    r.keys_ready = keys_ready;
     r.scan_bsum = bsum;
     r.total_cbins = total_cbins;
     r.fill = fill_bins;
@@ -602,6 +616,7 @@ struct Batch {
     r.fill_blocks = (int)h_fill_inst.size();
     launch_rebuild(r, n_blocks, stream);
     HIP_TRY(hipGetLastError());
+    keys_ready = false;
     TRY(time_end(&p));
     xc ^= 1;
     ic ^= 1;
@@ -631,7 +646,7 @@ struct Batch {
     a.block_inst = d_block_inst;
     a.xs = xs;
     a.ps = ps;
-    a.trig = trig;
+    a.trig = trig_of(ps);
     a.off = off;
     a.id = id[ic];
     a.block_pairs = block_pairs;
@@ -663,12 +678,24 @@ struct Batch {
                         n_blocks, stream);
   }
 
-  // sin/cos of the polarity angles of the stage a sweep is about to read
-  int prepare_trig(const float2* pol_stage) {
-    if (!has_pol) return TECS_OK;
-    launch_polar_trig(pol_stage, trig, total_slots, stream);
-    HIP_TRY(hipGetLastError());
+  // The next full evaluation rebuilds (maybe_rebuild), so the sweep that ends this step can
+  // count the keys of the positions it writes.
+  bool emit_keys() const { return is_binned(finder) && step_count >= rebuild_every_n; }
+
+  // Positions are about to change outside a step: counts of the old ones must not survive.
+  int drop_keys() {
+    if (!keys_ready) return TECS_OK;
+    HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(int) * (size_t)total_cbins, stream));
+    keys_ready = false;
     return TECS_OK;
+  }
+
+  // the trig buffer that belongs to a polarity buffer
+  float4* trig_of(const float2* ps) const {
+    if (!has_pol || !ps) return nullptr;
+    if (ps == pol[0]) return trig[0];
+    if (ps == pol[1]) return trig[1];
+    return trigp;
   }
 
   // Autotuner: time every exact sweep variant on the live state (an evaluation into scratch
@@ -680,7 +707,6 @@ struct Batch {
       tuned = TECS_SWEEP_THREAD;
       return TECS_OK;
     }
-    TRY(prepare_trig(pol[xc]));
     const int cands[4] = {TECS_SWEEP_TEAM4, TECS_SWEEP_TEAM8, TECS_SWEEP_TEAM16, TECS_SWEEP_THREAD};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIP_TRY(hipEventCreate(&e0));
@@ -741,6 +767,8 @@ struct Batch {
     u.d3_pol = d3p;
     u.xp = xp;
     u.polp = polp;
+    u.trig = trig_of(pol[xc]);
+    u.trigp = trigp;
     u.old_vel = track_ov ? ov[xc] : nullptr;
     launch_update(kind, has_pol, u, n_blocks, stream);
     HIP_TRY(hipGetLastError());
@@ -753,30 +781,34 @@ struct Batch {
     const bool com_on = com_fixer == TECS_GLOBAL_COM_FIXER;
     if (integrator == TECS_HEUN) {
       if (!com_on) {
-        TRY(prepare_trig(pol[xc]));
         SweepArgs a = base_args(x[xc], pol[xc]);
         a.o_pos = d1;
         a.o_pol = d1p;
         a.o2_pos = xp;
         a.o2_pol = polp;
+        a.o_trig = trigp;
         TRY(sweep(kModePred, a, kTimeSweepFull));
-        TRY(prepare_trig(polp));
         SweepArgs b = base_args(xp, polp);
         b.o2_pos = x[xc];
         b.o2_pol = pol[xc];
+        b.o_trig = trig_of(pol[xc]);
+        const bool emit = emit_keys();
+        if (emit) {
+          b.keys = keys;
+          b.cnt = cnt;
+        }
         b.d1_pos = d1;
         b.d1_pol = d1p;
         b.old_vel = track_ov ? ov[xc] : nullptr;
         TRY(sweep(kModeCorr, b, kTimeSweepStale));
+        keys_ready = emit;
       } else {
-        TRY(prepare_trig(pol[xc]));
         SweepArgs a = base_args(x[xc], pol[xc]);
         a.o_pos = d1;
         a.o_pol = d1p;
         a.com_partial = com_partial;
         TRY(sweep(kModeDelta, a, kTimeSweepFull));
         TRY(com_and_update(kUpdatePred));
-        TRY(prepare_trig(polp));
         SweepArgs b = base_args(xp, polp);
         b.o_pos = d3;
         b.o_pol = d3p;
@@ -785,14 +817,20 @@ struct Batch {
         TRY(com_and_update(kUpdateCorr));
       }
     } else {
-      TRY(prepare_trig(pol[xc]));
       if (!com_on) {
         SweepArgs a = base_args(x[xc], pol[xc]);
         a.o_pos = x[xc ^ 1];
         a.o_pol = pol[xc ^ 1];
+        a.o_trig = trig_of(pol[xc ^ 1]);
+        const bool emit = emit_keys();
+        if (emit) {
+          a.keys = keys;
+          a.cnt = cnt;
+        }
         a.old_vel = track_ov ? ov[xc ^ 1] : nullptr;
         TRY(sweep(kModeEuler, a, kTimeSweepFull));
         xc ^= 1;
+        keys_ready = emit;
       } else {
         SweepArgs a = base_args(x[xc], pol[xc]);
         a.o_pos = d1;
@@ -902,6 +940,7 @@ struct Batch {
       for (size_t i = 0; i < n; ++i)
         if (type[i] >= (unsigned)d.force.n_types)
           return fail(TECS_ERR_INVALID, "cell type outside the adhesion table");
+    TRY(drop_keys());
     HIP_TRY(hipMemcpyAsync(st_pos, pos, sizeof(float) * 3 * n, hipMemcpyHostToDevice, stream));
     const bool up_pol = has_pol && p;
     const bool up_type = has_type && type;
@@ -910,7 +949,7 @@ struct Batch {
     if (up_type) HIP_TRY(hipMemcpyAsync(st_type, type, n, hipMemcpyHostToDevice, stream));
     const InstanceDesc& in = h_inst[k];
     launch_upload(d_inst, k, in.block_begin, in.n_blocks, id[ic], st_pos, up_pol ? st_pol : nullptr,
-                  up_type ? st_type : nullptr, x[xc], pol[xc], stream);
+                  up_type ? st_type : nullptr, x[xc], pol[xc], trig_of(pol[xc]), stream);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(stream));
     return TECS_OK;
@@ -941,7 +980,6 @@ struct Batch {
   // One evaluation into the d1 scratch buffers, then out in agent order.
   int evaluate(bool is_full_step, float* dpos, float* dpol) {
     TRY(maybe_rebuild(is_full_step));
-    TRY(prepare_trig(pol[xc]));
     SweepArgs a = base_args(x[xc], pol[xc]);
     a.o_pos = d1;
     a.o_pol = d1p;
@@ -1083,15 +1121,19 @@ struct Batch {
     const InstanceDesc& a = src.h_inst[ks];
     const InstanceDesc& b = h_inst[kd];
     if (a.n != b.n) return fail(TECS_ERR_INVALID, "member sizes differ");
+    TRY(drop_keys());
     HIP_TRY(hipStreamSynchronize(src.stream));
     const size_t n = (size_t)a.n;
     HIP_TRY(hipMemcpyAsync(x[xc] + b.slot_begin, src.x[src.xc] + a.slot_begin, sizeof(float4) * n,
                            hipMemcpyDeviceToDevice, stream));
     HIP_TRY(hipMemcpyAsync(id[ic] + b.slot_begin, src.id[src.ic] + a.slot_begin, sizeof(int) * n,
                            hipMemcpyDeviceToDevice, stream));
-    if (has_pol)
+    if (has_pol) {
       HIP_TRY(hipMemcpyAsync(pol[xc] + b.slot_begin, src.pol[src.xc] + a.slot_begin,
                              sizeof(float2) * n, hipMemcpyDeviceToDevice, stream));
+      HIP_TRY(hipMemcpyAsync(trig[xc] + b.slot_begin, src.trig[src.xc] + a.slot_begin,
+                             sizeof(float4) * n, hipMemcpyDeviceToDevice, stream));
+    }
     if (track_ov && src.track_ov)
       HIP_TRY(hipMemcpyAsync(ov[xc] + b.slot_begin, src.ov[src.xc] + a.slot_begin,
                              sizeof(float4) * n, hipMemcpyDeviceToDevice, stream));
@@ -1104,7 +1146,7 @@ struct Batch {
     const size_t slots = ((size_t)d.agent_count + kBlock - 1) / kBlock * kBlock + kBlock;
     const bool pol = (d.fields & TECS_FIELD_POLARITY) != 0;
     size_t per_slot = 2 * 16 + 2 * 4 + 16 + 16 * 2 /* x, id, d1, ov */ + 5 * 4 /* keys.. */;
-    if (pol) per_slot += 2 * 8 + 8 + 16 /* pol, d1 pol, trig */;
+    if (pol) per_slot += 2 * 8 + 8 + 2 * 16 /* pol, d1 pol, trig */;
     if (d.integrator == TECS_HEUN) per_slot += 16 + (pol ? 8 : 0);
     if (d.integrator == TECS_HEUN && d.com_fixer == TECS_GLOBAL_COM_FIXER) per_slot += 16 + (pol ? 8 : 0);
     size_t bins = 0;

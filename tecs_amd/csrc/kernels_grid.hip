@@ -4,16 +4,20 @@
 // cells by key; on the Serial backend the members of a bin end up in ascending agent index.
 // The device version reproduces that order exactly and moves the cell state into it. Cells are
 // counted per segment of 2^shift consecutive bins (shift 0 = per bin for dense grids; a sparse
-// grid, e.g. a cell sheet on a sphere with ~40 bins per cell, counts segments of ~32 bins so
-// the scan runs over segments, not bins):
-//   1. keys:    key = flatten(clamp(int((p - min) / bin))), cnt[key >> shift] += 1 (atomics)
+// grid, e.g. a cell sheet on a sphere with ~40 bins per cell, counts segments of 4 bins, about
+// 8 segments per cell, so the scan runs over segments, not bins):
+//   1. keys:    key = flatten(clamp(int((p - min) / bin))), cnt[key >> shift] += 1 (atomics);
+//               done by the epilogue of the sweep that wrote the positions when the host knows
+//               the next step rebuilds (count_key), else by k_bin_keys
 //   2. scan:    coff = exclusive_scan(cnt) over all instances' segments (+ pad segments)
 //   3. scatter: slot = coff[seg] + (atomic decrement of cnt[seg]) - 1 -> tmp; this leaves cnt
 //               at zero for the next rebuild, so no clearing pass is needed
 //   4. gather:  each cell's final slot is coff[seg] + #(segment members before it in (key,
-//               agent index) order); its state (position, polarity, id, old velocity) moves
+//               agent index) order); its state (position, polarity and its sines/cosines, id,
+//               old velocity) moves
 //   5. fill:    (shift > 0 only) per-bin offsets off[b] = coff[seg] + #(members with key < b),
-//               one coalesced write per bin.
+//               one coalesced write per bin; it reads the members as step 3 left them, so it
+//               runs in the same launch as step 4.
 // Step 3's order within a segment is arbitrary, step 4 makes it canonical, so the result does
 // not depend on atomic arrival order.
 #include "tecs_internal.hpp"
@@ -24,50 +28,14 @@ namespace {
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;
 
-// Wavefront-aggregated counter update: lanes holding the same counter index in a run of
-// consecutive lanes (cells arrive in the previous bin order, so runs are common) issue one
-// atomic for the run. Returns the counter's value before the update, minus the lane's rank in
-// the run, so that with delta = -1 every lane gets a distinct position (old - 1 - rank).
-__device__ __forceinline__ int aggregated_atomic(int* counter_base, int idx, bool valid, int delta) {
-  const int lane = (int)(threadIdx.x & 63);
-  const int key = valid ? idx : -1;
-  const int prev = __shfl_up(key, 1, 64);
-  const bool head = valid && (lane == 0 || prev != key);
-  const unsigned long long heads = __ballot(head);
-  const unsigned long long valids = __ballot(valid);
-  // my run's head: the highest head at or below this lane
-  const unsigned long long below = heads & ((lane == 63) ? ~0ull : ((2ull << lane) - 1ull));
-  const int my_head = valid ? 63 - __clzll((long long)below) : 0;
-  // run length: lanes from the head up to (not including) the next head or invalid lane
-  const unsigned long long after = (heads | ~valids) & ~((2ull << my_head) - 1ull);
-  const int next = after ? __ffsll((long long)after) - 1 : 64;
-  const int run = next - my_head;
-  int old = 0;
-  if (head) old = atomicAdd(counter_base + idx, delta * run);
-  old = __shfl(old, my_head, 64);
-  return old - (lane - my_head);
-}
-
 __global__ __launch_bounds__(kBlock) void k_bin_keys(RebuildArgs a) {
   const int lb = logical_block();
   const InstanceDesc& in = a.inst[a.block_inst[lb]];
   const int s = lb * kBlock + (int)threadIdx.x;
   const int local = s - in.slot_begin;
-  const bool valid = local < in.n;
-  int g = 0;
-  if (valid) {
-    const float4 p = a.x_old[s];
-    int c[3];
-    bin_coords_div(in, p.x, p.y, p.z, c);  // grid.hpp:394-395
-    const int key = flatten(in, c[0], c[1], c[2]);
-    a.keys[s] = key;
-    g = in.cbin_begin + (key >> in.shift);
-  } else if (local == in.n) {
-    // the pad segment holds the instance's pad slots, so the global scan lands on
-    // slot_begin of the next instance
-    a.cnt[in.cbin_begin + in.cbin_count - 1] = in.padded - in.n;
-  }
-  aggregated_atomic(a.cnt, g, valid, 1);
+  const bool cell = local < in.n;
+  count_key(in, a.keys, a.cnt, s, cell, local == in.n,
+            cell ? a.x_old[s] : make_float4(0.0f, 0.0f, 0.0f, 0.0f));
 }
 
 __global__ __launch_bounds__(kBlock) void k_scatter(RebuildArgs a) {
@@ -93,8 +61,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(RebuildArgs a) {
 }
 
 // Final slot = segment start + #(segment members before it in (bin, agent index) order).
-__global__ __launch_bounds__(kBlock) void k_gather(RebuildArgs a) {
-  const int lb = logical_block();
+__device__ __forceinline__ void gather_block(const RebuildArgs& a, int lb) {
   const InstanceDesc& in = a.inst[a.block_inst[lb]];
   const int t = lb * kBlock + (int)threadIdx.x;
   if ((t - in.slot_begin) >= in.n) return;
@@ -111,54 +78,79 @@ __global__ __launch_bounds__(kBlock) void k_gather(RebuildArgs a) {
   const int dst = start + rank;
   a.id_new[dst] = my_id;
   a.x_new[dst] = a.x_old[o];
-  if (a.pol_new) a.pol_new[dst] = a.pol_old[o];
+  if (a.pol_new) {
+    a.pol_new[dst] = a.pol_old[o];
+    a.trig_new[dst] = a.trig_old[o];
+  }
   if (a.ov_new) a.ov_new[dst] = a.ov_old[o];
-  if (a.fill) a.skey[dst] = key;
 }
 
-// Per-bin offsets from the segment offsets: off[b] = segment start + #(members with key < b).
-// One thread per 4 consecutive bins (instance bin ranges start on multiples of 64 entries, so
-// the 16-byte stores of a wavefront cover 1 KiB of consecutive, aligned offsets); the threads
-// of a segment share its two offset loads, and only non-empty segments read member keys.
-__global__ __launch_bounds__(kBlock) void k_fill_offsets(RebuildArgs a) {
-  const int blk = (int)blockIdx.x;
+// Per-bin offsets from the segment offsets: off[b] = segment start + #(members with key < b),
+// counted over the segment's members as the scatter left them (their order does not matter),
+// so the fill does not wait for the gather.
+// A thread fills kFillQuads quads of 4 consecutive bins, kBlock quads apart (instance bin
+// ranges start on multiples of 64 entries, so the 16-byte stores of a wavefront cover 1 KiB of
+// consecutive, aligned offsets); all segment-offset loads are issued before the first store,
+// and only non-empty segments read member keys.
+__device__ __forceinline__ void fill_block(const RebuildArgs& a, int blk) {
   const InstanceDesc& in = a.inst[a.fill_inst[blk]];
-  const int b0 = (a.fill_base[blk] + (int)threadIdx.x) * 4;  // local bin of the quad
-  if (b0 > in.bin_count) return;
   int* out = a.off + in.bin_begin;
-  int v[4];
-  if (in.shift >= 2 && b0 + 3 < in.bin_count) {
-    // the quad lies in one segment: one pair of segment offsets, member keys only if occupied
-    const int g = in.cbin_begin + (b0 >> in.shift);
-    const int start = a.coff[g], end = a.coff[g + 1];
-    int c0 = 0, c1 = 0, c2 = 0, c3 = 0;
-    for (int u = start; u < end; ++u) {
-      const int k = a.skey[u];
-      c0 += k < b0 ? 1 : 0;
-      c1 += k < b0 + 1 ? 1 : 0;
-      c2 += k < b0 + 2 ? 1 : 0;
-      c3 += k < b0 + 3 ? 1 : 0;
-    }
-    v[0] = start + c0;
-    v[1] = start + c1;
-    v[2] = start + c2;
-    v[3] = start + c3;
-  } else {
+  const bool seg_quads = in.shift >= 2;  // a quad never straddles two segments
+  int b0[kFillQuads], start[kFillQuads], end[kFillQuads];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int b = b0 + q;
-      if (b >= in.bin_count) {
-        v[q] = in.slot_begin + in.n;  // the pad bin (and the alignment gap after it)
-        continue;
-      }
-      const int g = in.cbin_begin + (b >> in.shift);
-      const int start = a.coff[g], end = a.coff[g + 1];
-      int less = 0;
-      for (int u = start; u < end; ++u) less += a.skey[u] < b ? 1 : 0;
-      v[q] = start + less;
+  for (int j = 0; j < kFillQuads; ++j) {
+    b0[j] = (a.fill_base[blk] + (int)threadIdx.x + j * kBlock) * 4;
+    start[j] = end[j] = 0;
+    if (seg_quads && b0[j] + 3 < in.bin_count) {
+      const int g = in.cbin_begin + (b0[j] >> in.shift);
+      start[j] = a.coff[g];
+      end[j] = a.coff[g + 1];
     }
   }
-  *reinterpret_cast<int4*>(out + b0) = make_int4(v[0], v[1], v[2], v[3]);
+#pragma unroll
+  for (int j = 0; j < kFillQuads; ++j) {
+    if (b0[j] > in.bin_count) continue;
+    int v[4];
+    if (seg_quads && b0[j] + 3 < in.bin_count) {
+      int c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+      for (int u = start[j]; u < end[j]; ++u) {
+        const int k = a.tkey[u];
+        c0 += k < b0[j] ? 1 : 0;
+        c1 += k < b0[j] + 1 ? 1 : 0;
+        c2 += k < b0[j] + 2 ? 1 : 0;
+        c3 += k < b0[j] + 3 ? 1 : 0;
+      }
+      v[0] = start[j] + c0;
+      v[1] = start[j] + c1;
+      v[2] = start[j] + c2;
+      v[3] = start[j] + c3;
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int b = b0[j] + q;
+        if (b >= in.bin_count) {
+          v[q] = in.slot_begin + in.n;  // the pad bin (and the alignment gap after it)
+          continue;
+        }
+        const int g = in.cbin_begin + (b >> in.shift);
+        const int st = a.coff[g], en = a.coff[g + 1];
+        int less = 0;
+        for (int u = st; u < en; ++u) less += a.tkey[u] < b ? 1 : 0;
+        v[q] = st + less;
+      }
+    }
+    *reinterpret_cast<int4*>(out + b0[j]) = make_int4(v[0], v[1], v[2], v[3]);
+  }
+}
+
+// Steps 4 and 5 in one launch: workgroups [0, n_gather) gather (XCD-aware order), the rest fill.
+__global__ __launch_bounds__(kBlock) void k_gather_fill(RebuildArgs a, int n_gather) {
+  const int p = (int)blockIdx.x;
+  if (p < n_gather) {
+    gather_block(a, remap_block(p, n_gather));
+  } else {
+    fill_block(a, p - n_gather);
+  }
 }
 
 // ------------------------------------------------------------------ exclusive scan (int)
@@ -225,6 +217,35 @@ __global__ __launch_bounds__(kBlock) void k_scan_top(int* bsum, int nt, int* tot
   if (threadIdx.x == 0) *total_out = total;
 }
 
+// Scans of at most this many tiles skip k_scan_top: every k_scan_apply_fused workgroup sums
+// the tile sums before its own tile (at most 4096 reads from L2 per workgroup).
+constexpr int kScanFusedTiles = 4096;
+
+__global__ __launch_bounds__(kBlock) void k_scan_apply_fused(const int* in, int n, const int* bsum,
+                                                             int nt, int* out) {
+  __shared__ int lds[4];
+  __shared__ int lds_p[4];
+  const int tile = (int)blockIdx.x;
+  int part = 0;
+  for (int i = (int)threadIdx.x; i < tile; i += kBlock) part += bsum[i];
+  int prefix;
+  block_exclusive_scan(part, lds_p, &prefix);
+  const int base = tile * kScanTile + threadIdx.x * kScanItems;
+  int v[kScanItems];
+  load_items(in, n, base, v);
+  int sum = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) sum += v[k];
+  int total;
+  int run = prefix + block_exclusive_scan(sum, lds, &total);
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    if (base + k < n) out[base + k] = run;
+    run += v[k];
+  }
+  if (tile == nt - 1 && threadIdx.x == 0) out[n] = prefix + total;
+}
+
 __global__ __launch_bounds__(kBlock) void k_scan_apply(const int* in, int n, const int* bsum,
                                                        int* out) {
   __shared__ int lds[4];
@@ -257,7 +278,8 @@ __global__ __launch_bounds__(kBlock) void k_init_ids(const InstanceDesc* inst,
 __global__ __launch_bounds__(kBlock) void k_upload(const InstanceDesc* inst, int ii,
                                                    int block_begin, const int* id,
                                                    const float* pos, const float* pol,
-                                                   const uint8_t* type, float4* x, float2* p) {
+                                                   const uint8_t* type, float4* x, float2* p,
+                                                   float4* trig) {
   const InstanceDesc& in = inst[ii];
   const int s = (block_begin + (int)blockIdx.x) * kBlock + (int)threadIdx.x;
   if ((s - in.slot_begin) >= in.n) return;
@@ -266,6 +288,7 @@ __global__ __launch_bounds__(kBlock) void k_upload(const InstanceDesc* inst, int
   if (type) w = __int_as_float((int)type[i]);
   x[s] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], w);
   if (pol) p[s] = make_float2(pol[2 * i], pol[2 * i + 1]);
+  if (trig) trig[s] = polar_trig(p[s]);
 }
 
 __global__ __launch_bounds__(kBlock) void k_download(const InstanceDesc* inst, int ii,
@@ -296,14 +319,11 @@ __global__ __launch_bounds__(kBlock) void k_download(const InstanceDesc* inst, i
   }
 }
 
-// sin/cos of both polarity angles per slot: the values sinf/cosf give where the force
-// evaluates them (to_vector3 polarity.hpp:111-118, dot :131-136, the sin theta of :153)
 __global__ __launch_bounds__(kBlock) void k_polar_trig(const float2* __restrict__ pol,
                                                        float4* __restrict__ trig, int n) {
   const int s = blockIdx.x * kBlock + threadIdx.x;
   if (s >= n) return;
-  const float2 q = pol[s];
-  trig[s] = make_float4(sinf(q.x), cosf(q.x), sinf(q.y), cosf(q.y));
+  trig[s] = polar_trig(pol[s]);
 }
 
 __global__ __launch_bounds__(kBlock) void k_copy_probe(const float4* __restrict__ src,
@@ -330,17 +350,20 @@ int scan_tiles_for(int n) { return (n + kScanTile - 1) / kScanTile; }
 void launch_exclusive_scan(const int* in, int n, int* out, int* bsum, hipStream_t stream) {
   const int nt = scan_tiles_for(n);
   hipLaunchKernelGGL(k_scan_reduce, dim3(nt), dim3(kBlock), 0, stream, in, n, bsum);
+  if (nt <= kScanFusedTiles) {
+    hipLaunchKernelGGL(k_scan_apply_fused, dim3(nt), dim3(kBlock), 0, stream, in, n, bsum, nt, out);
+    return;
+  }
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kBlock), 0, stream, bsum, nt, out + n);
   hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(kBlock), 0, stream, in, n, bsum, out);
 }
 
 void launch_rebuild(const RebuildArgs& a, int n_blocks, hipStream_t stream) {
-  hipLaunchKernelGGL(k_bin_keys, dim3(n_blocks), dim3(kBlock), 0, stream, a);
+  if (!a.keys_ready) hipLaunchKernelGGL(k_bin_keys, dim3(n_blocks), dim3(kBlock), 0, stream, a);
   launch_exclusive_scan(a.cnt, a.total_cbins, a.coff, a.scan_bsum, stream);
   hipLaunchKernelGGL(k_scatter, dim3(n_blocks), dim3(kBlock), 0, stream, a);
-  hipLaunchKernelGGL(k_gather, dim3(n_blocks), dim3(kBlock), 0, stream, a);
-  if (a.fill)
-    hipLaunchKernelGGL(k_fill_offsets, dim3(a.fill_blocks), dim3(kBlock), 0, stream, a);
+  hipLaunchKernelGGL(k_gather_fill, dim3(n_blocks + (a.fill ? a.fill_blocks : 0)), dim3(kBlock), 0,
+                     stream, a, n_blocks);
 }
 
 void launch_init_ids(const InstanceDesc* inst, const int* block_inst, int* id, int n_blocks,
@@ -350,9 +373,9 @@ void launch_init_ids(const InstanceDesc* inst, const int* block_inst, int* id, i
 
 void launch_upload(const InstanceDesc* inst, int ii, int block_begin, int n_blocks, const int* id,
                    const float* pos, const float* pol, const uint8_t* type, float4* x, float2* p,
-                   hipStream_t stream) {
+                   float4* trig, hipStream_t stream) {
   hipLaunchKernelGGL(k_upload, dim3(n_blocks), dim3(kBlock), 0, stream, inst, ii, block_begin, id,
-                     pos, pol, type, x, p);
+                     pos, pol, type, x, p, trig);
 }
 
 void launch_download(const InstanceDesc* inst, int ii, int block_begin, int n_blocks,

@@ -156,6 +156,7 @@ __device__ __forceinline__ void finish(const SweepArgs& a, const InstanceDesc& i
   G[0] = 0.0f + cs.acc_pol[0];
   G[1] = 0.0f + cs.acc_pol[1];
   const float dt = in.dt;
+  float4 xnew = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // Corr / Euler: the new position
   if (active) {
     if (MODE == kModeDelta) {
       a.o_pos[s] = make_float4(F[0], F[1], F[2], 0.0f);
@@ -166,7 +167,9 @@ __device__ __forceinline__ void finish(const SweepArgs& a, const InstanceDesc& i
       a.o2_pos[s] = make_float4(me.x + F[0] * dt, me.y + F[1] * dt, me.z + F[2] * dt, me.w);
       if (HAS_POL) {
         a.o_pol[s] = make_float2(G[0], G[1]);
-        a.o2_pol[s] = make_float2(mp.x + G[0] * dt, mp.y + G[1] * dt);
+        const float2 np = make_float2(mp.x + G[0] * dt, mp.y + G[1] * dt);
+        a.o2_pol[s] = np;
+        a.o_trig[s] = polar_trig(np);
       }
     } else if (MODE == kModeCorr) {
       // apply_heun_corrector, heun.hpp:170-186
@@ -174,21 +177,31 @@ __device__ __forceinline__ void finish(const SweepArgs& a, const InstanceDesc& i
       const float s0 = d1.x + F[0], s1 = d1.y + F[1], s2 = d1.z + F[2];
       if (a.old_vel) a.old_vel[s] = make_float4(s0 * 0.5f, s1 * 0.5f, s2 * 0.5f, 0.0f);
       const float4 x0 = a.o2_pos[s];
-      a.o2_pos[s] = make_float4(x0.x + s0 * 0.5f * dt, x0.y + s1 * 0.5f * dt,
-                                x0.z + s2 * 0.5f * dt, x0.w);
+      xnew = make_float4(x0.x + s0 * 0.5f * dt, x0.y + s1 * 0.5f * dt, x0.z + s2 * 0.5f * dt, x0.w);
+      a.o2_pos[s] = xnew;
       if (HAS_POL) {
         const float2 e1 = a.d1_pol[s];
         const float2 p0 = a.o2_pol[s];
-        a.o2_pol[s] = make_float2(p0.x + (e1.x + G[0]) * 0.5f * dt,
-                                  p0.y + (e1.y + G[1]) * 0.5f * dt);
+        const float2 np = make_float2(p0.x + (e1.x + G[0]) * 0.5f * dt,
+                                      p0.y + (e1.y + G[1]) * 0.5f * dt);
+        a.o2_pol[s] = np;
+        a.o_trig[s] = polar_trig(np);
       }
     } else if (MODE == kModeEuler) {
       // apply_euler, euler.hpp:52-68
       if (a.old_vel) a.old_vel[s] = make_float4(F[0], F[1], F[2], 0.0f);
-      a.o_pos[s] = make_float4(me.x + F[0] * dt, me.y + F[1] * dt, me.z + F[2] * dt, me.w);
-      if (HAS_POL) a.o_pol[s] = make_float2(mp.x + G[0] * dt, mp.y + G[1] * dt);
+      xnew = make_float4(me.x + F[0] * dt, me.y + F[1] * dt, me.z + F[2] * dt, me.w);
+      a.o_pos[s] = xnew;
+      if (HAS_POL) {
+        const float2 np = make_float2(mp.x + G[0] * dt, mp.y + G[1] * dt);
+        a.o_pol[s] = np;
+        a.o_trig[s] = polar_trig(np);
+      }
     }
   }
+  // keys of the new positions for the rebuild that starts the next step
+  if ((MODE == kModeCorr || MODE == kModeEuler) && a.keys)
+    count_key(in, a.keys, a.cnt, s, active, s - in.slot_begin == in.n, xnew);
   if (MODE == kModeDelta && a.com_partial) {
     const float f0 = active ? F[0] : 0.0f, f1 = active ? F[1] : 0.0f, f2 = active ? F[2] : 0.0f;
     const float t0 = block_sum_f(f0, lds_f);
@@ -371,7 +384,9 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
     a.xp[s] = make_float4(x.x + d1.x * dt, x.y + d1.y * dt, x.z + d1.z * dt, x.w);
     if (HAS_POL) {
       const float2 p = a.pol[s], e = a.d1_pol[s];
-      a.polp[s] = make_float2(p.x + e.x * dt, p.y + e.y * dt);
+      const float2 np = make_float2(p.x + e.x * dt, p.y + e.y * dt);
+      a.polp[s] = np;
+      a.trigp[s] = polar_trig(np);
     }
   } else if (KIND == kUpdateCorr) {
     // heun.hpp:174-183 (d1 was COM-fixed by the predictor; com here is the corrector's)
@@ -384,7 +399,9 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
     a.x[s] = make_float4(x.x + s0 * 0.5f * dt, x.y + s1 * 0.5f * dt, x.z + s2 * 0.5f * dt, x.w);
     if (HAS_POL) {
       const float2 p = a.pol[s], e1 = a.d1_pol[s], e3 = a.d3_pol[s];
-      a.pol[s] = make_float2(p.x + (e1.x + e3.x) * 0.5f * dt, p.y + (e1.y + e3.y) * 0.5f * dt);
+      const float2 np = make_float2(p.x + (e1.x + e3.x) * 0.5f * dt, p.y + (e1.y + e3.y) * 0.5f * dt);
+      a.pol[s] = np;
+      a.trig[s] = polar_trig(np);
     }
   } else {
     // euler.hpp:56-66
@@ -393,7 +410,9 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
     a.x[s] = make_float4(x.x + d1.x * dt, x.y + d1.y * dt, x.z + d1.z * dt, x.w);
     if (HAS_POL) {
       const float2 p = a.pol[s], e = a.d1_pol[s];
-      a.pol[s] = make_float2(p.x + e.x * dt, p.y + e.y * dt);
+      const float2 np = make_float2(p.x + e.x * dt, p.y + e.y * dt);
+      a.pol[s] = np;
+      a.trig[s] = polar_trig(np);
     }
   }
 }

@@ -412,6 +412,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
   G[0] = 0.0f + acc_pol[0];
   G[1] = 0.0f + acc_pol[1];
   const float dt = in.dt;
+  float4 xnew = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // Corr / Euler: the new position
+  float2 npol = make_float2(0.0f, 0.0f);               // Pred / Corr / Euler: the new polarity
   if (leader) {
     if (MODE == kModeDelta) {
       a.o_pos[s] = make_float4(F[0], F[1], F[2], 0.0f);
@@ -422,7 +424,9 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
       a.o2_pos[s] = make_float4(me.x + F[0] * dt, me.y + F[1] * dt, me.z + F[2] * dt, me.w);
       if (HAS_POL) {
         a.o_pol[s] = make_float2(G[0], G[1]);
-        a.o2_pol[s] = make_float2(mp.x + G[0] * dt, mp.y + G[1] * dt);
+        const float2 np = make_float2(mp.x + G[0] * dt, mp.y + G[1] * dt);
+        a.o2_pol[s] = np;
+        npol = np;
       }
     } else if (MODE == kModeCorr) {
       // heun.hpp:170-186
@@ -430,21 +434,42 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
       const float s0 = d1.x + F[0], s1 = d1.y + F[1], s2 = d1.z + F[2];
       if (a.old_vel) a.old_vel[s] = make_float4(s0 * 0.5f, s1 * 0.5f, s2 * 0.5f, 0.0f);
       const float4 x0 = a.o2_pos[s];
-      a.o2_pos[s] = make_float4(x0.x + s0 * 0.5f * dt, x0.y + s1 * 0.5f * dt,
-                                x0.z + s2 * 0.5f * dt, x0.w);
+      xnew = make_float4(x0.x + s0 * 0.5f * dt, x0.y + s1 * 0.5f * dt, x0.z + s2 * 0.5f * dt, x0.w);
+      a.o2_pos[s] = xnew;
       if (HAS_POL) {
         const float2 e1 = a.d1_pol[s];
         const float2 p0 = a.o2_pol[s];
-        a.o2_pol[s] = make_float2(p0.x + (e1.x + G[0]) * 0.5f * dt,
-                                  p0.y + (e1.y + G[1]) * 0.5f * dt);
+        const float2 np = make_float2(p0.x + (e1.x + G[0]) * 0.5f * dt,
+                                      p0.y + (e1.y + G[1]) * 0.5f * dt);
+        a.o2_pol[s] = np;
+        npol = np;
       }
     } else if (MODE == kModeEuler) {
       // euler.hpp:52-68
       if (a.old_vel) a.old_vel[s] = make_float4(F[0], F[1], F[2], 0.0f);
-      a.o_pos[s] = make_float4(me.x + F[0] * dt, me.y + F[1] * dt, me.z + F[2] * dt, me.w);
-      if (HAS_POL) a.o_pol[s] = make_float2(mp.x + G[0] * dt, mp.y + G[1] * dt);
+      xnew = make_float4(me.x + F[0] * dt, me.y + F[1] * dt, me.z + F[2] * dt, me.w);
+      a.o_pos[s] = xnew;
+      if (HAS_POL) {
+        const float2 np = make_float2(mp.x + G[0] * dt, mp.y + G[1] * dt);
+        a.o_pol[s] = np;
+        npol = np;
+      }
     }
   }
+  if (HAS_POL && (MODE == kModePred || MODE == kModeCorr || MODE == kModeEuler)) {
+    // polar_trig of the new polarity, split over the team: lane 0 takes theta, lane 1 phi
+    const int base_lane = (int)(threadIdx.x & 63) - lane;
+    const float th = __shfl(npol.x, base_lane, 64);
+    const float ph = __shfl(npol.y, base_lane, 64);
+    float sv, cv;
+    sincosf(lane == 1 ? ph : th, &sv, &cv);
+    const float sp = __shfl(sv, base_lane + 1, 64);
+    const float cp = __shfl(cv, base_lane + 1, 64);
+    if (leader) a.o_trig[s] = make_float4(sv, cv, sp, cp);
+  }
+  // keys of the new positions for the rebuild that starts the next step
+  if ((MODE == kModeCorr || MODE == kModeEuler) && a.keys)
+    count_key(in, a.keys, a.cnt, s, leader, lane == 0 && s - in.slot_begin == in.n, xnew);
   if (MODE == kModeDelta && a.com_partial) {
     const float t0 = tblock_sum_f(leader ? F[0] : 0.0f, lds_f);
     const float t1 = tblock_sum_f(leader ? F[1] : 0.0f, lds_f);

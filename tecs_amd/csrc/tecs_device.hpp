@@ -61,13 +61,13 @@ struct InstanceDesc {
 // Logical workgroup index with an XCD-aware remap: the hardware hands consecutive workgroup
 // ids to the 8 XCDs round-robin; remapping gives each XCD a contiguous run of logical blocks,
 // so spatially neighbouring slots (which share neighbour bins) meet in the same L2.
-__device__ __forceinline__ int logical_block() {
-  const int p = (int)blockIdx.x;
-  const int nb = (int)gridDim.x;
+// The map is a bijection on [0, nb) for any nb.
+__device__ __forceinline__ int remap_block(int p, int nb) {
   const int q = nb >> 3, r = nb & 7;
   const int xcd = p & 7, idx = p >> 3;
   return (xcd < r) ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
 }
+__device__ __forceinline__ int logical_block() { return remap_block((int)blockIdx.x, (int)gridDim.x); }
 
 // ---------------------------------------------------------------- bin arithmetic (grid.hpp)
 __device__ __forceinline__ int clampi(int v, int hi) { return max(0, min(v, hi)); }
@@ -89,6 +89,69 @@ __device__ __forceinline__ void bin_coords_mul(const InstanceDesc& in, float x, 
 // Grid::flatten_bin_index, grid.hpp:243-245.
 __device__ __forceinline__ int flatten(const InstanceDesc& in, int c0, int c1, int c2) {
   return c0 + c1 * in.ext[0] + c2 * (in.ext[0] * in.ext[1]);
+}
+
+// Wavefront-aggregated counter update: lanes holding the same counter index in a run of
+// consecutive lanes (cells arrive in the previous bin order, so runs are common) issue one
+// atomic for the run. Returns the counter's value before the update, minus the lane's rank in
+// the run, so that with delta = -1 every lane gets a distinct position (old - 1 - rank).
+struct LaneRun {
+  bool head;    // the lane issues the run's atomic
+  int my_head;  // lane of the run's head
+  int run;      // lanes in the run
+};
+__device__ __forceinline__ LaneRun lane_run(int idx, bool valid) {
+  const int lane = (int)(threadIdx.x & 63);
+  const int key = valid ? idx : -1;
+  const int prev = __shfl_up(key, 1, 64);
+  const bool head = valid && (lane == 0 || prev != key);
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long valids = __ballot(valid);
+  // my run's head: the highest head at or below this lane
+  const unsigned long long below = heads & ((lane == 63) ? ~0ull : ((2ull << lane) - 1ull));
+  const int my_head = valid ? 63 - __clzll((long long)below) : 0;
+  // run length: lanes from the head up to (not including) the next head or invalid lane
+  const unsigned long long after = (heads | ~valids) & ~((2ull << my_head) - 1ull);
+  const int next = after ? __ffsll((long long)after) - 1 : 64;
+  return LaneRun{head, my_head, next - my_head};
+}
+__device__ __forceinline__ int aggregated_atomic(int* counter_base, int idx, bool valid, int delta) {
+  const int lane = (int)(threadIdx.x & 63);
+  const LaneRun lr = lane_run(idx, valid);
+  const bool head = lr.head;
+  const int my_head = lr.my_head, run = lr.run;
+  int old = 0;
+  if (head) old = atomicAdd(counter_base + idx, delta * run);
+  old = __shfl(old, my_head, 64);
+  return old - (lane - my_head);
+}
+
+// Grid::rebuild's key of a position (grid.hpp:394-395) and its counting-sort segment.
+__device__ __forceinline__ int bin_key(const InstanceDesc& in, float4 p) {
+  int c[3];
+  bin_coords_div(in, p.x, p.y, p.z, c);
+  return flatten(in, c[0], c[1], c[2]);
+}
+
+// Step 1 of the rebuild (kernels_grid.hip) for one slot: store the key of the cell's position
+// and count it in its segment; the slot just past the instance's cells sets the pad segment's
+// count. Called by every lane of the wavefront (the count is wavefront-aggregated); `cell`
+// marks lanes that hold a cell, `pad` the lane of the slot just past the cells.
+__device__ __forceinline__ void count_key(const InstanceDesc& in, int* keys, int* cnt, int s,
+                                          bool cell, bool pad, float4 p) {
+  int g = 0;
+  if (cell) {
+    const int key = bin_key(in, p);
+    keys[s] = key;
+    g = in.cbin_begin + (key >> in.shift);
+  } else if (pad) {
+    // the pad segment holds the instance's pad slots, so the global scan lands on
+    // slot_begin of the next instance
+    cnt[in.cbin_begin + in.cbin_count - 1] = in.padded - in.n;
+  }
+  // counting needs no return value: the wavefront does not wait for the atomics
+  const LaneRun lr = lane_run(g, cell);
+  if (lr.head) __hip_atomic_fetch_add(cnt + g, lr.run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ------------------------------------------------------------------- vector.hpp:312-343
@@ -130,6 +193,16 @@ struct Pol {
   float4 trig;   // sin t, cos t, sin p, cos p
 };
 __device__ __forceinline__ Pol make_pol(float2 q, float4 trig) { return Pol{q.x, q.y, trig}; }
+// (sin theta, cos theta, sin phi, cos phi) of a polarity, from sincosf: the sines and cosines
+// the reference's force evaluates (to_vector3 polarity.hpp:111-118, dot :131-136, the
+// sin theta of :153). Every kernel that writes a polarity also writes this of it, so a
+// polarity buffer and its trig buffer always agree.
+__device__ __forceinline__ float4 polar_trig(float2 q) {
+  float st, ct, sp, cp;
+  sincosf(q.x, &st, &ct);
+  sincosf(q.y, &sp, &cp);
+  return make_float4(st, ct, sp, cp);
+}
 
 // One invocation of the fused pairwise force (see tecs_force_desc in tecs_amd.h), adding the
 // contributions to the accumulators in the reference's statement order.

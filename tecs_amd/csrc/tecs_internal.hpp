@@ -26,7 +26,7 @@ struct SweepArgs {
   const int* block_inst;
   const float4* xs;  // positions of the evaluated stage
   const float2* ps;  // polarities of the evaluated stage (may be null)
-  const float4* trig;  // sin/cos of the polarity angles of the evaluated stage (k_polar_trig)
+  const float4* trig;  // polar_trig of ps (kept current by every writer of a polarity buffer)
   const int* off;    // concatenated bin offsets (global slots)
   const int* id;     // slot -> agent index
   float4* o_pos;     // Delta: F; Pred: d1; Euler: new positions
@@ -35,6 +35,9 @@ struct SweepArgs {
   float2* o2_pol;
   const float4* d1_pos;  // Corr: first evaluation
   const float2* d1_pol;
+  float4* o_trig;        // Pred / Corr / Euler: polar_trig of the polarity written
+  int* keys;             // Corr / Euler, may be null: count_key of the new positions for the
+  int* cnt;              // next rebuild
   float4* old_vel;       // Corr / Euler, may be null
   float4* com_partial;   // Delta: per-block sums of F (may be null)
   unsigned long long* block_pairs;  // per-block pair-interaction counters (accumulated)
@@ -58,6 +61,8 @@ struct UpdateArgs {
   const float2* d3_pol;
   float4* xp;            // predicted positions (Heun predictor)
   float2* polp;
+  float4* trig;          // polar_trig of pol (corrector, Euler) ...
+  float4* trigp;         // ... and of polp (predictor)
   float4* old_vel;
 };
 enum UpdateKind { kUpdatePred = 0, kUpdateCorr = 1, kUpdateEuler = 2 };
@@ -71,6 +76,8 @@ void launch_com_finalize(const InstanceDesc* inst, int n_inst, const float4* par
 // as n_blocks * team_lanes workgroups; block_pairs and com_partial need
 // n_blocks * kMaxTeamLanes entries.
 constexpr int kMaxTeamLanes = 16;
+// Quads of 4 bins per thread of the per-bin offset fill.
+constexpr int kFillQuads = 4;
 void launch_sweep_team(int team_lanes, bool gabriel, bool has_type, bool has_pol, int mode,
                        const SweepArgs& a, int n_blocks, hipStream_t stream);
 
@@ -79,26 +86,29 @@ struct RebuildArgs {
   const int* block_inst;
   const float4* x_old;
   const float2* pol_old;
+  const float4* trig_old;
   const int* id_old;
   const float4* ov_old;
   float4* x_new;
   float2* pol_new;
+  float4* trig_new;
   int* id_new;
   float4* ov_new;
   int* keys;
   int* tmp;
   int* tkey;       // bin key of the cell scattered to each temporary slot
   int* tid;        // agent index of the cell scattered to each temporary slot
-  int* cnt;        // per-segment counts (zero between rebuilds)
+  int* cnt;        // per-segment counts (zero between rebuilds, or the counts of the current
+                   // positions when keys_ready)
+  bool keys_ready; // a sweep epilogue already stored keys and counted them (count_key)
   int* coff;       // per-segment offsets (global slots); aliases off when every shift is 0
   int* off;        // per-bin offsets (global slots)
-  int* skey;       // bin key of every slot in the new order (needed by the fill)
   int* scan_bsum;
   int total_cbins;  // segments of all instances (scan length)
   // per-bin fill (only when some instance counts segments of several bins)
   bool fill;
   const int* fill_inst;  // fill workgroup -> instance
-  const int* fill_base;  // fill workgroup -> first local 4-bin quad
+  const int* fill_base;  // fill workgroup -> first local 4-bin quad (kFillQuads per thread)
   int fill_blocks;
 };
 void launch_rebuild(const RebuildArgs& a, int n_blocks, hipStream_t stream);
@@ -109,12 +119,12 @@ void launch_init_ids(const InstanceDesc* inst, const int* block_inst, int* id, i
                      hipStream_t stream);
 void launch_upload(const InstanceDesc* inst, int ii, int block_begin, int n_blocks, const int* id,
                    const float* pos, const float* pol, const uint8_t* type, float4* x, float2* p,
-                   hipStream_t stream);
+                   float4* trig, hipStream_t stream);
 void launch_download(const InstanceDesc* inst, int ii, int block_begin, int n_blocks,
                      const int* id, const float4* x, const float2* p, const float4* ov,
                      float* pos, float* pol, float* old_vel, hipStream_t stream);
 
-// trig[s] = (sin theta, cos theta, sin phi, cos phi) of pol[s] for all slots of the batch
+// trig[s] = polar_trig(pol[s]) for all slots of the batch
 void launch_polar_trig(const float2* pol, float4* trig, int total_slots, hipStream_t stream);
 void launch_copy_probe(const float4* src, float4* dst, int64_t n, hipStream_t stream);
 
