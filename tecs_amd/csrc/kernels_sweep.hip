@@ -111,6 +111,95 @@ __device__ __forceinline__ void walk_binned(const SweepArgs& a, const InstanceDe
   }
 }
 
+// Accepted neighbours a thread of the deferred-force sweep keeps in LDS.
+constexpr int kDeferMax = 16;
+
+// The binned sweep with the force evaluations taken out of the candidate loop (search radius
+// 1, force modes). In the one-thread-per-cell walk, the lanes of a wavefront accept neighbours
+// at different iterations, so the force code runs in nearly every iteration with few lanes
+// active. Here
+//   1. each lane walks its candidates as walk_binned does and appends the accepted slots to
+//      its LDS list (at most kDeferMax; a lane whose list is full remembers where it stopped);
+//   2. the wavefront evaluates the forces list entry by list entry, every lane its own e-th
+//      pair, so the force code runs max(list length) times with most lanes active;
+//   3. a lane that stopped early walks the rest of its candidates with inline forces.
+// Each lane still adds its contributions in its own visiting order, so the sums are bitwise
+// those of walk_binned. Called by every lane of the wavefront (inactive lanes have no cell).
+template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE>
+__device__ __forceinline__ void walk_binned_deferred(const SweepArgs& a, const InstanceDesc& in,
+                                                     int s, bool active, float4 me,
+                                                     const Pol& pme, CellState& cs, int* list) {
+  const float4* __restrict__ xs = a.xs;
+  const int* __restrict__ offb = a.off + in.bin_begin;
+  const float r2 = in.cutoff2;
+  int na = 0;
+  int rs[9], re[9];
+  int stop_r = 9, stop_m = 0;  // resume point of a lane whose list filled up
+  if (active) {
+    int bc[3];
+    if (GABRIEL)
+      bin_coords_div(in, me.x, me.y, me.z, bc);  // binned_gabriel.hpp:101
+    else
+      bin_coords_mul(in, me.x, me.y, me.z, bc);  // binned_all_pairs.hpp:103-109
+    const int x_lo = max(bc[0] - 1, 0);
+    const int x_hi = min(bc[0] + 1, in.ext[0] - 1);
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+      const int z = bc[2] + r / 3 - 1, y = bc[1] + r % 3 - 1;
+      const bool ok = z >= 0 && z < in.ext[2] && y >= 0 && y < in.ext[1];
+      rs[r] = ok ? offb[flatten(in, x_lo, y, z)] : 0;
+      re[r] = ok ? offb[flatten(in, x_hi, y, z) + 1] : 0;
+    }
+    for (int r = 0; r < 9 && stop_r == 9; ++r) {
+      const int end = re[r];
+      for (int m = rs[r]; m < end; ++m) {
+        if (m == s) continue;
+        const float4 pj = xs[m];
+        const float disp[3] = {me.x - pj.x, me.y - pj.y, me.z - pj.z};
+        const float d2 = len2(disp[0], disp[1], disp[2]);
+        if (d2 >= r2) continue;
+        if (GABRIEL) {
+          if (gabriel_blocked(in, xs, offb, s, m, me, disp, d2)) continue;
+        }
+        if (na == kDeferMax) {
+          stop_r = r;
+          stop_m = m;
+          break;
+        }
+        list[na * kBlock] = m;
+        ++na;
+      }
+    }
+  }
+  // 2. forces, one list entry per lane per round
+  for (int e = 0; __ballot(e < na) != 0ull; ++e) {
+    if (e < na) {
+      const int m = list[e * kBlock];
+      const float4 pj = xs[m];
+      const float disp[3] = {me.x - pj.x, me.y - pj.y, me.z - pj.z};
+      const float d2 = len2(disp[0], disp[1], disp[2]);
+      visit<HAS_TYPE, HAS_POL, MODE>(a, in, cs, me, pme, m, pj, disp, d2);
+    }
+  }
+  // 3. the rest of a long walk, inline
+  if (stop_r < 9) {
+    for (int r = stop_r; r < 9; ++r) {
+      const int end = re[r];
+      for (int m = (r == stop_r ? stop_m : rs[r]); m < end; ++m) {
+        if (m == s) continue;
+        const float4 pj = xs[m];
+        const float disp[3] = {me.x - pj.x, me.y - pj.y, me.z - pj.z};
+        const float d2 = len2(disp[0], disp[1], disp[2]);
+        if (d2 >= r2) continue;
+        if (GABRIEL) {
+          if (gabriel_blocked(in, xs, offb, s, m, me, disp, d2)) continue;
+        }
+        visit<HAS_TYPE, HAS_POL, MODE>(a, in, cs, me, pme, m, pj, disp, d2);
+      }
+    }
+  }
+}
+
 // Deterministic block sum of one float per thread (fixed butterfly + fixed wave order).
 __device__ __forceinline__ float block_sum_f(float v, float* lds) {
 #pragma unroll
@@ -223,14 +312,22 @@ __global__ __launch_bounds__(kBlock) void k_sweep_binned(SweepArgs a) {
   CellState cs{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f}, 0u, 0};
   float4 me = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   float2 mp = make_float2(0.0f, 0.0f);
+  Pol pme{};
   if (active) {
     me = a.xs[s];
-    Pol pme{};
     if (HAS_POL) {
       mp = a.ps[s];
       pme = make_pol(mp, a.trig[s]);
     }
     if (MODE == kModeEmit) cs.emit_at = a.row_ptr[a.id[s]];
+  }
+  constexpr bool kForce = MODE != kModeCount && MODE != kModeEmit;
+  if (kForce && in.search_radius == 1) {
+    // the instance decides the path for the whole workgroup (all its slots are one instance)
+    __shared__ int list[kDeferMax * kBlock];
+    walk_binned_deferred<GABRIEL, HAS_TYPE, HAS_POL, MODE>(a, in, s, active, me, pme, cs,
+                                                           list + threadIdx.x);
+  } else if (active) {
     walk_binned<GABRIEL, HAS_TYPE, HAS_POL, MODE>(a, in, s, me, pme, cs);
   }
   finish<HAS_POL, MODE>(a, in, lb, s, active, me, mp, cs);
