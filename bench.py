@@ -26,6 +26,22 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def measured_traffic(workload: str, variant: str, phase: str):
+    """HBM bytes per launch of a sweep as rocprofv3 measured them: profiles/traffic.txt holds
+    FETCH_SIZE and WRITE_SIZE per dispatch, already scaled by the factors the same counters
+    read for the k_copy_probe streaming copy. None when the file has no line for this
+    workload, sweep variant and phase."""
+    path = os.path.join(ROOT, "profiles", "traffic.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split("#", 1)[0].split()
+            if len(parts) == 5 and parts[:3] == [workload, variant, phase]:
+                return {"fetch": int(parts[3]), "write": int(parts[4])}
+    return None
+
+
 def algorithmic_bytes_per_cell(has_pol: bool, has_type: bool):
     """Compulsory HBM bytes per cell of the two Heun sweeps: s = state bytes (position 12,
     + polarity 8), a = attribute bytes (type 1).
@@ -184,6 +200,7 @@ def bench_single(args, w):
     else:
         kernel_names = {"full": f"{kname} {fname} mode Pred (Heun substep 1)",
                         "stale": f"{kname} {fname} mode Corr (Heun substep 2, stale grid)"}
+    traffic = measured_traffic(w.name, variant, dom)
     print(
         f"bench: sweep full {per['full'][0]*1e3:.1f} us, stale {per['stale'][0]*1e3:.1f} us, "
         f"rebuild {reb_ms/max(reb_n,1)*1e3:.1f} us per launch",
@@ -218,7 +235,11 @@ def bench_single(args, w):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": None if traffic is None else traffic["fetch"] + traffic["write"],
+            "traffic_fetch": None if traffic is None else traffic["fetch"],
+            "traffic_write": None if traffic is None else traffic["write"],
+            "traffic_over_algorithmic": None if traffic is None
+            else (traffic["fetch"] + traffic["write"]) / (b_cell * w.n),
             "avg_launch_us": avg_ms * 1e3,
             "algorithmic_bytes_per_cell": b_cell,
             "sweep_full_us": per["full"][0] * 1e3,

@@ -150,6 +150,11 @@ int tecs_sim_download(tecs_sim* sim, float* positions, float* polarities, float*
  * Asynchronous on the simulation's stream. */
 int tecs_sim_take_step(tecs_sim* sim, double dt, int32_t n_steps);
 
+/* Replace the pairwise force used by the following steps (Simulation::take_step(dt, forces...),
+ * simulation.hpp:548-556, takes the force with every call). Same validation as create; the
+ * field set cannot change. Ordered with respect to the simulation's stream. */
+int tecs_sim_set_force(tecs_sim* sim, const tecs_force_desc* force);
+
 /* One pair-finder evaluation on the current state into zeroed deltas, exactly as
  * PairFinder::evaluate_force(in, out, ..., is_full_step) (binned_all_pairs.hpp:66-183):
  * a full evaluation rebuilds the grid when due and advances the rebuild counter.
@@ -230,6 +235,8 @@ int tecs_batch_download(tecs_batch* batch, int32_t index, float* positions, floa
                         float* old_velocities);
 /* dts: one dt per member. */
 int tecs_batch_take_step(tecs_batch* batch, const double* dts, int32_t n_steps);
+/* tecs_sim_set_force for member `index`. */
+int tecs_batch_set_force(tecs_batch* batch, int32_t index, const tecs_force_desc* force);
 int tecs_batch_synchronize(tecs_batch* batch);
 int tecs_batch_get_stats(tecs_batch* batch, tecs_stats* stats);
 int tecs_batch_reset_stats(tecs_batch* batch);

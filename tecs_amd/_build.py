@@ -81,3 +81,20 @@ def build(verbose: bool = False, jobs: int = 3) -> Path:
 
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+
+
+ADAPTER_SRC = PKG.parent / "tests" / "cpp" / "adapter_check.cpp"
+ADAPTER_BIN = PKG.parent / "tests" / "cpp" / "_build" / "adapter_check"
+
+
+def build_adapter_check() -> Path:
+    """The C++ adapter test driver (include/tecs_amd.hpp over the C ABI), linked against the
+    engine library with a relative run path so the tree can move."""
+    ADAPTER_BIN.parent.mkdir(parents=True, exist_ok=True)
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-I", str(PKG.parent / "include"),
+           str(ADAPTER_SRC), "-o", str(ADAPTER_BIN), "-L", str(OUT_DIR), "-ltecs_amd",
+           "-Wl,-rpath,$ORIGIN/../../../tecs_amd/_lib"]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    if r.returncode != 0:
+        raise RuntimeError("adapter check build failed:\n" + r.stdout.decode(errors="replace"))
+    return ADAPTER_BIN

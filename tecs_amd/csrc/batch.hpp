@@ -55,6 +55,15 @@ enum TimingClass { kTimeSweepFull = 0, kTimeSweepStale = 1, kTimeRebuild = 2, kT
 
 inline bool is_binned(int finder) { return finder == kBinnedAllPairs || finder == kBinnedGabriel; }
 
+inline int validate_force(const tecs_force_desc& f, uint32_t fields) {
+  if (f.n_types < 0 || f.n_types > TECS_MAX_TYPES) return fail(TECS_ERR_INVALID, "bad n_types");
+  if (f.n_types > 0 && !(fields & TECS_FIELD_TYPE))
+    return fail(TECS_ERR_INVALID, "type table needs TECS_FIELD_TYPE");
+  if ((f.bending_strength != 0.0f || f.alignment_strength != 0.0f) && !(fields & TECS_FIELD_POLARITY))
+    return fail(TECS_ERR_INVALID, "polarity forces need TECS_FIELD_POLARITY");
+  return TECS_OK;
+}
+
 inline int validate_desc(const tecs_sim_desc* d) {
   if (!d) return fail(TECS_ERR_INVALID, "null descriptor");
   if (d->capacity < d->agent_count) return fail(TECS_ERR_INVALID, "capacity < agent_count");
@@ -64,13 +73,7 @@ inline int validate_desc(const tecs_sim_desc* d) {
     return fail(TECS_ERR_INVALID, "bad integrator");
   if (d->com_fixer != TECS_NO_COM_FIXER && d->com_fixer != TECS_GLOBAL_COM_FIXER)
     return fail(TECS_ERR_INVALID, "bad com_fixer");
-  const tecs_force_desc& f = d->force;
-  if (f.n_types < 0 || f.n_types > TECS_MAX_TYPES) return fail(TECS_ERR_INVALID, "bad n_types");
-  if (f.n_types > 0 && !(d->fields & TECS_FIELD_TYPE))
-    return fail(TECS_ERR_INVALID, "type table needs TECS_FIELD_TYPE");
-  if ((f.bending_strength != 0.0f || f.alignment_strength != 0.0f) &&
-      !(d->fields & TECS_FIELD_POLARITY))
-    return fail(TECS_ERR_INVALID, "polarity forces need TECS_FIELD_POLARITY");
+  TRY(validate_force(d->force, d->fields));
   if (!(d->cutoff_distance > 0.0f) || !std::isfinite(d->cutoff_distance))
     return fail(TECS_ERR_INVALID, "cutoff_distance must be finite and > 0");
   if (is_binned(d->pair_finder)) {
@@ -389,14 +392,7 @@ struct Batch {
       }
       in.cbin_count = ((in.bin_count + (1 << in.shift) - 1) >> in.shift) + 1;
       in.dt = 0.0f;
-      ForceParams& f = in.force;
-      f.h = d.force.homeostatic_radius;
-      f.krep = d.force.repulsion_scale;
-      f.kadh = d.force.adhesion_scale;
-      f.n_types = d.force.n_types;
-      std::memcpy(f.table, d.force.adhesion_table, sizeof(f.table));
-      f.bend = d.force.bending_strength;
-      f.align = d.force.alignment_strength;
+      load_force(in.force, d.force);
       slot += in.padded;
       blocks += nb;
       // bin ranges start on multiples of 64 entries (aligned 16-byte stores in the fill) and
@@ -573,6 +569,26 @@ struct Batch {
     HIP_TRY(hipMemcpyAsync(d_inst, h_inst.data(), sizeof(InstanceDesc) * n_inst,
                            hipMemcpyHostToDevice, stream));
     inst_dirty = false;
+    return TECS_OK;
+  }
+  static void load_force(ForceParams& f, const tecs_force_desc& d) {
+    f.h = d.homeostatic_radius;
+    f.krep = d.repulsion_scale;
+    f.kadh = d.adhesion_scale;
+    f.n_types = d.n_types;
+    std::memcpy(f.table, d.adhesion_table, sizeof(f.table));
+    f.bend = d.bending_strength;
+    f.align = d.alignment_strength;
+  }
+  // The force of member k for the following steps. The kernels read it from the device
+  // instance table, which is refreshed before the next launch (captured graphs stay valid).
+  int set_force(int k, const tecs_force_desc* f) {
+    TRY(check_index(k));
+    if (!f) return fail(TECS_ERR_INVALID, "null force");
+    TRY(validate_force(*f, descs[k].fields));
+    descs[k].force = *f;
+    load_force(h_inst[k].force, *f);
+    inst_dirty = true;
     return TECS_OK;
   }
   void set_dt(int k, double dt) {
@@ -979,6 +995,7 @@ struct Batch {
 
   // One evaluation into the d1 scratch buffers, then out in agent order.
   int evaluate(bool is_full_step, float* dpos, float* dpol) {
+    TRY(sync_inst());
     TRY(maybe_rebuild(is_full_step));
     SweepArgs a = base_args(x[xc], pol[xc]);
     a.o_pos = d1;

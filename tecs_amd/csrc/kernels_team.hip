@@ -45,11 +45,14 @@ struct TeamLds {
   //   ent: Gabriel - its position (w = type bits) until the Gabriel tests are done, then
   //        the pair's position contribution; all-pairs - the contribution. w = 1 when the
   //        pair counts.
-  float4 ent[TB][AM];
-  int aslot[TB][AM];    // its slot
-  int abit[TG][AM];     // one-hot bit rx + 3 ry + 9 rz of its (stale) bin relative to
-                        // i's centre bin - 1 (r* in 0..2)
-  float2 cpol[TP][AM];  // polarity contribution
+  // Rows are padded by one entry: the teams of a wavefront read the same entry index of
+  // their own rows at once, and an unpadded row stride (a multiple of 128 bytes for ent and
+  // cpol) would put all of them in the same LDS banks.
+  float4 ent[TB][AM + 1];
+  int aslot[TB][AM + 1];    // its slot
+  int abit[TG][AM + 1];     // one-hot bit rx + 3 ry + 9 rz of its (stale) bin relative to
+                            // i's centre bin - 1 (r* in 0..2)
+  float2 cpol[TP][AM + 1];  // polarity contribution
   int row_start[TB][9];
   int row_len[TB][9];
   int row_b1[TG][9];    // first slot of the row's second bin (Gabriel)

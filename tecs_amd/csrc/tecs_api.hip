@@ -113,6 +113,11 @@ int tecs_sim_take_step(tecs_sim* sim, double dt, int32_t n_steps) {
   return sim->b.take_step(&dt, n_steps);
 }
 
+int tecs_sim_set_force(tecs_sim* sim, const tecs_force_desc* force) {
+  GUARD(sim);
+  return sim->b.set_force(0, force);
+}
+
 int tecs_sim_evaluate_force(tecs_sim* sim, int32_t is_full_step, float* dpos, float* dpol) {
   GUARD(sim);
   return sim->b.evaluate(is_full_step != 0, dpos, dpol);
@@ -240,6 +245,11 @@ int tecs_batch_take_step(tecs_batch* batch, const double* dts, int32_t n_steps) 
   GUARD(batch);
   if (!dts) return fail(TECS_ERR_INVALID, "null dt array");
   return batch->b.take_step(dts, n_steps);
+}
+
+int tecs_batch_set_force(tecs_batch* batch, int32_t index, const tecs_force_desc* force) {
+  GUARD(batch);
+  return batch->b.set_force(index, force);
 }
 
 int tecs_batch_synchronize(tecs_batch* batch) {

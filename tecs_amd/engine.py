@@ -77,6 +77,10 @@ class GpuSimulation:
     def take_step(self, dt: float, n_steps: int = 1):
         check(self._lib.tecs_sim_take_step(self._h, float(dt), int(n_steps)))
 
+    def set_force(self, force):
+        """The tecs_force_desc (abi.ForceDesc) of the following steps."""
+        check(self._lib.tecs_sim_set_force(self._h, C.byref(force)))
+
     def write(self, writer, time: float):
         """Simulation::write(time): asynchronous snapshot into an io.Writer."""
         check(self._lib.tecs_sim_write(self._h, writer.handle, float(time)))
@@ -194,6 +198,9 @@ class GpuBatch:
     def take_step(self, dts, n_steps: int = 1):
         dts = np.ascontiguousarray(np.broadcast_to(np.asarray(dts, np.float64), (len(self.descs),)))
         check(self._lib.tecs_batch_take_step(self._h, _ptr(dts), int(n_steps)))
+
+    def set_force(self, index: int, force):
+        check(self._lib.tecs_batch_set_force(self._h, int(index), C.byref(force)))
 
     def synchronize(self):
         check(self._lib.tecs_batch_synchronize(self._h))

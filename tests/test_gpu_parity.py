@@ -464,3 +464,33 @@ def test_long_run_ensemble_statistics(gpu, oracle_lib):
         assert abs(sg[k] - so[k]) <= max(0.01 * abs(so[k]), 0.01), k
     # the run went somewhere: the shell relaxed away from its initial spread
     assert abs(sg["radius_std"] - s0["radius_std"]) > 1e-3
+
+
+def test_set_force_between_steps(gpu, oracle_lib):
+    """take_step(dt, force) with a new force: 10 steps with PiecewiseLinear(0.8, 2, 1), then 10
+    with (0.7, 1.5, 0.5); bitwise the oracle run as two simulations with the two forces."""
+    w = _workload("relaxation", 3000, abi.BINNED_ALL_PAIRS)
+    print("case set_force relaxation n=3000", flush=True)
+    d2 = abi.copy_desc(w.desc)
+    d2.force.homeostatic_radius = 0.7
+    d2.force.repulsion_scale = 1.5
+    d2.force.adhesion_scale = 0.5
+    g = GpuSimulation(w.desc)
+    g.upload(w.positions, w.polarities, w.types)
+    g.take_step(w.dt, 10)
+    g.set_force(d2.force)
+    g.take_step(w.dt, 10)
+    o1 = oracle_lib.OracleSim(w.desc)
+    o1.upload(w.positions, w.polarities, w.types)
+    o1.take_step(w.dt, 10)
+    x1, _, v1 = o1.download()
+    o2 = oracle_lib.OracleSim(d2)
+    o2.upload(x1, None, None)
+    o2.take_step(w.dt, 10)
+    xg, _, _ = g.download()
+    xo, _, _ = o2.download()
+    assert np.array_equal(xg.view(np.uint32), xo.view(np.uint32))
+    bad = abi.copy_desc(w.desc).force
+    bad.bending_strength = 1.0  # needs the polarity field
+    with pytest.raises(Exception):
+        g.set_force(bad)
