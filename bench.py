@@ -89,6 +89,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=200_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--sweep-variant", default="auto", choices=["auto", "thread", "team4", "team8", "team16"],
+                    help="pin the pair-sweep kernel variant instead of autotuning it")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the streaming copy kernel used to calibrate HBM byte counters")
@@ -153,6 +155,8 @@ def bench_single(args, w):
     from tecs_amd import GpuSimulation, abi
 
     sim = GpuSimulation(w.desc)
+    if args.sweep_variant != "auto":
+        sim.set_sweep_variant({v: k for k, v in abi.SWEEP_NAMES.items()}[args.sweep_variant])
     sim.upload(w.positions, w.polarities, w.types)
     print(f"bench: {w.name} n={w.n} warmup={args.warmup} steps={args.steps}", file=sys.stderr, flush=True)
     sim.take_step(w.dt, args.warmup)

@@ -714,9 +714,10 @@ struct Batch {
     return trigp;
   }
 
-  // Autotuner: time every exact sweep variant on the live state (an evaluation into scratch
-  // buffers with scratch counters, so neither the state nor the statistics change) and keep
-  // the fastest. Runs once, after the first grid exists.
+  // Autotuner: time every exact sweep variant on the live state (the first sweep of a step,
+  // into buffers the step rewrites, with scratch counters, so neither the state nor the
+  // statistics change) and keep the fastest of 5 timed launches each. Runs once, after the
+  // first grid exists.
   int autotune() {
     if (tuned >= 0) return TECS_OK;
     if (!team_ok) {
@@ -730,15 +731,31 @@ struct Batch {
     double best = 1e30;
     int best_v = TECS_SWEEP_TEAM8;
     int rc = TECS_OK;
+    // time the sweep the steps start with, writing only buffers that step overwrites before
+    // reading them (Heun: d1 and the predicted state; Euler: the other state buffer)
+    const bool com_on = com_fixer == TECS_GLOBAL_COM_FIXER;
+    int mode = kModeDelta;
+    SweepArgs t = base_args(x[xc], pol[xc]);
+    t.o_pos = d1;
+    t.o_pol = d1p;
+    t.block_pairs = tune_pairs;
+    if (!com_on && integrator == TECS_HEUN) {
+      mode = kModePred;
+      t.o2_pos = xp;
+      t.o2_pol = polp;
+      t.o_trig = trigp;
+    } else if (!com_on) {
+      mode = kModeEuler;
+      t.o_pos = x[xc ^ 1];
+      t.o_pol = pol[xc ^ 1];
+      t.o_trig = trig_of(pol[xc ^ 1]);
+      t.old_vel = track_ov ? ov[xc ^ 1] : nullptr;
+    }
     for (int v : cands) {
-      SweepArgs t = base_args(x[xc], pol[xc]);
-      t.o_pos = d1;
-      t.o_pol = d1p;
-      t.block_pairs = tune_pairs;
       float ms_min = 1e30f;
-      for (int rep = 0; rep < 4 && rc == TECS_OK; ++rep) {
+      for (int rep = 0; rep < 6 && rc == TECS_OK; ++rep) {
         hipEventRecord(e0, stream);
-        launch_variant(v, kModeDelta, t);
+        launch_variant(v, mode, t);
         hipEventRecord(e1, stream);
         hipError_t e = hipEventSynchronize(e1);
         float ms = 0.0f;
