@@ -183,8 +183,11 @@ struct Batch {
   int* tkey = nullptr;
   int* tidb = nullptr;
   int* cnt = nullptr;
-  int* coff = nullptr;
-  int* off = nullptr;
+  int* coff[2] = {nullptr, nullptr};   // segment offsets of the order held in id[b]
+  int* off = nullptr;                  // per-bin offsets (sparse grids only)
+  int* skey[2] = {nullptr, nullptr};   // key of every slot of the order held in id[b]
+  int* head[2] = {nullptr, nullptr};   // incremental rebuild: arrival lists per segment
+  int* nxt = nullptr;
   int* d_fill_inst = nullptr;
   int* d_fill_base = nullptr;
   int* bsum = nullptr;
@@ -212,9 +215,12 @@ struct Batch {
   int xc = 0, ic = 0;  // current position/polarity/old-velocity buffer, current id buffer
   int step_count = 0;
   bool grid_built = false;
-  // the last step's final sweep stored the keys of x[xc] and counted them in cnt (count_key),
-  // so the rebuild that starts the next step skips k_bin_keys
+  // the last step's final sweep stored the keys of x[xc] and counted them (count_key), so the
+  // rebuild that starts the next step skips k_bin_keys
   bool keys_ready = false;
+  // the next rebuild derives the new order from the last one: a grid exists and the state has
+  // only moved by steps since (an upload or a member copy clears this)
+  bool inc_ok = false;
   tecs_stats stats{};
 
   bool timing = false;
@@ -234,9 +240,10 @@ struct Batch {
   int graph_mode = TECS_GRAPHS_AUTO;
   struct HostState {
     int xc, ic, step_count;
-    bool keys_ready;
+    bool keys_ready, inc_ok;
     bool operator==(const HostState& o) const {
-      return xc == o.xc && ic == o.ic && step_count == o.step_count && keys_ready == o.keys_ready;
+      return xc == o.xc && ic == o.ic && step_count == o.step_count &&
+             keys_ready == o.keys_ready && inc_ok == o.inc_ok;
     }
   };
   struct StepGraph {
@@ -255,7 +262,7 @@ struct Batch {
   hipStream_t side = nullptr;
   std::vector<unsigned> write_step_count;  // Simulation::current_step per member
 
-  HostState host_state() const { return HostState{xc, ic, step_count, keys_ready}; }
+  HostState host_state() const { return HostState{xc, ic, step_count, keys_ready, inc_ok}; }
   int graph_period() const { return 2 * std::max(1, is_binned(finder) ? rebuild_every_n : 1); }
 
   void drop_graphs() {
@@ -464,7 +471,14 @@ struct Batch {
       TRY(alloc(&tkey, S));
       TRY(alloc(&tidb, S));
       TRY(alloc(&cnt, (size_t)total_cbins));
-      TRY(alloc(&coff, (size_t)total_cbins + 1));
+      TRY(alloc(&nxt, S));
+      for (int b = 0; b < 2; ++b) {
+        TRY(alloc(&coff[b], (size_t)total_cbins + 1));
+        TRY(alloc(&skey[b], S));
+        TRY(alloc(&head[b], (size_t)total_cbins));
+        HIP_TRY(hipMemsetAsync(coff[b], 0, sizeof(int) * ((size_t)total_cbins + 1), stream));
+        HIP_TRY(hipMemsetAsync(head[b], 0xff, sizeof(int) * (size_t)total_cbins, stream));
+      }
       if (fill_bins) {
         TRY(alloc(&off, (size_t)total_bins + 1));
         TRY(alloc(&d_fill_inst, h_fill_inst.size()));
@@ -474,12 +488,9 @@ struct Batch {
         HIP_TRY(hipMemcpyAsync(d_fill_base, h_fill_base.data(), sizeof(int) * h_fill_base.size(),
                                hipMemcpyHostToDevice, stream));
         HIP_TRY(hipMemsetAsync(off, 0, sizeof(int) * ((size_t)total_bins + 1), stream));
-      } else {
-        off = coff;
       }
       TRY(alloc(&bsum, (size_t)scan_tiles_for(total_cbins)));
       HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(int) * (size_t)total_cbins, stream));
-      HIP_TRY(hipMemsetAsync(coff, 0, sizeof(int) * ((size_t)total_cbins + 1), stream));
     }
     // team sweep conditions: search radius 1 and, for Gabriel, c <= 0.95 (kernels_team.hip)
     team_ok = is_binned(finder);
@@ -616,12 +627,15 @@ struct Batch {
     r.trig_new = trig_of(pol[xc ^ 1]);
     r.id_new = id[ic ^ 1];
     r.ov_new = ov[xc ^ 1];
-    r.keys = keys;
+    r.kc = key_count();
     r.tmp = tmp;
     r.tkey = tkey;
     r.tid = tidb;
-    r.cnt = cnt;
-    r.coff = coff;
+    r.incremental = inc_ok;
+    r.coff_old = coff[ic];
+    r.coff = coff[ic ^ 1];
+    r.skey_new = skey[ic ^ 1];
+    r.head_clear = head[ic ^ 1];
     r.off = off;
     r.keys_ready = keys_ready;
     r.scan_bsum = bsum;
@@ -637,6 +651,7 @@ struct Batch {
     xc ^= 1;
     ic ^= 1;
     grid_built = true;
+    inc_ok = true;
     stats.rebuilds++;
     return TECS_OK;
   }
@@ -663,7 +678,7 @@ struct Batch {
     a.xs = xs;
     a.ps = ps;
     a.trig = trig_of(ps);
-    a.off = off;
+    a.off = fill_bins ? off : coff[ic];
     a.id = id[ic];
     a.block_pairs = block_pairs;
     return a;
@@ -698,11 +713,30 @@ struct Batch {
   // count the keys of the positions it writes.
   bool emit_keys() const { return is_binned(finder) && step_count >= rebuild_every_n; }
 
-  // Positions are about to change outside a step: counts of the old ones must not survive.
+  // What count_key writes for the positions of the current order: counts for a full rebuild,
+  // moves and arrivals for an incremental one.
+  KeyCount key_count() const {
+    KeyCount k{};
+    k.keys = keys;
+    k.cnt = cnt;
+    if (inc_ok) {
+      k.skey = skey[ic];
+      k.head = head[ic];
+      k.nxt = nxt;
+    }
+    return k;
+  }
+
+  // Positions are about to change outside a step: the next rebuild starts from scratch, with
+  // zero counts (cnt holds the last rebuild's counts, and possibly moves of this order) and
+  // empty arrival lists.
   int drop_keys() {
-    if (!keys_ready) return TECS_OK;
-    HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(int) * (size_t)total_cbins, stream));
+    if (is_binned(finder) && (keys_ready || inc_ok)) {
+      HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(int) * (size_t)total_cbins, stream));
+      if (keys_ready) HIP_TRY(hipMemsetAsync(head[ic], 0xff, sizeof(int) * (size_t)total_cbins, stream));
+    }
     keys_ready = false;
+    inc_ok = false;
     return TECS_OK;
   }
 
@@ -827,8 +861,7 @@ struct Batch {
         b.o_trig = trig_of(pol[xc]);
         const bool emit = emit_keys();
         if (emit) {
-          b.keys = keys;
-          b.cnt = cnt;
+          b.kc = key_count();
         }
         b.d1_pos = d1;
         b.d1_pol = d1p;
@@ -857,8 +890,7 @@ struct Batch {
         a.o_trig = trig_of(pol[xc ^ 1]);
         const bool emit = emit_keys();
         if (emit) {
-          a.keys = keys;
-          a.cnt = cnt;
+          a.kc = key_count();
         }
         a.old_vel = track_ov ? ov[xc ^ 1] : nullptr;
         TRY(sweep(kModeEuler, a, kTimeSweepFull));
@@ -1179,7 +1211,7 @@ struct Batch {
   static size_t estimate_bytes(const tecs_sim_desc& d) {
     const size_t slots = ((size_t)d.agent_count + kBlock - 1) / kBlock * kBlock + kBlock;
     const bool pol = (d.fields & TECS_FIELD_POLARITY) != 0;
-    size_t per_slot = 2 * 16 + 2 * 4 + 16 + 16 * 2 /* x, id, d1, ov */ + 5 * 4 /* keys.. */;
+    size_t per_slot = 2 * 16 + 2 * 4 + 16 + 16 * 2 /* x, id, d1, ov */ + 8 * 4 /* keys.. */;
     if (pol) per_slot += 2 * 8 + 8 + 2 * 16 /* pol, d1 pol, trig */;
     if (d.integrator == TECS_HEUN) per_slot += 16 + (pol ? 8 : 0);
     if (d.integrator == TECS_HEUN && d.com_fixer == TECS_GLOBAL_COM_FIXER) per_slot += 16 + (pol ? 8 : 0);
@@ -1191,7 +1223,7 @@ struct Batch {
       for (int k = 0; k < 3; ++k) b *= std::ceil((double)((s.max_bounds[k] - s.min_bounds[k]) / bin));
       bins = (size_t)b + 64;
     }
-    return slots * per_slot + bins * 12 + 2 * 4096;
+    return slots * per_slot + bins * 28 + 2 * 4096;
   }
 
   int get_stats(tecs_stats* out) {

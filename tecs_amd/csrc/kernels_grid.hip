@@ -34,10 +34,10 @@ __global__ __launch_bounds__(kBlock) void k_bin_keys(RebuildArgs a) {
   const int s = lb * kBlock + (int)threadIdx.x;
   const int local = s - in.slot_begin;
   const bool cell = local < in.n;
-  count_key(in, a.keys, a.cnt, s, cell, local == in.n,
-            cell ? a.x_old[s] : make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+  count_key(in, a.kc, s, cell, local == in.n, cell ? a.x_old[s] : make_float4(0.0f, 0.0f, 0.0f, 0.0f));
 }
 
+// Full rebuild, step 3: each cell takes a temporary slot in its segment's new range.
 __global__ __launch_bounds__(kBlock) void k_scatter(RebuildArgs a) {
   const int lb = logical_block();
   const InstanceDesc& in = a.inst[a.block_inst[lb]];
@@ -46,12 +46,12 @@ __global__ __launch_bounds__(kBlock) void k_scatter(RebuildArgs a) {
   const bool valid = local < in.n;
   int g = 0, key = 0;
   if (valid) {
-    key = a.keys[s];
+    key = a.kc.keys[s];
     g = in.cbin_begin + (key >> in.shift);
   } else if (local == in.n) {
-    a.cnt[in.cbin_begin + in.cbin_count - 1] = 0;
+    a.kc.cnt[in.cbin_begin + in.cbin_count - 1] = 0;
   }
-  const int old = aggregated_atomic(a.cnt, g, valid, -1);
+  const int old = aggregated_atomic(a.kc.cnt, g, valid, -1);
   if (valid) {
     const int pos = a.coff[g] + old - 1;
     a.tmp[pos] = s;
@@ -60,7 +60,23 @@ __global__ __launch_bounds__(kBlock) void k_scatter(RebuildArgs a) {
   }
 }
 
-// Final slot = segment start + #(segment members before it in (bin, agent index) order).
+__device__ __forceinline__ bool before(int ka, int ia, int kb, int ib) {
+  return ka < kb || (ka == kb && ia < ib);
+}
+
+__device__ __forceinline__ void move_cell(const RebuildArgs& a, int o, int dst, int id, int key) {
+  a.id_new[dst] = id;
+  a.skey_new[dst] = key;
+  a.x_new[dst] = a.x_old[o];
+  if (a.pol_new) {
+    a.pol_new[dst] = a.pol_old[o];
+    a.trig_new[dst] = a.trig_old[o];
+  }
+  if (a.ov_new) a.ov_new[dst] = a.ov_old[o];
+}
+
+// Full rebuild, step 4: final slot = segment start + #(segment members before it in (key,
+// agent index) order), the members being the cells the scatter put in the segment's range.
 __device__ __forceinline__ void gather_block(const RebuildArgs& a, int lb) {
   const InstanceDesc& in = a.inst[a.block_inst[lb]];
   const int t = lb * kBlock + (int)threadIdx.x;
@@ -70,28 +86,45 @@ __device__ __forceinline__ void gather_block(const RebuildArgs& a, int lb) {
   const int g = in.cbin_begin + (key >> in.shift);
   const int start = a.coff[g], end = a.coff[g + 1];
   int rank = 0;
-  for (int u = start; u < end; ++u) {
-    const int ku = a.tkey[u];
-    rank += (ku < key || (ku == key && a.tid[u] < my_id)) ? 1 : 0;
-  }
-  const int o = a.tmp[t];
-  const int dst = start + rank;
-  a.id_new[dst] = my_id;
-  a.x_new[dst] = a.x_old[o];
-  if (a.pol_new) {
-    a.pol_new[dst] = a.pol_old[o];
-    a.trig_new[dst] = a.trig_old[o];
-  }
-  if (a.ov_new) a.ov_new[dst] = a.ov_old[o];
+  for (int u = start; u < end; ++u) rank += before(a.tkey[u], a.tid[u], key, my_id) ? 1 : 0;
+  move_cell(a, a.tmp[t], start + rank, my_id, key);
 }
 
-// Per-bin offsets from the segment offsets: off[b] = segment start + #(members with key < b),
-// counted over the segment's members as the scatter left them (their order does not matter),
-// so the fill does not wait for the gather.
+// Incremental rebuild: the members of segment g are the cells of its range at the last
+// rebuild whose segment did not change, plus the cells on its arrival list. Calls f(key, id)
+// for each member (old slot order, so keys and ids are read from the current arrays).
+template <class F>
+__device__ __forceinline__ void for_members(const RebuildArgs& a, const InstanceDesc& in, int g,
+                                            int s0, int s1, int head, F f) {
+  for (int u = s0; u < s1; ++u) {
+    const int ku = a.kc.keys[u];
+    if (in.cbin_begin + (ku >> in.shift) == g) f(ku, a.id_old[u]);
+  }
+  for (int u = head; u >= 0; u = a.kc.nxt[u]) f(a.kc.keys[u], a.id_old[u]);
+}
+
+// Incremental rebuild, step 4 (no scatter): cells are read in the current order.
+__device__ __forceinline__ void gather_block_inc(const RebuildArgs& a, int lb) {
+  const InstanceDesc& in = a.inst[a.block_inst[lb]];
+  const int t = lb * kBlock + (int)threadIdx.x;
+  if ((t - in.slot_begin) >= in.n) return;
+  const int key = a.kc.keys[t];
+  const int my_id = a.id_old[t];
+  const int g = in.cbin_begin + (key >> in.shift);
+  const int s0 = a.coff_old[g], s1 = a.coff_old[g + 1], head = a.kc.head[g], dst0 = a.coff[g];
+  int rank = 0;
+  for_members(a, in, g, s0, s1, head, [&](int ku, int iu) { rank += before(ku, iu, key, my_id) ? 1 : 0; });
+  move_cell(a, t, dst0 + rank, my_id, key);
+}
+
+// Per-bin offsets from the segment offsets: off[b] = segment start + #(members with key < b).
+// Member order does not matter, so the fill reads the members as the scatter left them (full)
+// or as for_members lists them (incremental) and does not wait for the gather.
 // A thread fills kFillQuads quads of 4 consecutive bins, kBlock quads apart (instance bin
 // ranges start on multiples of 64 entries, so the 16-byte stores of a wavefront cover 1 KiB of
 // consecutive, aligned offsets); all segment-offset loads are issued before the first store,
 // and only non-empty segments read member keys.
+template <bool INC>
 __device__ __forceinline__ void fill_block(const RebuildArgs& a, int blk) {
   const InstanceDesc& in = a.inst[a.fill_inst[blk]];
   int* out = a.off + in.bin_begin;
@@ -107,18 +140,37 @@ __device__ __forceinline__ void fill_block(const RebuildArgs& a, int blk) {
       end[j] = a.coff[g + 1];
     }
   }
+  // incremental: the old range and arrivals, loaded only for occupied segments
+  int os0[kFillQuads], os1[kFillQuads], hd[kFillQuads];
+#pragma unroll
+  for (int j = 0; j < kFillQuads; ++j) {
+    os0[j] = os1[j] = 0;
+    hd[j] = -1;
+    if (INC && end[j] > start[j]) {
+      const int g = in.cbin_begin + (b0[j] >> in.shift);
+      os0[j] = a.coff_old[g];
+      os1[j] = a.coff_old[g + 1];
+      hd[j] = a.kc.head[g];
+    }
+  }
 #pragma unroll
   for (int j = 0; j < kFillQuads; ++j) {
     if (b0[j] > in.bin_count) continue;
     int v[4];
     if (seg_quads && b0[j] + 3 < in.bin_count) {
       int c0 = 0, c1 = 0, c2 = 0, c3 = 0;
-      for (int u = start[j]; u < end[j]; ++u) {
-        const int k = a.tkey[u];
-        c0 += k < b0[j] ? 1 : 0;
-        c1 += k < b0[j] + 1 ? 1 : 0;
-        c2 += k < b0[j] + 2 ? 1 : 0;
-        c3 += k < b0[j] + 3 ? 1 : 0;
+      const int bq = b0[j];
+      auto count = [&](int k, int) {
+        c0 += k < bq ? 1 : 0;
+        c1 += k < bq + 1 ? 1 : 0;
+        c2 += k < bq + 2 ? 1 : 0;
+        c3 += k < bq + 3 ? 1 : 0;
+      };
+      if (end[j] > start[j]) {
+        if (INC)
+          for_members(a, in, in.cbin_begin + (bq >> in.shift), os0[j], os1[j], hd[j], count);
+        else
+          for (int u = start[j]; u < end[j]; ++u) count(a.tkey[u], 0);
       }
       v[0] = start[j] + c0;
       v[1] = start[j] + c1;
@@ -135,7 +187,13 @@ __device__ __forceinline__ void fill_block(const RebuildArgs& a, int blk) {
         const int g = in.cbin_begin + (b >> in.shift);
         const int st = a.coff[g], en = a.coff[g + 1];
         int less = 0;
-        for (int u = st; u < en; ++u) less += a.tkey[u] < b ? 1 : 0;
+        if (en > st) {
+          if (INC)
+            for_members(a, in, g, a.coff_old[g], a.coff_old[g + 1], a.kc.head[g],
+                        [&](int k, int) { less += k < b ? 1 : 0; });
+          else
+            for (int u = st; u < en; ++u) less += a.tkey[u] < b ? 1 : 0;
+        }
         v[q] = st + less;
       }
     }
@@ -143,13 +201,24 @@ __device__ __forceinline__ void fill_block(const RebuildArgs& a, int blk) {
   }
 }
 
+// After a full rebuild the scatter has left cnt at zero; an incremental rebuild starts from the
+// counts of the last rebuild: cnt[g] = coff[g + 1] - coff[g].
+__global__ __launch_bounds__(kBlock) void k_counts_from_offsets(const int* coff, int n, int* cnt) {
+  const int g = (int)blockIdx.x * kBlock + (int)threadIdx.x;
+  if (g < n) cnt[g] = coff[g + 1] - coff[g];
+}
+
 // Steps 4 and 5 in one launch: workgroups [0, n_gather) gather (XCD-aware order), the rest fill.
+template <bool INC>
 __global__ __launch_bounds__(kBlock) void k_gather_fill(RebuildArgs a, int n_gather) {
   const int p = (int)blockIdx.x;
   if (p < n_gather) {
-    gather_block(a, remap_block(p, n_gather));
+    if (INC)
+      gather_block_inc(a, remap_block(p, n_gather));
+    else
+      gather_block(a, remap_block(p, n_gather));
   } else {
-    fill_block(a, p - n_gather);
+    fill_block<INC>(a, p - n_gather);
   }
 }
 
@@ -172,8 +241,8 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* lds_w, int* tota
   return wave_off + x - v;
 }
 
-// Loads 8 consecutive ints per thread; full tiles use two 16-byte loads.
-__device__ __forceinline__ void load_items(const int* in, int n, int base, int* v) {
+// Loads 8 consecutive ints per thread; full aligned tiles use two 16-byte loads.
+__device__ __forceinline__ void load8(const int* in, int n, int base, int* v) {
   if (base + kScanItems <= n && ((reinterpret_cast<uintptr_t>(in + base) & 15) == 0)) {
     const int4 a = *reinterpret_cast<const int4*>(in + base);
     const int4 b = *reinterpret_cast<const int4*>(in + base + 4);
@@ -185,11 +254,25 @@ __device__ __forceinline__ void load_items(const int* in, int n, int base, int* 
   }
 }
 
+// Sets the 8 entries of an item range to -1 (the arrival heads of the following rebuild).
+__device__ __forceinline__ void clear_items(int* head_clear, int n, int base) {
+  if (!head_clear) return;
+  if (base + kScanItems <= n && ((reinterpret_cast<uintptr_t>(head_clear + base) & 15) == 0)) {
+    const int4 m = make_int4(-1, -1, -1, -1);
+    *reinterpret_cast<int4*>(head_clear + base) = m;
+    *reinterpret_cast<int4*>(head_clear + base + 4) = m;
+  } else {
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+      if (base + k < n) head_clear[base + k] = -1;
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_scan_reduce(const int* in, int n, int* bsum) {
   __shared__ int lds[4];
   const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
   int items[kScanItems];
-  load_items(in, n, base, items);
+  load8(in, n, base, items);
   int v = 0;
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) v += items[k];
@@ -222,7 +305,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_top(int* bsum, int nt, int* tot
 constexpr int kScanFusedTiles = 4096;
 
 __global__ __launch_bounds__(kBlock) void k_scan_apply_fused(const int* in, int n, const int* bsum,
-                                                             int nt, int* out) {
+                                                             int nt, int* out, int* head_clear) {
   __shared__ int lds[4];
   __shared__ int lds_p[4];
   const int tile = (int)blockIdx.x;
@@ -232,7 +315,8 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply_fused(const int* in, int 
   block_exclusive_scan(part, lds_p, &prefix);
   const int base = tile * kScanTile + threadIdx.x * kScanItems;
   int v[kScanItems];
-  load_items(in, n, base, v);
+  load8(in, n, base, v);
+  clear_items(head_clear, n, base);
   int sum = 0;
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) sum += v[k];
@@ -246,12 +330,13 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply_fused(const int* in, int 
   if (tile == nt - 1 && threadIdx.x == 0) out[n] = prefix + total;
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan_apply(const int* in, int n, const int* bsum,
-                                                       int* out) {
+__global__ __launch_bounds__(kBlock) void k_scan_apply(const int* in, int n, const int* bsum, int* out,
+                                                       int* head_clear) {
   __shared__ int lds[4];
   const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
   int v[kScanItems];
-  load_items(in, n, base, v);
+  load8(in, n, base, v);
+  clear_items(head_clear, n, base);
   int sum = 0;
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) sum += v[k];
@@ -346,24 +431,33 @@ void launch_copy_probe(const float4* src, float4* dst, int64_t n, hipStream_t st
 
 int scan_tiles_for(int n) { return (n + kScanTile - 1) / kScanTile; }
 
-// out[0..n] = exclusive scan of in[0..n), out[n] = total. bsum needs scan_tiles_for(n) ints.
-void launch_exclusive_scan(const int* in, int n, int* out, int* bsum, hipStream_t stream) {
+// out[0..n] = exclusive scan of in[0..n), out[n] = total; head_clear (may be null) is set to -1
+// over the same range. bsum needs scan_tiles_for(n) ints.
+void launch_exclusive_scan(const int* in, int n, int* out, int* bsum, int* head_clear,
+                           hipStream_t stream) {
   const int nt = scan_tiles_for(n);
   hipLaunchKernelGGL(k_scan_reduce, dim3(nt), dim3(kBlock), 0, stream, in, n, bsum);
   if (nt <= kScanFusedTiles) {
-    hipLaunchKernelGGL(k_scan_apply_fused, dim3(nt), dim3(kBlock), 0, stream, in, n, bsum, nt, out);
+    hipLaunchKernelGGL(k_scan_apply_fused, dim3(nt), dim3(kBlock), 0, stream, in, n, bsum, nt, out,
+                       head_clear);
     return;
   }
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kBlock), 0, stream, bsum, nt, out + n);
-  hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(kBlock), 0, stream, in, n, bsum, out);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(kBlock), 0, stream, in, n, bsum, out, head_clear);
 }
 
 void launch_rebuild(const RebuildArgs& a, int n_blocks, hipStream_t stream) {
   if (!a.keys_ready) hipLaunchKernelGGL(k_bin_keys, dim3(n_blocks), dim3(kBlock), 0, stream, a);
-  launch_exclusive_scan(a.cnt, a.total_cbins, a.coff, a.scan_bsum, stream);
-  hipLaunchKernelGGL(k_scatter, dim3(n_blocks), dim3(kBlock), 0, stream, a);
-  hipLaunchKernelGGL(k_gather_fill, dim3(n_blocks + (a.fill ? a.fill_blocks : 0)), dim3(kBlock), 0,
-                     stream, a, n_blocks);
+  launch_exclusive_scan(a.kc.cnt, a.total_cbins, a.coff, a.scan_bsum, a.head_clear, stream);
+  const dim3 gf(n_blocks + (a.fill ? a.fill_blocks : 0));
+  if (a.incremental) {
+    hipLaunchKernelGGL(k_gather_fill<true>, gf, dim3(kBlock), 0, stream, a, n_blocks);
+  } else {
+    hipLaunchKernelGGL(k_scatter, dim3(n_blocks), dim3(kBlock), 0, stream, a);
+    hipLaunchKernelGGL(k_gather_fill<false>, gf, dim3(kBlock), 0, stream, a, n_blocks);
+    hipLaunchKernelGGL(k_counts_from_offsets, dim3((a.total_cbins + kBlock - 1) / kBlock),
+                       dim3(kBlock), 0, stream, a.coff, a.total_cbins, a.kc.cnt);
+  }
 }
 
 void launch_init_ids(const InstanceDesc* inst, const int* block_inst, int* id, int n_blocks,

@@ -224,11 +224,24 @@ __device__ __forceinline__ unsigned block_sum_u(unsigned v, unsigned* lds) {
   return r;
 }
 
+template <bool HAS_POL>
+__device__ __forceinline__ CorrIn load_corr(const SweepArgs& a, int s) {
+  CorrIn c{};
+  c.d1 = a.d1_pos[s];
+  c.x0 = a.o2_pos[s];
+  if (HAS_POL) {
+    c.e1 = a.d1_pol[s];
+    c.p0 = a.o2_pol[s];
+  }
+  return c;
+}
+
 // Epilogue shared by the binned and naive sweeps: turn the accumulators into the evaluation
 // result and apply the mode's integrator update.
 template <bool HAS_POL, int MODE>
 __device__ __forceinline__ void finish(const SweepArgs& a, const InstanceDesc& in, int lb, int s,
-                                       bool active, float4 me, float2 mp, CellState& cs) {
+                                       bool active, float4 me, float2 mp, CellState& cs,
+                                       const CorrIn& ci) {
   __shared__ float lds_f[4];
   __shared__ unsigned lds_u[4];
   if (MODE == kModeCount) {
@@ -262,15 +275,15 @@ __device__ __forceinline__ void finish(const SweepArgs& a, const InstanceDesc& i
       }
     } else if (MODE == kModeCorr) {
       // apply_heun_corrector, heun.hpp:170-186
-      const float4 d1 = a.d1_pos[s];
+      const float4 d1 = ci.d1;
       const float s0 = d1.x + F[0], s1 = d1.y + F[1], s2 = d1.z + F[2];
       if (a.old_vel) a.old_vel[s] = make_float4(s0 * 0.5f, s1 * 0.5f, s2 * 0.5f, 0.0f);
-      const float4 x0 = a.o2_pos[s];
+      const float4 x0 = ci.x0;
       xnew = make_float4(x0.x + s0 * 0.5f * dt, x0.y + s1 * 0.5f * dt, x0.z + s2 * 0.5f * dt, x0.w);
       a.o2_pos[s] = xnew;
       if (HAS_POL) {
-        const float2 e1 = a.d1_pol[s];
-        const float2 p0 = a.o2_pol[s];
+        const float2 e1 = ci.e1;
+        const float2 p0 = ci.p0;
         const float2 np = make_float2(p0.x + (e1.x + G[0]) * 0.5f * dt,
                                       p0.y + (e1.y + G[1]) * 0.5f * dt);
         a.o2_pol[s] = np;
@@ -289,8 +302,8 @@ __device__ __forceinline__ void finish(const SweepArgs& a, const InstanceDesc& i
     }
   }
   // keys of the new positions for the rebuild that starts the next step
-  if ((MODE == kModeCorr || MODE == kModeEuler) && a.keys)
-    count_key(in, a.keys, a.cnt, s, active, s - in.slot_begin == in.n, xnew);
+  if ((MODE == kModeCorr || MODE == kModeEuler) && a.kc.keys)
+    count_key(in, a.kc, s, active, s - in.slot_begin == in.n, xnew);
   if (MODE == kModeDelta && a.com_partial) {
     const float f0 = active ? F[0] : 0.0f, f1 = active ? F[1] : 0.0f, f2 = active ? F[2] : 0.0f;
     const float t0 = block_sum_f(f0, lds_f);
@@ -313,6 +326,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_binned(SweepArgs a) {
   float4 me = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   float2 mp = make_float2(0.0f, 0.0f);
   Pol pme{};
+  CorrIn ci{};
   if (active) {
     me = a.xs[s];
     if (HAS_POL) {
@@ -320,6 +334,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_binned(SweepArgs a) {
       pme = make_pol(mp, a.trig[s]);
     }
     if (MODE == kModeEmit) cs.emit_at = a.row_ptr[a.id[s]];
+    if (MODE == kModeCorr) ci = load_corr<HAS_POL>(a, s);
   }
   constexpr bool kForce = MODE != kModeCount && MODE != kModeEmit;
   if (kForce && in.search_radius == 1) {
@@ -330,7 +345,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_binned(SweepArgs a) {
   } else if (active) {
     walk_binned<GABRIEL, HAS_TYPE, HAS_POL, MODE>(a, in, s, me, pme, cs);
   }
-  finish<HAS_POL, MODE>(a, in, lb, s, active, me, mp, cs);
+  finish<HAS_POL, MODE>(a, in, lb, s, active, me, mp, cs, ci);
 }
 
 // Naive finders (naive_all_pairs.hpp:41-103, naive_gabriel.hpp:47-119): j runs over all agents
@@ -348,6 +363,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_naive(SweepArgs a) {
   float4 me = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   float2 mp = make_float2(0.0f, 0.0f);
   Pol pme{};
+  CorrIn ci{};
   if (active) {
     me = a.xs[s];
     if (HAS_POL) {
@@ -355,6 +371,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_naive(SweepArgs a) {
       pme = make_pol(mp, a.trig[s]);
     }
     if (MODE == kModeEmit) cs.emit_at = a.row_ptr[a.id[s]];
+    if (MODE == kModeCorr) ci = load_corr<HAS_POL>(a, s);
   }
   const int begin = in.slot_begin, n = in.n;
   const float r2 = in.cutoff2;
@@ -391,7 +408,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_naive(SweepArgs a) {
       visit<HAS_TYPE, HAS_POL, MODE>(a, in, cs, me, pme, m, pj, disp, d2);
     }
   }
-  finish<HAS_POL, MODE>(a, in, lb, s, active, me, mp, cs);
+  finish<HAS_POL, MODE>(a, in, lb, s, active, me, mp, cs, ci);
 }
 
 template <bool GABRIEL, bool BINNED, bool HAS_TYPE, bool HAS_POL, int MODE>

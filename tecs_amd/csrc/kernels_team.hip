@@ -471,8 +471,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
     if (leader) a.o_trig[s] = make_float4(sv, cv, sp, cp);
   }
   // keys of the new positions for the rebuild that starts the next step
-  if ((MODE == kModeCorr || MODE == kModeEuler) && a.keys)
-    count_key(in, a.keys, a.cnt, s, leader, lane == 0 && s - in.slot_begin == in.n, xnew);
+  if ((MODE == kModeCorr || MODE == kModeEuler) && a.kc.keys)
+    count_key(in, a.kc, s, leader, lane == 0 && s - in.slot_begin == in.n, xnew);
   if (MODE == kModeDelta && a.com_partial) {
     const float t0 = tblock_sum_f(leader ? F[0] : 0.0f, lds_f);
     const float t1 = tblock_sum_f(leader ? F[1] : 0.0f, lds_f);

@@ -133,25 +133,52 @@ __device__ __forceinline__ int bin_key(const InstanceDesc& in, float4 p) {
   return flatten(in, c[0], c[1], c[2]);
 }
 
+// Step 1 of the rebuild (count_key): the key of every cell and its count per segment. Full
+// rebuild: cnt starts at zero and every cell counts. Incremental rebuild: cnt holds the counts
+// of the last rebuild and only a cell whose segment changed moves its count (old segment - 1,
+// new segment + 1) and joins the new segment's arrival list head[] / nxt[].
+struct KeyCount {
+  int* keys;        // key per slot (current order)
+  int* cnt;         // per-segment counts
+  const int* skey;  // incremental: key per slot at the last rebuild (null = full)
+  int* head;
+  int* nxt;
+};
+
 // Step 1 of the rebuild (kernels_grid.hip) for one slot: store the key of the cell's position
-// and count it in its segment; the slot just past the instance's cells sets the pad segment's
-// count. Called by every lane of the wavefront (the count is wavefront-aggregated); `cell`
-// marks lanes that hold a cell, `pad` the lane of the slot just past the cells.
-__device__ __forceinline__ void count_key(const InstanceDesc& in, int* keys, int* cnt, int s,
+// and count it. Full rebuild: every cell counts in its segment (wavefront-aggregated atomics)
+// and the slot just past the instance's cells sets the pad segment's count. Incremental
+// rebuild: only a cell whose segment differs from its segment at the last rebuild moves its
+// count and joins its new segment's arrival list. Called by every lane of the wavefront;
+// `cell` marks lanes that hold a cell, `pad` the lane of the slot just past the cells. No
+// atomic here returns a value the wavefront waits for, except the arrival-list exchange.
+__device__ __forceinline__ void count_key(const InstanceDesc& in, const KeyCount& k, int s,
                                           bool cell, bool pad, float4 p) {
+  if (k.skey) {
+    if (!cell) return;
+    const int key = bin_key(in, p);
+    k.keys[s] = key;
+    const int gn = in.cbin_begin + (key >> in.shift);
+    const int go = in.cbin_begin + (k.skey[s] >> in.shift);
+    if (gn != go) {
+      __hip_atomic_fetch_add(k.cnt + go, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(k.cnt + gn, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      k.nxt[s] = atomicExch(k.head + gn, s);
+    }
+    return;
+  }
   int g = 0;
   if (cell) {
     const int key = bin_key(in, p);
-    keys[s] = key;
+    k.keys[s] = key;
     g = in.cbin_begin + (key >> in.shift);
   } else if (pad) {
     // the pad segment holds the instance's pad slots, so the global scan lands on
     // slot_begin of the next instance
-    cnt[in.cbin_begin + in.cbin_count - 1] = in.padded - in.n;
+    k.cnt[in.cbin_begin + in.cbin_count - 1] = in.padded - in.n;
   }
-  // counting needs no return value: the wavefront does not wait for the atomics
   const LaneRun lr = lane_run(g, cell);
-  if (lr.head) __hip_atomic_fetch_add(cnt + g, lr.run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (lr.head) __hip_atomic_fetch_add(k.cnt + g, lr.run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ------------------------------------------------------------------- vector.hpp:312-343
@@ -188,6 +215,16 @@ __device__ __forceinline__ void piecewise_linear(const float* disp, float d, flo
 // ---------------------------------------------------------------- polarity.hpp:111-198
 // Polarity of a cell with the sines and cosines of its angles: trig = (sin theta, cos theta,
 // sin phi, cos phi), computed once per substep per slot (k_polar_trig).
+// The Heun corrector's per-cell inputs besides the evaluation: the first evaluation d1 and the
+// stage-0 state it updates in place. The sweeps load them before walking the neighbours, so
+// the loads overlap the walk instead of stalling the epilogue.
+struct CorrIn {
+  float4 d1;
+  float4 x0;
+  float2 e1;
+  float2 p0;
+};
+
 struct Pol {
   float t, p;    // theta, phi
   float4 trig;   // sin t, cos t, sin p, cos p

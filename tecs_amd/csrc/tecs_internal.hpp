@@ -36,8 +36,8 @@ struct SweepArgs {
   const float4* d1_pos;  // Corr: first evaluation
   const float2* d1_pol;
   float4* o_trig;        // Pred / Corr / Euler: polar_trig of the polarity written
-  int* keys;             // Corr / Euler, may be null: count_key of the new positions for the
-  int* cnt;              // next rebuild
+  KeyCount kc;           // Corr / Euler: count_key of the new positions for the next rebuild
+                         // (kc.keys null = not this step)
   float4* old_vel;       // Corr / Euler, may be null
   float4* com_partial;   // Delta: per-block sums of F (may be null)
   unsigned long long* block_pairs;  // per-block pair-interaction counters (accumulated)
@@ -94,15 +94,18 @@ struct RebuildArgs {
   float4* trig_new;
   int* id_new;
   float4* ov_new;
-  int* keys;
+  KeyCount kc;     // keys, counts and (incremental) moves of the cells in the current order
   int* tmp;
   int* tkey;       // bin key of the cell scattered to each temporary slot
   int* tid;        // agent index of the cell scattered to each temporary slot
-  int* cnt;        // per-segment counts (zero between rebuilds, or the counts of the current
-                   // positions when keys_ready)
-  bool keys_ready; // a sweep epilogue already stored keys and counted them (count_key)
-  int* coff;       // per-segment offsets (global slots); aliases off when every shift is 0
-  int* off;        // per-bin offsets (global slots)
+  bool keys_ready; // a sweep epilogue already ran count_key for these positions
+  bool incremental;     // derive the new order from the last one (the moved counts and the
+                        // arrival lists of kc) instead of counting and scattering every cell
+  const int* coff_old;  // segment offsets of the last rebuild (global slots)
+  int* coff;            // segment offsets being built; the per-bin offsets when every shift is 0
+  int* skey_new;        // key of every slot in the new order (the next rebuild's kc.skey)
+  int* head_clear;      // arrival heads of the following rebuild, reset to -1 by the scan
+  int* off;             // per-bin offsets (global slots) when some shift > 0
   int* scan_bsum;
   int total_cbins;  // segments of all instances (scan length)
   // per-bin fill (only when some instance counts segments of several bins)
@@ -113,7 +116,8 @@ struct RebuildArgs {
 };
 void launch_rebuild(const RebuildArgs& a, int n_blocks, hipStream_t stream);
 int scan_tiles_for(int n);
-void launch_exclusive_scan(const int* in, int n, int* out, int* bsum, hipStream_t stream);
+void launch_exclusive_scan(const int* in, int n, int* out, int* bsum, int* head_clear,
+                           hipStream_t stream);
 
 void launch_init_ids(const InstanceDesc* inst, const int* block_inst, int* id, int n_blocks,
                      hipStream_t stream);
