@@ -374,6 +374,11 @@ struct Batch {
         // binned_all_pairs.hpp:36-38, grid.hpp:201-215
         in.bin_size = ps.bin_size_scale * cutoff;
         in.inv_bin = 1.0f / in.bin_size;
+        {
+          int e = 0;
+          const float m = std::frexp(in.bin_size, &e);  // bin = m * 2^e, m in [0.5, 1)
+          in.bin_pow2 = (m == 0.5f && std::isfinite(in.inv_bin) && in.inv_bin * in.bin_size == 1.0f) ? 1 : 0;
+        }
         in.search_radius = (int)std::ceil(cutoff * in.inv_bin);
         int bc = 1;
         for (int q = 0; q < 3; ++q) {

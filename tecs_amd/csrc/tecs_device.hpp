@@ -54,6 +54,7 @@ struct InstanceDesc {
   float gabriel_factor;  // c*c*0.25 (binned_gabriel.hpp:42)
   float gabriel_c2;      // c*c     (naive_gabriel.hpp:392)
   int search_radius;
+  int bin_pow2;          // bin_size is a power of two: (p - min) / bin == (p - min) * inv_bin
   float dt;              // float(dt) of the current step
   ForceParams force;
 };
@@ -73,8 +74,16 @@ __device__ __forceinline__ int logical_block() { return remap_block((int)blockId
 __device__ __forceinline__ int clampi(int v, int hi) { return max(0, min(v, hi)); }
 
 // Grid::calc_bin_coords_from_point, grid.hpp:222-230: (p - min) / bin, truncation, clamp.
+// Dividing by a power of two and multiplying by its exact reciprocal round the same exact
+// value, so the division is replaced by the product when bin_pow2 is set.
 __device__ __forceinline__ void bin_coords_div(const InstanceDesc& in, float x, float y, float z,
                                                int* c) {
+  if (in.bin_pow2) {
+    c[0] = clampi((int)((x - in.minb[0]) * in.inv_bin), in.ext[0] - 1);
+    c[1] = clampi((int)((y - in.minb[1]) * in.inv_bin), in.ext[1] - 1);
+    c[2] = clampi((int)((z - in.minb[2]) * in.inv_bin), in.ext[2] - 1);
+    return;
+  }
   c[0] = clampi((int)((x - in.minb[0]) / in.bin_size), in.ext[0] - 1);
   c[1] = clampi((int)((y - in.minb[1]) / in.bin_size), in.ext[1] - 1);
   c[2] = clampi((int)((z - in.minb[2]) / in.bin_size), in.ext[2] - 1);

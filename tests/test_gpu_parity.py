@@ -269,10 +269,11 @@ def test_team_variant_requires_search_radius_one(gpu):
     assert g.sweep_variant() == abi.SWEEP_THREAD
 
 
-@pytest.mark.parametrize("scale", [0.5, 2.0])
+@pytest.mark.parametrize("scale", [0.5, 2.0, 1.3])
 @pytest.mark.parametrize("finder", [abi.BINNED_ALL_PAIRS, abi.BINNED_GABRIEL], ids=lambda f: abi.FINDER_NAMES[f])
 def test_bin_size_scale_bit_exact(gpu, oracle_lib, finder, scale):
-    """Bins of half / twice the cutoff: search radius 2 (thread sweep) and 1 (team sweep)."""
+    """Bins of half / twice / 1.3 times the cutoff: search radius 2 (thread sweep) and 1 (team
+    sweep); 1.3 is not a power of two, so the Gabriel bin coordinates take the division path."""
     w = _workload("relaxation", 3000, finder)
     d = _with(w.desc, bin_size_scale=scale)
     print(f"case bin_size_scale={scale} finder={abi.FINDER_NAMES[finder]}", flush=True)
@@ -494,3 +495,21 @@ def test_set_force_between_steps(gpu, oracle_lib):
     bad.bending_strength = 1.0  # needs the polarity field
     with pytest.raises(Exception):
         g.set_force(bad)
+
+
+def test_upload_mid_run_restarts_the_grid(gpu, oracle_lib):
+    """Steps rebuild incrementally from the last order; an upload of another state in the middle
+    of a run makes the next rebuild start from scratch. 30 steps, upload, 30 steps: bitwise the
+    oracle doing the same."""
+    w = _workload("sorting", 20000, abi.BINNED_ALL_PAIRS)
+    w2 = W.sorting(20000, seed_stream=77)
+    print("case upload mid-run sorting n=20000", flush=True)
+    g, o = _pair(oracle_lib, w.desc, w)
+    for s in (g, o):
+        s.take_step(w.dt, 30)
+        s.upload(w2.positions, w2.polarities, w2.types)
+        s.take_step(w.dt, 30)
+    xg, _, _ = g.download()
+    xo, _, _ = o.download()
+    assert np.array_equal(xg.view(np.uint32), xo.view(np.uint32))
+    assert g.stats()["pair_interactions"] == o.stats()["pair_interactions"]
