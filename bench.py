@@ -411,23 +411,31 @@ def bench_serve(args):
         plain_pairs = 0
         t1 = time.perf_counter()
         lat_plain = []
+        by_model = {}
         for r in reqs:
             now = time.perf_counter() - t1
             if r.arrival > now:
                 time.sleep(r.arrival - now)
+            t_req = time.perf_counter()
             s = GpuSimulation(r.workload.desc)
             s.upload(r.workload.positions, r.workload.polarities, r.workload.types)
             s.take_step(r.workload.dt, r.n_steps)
             s.synchronize()
             s.download()
-            plain_pairs += s.stats()["pair_interactions"]
+            pairs_r = s.stats()["pair_interactions"]
+            plain_pairs += pairs_r
             lat_plain.append(time.perf_counter() - t1 - r.arrival)
             s.close()
+            m = by_model.setdefault(r.model, {"requests": 0, "seconds": 0.0, "pair_interactions": 0})
+            m["requests"] += 1
+            m["seconds"] += time.perf_counter() - t_req
+            m["pair_interactions"] += pairs_r
         plain_span = time.perf_counter() - t1
         q50, q99 = _percentiles(lat_plain)
         out["config"].update({"plain_value": plain_pairs / plain_span, "plain_makespan_s": plain_span,
                               "plain_latency_p50_s": q50, "plain_latency_p99_s": q99,
-                              "served_over_plain": served / (plain_pairs / plain_span)})
+                              "served_over_plain": served / (plain_pairs / plain_span),
+                              "plain_by_model": by_model})
         print(f"bench: plain makespan {plain_span:.2f} s p50 {q50:.2f} s p99 {q99:.2f} s",
               file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)

@@ -5,10 +5,11 @@
 //       host-side checks that need no GPU: the Settings defaults are the reference's, the
 //       force builders fill tecs_force_desc in the documented order, bad compositions throw,
 //       and on a machine without a HIP device creating a Simulation throws tecs_amd::Error.
-//   adapter_check epithelium <in.bin> <out.bin> <steps> <dt>
+//   adapter_check epithelium <in.bin> <out.bin> <steps> <dt> [<snapshots>]
 //       BinnedGabriel + Heun with polarity, force PiecewiseLinear(0.8, 2, 1) + Bending() +
 //       Alignment(); in.bin = uint32 n, float32 bounds[6], positions[n*3], polarities[n*2];
-//       out.bin = positions[n*3], polarities[n*2], uint64 pair interactions.
+//       out.bin = positions[n*3], polarities[n*2], uint64 pair interactions. With <snapshots>,
+//       Simulation::write at the start and the end into <snapshots>.h5 (groups t0, t1).
 //   adapter_check sorting <in.bin> <out.bin> <steps> <dt> <tau_ab>
 //       BinnedAllPairs + Heun with types, DifferentialAdhesion(0.8, 2, {{1, tau}, {tau, 1}});
 //       in.bin = uint32 n, float32 bounds[6], positions[n*3], uint8 types[n]; out.bin =
@@ -17,6 +18,7 @@
 #include <cstdlib>
 #include <fstream>
 #include <iostream>
+#include <memory>
 #include <type_traits>
 
 #include "tecs_amd.hpp"
@@ -95,7 +97,8 @@ static void read_vec(std::ifstream& in, std::vector<T>& v) {
   in.read(reinterpret_cast<char*>(v.data()), (std::streamsize)(v.size() * sizeof(T)));
 }
 
-static int run_epithelium(const char* in_path, const char* out_path, int steps, double dt) {
+static int run_epithelium(const char* in_path, const char* out_path, int steps, double dt,
+                          const char* h5_path) {
   std::ifstream in(in_path, std::ios::binary);
   uint32_t n = 0;
   float b[6];
@@ -114,7 +117,16 @@ static int run_epithelium(const char* in_path, const char* out_path, int steps, 
   Simulation<EpitheliumConfig> sim(n, 1.0f, s);
   sim.set_state(x, p);
   const auto force = forces::PiecewiseLinear(0.8f, 2.0f, 1.0f) + forces::Bending() + forces::Alignment();
+  std::unique_ptr<Writer> writer;
+  if (h5_path) {
+    writer.reset(new Writer(h5_path));
+    sim.write(*writer, 0.0);
+  }
   for (int i = 0; i < steps; ++i) sim.take_step(dt, force);
+  if (writer) {
+    sim.write(*writer, steps * dt);
+    writer->flush();
+  }
   sim.synchronize();
   const auto xo = sim.positions();
   const auto po = sim.polarities();
@@ -161,8 +173,9 @@ int main(int argc, char** argv) {
   try {
     const std::string mode = argc > 1 ? argv[1] : "host";
     if (mode == "host") return host_checks();
-    if (mode == "epithelium" && argc == 6)
-      return run_epithelium(argv[2], argv[3], std::atoi(argv[4]), std::atof(argv[5]));
+    if (mode == "epithelium" && (argc == 6 || argc == 7))
+      return run_epithelium(argv[2], argv[3], std::atoi(argv[4]), std::atof(argv[5]),
+                            argc == 7 ? argv[6] : nullptr);
     if (mode == "sorting" && argc == 7)
       return run_sorting(argv[2], argv[3], std::atoi(argv[4]), std::atof(argv[5]),
                          (float)std::atof(argv[6]));

@@ -11,7 +11,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from tecs_amd import GpuSimulation, _build, workloads as W
+from tecs_amd import GpuSimulation, _build, io, workloads as W
 
 BIN = _build.ADAPTER_BIN
 
@@ -49,7 +49,8 @@ def test_adapter_epithelium_matches_c_abi(tmp_path):
         f.write(np.ascontiguousarray(w.positions, np.float32).tobytes())
         f.write(np.ascontiguousarray(w.polarities, np.float32).tobytes())
     print(f"case adapter epithelium n={w.n} steps={steps}", flush=True)
-    r = subprocess.run([_driver(), "epithelium", str(inp), str(out), str(steps), str(w.dt)],
+    snap = tmp_path / "snap"
+    r = subprocess.run([_driver(), "epithelium", str(inp), str(out), str(steps), str(w.dt), str(snap)],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     raw = out.read_bytes()
@@ -64,6 +65,11 @@ def test_adapter_epithelium_matches_c_abi(tmp_path):
     assert np.array_equal(x.view(np.uint32), xa.view(np.uint32))
     assert np.array_equal(p.view(np.uint32), pa.view(np.uint32))
     assert pairs == s.stats()["pair_interactions"]
+    # the adapter's Simulation::write: t0 = initial state, t1 = final state, agent order
+    h5 = str(snap) + ".h5"
+    assert np.array_equal(io.read_dataset(h5, "t0/position"), w.positions)
+    assert np.array_equal(io.read_dataset(h5, "t1/position"), x)
+    assert np.array_equal(io.read_dataset(h5, "t1/polarity"), p)
 
 
 @pytest.mark.gpu

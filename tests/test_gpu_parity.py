@@ -513,3 +513,28 @@ def test_upload_mid_run_restarts_the_grid(gpu, oracle_lib):
     xo, _, _ = o.download()
     assert np.array_equal(xg.view(np.uint32), xo.view(np.uint32))
     assert g.stats()["pair_interactions"] == o.stats()["pair_interactions"]
+
+
+@pytest.mark.parametrize("finder", FINDERS, ids=lambda f: abi.FINDER_NAMES[f])
+@pytest.mark.parametrize("n", [0, 1, 2])
+def test_tiny_populations(gpu, oracle_lib, finder, n):
+    """Empty, single-cell and two-cell simulations (the two cells coincide: d = 0, where
+    PiecewiseLinear returns zero, predefined.hpp:125-127): no faults, the oracle's pairs and
+    trajectory bitwise."""
+    w = W.relaxation(max(n, 1), finder=finder)
+    pos = np.zeros((n, 3), np.float32) + np.float32(0.25)
+    d = abi.copy_desc(w.desc)
+    d.agent_count = d.capacity = n
+    print(f"case tiny n={n} finder={abi.FINDER_NAMES[finder]}", flush=True)
+    g = GpuSimulation(d)
+    o = oracle_lib.OracleSim(d)
+    for s in (g, o):
+        s.upload(pos, None, None)
+        s.take_step(w.dt, 3)
+    rg, cg = g.pairs(True)
+    ro, co = o.pairs(True)
+    assert np.array_equal(rg, ro) and np.array_equal(cg, co)
+    assert len(cg) == (2 if n == 2 else 0)
+    xg, _, _ = g.download()
+    xo, _, _ = o.download()
+    assert np.array_equal(xg.view(np.uint32), xo.view(np.uint32))
