@@ -7,6 +7,10 @@
  *                                                    include/pair_finders/naive_all_pairs.hpp:32-104
  *   binned / naive Gabriel finders           replace include/pair_finders/binned_gabriel.hpp:71-362,
  *                                                    include/pair_finders/naive_gabriel.hpp:36-120
+ *   symmetric finders (each unordered pair   replace include/pair_finders/binned_all_pairs_symmetric.hpp:121-220,
+ *     evaluated once, j < i)                         include/pair_finders/naive_all_pairs_symmetric.hpp:82-155,
+ *                                                    include/pair_finders/binned_gabriel_symmetric.hpp:128-306,
+ *                                                    include/pair_finders/naive_gabriel_symmetric.hpp:89-180
  *   pairwise force laws (PiecewiseLinear,    replace include/forces/predefined.hpp:107-144,
  *     bending, polarity alignment)                   include/types/polarity.hpp:131-198
  *   Euler / Heun integrators                 replace include/integrators/euler.hpp:49-85,
@@ -50,7 +54,15 @@ enum {
   TECS_NAIVE_ALL_PAIRS = 0,  /* pair_finders::NaiveAllPairs  */
   TECS_BINNED_ALL_PAIRS = 1, /* pair_finders::BinnedAllPairs */
   TECS_NAIVE_GABRIEL = 2,    /* pair_finders::NaiveGabriel   */
-  TECS_BINNED_GABRIEL = 3    /* pair_finders::BinnedGabriel  */
+  TECS_BINNED_GABRIEL = 3,   /* pair_finders::BinnedGabriel  */
+  /* The symmetric finders visit, for each cell i, only partners j < i (agent index) and apply
+   * the functor's result once per unordered pair: delta(i) += v, delta(j) -= v, for every field.
+   * The reference accumulates with atomics, so its sums have no fixed order; this engine's
+   * do not either (forces agree to rounding, pair sets exactly). */
+  TECS_NAIVE_ALL_PAIRS_SYMMETRIC = 4,  /* pair_finders::NaiveAllPairsSymmetric  */
+  TECS_BINNED_ALL_PAIRS_SYMMETRIC = 5, /* pair_finders::BinnedAllPairsSymmetric */
+  TECS_NAIVE_GABRIEL_SYMMETRIC = 6,    /* pair_finders::NaiveGabrielSymmetric   */
+  TECS_BINNED_GABRIEL_SYMMETRIC = 7    /* pair_finders::BinnedGabrielSymmetric  */
 };
 enum { TECS_EULER = 0, TECS_HEUN = 1 };                        /* integrators::Euler / Heun */
 enum { TECS_NO_COM_FIXER = 0, TECS_GLOBAL_COM_FIXER = 1 };     /* com_fixers::*             */
@@ -100,7 +112,7 @@ typedef struct tecs_sim_desc {
   uint32_t capacity;        /* allocated cells, >= agent_count (GlobalComFixer divides by it) */
   uint32_t fields;          /* TECS_FIELD_* bit mask */
   float cutoff_distance;    /* r_max */
-  int32_t pair_finder;      /* TECS_*_ALL_PAIRS / TECS_*_GABRIEL */
+  int32_t pair_finder;      /* TECS_*_ALL_PAIRS / TECS_*_GABRIEL[_SYMMETRIC] */
   tecs_pair_finder_settings pair_finder_settings;
   int32_t integrator;       /* TECS_EULER / TECS_HEUN */
   int32_t com_fixer;        /* TECS_NO_COM_FIXER / TECS_GLOBAL_COM_FIXER */
@@ -163,7 +175,8 @@ int tecs_sim_evaluate_force(tecs_sim* sim, int32_t is_full_step, float* dpos, fl
 
 /* The ordered pairs (i, j) such an evaluation would hand to the force functor, as CSR over i:
  * row_ptr[agent_count+1], cols[n_pairs], each row in the reference's visiting order
- * (neighbour bins x-fastest, members in ascending agent index). Rebuild semantics as
+ * (neighbour bins x-fastest, members in ascending agent index); a symmetric finder's row i
+ * holds only partners j < i. Rebuild semantics as
  * tecs_sim_evaluate_force. Call with cols == NULL to obtain n_pairs only (nothing advances
  * then); returns TECS_ERR_CAPACITY when cols_capacity < n_pairs. Synchronous. */
 int tecs_sim_pairs(tecs_sim* sim, int32_t is_full_step, int64_t* row_ptr, int32_t* cols,

@@ -14,8 +14,9 @@
  *   for (...) { sim.take_step(dt, layer_force); sim.write(writer, t); }
  *
  * The names follow the reference: pair_finders::{NaiveAllPairs, BinnedAllPairs, NaiveGabriel,
- * BinnedGabriel} with their Settings (binned_all_pairs.hpp:20-25, binned_gabriel.hpp:20-26,
- * naive_gabriel.hpp:22-24), integrators::{Euler, Heun}, com_fixers::{NoComFixer,
+ * BinnedGabriel} and their *Symmetric variants with their Settings (binned_all_pairs.hpp:20-25,
+ * binned_gabriel.hpp:20-26, naive_gabriel.hpp:22-24, binned_gabriel_symmetric.hpp:46-52),
+ * integrators::{Euler, Heun}, com_fixers::{NoComFixer,
  * GlobalComFixer}, Simulation<Config>::take_step(dt, force) (simulation.hpp:548-556) and
  * write(time) (simulation.hpp:607-611). A force is not a lambda here: the engine evaluates the
  * laws of tecs_force_desc, which the forces:: builders compose in the reference's statement
@@ -88,6 +89,33 @@ struct NaiveGabriel {
   struct Settings {
     float gabriel_coefficient = 0.8f;
   };
+};
+// The symmetric finders: each unordered pair once (j < i), delta(i) += v, delta(j) -= v.
+// binned_all_pairs_symmetric.hpp:43-48 (the same members as BinnedAllPairs)
+struct BinnedAllPairsSymmetric {
+  static constexpr int32_t kind = TECS_BINNED_ALL_PAIRS_SYMMETRIC;
+  using Settings = BinnedAllPairs::Settings;
+};
+// naive_all_pairs_symmetric.hpp:37
+struct NaiveAllPairsSymmetric {
+  static constexpr int32_t kind = TECS_NAIVE_ALL_PAIRS_SYMMETRIC;
+  using Settings = NaiveAllPairs::Settings;
+};
+// binned_gabriel_symmetric.hpp:46-52 (its own member order)
+struct BinnedGabrielSymmetric {
+  static constexpr int32_t kind = TECS_BINNED_GABRIEL_SYMMETRIC;
+  struct Settings {
+    float gabriel_coefficient = 0.8f;
+    std::array<float, 3> min_bounds{{-20.0f, -20.0f, -20.0f}};
+    std::array<float, 3> max_bounds{{20.0f, 20.0f, 20.0f}};
+    float bin_size_scale = 1.0f;
+    unsigned rebuild_every_n = 0;
+  };
+};
+// naive_gabriel_symmetric.hpp:39-41
+struct NaiveGabrielSymmetric {
+  static constexpr int32_t kind = TECS_NAIVE_GABRIEL_SYMMETRIC;
+  using Settings = NaiveGabriel::Settings;
 };
 }  // namespace pair_finders
 
@@ -330,6 +358,16 @@ class Simulation {
     o->rebuild_every_n = (int32_t)s.rebuild_every_n;
   }
   static void apply_settings(const pair_finders::BinnedGabriel::Settings& s,
+                             tecs_pair_finder_settings* o) {
+    for (int k = 0; k < 3; ++k) {
+      o->min_bounds[k] = s.min_bounds[k];
+      o->max_bounds[k] = s.max_bounds[k];
+    }
+    o->bin_size_scale = s.bin_size_scale;
+    o->rebuild_every_n = (int32_t)s.rebuild_every_n;
+    o->gabriel_coefficient = s.gabriel_coefficient;
+  }
+  static void apply_settings(const pair_finders::BinnedGabrielSymmetric::Settings& s,
                              tecs_pair_finder_settings* o) {
     for (int k = 0; k < 3; ++k) {
       o->min_bounds[k] = s.min_bounds[k];

@@ -59,7 +59,7 @@ def load():
         "orc_evaluate_force": (C.c_int, [P, C.c_int, P, P]),
         "orc_pairs": (C.c_int, [P, C.c_int, P, P, C.c_int64, C.POINTER(C.c_int64)]),
         "orc_get_stats": (C.c_int, [P, C.POINTER(abi.Stats)]),
-        "orc_forces_f64": (C.c_int, [P, P, P, P, P, P, P]),
+        "orc_forces_f64": (C.c_int, [P, C.c_int, P, P, P, P, P, P]),
         "orc_vector_op": (C.c_int, [C.c_int, P, P]),
         "orc_polarity_op": (C.c_int, [C.c_int, P, P]),
         "orc_com_fix": (C.c_int, [P, C.c_uint, P]),
@@ -147,9 +147,12 @@ class OracleSim:
         _check(self._lib.orc_get_stats(self._h, C.byref(s)))
         return {k: int(getattr(s, k)) for k, _ in abi.Stats._fields_}
 
-    def forces_f64(self, row_ptr, cols):
+    def forces_f64(self, row_ptr, cols, symmetric=None):
         """FP64 forces of the current stage-0 state over the given pair set, and per component
-        the sum of absolute pair contributions."""
+        the sum of absolute pair contributions. A symmetric pair set (the symmetric finders)
+        applies each pair's value v to i and -v to j; by default the finder decides."""
+        if symmetric is None:
+            symmetric = int(self.desc.pair_finder) >= abi.NAIVE_ALL_PAIRS_SYMMETRIC
         rp = np.ascontiguousarray(row_ptr, np.int64)
         cc = np.ascontiguousarray(cols, np.int32)
         if cc.size == 0:
@@ -158,7 +161,8 @@ class OracleSim:
         spos = np.empty((self.n, 3), np.float64)
         dpol = np.empty((self.n, 2), np.float64)
         spol = np.empty((self.n, 2), np.float64)
-        _check(self._lib.orc_forces_f64(self._h, _ptr(rp), _ptr(cc), _ptr(dpos), _ptr(dpol), _ptr(spos), _ptr(spol)))
+        _check(self._lib.orc_forces_f64(self._h, 1 if symmetric else 0, _ptr(rp), _ptr(cc), _ptr(dpos),
+                                        _ptr(dpol), _ptr(spos), _ptr(spol)))
         return dpos, dpol, spos, spol
 
 

@@ -11,6 +11,8 @@
 //   include/pair_finders/naive_all_pairs.hpp  NaiveAllPairs::evaluate_force       (eval_naive_all_pairs)
 //   include/pair_finders/binned_gabriel.hpp   BinnedGabriel::evaluate_force       (eval_binned_gabriel)
 //   include/pair_finders/naive_gabriel.hpp    NaiveGabriel::evaluate_force        (eval_naive_gabriel)
+//   include/pair_finders/*_symmetric.hpp      the four symmetric finders          (the same walks
+//                                             with j < i, then evaluate_symmetric)
 //   include/forces/predefined.hpp:107-144     PiecewiseLinear                     (piecewise_linear)
 //   include/types/polarity.hpp:111-198        Polarity_ math                      (pol_*)
 //   include/types/vector.hpp:312-343          dot / length_squared / distance_to_squared
@@ -27,7 +29,9 @@
 //
 // The Kokkos team reductions of the Gabriel and naive finders and the parallel_reduce of the
 // GlobalComFixer have a backend-defined summation order; this restatement sums sequentially
-// (parity of those sums is to a tolerance, not bitwise).
+// (parity of those sums is to a tolerance, not bitwise). The symmetric finders accumulate
+// with atomics; here they apply the pair results in the Serial backend's order (cells i
+// ascending, each cell's pairs in visiting order, delta(i) += v then delta(j) -= v).
 //
 // A double-precision evaluator (orc_forces_f64) recomputes the forces of a given pair set in
 // FP64 for the force-accuracy checks.
@@ -292,17 +296,22 @@ inline void commit(const Ctx& x, unsigned i, const Acc& acc, float total_drag,
   for (int c = 0; c < 3; ++c) x.out_pos[3 * (size_t)i + c] += total_vel[c] / den;
 }
 
-// BinnedAllPairs::evaluate_force, binned_all_pairs.hpp:86-182 (per cell i).
+// BinnedAllPairs::evaluate_force, binned_all_pairs.hpp:86-182 (per cell i). With `sym`:
+// BinnedAllPairsSymmetric, binned_all_pairs_symmetric.hpp:151-201 (i's bin by
+// calc_bin_coords_from_point, partners j < i only).
 void binned_all_pairs_cell(const Ctx& x, unsigned i, uint64_t& local_pairs,
-                           std::vector<int>* row) {
+                           std::vector<int>* row, bool sym) {
   const Sim& s = *x.s;
   const Grid& g = s.grid;
   const float* p_i = &x.pos[3 * (size_t)i];
   const int side = 2 * s.search_radius + 1;
   const int task_count = side * side * side;
   int bc[3];
-  for (int d = 0; d < 3; ++d)
-    bc[d] = std::max(0, std::min((int)((p_i[d] - g.minb[d]) * s.inv_bin), g.ext[d] - 1));
+  if (sym)
+    g.coords(p_i, bc);  // binned_all_pairs_symmetric.hpp:154
+  else
+    for (int d = 0; d < 3; ++d)
+      bc[d] = std::max(0, std::min((int)((p_i[d] - g.minb[d]) * s.inv_bin), g.ext[d] - 1));
   Acc acc{};
   float total_drag = 0.0f;
   float total_vel[3] = {0.0f, 0.0f, 0.0f};
@@ -320,7 +329,7 @@ void binned_all_pairs_cell(const Ctx& x, unsigned i, uint64_t& local_pairs,
     const int start = g.offsets[bi], end = g.offsets[(size_t)bi + 1];
     for (int idx = start; idx < end; ++idx) {
       const unsigned j = (unsigned)g.perm[idx];
-      if (j == i) continue;
+      if (sym ? j >= i : j == i) continue;
       const float* p_j = &x.pos[3 * (size_t)j];
       float disp[3];
       for (int c = 0; c < 3; ++c) disp[c] = p_i[c] - p_j[c];
@@ -332,15 +341,16 @@ void binned_all_pairs_cell(const Ctx& x, unsigned i, uint64_t& local_pairs,
   if (!row) commit(x, i, acc, total_drag, total_vel);
 }
 
-// NaiveAllPairs::evaluate_force, naive_all_pairs.hpp:41-103 (sequential j order).
+// NaiveAllPairs::evaluate_force, naive_all_pairs.hpp:41-103 (sequential j order). With `sym`:
+// NaiveAllPairsSymmetric, naive_all_pairs_symmetric.hpp:100-143 (j < i).
 void naive_all_pairs_cell(const Ctx& x, unsigned i, uint64_t& local_pairs,
-                          std::vector<int>* row) {
+                          std::vector<int>* row, bool sym) {
   const Sim& s = *x.s;
   const float* p_i = &x.pos[3 * (size_t)i];
   Acc acc{};
   float total_drag = 0.0f;
   float total_vel[3] = {0.0f, 0.0f, 0.0f};
-  for (unsigned j = 0; j < s.n; ++j) {
+  for (unsigned j = 0; j < (sym ? i : s.n); ++j) {
     if (i == j) continue;
     const float* p_j = &x.pos[3 * (size_t)j];
     float disp[3];
@@ -352,9 +362,11 @@ void naive_all_pairs_cell(const Ctx& x, unsigned i, uint64_t& local_pairs,
   if (!row) commit(x, i, acc, total_drag, total_vel);
 }
 
-// BinnedGabriel::evaluate_force, binned_gabriel.hpp:95-361 (tasks and members in order).
+// BinnedGabriel::evaluate_force, binned_gabriel.hpp:95-361 (tasks and members in order). With
+// `sym`: BinnedGabrielSymmetric, binned_gabriel_symmetric.hpp:159-287 (partners j < i; the
+// blockers k are all members but i and j, as before).
 void binned_gabriel_cell(const Ctx& x, unsigned i, uint64_t& local_pairs,
-                         std::vector<int>* row) {
+                         std::vector<int>* row, bool sym) {
   const Sim& s = *x.s;
   const Grid& g = s.grid;
   const float* p_i = &x.pos[3 * (size_t)i];
@@ -377,7 +389,7 @@ void binned_gabriel_cell(const Ctx& x, unsigned i, uint64_t& local_pairs,
     const int start = g.offsets[b], end = g.offsets[(size_t)b + 1];
     for (int idx = start; idx < end; ++idx) {
       const unsigned j = (unsigned)g.perm[idx];
-      if (j == i) continue;
+      if (sym ? j >= i : j == i) continue;
       const float* p_j = &x.pos[3 * (size_t)j];
       float disp[3];
       for (int c = 0; c < 3; ++c) disp[c] = p_i[c] - p_j[c];
@@ -421,15 +433,16 @@ void binned_gabriel_cell(const Ctx& x, unsigned i, uint64_t& local_pairs,
   if (!row) commit(x, i, acc, total_drag, total_vel);
 }
 
-// NaiveGabriel::evaluate_force, naive_gabriel.hpp:47-119.
+// NaiveGabriel::evaluate_force, naive_gabriel.hpp:47-119. With `sym`: NaiveGabrielSymmetric,
+// naive_gabriel_symmetric.hpp:107-168 (j < i; rho^2 = (d2 * 0.25) * c^2 as before).
 void naive_gabriel_cell(const Ctx& x, unsigned i, uint64_t& local_pairs,
-                        std::vector<int>* row) {
+                        std::vector<int>* row, bool sym) {
   const Sim& s = *x.s;
   const float* p_i = &x.pos[3 * (size_t)i];
   Acc acc{};
   float total_drag = 0.0f;
   float total_vel[3] = {0.0f, 0.0f, 0.0f};
-  for (unsigned j = 0; j < s.n; ++j) {
+  for (unsigned j = 0; j < (sym ? i : s.n); ++j) {
     if (i == j) continue;
     const float* p_j = &x.pos[3 * (size_t)j];
     float disp[3];
@@ -453,8 +466,11 @@ void naive_gabriel_cell(const Ctx& x, unsigned i, uint64_t& local_pairs,
 }
 
 bool finder_is_binned(const Sim& s) {
-  return s.desc.pair_finder == TECS_BINNED_ALL_PAIRS || s.desc.pair_finder == TECS_BINNED_GABRIEL;
+  const int f = s.desc.pair_finder;
+  return f == TECS_BINNED_ALL_PAIRS || f == TECS_BINNED_GABRIEL ||
+         f == TECS_BINNED_ALL_PAIRS_SYMMETRIC || f == TECS_BINNED_GABRIEL_SYMMETRIC;
 }
+bool finder_is_symmetric(const Sim& s) { return s.desc.pair_finder >= TECS_NAIVE_ALL_PAIRS_SYMMETRIC; }
 
 // The rebuild schedule of the binned finders, binned_all_pairs.hpp:77-84.
 void maybe_rebuild(Sim& s, const float* pos, bool is_full_step) {
@@ -468,6 +484,50 @@ void maybe_rebuild(Sim& s, const float* pos, bool is_full_step) {
   s.step_count++;
 }
 
+// The force part of the symmetric finders (binned_all_pairs_symmetric.hpp:176-218 and the
+// same lines of the other three): for every pair (i, j < i) the functor runs once on a zeroed
+// accumulator pack; atomic_add(out(i), v) and atomic_add(out(j), -v) for every field, then the
+// drag pass adds total_velocity / fmax(total_drag, eps), which is +0 because no force term
+// writes drag. Pair results are computed in parallel and applied in the Serial order.
+void apply_symmetric(const Ctx& x, const std::vector<std::vector<int>>& rows) {
+  const Sim& s = *x.s;
+  const long n = (long)s.n;
+  std::vector<int64_t> rp((size_t)n + 1, 0);
+  for (long i = 0; i < n; ++i) rp[i + 1] = rp[i] + (int64_t)rows[i].size();
+  std::vector<Acc> vals((size_t)rp[n]);
+  static const float zero2[2] = {0.0f, 0.0f};
+#pragma omp parallel for schedule(dynamic, 256)
+  for (long i = 0; i < n; ++i) {
+    const float* p_i = &x.pos[3 * (size_t)i];
+    for (size_t e = 0; e < rows[i].size(); ++e) {
+      const int j = rows[i][e];
+      const float* p_j = &x.pos[3 * (size_t)j];
+      float disp[3];
+      for (int c = 0; c < 3; ++c) disp[c] = p_i[c] - p_j[c];
+      const float d = std::sqrt(dot3(disp, disp));
+      Acc v{};
+      apply_force(s.force, disp, d, x.pol ? &x.pol[2 * (size_t)i] : zero2,
+                  x.pol ? &x.pol[2 * (size_t)j] : zero2, x.type ? x.type[i] : 0,
+                  x.type ? x.type[j] : 0, v);
+      vals[(size_t)rp[i] + e] = v;
+    }
+  }
+  for (long i = 0; i < n; ++i) {
+    for (size_t e = 0; e < rows[i].size(); ++e) {
+      const size_t j = (size_t)rows[i][e];
+      const Acc& v = vals[(size_t)rp[i] + e];
+      for (int c = 0; c < 3; ++c) x.out_pos[3 * (size_t)i + c] += v.pos[c];
+      for (int c = 0; c < 3; ++c) x.out_pos[3 * j + c] += -v.pos[c];
+      if (x.out_pol) {
+        for (int c = 0; c < 2; ++c) x.out_pol[2 * (size_t)i + c] += v.pol[c];
+        for (int c = 0; c < 2; ++c) x.out_pol[2 * j + c] += -v.pol[c];
+      }
+    }
+  }
+  for (long i = 0; i < n; ++i)
+    for (int c = 0; c < 3; ++c) x.out_pos[3 * (size_t)i + c] += 0.0f;
+}
+
 // PairFinder::evaluate_force for any finder: in = (pos, pol), out += deltas.
 int evaluate(Sim& s, const float* pos, const float* pol, float* out_pos, float* out_pol,
              bool is_full_step, std::vector<std::vector<int>>* emit) {
@@ -476,22 +536,32 @@ int evaluate(Sim& s, const float* pos, const float* pol, float* out_pos, float* 
     return fail(TECS_ERR_STATE, "binned finder evaluated before any full step built a grid");
   Ctx x{&s, pos, s.has_pol ? pol : nullptr, s.has_type ? s.type.data() : nullptr,
         out_pos, s.has_pol ? out_pol : nullptr, s.old_vel.data(), emit, nullptr};
+  const bool sym = finder_is_symmetric(s);
+  // a symmetric force evaluation first collects each cell's pairs, then applies them
+  std::vector<std::vector<int>> sym_rows;
+  std::vector<std::vector<int>>* rows = emit;
+  if (sym && !emit) {
+    sym_rows.assign(s.n, {});
+    rows = &sym_rows;
+    x.emit = rows;
+  }
   uint64_t total_pairs = 0;
   const long n = (long)s.n;
-  const int finder = s.desc.pair_finder;
+  const int finder = sym ? s.desc.pair_finder - TECS_NAIVE_ALL_PAIRS_SYMMETRIC : s.desc.pair_finder;
 #pragma omp parallel for schedule(dynamic, 256) reduction(+ : total_pairs)
   for (long ii = 0; ii < n; ++ii) {
     const unsigned i = (unsigned)ii;
     uint64_t lp = 0;
-    std::vector<int>* row = emit ? &(*emit)[i] : nullptr;
+    std::vector<int>* row = rows ? &(*rows)[i] : nullptr;
     switch (finder) {
-      case TECS_BINNED_ALL_PAIRS: binned_all_pairs_cell(x, i, lp, row); break;
-      case TECS_NAIVE_ALL_PAIRS: naive_all_pairs_cell(x, i, lp, row); break;
-      case TECS_BINNED_GABRIEL: binned_gabriel_cell(x, i, lp, row); break;
-      default: naive_gabriel_cell(x, i, lp, row); break;
+      case TECS_BINNED_ALL_PAIRS: binned_all_pairs_cell(x, i, lp, row, sym); break;
+      case TECS_NAIVE_ALL_PAIRS: naive_all_pairs_cell(x, i, lp, row, sym); break;
+      case TECS_BINNED_GABRIEL: binned_gabriel_cell(x, i, lp, row, sym); break;
+      default: naive_gabriel_cell(x, i, lp, row, sym); break;
     }
     total_pairs += lp;
   }
+  if (sym && !emit) apply_symmetric(x, sym_rows);
   if (!emit) {
     s.stats.pair_interactions += total_pairs;
     s.stats.evaluations++;
@@ -584,7 +654,8 @@ int step_heun(Sim& s, double dt) {
 int validate(const tecs_sim_desc* d) {
   if (!d) return fail(TECS_ERR_INVALID, "null descriptor");
   if (d->capacity < d->agent_count) return fail(TECS_ERR_INVALID, "capacity < agent_count");
-  if (d->pair_finder < 0 || d->pair_finder > 3) return fail(TECS_ERR_INVALID, "bad pair_finder");
+  if (d->pair_finder < 0 || d->pair_finder > TECS_BINNED_GABRIEL_SYMMETRIC)
+    return fail(TECS_ERR_INVALID, "bad pair_finder");
   if (d->integrator != TECS_EULER && d->integrator != TECS_HEUN)
     return fail(TECS_ERR_INVALID, "bad integrator");
   if (d->com_fixer != TECS_NO_COM_FIXER && d->com_fixer != TECS_GLOBAL_COM_FIXER)
@@ -598,6 +669,80 @@ int validate(const tecs_sim_desc* d) {
     return fail(TECS_ERR_INVALID, "polarity forces need TECS_FIELD_POLARITY");
   if (!(d->cutoff_distance > 0.0f)) return fail(TECS_ERR_INVALID, "cutoff must be > 0");
   return TECS_OK;
+}
+
+// Double-precision value of the fused force of one pair (i as self, j as other) at the current
+// stage-0 state, and per component the sum of the absolute values of its terms.
+void pair_force_f64(const Sim& s, long i, long j, double* vpos, double* vpol, double* apos,
+                    double* apol) {
+  const Force& f = s.force;
+  for (int c = 0; c < 3; ++c) vpos[c] = apos[c] = 0.0;
+  for (int c = 0; c < 2; ++c) vpol[c] = apol[c] = 0.0;
+  const double pi[3] = {s.pos[3 * i], s.pos[3 * i + 1], s.pos[3 * i + 2]};
+  double disp[3];
+  for (int c = 0; c < 3; ++c) disp[c] = pi[c] - (double)s.pos[3 * j + c];
+  const double d = std::sqrt(disp[0] * disp[0] + disp[1] * disp[1] + disp[2] * disp[2]);
+  double kadh = f.kadh;
+  if (f.n_types > 0) kadh = f.table[s.type[i] * f.n_types + s.type[j]];
+  if (d != 0.0) {
+    const double F = (double)f.krep * std::fmax((double)f.h - d, 0.0) -
+                     kadh * std::fmax(d - (double)f.h, 0.0);
+    for (int c = 0; c < 3; ++c) {
+      const double v = disp[c] * F / d;
+      vpos[c] += v;
+      apos[c] += std::fabs(v);
+    }
+  }
+  if (!s.has_pol) return;
+  const double li[2] = {s.pol[2 * i], s.pol[2 * i + 1]};
+  const double lj[2] = {s.pol[2 * j], s.pol[2 * j + 1]};
+  auto to_vec = [](const double* p, double* v) {
+    v[0] = std::sin(p[0]) * std::cos(p[1]);
+    v[1] = std::sin(p[0]) * std::sin(p[1]);
+    v[2] = std::cos(p[0]);
+  };
+  auto uni = [](const double* a, const double* b, double* r) {
+    r[0] = std::cos(a[0]) * std::sin(b[0]) * std::cos(a[1] - b[1]) - std::sin(a[0]) * std::cos(b[0]);
+    r[1] = 0.0;
+    const double st = std::sin(a[0]);
+    // same guard as the FP32 path (polarity.hpp:154), evaluated on the FP32 sin
+    if (std::fabs(std::sin((float)a[0])) > FLT_EPSILON)
+      r[1] = -std::sin(b[0]) * std::sin(a[1] - b[1]) / st;
+  };
+  if (f.bend != 0.0f) {
+    double tv[3], ov[3];
+    to_vec(li, tv);
+    to_vec(lj, ov);
+    const double prod_i = (tv[0] * disp[0] + tv[1] * disp[1] + tv[2] * disp[2]) / d;
+    const double prod_j = (ov[0] * disp[0] + ov[1] * disp[1] + ov[2] * disp[2]) / d;
+    for (int c = 0; c < 3; ++c) {
+      const double v = f.bend * (-prod_i / d * tv[c] + prod_i * prod_i / (d * d) * disp[c] -
+                                 prod_j / d * ov[c] + prod_j * prod_j / (d * d) * disp[c]);
+      vpos[c] += v;
+      apos[c] += std::fabs(-prod_i / d * tv[c]) + std::fabs(prod_i * prod_i / (d * d) * disp[c]) +
+                 std::fabs(prod_j / d * ov[c]) + std::fabs(prod_j * prod_j / (d * d) * disp[c]);
+    }
+    double q[2], u[2];
+    q[0] = std::acos(std::fmin(1.0, std::fmax(-1.0, disp[2] / d)));
+    q[1] = std::atan2(disp[1], disp[0]);
+    uni(li, q, u);
+    for (int c = 0; c < 2; ++c) {
+      const double v = f.bend * (-prod_i * u[c]);
+      vpol[c] += v;
+      apol[c] += std::fabs(v);
+    }
+  }
+  if (f.align != 0.0f) {
+    double u[2];
+    uni(li, lj, u);
+    const double dd =
+        std::sin(li[0]) * std::sin(lj[0]) * std::cos(li[1] - lj[1]) + std::cos(li[0]) * std::cos(lj[0]);
+    for (int c = 0; c < 2; ++c) {
+      const double v = f.align * dd * u[c];
+      vpol[c] += v;
+      apol[c] += std::fabs(v);
+    }
+  }
 }
 
 }  // namespace
@@ -751,91 +896,55 @@ int orc_get_stats(orc_sim* h, tecs_stats* st) {
 // Double-precision evaluation of the fused force on a given pair set (CSR over i in the
 // visiting order) at the current stage-0 state. Writes the FP64 deltas and, per component, the
 // sum of absolute pair contributions (the scale the FP32 error is measured against).
-int orc_forces_f64(orc_sim* h, const int64_t* row_ptr, const int32_t* cols, double* dpos,
-                   double* dpol, double* scale_pos, double* scale_pol) {
+// symmetric = 0: each listed pair (i, j) adds its value to i only (the ordered finders).
+// symmetric = 1: each listed pair adds its value v to i and -v to j (the symmetric finders).
+int orc_forces_f64(orc_sim* h, int symmetric, const int64_t* row_ptr, const int32_t* cols,
+                   double* dpos, double* dpol, double* scale_pos, double* scale_pol) {
   Sim& s = *reinterpret_cast<Sim*>(h);
-  const Force& f = s.force;
   const long n = (long)s.n;
+  std::vector<double> ap(3 * (size_t)n, 0.0), al(2 * (size_t)n, 0.0), sp(3 * (size_t)n, 0.0),
+      sl(2 * (size_t)n, 0.0);
+  if (!symmetric) {
 #pragma omp parallel for schedule(dynamic, 256)
-  for (long i = 0; i < n; ++i) {
-    double ap[3] = {0, 0, 0}, al[2] = {0, 0}, sp[3] = {0, 0, 0}, sl[2] = {0, 0};
-    const double pi[3] = {s.pos[3 * i], s.pos[3 * i + 1], s.pos[3 * i + 2]};
-    const double li[2] = {s.has_pol ? s.pol[2 * i] : 0.0, s.has_pol ? s.pol[2 * i + 1] : 0.0};
-    for (int64_t e = row_ptr[i]; e < row_ptr[i + 1]; ++e) {
-      const long j = cols[e];
-      double disp[3];
-      for (int c = 0; c < 3; ++c) disp[c] = pi[c] - (double)s.pos[3 * j + c];
-      const double d = std::sqrt(disp[0] * disp[0] + disp[1] * disp[1] + disp[2] * disp[2]);
-      double kadh = f.kadh;
-      if (f.n_types > 0) kadh = f.table[s.type[i] * f.n_types + s.type[j]];
-      if (d != 0.0) {
-        const double F = (double)f.krep * std::fmax((double)f.h - d, 0.0) -
-                         kadh * std::fmax(d - (double)f.h, 0.0);
+    for (long i = 0; i < n; ++i) {
+      for (int64_t e = row_ptr[i]; e < row_ptr[i + 1]; ++e) {
+        double vp[3], vl[2], xp[3], xl[2];
+        pair_force_f64(s, i, cols[e], vp, vl, xp, xl);
         for (int c = 0; c < 3; ++c) {
-          const double v = disp[c] * F / d;
-          ap[c] += v;
-          sp[c] += std::fabs(v);
+          ap[3 * i + c] += vp[c];
+          sp[3 * i + c] += xp[c];
         }
-      }
-      if (!s.has_pol) continue;
-      const double lj[2] = {s.pol[2 * j], s.pol[2 * j + 1]};
-      auto to_vec = [](const double* p, double* v) {
-        v[0] = std::sin(p[0]) * std::cos(p[1]);
-        v[1] = std::sin(p[0]) * std::sin(p[1]);
-        v[2] = std::cos(p[0]);
-      };
-      auto uni = [](const double* a, const double* b, double* r) {
-        r[0] = std::cos(a[0]) * std::sin(b[0]) * std::cos(a[1] - b[1]) - std::sin(a[0]) * std::cos(b[0]);
-        r[1] = 0.0;
-        const double st = std::sin(a[0]);
-        // same guard as the FP32 path (polarity.hpp:154), evaluated on the FP32 sin
-        if (std::fabs(std::sin((float)a[0])) > FLT_EPSILON)
-          r[1] = -std::sin(b[0]) * std::sin(a[1] - b[1]) / st;
-      };
-      if (f.bend != 0.0f) {
-        double tv[3], ov[3];
-        to_vec(li, tv);
-        to_vec(lj, ov);
-        const double prod_i = (tv[0] * disp[0] + tv[1] * disp[1] + tv[2] * disp[2]) / d;
-        const double prod_j = (ov[0] * disp[0] + ov[1] * disp[1] + ov[2] * disp[2]) / d;
-        for (int c = 0; c < 3; ++c) {
-          const double v = f.bend * (-prod_i / d * tv[c] + prod_i * prod_i / (d * d) * disp[c] -
-                                     prod_j / d * ov[c] + prod_j * prod_j / (d * d) * disp[c]);
-          ap[c] += v;
-          sp[c] += std::fabs(-prod_i / d * tv[c]) + std::fabs(prod_i * prod_i / (d * d) * disp[c]) +
-                   std::fabs(prod_j / d * ov[c]) + std::fabs(prod_j * prod_j / (d * d) * disp[c]);
-        }
-        double q[2], u[2];
-        q[0] = std::acos(std::fmin(1.0, std::fmax(-1.0, disp[2] / d)));
-        q[1] = std::atan2(disp[1], disp[0]);
-        uni(li, q, u);
         for (int c = 0; c < 2; ++c) {
-          const double v = f.bend * (-prod_i * u[c]);
-          al[c] += v;
-          sl[c] += std::fabs(v);
-        }
-      }
-      if (f.align != 0.0f) {
-        double u[2];
-        uni(li, lj, u);
-        const double dd =
-            std::sin(li[0]) * std::sin(lj[0]) * std::cos(li[1] - lj[1]) + std::cos(li[0]) * std::cos(lj[0]);
-        for (int c = 0; c < 2; ++c) {
-          const double v = f.align * dd * u[c];
-          al[c] += v;
-          sl[c] += std::fabs(v);
+          al[2 * i + c] += vl[c];
+          sl[2 * i + c] += xl[c];
         }
       }
     }
-    for (int c = 0; c < 3; ++c) {
-      if (dpos) dpos[3 * i + c] = ap[c];
-      if (scale_pos) scale_pos[3 * i + c] = sp[c];
-    }
-    for (int c = 0; c < 2; ++c) {
-      if (dpol) dpol[2 * i + c] = al[c];
-      if (scale_pol) scale_pol[2 * i + c] = sl[c];
+  } else {
+    for (long i = 0; i < n; ++i) {
+      for (int64_t e = row_ptr[i]; e < row_ptr[i + 1]; ++e) {
+        const long j = cols[e];
+        double vp[3], vl[2], xp[3], xl[2];
+        pair_force_f64(s, i, j, vp, vl, xp, xl);
+        for (int c = 0; c < 3; ++c) {
+          ap[3 * i + c] += vp[c];
+          ap[3 * j + c] -= vp[c];
+          sp[3 * i + c] += xp[c];
+          sp[3 * j + c] += xp[c];
+        }
+        for (int c = 0; c < 2; ++c) {
+          al[2 * i + c] += vl[c];
+          al[2 * j + c] -= vl[c];
+          sl[2 * i + c] += xl[c];
+          sl[2 * j + c] += xl[c];
+        }
+      }
     }
   }
+  if (dpos) std::memcpy(dpos, ap.data(), sizeof(double) * ap.size());
+  if (dpol) std::memcpy(dpol, al.data(), sizeof(double) * al.size());
+  if (scale_pos) std::memcpy(scale_pos, sp.data(), sizeof(double) * sp.size());
+  if (scale_pol) std::memcpy(scale_pol, sl.data(), sizeof(double) * sl.size());
   return TECS_OK;
 }
 

@@ -22,12 +22,23 @@ NAIVE_ALL_PAIRS = 0
 BINNED_ALL_PAIRS = 1
 NAIVE_GABRIEL = 2
 BINNED_GABRIEL = 3
+NAIVE_ALL_PAIRS_SYMMETRIC = 4
+BINNED_ALL_PAIRS_SYMMETRIC = 5
+NAIVE_GABRIEL_SYMMETRIC = 6
+BINNED_GABRIEL_SYMMETRIC = 7
 FINDER_NAMES = {
     NAIVE_ALL_PAIRS: "NaiveAllPairs",
     BINNED_ALL_PAIRS: "BinnedAllPairs",
     NAIVE_GABRIEL: "NaiveGabriel",
     BINNED_GABRIEL: "BinnedGabriel",
+    NAIVE_ALL_PAIRS_SYMMETRIC: "NaiveAllPairsSymmetric",
+    BINNED_ALL_PAIRS_SYMMETRIC: "BinnedAllPairsSymmetric",
+    NAIVE_GABRIEL_SYMMETRIC: "NaiveGabrielSymmetric",
+    BINNED_GABRIEL_SYMMETRIC: "BinnedGabrielSymmetric",
 }
+SYMMETRIC_FINDERS = (NAIVE_ALL_PAIRS_SYMMETRIC, BINNED_ALL_PAIRS_SYMMETRIC, NAIVE_GABRIEL_SYMMETRIC,
+                     BINNED_GABRIEL_SYMMETRIC)
+BINNED_FINDERS = (BINNED_ALL_PAIRS, BINNED_GABRIEL, BINNED_ALL_PAIRS_SYMMETRIC, BINNED_GABRIEL_SYMMETRIC)
 
 EULER = 0
 HEUN = 1

@@ -53,7 +53,6 @@ inline int fail(int code, const std::string& msg) {
 
 enum TimingClass { kTimeSweepFull = 0, kTimeSweepStale = 1, kTimeRebuild = 2, kTimeClasses = 3 };
 
-inline bool is_binned(int finder) { return finder == kBinnedAllPairs || finder == kBinnedGabriel; }
 
 inline int validate_force(const tecs_force_desc& f, uint32_t fields) {
   if (f.n_types < 0 || f.n_types > TECS_MAX_TYPES) return fail(TECS_ERR_INVALID, "bad n_types");
@@ -68,7 +67,8 @@ inline int validate_desc(const tecs_sim_desc* d) {
   if (!d) return fail(TECS_ERR_INVALID, "null descriptor");
   if (d->capacity < d->agent_count) return fail(TECS_ERR_INVALID, "capacity < agent_count");
   if (d->agent_count > (1u << 30)) return fail(TECS_ERR_INVALID, "agent_count too large");
-  if (d->pair_finder < 0 || d->pair_finder > 3) return fail(TECS_ERR_INVALID, "bad pair_finder");
+  if (d->pair_finder < 0 || d->pair_finder > kBinnedGabrielSym)
+    return fail(TECS_ERR_INVALID, "bad pair_finder");
   if (d->integrator != TECS_EULER && d->integrator != TECS_HEUN)
     return fail(TECS_ERR_INVALID, "bad integrator");
   if (d->com_fixer != TECS_NO_COM_FIXER && d->com_fixer != TECS_GLOBAL_COM_FIXER)
@@ -221,6 +221,9 @@ struct Batch {
   // the next rebuild derives the new order from the last one: a grid exists and the state has
   // only moved by steps since (an upload or a member copy clears this)
   bool inc_ok = false;
+  // symmetric finders: d1 (and d3) hold zeros, ready for the next atomic accumulation (the
+  // updates clear what they consume; evaluate() leaves its result in d1)
+  bool deltas_zero = false;
   tecs_stats stats{};
 
   bool timing = false;
@@ -240,10 +243,10 @@ struct Batch {
   int graph_mode = TECS_GRAPHS_AUTO;
   struct HostState {
     int xc, ic, step_count;
-    bool keys_ready, inc_ok;
+    bool keys_ready, inc_ok, deltas_zero;
     bool operator==(const HostState& o) const {
       return xc == o.xc && ic == o.ic && step_count == o.step_count &&
-             keys_ready == o.keys_ready && inc_ok == o.inc_ok;
+             keys_ready == o.keys_ready && inc_ok == o.inc_ok && deltas_zero == o.deltas_zero;
     }
   };
   struct StepGraph {
@@ -262,7 +265,9 @@ struct Batch {
   hipStream_t side = nullptr;
   std::vector<unsigned> write_step_count;  // Simulation::current_step per member
 
-  HostState host_state() const { return HostState{xc, ic, step_count, keys_ready, inc_ok}; }
+  HostState host_state() const {
+    return HostState{xc, ic, step_count, keys_ready, inc_ok, deltas_zero};
+  }
   int graph_period() const { return 2 * std::max(1, is_binned(finder) ? rebuild_every_n : 1); }
 
   void drop_graphs() {
@@ -466,7 +471,7 @@ struct Batch {
       TRY(alloc(&xp, S));
       if (has_pol) TRY(alloc(&polp, S));
     }
-    if (integrator == TECS_HEUN && com_fixer == TECS_GLOBAL_COM_FIXER) {
+    if (integrator == TECS_HEUN && (com_fixer == TECS_GLOBAL_COM_FIXER || is_symmetric(finder))) {
       TRY(alloc(&d3, S));
       if (has_pol) TRY(alloc(&d3p, S));
     }
@@ -497,8 +502,9 @@ struct Batch {
       TRY(alloc(&bsum, (size_t)scan_tiles_for(total_cbins)));
       HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(int) * (size_t)total_cbins, stream));
     }
-    // team sweep conditions: search radius 1 and, for Gabriel, c <= 0.95 (kernels_team.hip)
-    team_ok = is_binned(finder);
+    // team sweep conditions: an ordered binned finder, search radius 1 and, for Gabriel,
+    // c <= 0.95 (kernels_team.hip)
+    team_ok = is_binned(finder) && !is_symmetric(finder);
     for (int k = 0; k < count; ++k) {
       if (h_inst[k].search_radius != 1) team_ok = false;
       const float c = ds[k].pair_finder_settings.gabriel_coefficient;
@@ -706,12 +712,15 @@ struct Batch {
   }
   int partials_per_block() const { return lanes_of(concrete_variant()); }
 
-  void launch_variant(int v, int mode, const SweepArgs& a) {
-    if (v == TECS_SWEEP_THREAD)
-      launch_sweep(finder, has_type, has_pol, mode, a, n_blocks, stream);
-    else
+  int launch_variant(int v, int mode, const SweepArgs& a) {
+    if (v == TECS_SWEEP_THREAD) {
+      if (launch_sweep(finder, has_type, has_pol, mode, a, n_blocks, stream) != 0)
+        return fail(TECS_ERR_STATE, "sweep mode not available for this pair finder");
+    } else {
       launch_sweep_team(lanes_of(v), finder == kBinnedGabriel, has_type, has_pol, mode, a,
                         n_blocks, stream);
+    }
+    return TECS_OK;
   }
 
   // The next full evaluation rebuilds (maybe_rebuild), so the sweep that ends this step can
@@ -794,7 +803,8 @@ struct Batch {
       float ms_min = 1e30f;
       for (int rep = 0; rep < 6 && rc == TECS_OK; ++rep) {
         hipEventRecord(e0, stream);
-        launch_variant(v, mode, t);
+        rc = launch_variant(v, mode, t);
+        if (rc != TECS_OK) break;
         hipEventRecord(e1, stream);
         hipError_t e = hipEventSynchronize(e1);
         float ms = 0.0f;
@@ -818,15 +828,23 @@ struct Batch {
   int sweep(int mode, const SweepArgs& a, int cls) {
     Pending p{};
     TRY(time_begin(&p, cls));
-    launch_variant(concrete_variant(), mode, a);
+    TRY(launch_variant(concrete_variant(), mode, a));
     HIP_TRY(hipGetLastError());
     TRY(time_end(&p));
     if (mode != kModeCount && mode != kModeEmit) stats.evaluations++;
     return TECS_OK;
   }
 
-  int com_and_update(int kind) {
-    launch_com_finalize(d_inst, n_inst, com_partial, partials_per_block(), com, stream);
+  // COM fix (when on) and the integrator update of a Delta-mode evaluation whose position
+  // deltas are F. The ordered sweeps wrote per-workgroup COM partials themselves; the
+  // symmetric ones cannot (deltas are final only when the sweep ends), so a pass computes them
+  // and the update clears the deltas it consumed for the next atomic accumulation.
+  int com_and_update(int kind, const float4* F = nullptr) {
+    const bool sym = is_symmetric(finder);
+    if (com_fixer == TECS_GLOBAL_COM_FIXER) {
+      if (sym) launch_com_partials(d_inst, d_block_inst, F, com_partial, n_blocks, stream);
+      launch_com_finalize(d_inst, n_inst, com_partial, sym ? 1 : partials_per_block(), com, stream);
+    }
     UpdateArgs u{};
     u.inst = d_inst;
     u.block_inst = d_block_inst;
@@ -842,14 +860,47 @@ struct Batch {
     u.trig = trig_of(pol[xc]);
     u.trigp = trigp;
     u.old_vel = track_ov ? ov[xc] : nullptr;
+    u.clear = sym;
     launch_update(kind, has_pol, u, n_blocks, stream);
     HIP_TRY(hipGetLastError());
+    return TECS_OK;
+  }
+
+  // d1 (and d3) = 0 unless known to be
+  void zero_deltas() {
+    if (deltas_zero) return;
+    launch_zero_deltas(d1, d1p, total_slots, stream);
+    if (d3) launch_zero_deltas(d3, d3p, total_slots, stream);
+    deltas_zero = true;
+  }
+
+  // A step with a symmetric finder: each evaluation accumulates atomically into zeroed deltas,
+  // then the COM fix and the update run as their own kernels (the update zeroes the deltas
+  // again). Heun: evaluate -> predictor -> evaluate (stale grid) -> corrector.
+  int step_symmetric() {
+    zero_deltas();
+    SweepArgs a = base_args(x[xc], pol[xc]);
+    a.o_pos = d1;
+    a.o_pol = d1p;
+    TRY(sweep(kModeDelta, a, kTimeSweepFull));
+    if (integrator == TECS_HEUN) {
+      TRY(com_and_update(kUpdatePred, d1));
+      SweepArgs b = base_args(xp, polp);
+      b.o_pos = d3;
+      b.o_pol = d3p;
+      TRY(sweep(kModeDelta, b, kTimeSweepStale));
+      TRY(com_and_update(kUpdateCorr, d3));
+    } else {
+      TRY(com_and_update(kUpdateEuler, d1));
+    }
+    stats.steps++;
     return TECS_OK;
   }
 
   // One integrator step (Euler::integrate euler.hpp:81-85, Heun::integrate heun.hpp:201-208).
   int step_once() {
     TRY(maybe_rebuild(true));
+    if (is_symmetric(finder)) return step_symmetric();
     const bool com_on = com_fixer == TECS_GLOBAL_COM_FIXER;
     if (integrator == TECS_HEUN) {
       if (!com_on) {
@@ -1051,10 +1102,12 @@ struct Batch {
   int evaluate(bool is_full_step, float* dpos, float* dpol) {
     TRY(sync_inst());
     TRY(maybe_rebuild(is_full_step));
+    if (is_symmetric(finder)) zero_deltas();
     SweepArgs a = base_args(x[xc], pol[xc]);
     a.o_pos = d1;
     a.o_pol = d1p;
     TRY(sweep(kModeDelta, a, is_full_step ? kTimeSweepFull : kTimeSweepStale));
+    deltas_zero = false;  // d1 holds the result
     const InstanceDesc& in = h_inst[0];
     launch_download(d_inst, 0, in.block_begin, in.n_blocks, id[ic], d1, d1p, nullptr,
                     dpos ? st_pos : nullptr, (dpol && has_pol) ? st_pol : nullptr, nullptr, stream);
@@ -1219,7 +1272,8 @@ struct Batch {
     size_t per_slot = 2 * 16 + 2 * 4 + 16 + 16 * 2 /* x, id, d1, ov */ + 8 * 4 /* keys.. */;
     if (pol) per_slot += 2 * 8 + 8 + 2 * 16 /* pol, d1 pol, trig */;
     if (d.integrator == TECS_HEUN) per_slot += 16 + (pol ? 8 : 0);
-    if (d.integrator == TECS_HEUN && d.com_fixer == TECS_GLOBAL_COM_FIXER) per_slot += 16 + (pol ? 8 : 0);
+    if (d.integrator == TECS_HEUN && (d.com_fixer == TECS_GLOBAL_COM_FIXER || is_symmetric(d.pair_finder)))
+      per_slot += 16 + (pol ? 8 : 0);
     size_t bins = 0;
     if (is_binned(d.pair_finder)) {
       const tecs_pair_finder_settings& s = d.pair_finder_settings;

@@ -13,6 +13,12 @@
 // reads predicted positions while the slot order and bin offsets stay those of the last
 // rebuild; the centre bin of i is recomputed from its current position, exactly like the
 // reference.
+//
+// Symmetric finders (SYM, binned_all_pairs_symmetric.hpp:148-219 and its three siblings): the
+// same walks keep only partners with a smaller agent index; the functor's result v of a pair
+// goes to i's register sum and, negated, to j with a hardware float atomic; i's sum is added
+// atomically at the end. The delta buffer must be zero before the sweep, and only the Delta,
+// Count and Emit modes exist (the integrator update runs as its own kernel afterwards).
 #include "tecs_internal.hpp"
 
 namespace tecs_amd {
@@ -26,7 +32,22 @@ struct CellState {
   int64_t emit_at;
 };
 
-template <bool HAS_TYPE, bool HAS_POL, int MODE>
+// delta(j) += v with hardware float atomics (no return value, relaxed, device scope)
+template <bool HAS_POL>
+__device__ __forceinline__ void atomic_add_delta(const SweepArgs& a, int m, const float* vp,
+                                                 const float* vl) {
+  float* q = &a.o_pos[m].x;
+  unsafeAtomicAdd(q + 0, vp[0]);
+  unsafeAtomicAdd(q + 1, vp[1]);
+  unsafeAtomicAdd(q + 2, vp[2]);
+  if (HAS_POL) {
+    float* r = &a.o_pol[m].x;
+    unsafeAtomicAdd(r + 0, vl[0]);
+    unsafeAtomicAdd(r + 1, vl[1]);
+  }
+}
+
+template <bool HAS_TYPE, bool HAS_POL, int MODE, bool SYM>
 __device__ __forceinline__ void visit(const SweepArgs& a, const InstanceDesc& in, CellState& cs,
                                       float4 me, const Pol& pme, int m, float4 pj,
                                       const float* disp, float d2) {
@@ -39,6 +60,20 @@ __device__ __forceinline__ void visit(const SweepArgs& a, const InstanceDesc& in
   const float d = sqrtf(d2);
   Pol po{};
   if (HAS_POL) po = make_pol(a.ps[m], a.trig[m]);
+  if (SYM) {
+    // the functor on a zeroed accumulator pack, then out(i) += v, out(j) -= v
+    // (binned_all_pairs_symmetric.hpp:176-200)
+    float vp[3] = {0.0f, 0.0f, 0.0f}, vl[2] = {0.0f, 0.0f};
+    apply_force<HAS_TYPE, HAS_POL>(in.force, disp, d, __float_as_int(me.w), __float_as_int(pj.w),
+                                   pme, po, vp, vl);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) cs.acc_pos[c] += vp[c];
+    cs.acc_pol[0] += vl[0];
+    cs.acc_pol[1] += vl[1];
+    const float np[3] = {-vp[0], -vp[1], -vp[2]}, nl[2] = {-vl[0], -vl[1]};
+    atomic_add_delta<HAS_POL>(a, m, np, nl);
+    return;
+  }
   apply_force<HAS_TYPE, HAS_POL>(in.force, disp, d, __float_as_int(me.w), __float_as_int(pj.w),
                                  pme, po, cs.acc_pos, cs.acc_pol);
 }
@@ -74,14 +109,14 @@ __device__ __forceinline__ bool gabriel_blocked(const InstanceDesc& in, const fl
   return false;
 }
 
-template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE>
+template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE, bool SYM>
 __device__ __forceinline__ void walk_binned(const SweepArgs& a, const InstanceDesc& in, int s,
-                                            float4 me, const Pol& pme, CellState& cs) {
+                                            int me_id, float4 me, const Pol& pme, CellState& cs) {
   const float4* __restrict__ xs = a.xs;
   const int* __restrict__ offb = a.off + in.bin_begin;
   int bc[3];
-  if (GABRIEL)
-    bin_coords_div(in, me.x, me.y, me.z, bc);  // binned_gabriel.hpp:101
+  if (GABRIEL || SYM)
+    bin_coords_div(in, me.x, me.y, me.z, bc);  // binned_gabriel.hpp:101, *_symmetric.hpp
   else
     bin_coords_mul(in, me.x, me.y, me.z, bc);  // binned_all_pairs.hpp:103-109
   const int R = in.search_radius;
@@ -102,10 +137,11 @@ __device__ __forceinline__ void walk_binned(const SweepArgs& a, const InstanceDe
         const float disp[3] = {me.x - pj.x, me.y - pj.y, me.z - pj.z};
         const float d2 = len2(disp[0], disp[1], disp[2]);
         if (d2 >= r2) continue;
+        if (SYM && a.id[m] >= me_id) continue;  // j < i only
         if (GABRIEL) {
           if (gabriel_blocked(in, xs, offb, s, m, me, disp, d2)) continue;
         }
-        visit<HAS_TYPE, HAS_POL, MODE>(a, in, cs, me, pme, m, pj, disp, d2);
+        visit<HAS_TYPE, HAS_POL, MODE, SYM>(a, in, cs, me, pme, m, pj, disp, d2);
       }
     }
   }
@@ -125,9 +161,9 @@ constexpr int kDeferMax = 16;
 //   3. a lane that stopped early walks the rest of its candidates with inline forces.
 // Each lane still adds its contributions in its own visiting order, so the sums are bitwise
 // those of walk_binned. Called by every lane of the wavefront (inactive lanes have no cell).
-template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE>
+template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE, bool SYM>
 __device__ __forceinline__ void walk_binned_deferred(const SweepArgs& a, const InstanceDesc& in,
-                                                     int s, bool active, float4 me,
+                                                     int s, int me_id, bool active, float4 me,
                                                      const Pol& pme, CellState& cs, int* list) {
   const float4* __restrict__ xs = a.xs;
   const int* __restrict__ offb = a.off + in.bin_begin;
@@ -137,8 +173,8 @@ __device__ __forceinline__ void walk_binned_deferred(const SweepArgs& a, const I
   int stop_r = 9, stop_m = 0;  // resume point of a lane whose list filled up
   if (active) {
     int bc[3];
-    if (GABRIEL)
-      bin_coords_div(in, me.x, me.y, me.z, bc);  // binned_gabriel.hpp:101
+    if (GABRIEL || SYM)
+      bin_coords_div(in, me.x, me.y, me.z, bc);  // binned_gabriel.hpp:101, *_symmetric.hpp
     else
       bin_coords_mul(in, me.x, me.y, me.z, bc);  // binned_all_pairs.hpp:103-109
     const int x_lo = max(bc[0] - 1, 0);
@@ -158,6 +194,7 @@ __device__ __forceinline__ void walk_binned_deferred(const SweepArgs& a, const I
         const float disp[3] = {me.x - pj.x, me.y - pj.y, me.z - pj.z};
         const float d2 = len2(disp[0], disp[1], disp[2]);
         if (d2 >= r2) continue;
+        if (SYM && a.id[m] >= me_id) continue;  // j < i only
         if (GABRIEL) {
           if (gabriel_blocked(in, xs, offb, s, m, me, disp, d2)) continue;
         }
@@ -178,7 +215,7 @@ __device__ __forceinline__ void walk_binned_deferred(const SweepArgs& a, const I
       const float4 pj = xs[m];
       const float disp[3] = {me.x - pj.x, me.y - pj.y, me.z - pj.z};
       const float d2 = len2(disp[0], disp[1], disp[2]);
-      visit<HAS_TYPE, HAS_POL, MODE>(a, in, cs, me, pme, m, pj, disp, d2);
+      visit<HAS_TYPE, HAS_POL, MODE, SYM>(a, in, cs, me, pme, m, pj, disp, d2);
     }
   }
   // 3. the rest of a long walk, inline
@@ -191,10 +228,11 @@ __device__ __forceinline__ void walk_binned_deferred(const SweepArgs& a, const I
         const float disp[3] = {me.x - pj.x, me.y - pj.y, me.z - pj.z};
         const float d2 = len2(disp[0], disp[1], disp[2]);
         if (d2 >= r2) continue;
+        if (SYM && a.id[m] >= me_id) continue;  // j < i only
         if (GABRIEL) {
           if (gabriel_blocked(in, xs, offb, s, m, me, disp, d2)) continue;
         }
-        visit<HAS_TYPE, HAS_POL, MODE>(a, in, cs, me, pme, m, pj, disp, d2);
+        visit<HAS_TYPE, HAS_POL, MODE, SYM>(a, in, cs, me, pme, m, pj, disp, d2);
       }
     }
   }
@@ -238,7 +276,7 @@ __device__ __forceinline__ CorrIn load_corr(const SweepArgs& a, int s) {
 
 // Epilogue shared by the binned and naive sweeps: turn the accumulators into the evaluation
 // result and apply the mode's integrator update.
-template <bool HAS_POL, int MODE>
+template <bool HAS_POL, int MODE, bool SYM>
 __device__ __forceinline__ void finish(const SweepArgs& a, const InstanceDesc& in, int lb, int s,
                                        bool active, float4 me, float2 mp, CellState& cs,
                                        const CorrIn& ci) {
@@ -249,6 +287,14 @@ __device__ __forceinline__ void finish(const SweepArgs& a, const InstanceDesc& i
     return;
   }
   if (MODE == kModeEmit) return;
+  if (SYM) {
+    // i's own sum joins the partners' contributions already in the buffer; the drag pass
+    // (binned_all_pairs_symmetric.hpp:212-218) adds +0
+    if (active && cs.pairs > 0) atomic_add_delta<HAS_POL>(a, s, cs.acc_pos, cs.acc_pol);
+    const unsigned tp = block_sum_u(active ? cs.pairs : 0u, lds_u);
+    if (threadIdx.x == 0) a.block_pairs[lb] += tp;
+    return;
+  }
   // out(i) += total (the delta buffer is zero before an evaluation), then
   // out.pos(i) += total_velocity / fmax(total_drag, eps) which is +0 because no force term
   // writes drag (binned_all_pairs.hpp:174-180).
@@ -315,7 +361,7 @@ __device__ __forceinline__ void finish(const SweepArgs& a, const InstanceDesc& i
   if (threadIdx.x == 0) a.block_pairs[lb] += tp;
 }
 
-template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE>
+template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE, bool SYM>
 __global__ __launch_bounds__(kBlock) void k_sweep_binned(SweepArgs a) {
   const int lb = logical_block();
   const int ii = a.block_inst[lb];
@@ -327,8 +373,10 @@ __global__ __launch_bounds__(kBlock) void k_sweep_binned(SweepArgs a) {
   float2 mp = make_float2(0.0f, 0.0f);
   Pol pme{};
   CorrIn ci{};
+  int me_id = 0;
   if (active) {
     me = a.xs[s];
+    if (SYM) me_id = a.id[s];
     if (HAS_POL) {
       mp = a.ps[s];
       pme = make_pol(mp, a.trig[s]);
@@ -340,18 +388,19 @@ __global__ __launch_bounds__(kBlock) void k_sweep_binned(SweepArgs a) {
   if (kForce && in.search_radius == 1) {
     // the instance decides the path for the whole workgroup (all its slots are one instance)
     __shared__ int list[kDeferMax * kBlock];
-    walk_binned_deferred<GABRIEL, HAS_TYPE, HAS_POL, MODE>(a, in, s, active, me, pme, cs,
-                                                           list + threadIdx.x);
+    walk_binned_deferred<GABRIEL, HAS_TYPE, HAS_POL, MODE, SYM>(a, in, s, me_id, active, me, pme,
+                                                                cs, list + threadIdx.x);
   } else if (active) {
-    walk_binned<GABRIEL, HAS_TYPE, HAS_POL, MODE>(a, in, s, me, pme, cs);
+    walk_binned<GABRIEL, HAS_TYPE, HAS_POL, MODE, SYM>(a, in, s, me_id, me, pme, cs);
   }
-  finish<HAS_POL, MODE>(a, in, lb, s, active, me, mp, cs, ci);
+  finish<HAS_POL, MODE, SYM>(a, in, lb, s, active, me, mp, cs, ci);
 }
 
 // Naive finders (naive_all_pairs.hpp:41-103, naive_gabriel.hpp:47-119): j runs over all agents
 // in ascending order; tiles of j positions are staged in LDS and shared by the workgroup.
-// Slots of a naive-finder instance are never re-sorted, so slot order is agent order.
-template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE>
+// Slots of a naive-finder instance are never re-sorted, so slot order is agent order (and the
+// symmetric finders' j < i is m < s).
+template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE, bool SYM>
 __global__ __launch_bounds__(kBlock) void k_sweep_naive(SweepArgs a) {
   __shared__ float4 tile[kBlock];
   const int lb = logical_block();
@@ -383,7 +432,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_naive(SweepArgs a) {
     if (!active) continue;
     for (int t = 0; t < cnt; ++t) {
       const int m = begin + base + t;
-      if (m == s) continue;
+      if (SYM ? m >= s : m == s) continue;
       const float4 pj = tile[t];
       const float disp[3] = {me.x - pj.x, me.y - pj.y, me.z - pj.z};
       const float d2 = len2(disp[0], disp[1], disp[2]);
@@ -405,39 +454,39 @@ __global__ __launch_bounds__(kBlock) void k_sweep_naive(SweepArgs a) {
         }
         if (blocked) continue;
       }
-      visit<HAS_TYPE, HAS_POL, MODE>(a, in, cs, me, pme, m, pj, disp, d2);
+      visit<HAS_TYPE, HAS_POL, MODE, SYM>(a, in, cs, me, pme, m, pj, disp, d2);
     }
   }
-  finish<HAS_POL, MODE>(a, in, lb, s, active, me, mp, cs, ci);
+  finish<HAS_POL, MODE, SYM>(a, in, lb, s, active, me, mp, cs, ci);
 }
 
-template <bool GABRIEL, bool BINNED, bool HAS_TYPE, bool HAS_POL, int MODE>
+template <bool GABRIEL, bool BINNED, bool HAS_TYPE, bool HAS_POL, int MODE, bool SYM>
 void launch_one(const SweepArgs& a, int n_blocks, hipStream_t stream) {
   if (BINNED)
-    hipLaunchKernelGGL((k_sweep_binned<GABRIEL, HAS_TYPE, HAS_POL, MODE>), dim3(n_blocks),
+    hipLaunchKernelGGL((k_sweep_binned<GABRIEL, HAS_TYPE, HAS_POL, MODE, SYM>), dim3(n_blocks),
                        dim3(kBlock), 0, stream, a);
   else
-    hipLaunchKernelGGL((k_sweep_naive<GABRIEL, HAS_TYPE, HAS_POL, MODE>), dim3(n_blocks),
+    hipLaunchKernelGGL((k_sweep_naive<GABRIEL, HAS_TYPE, HAS_POL, MODE, SYM>), dim3(n_blocks),
                        dim3(kBlock), 0, stream, a);
 }
 
-template <bool GABRIEL, bool BINNED, int MODE>
+template <bool GABRIEL, bool BINNED, int MODE, bool SYM>
 void launch_force(bool has_type, bool has_pol, const SweepArgs& a, int n_blocks,
                   hipStream_t stream) {
   if (MODE == kModeCount || MODE == kModeEmit) {
-    launch_one<GABRIEL, BINNED, false, false, MODE>(a, n_blocks, stream);
+    launch_one<GABRIEL, BINNED, false, false, MODE, SYM>(a, n_blocks, stream);
     return;
   }
   if (has_type) {
     if (has_pol)
-      launch_one<GABRIEL, BINNED, true, true, MODE>(a, n_blocks, stream);
+      launch_one<GABRIEL, BINNED, true, true, MODE, SYM>(a, n_blocks, stream);
     else
-      launch_one<GABRIEL, BINNED, true, false, MODE>(a, n_blocks, stream);
+      launch_one<GABRIEL, BINNED, true, false, MODE, SYM>(a, n_blocks, stream);
   } else {
     if (has_pol)
-      launch_one<GABRIEL, BINNED, false, true, MODE>(a, n_blocks, stream);
+      launch_one<GABRIEL, BINNED, false, true, MODE, SYM>(a, n_blocks, stream);
     else
-      launch_one<GABRIEL, BINNED, false, false, MODE>(a, n_blocks, stream);
+      launch_one<GABRIEL, BINNED, false, false, MODE, SYM>(a, n_blocks, stream);
   }
 }
 
@@ -445,16 +494,53 @@ template <bool GABRIEL, bool BINNED>
 void launch_mode(int mode, bool has_type, bool has_pol, const SweepArgs& a, int n_blocks,
                  hipStream_t stream) {
   switch (mode) {
-    case kModeDelta: launch_force<GABRIEL, BINNED, kModeDelta>(has_type, has_pol, a, n_blocks, stream); break;
-    case kModePred: launch_force<GABRIEL, BINNED, kModePred>(has_type, has_pol, a, n_blocks, stream); break;
-    case kModeCorr: launch_force<GABRIEL, BINNED, kModeCorr>(has_type, has_pol, a, n_blocks, stream); break;
-    case kModeEuler: launch_force<GABRIEL, BINNED, kModeEuler>(has_type, has_pol, a, n_blocks, stream); break;
-    case kModeCount: launch_force<GABRIEL, BINNED, kModeCount>(has_type, has_pol, a, n_blocks, stream); break;
-    default: launch_force<GABRIEL, BINNED, kModeEmit>(has_type, has_pol, a, n_blocks, stream); break;
+    case kModeDelta: launch_force<GABRIEL, BINNED, kModeDelta, false>(has_type, has_pol, a, n_blocks, stream); break;
+    case kModePred: launch_force<GABRIEL, BINNED, kModePred, false>(has_type, has_pol, a, n_blocks, stream); break;
+    case kModeCorr: launch_force<GABRIEL, BINNED, kModeCorr, false>(has_type, has_pol, a, n_blocks, stream); break;
+    case kModeEuler: launch_force<GABRIEL, BINNED, kModeEuler, false>(has_type, has_pol, a, n_blocks, stream); break;
+    case kModeCount: launch_force<GABRIEL, BINNED, kModeCount, false>(has_type, has_pol, a, n_blocks, stream); break;
+    default: launch_force<GABRIEL, BINNED, kModeEmit, false>(has_type, has_pol, a, n_blocks, stream); break;
   }
 }
 
+// Symmetric finders: Delta (atomic accumulation into a zeroed buffer), Count and Emit.
+template <bool GABRIEL, bool BINNED>
+int launch_mode_sym(int mode, bool has_type, bool has_pol, const SweepArgs& a, int n_blocks,
+                    hipStream_t stream) {
+  switch (mode) {
+    case kModeDelta: launch_force<GABRIEL, BINNED, kModeDelta, true>(has_type, has_pol, a, n_blocks, stream); return 0;
+    case kModeCount: launch_force<GABRIEL, BINNED, kModeCount, true>(has_type, has_pol, a, n_blocks, stream); return 0;
+    case kModeEmit: launch_force<GABRIEL, BINNED, kModeEmit, true>(has_type, has_pol, a, n_blocks, stream); return 0;
+    default: return -1;
+  }
+}
+
+// The delta buffers the symmetric sweeps accumulate into start at zero.
+__global__ __launch_bounds__(kBlock) void k_zero_deltas(float4* pos, float2* pol, int total) {
+  const int s = blockIdx.x * kBlock + (int)threadIdx.x;
+  if (s >= total) return;
+  pos[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (pol) pol[s] = make_float2(0.0f, 0.0f);
+}
+
 // ------------------------------------------------------------------ COM fix and updates
+// Per-workgroup sums of the finished position deltas (the symmetric sweeps cannot sum them in
+// their epilogue: other cells still add to a cell's delta until the sweep ends).
+__global__ __launch_bounds__(kBlock) void k_com_partials(const InstanceDesc* inst,
+                                                         const int* block_inst, const float4* d,
+                                                         float4* partial) {
+  __shared__ float lds[4];
+  const int lb = logical_block();
+  const InstanceDesc& in = inst[block_inst[lb]];
+  const int s = lb * kBlock + (int)threadIdx.x;
+  const bool active = (s - in.slot_begin) < in.n;
+  const float4 v = active ? d[s] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const float t0 = block_sum_f(v.x, lds);
+  const float t1 = block_sum_f(v.y, lds);
+  const float t2 = block_sum_f(v.z, lds);
+  if (threadIdx.x == 0) partial[lb] = make_float4(t0, t1, t2, 0.0f);
+}
+
 // GlobalComFixer::fix, com_fixers.hpp:35-43: sum of the position deltas over the capacity,
 // divided by the capacity; deterministic fixed-order reduction of the per-block partials.
 __global__ __launch_bounds__(kBlock) void k_com_finalize(const InstanceDesc* inst,
@@ -517,6 +603,14 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
       a.pol[s] = np;
       a.trig[s] = polar_trig(np);
     }
+    if (a.clear) {  // d1 = d3 = 0 (heun.hpp:180-181) for the next accumulation
+      a.d1_pos[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      a.d3_pos[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (HAS_POL) {
+        a.d1_pol[s] = make_float2(0.0f, 0.0f);
+        a.d3_pol[s] = make_float2(0.0f, 0.0f);
+      }
+    }
   } else {
     // euler.hpp:56-66
     if (a.old_vel) a.old_vel[s] = d1;
@@ -528,19 +622,39 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
       a.pol[s] = np;
       a.trig[s] = polar_trig(np);
     }
+    if (a.clear) {  // d = 0 (euler.hpp:63-64) for the next accumulation
+      a.d1_pos[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (HAS_POL) a.d1_pol[s] = make_float2(0.0f, 0.0f);
+    }
   }
 }
 
 }  // namespace
 
-void launch_sweep(int finder, bool has_type, bool has_pol, int mode, const SweepArgs& a,
-                  int n_blocks, hipStream_t stream) {
+int launch_sweep(int finder, bool has_type, bool has_pol, int mode, const SweepArgs& a,
+                 int n_blocks, hipStream_t stream) {
   switch (finder) {
-    case kBinnedAllPairs: launch_mode<false, true>(mode, has_type, has_pol, a, n_blocks, stream); break;
-    case kBinnedGabriel: launch_mode<true, true>(mode, has_type, has_pol, a, n_blocks, stream); break;
-    case kNaiveAllPairs: launch_mode<false, false>(mode, has_type, has_pol, a, n_blocks, stream); break;
-    default: launch_mode<true, false>(mode, has_type, has_pol, a, n_blocks, stream); break;
+    case kBinnedAllPairs: launch_mode<false, true>(mode, has_type, has_pol, a, n_blocks, stream); return 0;
+    case kBinnedGabriel: launch_mode<true, true>(mode, has_type, has_pol, a, n_blocks, stream); return 0;
+    case kNaiveAllPairs: launch_mode<false, false>(mode, has_type, has_pol, a, n_blocks, stream); return 0;
+    case kNaiveGabriel: launch_mode<true, false>(mode, has_type, has_pol, a, n_blocks, stream); return 0;
+    case kBinnedAllPairsSym: return launch_mode_sym<false, true>(mode, has_type, has_pol, a, n_blocks, stream);
+    case kBinnedGabrielSym: return launch_mode_sym<true, true>(mode, has_type, has_pol, a, n_blocks, stream);
+    case kNaiveAllPairsSym: return launch_mode_sym<false, false>(mode, has_type, has_pol, a, n_blocks, stream);
+    case kNaiveGabrielSym: return launch_mode_sym<true, false>(mode, has_type, has_pol, a, n_blocks, stream);
+    default: return -1;
   }
+}
+
+void launch_zero_deltas(float4* pos, float2* pol, int total_slots, hipStream_t stream) {
+  hipLaunchKernelGGL(k_zero_deltas, dim3((total_slots + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                     stream, pos, pol, total_slots);
+}
+
+void launch_com_partials(const InstanceDesc* inst, const int* block_inst, const float4* d,
+                         float4* partial, int n_blocks, hipStream_t stream) {
+  hipLaunchKernelGGL(k_com_partials, dim3(n_blocks), dim3(kBlock), 0, stream, inst, block_inst, d,
+                     partial);
 }
 
 void launch_com_finalize(const InstanceDesc* inst, int n_inst, const float4* partial,

@@ -8,7 +8,25 @@
 
 namespace tecs_amd {
 
-enum Finder { kNaiveAllPairs = 0, kBinnedAllPairs = 1, kNaiveGabriel = 2, kBinnedGabriel = 3 };
+enum Finder {
+  kNaiveAllPairs = 0,
+  kBinnedAllPairs = 1,
+  kNaiveGabriel = 2,
+  kBinnedGabriel = 3,
+  kNaiveAllPairsSym = 4,  // the symmetric finders: partners j < i, out(i) += v, out(j) -= v
+  kBinnedAllPairsSym = 5,
+  kNaiveGabrielSym = 6,
+  kBinnedGabrielSym = 7
+};
+inline bool is_symmetric(int finder) { return finder >= kNaiveAllPairsSym && finder <= kBinnedGabrielSym; }
+inline bool is_binned(int finder) {
+  return finder == kBinnedAllPairs || finder == kBinnedGabriel || finder == kBinnedAllPairsSym ||
+         finder == kBinnedGabrielSym;
+}
+inline bool is_gabriel(int finder) {
+  return finder == kNaiveGabriel || finder == kBinnedGabriel || finder == kNaiveGabrielSym ||
+         finder == kBinnedGabrielSym;
+}
 
 // What a sweep does with the force sum F of each cell (dt and the COM fix come from the
 // instance descriptor / com buffer).
@@ -46,8 +64,14 @@ struct SweepArgs {
   int32_t* cols;           // Emit
 };
 
-void launch_sweep(int finder, bool has_type, bool has_pol, int mode, const SweepArgs& a,
-                  int n_blocks, hipStream_t stream);
+// Returns nonzero for a mode the finder does not have (symmetric finders: Delta, Count, Emit;
+// Delta accumulates atomically into o_pos / o_pol, which must hold zeros).
+int launch_sweep(int finder, bool has_type, bool has_pol, int mode, const SweepArgs& a,
+                 int n_blocks, hipStream_t stream);
+void launch_zero_deltas(float4* pos, float2* pol, int total_slots, hipStream_t stream);
+// com partial of every 256-slot workgroup from the finished deltas d (one per workgroup)
+void launch_com_partials(const InstanceDesc* inst, const int* block_inst, const float4* d,
+                         float4* partial, int n_blocks, hipStream_t stream);
 
 struct UpdateArgs {
   const InstanceDesc* inst;
@@ -57,13 +81,14 @@ struct UpdateArgs {
   float2* pol;
   float4* d1_pos;     // first (or only) evaluation, COM-fixed in place
   float2* d1_pol;
-  const float4* d3_pos;  // second evaluation (Heun corrector)
-  const float2* d3_pol;
+  float4* d3_pos;        // second evaluation (Heun corrector; cleared with `clear`)
+  float2* d3_pol;
   float4* xp;            // predicted positions (Heun predictor)
   float2* polp;
   float4* trig;          // polar_trig of pol (corrector, Euler) ...
   float4* trigp;         // ... and of polp (predictor)
   float4* old_vel;
+  bool clear;            // corrector / Euler: zero the consumed deltas (symmetric sweeps add to them)
 };
 enum UpdateKind { kUpdatePred = 0, kUpdateCorr = 1, kUpdateEuler = 2 };
 void launch_update(int kind, bool has_pol, const UpdateArgs& a, int n_blocks, hipStream_t stream);

@@ -51,6 +51,11 @@ static int host_checks() {
   EXPECT(g.bin_size_scale == 1.0f && g.rebuild_every_n == 0 && g.gabriel_coefficient == 0.8f);
   pair_finders::NaiveGabriel::Settings ng;
   EXPECT(ng.gabriel_coefficient == 0.8f);
+  pair_finders::BinnedGabrielSymmetric::Settings gs{0.9f};  // coefficient first, as in the reference
+  EXPECT(gs.gabriel_coefficient == 0.9f && gs.min_bounds[1] == -20.0f && gs.bin_size_scale == 1.0f);
+  EXPECT((std::is_same<pair_finders::BinnedAllPairsSymmetric::Settings,
+                       pair_finders::BinnedAllPairs::Settings>::value));
+  EXPECT(pair_finders::NaiveGabrielSymmetric::kind == TECS_NAIVE_GABRIEL_SYMMETRIC);
   EXPECT((std::is_same<DefaultSimulationConfig::pair_finder_t, pair_finders::NaiveAllPairs>::value));
   EXPECT((std::is_same<DefaultSimulationConfig::integrator_t, integrators::Heun>::value));
 
