@@ -105,6 +105,9 @@ def main():
     ap.add_argument("--requests", type=int, default=48, help="serve workload: number of requests")
     ap.add_argument("--rate", type=float, default=8.0, help="serve workload: arrivals per second")
     ap.add_argument("--no-plain", action="store_true", help="serve workload: skip the one-at-a-time run")
+    ap.add_argument("--symmetric", action="store_true",
+                    help="epithelium / relaxation / sorting with the symmetric variant of their pair "
+                    "finder (one functor call per unordered pair)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this engine runs each simulation on a single device; "
@@ -132,12 +135,14 @@ def main():
         return bench_sweep(args)
     if args.workload == "serve":
         return bench_serve(args)
+    sym = {"finder": abi.BINNED_GABRIEL_SYMMETRIC if args.workload == "epithelium"
+           else abi.BINNED_ALL_PAIRS_SYMMETRIC} if args.symmetric else {}
     if args.workload == "relaxation":
-        w = workloads.relaxation(10_000 if args.n == 200_000 else args.n)
+        w = workloads.relaxation(10_000 if args.n == 200_000 else args.n, **sym)
     elif args.workload == "sorting":
-        w = workloads.sorting(4_000_000 if args.n == 200_000 else args.n)
+        w = workloads.sorting(4_000_000 if args.n == 200_000 else args.n, **sym)
     else:
-        w = workloads.epithelium(args.n)
+        w = workloads.epithelium(args.n, **sym)
     return bench_single(args, w)
 
 
@@ -185,7 +190,13 @@ def bench_single(args, w):
     has_pol = bool(w.desc.fields & abi.FIELD_POLARITY)
     has_type = bool(w.desc.fields & abi.FIELD_TYPE)
     bpc = algorithmic_bytes_per_cell(has_pol, has_type)
-    if w.desc.integrator == abi.EULER:
+    symmetric = w.desc.pair_finder in abi.SYMMETRIC_FINDERS
+    if symmetric:
+        # Delta sweep into a zeroed buffer: read s + a, the delta written once (s); the
+        # integrator update is its own kernel
+        s_bytes = 12 + (8 if has_pol else 0)
+        bpc = {"full": 2 * s_bytes + (1 if has_type else 0), "stale": 2 * s_bytes + (1 if has_type else 0)}
+    elif w.desc.integrator == abi.EULER:
         # Euler sweep: read s + a, write x' (s) and the old velocity (12)
         s_bytes = 12 + (8 if has_pol else 0)
         bpc = {"full": 2 * s_bytes + (1 if has_type else 0) + 12, "stale": 0}
@@ -199,12 +210,15 @@ def bench_single(args, w):
     variant = abi.SWEEP_NAMES.get(sim.sweep_variant(), "?")
     kname = "k_sweep_binned" if variant == "thread" else f"k_sweep_team<{variant[4:]}>"
     fname = abi.FINDER_NAMES[w.desc.pair_finder]
-    if w.desc.integrator == abi.EULER:
+    if symmetric:
+        kernel_names = {"full": f"{kname} {fname} mode Delta (atomic)",
+                        "stale": f"{kname} {fname} mode Delta (atomic, stale grid)"}
+    elif w.desc.integrator == abi.EULER:
         kernel_names = {"full": f"{kname} {fname} mode Euler", "stale": "-"}
     else:
         kernel_names = {"full": f"{kname} {fname} mode Pred (Heun substep 1)",
                         "stale": f"{kname} {fname} mode Corr (Heun substep 2, stale grid)"}
-    traffic = measured_traffic(w.name, variant, dom)
+    traffic = None if symmetric else measured_traffic(w.name, variant, dom)
     print(
         f"bench: sweep full {per['full'][0]*1e3:.1f} us, stale {per['stale'][0]*1e3:.1f} us, "
         f"rebuild {reb_ms/max(reb_n,1)*1e3:.1f} us per launch",
@@ -227,7 +241,8 @@ def bench_single(args, w):
         "dtype": "f32",
         "data": "synthetic (seeded Philox, seed 2807)",
         "config": {
-            "workload": WORKLOAD_TEXT[w.name].format(n=w.n),
+            "workload": WORKLOAD_TEXT[w.name].format(n=w.n).replace(
+                fname.replace("Symmetric", ""), fname),
             "cells": w.n,
             "pair_interactions_per_step": pairs / max(args.steps, 1),
             "parallelism": "single GPU",

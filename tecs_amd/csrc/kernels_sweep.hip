@@ -150,15 +150,33 @@ __device__ __forceinline__ void walk_binned(const SweepArgs& a, const InstanceDe
 // Accepted neighbours a thread of the deferred-force sweep keeps in LDS.
 constexpr int kDeferMax = 16;
 
+// One candidate m of cell s: the reference's tests in its order (distance, for the symmetric
+// finders the partner index, then the Gabriel ball; binned_all_pairs.hpp:140-151,
+// binned_gabriel.hpp:123-209). A NaN distance is accepted, as `d2 >= r2` is false for it.
+template <bool GABRIEL, bool SYM>
+__device__ __forceinline__ bool accepted(const SweepArgs& a, const InstanceDesc& in,
+                                         const float4* __restrict__ xs,
+                                         const int* __restrict__ offb, int s, int me_id,
+                                         float4 me, int m, float4 pj) {
+  const float disp[3] = {me.x - pj.x, me.y - pj.y, me.z - pj.z};
+  const float d2 = len2(disp[0], disp[1], disp[2]);
+  if (d2 >= in.cutoff2 || m == s) return false;
+  if (SYM && a.id[m] >= me_id) return false;
+  if (GABRIEL && gabriel_blocked(in, xs, offb, s, m, me, disp, d2)) return false;
+  return true;
+}
+
 // The binned sweep with the force evaluations taken out of the candidate loop (search radius
 // 1, force modes). In the one-thread-per-cell walk, the lanes of a wavefront accept neighbours
 // at different iterations, so the force code runs in nearly every iteration with few lanes
 // active. Here
-//   1. each lane walks its candidates as walk_binned does and appends the accepted slots to
-//      its LDS list (at most kDeferMax; a lane whose list is full remembers where it stopped);
+//   1. each lane walks its candidates in the reference's order, two per iteration (both
+//      positions are loaded before either is tested), and appends the accepted slots to its
+//      LDS list; it counts every accepted candidate, the list keeps the first kDeferMax;
 //   2. the wavefront evaluates the forces list entry by list entry, every lane its own e-th
 //      pair, so the force code runs max(list length) times with most lanes active;
-//   3. a lane that stopped early walks the rest of its candidates with inline forces.
+//   3. a lane that accepted more than kDeferMax walks again and evaluates the ones after the
+//      first kDeferMax inline.
 // Each lane still adds its contributions in its own visiting order, so the sums are bitwise
 // those of walk_binned. Called by every lane of the wavefront (inactive lanes have no cell).
 template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE, bool SYM>
@@ -167,10 +185,8 @@ __device__ __forceinline__ void walk_binned_deferred(const SweepArgs& a, const I
                                                      const Pol& pme, CellState& cs, int* list) {
   const float4* __restrict__ xs = a.xs;
   const int* __restrict__ offb = a.off + in.bin_begin;
-  const float r2 = in.cutoff2;
-  int na = 0;
+  int na = 0;  // accepted candidates, also those beyond the list
   int rs[9], re[9];
-  int stop_r = 9, stop_m = 0;  // resume point of a lane whose list filled up
   if (active) {
     int bc[3];
     if (GABRIEL || SYM)
@@ -186,31 +202,36 @@ __device__ __forceinline__ void walk_binned_deferred(const SweepArgs& a, const I
       rs[r] = ok ? offb[flatten(in, x_lo, y, z)] : 0;
       re[r] = ok ? offb[flatten(in, x_hi, y, z) + 1] : 0;
     }
-    for (int r = 0; r < 9 && stop_r == 9; ++r) {
+    for (int r = 0; r < 9; ++r) {
       const int end = re[r];
-      for (int m = rs[r]; m < end; ++m) {
-        if (m == s) continue;
-        const float4 pj = xs[m];
-        const float disp[3] = {me.x - pj.x, me.y - pj.y, me.z - pj.z};
-        const float d2 = len2(disp[0], disp[1], disp[2]);
-        if (d2 >= r2) continue;
-        if (SYM && a.id[m] >= me_id) continue;  // j < i only
-        if (GABRIEL) {
-          if (gabriel_blocked(in, xs, offb, s, m, me, disp, d2)) continue;
+      int m = rs[r];
+      for (; m + 1 < end; m += 2) {
+        const float4 p0 = xs[m];
+        const float4 p1 = xs[m + 1];
+        const bool a0 = accepted<GABRIEL, SYM>(a, in, xs, offb, s, me_id, me, m, p0);
+        const bool a1 = accepted<GABRIEL, SYM>(a, in, xs, offb, s, me_id, me, m + 1, p1);
+        if (a0) {
+          if (na < kDeferMax) list[na * kBlock] = m;
+          ++na;
         }
-        if (na == kDeferMax) {
-          stop_r = r;
-          stop_m = m;
-          break;
+        if (a1) {
+          if (na < kDeferMax) list[na * kBlock] = m + 1;
+          ++na;
         }
-        list[na * kBlock] = m;
-        ++na;
+      }
+      if (m < end) {
+        const float4 p0 = xs[m];
+        if (accepted<GABRIEL, SYM>(a, in, xs, offb, s, me_id, me, m, p0)) {
+          if (na < kDeferMax) list[na * kBlock] = m;
+          ++na;
+        }
       }
     }
   }
   // 2. forces, one list entry per lane per round
-  for (int e = 0; __ballot(e < na) != 0ull; ++e) {
-    if (e < na) {
+  const int nl = min(na, kDeferMax);
+  for (int e = 0; __ballot(e < nl) != 0ull; ++e) {
+    if (e < nl) {
       const int m = list[e * kBlock];
       const float4 pj = xs[m];
       const float disp[3] = {me.x - pj.x, me.y - pj.y, me.z - pj.z};
@@ -218,20 +239,17 @@ __device__ __forceinline__ void walk_binned_deferred(const SweepArgs& a, const I
       visit<HAS_TYPE, HAS_POL, MODE, SYM>(a, in, cs, me, pme, m, pj, disp, d2);
     }
   }
-  // 3. the rest of a long walk, inline
-  if (stop_r < 9) {
-    for (int r = stop_r; r < 9; ++r) {
+  // 3. the accepted candidates after the first kDeferMax, inline and in order
+  if (na > kDeferMax) {
+    int k = 0;
+    for (int r = 0; r < 9; ++r) {
       const int end = re[r];
-      for (int m = (r == stop_r ? stop_m : rs[r]); m < end; ++m) {
-        if (m == s) continue;
+      for (int m = rs[r]; m < end; ++m) {
         const float4 pj = xs[m];
+        if (!accepted<GABRIEL, SYM>(a, in, xs, offb, s, me_id, me, m, pj)) continue;
+        if (k++ < kDeferMax) continue;
         const float disp[3] = {me.x - pj.x, me.y - pj.y, me.z - pj.z};
         const float d2 = len2(disp[0], disp[1], disp[2]);
-        if (d2 >= r2) continue;
-        if (SYM && a.id[m] >= me_id) continue;  // j < i only
-        if (GABRIEL) {
-          if (gabriel_blocked(in, xs, offb, s, m, me, disp, d2)) continue;
-        }
         visit<HAS_TYPE, HAS_POL, MODE, SYM>(a, in, cs, me, pme, m, pj, disp, d2);
       }
     }
