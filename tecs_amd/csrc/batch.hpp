@@ -178,6 +178,8 @@ struct Batch {
   float2* d1p = nullptr;
   float4* d3 = nullptr;
   float2* d3p = nullptr;
+  float4* q = nullptr;   // binned symmetric finders: contributions scattered by movers (zero
+  float2* qp = nullptr;  // between evaluations; the consuming update clears what it adds)
   int* keys = nullptr;
   int* tmp = nullptr;
   int* tkey = nullptr;
@@ -221,9 +223,8 @@ struct Batch {
   // the next rebuild derives the new order from the last one: a grid exists and the state has
   // only moved by steps since (an upload or a member copy clears this)
   bool inc_ok = false;
-  // symmetric finders: d1 (and d3) hold zeros, ready for the next atomic accumulation (the
-  // updates clear what they consume; evaluate() leaves its result in d1)
-  bool deltas_zero = false;
+  // the last maybe_rebuild built the grid: the evaluation that follows sees a fresh grid
+  bool rebuilt_now = false;
   tecs_stats stats{};
 
   bool timing = false;
@@ -243,10 +244,10 @@ struct Batch {
   int graph_mode = TECS_GRAPHS_AUTO;
   struct HostState {
     int xc, ic, step_count;
-    bool keys_ready, inc_ok, deltas_zero;
+    bool keys_ready, inc_ok;
     bool operator==(const HostState& o) const {
       return xc == o.xc && ic == o.ic && step_count == o.step_count &&
-             keys_ready == o.keys_ready && inc_ok == o.inc_ok && deltas_zero == o.deltas_zero;
+             keys_ready == o.keys_ready && inc_ok == o.inc_ok;
     }
   };
   struct StepGraph {
@@ -266,7 +267,7 @@ struct Batch {
   std::vector<unsigned> write_step_count;  // Simulation::current_step per member
 
   HostState host_state() const {
-    return HostState{xc, ic, step_count, keys_ready, inc_ok, deltas_zero};
+    return HostState{xc, ic, step_count, keys_ready, inc_ok};
   }
   int graph_period() const { return 2 * std::max(1, is_binned(finder) ? rebuild_every_n : 1); }
 
@@ -475,6 +476,14 @@ struct Batch {
       TRY(alloc(&d3, S));
       if (has_pol) TRY(alloc(&d3p, S));
     }
+    if (is_symmetric(finder) && is_binned(finder)) {
+      TRY(alloc(&q, S));
+      HIP_TRY(hipMemsetAsync(q, 0, sizeof(float4) * S, stream));
+      if (has_pol) {
+        TRY(alloc(&qp, S));
+        HIP_TRY(hipMemsetAsync(qp, 0, sizeof(float2) * S, stream));
+      }
+    }
     if (is_binned(finder)) {
       TRY(alloc(&keys, S));
       TRY(alloc(&tmp, S));
@@ -502,13 +511,13 @@ struct Batch {
       TRY(alloc(&bsum, (size_t)scan_tiles_for(total_cbins)));
       HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(int) * (size_t)total_cbins, stream));
     }
-    // team sweep conditions: an ordered binned finder, search radius 1 and, for Gabriel,
-    // c <= 0.95 (kernels_team.hip)
-    team_ok = is_binned(finder) && !is_symmetric(finder);
+    // team sweep conditions: a binned finder, search radius 1 and, for Gabriel, c <= 0.95
+    // (kernels_team.hip)
+    team_ok = is_binned(finder);
     for (int k = 0; k < count; ++k) {
       if (h_inst[k].search_radius != 1) team_ok = false;
       const float c = ds[k].pair_finder_settings.gabriel_coefficient;
-      if (finder == kBinnedGabriel && !(c >= 0.0f && c <= 0.95f)) team_ok = false;
+      if (is_gabriel(finder) && !(c >= 0.0f && c <= 0.95f)) team_ok = false;
     }
     TRY(alloc(&block_pairs, (size_t)n_blocks * kMaxTeamLanes));
     TRY(alloc(&tune_pairs, (size_t)n_blocks * kMaxTeamLanes));
@@ -669,11 +678,13 @@ struct Batch {
 
   // The reference rebuild schedule for a full evaluation (binned_all_pairs.hpp:77-84).
   int maybe_rebuild(bool is_full_step) {
+    rebuilt_now = false;
     if (!is_binned(finder)) return TECS_OK;
     if (is_full_step) {
       if (step_count == 0 || step_count >= rebuild_every_n) {
         TRY(rebuild());
         step_count = 0;
+        rebuilt_now = true;
       }
       step_count++;
     }
@@ -716,9 +727,9 @@ struct Batch {
     if (v == TECS_SWEEP_THREAD) {
       if (launch_sweep(finder, has_type, has_pol, mode, a, n_blocks, stream) != 0)
         return fail(TECS_ERR_STATE, "sweep mode not available for this pair finder");
-    } else {
-      launch_sweep_team(lanes_of(v), finder == kBinnedGabriel, has_type, has_pol, mode, a,
-                        n_blocks, stream);
+    } else if (launch_sweep_team(lanes_of(v), is_gabriel(finder), is_symmetric(finder), has_type,
+                                 has_pol, mode, a, n_blocks, stream) != 0) {
+      return fail(TECS_ERR_STATE, "sweep mode not available for this pair finder");
     }
     return TECS_OK;
   }
@@ -787,7 +798,9 @@ struct Batch {
     t.o_pos = d1;
     t.o_pol = d1p;
     t.block_pairs = tune_pairs;
-    if (!com_on && integrator == TECS_HEUN) {
+    // symmetric finders: a Delta sweep as on a fresh grid (no movers, so nothing goes to q)
+    if (is_symmetric(finder)) {
+    } else if (!com_on && integrator == TECS_HEUN) {
       mode = kModePred;
       t.o2_pos = xp;
       t.o2_pol = polp;
@@ -828,7 +841,9 @@ struct Batch {
   int sweep(int mode, const SweepArgs& a, int cls) {
     Pending p{};
     TRY(time_begin(&p, cls));
-    TRY(launch_variant(concrete_variant(), mode, a));
+    // the symmetric team sweeps evaluate forces only; the pair export walks one thread per cell
+    const int v = is_symmetric(finder) && mode != kModeDelta ? TECS_SWEEP_THREAD : concrete_variant();
+    TRY(launch_variant(v, mode, a));
     HIP_TRY(hipGetLastError());
     TRY(time_end(&p));
     if (mode != kModeCount && mode != kModeEmit) stats.evaluations++;
@@ -836,13 +851,13 @@ struct Batch {
   }
 
   // COM fix (when on) and the integrator update of a Delta-mode evaluation whose position
-  // deltas are F. The ordered sweeps wrote per-workgroup COM partials themselves; the
-  // symmetric ones cannot (deltas are final only when the sweep ends), so a pass computes them
-  // and the update clears the deltas it consumed for the next atomic accumulation.
-  int com_and_update(int kind, const float4* F = nullptr) {
+  // deltas are F. The ordered sweeps wrote per-workgroup COM partials themselves; for the
+  // symmetric ones a pass computes them from F (+ q, the movers' contributions, when the
+  // evaluation saw a stale grid: with_q), and the update adds q and clears it.
+  int com_and_update(int kind, const float4* F = nullptr, bool with_q = false) {
     const bool sym = is_symmetric(finder);
     if (com_fixer == TECS_GLOBAL_COM_FIXER) {
-      if (sym) launch_com_partials(d_inst, d_block_inst, F, com_partial, n_blocks, stream);
+      if (sym) launch_com_partials(d_inst, d_block_inst, F, with_q ? q : nullptr, com_partial, n_blocks, stream);
       launch_com_finalize(d_inst, n_inst, com_partial, sym ? 1 : partials_per_block(), com, stream);
     }
     UpdateArgs u{};
@@ -860,38 +875,43 @@ struct Batch {
     u.trig = trig_of(pol[xc]);
     u.trigp = trigp;
     u.old_vel = track_ov ? ov[xc] : nullptr;
-    u.clear = sym;
+    u.q_pos = with_q ? q : nullptr;
+    u.q_pol = with_q ? qp : nullptr;
     launch_update(kind, has_pol, u, n_blocks, stream);
     HIP_TRY(hipGetLastError());
     return TECS_OK;
   }
 
-  // d1 (and d3) = 0 unless known to be
-  void zero_deltas() {
-    if (deltas_zero) return;
-    launch_zero_deltas(d1, d1p, total_slots, stream);
-    if (d3) launch_zero_deltas(d3, d3p, total_slots, stream);
-    deltas_zero = true;
+  // A symmetric evaluation on a stale grid (positions other than those the grid was built
+  // from) hands the sweep the rebuild keys and the movers' buffer.
+  void sym_grid_args(SweepArgs* a, bool stale) const {
+    if (!stale) return;
+    a->skey = skey[ic];
+    a->q_pos = q;
+    a->q_pol = qp;
   }
 
-  // A step with a symmetric finder: each evaluation accumulates atomically into zeroed deltas,
-  // then the COM fix and the update run as their own kernels (the update zeroes the deltas
-  // again). Heun: evaluate -> predictor -> evaluate (stale grid) -> corrector.
+  // A step with a symmetric finder: each evaluation writes its deltas (the movers' part into
+  // q on a stale grid), then the COM fix and the update run as their own kernels.
+  // Heun: evaluate -> predictor -> evaluate (stale grid) -> corrector.
   int step_symmetric() {
-    zero_deltas();
+    const bool binned = is_binned(finder);
+    const bool stale1 = binned && !rebuilt_now;
     SweepArgs a = base_args(x[xc], pol[xc]);
     a.o_pos = d1;
     a.o_pol = d1p;
+    sym_grid_args(&a, stale1);
     TRY(sweep(kModeDelta, a, kTimeSweepFull));
     if (integrator == TECS_HEUN) {
-      TRY(com_and_update(kUpdatePred, d1));
+      TRY(com_and_update(kUpdatePred, d1, stale1));
       SweepArgs b = base_args(xp, polp);
       b.o_pos = d3;
       b.o_pol = d3p;
+      sym_grid_args(&b, binned);
       TRY(sweep(kModeDelta, b, kTimeSweepStale));
-      TRY(com_and_update(kUpdateCorr, d3));
+      TRY(com_and_update(kUpdateCorr, d3, binned));
     } else {
-      TRY(com_and_update(kUpdateEuler, d1));
+      TRY(com_and_update(kUpdateEuler, d1, stale1));
     }
     stats.steps++;
     return TECS_OK;
@@ -1102,12 +1122,13 @@ struct Batch {
   int evaluate(bool is_full_step, float* dpos, float* dpol) {
     TRY(sync_inst());
     TRY(maybe_rebuild(is_full_step));
-    if (is_symmetric(finder)) zero_deltas();
+    const bool sym_stale = is_symmetric(finder) && is_binned(finder) && !rebuilt_now;
     SweepArgs a = base_args(x[xc], pol[xc]);
     a.o_pos = d1;
     a.o_pol = d1p;
+    sym_grid_args(&a, sym_stale);
     TRY(sweep(kModeDelta, a, is_full_step ? kTimeSweepFull : kTimeSweepStale));
-    deltas_zero = false;  // d1 holds the result
+    if (sym_stale) launch_fold_q(d1, d1p, q, qp, total_slots, stream);
     const InstanceDesc& in = h_inst[0];
     launch_download(d_inst, 0, in.block_begin, in.n_blocks, id[ic], d1, d1p, nullptr,
                     dpos ? st_pos : nullptr, (dpol && has_pol) ? st_pol : nullptr, nullptr, stream);
@@ -1274,6 +1295,7 @@ struct Batch {
     if (d.integrator == TECS_HEUN) per_slot += 16 + (pol ? 8 : 0);
     if (d.integrator == TECS_HEUN && (d.com_fixer == TECS_GLOBAL_COM_FIXER || is_symmetric(d.pair_finder)))
       per_slot += 16 + (pol ? 8 : 0);
+    if (is_symmetric(d.pair_finder) && is_binned(d.pair_finder)) per_slot += 16 + (pol ? 8 : 0);
     size_t bins = 0;
     if (is_binned(d.pair_finder)) {
       const tecs_pair_finder_settings& s = d.pair_finder_settings;

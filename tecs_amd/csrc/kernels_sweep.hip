@@ -14,15 +14,16 @@
 // rebuild; the centre bin of i is recomputed from its current position, exactly like the
 // reference.
 //
-// Symmetric finders (SYM, binned_all_pairs_symmetric.hpp:148-219 and its three siblings): the
-// same walks keep only partners with a smaller agent index; the functor's result v of a pair
-// goes to i's register sum and, negated, to j with a hardware float atomic; i's sum is added
-// atomically at the end. The delta buffer must be zero before the sweep, and only the Delta,
-// Count and Emit modes exist (the integrator update runs as its own kernel afterwards).
+// Symmetric finders (binned_all_pairs_symmetric.hpp:148-219 and its three siblings): the pair
+// export walks keep only partners with a smaller agent index (SYM); the force evaluation runs
+// in the k_sym_* kernels below.
 #include "tecs_internal.hpp"
+#include "walks.hpp"
 
 namespace tecs_amd {
 namespace {
+
+using namespace walks;
 
 // Per-pair visit context; `Acc` carries the force accumulators of cell i.
 struct CellState {
@@ -31,21 +32,6 @@ struct CellState {
   unsigned pairs;
   int64_t emit_at;
 };
-
-// delta(j) += v with hardware float atomics (no return value, relaxed, device scope)
-template <bool HAS_POL>
-__device__ __forceinline__ void atomic_add_delta(const SweepArgs& a, int m, const float* vp,
-                                                 const float* vl) {
-  float* q = &a.o_pos[m].x;
-  unsafeAtomicAdd(q + 0, vp[0]);
-  unsafeAtomicAdd(q + 1, vp[1]);
-  unsafeAtomicAdd(q + 2, vp[2]);
-  if (HAS_POL) {
-    float* r = &a.o_pol[m].x;
-    unsafeAtomicAdd(r + 0, vl[0]);
-    unsafeAtomicAdd(r + 1, vl[1]);
-  }
-}
 
 template <bool HAS_TYPE, bool HAS_POL, int MODE, bool SYM>
 __device__ __forceinline__ void visit(const SweepArgs& a, const InstanceDesc& in, CellState& cs,
@@ -60,53 +46,9 @@ __device__ __forceinline__ void visit(const SweepArgs& a, const InstanceDesc& in
   const float d = sqrtf(d2);
   Pol po{};
   if (HAS_POL) po = make_pol(a.ps[m], a.trig[m]);
-  if (SYM) {
-    // the functor on a zeroed accumulator pack, then out(i) += v, out(j) -= v
-    // (binned_all_pairs_symmetric.hpp:176-200)
-    float vp[3] = {0.0f, 0.0f, 0.0f}, vl[2] = {0.0f, 0.0f};
-    apply_force<HAS_TYPE, HAS_POL>(in.force, disp, d, __float_as_int(me.w), __float_as_int(pj.w),
-                                   pme, po, vp, vl);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) cs.acc_pos[c] += vp[c];
-    cs.acc_pol[0] += vl[0];
-    cs.acc_pol[1] += vl[1];
-    const float np[3] = {-vp[0], -vp[1], -vp[2]}, nl[2] = {-vl[0], -vl[1]};
-    atomic_add_delta<HAS_POL>(a, m, np, nl);
-    return;
-  }
+  static_assert(!SYM || MODE == kModeCount || MODE == kModeEmit, "symmetric walks export pairs only");
   apply_force<HAS_TYPE, HAS_POL>(in.force, disp, d, __float_as_int(me.w), __float_as_int(pj.w),
                                  pme, po, cs.acc_pos, cs.acc_pol);
-}
-
-// Gabriel emptiness test of the ball with diameter c*|p_i - p_j| around the midpoint over the
-// members of the (stale) grid bins overlapping its bounding box, binned_gabriel.hpp:133-209.
-__device__ __forceinline__ bool gabriel_blocked(const InstanceDesc& in, const float4* __restrict__ xs,
-                                                const int* __restrict__ offb, int s, int m,
-                                                float4 me, const float* disp, float d2) {
-  const float mid0 = me.x - disp[0] * 0.5f;
-  const float mid1 = me.y - disp[1] * 0.5f;
-  const float mid2 = me.z - disp[2] * 0.5f;
-  const float rho2 = d2 * in.gabriel_factor;
-  const float rho = sqrtf(rho2);
-  const float mid[3] = {mid0, mid1, mid2};
-  int mn[3], mx[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    mn[d] = clampi((int)floorf((mid[d] - rho - in.minb[d]) * in.inv_bin), in.ext[d] - 1);
-    mx[d] = clampi((int)floorf((mid[d] + rho - in.minb[d]) * in.inv_bin), in.ext[d] - 1);
-  }
-  for (int b2 = mn[2]; b2 <= mx[2]; ++b2) {
-    for (int b1 = mn[1]; b1 <= mx[1]; ++b1) {
-      const int start = offb[flatten(in, mn[0], b1, b2)];
-      const int end = offb[flatten(in, mx[0], b1, b2) + 1];
-      for (int k = start; k < end; ++k) {
-        if (k == s || k == m) continue;
-        const float4 pk = xs[k];
-        if (len2(pk.x - mid0, pk.y - mid1, pk.z - mid2) < rho2) return true;
-      }
-    }
-  }
-  return false;
 }
 
 template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE, bool SYM>
@@ -305,14 +247,6 @@ __device__ __forceinline__ void finish(const SweepArgs& a, const InstanceDesc& i
     return;
   }
   if (MODE == kModeEmit) return;
-  if (SYM) {
-    // i's own sum joins the partners' contributions already in the buffer; the drag pass
-    // (binned_all_pairs_symmetric.hpp:212-218) adds +0
-    if (active && cs.pairs > 0) atomic_add_delta<HAS_POL>(a, s, cs.acc_pos, cs.acc_pol);
-    const unsigned tp = block_sum_u(active ? cs.pairs : 0u, lds_u);
-    if (threadIdx.x == 0) a.block_pairs[lb] += tp;
-    return;
-  }
   // out(i) += total (the delta buffer is zero before an evaluation), then
   // out.pos(i) += total_velocity / fmax(total_drag, eps) which is +0 because no force term
   // writes drag (binned_all_pairs.hpp:174-180).
@@ -478,6 +412,187 @@ __global__ __launch_bounds__(kBlock) void k_sweep_naive(SweepArgs a) {
   finish<HAS_POL, MODE, SYM>(a, in, lb, s, active, me, mp, cs, ci);
 }
 
+// ------------------------------------------------------------------ symmetric finders
+// Binned symmetric finders, Delta mode, the two-walk rule: o_pos / o_pol receive each cell's
+// own and gathered contributions (plain stores), q_pos / q_pol the movers' scattered ones.
+// STALE: a.skey is set (a stale grid, movers possible). Search radius 1: the stayers defer
+// their forces as in walk_binned_deferred, list entries carrying the role in bits 30-31; the
+// rare movers walk serially.
+template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, bool STALE>
+__global__ __launch_bounds__(kBlock) void k_sym_binned(SweepArgs a) {
+  __shared__ int list_all[kDeferMax * kBlock];
+  __shared__ unsigned lds_u[4];
+  const int lb = logical_block();
+  const InstanceDesc& in = a.inst[a.block_inst[lb]];
+  const int s = lb * kBlock + (int)threadIdx.x;
+  const bool active = (s - in.slot_begin) < in.n;
+  const float4* __restrict__ xs = a.xs;
+  const int* __restrict__ offb = a.off + in.bin_begin;
+  const int* skey = STALE ? a.skey : nullptr;
+  int* list = list_all + threadIdx.x;
+  float4 me = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  Pol pme{};
+  int me_id = 0;
+  int bc[3] = {0, 0, 0};
+  bool mover = false;
+  if (active) {
+    me = xs[s];
+    if (HAS_POL) pme = make_pol(a.ps[s], a.trig[s]);
+    me_id = a.id[s];
+    bin_coords_div(in, me.x, me.y, me.z, bc);  // calc_bin_coords_from_point, *_symmetric.hpp
+    if (STALE) mover = flatten(in, bc[0], bc[1], bc[2]) != skey[s];
+  }
+  float acc_pos[3] = {0.0f, 0.0f, 0.0f}, acc_pol[2] = {0.0f, 0.0f};
+  unsigned pairs = 0;
+  const bool stayer = active && !mover;
+  if (in.search_radius == 1) {
+    int na = 0;
+    int rs[9], re[9];
+    if (stayer) {
+      const int x_lo = max(bc[0] - 1, 0);
+      const int x_hi = min(bc[0] + 1, in.ext[0] - 1);
+#pragma unroll
+      for (int r = 0; r < 9; ++r) {
+        const int z = bc[2] + r / 3 - 1, y = bc[1] + r % 3 - 1;
+        const bool ok = z >= 0 && z < in.ext[2] && y >= 0 && y < in.ext[1];
+        rs[r] = ok ? offb[flatten(in, x_lo, y, z)] : 0;
+        re[r] = ok ? offb[flatten(in, x_hi, y, z) + 1] : 0;
+      }
+      for (int r = 0; r < 9; ++r) {
+        const int end = re[r];
+        int m = rs[r];
+        for (; m + 1 < end; m += 2) {
+          // positions and agent indices of both candidates are loaded before any test
+          const float4 p0 = xs[m];
+          const float4 p1 = xs[m + 1];
+          const int i0 = a.id[m], i1 = a.id[m + 1];
+          const int r0 = sym_role<GABRIEL, false>(in, skey, xs, offb, s, me_id, false, me, m, p0, i0);
+          const int r1 = sym_role<GABRIEL, false>(in, skey, xs, offb, s, me_id, false, me, m + 1, p1, i1);
+          if (r0 != kRoleNone) {
+            if (na < kDeferMax) list[na * kBlock] = (int)((unsigned)m | ((unsigned)r0 << 30));
+            ++na;
+          }
+          if (r1 != kRoleNone) {
+            if (na < kDeferMax) list[na * kBlock] = (int)((unsigned)(m + 1) | ((unsigned)r1 << 30));
+            ++na;
+          }
+        }
+        if (m < end) {
+          const int r0 =
+              sym_role<GABRIEL, false>(in, skey, xs, offb, s, me_id, false, me, m, xs[m], a.id[m]);
+          if (r0 != kRoleNone) {
+            if (na < kDeferMax) list[na * kBlock] = (int)((unsigned)m | ((unsigned)r0 << 30));
+            ++na;
+          }
+        }
+      }
+    }
+    // stayers have no role that scatters
+    const int nl = min(na, kDeferMax);
+    for (int e = 0; __ballot(e < nl) != 0ull; ++e) {
+      if (e < nl) {
+        const unsigned ent = (unsigned)list[e * kBlock];
+        const int role = (int)(ent >> 30);
+        sym_eval<HAS_TYPE, HAS_POL, false>(a, in, me, pme, (int)(ent & 0x3fffffffu), role, acc_pos,
+                                           acc_pol);
+        pairs += role != kRoleGather;
+      }
+    }
+    if (na > kDeferMax) {
+      int k = 0;
+      for (int r = 0; r < 9; ++r) {
+        const int end = re[r];
+        for (int m = rs[r]; m < end; ++m) {
+          const int role =
+              sym_role<GABRIEL, false>(in, skey, xs, offb, s, me_id, false, me, m, xs[m], a.id[m]);
+          if (role == kRoleNone || k++ < kDeferMax) continue;
+          sym_eval<HAS_TYPE, HAS_POL, false>(a, in, me, pme, m, role, acc_pos, acc_pol);
+          pairs += role != kRoleGather;
+        }
+      }
+    }
+  } else if (stayer) {
+    sym_cell_serial<GABRIEL, HAS_TYPE, HAS_POL, false>(a, in, s, me_id, false, me, pme, bc,
+                                                       acc_pos, acc_pol, pairs);
+  }
+  if (STALE && mover)
+    sym_cell_serial<GABRIEL, HAS_TYPE, HAS_POL, false>(a, in, s, me_id, true, me, pme, bc, acc_pos,
+                                                       acc_pol, pairs);
+  if (active) {
+    a.o_pos[s] = make_float4(acc_pos[0], acc_pos[1], acc_pos[2], 0.0f);
+    if (HAS_POL) a.o_pol[s] = make_float2(acc_pol[0], acc_pol[1]);
+  }
+  const unsigned tp = block_sum_u(active ? pairs : 0u, lds_u);
+  if (threadIdx.x == 0) a.block_pairs[lb] += tp;
+}
+
+// Naive symmetric finders, Delta mode (naive_all_pairs_symmetric.hpp:100-143,
+// naive_gabriel_symmetric.hpp:107-168): slot order is agent order, so cell s plays the own
+// role for m < s and the gather role for m > s; no grid, no movers.
+template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL>
+__global__ __launch_bounds__(kBlock) void k_sym_naive(SweepArgs a) {
+  __shared__ float4 tile[kBlock];
+  __shared__ unsigned lds_u[4];
+  const int lb = logical_block();
+  const InstanceDesc& in = a.inst[a.block_inst[lb]];
+  const int s = lb * kBlock + (int)threadIdx.x;
+  const bool active = (s - in.slot_begin) < in.n;
+  float4 me = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  Pol pme{};
+  if (active) {
+    me = a.xs[s];
+    if (HAS_POL) pme = make_pol(a.ps[s], a.trig[s]);
+  }
+  float acc_pos[3] = {0.0f, 0.0f, 0.0f}, acc_pol[2] = {0.0f, 0.0f};
+  unsigned pairs = 0;
+  const int begin = in.slot_begin, n = in.n;
+  for (int base = 0; base < n; base += kBlock) {
+    __syncthreads();
+    if (base + (int)threadIdx.x < n) tile[threadIdx.x] = a.xs[begin + base + threadIdx.x];
+    __syncthreads();
+    const int cnt = min(kBlock, n - base);
+    if (!active) continue;
+    for (int t = 0; t < cnt; ++t) {
+      const int m = begin + base + t;
+      if (m == s) continue;
+      const float4 pm = tile[t];
+      const float disp[3] = {me.x - pm.x, me.y - pm.y, me.z - pm.z};
+      const float d2 = len2(disp[0], disp[1], disp[2]);
+      if (d2 >= in.cutoff2) continue;
+      const bool own = m < s;
+      if (GABRIEL) {
+        // the ball of the pair as its larger-index cell sees it: rho^2 = (d2 * 0.25) * c^2,
+        // strict test over every other cell (naive_gabriel_symmetric.hpp:123-139)
+        const float4 pi = own ? me : pm;
+        const float sgn = own ? 1.0f : -1.0f;
+        const float mid0 = pi.x - (sgn * disp[0]) * 0.5f;
+        const float mid1 = pi.y - (sgn * disp[1]) * 0.5f;
+        const float mid2 = pi.z - (sgn * disp[2]) * 0.5f;
+        const float rho2 = d2 * 0.25f * in.gabriel_c2;
+        bool blocked = false;
+        for (int k = begin; k < begin + n; ++k) {
+          if (k == s || k == m) continue;
+          const float4 pk = a.xs[k];
+          if (len2(pk.x - mid0, pk.y - mid1, pk.z - mid2) < rho2) {
+            blocked = true;
+            break;
+          }
+        }
+        if (blocked) continue;
+      }
+      sym_eval<HAS_TYPE, HAS_POL, false>(a, in, me, pme, m, own ? kRoleOwn : kRoleGather, acc_pos,
+                                         acc_pol);
+      pairs += own;
+    }
+  }
+  if (active) {
+    a.o_pos[s] = make_float4(acc_pos[0], acc_pos[1], acc_pos[2], 0.0f);
+    if (HAS_POL) a.o_pol[s] = make_float2(acc_pol[0], acc_pol[1]);
+  }
+  const unsigned tp = block_sum_u(active ? pairs : 0u, lds_u);
+  if (threadIdx.x == 0) a.block_pairs[lb] += tp;
+}
+
 template <bool GABRIEL, bool BINNED, bool HAS_TYPE, bool HAS_POL, int MODE, bool SYM>
 void launch_one(const SweepArgs& a, int n_blocks, hipStream_t stream) {
   if (BINNED)
@@ -521,24 +636,56 @@ void launch_mode(int mode, bool has_type, bool has_pol, const SweepArgs& a, int 
   }
 }
 
-// Symmetric finders: Delta (atomic accumulation into a zeroed buffer), Count and Emit.
+template <bool GABRIEL, bool BINNED, bool HAS_TYPE, bool HAS_POL>
+void launch_sym(const SweepArgs& a, int n_blocks, hipStream_t stream) {
+  if (BINNED && a.skey)
+    hipLaunchKernelGGL((k_sym_binned<GABRIEL, HAS_TYPE, HAS_POL, true>), dim3(n_blocks),
+                       dim3(kBlock), 0, stream, a);
+  else if (BINNED)
+    hipLaunchKernelGGL((k_sym_binned<GABRIEL, HAS_TYPE, HAS_POL, false>), dim3(n_blocks),
+                       dim3(kBlock), 0, stream, a);
+  else
+    hipLaunchKernelGGL((k_sym_naive<GABRIEL, HAS_TYPE, HAS_POL>), dim3(n_blocks), dim3(kBlock), 0,
+                       stream, a);
+}
+
+// Symmetric finders: Delta (the k_sym_* kernels), Count and Emit (the ordered walks restricted
+// to partners j < i).
 template <bool GABRIEL, bool BINNED>
 int launch_mode_sym(int mode, bool has_type, bool has_pol, const SweepArgs& a, int n_blocks,
                     hipStream_t stream) {
   switch (mode) {
-    case kModeDelta: launch_force<GABRIEL, BINNED, kModeDelta, true>(has_type, has_pol, a, n_blocks, stream); return 0;
+    case kModeDelta:
+      if (has_type) {
+        if (has_pol) launch_sym<GABRIEL, BINNED, true, true>(a, n_blocks, stream);
+        else launch_sym<GABRIEL, BINNED, true, false>(a, n_blocks, stream);
+      } else {
+        if (has_pol) launch_sym<GABRIEL, BINNED, false, true>(a, n_blocks, stream);
+        else launch_sym<GABRIEL, BINNED, false, false>(a, n_blocks, stream);
+      }
+      return 0;
     case kModeCount: launch_force<GABRIEL, BINNED, kModeCount, true>(has_type, has_pol, a, n_blocks, stream); return 0;
     case kModeEmit: launch_force<GABRIEL, BINNED, kModeEmit, true>(has_type, has_pol, a, n_blocks, stream); return 0;
     default: return -1;
   }
 }
 
-// The delta buffers the symmetric sweeps accumulate into start at zero.
-__global__ __launch_bounds__(kBlock) void k_zero_deltas(float4* pos, float2* pol, int total) {
+// d += q and q = 0 for every slot (the movers' contributions of a symmetric evaluation that no
+// integrator update consumes).
+__global__ __launch_bounds__(kBlock) void k_fold_q(float4* d, float2* dp, float4* q, float2* qp,
+                                                   int total) {
   const int s = blockIdx.x * kBlock + (int)threadIdx.x;
   if (s >= total) return;
-  pos[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  if (pol) pol[s] = make_float2(0.0f, 0.0f);
+  const float4 v = q[s];
+  const float4 x = d[s];
+  d[s] = make_float4(x.x + v.x, x.y + v.y, x.z + v.z, x.w);
+  q[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (dp) {
+    const float2 w = qp[s];
+    const float2 y = dp[s];
+    dp[s] = make_float2(y.x + w.x, y.y + w.y);
+    qp[s] = make_float2(0.0f, 0.0f);
+  }
 }
 
 // ------------------------------------------------------------------ COM fix and updates
@@ -546,13 +693,17 @@ __global__ __launch_bounds__(kBlock) void k_zero_deltas(float4* pos, float2* pol
 // their epilogue: other cells still add to a cell's delta until the sweep ends).
 __global__ __launch_bounds__(kBlock) void k_com_partials(const InstanceDesc* inst,
                                                          const int* block_inst, const float4* d,
-                                                         float4* partial) {
+                                                         const float4* q, float4* partial) {
   __shared__ float lds[4];
   const int lb = logical_block();
   const InstanceDesc& in = inst[block_inst[lb]];
   const int s = lb * kBlock + (int)threadIdx.x;
   const bool active = (s - in.slot_begin) < in.n;
-  const float4 v = active ? d[s] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float4 v = active ? d[s] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (active && q) {
+    const float4 w = q[s];
+    v = make_float4(v.x + w.x, v.y + w.y, v.z + w.z, 0.0f);
+  }
   const float t0 = block_sum_f(v.x, lds);
   const float t1 = block_sum_f(v.y, lds);
   const float t2 = block_sum_f(v.z, lds);
@@ -593,11 +744,28 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
   if ((s - in.slot_begin) >= in.n) return;
   const float4 com = a.com[ii];
   const float dt = in.dt;
+  // a symmetric evaluation on a stale grid: the movers' contributions in q join the evaluated
+  // deltas (d1, or d3 for the corrector) and q is cleared for the next one
+  float4 qv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float2 ql = make_float2(0.0f, 0.0f);
+  if (a.q_pos) {
+    qv = a.q_pos[s];
+    if (qv.x != 0.0f || qv.y != 0.0f || qv.z != 0.0f) a.q_pos[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (HAS_POL) {
+      ql = a.q_pol[s];
+      if (ql.x != 0.0f || ql.y != 0.0f) a.q_pol[s] = make_float2(0.0f, 0.0f);
+    }
+  }
   float4 d1 = a.d1_pos[s];
+  if (KIND != kUpdateCorr && a.q_pos) d1 = make_float4(d1.x + qv.x, d1.y + qv.y, d1.z + qv.z, 0.0f);
   d1 = make_float4(d1.x - com.x, d1.y - com.y, d1.z - com.z, 0.0f);
   if (KIND == kUpdatePred) {
     // heun.hpp:150-154
     a.d1_pos[s] = d1;
+    if (HAS_POL && a.q_pos) {
+      const float2 e = a.d1_pol[s];
+      a.d1_pol[s] = make_float2(e.x + ql.x, e.y + ql.y);
+    }
     const float4 x = a.x[s];
     a.xp[s] = make_float4(x.x + d1.x * dt, x.y + d1.y * dt, x.z + d1.z * dt, x.w);
     if (HAS_POL) {
@@ -610,24 +778,19 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
     // heun.hpp:174-183 (d1 was COM-fixed by the predictor; com here is the corrector's)
     const float4 d1f = a.d1_pos[s];
     float4 d3 = a.d3_pos[s];
+    if (a.q_pos) d3 = make_float4(d3.x + qv.x, d3.y + qv.y, d3.z + qv.z, 0.0f);
     d3 = make_float4(d3.x - com.x, d3.y - com.y, d3.z - com.z, 0.0f);
     const float s0 = d1f.x + d3.x, s1 = d1f.y + d3.y, s2 = d1f.z + d3.z;
     if (a.old_vel) a.old_vel[s] = make_float4(s0 * 0.5f, s1 * 0.5f, s2 * 0.5f, 0.0f);
     const float4 x = a.x[s];
     a.x[s] = make_float4(x.x + s0 * 0.5f * dt, x.y + s1 * 0.5f * dt, x.z + s2 * 0.5f * dt, x.w);
     if (HAS_POL) {
-      const float2 p = a.pol[s], e1 = a.d1_pol[s], e3 = a.d3_pol[s];
+      float2 e3 = a.d3_pol[s];
+      e3 = make_float2(e3.x + ql.x, e3.y + ql.y);  // ql = 0 without q
+      const float2 p = a.pol[s], e1 = a.d1_pol[s];
       const float2 np = make_float2(p.x + (e1.x + e3.x) * 0.5f * dt, p.y + (e1.y + e3.y) * 0.5f * dt);
       a.pol[s] = np;
       a.trig[s] = polar_trig(np);
-    }
-    if (a.clear) {  // d1 = d3 = 0 (heun.hpp:180-181) for the next accumulation
-      a.d1_pos[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      a.d3_pos[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      if (HAS_POL) {
-        a.d1_pol[s] = make_float2(0.0f, 0.0f);
-        a.d3_pol[s] = make_float2(0.0f, 0.0f);
-      }
     }
   } else {
     // euler.hpp:56-66
@@ -635,14 +798,12 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
     const float4 x = a.x[s];
     a.x[s] = make_float4(x.x + d1.x * dt, x.y + d1.y * dt, x.z + d1.z * dt, x.w);
     if (HAS_POL) {
-      const float2 p = a.pol[s], e = a.d1_pol[s];
+      float2 e = a.d1_pol[s];
+      e = make_float2(e.x + ql.x, e.y + ql.y);
+      const float2 p = a.pol[s];
       const float2 np = make_float2(p.x + e.x * dt, p.y + e.y * dt);
       a.pol[s] = np;
       a.trig[s] = polar_trig(np);
-    }
-    if (a.clear) {  // d = 0 (euler.hpp:63-64) for the next accumulation
-      a.d1_pos[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      if (HAS_POL) a.d1_pol[s] = make_float2(0.0f, 0.0f);
     }
   }
 }
@@ -664,15 +825,16 @@ int launch_sweep(int finder, bool has_type, bool has_pol, int mode, const SweepA
   }
 }
 
-void launch_zero_deltas(float4* pos, float2* pol, int total_slots, hipStream_t stream) {
-  hipLaunchKernelGGL(k_zero_deltas, dim3((total_slots + kBlock - 1) / kBlock), dim3(kBlock), 0,
-                     stream, pos, pol, total_slots);
+void launch_fold_q(float4* d, float2* dp, float4* q, float2* qp, int total_slots,
+                   hipStream_t stream) {
+  hipLaunchKernelGGL(k_fold_q, dim3((total_slots + kBlock - 1) / kBlock), dim3(kBlock), 0, stream,
+                     d, dp, q, qp, total_slots);
 }
 
 void launch_com_partials(const InstanceDesc* inst, const int* block_inst, const float4* d,
-                         float4* partial, int n_blocks, hipStream_t stream) {
+                         const float4* q, float4* partial, int n_blocks, hipStream_t stream) {
   hipLaunchKernelGGL(k_com_partials, dim3(n_blocks), dim3(kBlock), 0, stream, inst, block_inst, d,
-                     partial);
+                     q, partial);
 }
 
 void launch_com_finalize(const InstanceDesc* inst, int n_inst, const float4* partial,

@@ -22,10 +22,21 @@
 // T trades lane utilisation against per-team overhead: the force phase runs one round per
 // T accepted pairs, the row and candidate phases one per T rows / candidates. The host's
 // autotuner times T = 4, 8, 16 (and the thread-per-cell sweep) on the live state.
+//
+// Symmetric finders (SYM, Delta mode; the roles and the scatter rule used here are described
+// in walks.hpp): an accepted
+// candidate is an own pair (smaller agent index; scattered when a mover is involved), a gather
+// pair (larger index, both cells stayers) or only a potential blocker (a pair with a mover
+// that the other cell delivers). The ball of a gather pair (m, i) is the one m's walk tests;
+// it lies within (c/2 + 1/2) d(m, i) of i as well, so its blockers are accepted candidates of
+// i too.
 #include "tecs_internal.hpp"
+#include "walks.hpp"
 
 namespace tecs_amd {
 namespace {
+
+using namespace walks;
 
 // Accepted neighbours held in LDS per cell: a Gabriel cell on a sheet or in a tissue has ~5-8
 // neighbours within the cutoff, an all-pairs cell in a dense tissue ~6-10; a cell with more
@@ -161,8 +172,12 @@ __device__ __forceinline__ void team_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-template <int T, bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE>
+// abit bits 28-29: the entry's role in a symmetric sweep (SymRole; 0 = blocker only)
+constexpr int kRoleShift = 28;
+
+template <int T, bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE, bool SYM>
 __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
+  static_assert(!SYM || MODE == kModeDelta, "symmetric team sweeps evaluate forces only");
   constexpr int TB = TeamLds<T, GABRIEL, HAS_POL>::TB;
   constexpr int AM = TeamLds<T, GABRIEL, HAS_POL>::AM;
   __shared__ TeamLds<T, GABRIEL, HAS_POL> L;
@@ -186,16 +201,22 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
   float2 mp = make_float2(0.0f, 0.0f);
   Pol pme{};
   int bc[3] = {0, 0, 0};
+  int me_id = 0;
+  bool mover = false;  // symmetric: this cell left its rebuild bin
   if (active) {
     me = a.xs[s];
     if (HAS_POL) {
       mp = a.ps[s];
       pme = make_pol(mp, a.trig[s]);
     }
-    if (GABRIEL)
-      bin_coords_div(in, me.x, me.y, me.z, bc);  // binned_gabriel.hpp:101
+    if (GABRIEL || SYM)
+      bin_coords_div(in, me.x, me.y, me.z, bc);  // binned_gabriel.hpp:101, *_symmetric.hpp
     else
       bin_coords_mul(in, me.x, me.y, me.z, bc);  // binned_all_pairs.hpp:103-109
+    if (SYM) {
+      me_id = a.id[s];
+      mover = a.skey != nullptr && flatten(in, bc[0], bc[1], bc[2]) != a.skey[s];
+    }
     // row rr = (dz + 1) * 3 + (dy + 1), looked up by lane rr mod T
     const int x_lo = max(bc[0] - 1, 0), x_hi = min(bc[0] + 1, in.ext[0] - 1);
     const int* offb = a.off + in.bin_begin;
@@ -269,6 +290,17 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
         acc = !(d2 >= in.cutoff2);
       }
     }
+    int role = kRoleOwn;
+    if (SYM && acc) {
+      // the scatter rule: a pair with a mover goes through its larger-index cell
+      const bool other_mover = is_mover(in, a.skey, m, pj);
+      if (a.id[m] < me_id)
+        role = mover || other_mover ? kRoleOwnScatter : kRoleOwn;
+      else
+        role = mover || other_mover ? kRoleNone : kRoleGather;
+      // all-pairs has no blockers: a candidate without a role is dropped
+      if (!GABRIEL && role == kRoleNone) acc = false;
+    }
     const unsigned long long bal = __ballot(acc);
     const unsigned tm = (unsigned)(bal >> team_shift) & team_bits;
     const int pos = base + __popc(tm & ((1u << lane) - 1u));
@@ -281,7 +313,13 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
                        (m >= L.row_b2[team][rr_sel] ? 1 : 0);
         L.ent[team][pos] = pj;
         // row rr_sel = 3 rz + ry, so 3 ry + 9 rz = 3 rr_sel
-        L.abit[team][pos] = 1 << ((bx - bc[0] + 1) + 3 * rr_sel);
+        L.abit[team][pos] = (1 << ((bx - bc[0] + 1) + 3 * rr_sel)) | (SYM ? role << kRoleShift : 0);
+      } else if (SYM) {
+        // all-pairs, symmetric: own v(i, m) or gather -v(m, i), computed here
+        float cp[3] = {0.0f, 0.0f, 0.0f}, cl[2] = {0.0f, 0.0f};
+        sym_eval<HAS_TYPE, HAS_POL>(a, in, me, pme, m, role, cp, cl);
+        L.ent[team][pos] = make_float4(cp[0], cp[1], cp[2], role == kRoleGather ? 2.0f : 1.0f);
+        if (HAS_POL) L.cpol[team][pos] = make_float2(cl[0], cl[1]);
       } else {
         // all-pairs: the force needs nothing else, compute it here
         float cp[3] = {0.0f, 0.0f, 0.0f}, cl[2] = {0.0f, 0.0f};
@@ -308,11 +346,20 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
     int r = 0;
     for (int e = lane; e < na; e += T, ++r) {
       const float4 pj = L.ent[team][e];
-      const float disp[3] = {me.x - pj.x, me.y - pj.y, me.z - pj.z};
+      const int erole = SYM ? (L.abit[team][e] >> kRoleShift) : kRoleOwn;
+      if (erole == kRoleNone) {  // a blocker only: no pair to test
+        blocked_bits |= 1u << r;
+        continue;
+      }
+      // the ball as the pair's larger-index cell tests it: i's for an own pair, m's for a
+      // gather pair (mid = p_m - (p_m - p_i) * 0.5; d2 is the same either way)
+      const float4 ps = erole != kRoleGather ? me : pj;
+      const float4 po = erole != kRoleGather ? pj : me;
+      const float disp[3] = {ps.x - po.x, ps.y - po.y, ps.z - po.z};
       const float d2 = len2(disp[0], disp[1], disp[2]);
-      const float mid0 = me.x - disp[0] * 0.5f;
-      const float mid1 = me.y - disp[1] * 0.5f;
-      const float mid2 = me.z - disp[2] * 0.5f;
+      const float mid0 = ps.x - disp[0] * 0.5f;
+      const float mid1 = ps.y - disp[1] * 0.5f;
+      const float mid2 = ps.z - disp[2] * 0.5f;
       const float rho2 = d2 * in.gabriel_factor;
       const float rho = sqrtf(rho2);
       // bounding-box bins (binned_gabriel.hpp:142-153), relative to bc - 1
@@ -340,7 +387,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
         if ((dmask[2] >> z) & 1) box |= m9 << (9 * z);
       bool blocked = false;
       for (int k = 0; k < na; ++k) {
-        const int bits = L.abit[team][k];
+        const int bits = SYM ? (L.abit[team][k] & ((1 << 27) - 1)) : L.abit[team][k];
         const float4 pk = L.ent[team][k];
         const bool in_box = k != e && (bits & box) != 0;
         const bool in_ball = len2(pk.x - mid0, pk.y - mid1, pk.z - mid2) < rho2;
@@ -358,7 +405,10 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
       const float4 pj = L.ent[team][e];
       const bool blocked = (blocked_bits >> r) & 1u;
       float cp[3] = {0.0f, 0.0f, 0.0f}, cl[2] = {0.0f, 0.0f};
-      if (!blocked && MODE != kModeCount && MODE != kModeEmit) {
+      const int erole = SYM ? (L.abit[team][e] >> kRoleShift) : kRoleOwn;
+      if (SYM && !blocked) {
+        sym_eval<HAS_TYPE, HAS_POL>(a, in, me, pme, L.aslot[team][e], erole, cp, cl);
+      } else if (!blocked && MODE != kModeCount && MODE != kModeEmit) {
         const int m = L.aslot[team][e];
         const float disp[3] = {me.x - pj.x, me.y - pj.y, me.z - pj.z};
         const float d = sqrtf(len2(disp[0], disp[1], disp[2]));
@@ -367,7 +417,9 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
         apply_force<HAS_TYPE, HAS_POL>(in.force, disp, d, __float_as_int(me.w),
                                        __float_as_int(pj.w), pme, po, cp, cl);
       }
-      L.ent[team][e] = make_float4(cp[0], cp[1], cp[2], blocked ? 0.0f : 1.0f);
+      // w: 0 no pair, 1 a pair the reference counts (ordered, or own), 2 a gathered pair
+      L.ent[team][e] = make_float4(cp[0], cp[1], cp[2],
+                                   blocked ? 0.0f : (erole == kRoleGather ? 2.0f : 1.0f));
       if (HAS_POL) L.cpol[team][e] = make_float2(cl[0], cl[1]);
     }
   }
@@ -378,7 +430,10 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
   unsigned pairs = 0;
   const bool leader = lane == 0 && active;
   if (leader) {
-    if (overflow || L.unsafe[team]) {
+    if (SYM && (overflow || L.unsafe[team])) {
+      sym_cell_serial<GABRIEL, HAS_TYPE, HAS_POL, true>(a, in, s, me_id, mover, me, pme, bc,
+                                                        acc_pos, acc_pol, pairs);
+    } else if (overflow || L.unsafe[team]) {
       serial_cell<GABRIEL, HAS_TYPE, HAS_POL, MODE>(a, in, s, me, pme, acc_pos, acc_pol, pairs);
     } else {
       int64_t emit_at = 0;
@@ -386,7 +441,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
       for (int e = 0; e < na; ++e) {
         const float4 c = L.ent[team][e];
         if (c.w == 0.0f) continue;
-        pairs++;
+        pairs += c.w == 1.0f;
         if (MODE == kModeEmit) {
           a.cols[emit_at++] = a.id[L.aslot[team][e]];
           continue;
@@ -483,22 +538,22 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
   if ((threadIdx.x & 63) == 0 && tp != 0u) atomicAdd(&a.block_pairs[tb], (unsigned long long)tp);
 }
 
-template <int T, bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE>
+template <int T, bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE, bool SYM = false>
 void launch_t(const SweepArgs& a, int n_blocks, hipStream_t stream) {
-  hipLaunchKernelGGL((k_sweep_team<T, GABRIEL, HAS_TYPE, HAS_POL, MODE>), dim3(n_blocks * T),
+  hipLaunchKernelGGL((k_sweep_team<T, GABRIEL, HAS_TYPE, HAS_POL, MODE, SYM>), dim3(n_blocks * T),
                      dim3(kBlock), 0, stream, a);
 }
 
-template <int T, bool GABRIEL, int MODE>
+template <int T, bool GABRIEL, int MODE, bool SYM = false>
 void launch_force_t(bool has_type, bool has_pol, const SweepArgs& a, int nb, hipStream_t st) {
   if (MODE == kModeCount || MODE == kModeEmit) {
-    launch_t<T, GABRIEL, false, false, MODE>(a, nb, st);
+    launch_t<T, GABRIEL, false, false, MODE, SYM>(a, nb, st);
   } else if (has_type) {
-    if (has_pol) launch_t<T, GABRIEL, true, true, MODE>(a, nb, st);
-    else launch_t<T, GABRIEL, true, false, MODE>(a, nb, st);
+    if (has_pol) launch_t<T, GABRIEL, true, true, MODE, SYM>(a, nb, st);
+    else launch_t<T, GABRIEL, true, false, MODE, SYM>(a, nb, st);
   } else {
-    if (has_pol) launch_t<T, GABRIEL, false, true, MODE>(a, nb, st);
-    else launch_t<T, GABRIEL, false, false, MODE>(a, nb, st);
+    if (has_pol) launch_t<T, GABRIEL, false, true, MODE, SYM>(a, nb, st);
+    else launch_t<T, GABRIEL, false, false, MODE, SYM>(a, nb, st);
   }
 }
 
@@ -516,23 +571,31 @@ void launch_mode_t(int mode, bool has_type, bool has_pol, const SweepArgs& a, in
 }
 
 template <int T>
-void launch_team_width(bool gabriel, bool has_type, bool has_pol, int mode, const SweepArgs& a,
-                       int n_blocks, hipStream_t stream) {
-  if (gabriel)
+void launch_team_width(bool gabriel, bool sym, bool has_type, bool has_pol, int mode,
+                       const SweepArgs& a, int n_blocks, hipStream_t stream) {
+  if (sym) {  // Delta only (checked by the caller)
+    if (gabriel)
+      launch_force_t<T, true, kModeDelta, true>(has_type, has_pol, a, n_blocks, stream);
+    else
+      launch_force_t<T, false, kModeDelta, true>(has_type, has_pol, a, n_blocks, stream);
+  } else if (gabriel) {
     launch_mode_t<T, true>(mode, has_type, has_pol, a, n_blocks, stream);
-  else
+  } else {
     launch_mode_t<T, false>(mode, has_type, has_pol, a, n_blocks, stream);
+  }
 }
 
 }  // namespace
 
-void launch_sweep_team(int team_lanes, bool gabriel, bool has_type, bool has_pol, int mode,
-                       const SweepArgs& a, int n_blocks, hipStream_t stream) {
+int launch_sweep_team(int team_lanes, bool gabriel, bool sym, bool has_type, bool has_pol,
+                      int mode, const SweepArgs& a, int n_blocks, hipStream_t stream) {
+  if (sym && mode != kModeDelta) return -1;
   switch (team_lanes) {
-    case 4: launch_team_width<4>(gabriel, has_type, has_pol, mode, a, n_blocks, stream); break;
-    case 8: launch_team_width<8>(gabriel, has_type, has_pol, mode, a, n_blocks, stream); break;
-    default: launch_team_width<16>(gabriel, has_type, has_pol, mode, a, n_blocks, stream); break;
+    case 4: launch_team_width<4>(gabriel, sym, has_type, has_pol, mode, a, n_blocks, stream); break;
+    case 8: launch_team_width<8>(gabriel, sym, has_type, has_pol, mode, a, n_blocks, stream); break;
+    default: launch_team_width<16>(gabriel, sym, has_type, has_pol, mode, a, n_blocks, stream); break;
   }
+  return 0;
 }
 
 }  // namespace tecs_amd

@@ -54,6 +54,10 @@ struct SweepArgs {
   const float4* d1_pos;  // Corr: first evaluation
   const float2* d1_pol;
   float4* o_trig;        // Pred / Corr / Euler: polar_trig of the polarity written
+  const int* skey;       // symmetric Delta: bin key of every slot at the last rebuild (null: the
+                         // grid was built from these positions)
+  float4* q_pos;         // symmetric Delta on a stale grid: the movers' scattered contributions
+  float2* q_pol;
   KeyCount kc;           // Corr / Euler: count_key of the new positions for the next rebuild
                          // (kc.keys null = not this step)
   float4* old_vel;       // Corr / Euler, may be null
@@ -65,13 +69,16 @@ struct SweepArgs {
 };
 
 // Returns nonzero for a mode the finder does not have (symmetric finders: Delta, Count, Emit;
-// Delta accumulates atomically into o_pos / o_pol, which must hold zeros).
+// their Delta writes o_pos / o_pol and, on a stale grid (skey set), adds the contributions of
+// cells outside their rebuild bin to q_pos / q_pol, which must hold zeros).
 int launch_sweep(int finder, bool has_type, bool has_pol, int mode, const SweepArgs& a,
                  int n_blocks, hipStream_t stream);
-void launch_zero_deltas(float4* pos, float2* pol, int total_slots, hipStream_t stream);
-// com partial of every 256-slot workgroup from the finished deltas d (one per workgroup)
+// d += q, q = 0 over all slots (dp / qp may be null)
+void launch_fold_q(float4* d, float2* dp, float4* q, float2* qp, int total_slots,
+                   hipStream_t stream);
+// com partial of every 256-slot workgroup from the finished deltas d (+ q when non-null)
 void launch_com_partials(const InstanceDesc* inst, const int* block_inst, const float4* d,
-                         float4* partial, int n_blocks, hipStream_t stream);
+                         const float4* q, float4* partial, int n_blocks, hipStream_t stream);
 
 struct UpdateArgs {
   const InstanceDesc* inst;
@@ -81,14 +88,15 @@ struct UpdateArgs {
   float2* pol;
   float4* d1_pos;     // first (or only) evaluation, COM-fixed in place
   float2* d1_pol;
-  float4* d3_pos;        // second evaluation (Heun corrector; cleared with `clear`)
+  float4* d3_pos;        // second evaluation (Heun corrector)
   float2* d3_pol;
   float4* xp;            // predicted positions (Heun predictor)
   float2* polp;
   float4* trig;          // polar_trig of pol (corrector, Euler) ...
   float4* trigp;         // ... and of polp (predictor)
   float4* old_vel;
-  bool clear;            // corrector / Euler: zero the consumed deltas (symmetric sweeps add to them)
+  float4* q_pos;         // symmetric evaluation on a stale grid: movers' contributions to add to
+  float2* q_pol;         // the evaluated deltas (then cleared); null otherwise
 };
 enum UpdateKind { kUpdatePred = 0, kUpdateCorr = 1, kUpdateEuler = 2 };
 void launch_update(int kind, bool has_pol, const UpdateArgs& a, int n_blocks, hipStream_t stream);
@@ -103,8 +111,9 @@ void launch_com_finalize(const InstanceDesc* inst, int n_inst, const float4* par
 constexpr int kMaxTeamLanes = 16;
 // Quads of 4 bins per thread of the per-bin offset fill.
 constexpr int kFillQuads = 4;
-void launch_sweep_team(int team_lanes, bool gabriel, bool has_type, bool has_pol, int mode,
-                       const SweepArgs& a, int n_blocks, hipStream_t stream);
+// sym: a symmetric finder (Delta mode only; returns nonzero for another mode)
+int launch_sweep_team(int team_lanes, bool gabriel, bool sym, bool has_type, bool has_pol,
+                      int mode, const SweepArgs& a, int n_blocks, hipStream_t stream);
 
 struct RebuildArgs {
   const InstanceDesc* inst;

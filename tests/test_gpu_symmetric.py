@@ -104,6 +104,39 @@ def test_forces_within_1e5_of_fp64(gpu, oracle_lib, finder, name, n):
     assert g.stats()["pair_interactions"] == len(cols)
 
 
+VARIANTS = [abi.SWEEP_THREAD, abi.SWEEP_TEAM4, abi.SWEEP_TEAM8, abi.SWEEP_TEAM16]
+VARIANT_IDS = ["thread", "team4", "team8", "team16"]
+
+
+@pytest.mark.parametrize("variant", VARIANTS, ids=VARIANT_IDS)
+@pytest.mark.parametrize("finder", BINNED_SYM, ids=NAME)
+@pytest.mark.parametrize("name,n", [("epithelium", 4000), ("sorting", 20000)])
+@pytest.mark.parametrize("stale", [False, True], ids=["fresh", "stale"])
+def test_forces_every_variant(gpu, oracle_lib, finder, name, n, variant, stale):
+    """Every sweep variant, on the grid built from the positions and on a stale grid whose
+    cells moved by up to 0.3 (many change bin: the movers' scatter path)."""
+    w = _workload(name, n, finder)
+    print(f"case sym variant {name} n={w.n} finder={NAME(finder)} variant={variant} stale={stale}",
+          flush=True)
+    g = GpuSimulation(w.desc)
+    g.set_sweep_variant(variant)
+    o = oracle_lib.OracleSim(w.desc)
+    for s in (g, o):
+        s.upload(w.positions, w.polarities, w.types)
+    if stale:
+        moved = (w.positions + W.rng(78).uniform(-0.3, 0.3, w.positions.shape)).astype(np.float32)
+        for s in (g, o):
+            s.rebuild()
+            s.upload(moved, w.polarities, w.types)
+    fg, pg = g.evaluate_force(not stale)
+    rp, cols = o.pairs(not stale)
+    f64, p64, s64, sp64 = o.forces_f64(rp, cols)
+    assert oracle_lib.force_excess(fg, f64, s64) <= 0.0
+    if pg is not None:
+        assert oracle_lib.force_excess(pg, p64, sp64) <= 0.0
+    assert g.stats()["pair_interactions"] == len(cols)
+
+
 def test_half_the_functor_calls(gpu):
     """Fresh grid, all-pairs: the symmetric finder calls the functor once per unordered pair,
     the ordered one twice."""
