@@ -42,6 +42,25 @@ def measured_traffic(workload: str, variant: str, phase: str):
     return None
 
 
+# VALU issue peak: 256 CUs x 4 SIMDs, a wavefront64 FP32 instruction (unpacked) occupies its
+# SIMD for 4 cycles, 2.4 GHz engine clock
+VALU_PEAK_WAVE_INSTR_PER_S = 256 * 4 * 2.4e9 / 4
+
+
+def measured_valu(workload: str, variant: str, phase: str):
+    """VALU wavefront-instructions per launch of a sweep as rocprofv3 counted them
+    (profiles/valu.txt), or None."""
+    path = os.path.join(ROOT, "profiles", "valu.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split("#", 1)[0].split()
+            if len(parts) == 4 and parts[:3] == [workload, variant, phase]:
+                return int(parts[3])
+    return None
+
+
 def algorithmic_bytes_per_cell(has_pol: bool, has_type: bool):
     """Compulsory HBM bytes per cell of the two Heun sweeps: s = state bytes (position 12,
     + polarity 8), a = attribute bytes (type 1).
@@ -219,6 +238,7 @@ def bench_single(args, w):
         kernel_names = {"full": f"{kname} {fname} mode Pred (Heun substep 1)",
                         "stale": f"{kname} {fname} mode Corr (Heun substep 2, stale grid)"}
     traffic = None if symmetric else measured_traffic(w.name, variant, dom)
+    valu = None if symmetric else measured_valu(w.name, variant, dom)
     print(
         f"bench: sweep full {per['full'][0]*1e3:.1f} us, stale {per['stale'][0]*1e3:.1f} us, "
         f"rebuild {reb_ms/max(reb_n,1)*1e3:.1f} us per launch",
@@ -266,6 +286,14 @@ def bench_single(args, w):
             "rebuild_us": reb_ms / max(reb_n, 1) * 1e3,
             "sweep_variant": variant,
             "autotune_ms": sim.autotune_ms(),
+            # the bound the sweeps actually meet: VALU issue (counted instructions at the issue
+            # peak, against the launch time measured here)
+            "valu": None if valu is None else {
+                "instructions_per_launch": valu,
+                "peak_instructions_per_s": VALU_PEAK_WAVE_INSTR_PER_S,
+                "issue_us": valu / VALU_PEAK_WAVE_INSTR_PER_S * 1e6,
+                "frac": (valu / VALU_PEAK_WAVE_INSTR_PER_S * 1e6) / (avg_ms * 1e3),
+            },
         },
     }
     if not args.no_cpu_baseline:
