@@ -66,6 +66,7 @@ struct TeamLds {
   float2 cpol[TP][AM + 1];  // polarity contribution
   int row_start[TB][9];
   int row_len[TB][9];
+  int row_base[TB][9];  // row_start - the row's first candidate index (slot = base + c)
   int row_b1[TG][9];    // first slot of the row's second bin (Gabriel)
   int row_b2[TG][9];    // first slot of the row's third bin (Gabriel)
   int row_xlo[TG][9];
@@ -254,6 +255,12 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
     total += L.row_len[team][rr];
   }
   if (!active) total = 0;
+  if (lane == 0) {
+#pragma unroll
+    for (int rr = 0; rr < 9; ++rr) L.row_base[team][rr] = row_start[rr] - row_pre[rr];
+  }
+  team_sync();
+  const float cut2 = in.cutoff2;
 
   // candidates -> accepted list in LDS, in visiting order
   int base = 0;
@@ -268,26 +275,19 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
     float disp[3] = {0.0f, 0.0f, 0.0f};
     float d2 = 0.0f;
     if (c < total) {
-      // the row holding candidate c: the last row whose prefix is <= c (empty rows share
-      // their prefix with the next row)
+      // the row holding candidate c: the last row whose prefix is <= c, i.e. the number of
+      // rows 1..8 whose prefix is <= c (prefixes never decrease; an empty row shares its
+      // prefix with the next one)
 #pragma unroll
-      for (int rr = 1; rr < 9; ++rr)
-        if (c >= row_pre[rr]) rr_sel = rr;
-      int st = row_start[0], pre = row_pre[0];
-#pragma unroll
-      for (int rr = 1; rr < 9; ++rr)
-        if (rr == rr_sel) {
-          st = row_start[rr];
-          pre = row_pre[rr];
-        }
-      m = st + (c - pre);
+      for (int rr = 1; rr < 9; ++rr) rr_sel += c >= row_pre[rr] ? 1 : 0;
+      m = L.row_base[team][rr_sel] + c;
       if (m != s) {
         pj = a.xs[m];
         disp[0] = me.x - pj.x;
         disp[1] = me.y - pj.y;
         disp[2] = me.z - pj.z;
         d2 = len2(disp[0], disp[1], disp[2]);
-        acc = !(d2 >= in.cutoff2);
+        acc = !(d2 >= cut2);
       }
     }
     int role = kRoleOwn;
@@ -377,14 +377,14 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
         dmask[d] = lo <= hi ? ((2 << hi) - (1 << lo)) : 0;
       }
       if (out_of_reach) L.unsafe[team] = 1;
-      // the box as a set of the 27 relative bins (bit rx + 3 ry + 9 rz)
-      int m9 = 0, box = 0;
-#pragma unroll
-      for (int y = 0; y < 3; ++y)
-        if ((dmask[1] >> y) & 1) m9 |= dmask[0] << (3 * y);
-#pragma unroll
-      for (int z = 0; z < 3; ++z)
-        if ((dmask[2] >> z) & 1) box |= m9 << (9 * z);
+      // the box as a set of the 27 relative bins (bit rx + 3 ry + 9 rz): the x mask repeated in
+      // the three y groups, ANDed with 3-bit runs at the y bits, repeated in the three z groups,
+      // ANDed with 9-bit runs at the z bits (all products fit 24-bit multiplies)
+      const unsigned dx = (unsigned)dmask[0], dy = (unsigned)dmask[1], dz = (unsigned)dmask[2];
+      const unsigned x9 = (dx & 7u) * 0x49u;                                          // bits x + 3 y
+      const unsigned y9 = ((dy & 1u) | ((dy & 2u) << 2) | ((dy & 4u) << 4)) * 7u;  // 3 ones per y
+      const unsigned z27 = ((dz & 1u) | ((dz & 2u) << 8) | ((dz & 4u) << 16)) * 0x1ffu;
+      const int box = (int)(((x9 & y9) * 0x40201u) & z27);
       bool blocked = false;
       for (int k = 0; k < na; ++k) {
         const int bits = SYM ? (L.abit[team][k] & ((1 << 27) - 1)) : L.abit[team][k];
