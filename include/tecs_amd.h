@@ -158,6 +158,25 @@ int tecs_sim_upload(tecs_sim* sim, const float* positions, const float* polariti
 /* Copy state out; any pointer may be NULL. old_velocities: agent_count*3 floats. */
 int tecs_sim_download(tecs_sim* sim, float* positions, float* polarities, float* old_velocities);
 
+/* ---- population (Simulation::set_capacity / set_agent_count, simulation.hpp:266-301) ------
+ * Cells are indexed 0..capacity-1; the first agent_count are active. As with the reference's
+ * views, a cell keeps its state while inactive: cells that become active again (or for the
+ * first time: zero state) start from what their index last held. */
+/* Write cells [first, first + count) (active or not) from host arrays in agent order; positions
+ * required, polarities / types may be NULL. first + count <= capacity. */
+int tecs_sim_upload_range(tecs_sim* sim, uint32_t first, uint32_t count, const float* positions,
+                          const float* polarities, const uint8_t* types);
+/* Simulation::set_agent_count (simulation.hpp:285-301): agent_count <= capacity. Resets the
+ * rebuild counter (pair_finder.set_agent_count, binned_all_pairs.hpp:60-63), so the next full
+ * evaluation builds the grid; a non-full evaluation before it fails with TECS_ERR_STATE. */
+int tecs_sim_set_agent_count(tecs_sim* sim, uint32_t agent_count);
+/* Simulation::set_capacity (simulation.hpp:266-274): capacity >= agent_count; GlobalComFixer
+ * divides by it. Cells beyond a smaller capacity are dropped (they start from zero state if the
+ * capacity grows again); growing past the allocation moves the state into new device buffers
+ * (synchronous) and rebuilds the grid at the next full step. */
+int tecs_sim_set_capacity(tecs_sim* sim, uint32_t capacity);
+int tecs_sim_get_population(tecs_sim* sim, uint32_t* agent_count, uint32_t* capacity);
+
 /* n_steps integrator steps of size dt (Integrator::integrate, euler.hpp:81-85 / heun.hpp:201-208).
  * Asynchronous on the simulation's stream. */
 int tecs_sim_take_step(tecs_sim* sim, double dt, int32_t n_steps);
@@ -250,6 +269,14 @@ int tecs_batch_download(tecs_batch* batch, int32_t index, float* positions, floa
 int tecs_batch_take_step(tecs_batch* batch, const double* dts, int32_t n_steps);
 /* tecs_sim_set_force for member `index`. */
 int tecs_batch_set_force(tecs_batch* batch, int32_t index, const tecs_force_desc* force);
+/* tecs_sim_upload_range / tecs_sim_set_agent_count for member `index`. A batch shares one
+ * rebuild schedule, so a binned batch of several members with rebuild_every_n > 0 refuses
+ * set_agent_count (TECS_ERR_UNSUPPORTED); the capacity of a member can change within the
+ * member's allocation only (the batch layout is fixed). */
+int tecs_batch_upload_range(tecs_batch* batch, int32_t index, uint32_t first, uint32_t count,
+                            const float* positions, const float* polarities, const uint8_t* types);
+int tecs_batch_set_agent_count(tecs_batch* batch, int32_t index, uint32_t agent_count);
+int tecs_batch_set_capacity(tecs_batch* batch, int32_t index, uint32_t capacity);
 int tecs_batch_synchronize(tecs_batch* batch);
 int tecs_batch_get_stats(tecs_batch* batch, tecs_stats* stats);
 int tecs_batch_reset_stats(tecs_batch* batch);

@@ -300,6 +300,31 @@ class Simulation {
                           polarities.empty() ? nullptr : polarities[0].data(),
                           types.empty() ? nullptr : types.data()));
   }
+  // Writes cells first .. first + positions.size() - 1, active or not (the reference writes into
+  // its views at those indices before set_agent_count makes them active).
+  void set_state_range(unsigned first, const std::vector<std::array<float, 3>>& positions,
+                       const std::vector<std::array<float, 2>>& polarities = {},
+                       const std::vector<uint8_t>& types = {}) {
+    const size_t k = positions.size();
+    if (!polarities.empty() && polarities.size() != k) throw Error(TECS_ERR_INVALID, "polarities size");
+    if (!types.empty() && types.size() != k) throw Error(TECS_ERR_INVALID, "types size");
+    check(tecs_sim_upload_range(h_, first, (uint32_t)k, k ? positions[0].data() : nullptr,
+                                polarities.empty() ? nullptr : polarities[0].data(),
+                                types.empty() ? nullptr : types.data()));
+  }
+  // Simulation::set_agent_count / set_capacity / get_* (simulation.hpp:257-301)
+  void set_agent_count(unsigned n) {
+    check(tecs_sim_set_agent_count(h_, n));
+    n_ = n;
+  }
+  void set_capacity(unsigned capacity) { check(tecs_sim_set_capacity(h_, capacity)); }
+  unsigned get_agent_count() const { return n_; }
+  unsigned get_capacity() const {
+    uint32_t n = 0, c = 0;
+    check(tecs_sim_get_population(h_, &n, &c));
+    return c;
+  }
+
   std::vector<std::array<float, 3>> positions() {
     std::vector<std::array<float, 3>> x(n_);
     check(tecs_sim_download(h_, x.empty() ? nullptr : x[0].data(), nullptr, nullptr));

@@ -63,6 +63,9 @@ def load():
         "orc_vector_op": (C.c_int, [C.c_int, P, P]),
         "orc_polarity_op": (C.c_int, [C.c_int, P, P]),
         "orc_com_fix": (C.c_int, [P, C.c_uint, P]),
+        "orc_upload_range": (C.c_int, [P, C.c_uint, C.c_uint, P, P, P]),
+        "orc_set_agent_count": (C.c_int, [P, C.c_uint]),
+        "orc_set_capacity": (C.c_int, [P, C.c_uint]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -122,6 +125,20 @@ class OracleSim:
 
     def take_step(self, dt: float, n_steps: int = 1):
         _check(self._lib.orc_take_step(self._h, float(dt), int(n_steps)))
+
+    def upload_range(self, first: int, positions, polarities=None, types=None):
+        p = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+        k = p.shape[0]
+        q = np.ascontiguousarray(polarities, np.float32).reshape(k, 2) if polarities is not None else None
+        t = np.ascontiguousarray(types, np.uint8).reshape(k) if types is not None else None
+        _check(self._lib.orc_upload_range(self._h, int(first), k, _ptr(p), _ptr(q), _ptr(t)))
+
+    def set_agent_count(self, n: int):
+        _check(self._lib.orc_set_agent_count(self._h, int(n)))
+        self.n = int(n)
+
+    def set_capacity(self, capacity: int):
+        _check(self._lib.orc_set_capacity(self._h, int(capacity)))
 
     def rebuild(self):
         _check(self._lib.orc_rebuild(self._h))

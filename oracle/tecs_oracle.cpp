@@ -824,6 +824,54 @@ int orc_upload(orc_sim* h, const float* pos, const float* pol, const uint8_t* ty
   return TECS_OK;
 }
 
+// Writes into the views at indices first .. first + count - 1 (active or not).
+int orc_upload_range(orc_sim* h, unsigned first, unsigned count, const float* pos, const float* pol,
+                     const uint8_t* type) {
+  Sim& s = *reinterpret_cast<Sim*>(h);
+  if ((uint64_t)first + count > s.cap) return fail(TECS_ERR_INVALID, "cells beyond the capacity");
+  if (count && !pos) return fail(TECS_ERR_INVALID, "positions required");
+  if (s.has_type && type && s.force.n_types > 0)
+    for (unsigned i = 0; i < count; ++i)
+      if (type[i] >= (unsigned)s.force.n_types) return fail(TECS_ERR_INVALID, "type out of range");
+  std::memcpy(&s.pos[3 * (size_t)first], pos, sizeof(float) * 3 * count);
+  if (s.has_pol && pol) std::memcpy(&s.pol[2 * (size_t)first], pol, sizeof(float) * 2 * count);
+  if (s.has_type && type) std::memcpy(&s.type[first], type, count);
+  return TECS_OK;
+}
+
+// Simulation::set_agent_count, simulation.hpp:285-301: the views keep their contents, the pair
+// finder's rebuild counter restarts (binned_all_pairs.hpp:60-63).
+int orc_set_agent_count(orc_sim* h, unsigned n) {
+  Sim& s = *reinterpret_cast<Sim*>(h);
+  if (n > s.cap) return fail(TECS_ERR_INVALID, "agent_count above the capacity");
+  s.n = n;
+  s.desc.agent_count = n;
+  s.step_count = 0;
+  s.grid.built = false;
+  return TECS_OK;
+}
+
+// Simulation::set_capacity, simulation.hpp:266-274: Kokkos::resize of every view keeps the
+// first min(old, new) entries and zero-fills the rest.
+int orc_set_capacity(orc_sim* h, unsigned cap) {
+  Sim& s = *reinterpret_cast<Sim*>(h);
+  if (cap < s.n) return fail(TECS_ERR_INVALID, "capacity below agent_count");
+  const size_t c = cap;
+  for (auto* v : {&s.pos, &s.d1pos, &s.x2pos, &s.d3pos, &s.old_vel}) {
+    v->resize(std::min<size_t>(v->size(), 3 * c));
+    v->resize(3 * c, 0.0f);
+  }
+  for (auto* v : {&s.pol, &s.d1pol, &s.x2pol, &s.d3pol}) {
+    v->resize(std::min<size_t>(v->size(), 2 * c));
+    v->resize(2 * c, 0.0f);
+  }
+  s.type.resize(std::min<size_t>(s.type.size(), c));
+  s.type.resize(c, 0);
+  s.cap = cap;
+  s.desc.capacity = cap;
+  return TECS_OK;
+}
+
 int orc_download(orc_sim* h, float* pos, float* pol, float* old_vel) {
   Sim& s = *reinterpret_cast<Sim*>(h);
   if (pos) std::memcpy(pos, s.pos.data(), sizeof(float) * 3 * s.n);

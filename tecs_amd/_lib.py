@@ -28,6 +28,10 @@ PROTOTYPES = [
     ("tecs_sim_destroy", C.c_int, [_P]),
     ("tecs_sim_upload", C.c_int, [_P, _F32P, _F32P, _P]),
     ("tecs_sim_download", C.c_int, [_P, _F32P, _F32P, _F32P]),
+    ("tecs_sim_upload_range", C.c_int, [_P, C.c_uint32, C.c_uint32, _F32P, _F32P, _P]),
+    ("tecs_sim_set_agent_count", C.c_int, [_P, C.c_uint32]),
+    ("tecs_sim_set_capacity", C.c_int, [_P, C.c_uint32]),
+    ("tecs_sim_get_population", C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     ("tecs_sim_take_step", C.c_int, [_P, C.c_double, _I32]),
     ("tecs_sim_set_force", C.c_int, [_P, _P]),
     ("tecs_sim_evaluate_force", C.c_int, [_P, _I32, _F32P, _F32P]),
@@ -49,6 +53,9 @@ PROTOTYPES = [
     ("tecs_batch_download", C.c_int, [_P, _I32, _F32P, _F32P, _F32P]),
     ("tecs_batch_take_step", C.c_int, [_P, _P, _I32]),
     ("tecs_batch_set_force", C.c_int, [_P, _I32, _P]),
+    ("tecs_batch_upload_range", C.c_int, [_P, _I32, C.c_uint32, C.c_uint32, _F32P, _F32P, _P]),
+    ("tecs_batch_set_agent_count", C.c_int, [_P, _I32, C.c_uint32]),
+    ("tecs_batch_set_capacity", C.c_int, [_P, _I32, C.c_uint32]),
     ("tecs_batch_synchronize", C.c_int, [_P]),
     ("tecs_batch_get_stats", C.c_int, [_P, C.POINTER(abi.Stats)]),
     ("tecs_batch_reset_stats", C.c_int, [_P]),

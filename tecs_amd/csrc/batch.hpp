@@ -367,7 +367,8 @@ struct Batch {
       in.slot_begin = (int)slot;
       in.n = (int)d.agent_count;
       in.cap = (int)d.capacity;
-      const int nb = std::max(1, (int)((d.agent_count + kBlock - 1) / kBlock));
+      // slots for the whole capacity: set_agent_count moves within it
+      const int nb = std::max(1, (int)((d.capacity + kBlock - 1) / kBlock));
       in.padded = nb * kBlock;
       in.block_begin = (int)blocks;
       in.n_blocks = nb;
@@ -416,7 +417,7 @@ struct Batch {
       // bin ranges start on multiples of 64 entries (aligned 16-byte stores in the fill) and
       // leave room for the fill's last 4-bin quad, which may reach bin_count + 3
       bins += (((int64_t)in.bin_count + 4 + 63) / 64) * 64;
-      max_n = std::max(max_n, in.n);
+      max_n = std::max(max_n, in.cap);
       if (slot > (int64_t)1 << 30 || bins > (int64_t)2000000000)
         return fail(TECS_ERR_INVALID, "batch too large for 32-bit slot/bin indices");
     }
@@ -1074,14 +1075,24 @@ struct Batch {
 
   int upload(int k, const float* pos, const float* p, const uint8_t* type) {
     TRY(check_index(k));
-    if (!pos) return fail(TECS_ERR_INVALID, "positions are required");
+    return upload_range(k, 0, descs[k].agent_count, pos, p, type);
+  }
+
+  // Cells first .. first + count - 1 of member k (active or not) from host arrays.
+  int upload_range(int k, uint32_t first, uint32_t count, const float* pos, const float* p,
+                   const uint8_t* type) {
+    TRY(check_index(k));
+    if (!pos && count > 0) return fail(TECS_ERR_INVALID, "positions are required");
     const tecs_sim_desc& d = descs[k];
-    const size_t n = d.agent_count;
+    if ((uint64_t)first + count > d.capacity)
+      return fail(TECS_ERR_INVALID, "cells beyond the capacity");
+    const size_t n = count;
     if (has_type && type && d.force.n_types > 0)
       for (size_t i = 0; i < n; ++i)
         if (type[i] >= (unsigned)d.force.n_types)
           return fail(TECS_ERR_INVALID, "cell type outside the adhesion table");
     TRY(drop_keys());
+    if (n == 0) return TECS_OK;
     HIP_TRY(hipMemcpyAsync(st_pos, pos, sizeof(float) * 3 * n, hipMemcpyHostToDevice, stream));
     const bool up_pol = has_pol && p;
     const bool up_type = has_type && type;
@@ -1089,9 +1100,148 @@ struct Batch {
       HIP_TRY(hipMemcpyAsync(st_pol, p, sizeof(float) * 2 * n, hipMemcpyHostToDevice, stream));
     if (up_type) HIP_TRY(hipMemcpyAsync(st_type, type, n, hipMemcpyHostToDevice, stream));
     const InstanceDesc& in = h_inst[k];
-    launch_upload(d_inst, k, in.block_begin, in.n_blocks, id[ic], st_pos, up_pol ? st_pol : nullptr,
-                  up_type ? st_type : nullptr, x[xc], pol[xc], trig_of(pol[xc]), stream);
+    UploadArgs u{};
+    u.inst = d_inst;
+    u.ii = k;
+    u.block_begin = in.block_begin;
+    u.id = id[ic];
+    u.first = (int)first;
+    u.count = (int)count;
+    u.pos = st_pos;
+    u.pol = up_pol ? st_pol : nullptr;
+    u.type = up_type ? st_type : nullptr;
+    u.x = x[xc];
+    u.p = pol[xc];
+    u.trig = trig_of(pol[xc]);
+    u.x2 = x[xc ^ 1];
+    u.p2 = pol[xc ^ 1];
+    u.trig2 = trig_of(pol[xc ^ 1]);
+    launch_upload(u, in.n_blocks, stream);
     HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(stream));
+    return TECS_OK;
+  }
+
+  // Simulation::set_agent_count (simulation.hpp:285-301) for member k. Every member's cells move
+  // into the other buffer generation, member k's into index order (so its first n slots hold
+  // cells 0..n-1, active or newly active, and the rest keep their state in index order), and
+  // the next full step rebuilds every grid (pair_finder.set_agent_count resets the counter,
+  // binned_all_pairs.hpp:60-63).
+  int set_agent_count(int k, uint32_t n) {
+    TRY(check_index(k));
+    if (n > descs[k].capacity) return fail(TECS_ERR_INVALID, "agent_count above the capacity");
+    if (n_inst > 1 && is_binned(finder) && rebuild_every_n > 0)
+      return fail(TECS_ERR_UNSUPPORTED,
+                  "set_agent_count in a batch that shares a rebuild period > 0 would move every "
+                  "member's rebuild");
+    TRY(drop_keys());
+    ReorderArgs r{};
+    r.inst = d_inst;
+    r.block_inst = d_block_inst;
+    r.target = k;
+    r.id_old = id[ic];
+    r.id_new = id[ic ^ 1];
+    r.x_old = x[xc];
+    r.x_new = x[xc ^ 1];
+    if (has_pol) {
+      r.pol_old = pol[xc];
+      r.pol_new = pol[xc ^ 1];
+      r.trig_old = trig[xc];
+      r.trig_new = trig[xc ^ 1];
+    }
+    if (track_ov) {
+      r.ov_old = ov[xc];
+      r.ov_new = ov[xc ^ 1];
+    }
+    launch_reorder_identity(r, n_blocks, stream);
+    HIP_TRY(hipGetLastError());
+    xc ^= 1;
+    ic ^= 1;
+    // both generations hold member k's inactive cells (in index order)
+    const InstanceDesc& in = h_inst[k];
+    const size_t b0 = (size_t)in.slot_begin + n, len = (size_t)in.padded - n;
+    if (len > 0) {
+      HIP_TRY(hipMemcpyAsync(x[xc ^ 1] + b0, x[xc] + b0, sizeof(float4) * len, hipMemcpyDeviceToDevice, stream));
+      HIP_TRY(hipMemcpyAsync(id[ic ^ 1] + b0, id[ic] + b0, sizeof(int) * len, hipMemcpyDeviceToDevice, stream));
+      if (has_pol) {
+        HIP_TRY(hipMemcpyAsync(pol[xc ^ 1] + b0, pol[xc] + b0, sizeof(float2) * len, hipMemcpyDeviceToDevice, stream));
+        HIP_TRY(hipMemcpyAsync(trig[xc ^ 1] + b0, trig[xc] + b0, sizeof(float4) * len, hipMemcpyDeviceToDevice, stream));
+      }
+      if (track_ov)
+        HIP_TRY(hipMemcpyAsync(ov[xc ^ 1] + b0, ov[xc] + b0, sizeof(float4) * len, hipMemcpyDeviceToDevice, stream));
+    }
+    h_inst[k].n = (int)n;
+    descs[k].agent_count = n;
+    inst_dirty = true;
+    TRY(sync_inst());
+    step_count = 0;
+    grid_built = false;
+    rebuilt_now = false;
+    HIP_TRY(hipStreamSynchronize(stream));
+    return TECS_OK;
+  }
+
+  // Simulation::set_capacity (simulation.hpp:266-274) for member k, within its allocation.
+  int set_capacity(int k, uint32_t cap) {
+    TRY(check_index(k));
+    if (cap < descs[k].agent_count) return fail(TECS_ERR_INVALID, "capacity below agent_count");
+    InstanceDesc& in = h_inst[k];
+    if (cap > (uint32_t)in.padded)
+      return fail(TECS_ERR_UNSUPPORTED, "capacity beyond the member's allocation in this batch");
+    if ((int)cap < in.cap) {
+      ClearArgs c{};
+      c.inst = d_inst;
+      c.ii = k;
+      c.from = (int)cap;
+      c.to = in.cap;
+      for (int g = 0; g < 2; ++g) {
+        c.x[g] = x[g];
+        c.p[g] = has_pol ? pol[g] : nullptr;
+        c.trig[g] = has_pol ? trig[g] : nullptr;
+        c.ov[g] = track_ov ? ov[g] : nullptr;
+      }
+      launch_clear_cells(c, stream);
+      HIP_TRY(hipGetLastError());
+    }
+    in.cap = (int)cap;
+    descs[k].capacity = cap;
+    inst_dirty = true;
+    TRY(sync_inst());
+    HIP_TRY(hipStreamSynchronize(stream));
+    return TECS_OK;
+  }
+
+  // The state of every slot of `old`'s single member, carried into this batch (created for the
+  // same member with a larger capacity): both buffer generations get the cells of the current
+  // one, so its inactive cells stay in index order in both. The grid is rebuilt at the next full
+  // step.
+  int adopt_member(Batch& old) {
+    if (n_inst != 1 || old.n_inst != 1) return fail(TECS_ERR_INVALID, "adopt_member: single members only");
+    HIP_TRY(hipStreamSynchronize(old.stream));
+    const size_t len = (size_t)old.h_inst[0].padded;
+    for (int g = 0; g < 2; ++g) {
+      HIP_TRY(hipMemcpyAsync(x[g], old.x[old.xc], sizeof(float4) * len, hipMemcpyDeviceToDevice, stream));
+      HIP_TRY(hipMemcpyAsync(id[g], old.id[old.ic], sizeof(int) * len, hipMemcpyDeviceToDevice, stream));
+      if (has_pol) {
+        HIP_TRY(hipMemcpyAsync(pol[g], old.pol[old.xc], sizeof(float2) * len, hipMemcpyDeviceToDevice, stream));
+        HIP_TRY(hipMemcpyAsync(trig[g], old.trig[old.xc], sizeof(float4) * len, hipMemcpyDeviceToDevice, stream));
+      }
+      if (track_ov && old.track_ov)
+        HIP_TRY(hipMemcpyAsync(ov[g], old.ov[old.xc], sizeof(float4) * len, hipMemcpyDeviceToDevice, stream));
+    }
+    variant = old.variant;
+    graph_mode = old.graph_mode;
+    timing = old.timing;
+    stats = old.stats;
+    write_step_count = old.write_step_count;
+    // the per-block pair counters carry the statistics: move their total into block 0
+    {
+      tecs_stats st{};
+      TRY(old.get_stats(&st));
+      stats.pair_interactions = 0;
+      const unsigned long long total = st.pair_interactions;
+      HIP_TRY(hipMemcpyAsync(block_pairs, &total, sizeof(total), hipMemcpyHostToDevice, stream));
+    }
     HIP_TRY(hipStreamSynchronize(stream));
     return TECS_OK;
   }
@@ -1288,7 +1438,7 @@ struct Batch {
 
   // Device bytes a member of this description adds to a batch (state, scratch and grid).
   static size_t estimate_bytes(const tecs_sim_desc& d) {
-    const size_t slots = ((size_t)d.agent_count + kBlock - 1) / kBlock * kBlock + kBlock;
+    const size_t slots = ((size_t)d.capacity + kBlock - 1) / kBlock * kBlock + kBlock;
     const bool pol = (d.fields & TECS_FIELD_POLARITY) != 0;
     size_t per_slot = 2 * 16 + 2 * 4 + 16 + 16 * 2 /* x, id, d1, ov */ + 8 * 4 /* keys.. */;
     if (pol) per_slot += 2 * 8 + 8 + 2 * 16 /* pol, d1 pol, trig */;

@@ -355,25 +355,72 @@ __global__ __launch_bounds__(kBlock) void k_init_ids(const InstanceDesc* inst,
   const int lb = blockIdx.x;
   const InstanceDesc& in = inst[block_inst[lb]];
   const int s = lb * kBlock + (int)threadIdx.x;
-  const int local = s - in.slot_begin;
-  id[s] = local < in.n ? local : -1;
+  // every slot starts with the cell of its own index; inactive slots keep that order
+  id[s] = s - in.slot_begin;
 }
 
-// Host arrays are in agent order; slot s holds agent id[s].
-__global__ __launch_bounds__(kBlock) void k_upload(const InstanceDesc* inst, int ii,
-                                                   int block_begin, const int* id,
-                                                   const float* pos, const float* pol,
-                                                   const uint8_t* type, float4* x, float2* p,
-                                                   float4* trig) {
-  const InstanceDesc& in = inst[ii];
-  const int s = (block_begin + (int)blockIdx.x) * kBlock + (int)threadIdx.x;
-  if ((s - in.slot_begin) >= in.n) return;
-  const int i = id[s];
-  float w = x[s].w;
-  if (type) w = __int_as_float((int)type[i]);
-  x[s] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], w);
-  if (pol) p[s] = make_float2(pol[2 * i], pol[2 * i + 1]);
-  if (trig) trig[s] = polar_trig(p[s]);
+// Host arrays hold cells first .. first + count - 1 in agent order; slot s holds agent id[s]
+// (an inactive slot: the cell of its own index, kept in both buffer generations x / x2).
+__global__ __launch_bounds__(kBlock) void k_upload(UploadArgs a) {
+  const InstanceDesc& in = a.inst[a.ii];
+  const int s = (a.block_begin + (int)blockIdx.x) * kBlock + (int)threadIdx.x;
+  const int local = s - in.slot_begin;
+  const bool active = local < in.n;
+  const int i = (active ? a.id[s] : local) - a.first;
+  if (i < 0 || i >= a.count) return;
+  float w = a.x[s].w;
+  if (a.type) w = __int_as_float((int)a.type[i]);
+  const float4 v = make_float4(a.pos[3 * i], a.pos[3 * i + 1], a.pos[3 * i + 2], w);
+  a.x[s] = v;
+  if (!active) a.x2[s] = v;
+  if (a.pol) {
+    const float2 q = make_float2(a.pol[2 * i], a.pol[2 * i + 1]);
+    a.p[s] = q;
+    const float4 t = polar_trig(q);
+    a.trig[s] = t;
+    if (!active) {
+      a.p2[s] = q;
+      a.trig2[s] = t;
+    }
+  }
+}
+
+// A population change of instance `target` (set_agent_count): every slot of the batch moves
+// into the other buffer generation. The target's cells go to the slot of their agent index
+// (the order its inactive slots always have, so its ids are a permutation of the padded range);
+// the other instances' slots are copied in place.
+__global__ __launch_bounds__(kBlock) void k_reorder_identity(ReorderArgs a) {
+  const int lb = blockIdx.x;
+  const int ii = a.block_inst[lb];
+  const InstanceDesc& in = a.inst[ii];
+  const int s = lb * kBlock + (int)threadIdx.x;
+  const int id = a.id_old[s];
+  if (ii == a.target && (unsigned)id >= (unsigned)in.padded) return;  // never: ids are a permutation
+  const int dst = ii == a.target ? in.slot_begin + id : s;
+  a.id_new[dst] = id;
+  a.x_new[dst] = a.x_old[s];
+  if (a.pol_old) {
+    a.pol_new[dst] = a.pol_old[s];
+    a.trig_new[dst] = a.trig_old[s];
+  }
+  if (a.ov_old) a.ov_new[dst] = a.ov_old[s];
+}
+
+// Cells [from, to) of instance ii (inactive, in index order) lose their state, in both buffer
+// generations: Kokkos::resize to a smaller extent drops them, a larger one brings zeros.
+__global__ __launch_bounds__(kBlock) void k_clear_cells(ClearArgs a) {
+  const InstanceDesc& in = a.inst[a.ii];
+  const int local = a.from + (int)(blockIdx.x * kBlock + threadIdx.x);
+  if (local >= a.to) return;
+  const int s = in.slot_begin + local;
+  for (int g = 0; g < 2; ++g) {
+    a.x[g][s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (a.p[g]) {
+      a.p[g][s] = make_float2(0.0f, 0.0f);
+      a.trig[g][s] = polar_trig(make_float2(0.0f, 0.0f));
+    }
+    if (a.ov[g]) a.ov[g][s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_download(const InstanceDesc* inst, int ii,
@@ -465,11 +512,18 @@ void launch_init_ids(const InstanceDesc* inst, const int* block_inst, int* id, i
   hipLaunchKernelGGL(k_init_ids, dim3(n_blocks), dim3(kBlock), 0, stream, inst, block_inst, id);
 }
 
-void launch_upload(const InstanceDesc* inst, int ii, int block_begin, int n_blocks, const int* id,
-                   const float* pos, const float* pol, const uint8_t* type, float4* x, float2* p,
-                   float4* trig, hipStream_t stream) {
-  hipLaunchKernelGGL(k_upload, dim3(n_blocks), dim3(kBlock), 0, stream, inst, ii, block_begin, id,
-                     pos, pol, type, x, p, trig);
+void launch_upload(const UploadArgs& a, int n_blocks, hipStream_t stream) {
+  hipLaunchKernelGGL(k_upload, dim3(n_blocks), dim3(kBlock), 0, stream, a);
+}
+
+void launch_reorder_identity(const ReorderArgs& a, int n_blocks, hipStream_t stream) {
+  hipLaunchKernelGGL(k_reorder_identity, dim3(n_blocks), dim3(kBlock), 0, stream, a);
+}
+
+void launch_clear_cells(const ClearArgs& a, hipStream_t stream) {
+  if (a.to <= a.from) return;
+  hipLaunchKernelGGL(k_clear_cells, dim3((a.to - a.from + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                     stream, a);
 }
 
 void launch_download(const InstanceDesc* inst, int ii, int block_begin, int n_blocks,

@@ -4,6 +4,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -14,8 +15,10 @@ thread_local std::string g_err;
 using tecs_amd::Batch;
 using tecs_amd::fail;
 
+// A simulation is a batch of one; the batch is replaced when set_capacity grows past its
+// allocation.
 struct tecs_sim {
-  Batch b;
+  std::unique_ptr<Batch> b;
 };
 struct tecs_batch {
   Batch b;
@@ -83,7 +86,12 @@ int tecs_sim_create(const tecs_sim_desc* desc, void* stream, tecs_sim** out) {
   *out = nullptr;
   tecs_sim* s = new (std::nothrow) tecs_sim();
   if (!s) return fail(TECS_ERR_OOM, "host allocation failed");
-  const int rc = s->b.init(desc, 1, stream);
+  s->b.reset(new (std::nothrow) Batch());
+  if (!s->b) {
+    delete s;
+    return fail(TECS_ERR_OOM, "host allocation failed");
+  }
+  const int rc = s->b->init(desc, 1, stream);
   if (rc != TECS_OK) {
     delete s;
     return rc;
@@ -100,45 +108,79 @@ int tecs_sim_destroy(tecs_sim* sim) {
 int tecs_sim_upload(tecs_sim* sim, const float* positions, const float* polarities,
                     const uint8_t* types) {
   GUARD(sim);
-  return sim->b.upload(0, positions, polarities, types);
+  return sim->b->upload(0, positions, polarities, types);
 }
 
 int tecs_sim_download(tecs_sim* sim, float* positions, float* polarities, float* old_velocities) {
   GUARD(sim);
-  return sim->b.download(0, positions, polarities, old_velocities);
+  return sim->b->download(0, positions, polarities, old_velocities);
+}
+
+int tecs_sim_upload_range(tecs_sim* sim, uint32_t first, uint32_t count, const float* positions,
+                          const float* polarities, const uint8_t* types) {
+  GUARD(sim);
+  return sim->b->upload_range(0, first, count, positions, polarities, types);
+}
+
+int tecs_sim_set_agent_count(tecs_sim* sim, uint32_t agent_count) {
+  GUARD(sim);
+  return sim->b->set_agent_count(0, agent_count);
+}
+
+int tecs_sim_set_capacity(tecs_sim* sim, uint32_t capacity) {
+  GUARD(sim);
+  Batch& old = *sim->b;
+  if (capacity < old.descs[0].agent_count) return fail(TECS_ERR_INVALID, "capacity below agent_count");
+  if (capacity <= (uint32_t)old.h_inst[0].padded) return old.set_capacity(0, capacity);
+  // past the allocation: a new batch for the larger capacity takes over the state
+  tecs_sim_desc d = old.descs[0];
+  d.capacity = capacity;
+  std::unique_ptr<Batch> nb(new (std::nothrow) Batch());
+  if (!nb) return fail(TECS_ERR_OOM, "host allocation failed");
+  TRY(nb->init(&d, 1, old.own_stream ? nullptr : old.stream));
+  TRY(nb->adopt_member(old));
+  sim->b = std::move(nb);
+  return TECS_OK;
+}
+
+int tecs_sim_get_population(tecs_sim* sim, uint32_t* agent_count, uint32_t* capacity) {
+  GUARD(sim);
+  if (agent_count) *agent_count = sim->b->descs[0].agent_count;
+  if (capacity) *capacity = sim->b->descs[0].capacity;
+  return TECS_OK;
 }
 
 int tecs_sim_take_step(tecs_sim* sim, double dt, int32_t n_steps) {
   GUARD(sim);
-  return sim->b.take_step(&dt, n_steps);
+  return sim->b->take_step(&dt, n_steps);
 }
 
 int tecs_sim_set_force(tecs_sim* sim, const tecs_force_desc* force) {
   GUARD(sim);
-  return sim->b.set_force(0, force);
+  return sim->b->set_force(0, force);
 }
 
 int tecs_sim_evaluate_force(tecs_sim* sim, int32_t is_full_step, float* dpos, float* dpol) {
   GUARD(sim);
-  return sim->b.evaluate(is_full_step != 0, dpos, dpol);
+  return sim->b->evaluate(is_full_step != 0, dpos, dpol);
 }
 
 int tecs_sim_pairs(tecs_sim* sim, int32_t is_full_step, int64_t* row_ptr, int32_t* cols,
                    int64_t cols_capacity, int64_t* n_pairs) {
   GUARD(sim);
-  return sim->b.pairs(is_full_step != 0, row_ptr, cols, cols_capacity, n_pairs);
+  return sim->b->pairs(is_full_step != 0, row_ptr, cols, cols_capacity, n_pairs);
 }
 
 int tecs_sim_rebuild(tecs_sim* sim) {
   GUARD(sim);
-  int rc = sim->b.rebuild();
-  if (rc == TECS_OK) sim->b.step_count = 1;  // as a due full step: reset to 0, then ++
+  int rc = sim->b->rebuild();
+  if (rc == TECS_OK) sim->b->step_count = 1;  // as a due full step: reset to 0, then ++
   return rc;
 }
 
 int tecs_sim_synchronize(tecs_sim* sim) {
   GUARD(sim);
-  const hipError_t e = hipStreamSynchronize(sim->b.stream);
+  const hipError_t e = hipStreamSynchronize(sim->b->stream);
   if (e != hipSuccess) return fail(TECS_ERR_DEVICE, hipGetErrorString(e));
   return TECS_OK;
 }
@@ -146,54 +188,54 @@ int tecs_sim_synchronize(tecs_sim* sim) {
 int tecs_sim_get_stats(tecs_sim* sim, tecs_stats* stats) {
   GUARD(sim);
   if (!stats) return fail(TECS_ERR_INVALID, "null stats");
-  return sim->b.get_stats(stats);
+  return sim->b->get_stats(stats);
 }
 
 int tecs_sim_reset_stats(tecs_sim* sim) {
   GUARD(sim);
-  return sim->b.reset_stats();
+  return sim->b->reset_stats();
 }
 
 int tecs_sim_set_sweep_variant(tecs_sim* sim, int32_t variant) {
   GUARD(sim);
   if (variant < TECS_SWEEP_AUTO || variant > TECS_SWEEP_TEAM16)
     return fail(TECS_ERR_INVALID, "bad sweep variant");
-  sim->b.variant = variant;
-  sim->b.drop_graphs();
+  sim->b->variant = variant;
+  sim->b->drop_graphs();
   return TECS_OK;
 }
 
 int tecs_sim_get_sweep_variant(tecs_sim* sim, int32_t* variant) {
   GUARD(sim);
   if (!variant) return fail(TECS_ERR_INVALID, "null variant");
-  *variant = sim->b.concrete_variant();
+  *variant = sim->b->concrete_variant();
   return TECS_OK;
 }
 
 int tecs_sim_get_autotune(tecs_sim* sim, double* ms_per_variant) {
   GUARD(sim);
   if (!ms_per_variant) return fail(TECS_ERR_INVALID, "null output");
-  for (int v = 0; v < 6; ++v) ms_per_variant[v] = sim->b.tune_ms[v];
+  for (int v = 0; v < 6; ++v) ms_per_variant[v] = sim->b->tune_ms[v];
   return TECS_OK;
 }
 
 int tecs_sim_set_graph_mode(tecs_sim* sim, int32_t mode) {
   GUARD(sim);
   if (mode < TECS_GRAPHS_AUTO || mode > TECS_GRAPHS_ON) return fail(TECS_ERR_INVALID, "bad graph mode");
-  sim->b.graph_mode = mode;
+  sim->b->graph_mode = mode;
   return TECS_OK;
 }
 
 int tecs_sim_get_graph_launches(tecs_sim* sim, int64_t* launches) {
   GUARD(sim);
   if (!launches) return fail(TECS_ERR_INVALID, "null output");
-  *launches = sim->b.graph_launches;
+  *launches = sim->b->graph_launches;
   return TECS_OK;
 }
 
 int tecs_sim_set_kernel_timing(tecs_sim* sim, int32_t enabled) {
   GUARD(sim);
-  sim->b.timing = enabled != 0;
+  sim->b->timing = enabled != 0;
   return TECS_OK;
 }
 
@@ -202,10 +244,10 @@ int tecs_sim_get_kernel_timing(tecs_sim* sim, int32_t kernel_class, double* ms,
   GUARD(sim);
   if (kernel_class < 0 || kernel_class >= tecs_amd::kTimeClasses)
     return fail(TECS_ERR_INVALID, "bad kernel class");
-  const int rc = sim->b.fold_timing();
+  const int rc = sim->b->fold_timing();
   if (rc) return rc;
-  if (ms) *ms = sim->b.t_ms[kernel_class];
-  if (launches) *launches = sim->b.t_launches[kernel_class];
+  if (ms) *ms = sim->b->t_ms[kernel_class];
+  if (launches) *launches = sim->b->t_launches[kernel_class];
   return TECS_OK;
 }
 
@@ -250,6 +292,22 @@ int tecs_batch_take_step(tecs_batch* batch, const double* dts, int32_t n_steps) 
 int tecs_batch_set_force(tecs_batch* batch, int32_t index, const tecs_force_desc* force) {
   GUARD(batch);
   return batch->b.set_force(index, force);
+}
+
+int tecs_batch_upload_range(tecs_batch* batch, int32_t index, uint32_t first, uint32_t count,
+                            const float* positions, const float* polarities, const uint8_t* types) {
+  GUARD(batch);
+  return batch->b.upload_range(index, first, count, positions, polarities, types);
+}
+
+int tecs_batch_set_agent_count(tecs_batch* batch, int32_t index, uint32_t agent_count) {
+  GUARD(batch);
+  return batch->b.set_agent_count(index, agent_count);
+}
+
+int tecs_batch_set_capacity(tecs_batch* batch, int32_t index, uint32_t capacity) {
+  GUARD(batch);
+  return batch->b.set_capacity(index, capacity);
 }
 
 int tecs_batch_synchronize(tecs_batch* batch) {
@@ -376,7 +434,7 @@ int tecs_writer_write_static(tecs_writer* w, const char* label, uint32_t n, cons
 int tecs_sim_write(tecs_sim* sim, tecs_writer* w, double time) {
   GUARD(sim);
   GUARD(w);
-  return sim->b.write_snapshot(0, &w->w, time);
+  return sim->b->write_snapshot(0, &w->w, time);
 }
 
 int tecs_batch_write(tecs_batch* batch, int32_t index, tecs_writer* w, double time) {

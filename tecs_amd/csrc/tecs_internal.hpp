@@ -155,9 +155,51 @@ void launch_exclusive_scan(const int* in, int n, int* out, int* bsum, int* head_
 
 void launch_init_ids(const InstanceDesc* inst, const int* block_inst, int* id, int n_blocks,
                      hipStream_t stream);
-void launch_upload(const InstanceDesc* inst, int ii, int block_begin, int n_blocks, const int* id,
-                   const float* pos, const float* pol, const uint8_t* type, float4* x, float2* p,
-                   float4* trig, hipStream_t stream);
+struct UploadArgs {
+  const InstanceDesc* inst;
+  int ii, block_begin;
+  const int* id;
+  int first, count;  // the host arrays hold cells first .. first + count - 1
+  const float* pos;
+  const float* pol;      // may be null
+  const uint8_t* type;   // may be null
+  float4* x;             // current generation
+  float2* p;
+  float4* trig;
+  float4* x2;            // the other generation (inactive cells are kept in both)
+  float2* p2;
+  float4* trig2;
+};
+void launch_upload(const UploadArgs& a, int n_blocks, hipStream_t stream);
+
+// set_agent_count: all slots into the other generation, the target instance in index order
+struct ReorderArgs {
+  const InstanceDesc* inst;
+  const int* block_inst;
+  int target;
+  const int* id_old;
+  int* id_new;
+  const float4* x_old;
+  float4* x_new;
+  const float2* pol_old;  // may be null
+  float2* pol_new;
+  const float4* trig_old;
+  float4* trig_new;
+  const float4* ov_old;   // may be null
+  float4* ov_new;
+};
+void launch_reorder_identity(const ReorderArgs& a, int n_blocks, hipStream_t stream);
+
+// set_capacity: cells [from, to) of instance ii back to zero state in both generations
+struct ClearArgs {
+  const InstanceDesc* inst;
+  int ii, from, to;
+  float4* x[2];
+  float2* p[2];
+  float4* trig[2];
+  float4* ov[2];
+};
+void launch_clear_cells(const ClearArgs& a, hipStream_t stream);
 void launch_download(const InstanceDesc* inst, int ii, int block_begin, int n_blocks,
                      const int* id, const float4* x, const float2* p, const float4* ov,
                      float* pos, float* pol, float* old_vel, hipStream_t stream);

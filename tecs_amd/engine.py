@@ -77,6 +77,30 @@ class GpuSimulation:
     def take_step(self, dt: float, n_steps: int = 1):
         check(self._lib.tecs_sim_take_step(self._h, float(dt), int(n_steps)))
 
+    def upload_range(self, first: int, positions, polarities=None, types=None):
+        """Cells first .. first + len(positions) - 1 (active or not)."""
+        p = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+        k = p.shape[0]
+        q = _f32(polarities, (k, 2)) if (polarities is not None and self.has_pol) else None
+        t = _u8(types, k) if (types is not None and self.has_type) else None
+        check(self._lib.tecs_sim_upload_range(self._h, int(first), k, _ptr(p), _ptr(q), _ptr(t)))
+
+    def set_agent_count(self, n: int):
+        """Simulation::set_agent_count (simulation.hpp:285-301)."""
+        check(self._lib.tecs_sim_set_agent_count(self._h, int(n)))
+        self.n = int(n)
+        self.desc.agent_count = self.n
+
+    def set_capacity(self, capacity: int):
+        """Simulation::set_capacity (simulation.hpp:266-274)."""
+        check(self._lib.tecs_sim_set_capacity(self._h, int(capacity)))
+        self.desc.capacity = int(capacity)
+
+    def population(self):
+        n, cap = C.c_uint32(0), C.c_uint32(0)
+        check(self._lib.tecs_sim_get_population(self._h, C.byref(n), C.byref(cap)))
+        return int(n.value), int(cap.value)
+
     def set_force(self, force):
         """The tecs_force_desc (abi.ForceDesc) of the following steps."""
         check(self._lib.tecs_sim_set_force(self._h, C.byref(force)))
@@ -182,6 +206,22 @@ class GpuBatch:
         q = _f32(polarities, (n, 2)) if (polarities is not None and d.fields & abi.FIELD_POLARITY) else None
         t = _u8(types, n) if (types is not None and d.fields & abi.FIELD_TYPE) else None
         check(self._lib.tecs_batch_upload(self._h, int(k), _ptr(p), _ptr(q), _ptr(t)))
+
+    def upload_range(self, k, first: int, positions, polarities=None, types=None):
+        d = self.descs[k]
+        p = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+        m = p.shape[0]
+        q = _f32(polarities, (m, 2)) if (polarities is not None and d.fields & abi.FIELD_POLARITY) else None
+        t = _u8(types, m) if (types is not None and d.fields & abi.FIELD_TYPE) else None
+        check(self._lib.tecs_batch_upload_range(self._h, int(k), int(first), m, _ptr(p), _ptr(q), _ptr(t)))
+
+    def set_agent_count(self, k, n: int):
+        check(self._lib.tecs_batch_set_agent_count(self._h, int(k), int(n)))
+        self.descs[k].agent_count = int(n)
+
+    def set_capacity(self, k, capacity: int):
+        check(self._lib.tecs_batch_set_capacity(self._h, int(k), int(capacity)))
+        self.descs[k].capacity = int(capacity)
 
     def download(self, k):
         d = self.descs[k]
