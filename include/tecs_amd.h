@@ -177,6 +177,26 @@ int tecs_sim_set_agent_count(tecs_sim* sim, uint32_t agent_count);
 int tecs_sim_set_capacity(tecs_sim* sim, uint32_t capacity);
 int tecs_sim_get_population(tecs_sim* sim, uint32_t* agent_count, uint32_t* capacity);
 
+/* ---- grid queries (acceleration::Grid, grid.hpp:413-637), binned finders ------------------
+ * Batched over n query points (points: n*3 floats), on the grid of the last rebuild and the
+ * current positions (the reference's Grid reads its positions view at query time). Bins and
+ * members are visited in the grid's order (bins x-fastest, members ascending in agent index).
+ * Output: one agent index per query, -1 where none qualifies. A random pick takes the caller's
+ * uniform draw u in [0, 1) per query and chooses candidate floor(u * total) of the `total`
+ * candidates in visiting order; the reference draws from its Kokkos pool (rng.rand / urand).
+ * Synchronous; TECS_ERR_STATE before any grid, TECS_ERR_UNSUPPORTED for naive finders. */
+/* get_nearest_point_index (grid.hpp:413-441): smallest squared distance, first in visiting
+ * order on ties, over the bins within search_radius bins of the point's bin. */
+int tecs_sim_nearest(tecs_sim* sim, int32_t n, const float* points, int32_t search_radius,
+                     int32_t* out);
+/* get_random_point_index_in_neighbourhood_bins (grid.hpp:453-495): any cell of those bins. */
+int tecs_sim_random_in_bins(tecs_sim* sim, int32_t n, const float* points, int32_t radius_bins,
+                            const float* uniforms, int32_t* out);
+/* get_random_point_index_in_neighbourhood (grid.hpp:507-564): a cell with squared distance
+ * <= radius^2 (bins: max(1, ceil(radius / bin_size))). */
+int tecs_sim_random_neighbour(tecs_sim* sim, int32_t n, const float* points, float radius,
+                              const float* uniforms, int32_t* out);
+
 /* n_steps integrator steps of size dt (Integrator::integrate, euler.hpp:81-85 / heun.hpp:201-208).
  * Asynchronous on the simulation's stream. */
 int tecs_sim_take_step(tecs_sim* sim, double dt, int32_t n_steps);

@@ -325,6 +325,16 @@ class Simulation {
     return c;
   }
 
+  // Grid::get_nearest_point_index (grid.hpp:413-441) for a batch of points on the simulation's
+  // grid; -1 where none of the searched bins holds a cell.
+  std::vector<int32_t> get_nearest_point_indices(const std::vector<std::array<float, 3>>& points,
+                                                 int search_radius = 1) {
+    std::vector<int32_t> out(points.size());
+    check(tecs_sim_nearest(h_, (int32_t)points.size(), points.empty() ? nullptr : points[0].data(),
+                           search_radius, out.data()));
+    return out;
+  }
+
   std::vector<std::array<float, 3>> positions() {
     std::vector<std::array<float, 3>> x(n_);
     check(tecs_sim_download(h_, x.empty() ? nullptr : x[0].data(), nullptr, nullptr));

@@ -66,6 +66,7 @@ def load():
         "orc_upload_range": (C.c_int, [P, C.c_uint, C.c_uint, P, P, P]),
         "orc_set_agent_count": (C.c_int, [P, C.c_uint]),
         "orc_set_capacity": (C.c_int, [P, C.c_uint]),
+        "orc_query": (C.c_int, [P, C.c_int, C.c_int, P, C.c_int, C.c_float, P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -139,6 +140,22 @@ class OracleSim:
 
     def set_capacity(self, capacity: int):
         _check(self._lib.orc_set_capacity(self._h, int(capacity)))
+
+    def _query(self, kind, points, radius_bins, radius, uniforms):
+        p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+        u = None if uniforms is None else np.ascontiguousarray(uniforms, np.float32).reshape(p.shape[0])
+        out = np.empty(p.shape[0], np.int32)
+        _check(self._lib.orc_query(self._h, int(kind), p.shape[0], _ptr(p), int(radius_bins), float(radius), _ptr(u), _ptr(out)))
+        return out
+
+    def nearest(self, points, search_radius: int = 1):
+        return self._query(0, points, search_radius, 0.0, None)
+
+    def random_in_bins(self, points, radius_bins: int, uniforms):
+        return self._query(1, points, radius_bins, 0.0, uniforms)
+
+    def random_neighbour(self, points, radius: float, uniforms):
+        return self._query(2, points, 0, radius, uniforms)
 
     def rebuild(self):
         _check(self._lib.orc_rebuild(self._h))

@@ -839,6 +839,63 @@ int orc_upload_range(orc_sim* h, unsigned first, unsigned count, const float* po
   return TECS_OK;
 }
 
+// Grid queries on the grid of the last rebuild and the current positions (grid.hpp:413-564):
+// kind 0 get_nearest_point_index(point, radius_bins); 1 get_random_point_index_in_neighbourhood_bins
+// (point, radius_bins, draw); 2 get_random_point_index_in_neighbourhood(point, radius, draw), whose
+// bin radius is max(1, int(ceil(radius / bin_size))). The draw picks candidate
+// min(int(u * total), total - 1) of the total in visiting order (the reference's rng.rand /
+// urand from its Kokkos pool).
+int orc_query(orc_sim* h, int kind, int n, const float* points, int radius_bins, float radius,
+              const float* uniforms, int32_t* out) {
+  Sim& s = *reinterpret_cast<Sim*>(h);
+  if (!finder_is_binned(s)) return fail(TECS_ERR_UNSUPPORTED, "grid queries need a binned finder");
+  if (!s.grid.built) return fail(TECS_ERR_STATE, "no grid yet");
+  const Grid& g = s.grid;
+  if (kind == 2) radius_bins = std::max(1, (int)std::ceil(radius / g.bin_size));
+  const float r2 = radius * radius;
+  for (int q = 0; q < n; ++q) {
+    const float* p = &points[3 * (size_t)q];
+    int c[3];
+    g.coords(p, c);  // calc_bin_coords_from_point
+    const int side = 2 * radius_bins + 1;
+    const int tasks = side * side * side;
+    // the members of the searched bins in visiting order (linear_index_to_offset, grid.hpp:308-322)
+    std::vector<int> members;
+    for (int t = 0; t < tasks; ++t) {
+      const int tmp = t / side;
+      const int ni[3] = {c[0] + (t % side) - radius_bins, c[1] + (tmp % side) - radius_bins,
+                         c[2] + (tmp / side) - radius_bins};
+      if (ni[0] < 0 || ni[0] >= g.ext[0] || ni[1] < 0 || ni[1] >= g.ext[1] || ni[2] < 0 ||
+          ni[2] >= g.ext[2])
+        continue;
+      const int b = g.flatten(ni);
+      for (int idx = g.offsets[b]; idx < g.offsets[(size_t)b + 1]; ++idx) members.push_back(g.perm[idx]);
+    }
+    int result = -1;
+    if (kind == 0) {
+      float best = FLT_MAX;
+      for (int j : members) {
+        const float d2 = dist2(p, &s.pos[3 * (size_t)j]);
+        if (d2 < best) {
+          best = d2;
+          result = j;
+        }
+      }
+    } else {
+      std::vector<int> cand;
+      for (int j : members)
+        if (kind == 1 || dist2(p, &s.pos[3 * (size_t)j]) <= r2) cand.push_back(j);
+      if (!cand.empty()) {
+        const int total = (int)cand.size();
+        const int target = std::min((int)(uniforms[q] * (float)total), total - 1);
+        result = cand[(size_t)target];
+      }
+    }
+    out[q] = result;
+  }
+  return TECS_OK;
+}
+
 // Simulation::set_agent_count, simulation.hpp:285-301: the views keep their contents, the pair
 // finder's rebuild counter restarts (binned_all_pairs.hpp:60-63).
 int orc_set_agent_count(orc_sim* h, unsigned n) {

@@ -1181,6 +1181,49 @@ struct Batch {
     return TECS_OK;
   }
 
+  // Grid queries (grid.hpp:413-564) for member 0 on the grid of the last rebuild; points and
+  // uniforms are host arrays of n queries, out receives n agent indices.
+  int query(int kind, int n, const float* points, int radius_bins, float radius,
+            const float* uniforms, int32_t* out) {
+    if (n < 0 || (n > 0 && (!points || !out))) return fail(TECS_ERR_INVALID, "bad query arrays");
+    if (kind != kQueryNearest && n > 0 && !uniforms) return fail(TECS_ERR_INVALID, "uniforms required");
+    if (!is_binned(finder)) return fail(TECS_ERR_UNSUPPORTED, "grid queries need a binned pair finder");
+    if (!grid_built) return fail(TECS_ERR_STATE, "no grid yet: queries follow a full step or rebuild");
+    if (radius_bins < 0) return fail(TECS_ERR_INVALID, "negative search radius");
+    if (n == 0) return TECS_OK;
+    void* buf = nullptr;
+    const size_t bytes = sizeof(float) * 4 * (size_t)n + sizeof(int32_t) * (size_t)n;
+    HIP_TRY(hipMalloc(&buf, bytes));
+    float* d_points = static_cast<float*>(buf);
+    float* d_u = d_points + 3 * (size_t)n;
+    int32_t* d_out = reinterpret_cast<int32_t*>(d_u + n);
+    int rc = TECS_OK;
+    hipError_t e = hipMemcpyAsync(d_points, points, sizeof(float) * 3 * n, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess && uniforms)
+      e = hipMemcpyAsync(d_u, uniforms, sizeof(float) * n, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) {
+      QueryArgs q{};
+      q.inst = d_inst;
+      q.xs = x[xc];
+      q.off = fill_bins ? off : coff[ic];
+      q.id = id[ic];
+      q.n = n;
+      q.points = d_points;
+      q.uniforms = d_u;
+      q.radius_bins = radius_bins;
+      q.radius = radius;
+      q.out = d_out;
+      launch_query(kind, q, stream);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(out, d_out, sizeof(int32_t) * n, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) rc = fail(TECS_ERR_DEVICE, std::string("grid query: ") + hipGetErrorString(e));
+    hipFree(buf);
+    return rc;
+  }
+
   // Simulation::set_capacity (simulation.hpp:266-274) for member k, within its allocation.
   int set_capacity(int k, uint32_t cap) {
     TRY(check_index(k));

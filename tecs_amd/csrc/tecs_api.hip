@@ -150,6 +150,28 @@ int tecs_sim_get_population(tecs_sim* sim, uint32_t* agent_count, uint32_t* capa
   return TECS_OK;
 }
 
+int tecs_sim_nearest(tecs_sim* sim, int32_t n, const float* points, int32_t search_radius,
+                     int32_t* out) {
+  GUARD(sim);
+  return sim->b->query(tecs_amd::kQueryNearest, n, points, search_radius, 0.0f, nullptr, out);
+}
+
+int tecs_sim_random_in_bins(tecs_sim* sim, int32_t n, const float* points, int32_t radius_bins,
+                            const float* uniforms, int32_t* out) {
+  GUARD(sim);
+  return sim->b->query(tecs_amd::kQueryRandomInBins, n, points, radius_bins, 0.0f, uniforms, out);
+}
+
+int tecs_sim_random_neighbour(tecs_sim* sim, int32_t n, const float* points, float radius,
+                              const float* uniforms, int32_t* out) {
+  GUARD(sim);
+  if (!(radius >= 0.0f)) return fail(TECS_ERR_INVALID, "radius must be >= 0");
+  // grid.hpp:514: radius_bins = max(1, int(ceil(search_radius / bin_size)))
+  const float bin = sim->b->h_inst[0].bin_size;
+  const int rb = std::max(1, (int)std::ceil(radius / bin));
+  return sim->b->query(tecs_amd::kQueryRandomNeighbour, n, points, rb, radius, uniforms, out);
+}
+
 int tecs_sim_take_step(tecs_sim* sim, double dt, int32_t n_steps) {
   GUARD(sim);
   return sim->b->take_step(&dt, n_steps);

@@ -205,6 +205,22 @@ void launch_download(const InstanceDesc* inst, int ii, int block_begin, int n_bl
                      float* pos, float* pol, float* old_vel, hipStream_t stream);
 
 // trig[s] = polar_trig(pol[s]) for all slots of the batch
+// Grid queries (kernels_query.hip) on instance 0 of a batch.
+enum QueryKind { kQueryNearest = 0, kQueryRandomInBins = 1, kQueryRandomNeighbour = 2 };
+struct QueryArgs {
+  const InstanceDesc* inst;
+  const float4* xs;     // current positions (slot order)
+  const int* off;       // per-bin offsets of the grid
+  const int* id;        // slot -> agent index
+  int n;                // queries
+  const float* points;  // n * 3
+  const float* uniforms;
+  int radius_bins;
+  float radius;
+  int32_t* out;
+};
+void launch_query(int kind, const QueryArgs& a, hipStream_t stream);
+
 void launch_polar_trig(const float2* pol, float4* trig, int total_slots, hipStream_t stream);
 void launch_copy_probe(const float4* src, float4* dst, int64_t n, hipStream_t stream);
 

@@ -96,6 +96,29 @@ class GpuSimulation:
         check(self._lib.tecs_sim_set_capacity(self._h, int(capacity)))
         self.desc.capacity = int(capacity)
 
+    def nearest(self, points, search_radius: int = 1):
+        """Grid::get_nearest_point_index for every row of points (grid.hpp:413-441)."""
+        p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+        out = np.empty(p.shape[0], np.int32)
+        check(self._lib.tecs_sim_nearest(self._h, p.shape[0], _ptr(p), int(search_radius), _ptr(out)))
+        return out
+
+    def random_in_bins(self, points, radius_bins: int, uniforms):
+        """Grid::get_random_point_index_in_neighbourhood_bins with the caller's draws."""
+        p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+        u = np.ascontiguousarray(uniforms, np.float32).reshape(p.shape[0])
+        out = np.empty(p.shape[0], np.int32)
+        check(self._lib.tecs_sim_random_in_bins(self._h, p.shape[0], _ptr(p), int(radius_bins), _ptr(u), _ptr(out)))
+        return out
+
+    def random_neighbour(self, points, radius: float, uniforms):
+        """Grid::get_random_point_index_in_neighbourhood with the caller's draws."""
+        p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+        u = np.ascontiguousarray(uniforms, np.float32).reshape(p.shape[0])
+        out = np.empty(p.shape[0], np.int32)
+        check(self._lib.tecs_sim_random_neighbour(self._h, p.shape[0], _ptr(p), float(radius), _ptr(u), _ptr(out)))
+        return out
+
     def population(self):
         n, cap = C.c_uint32(0), C.c_uint32(0)
         check(self._lib.tecs_sim_get_population(self._h, C.byref(n), C.byref(cap)))
