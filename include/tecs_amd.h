@@ -7,6 +7,7 @@
  *                                                    include/pair_finders/naive_all_pairs.hpp:32-104
  *   binned / naive Gabriel finders           replace include/pair_finders/binned_gabriel.hpp:71-362,
  *                                                    include/pair_finders/naive_gabriel.hpp:36-120
+ *   NaiveDelaunay                            replaces include/pair_finders/naive_delaunay.hpp:157-293
  *   symmetric finders (each unordered pair   replace include/pair_finders/binned_all_pairs_symmetric.hpp:121-220,
  *     evaluated once, j < i)                         include/pair_finders/naive_all_pairs_symmetric.hpp:82-155,
  *                                                    include/pair_finders/binned_gabriel_symmetric.hpp:128-306,
@@ -62,7 +63,11 @@ enum {
   TECS_NAIVE_ALL_PAIRS_SYMMETRIC = 4,  /* pair_finders::NaiveAllPairsSymmetric  */
   TECS_BINNED_ALL_PAIRS_SYMMETRIC = 5, /* pair_finders::BinnedAllPairsSymmetric */
   TECS_NAIVE_GABRIEL_SYMMETRIC = 6,    /* pair_finders::NaiveGabrielSymmetric   */
-  TECS_BINNED_GABRIEL_SYMMETRIC = 7    /* pair_finders::BinnedGabrielSymmetric  */
+  TECS_BINNED_GABRIEL_SYMMETRIC = 7,   /* pair_finders::BinnedGabrielSymmetric  */
+  /* pair_finders::NaiveDelaunay (naive_delaunay.hpp:157-293, 3-D): the functor runs for (i, j)
+   * once per third cell k for which some fourth cell l > k spans a circumsphere through i, j,
+   * k, l with no other cell inside or on it. O(N^5): single simulations of at most 512 cells. */
+  TECS_NAIVE_DELAUNAY = 8
 };
 enum { TECS_EULER = 0, TECS_HEUN = 1 };                        /* integrators::Euler / Heun */
 enum { TECS_NO_COM_FIXER = 0, TECS_GLOBAL_COM_FIXER = 1 };     /* com_fixers::*             */

@@ -117,6 +117,11 @@ struct NaiveGabrielSymmetric {
   static constexpr int32_t kind = TECS_NAIVE_GABRIEL_SYMMETRIC;
   using Settings = NaiveGabriel::Settings;
 };
+// naive_delaunay.hpp:13-16: no settings; at most 512 cells (the capacity)
+struct NaiveDelaunay {
+  static constexpr int32_t kind = TECS_NAIVE_DELAUNAY;
+  using Settings = NaiveAllPairs::Settings;
+};
 }  // namespace pair_finders
 
 namespace integrators {

@@ -186,7 +186,7 @@ class OracleSim:
         the sum of absolute pair contributions. A symmetric pair set (the symmetric finders)
         applies each pair's value v to i and -v to j; by default the finder decides."""
         if symmetric is None:
-            symmetric = int(self.desc.pair_finder) >= abi.NAIVE_ALL_PAIRS_SYMMETRIC
+            symmetric = int(self.desc.pair_finder) in abi.SYMMETRIC_FINDERS
         rp = np.ascontiguousarray(row_ptr, np.int64)
         cc = np.ascontiguousarray(cols, np.int32)
         if cc.size == 0:

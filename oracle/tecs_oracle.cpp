@@ -11,6 +11,7 @@
 //   include/pair_finders/naive_all_pairs.hpp  NaiveAllPairs::evaluate_force       (eval_naive_all_pairs)
 //   include/pair_finders/binned_gabriel.hpp   BinnedGabriel::evaluate_force       (eval_binned_gabriel)
 //   include/pair_finders/naive_gabriel.hpp    NaiveGabriel::evaluate_force        (eval_naive_gabriel)
+//   include/pair_finders/naive_delaunay.hpp   NaiveDelaunay, 3-D                  (naive_delaunay_cell)
 //   include/pair_finders/*_symmetric.hpp      the four symmetric finders          (the same walks
 //                                             with j < i, then evaluate_symmetric)
 //   include/forces/predefined.hpp:107-144     PiecewiseLinear                     (piecewise_linear)
@@ -465,12 +466,101 @@ void naive_gabriel_cell(const Ctx& x, unsigned i, uint64_t& local_pairs,
   if (!row) commit(x, i, acc, total_drag, total_vel);
 }
 
+// NaiveDelaunay::evaluate_force, 3-D branch (naive_delaunay.hpp:157-293), the work items
+// (i, j, k) of cell i in index order: j over the other cells, k over the cells other than i
+// and j (k shifted past min(i, j), then past max(i, j); a k that lands on agent_count does
+// nothing). An item calls the functor for (i, j) when d(i, j) < r and some l > k, l not in
+// {i, j, k}, spans a sphere that no other cell lies in or on (a box prefilter, then
+// d^2 <= r^2). Each item stops at its first such l. The sphere is the reference's: Cramer's
+// rule on the edges e = p - p_i with right-hand sides |p|^2 - |p_i|^2 (|denominator| <= eps
+// skips l) gives the circumcentre C itself, to which p_i is then added, and the radius^2 is
+// |C|^2; for p_i != 0 this is not the circumsphere and the pairs are not the Delaunay graph.
+void naive_delaunay_cell(const Ctx& x, unsigned i, uint64_t& local_pairs, std::vector<int>* row) {
+  const Sim& s = *x.s;
+  const unsigned n = s.n;
+  const float* P = x.pos;
+  auto pt = [&](unsigned a) { return &P[3 * (size_t)a]; };
+  auto len2v = [](const float* v) { return dot3(v, v); };
+  Acc acc{};
+  float total_drag = 0.0f;
+  float total_vel[3] = {0.0f, 0.0f, 0.0f};
+  if (n < 2) {
+    if (!row) commit(x, i, acc, total_drag, total_vel);
+    return;
+  }
+  for (unsigned jj = 0; jj + 1 < n; ++jj) {
+    const unsigned j = jj + (jj >= i ? 1u : 0u);
+    const float* pi = pt(i);
+    const float* pj = pt(j);
+    float disp[3];
+    for (int c = 0; c < 3; ++c) disp[c] = pi[c] - pj[c];
+    const float d2 = dot3(disp, disp);
+    if (d2 >= s.cutoff2) continue;
+    const unsigned lo = i < j ? i : j, hi = i < j ? j : i;
+    for (unsigned kk = 0; kk < n; ++kk) {
+      unsigned k = kk;
+      k += k >= lo ? 1u : 0u;
+      k += k >= hi ? 1u : 0u;
+      if (k >= n) continue;
+      const float* pk = pt(k);
+      bool found = false;
+      for (unsigned l = 0; l < n && !found; ++l) {
+        if (l == i || l == j || l == k || l <= k) continue;
+        const float* pl = pt(l);
+        float e1[3], e2[3], e3[3];
+        for (int c = 0; c < 3; ++c) {
+          e1[c] = pj[c] - pi[c];
+          e2[c] = pk[c] - pi[c];
+          e3[c] = pl[c] - pi[c];
+        }
+        const float den = 2.0f * (e1[0] * (e2[1] * e3[2] - e2[2] * e3[1]) -
+                                  e1[1] * (e2[0] * e3[2] - e2[2] * e3[0]) +
+                                  e1[2] * (e2[0] * e3[1] - e2[1] * e3[0]));
+        if (std::fabs(den) <= FLT_EPSILON) continue;
+        const float b1 = len2v(pj) - len2v(pi);
+        const float b2 = len2v(pk) - len2v(pi);
+        const float b3 = len2v(pl) - len2v(pi);
+        const float cc[3] = {
+            pi[0] + (b1 * (e2[1] * e3[2] - e2[2] * e3[1]) - e1[1] * (b2 * e3[2] - e2[2] * b3) +
+                     e1[2] * (b2 * e3[1] - e2[1] * b3)) / den,
+            pi[1] + (e1[0] * (b2 * e3[2] - e2[2] * b3) - b1 * (e2[0] * e3[2] - e2[2] * e3[0]) +
+                     e1[2] * (e2[0] * b3 - b2 * e3[0])) / den,
+            pi[2] + (e1[0] * (e2[1] * b3 - b2 * e3[1]) - e1[1] * (e2[0] * b3 - b2 * e3[0]) +
+                     b1 * (e2[0] * e3[1] - e2[1] * e3[0])) / den};
+        const float rad2 = dist2(pi, cc);
+        const float rad = std::sqrt(rad2);
+        bool valid = true;
+        for (unsigned m = 0; m < n && valid; ++m) {
+          if (m == i || m == j || m == k || m == l) continue;
+          const float* pm = pt(m);
+          if (std::fabs(pm[0] - cc[0]) > rad || std::fabs(pm[1] - cc[1]) > rad ||
+              std::fabs(pm[2] - cc[2]) > rad)
+            continue;
+          if (dist2(pm, cc) <= rad2) valid = false;
+        }
+        found = valid;
+      }
+      if (!found) continue;
+      // each work item evaluates the functor on a zeroed pack and adds that pack to out(i)
+      // (naive_delaunay.hpp:258-275); the items' adds are atomics, here in (j, k) order
+      Acc v{};
+      visit_pair(x, i, j, disp, d2, v, total_drag, total_vel, local_pairs, row);
+      for (int c = 0; c < 3; ++c) acc.pos[c] += v.pos[c];
+      for (int c = 0; c < 2; ++c) acc.pol[c] += v.pol[c];
+    }
+  }
+  if (!row) commit(x, i, acc, total_drag, total_vel);
+}
+
 bool finder_is_binned(const Sim& s) {
   const int f = s.desc.pair_finder;
   return f == TECS_BINNED_ALL_PAIRS || f == TECS_BINNED_GABRIEL ||
          f == TECS_BINNED_ALL_PAIRS_SYMMETRIC || f == TECS_BINNED_GABRIEL_SYMMETRIC;
 }
-bool finder_is_symmetric(const Sim& s) { return s.desc.pair_finder >= TECS_NAIVE_ALL_PAIRS_SYMMETRIC; }
+bool finder_is_symmetric(const Sim& s) {
+  return s.desc.pair_finder >= TECS_NAIVE_ALL_PAIRS_SYMMETRIC &&
+         s.desc.pair_finder <= TECS_BINNED_GABRIEL_SYMMETRIC;
+}
 
 // The rebuild schedule of the binned finders, binned_all_pairs.hpp:77-84.
 void maybe_rebuild(Sim& s, const float* pos, bool is_full_step) {
@@ -554,6 +644,7 @@ int evaluate(Sim& s, const float* pos, const float* pol, float* out_pos, float* 
     uint64_t lp = 0;
     std::vector<int>* row = rows ? &(*rows)[i] : nullptr;
     switch (finder) {
+      case TECS_NAIVE_DELAUNAY: naive_delaunay_cell(x, i, lp, row); break;
       case TECS_BINNED_ALL_PAIRS: binned_all_pairs_cell(x, i, lp, row, sym); break;
       case TECS_NAIVE_ALL_PAIRS: naive_all_pairs_cell(x, i, lp, row, sym); break;
       case TECS_BINNED_GABRIEL: binned_gabriel_cell(x, i, lp, row, sym); break;
@@ -654,7 +745,7 @@ int step_heun(Sim& s, double dt) {
 int validate(const tecs_sim_desc* d) {
   if (!d) return fail(TECS_ERR_INVALID, "null descriptor");
   if (d->capacity < d->agent_count) return fail(TECS_ERR_INVALID, "capacity < agent_count");
-  if (d->pair_finder < 0 || d->pair_finder > TECS_BINNED_GABRIEL_SYMMETRIC)
+  if (d->pair_finder < 0 || d->pair_finder > TECS_NAIVE_DELAUNAY)
     return fail(TECS_ERR_INVALID, "bad pair_finder");
   if (d->integrator != TECS_EULER && d->integrator != TECS_HEUN)
     return fail(TECS_ERR_INVALID, "bad integrator");

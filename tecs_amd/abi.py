@@ -36,6 +36,8 @@ FINDER_NAMES = {
     NAIVE_GABRIEL_SYMMETRIC: "NaiveGabrielSymmetric",
     BINNED_GABRIEL_SYMMETRIC: "BinnedGabrielSymmetric",
 }
+NAIVE_DELAUNAY = 8
+FINDER_NAMES[NAIVE_DELAUNAY] = "NaiveDelaunay"
 SYMMETRIC_FINDERS = (NAIVE_ALL_PAIRS_SYMMETRIC, BINNED_ALL_PAIRS_SYMMETRIC, NAIVE_GABRIEL_SYMMETRIC,
                      BINNED_GABRIEL_SYMMETRIC)
 BINNED_FINDERS = (BINNED_ALL_PAIRS, BINNED_GABRIEL, BINNED_ALL_PAIRS_SYMMETRIC, BINNED_GABRIEL_SYMMETRIC)

@@ -67,8 +67,10 @@ inline int validate_desc(const tecs_sim_desc* d) {
   if (!d) return fail(TECS_ERR_INVALID, "null descriptor");
   if (d->capacity < d->agent_count) return fail(TECS_ERR_INVALID, "capacity < agent_count");
   if (d->agent_count > (1u << 30)) return fail(TECS_ERR_INVALID, "agent_count too large");
-  if (d->pair_finder < 0 || d->pair_finder > kBinnedGabrielSym)
+  if (d->pair_finder < 0 || d->pair_finder > kNaiveDelaunay)
     return fail(TECS_ERR_INVALID, "bad pair_finder");
+  if (d->pair_finder == kNaiveDelaunay && d->capacity > (uint32_t)kDelaunayMaxCells)
+    return fail(TECS_ERR_INVALID, "NaiveDelaunay takes at most 512 cells (capacity)");
   if (d->integrator != TECS_EULER && d->integrator != TECS_HEUN)
     return fail(TECS_ERR_INVALID, "bad integrator");
   if (d->com_fixer != TECS_NO_COM_FIXER && d->com_fixer != TECS_GLOBAL_COM_FIXER)
@@ -204,6 +206,8 @@ struct Batch {
   int64_t* d_row_ptr = nullptr;
   int32_t* d_cols = nullptr;
   int64_t cols_cap = 0;
+  uint32_t* dl_cnt = nullptr;  // NaiveDelaunay: counters of the work items found (SweepArgs)
+  int dl_row = 0, dl_splits = 1;
   DevicePool* pool = nullptr;                          // optional source of device memory
   std::vector<std::pair<void*, size_t>> allocations;  // (block, size class; 0 = hipMalloc)
 
@@ -473,7 +477,7 @@ struct Batch {
       TRY(alloc(&xp, S));
       if (has_pol) TRY(alloc(&polp, S));
     }
-    if (integrator == TECS_HEUN && (com_fixer == TECS_GLOBAL_COM_FIXER || is_symmetric(finder))) {
+    if (integrator == TECS_HEUN && (com_fixer == TECS_GLOBAL_COM_FIXER || delta_only(finder))) {
       TRY(alloc(&d3, S));
       if (has_pol) TRY(alloc(&d3p, S));
     }
@@ -519,6 +523,14 @@ struct Batch {
       if (h_inst[k].search_radius != 1) team_ok = false;
       const float c = ds[k].pair_finder_settings.gabriel_coefficient;
       if (is_gabriel(finder) && !(c >= 0.0f && c <= 0.95f)) team_ok = false;
+    }
+    if (finder == kNaiveDelaunay) {
+      // a row of counters per slot of the largest member; the cell pass leaves them at zero
+      for (int k = 0; k < count; ++k) dl_row = std::max(dl_row, h_inst[k].padded);
+      dl_splits = std::max(1, dl_row / 32);
+      const size_t cnt_len = (size_t)n_inst * dl_row * dl_row;
+      TRY(alloc(&dl_cnt, cnt_len));
+      HIP_TRY(hipMemsetAsync(dl_cnt, 0, sizeof(uint32_t) * cnt_len, stream));
     }
     TRY(alloc(&block_pairs, (size_t)n_blocks * kMaxTeamLanes));
     TRY(alloc(&tune_pairs, (size_t)n_blocks * kMaxTeamLanes));
@@ -704,6 +716,10 @@ struct Batch {
     a.off = fill_bins ? off : coff[ic];
     a.id = id[ic];
     a.block_pairs = block_pairs;
+    a.dl_cnt = dl_cnt;
+    a.dl_row = dl_row;
+    a.dl_splits = dl_splits;
+    a.n_inst = n_inst;
     return a;
   }
 
@@ -800,7 +816,7 @@ struct Batch {
     t.o_pol = d1p;
     t.block_pairs = tune_pairs;
     // symmetric finders: a Delta sweep as on a fresh grid (no movers, so nothing goes to q)
-    if (is_symmetric(finder)) {
+    if (delta_only(finder)) {
     } else if (!com_on && integrator == TECS_HEUN) {
       mode = kModePred;
       t.o2_pos = xp;
@@ -856,7 +872,7 @@ struct Batch {
   // symmetric ones a pass computes them from F (+ q, the movers' contributions, when the
   // evaluation saw a stale grid: with_q), and the update adds q and clears it.
   int com_and_update(int kind, const float4* F = nullptr, bool with_q = false) {
-    const bool sym = is_symmetric(finder);
+    const bool sym = delta_only(finder);
     if (com_fixer == TECS_GLOBAL_COM_FIXER) {
       if (sym) launch_com_partials(d_inst, d_block_inst, F, with_q ? q : nullptr, com_partial, n_blocks, stream);
       launch_com_finalize(d_inst, n_inst, com_partial, sym ? 1 : partials_per_block(), com, stream);
@@ -892,10 +908,10 @@ struct Batch {
     a->q_pol = qp;
   }
 
-  // A step with a symmetric finder: each evaluation writes its deltas (the movers' part into
-  // q on a stale grid), then the COM fix and the update run as their own kernels.
-  // Heun: evaluate -> predictor -> evaluate (stale grid) -> corrector.
-  int step_symmetric() {
+  // A step with a symmetric finder or NaiveDelaunay: each evaluation writes its deltas (the
+  // movers' part into q on a stale grid), then the COM fix and the update run as their own
+  // kernels. Heun: evaluate -> predictor -> evaluate (stale grid) -> corrector.
+  int step_delta_only() {
     const bool binned = is_binned(finder);
     const bool stale1 = binned && !rebuilt_now;
     SweepArgs a = base_args(x[xc], pol[xc]);
@@ -921,7 +937,7 @@ struct Batch {
   // One integrator step (Euler::integrate euler.hpp:81-85, Heun::integrate heun.hpp:201-208).
   int step_once() {
     TRY(maybe_rebuild(true));
-    if (is_symmetric(finder)) return step_symmetric();
+    if (delta_only(finder)) return step_delta_only();
     const bool com_on = com_fixer == TECS_GLOBAL_COM_FIXER;
     if (integrator == TECS_HEUN) {
       if (!com_on) {
@@ -1486,7 +1502,7 @@ struct Batch {
     size_t per_slot = 2 * 16 + 2 * 4 + 16 + 16 * 2 /* x, id, d1, ov */ + 8 * 4 /* keys.. */;
     if (pol) per_slot += 2 * 8 + 8 + 2 * 16 /* pol, d1 pol, trig */;
     if (d.integrator == TECS_HEUN) per_slot += 16 + (pol ? 8 : 0);
-    if (d.integrator == TECS_HEUN && (d.com_fixer == TECS_GLOBAL_COM_FIXER || is_symmetric(d.pair_finder)))
+    if (d.integrator == TECS_HEUN && (d.com_fixer == TECS_GLOBAL_COM_FIXER || delta_only(d.pair_finder)))
       per_slot += 16 + (pol ? 8 : 0);
     if (is_symmetric(d.pair_finder) && is_binned(d.pair_finder)) per_slot += 16 + (pol ? 8 : 0);
     size_t bins = 0;
@@ -1497,7 +1513,9 @@ struct Batch {
       for (int k = 0; k < 3; ++k) b *= std::ceil((double)((s.max_bounds[k] - s.min_bounds[k]) / bin));
       bins = (size_t)b + 64;
     }
-    return slots * per_slot + bins * 28 + 2 * 4096;
+    size_t dl = 0;  // NaiveDelaunay counters, one per (cell, partner) slot pair
+    if (d.pair_finder == kNaiveDelaunay) dl = (slots - kBlock) * (slots - kBlock) * 4;
+    return slots * per_slot + bins * 28 + dl + 2 * 4096;
   }
 
   int get_stats(tecs_stats* out) {

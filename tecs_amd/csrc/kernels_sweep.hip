@@ -198,30 +198,6 @@ __device__ __forceinline__ void walk_binned_deferred(const SweepArgs& a, const I
   }
 }
 
-// Deterministic block sum of one float per thread (fixed butterfly + fixed wave order).
-__device__ __forceinline__ float block_sum_f(float v, float* lds) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = v + __shfl_xor(v, o, 64);
-  const int w = threadIdx.x >> 6;
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) lds[w] = v;
-  __syncthreads();
-  float r = 0.0f;
-  if (threadIdx.x == 0) r = ((lds[0] + lds[1]) + lds[2]) + lds[3];
-  return r;
-}
-__device__ __forceinline__ unsigned block_sum_u(unsigned v, unsigned* lds) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  const int w = threadIdx.x >> 6;
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) lds[w] = v;
-  __syncthreads();
-  unsigned r = 0;
-  if (threadIdx.x == 0) r = lds[0] + lds[1] + lds[2] + lds[3];
-  return r;
-}
-
 template <bool HAS_POL>
 __device__ __forceinline__ CorrIn load_corr(const SweepArgs& a, int s) {
   CorrIn c{};
@@ -821,6 +797,7 @@ int launch_sweep(int finder, bool has_type, bool has_pol, int mode, const SweepA
     case kBinnedGabrielSym: return launch_mode_sym<true, true>(mode, has_type, has_pol, a, n_blocks, stream);
     case kNaiveAllPairsSym: return launch_mode_sym<false, false>(mode, has_type, has_pol, a, n_blocks, stream);
     case kNaiveGabrielSym: return launch_mode_sym<true, false>(mode, has_type, has_pol, a, n_blocks, stream);
+    case kNaiveDelaunay: return launch_delaunay(has_type, has_pol, mode, a, n_blocks, stream);
     default: return -1;
   }
 }

@@ -321,4 +321,28 @@ __device__ __forceinline__ void apply_force(const ForceParams& f, const float* d
   }
 }
 
+// Deterministic block sum of one float per thread (fixed butterfly + fixed wave order).
+__device__ __forceinline__ float block_sum_f(float v, float* lds) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = v + __shfl_xor(v, o, 64);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) lds[w] = v;
+  __syncthreads();
+  float r = 0.0f;
+  if (threadIdx.x == 0) r = ((lds[0] + lds[1]) + lds[2]) + lds[3];
+  return r;
+}
+__device__ __forceinline__ unsigned block_sum_u(unsigned v, unsigned* lds) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) lds[w] = v;
+  __syncthreads();
+  unsigned r = 0;
+  if (threadIdx.x == 0) r = lds[0] + lds[1] + lds[2] + lds[3];
+  return r;
+}
+
 }  // namespace tecs_amd

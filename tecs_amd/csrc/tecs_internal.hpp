@@ -16,9 +16,15 @@ enum Finder {
   kNaiveAllPairsSym = 4,  // the symmetric finders: partners j < i, out(i) += v, out(j) -= v
   kBinnedAllPairsSym = 5,
   kNaiveGabrielSym = 6,
-  kBinnedGabrielSym = 7
+  kBinnedGabrielSym = 7,
+  kNaiveDelaunay = 8  // work items (i, j, k), functor once per item with an empty sphere
 };
+// NaiveDelaunay keeps every position of an instance in LDS: capacity at most this
+constexpr int kDelaunayMaxCells = 512;
 inline bool is_symmetric(int finder) { return finder >= kNaiveAllPairsSym && finder <= kBinnedGabrielSym; }
+// finders whose force evaluation only writes deltas (Delta mode); the COM fix and the
+// integrator update run as kernels of their own
+inline bool delta_only(int finder) { return is_symmetric(finder) || finder == kNaiveDelaunay; }
 inline bool is_binned(int finder) {
   return finder == kBinnedAllPairs || finder == kBinnedGabriel || finder == kBinnedAllPairsSym ||
          finder == kBinnedGabrielSym;
@@ -66,13 +72,23 @@ struct SweepArgs {
   int* row_count;        // Count
   const int64_t* row_ptr;  // Emit
   int32_t* cols;           // Emit
+  // NaiveDelaunay: the work items found per (cell, partner), row (ii * dl_row + i) of dl_row
+  // counters per instance ii and cell i; zero between evaluations (the cell pass clears them)
+  uint32_t* dl_cnt;
+  int dl_row;
+  int dl_splits;  // workgroups per cell of the counting pass
+  int n_inst;
 };
 
-// Returns nonzero for a mode the finder does not have (symmetric finders: Delta, Count, Emit;
-// their Delta writes o_pos / o_pol and, on a stale grid (skey set), adds the contributions of
-// cells outside their rebuild bin to q_pos / q_pol, which must hold zeros).
+// Returns nonzero for a mode the finder does not have (symmetric finders and NaiveDelaunay:
+// Delta, Count, Emit). A symmetric Delta writes o_pos / o_pol and, on a stale grid (skey set),
+// adds the contributions of cells outside their rebuild bin to q_pos / q_pol, which must hold
+// zeros.
 int launch_sweep(int finder, bool has_type, bool has_pol, int mode, const SweepArgs& a,
                  int n_blocks, hipStream_t stream);
+// NaiveDelaunay (kernels_delaunay.hip): the counting pass, then the per-cell pass of the mode
+int launch_delaunay(bool has_type, bool has_pol, int mode, const SweepArgs& a, int n_blocks,
+                    hipStream_t stream);
 // d += q, q = 0 over all slots (dp / qp may be null)
 void launch_fold_q(float4* d, float2* dp, float4* q, float2* qp, int total_slots,
                    hipStream_t stream);

@@ -1,4 +1,4 @@
-"""Seeded synthetic inputs and descriptors for the five benchmark workloads.
+"""Seeded synthetic inputs and descriptors for the benchmark workloads.
 
 Inputs come from numpy's Philox counter-based generator seeded with 2807 (the reference's
 default seed, include/simulation.hpp:89) plus a per-workload stream offset, computed in float64
@@ -15,6 +15,8 @@ examples/branching.cpp:48-55).
   sorting     N = 4,000,000 uniform in a cube at 1.5 cells per unit volume, types A/B p = 0.5;
               PiecewiseLinear(0.8, 2, tau) with tau_AA = tau_BB = 1, tau_AB = 0.3
               (examples/sorting.cpp:43-58); BinnedAllPairs; Heun dt 0.05
+  spheroid    N = 100 uniform in a ball of radius 3; PiecewiseLinear(0.5) x 3; NaiveDelaunay;
+              Heun dt 0.001 (examples/delaunay_spheroid.cpp)
   sweep       many independent sorting/relaxation instances of log-uniform size (batched)
 """
 from __future__ import annotations
@@ -125,6 +127,17 @@ def sorting(n: int = 4_000_000, seed_stream: int = 2, tau_ab: float = 0.3,
         adhesion_table=[[1.0, tau_ab], [tau_ab, 1.0]],
     )
     return Workload("sorting", d, pos, None, types, 0.05, 500)
+
+
+def spheroid(n: int = 100, seed_stream: int = 4, finder: int = abi.NAIVE_DELAUNAY) -> Workload:
+    """examples/delaunay_spheroid.cpp:19-53: n cells uniform in a ball of radius 3, r_max 1,
+    NaiveDelaunay, Heun dt 0.001 for 300 steps, PiecewiseLinear(0.5) (scales 1, 1,
+    forces/predefined.hpp:109-143) times 3, the factor folded into both scales."""
+    g = rng(seed_stream)
+    pos = ball(n, 3.0, g)
+    d = abi.make_desc(n, pair_finder=finder, integrator=abi.HEUN, homeostatic_radius=0.5,
+                      repulsion_scale=3.0, adhesion_scale=3.0)
+    return Workload("spheroid", d, pos, None, None, 0.001, 300)
 
 
 def sweep(count: int = 256, n_min: int = 1_000, n_max: int = 100_000, seed_stream: int = 3):

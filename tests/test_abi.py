@@ -109,7 +109,8 @@ def _create(desc):
 
 @pytest.mark.parametrize("mutate,needle", [
     (lambda d: setattr(d, "capacity", 1), "capacity"),
-    (lambda d: setattr(d, "pair_finder", 8), "pair_finder"),
+    (lambda d: setattr(d, "pair_finder", 9), "pair_finder"),
+    (lambda d: (setattr(d, "pair_finder", 8), setattr(d, "capacity", 513)), "512"),
     (lambda d: setattr(d, "pair_finder", -1), "pair_finder"),
     (lambda d: setattr(d, "integrator", 5), "integrator"),
     (lambda d: setattr(d, "com_fixer", 3), "com_fixer"),
