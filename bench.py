@@ -114,11 +114,13 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the streaming copy kernel used to calibrate HBM byte counters")
     ap.add_argument("--workload", default="epithelium",
-                    choices=["epithelium", "relaxation", "sorting", "sweep", "serve"],
+                    choices=["epithelium", "relaxation", "sorting", "sweep", "serve", "spheroid"],
                     help="epithelium (default): 200k cells, BinnedGabriel + polarity, Heun; "
                     "relaxation: 10k cells, BinnedAllPairs, Euler; sorting: 4M cells, "
                     "BinnedAllPairs + type table, Heun; sweep: 256 mixed-size instances, batched "
-                    "vs one simulation at a time")
+                    "vs one simulation at a time; spheroid: NaiveDelaunay spheroids of 100 cells, "
+                    "batched")
+    ap.add_argument("--instances", type=int, default=256, help="spheroid workload: batch size")
     ap.add_argument("--plain-steps", type=int, default=20,
                     help="sweep workload: steps per instance for the one-at-a-time comparison")
     ap.add_argument("--requests", type=int, default=48, help="serve workload: number of requests")
@@ -154,6 +156,8 @@ def main():
         return bench_sweep(args)
     if args.workload == "serve":
         return bench_serve(args)
+    if args.workload == "spheroid":
+        return bench_spheroid(args)
     sym = {"finder": abi.BINNED_GABRIEL_SYMMETRIC if args.workload == "epithelium"
            else abi.BINNED_ALL_PAIRS_SYMMETRIC} if args.symmetric else {}
     if args.workload == "relaxation":
@@ -299,6 +303,136 @@ def bench_single(args, w):
     if not args.no_cpu_baseline:
         print("bench: cpu baseline ...", file=sys.stderr, flush=True)
         cb = cpu_baseline(w, args.cpu_seconds)
+        out["cpu_baseline"] = cb
+        out["gpu_over_cpu"] = value / cb["value"]
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline_instances(ws, seconds_target: float):
+    """The oracle stepping the instances one after another (one step each, round robin) until
+    seconds_target has passed."""
+    import oracle
+
+    threads = int(os.environ.get("TECS_CPU_THREADS", min(16, os.cpu_count() or 1)))
+    oracle.set_threads(threads)
+    sims = []
+    pairs = 0
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        k = steps % len(ws)
+        if k == len(sims):
+            o = oracle.OracleSim(ws[k].desc)
+            o.upload(ws[k].positions, ws[k].polarities, ws[k].types)
+            sims.append(o)
+        before = sims[k].stats()["pair_interactions"]
+        sims[k].take_step(ws[k].dt, 1)
+        pairs += sims[k].stats()["pair_interactions"] - before
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds_target or steps >= 100000:
+            break
+    return {
+        "value": pairs / el,
+        "unit": "pair-interactions/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{steps} Heun steps over the first {len(sims)} instances of the same batch, "
+        f"from their initial state ({el:.1f} s, C++/OpenMP oracle, {threads} threads)",
+    }
+
+
+def bench_spheroid(args):
+    """NaiveDelaunay: `instances` spheroids of 100 cells (examples/delaunay_spheroid.cpp: a ball
+    of radius 3, r_max 1, PiecewiseLinear(0.5) x 3, Heun dt 0.001), stepped as one batch, and
+    the first 16 stepped one at a time for comparison. A launch of the timed sweep class is the
+    counting pass (k_dl_count) plus the per-cell pass (k_dl_cells); the roofline is VALU issue,
+    the bound of the counting pass (scalar FP32 predicate arithmetic over LDS)."""
+    from tecs_amd import GpuBatch, GpuSimulation, abi, workloads
+
+    B = args.instances
+    ws = [workloads.spheroid(100, seed_stream=100 + k) for k in range(B)]
+    print(f"bench: spheroid x{B} (NaiveDelaunay, 100 cells each)", file=sys.stderr, flush=True)
+    b = GpuBatch([w.desc for w in ws])
+    for k, w in enumerate(ws):
+        b.upload(k, w.positions)
+    dts = [w.dt for w in ws]
+    b.take_step(dts, args.warmup)
+    b.synchronize()
+    b.reset_stats()
+    t0 = time.perf_counter()
+    b.take_step(dts, args.steps)
+    b.synchronize()
+    el = time.perf_counter() - t0
+    pairs = b.stats()["pair_interactions"]
+    value = pairs / el
+    print(f"bench: batched {args.steps} steps in {el:.3f} s, {pairs} pair interactions",
+          file=sys.stderr, flush=True)
+    k_steps = min(max(args.steps // 4, 10), 200)
+    b.set_kernel_timing(True)
+    b.take_step(dts, k_steps)
+    b.synchronize()
+    full_ms, full_n = b.kernel_timing(abi.KT_SWEEP_FULL)
+    stale_ms, stale_n = b.kernel_timing(abi.KT_SWEEP_STALE)
+    b.set_kernel_timing(False)
+    avg_ms = (full_ms + stale_ms) / max(full_n + stale_n, 1)
+    plain_n = min(16, B)
+    plain_pairs = 0
+    plain_time = 0.0
+    for w in ws[:plain_n]:
+        s = GpuSimulation(w.desc)
+        s.upload(w.positions)
+        s.take_step(w.dt, 2)
+        s.synchronize()
+        s.reset_stats()
+        t1 = time.perf_counter()
+        s.take_step(w.dt, args.plain_steps)
+        s.synchronize()
+        plain_time += time.perf_counter() - t1
+        plain_pairs += s.stats()["pair_interactions"]
+        s.close()
+    plain = plain_pairs / plain_time
+    valu = measured_valu("spheroid", "thread", "count")
+    out = {
+        "metric": "pair-interactions/sec per Heun step, NaiveDelaunay batch",
+        "value": value,
+        "unit": "pair-interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * el / max(args.steps, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Philox, seed 2807)",
+        "config": {
+            "workload": f"Delaunay spheroids: {B} instances of 100 cells uniform in a ball of "
+            "radius 3, NaiveDelaunay r_max=1, PiecewiseLinear(0.5)x3, Heun dt=0.001",
+            "cells": 100 * B,
+            "instances": B,
+            "pair_interactions_per_step": pairs / max(args.steps, 1),
+            "plain_value": plain,
+            "batched_over_plain": value / plain,
+            "plain_instances": plain_n,
+            "plain_steps_per_instance": args.plain_steps,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "valu",
+            "kernel": "k_dl_count + k_dl_cells (NaiveDelaunay evaluation)",
+            "achieved": None if valu is None else valu / (avg_ms * 1e-3),
+            "peak": VALU_PEAK_WAVE_INSTR_PER_S,
+            "unit": "wave-instructions/s",
+            "frac": None if valu is None else (valu / (avg_ms * 1e-3)) / VALU_PEAK_WAVE_INSTR_PER_S,
+            "traffic": None,
+            "valu_instructions_per_launch": valu,
+            "avg_launch_us": avg_ms * 1e3,
+        },
+    }
+    if not args.no_cpu_baseline:
+        print("bench: cpu baseline ...", file=sys.stderr, flush=True)
+        cb = cpu_baseline_instances(ws, args.cpu_seconds)
         out["cpu_baseline"] = cb
         out["gpu_over_cpu"] = value / cb["value"]
     print(json.dumps(out), flush=True)

@@ -527,7 +527,7 @@ struct Batch {
     if (finder == kNaiveDelaunay) {
       // a row of counters per slot of the largest member; the cell pass leaves them at zero
       for (int k = 0; k < count; ++k) dl_row = std::max(dl_row, h_inst[k].padded);
-      dl_splits = std::max(1, dl_row / 32);
+      dl_splits = std::max(1, dl_row / 128);
       const size_t cnt_len = (size_t)n_inst * dl_row * dl_row;
       TRY(alloc(&dl_cnt, cnt_len));
       HIP_TRY(hipMemsetAsync(dl_cnt, 0, sizeof(uint32_t) * cnt_len, stream));

@@ -10,7 +10,8 @@
 //   k_dl_count: the items. Workgroup (instance, i, split) stages the instance's positions in
 //     LDS (at most kDelaunayMaxCells), lists the partners j within the cutoff in index order,
 //     and takes every dl_splits-th slice of kBlock items (j, k) of that list, one item per
-//     lane, consecutive lanes on consecutive k (similar l ranges). A lane runs the l and m loops
+//     lane, partner fastest (the lanes of a wavefront share a few consecutive k, so their l
+//     ranges nearly coincide). A lane runs the l and m loops
 //     serially over LDS, in the reference's FP32 operation order, and adds 1 to the (i, j)
 //     counter when the item finds an l. Counters are integers, so the result does not depend
 //     on which lane took which item.
@@ -25,17 +26,34 @@ namespace {
 
 constexpr float kFltEps = 1.1920928955078125e-07f;  // FLT_EPSILON
 
-// Whether item (i, j, k) finds an l (naive_delaunay.hpp:187-257): P holds the n positions.
-__device__ __forceinline__ bool dl_item_found(const float4* P, int n, int i, int j, int k) {
-  const float4 pi = P[i], pj = P[j], pk = P[k];
+// The cells of an instance as one workgroup stages them for cell i: sorted by squared
+// distance to p_i (ties by index), each with its index in .w, and each cell's place in that
+// order.
+struct NearOrder {
+  const float4* pos;  // pos[r]: (x, y, z, index as int bits) of the r-th nearest cell, r < n,
+                      // readable up to r = n + 2
+  const int* rank;    // rank[m]: place of cell m
+};
+
+// Whether item (i, j, k) finds an l (naive_delaunay.hpp:187-257). The item's result is
+// "some l > k (l not i, j) passes the denominator test and has no cell m (not i, j, k, l) that
+// passes the box prefilter and lies in or on its sphere", which does not depend on the order
+// in which l and m are visited; each per-(l, m) test is evaluated with the reference's FP32
+// operations. Both loops visit the cells nearest to p_i first: the sphere passes through p_i,
+// so a cell near p_i is the likeliest to lie inside and end the m loop early, and spheres
+// through near cells are the likeliest to be empty. The m loop is one predicate per cell and
+// one exit, so a wavefront's lanes stay converged until they leave.
+__device__ __forceinline__ bool dl_item_found(const NearOrder& o, int n, int i, int j, int k) {
+  const float4 pi = o.pos[o.rank[i]], pj = o.pos[o.rank[j]], pk = o.pos[o.rank[k]];
   const float e1[3] = {pj.x - pi.x, pj.y - pi.y, pj.z - pi.z};
   const float e2[3] = {pk.x - pi.x, pk.y - pi.y, pk.z - pi.z};
   const float li = len2(pi.x, pi.y, pi.z);
   const float b1 = len2(pj.x, pj.y, pj.z) - li;
   const float b2 = len2(pk.x, pk.y, pk.z) - li;
-  for (int l = k + 1; l < n; ++l) {
-    if (l == i || l == j) continue;
-    const float4 pl = P[l];
+  for (int rl = 0; rl < n; ++rl) {
+    const float4 pl = o.pos[rl];
+    const int l = __float_as_int(pl.w);
+    if (l <= k || l == i || l == j) continue;
     const float e3[3] = {pl.x - pi.x, pl.y - pi.y, pl.z - pi.z};
     // the three minors of the edge matrix that the denominator and the b1 terms share
     const float m0 = e2[1] * e3[2] - e2[2] * e3[1];
@@ -52,24 +70,34 @@ __device__ __forceinline__ bool dl_item_found(const float4* P, int n, int i, int
                              b1 * m2) / den;
     const float rad2 = len2(pi.x - c0, pi.y - c1, pi.z - c2);
     const float rad = sqrtf(rad2);
-    bool valid = true;
-    for (int m = 0; m < n; ++m) {
-      if (m == i || m == j || m == k || m == l) continue;
-      const float4 pm = P[m];
+    // two cells per iteration (the entry after the last one is padding, masked out), the
+    // next pair loaded while this one is tested
+    auto hit = [&](float4 pm) {
+      const int m = __float_as_int(pm.w);
       const float dx = pm.x - c0, dy = pm.y - c1, dz = pm.z - c2;
-      if (fabsf(dx) > rad || fabsf(dy) > rad || fabsf(dz) > rad) continue;
-      if (len2(dx, dy, dz) <= rad2) {
-        valid = false;
-        break;
-      }
+      const bool in_box = !(fabsf(dx) > rad || fabsf(dy) > rad || fabsf(dz) > rad);
+      const bool own = (m == i) | (m == j) | (m == k) | (m == l);
+      return !own & in_box & (len2(dx, dy, dz) <= rad2);
+    };
+    bool inside = false;
+    float4 qa = o.pos[0], qb = o.pos[1];
+    for (int rm = 0; rm < n && !inside; rm += 2) {
+      const float4 ca = qa, cb = qb;
+      qa = o.pos[rm + 2];
+      qb = o.pos[rm + 3];
+      inside = hit(ca) | ((rm + 1 < n) & hit(cb));
     }
-    if (valid) return true;
+    if (!inside) return true;
   }
   return false;
 }
 
 __global__ __launch_bounds__(kBlock) void k_dl_count(SweepArgs a) {
-  __shared__ float4 P[kDelaunayMaxCells];
+  // positions and indices, nearest to p_i first (+ 3 entries of padding read ahead by the m
+  // loop and never used)
+  __shared__ float4 P[kDelaunayMaxCells + 3];
+  __shared__ int Rk[kDelaunayMaxCells];    // place of each cell in that order
+  __shared__ float D[kDelaunayMaxCells];   // squared distance of each cell to p_i
   __shared__ int J[kDelaunayMaxCells];
   __shared__ int wave_n[kBlock / 64];
   const int S = a.dl_splits, R = a.dl_row;
@@ -81,22 +109,32 @@ __global__ __launch_bounds__(kBlock) void k_dl_count(SweepArgs a) {
   const int n = in.n;
   if (i >= n) return;  // the whole workgroup
   const float4* xs = a.xs + in.slot_begin;
-  for (int t = (int)threadIdx.x; t < n; t += kBlock) P[t] = xs[t];
+  const float4 pi = xs[i];
+  // d^2 of (i, m) as the reference's cutoff test forms it (naive_delaunay.hpp:177-182)
+  for (int m = (int)threadIdx.x; m < n; m += kBlock) {
+    const float4 pm = xs[m];
+    D[m] = len2(pi.x - pm.x, pi.y - pm.y, pi.z - pm.z);
+  }
   __syncthreads();
-  // partners within the cutoff (naive_delaunay.hpp:177-182), compacted in index order: the
-  // workgroups of one cell split the items by position in this list, so it must be the same
-  // list in each of them
-  const float4 pi = P[i];
+  for (int m = (int)threadIdx.x; m < n; m += kBlock) {
+    const float dm = D[m];
+    int r = 0;
+    for (int q = 0; q < n; ++q) {
+      const float dq = D[q];
+      r += (dq < dm) || (dq == dm && q < m);
+    }
+    Rk[m] = r;
+    const float4 pm = xs[m];
+    P[r] = make_float4(pm.x, pm.y, pm.z, __int_as_float(m));
+  }
+  // partners within the cutoff, compacted in index order: the workgroups of one cell split
+  // the items by position in this list, so it must be the same list in each of them
   const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
   int nj = 0;
   for (int base = 0; base + 1 < n; base += kBlock) {
     const int jj = base + (int)threadIdx.x;
     const int j = jj + (jj >= i);
-    bool in_range = false;
-    if (jj + 1 < n) {
-      const float4 pj = P[j];
-      in_range = len2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z) < in.cutoff2;
-    }
+    const bool in_range = jj + 1 < n && D[j] < in.cutoff2;
     const unsigned long long mask = __ballot(in_range);
     if (lane == 0) wave_n[wave] = __popcll(mask);
     __syncthreads();
@@ -107,17 +145,21 @@ __global__ __launch_bounds__(kBlock) void k_dl_count(SweepArgs a) {
     for (int q = 0; q < kBlock / 64; ++q) nj += wave_n[q];
     __syncthreads();
   }
+  __syncthreads();
+  const NearOrder o{P, Rk};
   const int items = nj * n;
   uint32_t* row = a.dl_cnt + ((size_t)ii * R + i) * R;
   for (int t = w * kBlock + (int)threadIdx.x; t < items; t += S * kBlock) {
-    const int j = J[t / n];
+    // partner fastest: the lanes of a wavefront share a few consecutive k, so their l ranges
+    // (l > k) nearly coincide
+    const int j = J[t % nj];
     // k skips i and j: shifted past the smaller, then past the larger (naive_delaunay.hpp:
     // 165-175); a k that lands on n has no l
-    int k = t % n;
+    int k = t / nj;
     k += k >= min(i, j);
     k += k >= max(i, j);
     if (k >= n) continue;
-    if (dl_item_found(P, n, i, j, k)) atomicAdd(row + j, 1u);
+    if (dl_item_found(o, n, i, j, k)) atomicAdd(row + j, 1u);
   }
 }
 
