@@ -56,6 +56,7 @@ static int host_checks() {
   EXPECT((std::is_same<pair_finders::BinnedAllPairsSymmetric::Settings,
                        pair_finders::BinnedAllPairs::Settings>::value));
   EXPECT(pair_finders::NaiveGabrielSymmetric::kind == TECS_NAIVE_GABRIEL_SYMMETRIC);
+  EXPECT(pair_finders::NaiveDelaunay::kind == TECS_NAIVE_DELAUNAY);
   EXPECT((std::is_same<DefaultSimulationConfig::pair_finder_t, pair_finders::NaiveAllPairs>::value));
   EXPECT((std::is_same<DefaultSimulationConfig::integrator_t, integrators::Heun>::value));
 
