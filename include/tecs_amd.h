@@ -202,6 +202,43 @@ int tecs_sim_random_in_bins(tecs_sim* sim, int32_t n, const float* points, int32
 int tecs_sim_random_neighbour(tecs_sim* sim, int32_t n, const float* points, float radius,
                               const float* uniforms, int32_t* out);
 
+/* ---- generic, link and update forces (Integrator::evaluate_force_impl, base.hpp:116-230) ---
+ * take_step evaluates, for every evaluation of a step, in the order fuse_forces groups the
+ * forces (kernel_fuser.hpp:148-156): the generic wall forces, then the pairwise force (the pair
+ * finder), then the link forces, all into the same deltas (and so into the COM fix and the
+ * integrator update). tecs_sim_evaluate_force and tecs_sim_pairs stay the pair finder's alone
+ * (PairFinder::evaluate_force). */
+#define TECS_MAX_WALLS 8
+/* A planar wall, the GENERIC_FORCE of examples/growth_w_wall.cpp:44-48 and
+ * growth_w_squeeze.cpp:37-52 (PlaneN, types/plane.hpp:17-60): the plane a x + b y + c z + d = 0
+ * is normalised as PlaneN's constructor does; a cell whose signed distance
+ * sd = d + a x + b y + c z is negative gets delta += normal * (-sd * strength), wall after wall. */
+typedef struct tecs_wall {
+  float plane[4]; /* a, b, c, d (any nonzero normal) */
+  float strength;
+} tecs_wall;
+/* Replace the walls (count <= TECS_MAX_WALLS; 0 removes them). */
+int tecs_sim_set_walls(tecs_sim* sim, const tecs_wall* walls, int32_t count);
+/* Replace the links (the Link views of Simulation, types/link.hpp; links: 2*count agent indices
+ * a, b) and the link force, the LINK_FORCE of examples/intercalation.cpp / sorting_prot.cpp:
+ * u = p_b - p_a, d = |u|, delta_a += (k u) / d, delta_b += (-k u) / d, each endpoint's
+ * contribution added atomically as base.hpp:193-230 does. Links with a == b, or with an
+ * endpoint at or beyond the active count when a step runs, are skipped. Endpoints must be below
+ * the capacity. count 0 removes them. */
+int tecs_sim_set_links(tecs_sim* sim, const int32_t* links, int64_t count, float strength);
+/* Cell division, the UPDATE_FUNC `proliferate` of examples/growth_w_wall.cpp:52-64 (run by
+ * Simulation::init, simulation.hpp:520-529): uniforms holds three draws in [0, 1) per active
+ * cell (the reference's rng.drand). Cell i divides when u0 <= rate; its daughter takes the next
+ * index (the reference's atomic counter orders daughters arbitrarily; here in the order of their
+ * parents) at p_i + offset * to_vector3(theta, phi), theta = float(acos(2 u1 - 1)),
+ * phi = float(u2 * 2 * pi) (polarity.hpp:111-118), sines and cosines within an ulp of the float
+ * functions. Writes the daughters' positions only (cells beyond the active count; their other
+ * fields keep what their index holds) and returns in *new_count the count to pass to
+ * tecs_sim_set_agent_count, as the example does with its counter. TECS_ERR_CAPACITY (nothing
+ * written) if the daughters do not fit in the capacity. Synchronous. */
+int tecs_sim_divide(tecs_sim* sim, float rate, float offset, const double* uniforms,
+                    uint32_t* new_count);
+
 /* n_steps integrator steps of size dt (Integrator::integrate, euler.hpp:81-85 / heun.hpp:201-208).
  * Asynchronous on the simulation's stream. */
 int tecs_sim_take_step(tecs_sim* sim, double dt, int32_t n_steps);
@@ -301,6 +338,12 @@ int tecs_batch_set_force(tecs_batch* batch, int32_t index, const tecs_force_desc
 int tecs_batch_upload_range(tecs_batch* batch, int32_t index, uint32_t first, uint32_t count,
                             const float* positions, const float* polarities, const uint8_t* types);
 int tecs_batch_set_agent_count(tecs_batch* batch, int32_t index, uint32_t agent_count);
+/* tecs_sim_set_walls / tecs_sim_set_links / tecs_sim_divide for member `index`. */
+int tecs_batch_set_walls(tecs_batch* batch, int32_t index, const tecs_wall* walls, int32_t count);
+int tecs_batch_set_links(tecs_batch* batch, int32_t index, const int32_t* links, int64_t count,
+                         float strength);
+int tecs_batch_divide(tecs_batch* batch, int32_t index, float rate, float offset,
+                      const double* uniforms, uint32_t* new_count);
 int tecs_batch_set_capacity(tecs_batch* batch, int32_t index, uint32_t capacity);
 int tecs_batch_synchronize(tecs_batch* batch);
 int tecs_batch_get_stats(tecs_batch* batch, tecs_stats* stats);

@@ -67,6 +67,9 @@ def load():
         "orc_set_agent_count": (C.c_int, [P, C.c_uint]),
         "orc_set_capacity": (C.c_int, [P, C.c_uint]),
         "orc_query": (C.c_int, [P, C.c_int, C.c_int, P, C.c_int, C.c_float, P, P]),
+        "orc_set_walls": (C.c_int, [P, P, C.c_int]),
+        "orc_set_links": (C.c_int, [P, P, C.c_int64, C.c_float]),
+        "orc_divide": (C.c_int, [P, C.c_float, C.c_float, P, C.POINTER(C.c_uint)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -140,6 +143,21 @@ class OracleSim:
 
     def set_capacity(self, capacity: int):
         _check(self._lib.orc_set_capacity(self._h, int(capacity)))
+
+    def set_walls(self, walls):
+        """walls: rows (a, b, c, d, strength)."""
+        w = np.ascontiguousarray(walls, np.float32).reshape(-1, 5)
+        _check(self._lib.orc_set_walls(self._h, _ptr(w) if w.size else None, w.shape[0]))
+
+    def set_links(self, links, strength: float):
+        lk = np.ascontiguousarray(links, np.int32).reshape(-1, 2)
+        _check(self._lib.orc_set_links(self._h, _ptr(lk) if lk.size else None, lk.shape[0], float(strength)))
+
+    def divide(self, rate: float, offset: float, uniforms) -> int:
+        u = np.ascontiguousarray(uniforms, np.float64).reshape(self.n, 3)
+        out = C.c_uint(0)
+        _check(self._lib.orc_divide(self._h, float(rate), float(offset), _ptr(u), C.byref(out)))
+        return int(out.value)
 
     def _query(self, kind, points, radius_bins, radius, uniforms):
         p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)

@@ -40,6 +40,7 @@
 #include "../include/tecs_amd.h"
 
 #include <algorithm>
+#include <array>
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -250,6 +251,10 @@ struct Sim {
   std::vector<float> pos, pol;
   std::vector<uint8_t> type;
   std::vector<float> d1pos, d1pol, x2pos, x2pol, d3pos, d3pol, old_vel;
+  // generic wall forces (normalised plane a, b, c, d and strength) and link forces
+  std::vector<std::array<float, 5>> walls;
+  std::vector<int32_t> links;  // a0, b0, a1, b1, ...
+  float link_strength = 0.0f;
   tecs_stats stats{};
 };
 
@@ -674,8 +679,56 @@ void com_fix(const Sim& s, const std::vector<float>& dpos, float* com) {
 }
 
 // Euler::integrate + apply_euler, euler.hpp:49-85.
+// The GENERIC_FORCE of examples/growth_w_wall.cpp:44-48 (and growth_w_squeeze.cpp:37-52) for
+// every wall: sd = PlaneN::signed_distance(p) (types/plane.hpp:36-42: d + a x + b y + c z in
+// that order); sd < 0 adds normal * (-sd * k) to the cell's delta, wall after wall.
+void wall_forces(const Sim& s, const float* pos, float* out) {
+  if (s.walls.empty()) return;
+  const long n = (long)s.n;
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < n; ++i) {
+    const float* p = &pos[3 * i];
+    for (const auto& w : s.walls) {
+      float sd = w[3];
+      for (int c = 0; c < 3; ++c) sd += w[c] * p[c];
+      if (sd < 0.0f)
+        for (int c = 0; c < 3; ++c) out[3 * i + c] += w[c] * (-sd * w[4]);
+    }
+  }
+}
+
+// The LINK_FORCE of examples/intercalation.cpp and sorting_prot.cpp, applied as base.hpp:193-230
+// applies link forces (a zeroed pack per endpoint, atomically added to its delta; here in link
+// order): u = p_b - p_a, d = |u|, delta_a += (k * u) / d, delta_b += (-k * u) / d. Links with
+// a == b are skipped (base.hpp:207-208), and so are links with an endpoint beyond the active
+// count (their stage positions are not maintained).
+void link_forces(const Sim& s, const float* pos, float* out) {
+  const size_t nl = s.links.size() / 2;
+  const float k = s.link_strength;
+  for (size_t q = 0; q < nl; ++q) {
+    const unsigned a = (unsigned)s.links[2 * q], b = (unsigned)s.links[2 * q + 1];
+    if (a == b || a >= s.n || b >= s.n) continue;
+    float u[3];
+    for (int c = 0; c < 3; ++c) u[c] = pos[3 * (size_t)b + c] - pos[3 * (size_t)a + c];
+    const float d = std::sqrt(dot3(u, u));
+    for (int c = 0; c < 3; ++c) out[3 * (size_t)a + c] += 0.0f + (k * u[c]) / d;
+    for (int c = 0; c < 3; ++c) out[3 * (size_t)b + c] += 0.0f + (-k * u[c]) / d;
+  }
+}
+
+// Simulation::take_step's forces in the order fuse_forces groups them (kernel_fuser.hpp:
+// 148-156): generic (walls), pairwise (the pair finder), link.
+int step_forces(Sim& s, const float* pos, const float* pol, float* out_pos, float* out_pol,
+                bool is_full_step) {
+  wall_forces(s, pos, out_pos);
+  const int rc = evaluate(s, pos, pol, out_pos, out_pol, is_full_step, nullptr);
+  if (rc) return rc;
+  link_forces(s, pos, out_pos);
+  return TECS_OK;
+}
+
 int step_euler(Sim& s, double dt) {
-  int rc = evaluate(s, s.pos.data(), s.pol.data(), s.d1pos.data(), s.d1pol.data(), true, nullptr);
+  int rc = step_forces(s, s.pos.data(), s.pol.data(), s.d1pos.data(), s.d1pol.data(), true);
   if (rc) return rc;
   float com[3];
   com_fix(s, s.d1pos, com);
@@ -704,7 +757,7 @@ int step_euler(Sim& s, double dt) {
 int step_heun(Sim& s, double dt) {
   const float dtf = (float)dt;
   const long n = (long)s.n;
-  int rc = evaluate(s, s.pos.data(), s.pol.data(), s.d1pos.data(), s.d1pol.data(), true, nullptr);
+  int rc = step_forces(s, s.pos.data(), s.pol.data(), s.d1pos.data(), s.d1pol.data(), true);
   if (rc) return rc;
   float com[3];
   com_fix(s, s.d1pos, com);
@@ -717,7 +770,7 @@ int step_heun(Sim& s, double dt) {
       for (int c = 0; c < 2; ++c)
         s.x2pol[2 * i + c] = s.pol[2 * i + c] + s.d1pol[2 * i + c] * dtf;
   }
-  rc = evaluate(s, s.x2pos.data(), s.x2pol.data(), s.d3pos.data(), s.d3pol.data(), false, nullptr);
+  rc = step_forces(s, s.x2pos.data(), s.x2pol.data(), s.d3pos.data(), s.d3pol.data(), false);
   if (rc) return rc;
   com_fix(s, s.d3pos, com);
   // apply_heun_corrector, heun.hpp:170-186
@@ -1017,6 +1070,64 @@ int orc_set_capacity(orc_sim* h, unsigned cap) {
   s.type.resize(c, 0);
   s.cap = cap;
   s.desc.capacity = cap;
+  return TECS_OK;
+}
+
+// Walls for the generic force: five floats each (a, b, c, d, strength); the plane is
+// normalised as PlaneN's constructor does (types/plane.hpp:24-31): inv = 1 / sqrt(a^2 + b^2 +
+// c^2), every coefficient times inv.
+int orc_set_walls(orc_sim* h, const float* walls, int count) {
+  Sim& s = *reinterpret_cast<Sim*>(h);
+  if (count < 0 || count > TECS_MAX_WALLS) return fail(TECS_ERR_INVALID, "bad wall count");
+  std::vector<std::array<float, 5>> ws;
+  for (int q = 0; q < count; ++q) {
+    const float* w = &walls[5 * q];
+    float sum = 0.0f;
+    for (int c = 0; c < 3; ++c) sum += w[c] * w[c];
+    if (!(sum > 0.0f)) return fail(TECS_ERR_INVALID, "wall normal is zero");
+    const float inv = 1.0f / std::sqrt(sum);
+    ws.push_back({w[0] * inv, w[1] * inv, w[2] * inv, w[3] * inv, w[4]});
+  }
+  s.walls = ws;
+  return TECS_OK;
+}
+
+int orc_set_links(orc_sim* h, const int32_t* links, int64_t count, float strength) {
+  Sim& s = *reinterpret_cast<Sim*>(h);
+  if (count < 0) return fail(TECS_ERR_INVALID, "bad link count");
+  for (int64_t q = 0; q < 2 * count; ++q)
+    if (links[q] < 0 || (unsigned)links[q] >= s.cap) return fail(TECS_ERR_INVALID, "link endpoint beyond the capacity");
+  s.links.assign(links, links + 2 * count);
+  s.link_strength = strength;
+  return TECS_OK;
+}
+
+// The UPDATE_FUNC `proliferate` of examples/growth_w_wall.cpp:52-64, run by Simulation::init
+// (simulation.hpp:520-529) over the active cells: cell i divides when u0 <= rate (the reference
+// returns when drand > rate); the new cell takes the next index (the reference's atomic counter;
+// here in the order of the parents), at p_i + offset * to_vector3(theta, phi) with theta =
+// float(acos(2 u1 - 1)), phi = float(u2 * 2 * double(pi_f)) (drand is double, Polarity narrows;
+// polarity.hpp:44, :111-118). The sines and cosines are the double functions rounded to float
+// (within an ulp of a float sin / cos). Only positions of the new cells are written.
+int orc_divide(orc_sim* h, float rate, float offset, const double* u, unsigned* new_count) {
+  Sim& s = *reinterpret_cast<Sim*>(h);
+  unsigned m = s.n;
+  for (unsigned i = 0; i < s.n; ++i) m += u[3 * (size_t)i] <= (double)rate;
+  if (m > s.cap) return fail(TECS_ERR_CAPACITY, "new cells beyond the capacity");
+  const double pi_f = (double)3.14159265358979323846f;
+  m = s.n;
+  for (unsigned i = 0; i < s.n; ++i) {
+    const double* v = &u[3 * (size_t)i];
+    if (v[0] > (double)rate) continue;
+    const float theta = (float)std::acos(2.0 * v[1] - 1);
+    const float phi = (float)(v[2] * 2.0 * pi_f);
+    const float st = (float)std::sin((double)theta), ct = (float)std::cos((double)theta);
+    const float sp = (float)std::sin((double)phi), cp = (float)std::cos((double)phi);
+    const float dir[3] = {st * cp, st * sp, ct};
+    for (int c = 0; c < 3; ++c) s.pos[3 * (size_t)m + c] = s.pos[3 * (size_t)i + c] + offset * dir[c];
+    ++m;
+  }
+  *new_count = m;
   return TECS_OK;
 }
 

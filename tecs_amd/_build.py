@@ -20,7 +20,7 @@ CSRC = PKG / "csrc"
 OUT_DIR = PKG / "_lib"
 LIB = OUT_DIR / "libtecs_amd.so"
 SOURCES = ["tecs_api.hip", "kernels_sweep.hip", "kernels_team.hip", "kernels_grid.hip", "kernels_query.hip",
-           "kernels_delaunay.hip", "io_snapshot.hip", "server.hip"]
+           "kernels_delaunay.hip", "kernels_extra.hip", "io_snapshot.hip", "server.hip"]
 ARCH = os.environ.get("TECS_AMD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -35,6 +35,9 @@ PROTOTYPES = [
     ("tecs_sim_nearest", C.c_int, [_P, _I32, _F32P, _I32, _P]),
     ("tecs_sim_random_in_bins", C.c_int, [_P, _I32, _F32P, _I32, _F32P, _P]),
     ("tecs_sim_random_neighbour", C.c_int, [_P, _I32, _F32P, C.c_float, _F32P, _P]),
+    ("tecs_sim_set_walls", C.c_int, [_P, _P, _I32]),
+    ("tecs_sim_set_links", C.c_int, [_P, _P, C.c_int64, C.c_float]),
+    ("tecs_sim_divide", C.c_int, [_P, C.c_float, C.c_float, _P, C.POINTER(C.c_uint32)]),
     ("tecs_sim_take_step", C.c_int, [_P, C.c_double, _I32]),
     ("tecs_sim_set_force", C.c_int, [_P, _P]),
     ("tecs_sim_evaluate_force", C.c_int, [_P, _I32, _F32P, _F32P]),
@@ -58,6 +61,9 @@ PROTOTYPES = [
     ("tecs_batch_set_force", C.c_int, [_P, _I32, _P]),
     ("tecs_batch_upload_range", C.c_int, [_P, _I32, C.c_uint32, C.c_uint32, _F32P, _F32P, _P]),
     ("tecs_batch_set_agent_count", C.c_int, [_P, _I32, C.c_uint32]),
+    ("tecs_batch_set_walls", C.c_int, [_P, _I32, _P, _I32]),
+    ("tecs_batch_set_links", C.c_int, [_P, _I32, _P, C.c_int64, C.c_float]),
+    ("tecs_batch_divide", C.c_int, [_P, _I32, C.c_float, C.c_float, _P, C.POINTER(C.c_uint32)]),
     ("tecs_batch_set_capacity", C.c_int, [_P, _I32, C.c_uint32]),
     ("tecs_batch_synchronize", C.c_int, [_P]),
     ("tecs_batch_get_stats", C.c_int, [_P, C.POINTER(abi.Stats)]),

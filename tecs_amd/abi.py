@@ -37,6 +37,7 @@ FINDER_NAMES = {
     BINNED_GABRIEL_SYMMETRIC: "BinnedGabrielSymmetric",
 }
 NAIVE_DELAUNAY = 8
+MAX_WALLS = 8
 FINDER_NAMES[NAIVE_DELAUNAY] = "NaiveDelaunay"
 SYMMETRIC_FINDERS = (NAIVE_ALL_PAIRS_SYMMETRIC, BINNED_ALL_PAIRS_SYMMETRIC, NAIVE_GABRIEL_SYMMETRIC,
                      BINNED_GABRIEL_SYMMETRIC)
@@ -103,6 +104,21 @@ class SimDesc(C.Structure):
         ("force", ForceDesc),
         ("track_old_velocities", C.c_int32),
     ]
+
+
+class Wall(C.Structure):
+    _fields_ = [("plane", C.c_float * 4), ("strength", C.c_float)]
+
+
+def make_walls(rows):
+    """A ctypes array of tecs_wall from rows (a, b, c, d, strength)."""
+    rows = [tuple(float(v) for v in r) for r in rows]
+    arr = (Wall * max(len(rows), 1))()
+    for q, r in enumerate(rows):
+        for j in range(4):
+            arr[q].plane[j] = r[j]
+        arr[q].strength = r[4]
+    return arr, len(rows)
 
 
 class Stats(C.Structure):

@@ -207,6 +207,15 @@ struct Batch {
   int32_t* d_cols = nullptr;
   int64_t cols_cap = 0;
   uint32_t* dl_cnt = nullptr;  // NaiveDelaunay: counters of the work items found (SweepArgs)
+  // generic wall forces (in the instance table) and link forces (kernels_extra.hip)
+  std::vector<std::vector<int32_t>> member_links;  // per member: a0, b0, a1, b1, ...
+  std::vector<float> member_link_k;
+  int4* d_links = nullptr;
+  size_t links_bytes = 0;  // size class of d_links (0: hipMalloc'd or none)
+  int64_t links_cap = 0;
+  int n_links = 0;
+  int* inv = nullptr;     // slot of every agent (links, division)
+  int* st_idx = nullptr;  // staging of agent indices (division)
   int dl_row = 0, dl_splits = 1;
   DevicePool* pool = nullptr;                          // optional source of device memory
   std::vector<std::pair<void*, size_t>> allocations;  // (block, size class; 0 = hipMalloc)
@@ -872,7 +881,7 @@ struct Batch {
   // symmetric ones a pass computes them from F (+ q, the movers' contributions, when the
   // evaluation saw a stale grid: with_q), and the update adds q and clears it.
   int com_and_update(int kind, const float4* F = nullptr, bool with_q = false) {
-    const bool sym = delta_only(finder);
+    const bool sym = separate_update();
     if (com_fixer == TECS_GLOBAL_COM_FIXER) {
       if (sym) launch_com_partials(d_inst, d_block_inst, F, with_q ? q : nullptr, com_partial, n_blocks, stream);
       launch_com_finalize(d_inst, n_inst, com_partial, sym ? 1 : partials_per_block(), com, stream);
@@ -908,25 +917,52 @@ struct Batch {
     a->q_pol = qp;
   }
 
-  // A step with a symmetric finder or NaiveDelaunay: each evaluation writes its deltas (the
-  // movers' part into q on a stale grid), then the COM fix and the update run as their own
-  // kernels. Heun: evaluate -> predictor -> evaluate (stale grid) -> corrector.
+  // Members with walls or links: the forces beyond the pair finder join every evaluation.
+  bool has_extra() const {
+    if (n_links > 0) return true;
+    for (const auto& in : h_inst)
+      if (in.n_walls > 0) return true;
+    return false;
+  }
+  // The steps take the separate-update path (below).
+  bool separate_update() const { return delta_only(finder) || has_extra(); }
+
+  // The generic wall forces (before the pair forces in the reference's order, so W + F) and the
+  // link forces (after them) of an evaluation at stage positions xs into its deltas F.
+  int apply_extra(const float4* xs, float4* F) {
+    bool walls = false;
+    for (const auto& in : h_inst) walls = walls || in.n_walls > 0;
+    if (walls) launch_walls(d_inst, d_block_inst, xs, F, n_blocks, stream);
+    if (n_links > 0) {
+      launch_inverse_ids(d_inst, d_block_inst, id[ic], inv, n_blocks, stream);
+      launch_links(d_inst, d_links, n_links, inv, xs, F, stream);
+    }
+    HIP_TRY(hipGetLastError());
+    return TECS_OK;
+  }
+
+  // A step with a symmetric finder, NaiveDelaunay, walls or links: each evaluation writes its
+  // deltas (a symmetric finder's movers' part into q on a stale grid), the wall and link
+  // forces join them, then the COM fix and the update run as their own kernels.
+  // Heun: evaluate -> predictor -> evaluate (stale grid) -> corrector.
   int step_delta_only() {
-    const bool binned = is_binned(finder);
-    const bool stale1 = binned && !rebuilt_now;
+    const bool sym_binned = is_symmetric(finder) && is_binned(finder);
+    const bool stale1 = sym_binned && !rebuilt_now;
     SweepArgs a = base_args(x[xc], pol[xc]);
     a.o_pos = d1;
     a.o_pol = d1p;
     sym_grid_args(&a, stale1);
     TRY(sweep(kModeDelta, a, kTimeSweepFull));
+    TRY(apply_extra(x[xc], d1));
     if (integrator == TECS_HEUN) {
       TRY(com_and_update(kUpdatePred, d1, stale1));
       SweepArgs b = base_args(xp, polp);
       b.o_pos = d3;
       b.o_pol = d3p;
-      sym_grid_args(&b, binned);
+      sym_grid_args(&b, sym_binned);
       TRY(sweep(kModeDelta, b, kTimeSweepStale));
-      TRY(com_and_update(kUpdateCorr, d3, binned));
+      TRY(apply_extra(xp, d3));
+      TRY(com_and_update(kUpdateCorr, d3, sym_binned));
     } else {
       TRY(com_and_update(kUpdateEuler, d1, stale1));
     }
@@ -937,7 +973,7 @@ struct Batch {
   // One integrator step (Euler::integrate euler.hpp:81-85, Heun::integrate heun.hpp:201-208).
   int step_once() {
     TRY(maybe_rebuild(true));
-    if (delta_only(finder)) return step_delta_only();
+    if (separate_update()) return step_delta_only();
     const bool com_on = com_fixer == TECS_GLOBAL_COM_FIXER;
     if (integrator == TECS_HEUN) {
       if (!com_on) {
@@ -1270,6 +1306,132 @@ struct Batch {
     return TECS_OK;
   }
 
+  // Release one block of `allocations` (the stream must be idle).
+  void free_one(void* v) {
+    for (size_t q = 0; q < allocations.size(); ++q)
+      if (allocations[q].first == v) {
+        if (pool && allocations[q].second)
+          pool->put(v, allocations[q].second);
+        else
+          hipFree(v);
+        allocations.erase(allocations.begin() + (long)q);
+        return;
+      }
+  }
+
+  // The Heun corrector's delta buffer, which the separate-update path needs (allocated when a
+  // member first gets walls or links).
+  int ensure_d3() {
+    if (integrator != TECS_HEUN || d3) return TECS_OK;
+    TRY(alloc(&d3, (size_t)total_slots));
+    if (has_pol) TRY(alloc(&d3p, (size_t)total_slots));
+    return TECS_OK;
+  }
+
+  // Generic wall forces of member k (PlaneN normalisation, types/plane.hpp:24-31, on the host
+  // in FP32). Captured step graphs are dropped: the step path depends on whether any member
+  // has walls or links.
+  int set_walls(int k, const tecs_wall* walls, int count) {
+    TRY(check_index(k));
+    if (count < 0 || count > kMaxWalls) return fail(TECS_ERR_INVALID, "bad wall count");
+    if (count > 0 && !walls) return fail(TECS_ERR_INVALID, "null walls");
+    InstanceDesc& in = h_inst[k];
+    float w[kMaxWalls][5];
+    for (int q = 0; q < count; ++q) {
+      const float* c = walls[q].plane;
+      float sum = 0.0f;
+      for (int j = 0; j < 3; ++j) sum += c[j] * c[j];
+      if (!(sum > 0.0f) || !std::isfinite(sum)) return fail(TECS_ERR_INVALID, "wall normal is zero");
+      const float inv_norm = 1.0f / std::sqrt(sum);
+      for (int j = 0; j < 4; ++j) w[q][j] = c[j] * inv_norm;
+      w[q][4] = walls[q].strength;
+    }
+    TRY(ensure_d3());
+    HIP_TRY(hipStreamSynchronize(stream));
+    drop_graphs();
+    in.n_walls = count;
+    std::memcpy(in.wall, w, sizeof(float) * 5 * (size_t)count);
+    inst_dirty = true;
+    return TECS_OK;
+  }
+
+  // Links of member k (agent indices below its capacity) and their strength.
+  int set_links(int k, const int32_t* links, int64_t count, float strength) {
+    TRY(check_index(k));
+    if (count < 0 || count > (int64_t)1 << 30) return fail(TECS_ERR_INVALID, "bad link count");
+    if (count > 0 && !links) return fail(TECS_ERR_INVALID, "null links");
+    for (int64_t q = 0; q < 2 * count; ++q)
+      if (links[q] < 0 || (uint32_t)links[q] >= descs[k].capacity)
+        return fail(TECS_ERR_INVALID, "link endpoint beyond the capacity");
+    TRY(ensure_d3());
+    HIP_TRY(hipStreamSynchronize(stream));
+    drop_graphs();
+    member_links.resize(n_inst);
+    member_link_k.resize(n_inst, 0.0f);
+    member_links[k].assign(links, links + 2 * count);
+    member_link_k[k] = strength;
+    std::vector<int4> all;
+    for (int m = 0; m < n_inst; ++m) {
+      int kb = 0;
+      std::memcpy(&kb, &member_link_k[m], sizeof(kb));
+      const auto& L = member_links[m];
+      for (size_t q = 0; q + 1 < L.size(); q += 2) all.push_back(make_int4(L[q], L[q + 1], kb, m));
+    }
+    if ((int64_t)all.size() > links_cap) {
+      if (d_links) free_one(d_links);
+      d_links = nullptr;
+      links_cap = 0;
+      TRY(alloc(&d_links, all.size()));
+      links_cap = (int64_t)all.size();
+    }
+    if (!inv) TRY(alloc(&inv, (size_t)total_slots));
+    if (!all.empty())
+      HIP_TRY(hipMemcpyAsync(d_links, all.data(), sizeof(int4) * all.size(), hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    n_links = (int)all.size();
+    return TECS_OK;
+  }
+
+  // Cell division of member k (the UPDATE_FUNC `proliferate` of examples/growth_w_wall.cpp:
+  // 52-64): the daughters' directions are formed on the host from the caller's draws, the
+  // daughters written on the device next to their parents' current positions.
+  int divide(int k, float rate, float offset, const double* u, uint32_t* new_count) {
+    TRY(check_index(k));
+    const uint32_t n = descs[k].agent_count;
+    if (n > 0 && !u) return fail(TECS_ERR_INVALID, "null uniforms");
+    std::vector<int> parents;
+    std::vector<float> dirs;
+    const double pi_f = (double)3.14159265358979323846f;  // Kokkos::numbers::pi_v<float>
+    for (uint32_t i = 0; i < n; ++i) {
+      const double* v = &u[3 * (size_t)i];
+      if (v[0] > (double)rate) continue;  // the reference returns when drand > rate
+      // Polarity(acos(2 drand - 1), drand * 2 * pi) narrows both angles to float
+      const float theta = (float)std::acos(2.0 * v[1] - 1);
+      const float phi = (float)(v[2] * 2.0 * pi_f);
+      const float st = (float)std::sin((double)theta), ct = (float)std::cos((double)theta);
+      const float sp = (float)std::sin((double)phi), cp = (float)std::cos((double)phi);
+      parents.push_back((int)i);
+      dirs.push_back(st * cp);  // to_vector3, polarity.hpp:111-118
+      dirs.push_back(st * sp);
+      dirs.push_back(ct);
+    }
+    const uint64_t total = (uint64_t)n + parents.size();
+    if (total > descs[k].capacity) return fail(TECS_ERR_CAPACITY, "new cells beyond the capacity");
+    if (new_count) *new_count = (uint32_t)total;
+    if (parents.empty()) return TECS_OK;
+    if (!inv) TRY(alloc(&inv, (size_t)total_slots));
+    if (!st_idx) TRY(alloc(&st_idx, (size_t)std::max(max_n, 1)));
+    TRY(sync_inst());
+    HIP_TRY(hipMemcpyAsync(st_idx, parents.data(), sizeof(int) * parents.size(), hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipMemcpyAsync(st_pos, dirs.data(), sizeof(float) * dirs.size(), hipMemcpyHostToDevice, stream));
+    launch_inverse_ids(d_inst, d_block_inst, id[ic], inv, n_blocks, stream);
+    launch_divide(d_inst, k, inv, st_idx, st_pos, (int)parents.size(), (int)n, offset, x[xc],
+                  x[xc ^ 1], stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(stream));
+    return TECS_OK;
+  }
+
   // The state of every slot of `old`'s single member, carried into this batch (created for the
   // same member with a larger capacity): both buffer generations get the cells of the current
   // one, so its inactive cells stay in index order in both. The grid is rebuilt at the next full
@@ -1288,6 +1450,17 @@ struct Batch {
       if (track_ov && old.track_ov)
         HIP_TRY(hipMemcpyAsync(ov[g], old.ov[old.xc], sizeof(float4) * len, hipMemcpyDeviceToDevice, stream));
     }
+    // the forces beyond the pair finder
+    if (old.h_inst[0].n_walls > 0) {
+      TRY(ensure_d3());
+      h_inst[0].n_walls = old.h_inst[0].n_walls;
+      std::memcpy(h_inst[0].wall, old.h_inst[0].wall, sizeof(h_inst[0].wall));
+      inst_dirty = true;
+      TRY(sync_inst());
+    }
+    if (old.n_links > 0)
+      TRY(set_links(0, old.member_links[0].data(), (int64_t)(old.member_links[0].size() / 2),
+                    old.member_link_k[0]));
     variant = old.variant;
     graph_mode = old.graph_mode;
     timing = old.timing;

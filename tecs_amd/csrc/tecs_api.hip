@@ -172,6 +172,22 @@ int tecs_sim_random_neighbour(tecs_sim* sim, int32_t n, const float* points, flo
   return sim->b->query(tecs_amd::kQueryRandomNeighbour, n, points, rb, radius, uniforms, out);
 }
 
+int tecs_sim_set_walls(tecs_sim* sim, const tecs_wall* walls, int32_t count) {
+  GUARD(sim);
+  return sim->b->set_walls(0, walls, count);
+}
+
+int tecs_sim_set_links(tecs_sim* sim, const int32_t* links, int64_t count, float strength) {
+  GUARD(sim);
+  return sim->b->set_links(0, links, count, strength);
+}
+
+int tecs_sim_divide(tecs_sim* sim, float rate, float offset, const double* uniforms,
+                    uint32_t* new_count) {
+  GUARD(sim);
+  return sim->b->divide(0, rate, offset, uniforms, new_count);
+}
+
 int tecs_sim_take_step(tecs_sim* sim, double dt, int32_t n_steps) {
   GUARD(sim);
   return sim->b->take_step(&dt, n_steps);
@@ -325,6 +341,23 @@ int tecs_batch_upload_range(tecs_batch* batch, int32_t index, uint32_t first, ui
 int tecs_batch_set_agent_count(tecs_batch* batch, int32_t index, uint32_t agent_count) {
   GUARD(batch);
   return batch->b.set_agent_count(index, agent_count);
+}
+
+int tecs_batch_set_walls(tecs_batch* batch, int32_t index, const tecs_wall* walls, int32_t count) {
+  GUARD(batch);
+  return batch->b.set_walls(index, walls, count);
+}
+
+int tecs_batch_set_links(tecs_batch* batch, int32_t index, const int32_t* links, int64_t count,
+                         float strength) {
+  GUARD(batch);
+  return batch->b.set_links(index, links, count, strength);
+}
+
+int tecs_batch_divide(tecs_batch* batch, int32_t index, float rate, float offset,
+                      const double* uniforms, uint32_t* new_count) {
+  GUARD(batch);
+  return batch->b.divide(index, rate, offset, uniforms, new_count);
 }
 
 int tecs_batch_set_capacity(tecs_batch* batch, int32_t index, uint32_t capacity) {

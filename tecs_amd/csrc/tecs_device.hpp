@@ -27,6 +27,7 @@ namespace tecs_amd {
 
 constexpr int kBlock = 256;  // slots per workgroup (4 wavefronts of 64)
 constexpr int kMaxTypes = 4;
+constexpr int kMaxWalls = 8;  // TECS_MAX_WALLS
 
 struct ForceParams {
   float h, krep, kadh;
@@ -57,6 +58,8 @@ struct InstanceDesc {
   int bin_pow2;          // bin_size is a power of two: (p - min) / bin == (p - min) * inv_bin
   float dt;              // float(dt) of the current step
   ForceParams force;
+  int n_walls;           // generic wall forces: normalised (a, b, c, d) and strength
+  float wall[kMaxWalls][5];
 };
 
 // Logical workgroup index with an XCD-aware remap: the hardware hands consecutive workgroup

@@ -237,6 +237,20 @@ struct QueryArgs {
 };
 void launch_query(int kind, const QueryArgs& a, hipStream_t stream);
 
+// Generic wall forces, link forces and division (kernels_extra.hip).
+// d += the walls' sum W (as W + d) for the active cells of instances with walls
+void launch_walls(const InstanceDesc* inst, const int* block_inst, const float4* xs, float4* d,
+                  int n_blocks, hipStream_t stream);
+// inv[slot_begin + agent] = slot over all slots
+void launch_inverse_ids(const InstanceDesc* inst, const int* block_inst, const int* id, int* inv,
+                        int n_blocks, hipStream_t stream);
+// links: (a, b, strength bits, instance), agent indices of the instance
+void launch_links(const InstanceDesc* inst, const int4* links, int n_links, const int* inv,
+                  const float4* xs, float4* d, hipStream_t stream);
+void launch_divide(const InstanceDesc* inst, int ii, const int* inv, const int* parents,
+                   const float* dirs, int count, int first, float offset, float4* x_cur,
+                   float4* x_other, hipStream_t stream);
+
 void launch_polar_trig(const float2* pol, float4* trig, int total_slots, hipStream_t stream);
 void launch_copy_probe(const float4* src, float4* dst, int64_t n, hipStream_t stream);
 

@@ -96,6 +96,26 @@ class GpuSimulation:
         check(self._lib.tecs_sim_set_capacity(self._h, int(capacity)))
         self.desc.capacity = int(capacity)
 
+    def set_walls(self, walls):
+        """Generic wall forces: rows (a, b, c, d, strength) of planes a x + b y + c z + d = 0
+        (examples/growth_w_wall.cpp:44-48)."""
+        arr, count = abi.make_walls(walls)
+        check(self._lib.tecs_sim_set_walls(self._h, C.addressof(arr), count))
+
+    def set_links(self, links, strength: float):
+        """Links (rows a, b of agent indices) and the link force strength
+        (examples/intercalation.cpp)."""
+        lk = np.ascontiguousarray(links, np.int32).reshape(-1, 2)
+        check(self._lib.tecs_sim_set_links(self._h, _ptr(lk) if lk.size else None, lk.shape[0], float(strength)))
+
+    def divide(self, rate: float, offset: float, uniforms) -> int:
+        """Cell division (examples/growth_w_wall.cpp:52-64) with three draws per active cell;
+        returns the count to pass to set_agent_count."""
+        u = np.ascontiguousarray(uniforms, np.float64).reshape(self.n, 3)
+        out = C.c_uint32(0)
+        check(self._lib.tecs_sim_divide(self._h, float(rate), float(offset), _ptr(u), C.byref(out)))
+        return int(out.value)
+
     def nearest(self, points, search_radius: int = 1):
         """Grid::get_nearest_point_index for every row of points (grid.hpp:413-441)."""
         p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
@@ -245,6 +265,22 @@ class GpuBatch:
     def set_capacity(self, k, capacity: int):
         check(self._lib.tecs_batch_set_capacity(self._h, int(k), int(capacity)))
         self.descs[k].capacity = int(capacity)
+
+    def set_walls(self, k, walls):
+        arr, count = abi.make_walls(walls)
+        check(self._lib.tecs_batch_set_walls(self._h, int(k), C.addressof(arr), count))
+
+    def set_links(self, k, links, strength: float):
+        lk = np.ascontiguousarray(links, np.int32).reshape(-1, 2)
+        check(self._lib.tecs_batch_set_links(self._h, int(k), _ptr(lk) if lk.size else None, lk.shape[0],
+                                             float(strength)))
+
+    def divide(self, k, rate: float, offset: float, uniforms) -> int:
+        n = int(self.descs[k].agent_count)
+        u = np.ascontiguousarray(uniforms, np.float64).reshape(n, 3)
+        out = C.c_uint32(0)
+        check(self._lib.tecs_batch_divide(self._h, int(k), float(rate), float(offset), _ptr(u), C.byref(out)))
+        return int(out.value)
 
     def download(self, k):
         d = self.descs[k]
