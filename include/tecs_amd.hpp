@@ -330,6 +330,25 @@ class Simulation {
     return c;
   }
 
+  // The generic wall forces (GENERIC_FORCE of examples/growth_w_wall.cpp:44-48): planes
+  // a x + b y + c z + d = 0 as PlaneN (types/plane.hpp) with a strength each.
+  void set_walls(const std::vector<tecs_wall>& walls) {
+    check(tecs_sim_set_walls(h_, walls.empty() ? nullptr : walls.data(), (int32_t)walls.size()));
+  }
+  // The links (types/link.hpp) and the LINK_FORCE of examples/intercalation.cpp.
+  void set_links(const std::vector<std::array<int32_t, 2>>& links, float strength) {
+    check(tecs_sim_set_links(h_, links.empty() ? nullptr : links[0].data(), (int64_t)links.size(),
+                             strength));
+  }
+  // The UPDATE_FUNC `proliferate` of examples/growth_w_wall.cpp:52-64 with three draws per
+  // active cell; returns the count for set_agent_count.
+  unsigned divide(float rate, float offset, const std::vector<std::array<double, 3>>& uniforms) {
+    if (uniforms.size() != n_) throw Error(TECS_ERR_INVALID, "uniforms size");
+    uint32_t out = 0;
+    check(tecs_sim_divide(h_, rate, offset, uniforms.empty() ? nullptr : uniforms[0].data(), &out));
+    return out;
+  }
+
   // Grid::get_nearest_point_index (grid.hpp:413-441) for a batch of points on the simulation's
   // grid; -1 where none of the searched bins holds a cell.
   std::vector<int32_t> get_nearest_point_indices(const std::vector<std::array<float, 3>>& points,

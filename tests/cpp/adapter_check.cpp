@@ -37,6 +37,15 @@ struct SortingConfig : DefaultSimulationConfig {
   TECS_CONFIG_FIELDS(fields::type)
 };
 
+struct DelaunayConfig : DefaultSimulationConfig {
+  TECS_CONFIG_PAIR_FINDER(pair_finders::NaiveDelaunay)
+};
+
+// every member of the adapter compiles for these configurations
+template class tecs_amd::Simulation<EpitheliumConfig>;
+template class tecs_amd::Simulation<SortingConfig>;
+template class tecs_amd::Simulation<DelaunayConfig>;
+
 #define EXPECT(c)                                                   \
   do {                                                              \
     if (!(c)) {                                                     \
