@@ -1620,6 +1620,9 @@ struct Batch {
     sl->has_pol = has_pol;
     sl->time = time;
     sl->step = write_step_count[k]++;
+    sl->links.clear();
+    if (k < (int)member_links.size())
+      sl->links.assign(member_links[k].begin(), member_links[k].end());
     launch_download(d_inst, k, in.block_begin, in.n_blocks, id[ic], x[xc], pol[xc], nullptr,
                     sl->d_pos, has_pol ? sl->d_pol : nullptr, nullptr, stream);
     hipError_t e = hipGetLastError();

@@ -52,6 +52,7 @@ struct H5Api {
   herr_t (*dclose)(hid_t) = nullptr;
   hid_t* native_float = nullptr;
   hid_t* native_uchar = nullptr;
+  hid_t* native_uint = nullptr;
   std::string error;
 
   bool load() {
@@ -93,6 +94,7 @@ struct H5Api {
     sym(dclose, "H5Dclose");
     sym(native_float, "H5T_NATIVE_FLOAT_g");
     sym(native_uchar, "H5T_NATIVE_UCHAR_g");
+    sym(native_uint, "H5T_NATIVE_UINT_g");
     if (ok && open() < 0) {
       ok = false;
       error = "H5open failed";
@@ -180,9 +182,11 @@ static bool write_dataset(hid_t group, const char* label, hsize_t n, hsize_t dim
   return ok;
 }
 
-// HDF5_Writer::write, hdf5_writer.hpp:306-321, for the fields position (+ polarity).
+// HDF5_Writer::write, hdf5_writer.hpp:306-321, for the fields position (+ polarity) and the
+// links (Simulation::write passes its links view last, simulation.hpp:607-611; a Link is two
+// uint32, types/link.hpp:30-84).
 int Writer::write_step(double time, unsigned step, size_t n, const float* pos, const float* pol,
-                       std::string* err) {
+                       std::string* err, const uint32_t* links, size_t n_links) {
   std::lock_guard<std::mutex> g(h5_mutex());
   const std::string gname = "t" + std::to_string(step);
   const hid_t group = h5().gcreate(file, gname.c_str(), kH5PDefault, kH5PDefault, kH5PDefault);
@@ -192,6 +196,7 @@ int Writer::write_step(double time, unsigned step, size_t n, const float* pos, c
   }
   bool ok = write_dataset(group, "position", n, 3, *h5().native_float, pos);
   if (ok && pol) ok = write_dataset(group, "polarity", n, 2, *h5().native_float, pol);
+  if (ok && links) ok = write_dataset(group, "links", n_links, 2, *h5().native_uint, links);
   h5().gclose(group);
   h5().fflush(file, 0);
   if (!ok) {
@@ -199,11 +204,13 @@ int Writer::write_step(double time, unsigned step, size_t n, const float* pos, c
     return -1;
   }
   if (!write_xmf || !xmf_file.is_open()) return 0;
-  // write_xmf_grid_start, hdf5_writer.hpp:163-197 (no links)
+  // write_xmf_grid_start, hdf5_writer.hpp:163-197: with links, the links are the topology
+  const bool has_links = links && n_links > 0;
   xmf_buffer += "\t\t\t<Grid Name=\"t" + std::to_string(step) + "\" GridType=\"Uniform\">\n";
   xmf_buffer += "\t\t\t\t<Time Value=\"" + std::to_string(time);
-  xmf_buffer += "\"/>\n\t\t\t\t<Topology TopologyType=\"Polyvertex\" NumberOfElements=\"" +
-                std::to_string(n);
+  if (!has_links)
+    xmf_buffer += "\"/>\n\t\t\t\t<Topology TopologyType=\"Polyvertex\" NumberOfElements=\"" +
+                  std::to_string(n);
   xmf_buffer += "\"/>\n\t\t\t\t<Geometry GeometryType=\"XYZ\">\n\t\t\t\t\t<DataItem Format=\"HDF\" "
                 "Dimensions=\"" + std::to_string(n) + " 3\">\n\t\t\t\t\t\t" + filename + ".h5:t" +
                 std::to_string(step) + "/position\n\t\t\t\t\t</DataItem>\n\t\t\t\t</Geometry>\n";
@@ -213,6 +220,13 @@ int Writer::write_step(double time, unsigned step, size_t n, const float* pos, c
                   "\t\t\t\t\t<DataItem Format=\"HDF\" Dimensions=\"" + std::to_string(n) +
                   " 2 \">\n\t\t\t\t\t\t" + filename + ".h5:" + gname +
                   "/polarity\n\t\t\t\t\t</DataItem>\n\t\t\t\t</Attribute>\n";
+  }
+  // write_xmf_grid_item for the links view: a polyline topology (hdf5_writer.hpp:241-266)
+  if (has_links) {
+    xmf_buffer += "\t\t\t\t<Topology TopologyType=\"Polyline\" NumberOfElements=\"" +
+                  std::to_string(n_links) + "\">\n\t\t\t\t\t<DataItem Format=\"HDF\" "
+                  "DataType=\"UInt32\" Dimensions=\"" + std::to_string(n_links) + " 2 \">\n\t\t\t\t\t\t" +
+                  filename + ".h5:" + gname + "/links\n\t\t\t\t\t</DataItem>\n\t\t\t\t</Topology>\n";
   }
   // write_xmf_grid_end, hdf5_writer.hpp:274-285
   xmf_buffer += xmf_static;
@@ -280,7 +294,8 @@ void Writer::run() {
       err = std::string("snapshot copy failed: ") + hipGetErrorString(e);
       rc = -1;
     } else {
-      rc = write_step(slot->time, slot->step, slot->n, slot->h_pos, slot->has_pol ? slot->h_pol : nullptr, &err);
+      rc = write_step(slot->time, slot->step, slot->n, slot->h_pos, slot->has_pol ? slot->h_pol : nullptr, &err,
+                      slot->links.empty() ? nullptr : slot->links.data(), slot->links.size() / 2);
     }
     {
       std::lock_guard<std::mutex> g(mu);

@@ -10,6 +10,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <vector>
 
 namespace tecs_amd {
 
@@ -26,6 +27,7 @@ struct SnapshotSlot {
   hipEvent_t copied = nullptr;
   double time = 0.0;
   unsigned step = 0;
+  std::vector<uint32_t> links;  // the member's links (a, b pairs) at the time of the write
   bool busy = false;
   std::mutex* mu = nullptr;               // the owner's slot mutex
   std::condition_variable* cv = nullptr;  // signalled when a slot frees
@@ -37,8 +39,9 @@ class Writer {
   int open(const std::string& path, bool xmf, std::string* err);
   int close(std::string* err);
   // synchronous writes (host data)
+  // links: n_links (a, b) pairs of uint32, or null
   int write_step(double time, unsigned step, size_t n, const float* pos, const float* pol,
-                 std::string* err);
+                 std::string* err, const uint32_t* links = nullptr, size_t n_links = 0);
   int write_static_u8(const std::string& label, size_t n, const uint8_t* data, std::string* err);
   // asynchronous: the slot's copy is in flight; the writer thread waits for it and writes
   void enqueue(SnapshotSlot* slot);

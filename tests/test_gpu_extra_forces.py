@@ -211,3 +211,32 @@ def test_walls_survive_capacity_reallocation(gpu, oracle_lib):
         s.set_capacity(3000)
         s.take_step(0.05, 3)
     _same(g, o, "after reallocation")
+
+
+def test_snapshot_writes_links(gpu, tmp_path):
+    """Simulation::write passes the links view last (simulation.hpp:607-611): a t<k>/links
+    dataset of (a, b) pairs and, in the XDMF, a polyline topology in place of the polyvertex
+    one (hdf5_writer.hpp:163-197, :241-266)."""
+    import xml.etree.ElementTree as ET
+
+    from tecs_amd import io
+
+    w = W.relaxation(1000)
+    links = np.arange(200, dtype=np.int32).reshape(-1, 2)
+    print("case snapshot with links", flush=True)
+    s = GpuSimulation(w.desc)
+    s.upload(w.positions)
+    base = str(tmp_path / "links" / "sim")
+    with io.Writer(base) as writer:
+        s.write(writer, 0.0)
+        s.set_links(links, 0.1)
+        s.take_step(0.05, 2)
+        s.write(writer, 0.1)
+    h5 = base + ".h5"
+    assert np.array_equal(io.read_dataset(h5, "t1/links"), links.astype(np.float32))
+    grids = ET.parse(base + ".xmf").getroot().find("Domain").find("Grid").findall("Grid")
+    assert grids[0].find("Topology").get("TopologyType") == "Polyvertex"
+    topo = grids[1].findall("Topology")
+    assert [t.get("TopologyType") for t in topo] == ["Polyline"]
+    assert topo[0].get("NumberOfElements") == "100"
+    assert topo[0].find("DataItem").text.strip() == "sim.h5:t1/links"
