@@ -414,13 +414,14 @@ struct Batch {
       }
       in.bin_begin = (int)bins;
       // sparse grids (many bins per cell, e.g. a sheet of cells on a sphere) count segments of
-      // 2^shift bins so the scan does not run over every empty bin: about 8 segments per cell,
+      // 2^shift bins so the scan does not run over every empty bin: about 4 segments per cell,
       // which keeps the segments that run along a sheet short (the gather ranks a cell among
-      // its segment's members)
+      // its segment's members); on the 200k-cell epithelium 4 per cell beat 8 (rebuild 38.3
+      // vs 40.7 us) and 2 (41.7 us)
       in.shift = 0;
       if (is_binned(finder) && in.n > 0) {
         const double bpc = (double)in.bin_count / (double)in.n;
-        if (bpc >= 8.0) in.shift = std::min(10, std::max(2, (int)std::floor(std::log2(bpc)) - 3));
+        if (bpc >= 8.0) in.shift = std::min(10, std::max(2, (int)std::floor(std::log2(bpc)) - 2));
       }
       in.cbin_count = ((in.bin_count + (1 << in.shift) - 1) >> in.shift) + 1;
       in.dt = 0.0f;
