@@ -240,3 +240,43 @@ def test_snapshot_writes_links(gpu, tmp_path):
     assert [t.get("TopologyType") for t in topo] == ["Polyline"]
     assert topo[0].get("NumberOfElements") == "100"
     assert topo[0].find("DataItem").text.strip() == "sim.h5:t1/links"
+
+
+def test_batch_links_and_division(gpu, oracle_lib):
+    """tecs_batch_set_links / tecs_batch_divide on one member of a batch: that member matches the
+    oracle, the other is untouched by them (bitwise as alone)."""
+    ws = [W.relaxation(n, seed_stream=40 + n, finder=abi.BINNED_GABRIEL) for n in (1200, 2000)]
+    descs = []
+    for w in ws:
+        d = abi.copy_desc(w.desc)
+        d.capacity = w.n + 400
+        descs.append(d)
+    print("case batch links + division", flush=True)
+    b = GpuBatch(descs)
+    for k, w in enumerate(ws):
+        b.upload(k, w.positions)
+    links = np.arange(600, dtype=np.int32).reshape(-1, 2)
+    b.set_links(1, links, 0.25)
+    o = oracle_lib.OracleSim(descs[1])
+    o.upload(ws[1].positions)
+    o.set_links(links, 0.25)
+    alone = GpuSimulation(descs[0])
+    alone.upload(ws[0].positions)
+    r = W.rng(41)
+    n1 = ws[1].n
+    for it in range(3):
+        b.take_step([0.05, 0.05], 2)
+        o.take_step(0.05, 2)
+        alone.take_step(0.05, 2)
+        u = r.random((n1, 3))
+        nb = b.divide(1, 0.05, 0.1875, u)
+        assert o.divide(0.05, 0.1875, u) == nb
+        b.set_agent_count(1, nb)
+        o.set_agent_count(nb)
+        n1 = nb
+    xb, _, _ = b.download(1)
+    xo, _, _ = o.download()
+    assert np.array_equal(xb.view(np.uint32), xo.view(np.uint32))
+    xa, _, _ = alone.download()
+    x0, _, _ = b.download(0)
+    assert np.array_equal(xa.view(np.uint32), x0.view(np.uint32))
