@@ -258,3 +258,32 @@ def test_capacity_limit(gpu):
     d = abi.make_desc(100, capacity=513, pair_finder=abi.NAIVE_DELAUNAY)
     with pytest.raises(TecsError, match="512"):
         GpuSimulation(d)
+
+
+def test_served_spheroids_equal_alone(gpu):
+    """Continuous batching of NaiveDelaunay requests (spheroids of 40 to 200 cells arriving
+    over several rounds): every result is bitwise the same simulation run alone."""
+    from tecs_amd.serving import Server
+
+    reqs = [(W.spheroid(n, seed_stream=60 + n), steps, arrive)
+            for n, steps, arrive in ((100, 9, 0), (40, 14, 0), (200, 6, 1), (70, 11, 2))]
+    print("case delaunay served", flush=True)
+    srv = Server(1 << 34, max_chunk_steps=4)
+    ids = {}
+    rounds = 0
+    while True:
+        for k, (w, steps, arrive) in enumerate(reqs):
+            if arrive == rounds and k not in ids:
+                ids[k] = srv.submit(w.desc, w.positions, None, None, 0.001, steps)
+        srv.advance()
+        rounds += 1
+        if len(ids) == len(reqs) and all(srv.info(i)["state"] == 2 for i in ids.values()):
+            break
+        assert rounds < 200
+    for k, (w, steps, _) in enumerate(reqs):
+        xs, _ = srv.result(ids[k])
+        s = GpuSimulation(w.desc)
+        s.upload(w.positions)
+        s.take_step(0.001, steps)
+        xa, _, _ = s.download()
+        assert _bitwise(xs, xa), k
