@@ -114,13 +114,19 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the streaming copy kernel used to calibrate HBM byte counters")
     ap.add_argument("--workload", default="epithelium",
-                    choices=["epithelium", "relaxation", "sorting", "sweep", "serve", "spheroid"],
+                    choices=["epithelium", "relaxation", "sorting", "sweep", "serve", "spheroid", "queries"],
                     help="epithelium (default): 200k cells, BinnedGabriel + polarity, Heun; "
                     "relaxation: 10k cells, BinnedAllPairs, Euler; sorting: 4M cells, "
                     "BinnedAllPairs + type table, Heun; sweep: 256 mixed-size instances, batched "
                     "vs one simulation at a time; spheroid: NaiveDelaunay spheroids of 100 cells, "
-                    "batched")
+                    "batched; queries: grid queries on the epithelium's grid")
     ap.add_argument("--instances", type=int, default=256, help="spheroid workload: batch size")
+    ap.add_argument("--walls", action="store_true",
+                    help="epithelium / relaxation / sorting with a generic wall force (a floor just "
+                    "inside the lowest cells, strength 1): the step takes the separate-update path")
+    ap.add_argument("--links", type=int, default=0,
+                    help="epithelium / relaxation / sorting with this many links between random cells "
+                    "and a random neighbour within distance 1 (strength 0.1)")
     ap.add_argument("--plain-steps", type=int, default=20,
                     help="sweep workload: steps per instance for the one-at-a-time comparison")
     ap.add_argument("--requests", type=int, default=48, help="serve workload: number of requests")
@@ -158,6 +164,8 @@ def main():
         return bench_serve(args)
     if args.workload == "spheroid":
         return bench_spheroid(args)
+    if args.workload == "queries":
+        return bench_queries(args)
     sym = {"finder": abi.BINNED_GABRIEL_SYMMETRIC if args.workload == "epithelium"
            else abi.BINNED_ALL_PAIRS_SYMMETRIC} if args.symmetric else {}
     if args.workload == "relaxation":
@@ -186,6 +194,23 @@ def bench_single(args, w):
     if args.sweep_variant != "auto":
         sim.set_sweep_variant({v: k for k, v in abi.SWEEP_NAMES.items()}[args.sweep_variant])
     sim.upload(w.positions, w.polarities, w.types)
+    extra = args.walls or args.links > 0
+    if args.walls:
+        # a floor just inside the lowest cells: z >= zmin + 0.5 (strength 1)
+        zmin = float(w.positions[:, 2].min())
+        sim.set_walls([[0.0, 0.0, 1.0, -(zmin + 0.5), 1.0]])
+    if args.links > 0:
+        import numpy as np
+
+        # each of `links` random cells linked to a random cell within distance 1 of it (the grid
+        # query of the first step's grid), as the protrusion examples link neighbours
+        g = np.random.default_rng(7)
+        sim.take_step(w.dt, 1)
+        a_idx = g.choice(w.n, size=min(args.links, w.n), replace=False)
+        x, _, _ = sim.download()
+        b_idx = sim.random_neighbour(x[a_idx], 1.0, g.random(len(a_idx)).astype(np.float32))
+        keep = b_idx >= 0
+        sim.set_links(np.stack([a_idx[keep], b_idx[keep]], axis=1), 0.1)
     print(f"bench: {w.name} n={w.n} warmup={args.warmup} steps={args.steps}", file=sys.stderr, flush=True)
     sim.take_step(w.dt, args.warmup)
     sim.synchronize()
@@ -214,7 +239,7 @@ def bench_single(args, w):
     has_type = bool(w.desc.fields & abi.FIELD_TYPE)
     bpc = algorithmic_bytes_per_cell(has_pol, has_type)
     symmetric = w.desc.pair_finder in abi.SYMMETRIC_FINDERS
-    if symmetric:
+    if symmetric or extra:
         # Delta sweep into a zeroed buffer: read s + a, the delta written once (s); the
         # integrator update is its own kernel
         s_bytes = 12 + (8 if has_pol else 0)
@@ -233,16 +258,16 @@ def bench_single(args, w):
     variant = abi.SWEEP_NAMES.get(sim.sweep_variant(), "?")
     kname = "k_sweep_binned" if variant == "thread" else f"k_sweep_team<{variant[4:]}>"
     fname = abi.FINDER_NAMES[w.desc.pair_finder]
-    if symmetric:
-        kernel_names = {"full": f"{kname} {fname} mode Delta (atomic)",
-                        "stale": f"{kname} {fname} mode Delta (atomic, stale grid)"}
+    if symmetric or extra:
+        kernel_names = {"full": f"{kname} {fname} mode Delta",
+                        "stale": f"{kname} {fname} mode Delta (stale grid)"}
     elif w.desc.integrator == abi.EULER:
         kernel_names = {"full": f"{kname} {fname} mode Euler", "stale": "-"}
     else:
         kernel_names = {"full": f"{kname} {fname} mode Pred (Heun substep 1)",
                         "stale": f"{kname} {fname} mode Corr (Heun substep 2, stale grid)"}
-    traffic = None if symmetric else measured_traffic(w.name, variant, dom)
-    valu = None if symmetric else measured_valu(w.name, variant, dom)
+    traffic = None if (symmetric or extra) else measured_traffic(w.name, variant, dom)
+    valu = None if (symmetric or extra) else measured_valu(w.name, variant, dom)
     print(
         f"bench: sweep full {per['full'][0]*1e3:.1f} us, stale {per['stale'][0]*1e3:.1f} us, "
         f"rebuild {reb_ms/max(reb_n,1)*1e3:.1f} us per launch",
@@ -269,6 +294,8 @@ def bench_single(args, w):
                 fname.replace("Symmetric", ""), fname),
             "cells": w.n,
             "pair_interactions_per_step": pairs / max(args.steps, 1),
+            "walls": 1 if args.walls else 0,
+            "links": args.links,
             "parallelism": "single GPU",
         },
         "roofline": {
@@ -305,6 +332,83 @@ def bench_single(args, w):
         cb = cpu_baseline(w, args.cpu_seconds)
         out["cpu_baseline"] = cb
         out["gpu_over_cpu"] = value / cb["value"]
+    print(json.dumps(out), flush=True)
+
+
+def bench_queries(args):
+    """Grid queries (grid.hpp:413-564) on the 200k-cell epithelium's grid after two steps:
+    1M query points at the cells plus N(0, 0.3) jitter, get_nearest_point_index with search
+    radius 1 and get_random_point_index_in_neighbourhood with radius 1 (caller draws). The
+    entry points take host buffers, so the rate includes the copies of the points in and the
+    answers out (16-20 bytes per query over PCIe); the kernel-only time is in profiles/."""
+    import numpy as np
+
+    from tecs_amd import GpuSimulation, workloads
+
+    w = workloads.epithelium(args.n)
+    sim = GpuSimulation(w.desc)
+    sim.upload(w.positions, w.polarities, w.types)
+    sim.take_step(w.dt, 2)
+    sim.synchronize()
+    g = workloads.rng(90)
+    Q = 1_000_000
+    pts = (w.positions[g.integers(0, w.n, Q)] + g.normal(0.0, 0.3, (Q, 3))).astype(np.float32)
+    u = g.random(Q).astype(np.float32)
+    print(f"bench: queries on epithelium n={w.n}, {Q} points", file=sys.stderr, flush=True)
+    reps = max(args.steps // 200, 3)
+    sim.nearest(pts, 1)
+    sim.random_neighbour(pts, 1.0, u)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sim.nearest(pts, 1)
+    t_near = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sim.random_neighbour(pts, 1.0, u)
+    t_rand = (time.perf_counter() - t0) / reps
+    out = {
+        "metric": "grid queries/sec (get_nearest_point_index, search radius 1 bin)",
+        "value": Q / t_near,
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": reps,
+        "warmup": 1,
+        "ms_per_step": 1e3 * t_near,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Philox, seed 2807)",
+        "config": {
+            "workload": f"grid queries: {Q} points (cells + N(0,0.3) jitter) on the grid of the "
+            f"{w.n}-cell polar epithelium (206^3 bins of edge 1), host buffers in and out",
+            "queries": Q,
+            "random_neighbour_queries_per_s": Q / t_rand,
+            "parallelism": "single GPU",
+        },
+        "roofline": None,
+    }
+    if not args.no_cpu_baseline:
+        import oracle
+
+        threads = int(os.environ.get("TECS_CPU_THREADS", min(16, os.cpu_count() or 1)))
+        oracle.set_threads(threads)
+        o = oracle.OracleSim(w.desc)
+        o.upload(w.positions, w.polarities, w.types)
+        o.take_step(w.dt, 2)
+        sample = 200_000
+        t0 = time.perf_counter()
+        o.nearest(pts[:sample], 1)
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": sample / el,
+            "unit": "queries/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": f"{sample} of the same nearest queries on the same grid ({el:.1f} s, C++ oracle, "
+            "serial loop)",
+        }
+        out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)
 
 

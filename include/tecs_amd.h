@@ -221,8 +221,9 @@ typedef struct tecs_wall {
 int tecs_sim_set_walls(tecs_sim* sim, const tecs_wall* walls, int32_t count);
 /* Replace the links (the Link views of Simulation, types/link.hpp; links: 2*count agent indices
  * a, b) and the link force, the LINK_FORCE of examples/intercalation.cpp / sorting_prot.cpp:
- * u = p_b - p_a, d = |u|, delta_a += (k u) / d, delta_b += (-k u) / d, each endpoint's
- * contribution added atomically as base.hpp:193-230 does. Links with a == b, or with an
+ * u = p_b - p_a, d = |u|, delta_a += (k u) / d, delta_b += (-k u) / d (base.hpp:193-230 adds
+ * the endpoints' contributions with atomics in no fixed order; here each cell adds its own in
+ * link order). Links with a == b, or with an
  * endpoint at or beyond the active count when a step runs, are skipped. Endpoints must be below
  * the capacity. count 0 removes them. */
 int tecs_sim_set_links(tecs_sim* sim, const int32_t* links, int64_t count, float strength);

@@ -210,10 +210,10 @@ struct Batch {
   // generic wall forces (in the instance table) and link forces (kernels_extra.hip)
   std::vector<std::vector<int32_t>> member_links;  // per member: a0, b0, a1, b1, ...
   std::vector<float> member_link_k;
-  int4* d_links = nullptr;
-  size_t links_bytes = 0;  // size class of d_links (0: hipMalloc'd or none)
-  int64_t links_cap = 0;
-  int n_links = 0;
+  int* d_link_rows = nullptr;   // incident links per cell: rows over agent space (slot_begin + agent)
+  int2* d_link_ent = nullptr;   // (other endpoint, +1 a end / -1 b end), link order per cell
+  int64_t link_ent_cap = 0;
+  int n_links = 0;              // incidences (two per link with distinct ends)
   int* inv = nullptr;     // slot of every agent (links, division)
   int* st_idx = nullptr;  // staging of agent indices (division)
   int dl_row = 0, dl_splits = 1;
@@ -929,15 +929,14 @@ struct Batch {
   bool separate_update() const { return delta_only(finder) || has_extra(); }
 
   // The generic wall forces (before the pair forces in the reference's order, so W + F) and the
-  // link forces (after them) of an evaluation at stage positions xs into its deltas F.
-  int apply_extra(const float4* xs, float4* F) {
-    bool walls = false;
-    for (const auto& in : h_inst) walls = walls || in.n_walls > 0;
-    if (walls) launch_walls(d_inst, d_block_inst, xs, F, n_blocks, stream);
-    if (n_links > 0) {
+  // link forces (after them) of an evaluation at stage positions xs into its deltas F. The
+  // agent -> slot map of the links is refreshed when `order_changed` (the first evaluation of a
+  // step: the slot order changes only at rebuilds).
+  int apply_extra(const float4* xs, float4* F, bool order_changed) {
+    if (n_links > 0 && order_changed)
       launch_inverse_ids(d_inst, d_block_inst, id[ic], inv, n_blocks, stream);
-      launch_links(d_inst, d_links, n_links, inv, xs, F, stream);
-    }
+    launch_extra(d_inst, d_block_inst, id[ic], inv, n_links > 0 ? d_link_rows : nullptr,
+                 d_link_ent, xs, F, n_blocks, stream);
     HIP_TRY(hipGetLastError());
     return TECS_OK;
   }
@@ -954,7 +953,7 @@ struct Batch {
     a.o_pol = d1p;
     sym_grid_args(&a, stale1);
     TRY(sweep(kModeDelta, a, kTimeSweepFull));
-    TRY(apply_extra(x[xc], d1));
+    if (has_extra()) TRY(apply_extra(x[xc], d1, true));
     if (integrator == TECS_HEUN) {
       TRY(com_and_update(kUpdatePred, d1, stale1));
       SweepArgs b = base_args(xp, polp);
@@ -962,7 +961,7 @@ struct Batch {
       b.o_pol = d3p;
       sym_grid_args(&b, sym_binned);
       TRY(sweep(kModeDelta, b, kTimeSweepStale));
-      TRY(apply_extra(xp, d3));
+      if (has_extra()) TRY(apply_extra(xp, d3, false));
       TRY(com_and_update(kUpdateCorr, d3, sym_binned));
     } else {
       TRY(com_and_update(kUpdateEuler, d1, stale1));
@@ -1371,25 +1370,49 @@ struct Batch {
     member_link_k.resize(n_inst, 0.0f);
     member_links[k].assign(links, links + 2 * count);
     member_link_k[k] = strength;
-    std::vector<int4> all;
+    h_inst[k].link_k = strength;
+    inst_dirty = true;
+    // every cell's incident links in link order (a counting sort of the endpoints)
+    std::vector<int> rows((size_t)total_slots + 1, 0);
     for (int m = 0; m < n_inst; ++m) {
-      int kb = 0;
-      std::memcpy(&kb, &member_link_k[m], sizeof(kb));
+      const int base = h_inst[m].slot_begin;
       const auto& L = member_links[m];
-      for (size_t q = 0; q + 1 < L.size(); q += 2) all.push_back(make_int4(L[q], L[q + 1], kb, m));
+      for (size_t q = 0; q + 1 < L.size(); q += 2) {
+        if (L[q] == L[q + 1]) continue;  // base.hpp:207-208
+        rows[(size_t)base + L[q] + 1]++;
+        rows[(size_t)base + L[q + 1] + 1]++;
+      }
     }
-    if ((int64_t)all.size() > links_cap) {
-      if (d_links) free_one(d_links);
-      d_links = nullptr;
-      links_cap = 0;
-      TRY(alloc(&d_links, all.size()));
-      links_cap = (int64_t)all.size();
+    for (size_t a = 0; a < (size_t)total_slots; ++a) rows[a + 1] += rows[a];
+    std::vector<int2> ent((size_t)rows[(size_t)total_slots]);
+    {
+      std::vector<int> fill(rows.begin(), rows.end() - 1);
+      for (int m = 0; m < n_inst; ++m) {
+        const int base = h_inst[m].slot_begin;
+        const auto& L = member_links[m];
+        for (size_t q = 0; q + 1 < L.size(); q += 2) {
+          const int a = L[q], b = L[q + 1];
+          if (a == b) continue;
+          ent[(size_t)fill[(size_t)base + a]++] = make_int2(b, 1);
+          ent[(size_t)fill[(size_t)base + b]++] = make_int2(a, -1);
+        }
+      }
     }
+    if ((int64_t)ent.size() > link_ent_cap) {
+      if (d_link_ent) free_one(d_link_ent);
+      d_link_ent = nullptr;
+      link_ent_cap = 0;
+      TRY(alloc(&d_link_ent, ent.size()));
+      link_ent_cap = (int64_t)ent.size();
+    }
+    if (!d_link_rows) TRY(alloc(&d_link_rows, (size_t)total_slots + 1));
     if (!inv) TRY(alloc(&inv, (size_t)total_slots));
-    if (!all.empty())
-      HIP_TRY(hipMemcpyAsync(d_links, all.data(), sizeof(int4) * all.size(), hipMemcpyHostToDevice, stream));
+    TRY(sync_inst());
+    HIP_TRY(hipMemcpyAsync(d_link_rows, rows.data(), sizeof(int) * rows.size(), hipMemcpyHostToDevice, stream));
+    if (!ent.empty())
+      HIP_TRY(hipMemcpyAsync(d_link_ent, ent.data(), sizeof(int2) * ent.size(), hipMemcpyHostToDevice, stream));
     HIP_TRY(hipStreamSynchronize(stream));
-    n_links = (int)all.size();
+    n_links = (int)ent.size();
     return TECS_OK;
   }
 
@@ -1459,7 +1482,7 @@ struct Batch {
       inst_dirty = true;
       TRY(sync_inst());
     }
-    if (old.n_links > 0)
+    if (!old.member_links.empty() && !old.member_links[0].empty())
       TRY(set_links(0, old.member_links[0].data(), (int64_t)(old.member_links[0].size() / 2),
                     old.member_link_k[0]));
     variant = old.variant;

@@ -4,26 +4,20 @@
 //
 // Within an evaluation the reference applies the generic forces before the pairwise force and
 // the link forces after it (fuse_forces, kernel_fuser.hpp:148-156). The pair sweep writes its
-// result F = (0 + acc) + 0 into the delta buffer; k_walls then forms the walls' sum W the way
+// result F = (0 + acc) + 0 into the delta buffer; k_extra then forms the walls' sum W the way
 // the reference's delta does (from zero, wall after wall) and stores W + F, which is bitwise
-// (W + acc) + 0 (addition commutes, and neither W nor F is -0). k_links adds each link's
-// contribution to both endpoints with float atomics, as the reference does.
+// (W + acc) + 0 (addition commutes, and neither W nor F is -0). The same thread then adds
+// the cell's link contributions, in link order (k_extra).
 #include "tecs_internal.hpp"
 
 namespace tecs_amd {
 namespace {
 
-__global__ __launch_bounds__(kBlock) void k_walls(const InstanceDesc* inst, const int* block_inst,
-                                                  const float4* xs, float4* d) {
-  const int lb = (int)blockIdx.x;
-  const InstanceDesc& in = inst[block_inst[lb]];
-  const int s = lb * kBlock + (int)threadIdx.x;
-  if (in.n_walls == 0 || s - in.slot_begin >= in.n) return;
-  const float4 p = xs[s];
-  float W[3] = {0.0f, 0.0f, 0.0f};
+// PlaneN::signed_distance (types/plane.hpp:36-42): d + a x + b y + c z, in that order; the
+// walls' sum W formed from zero, wall after wall.
+__device__ __forceinline__ void wall_sum(const InstanceDesc& in, float4 p, float* W) {
   for (int w = 0; w < in.n_walls; ++w) {
     const float* c = in.wall[w];
-    // PlaneN::signed_distance (types/plane.hpp:36-42): d + a x + b y + c z, in that order
     float sd = c[3];
     sd = sd + c[0] * p.x;
     sd = sd + c[1] * p.y;
@@ -35,8 +29,53 @@ __global__ __launch_bounds__(kBlock) void k_walls(const InstanceDesc* inst, cons
       W[2] = W[2] + c[2] * f;
     }
   }
-  const float4 F = d[s];
-  d[s] = make_float4(W[0] + F.x, W[1] + F.y, W[2] + F.z, F.w);
+}
+
+// One thread per cell: the walls (W + F), then its incident links in link order, rows[slot_begin
+// + agent] .. rows[slot_begin + agent + 1] of ent (the other endpoint, +1 where the cell is the
+// link's a, -1 where it is its b). u = p_b - p_a, d = |u|; the a end adds 0 + (k u) / d, the b
+// end 0 + (-k u) / d, as the reference's zeroed pack per endpoint (base.hpp:210-226). Each end
+// forms u and d itself, with the same operations, so both see the same values. The reference
+// adds the packs with atomics in no fixed order; here each cell adds them in link order.
+template <bool LINKS>
+__global__ __launch_bounds__(kBlock) void k_extra(const InstanceDesc* inst, const int* block_inst,
+                                                  const int* id, const int* inv, const int* rows,
+                                                  const int2* ent, const float4* xs, float4* d) {
+  const int lb = (int)blockIdx.x;
+  const InstanceDesc& in = inst[block_inst[lb]];
+  const int s = lb * kBlock + (int)threadIdx.x;
+  if (s - in.slot_begin >= in.n) return;
+  int e0 = 0, e1 = 0;
+  if (LINKS) {
+    const int me = id[s];
+    e0 = rows[in.slot_begin + me];
+    e1 = rows[in.slot_begin + me + 1];
+  }
+  if (in.n_walls == 0 && e0 == e1) return;
+  const float4 p = xs[s];
+  float4 F = d[s];
+  if (in.n_walls > 0) {
+    float W[3] = {0.0f, 0.0f, 0.0f};
+    wall_sum(in, p, W);
+    F = make_float4(W[0] + F.x, W[1] + F.y, W[2] + F.z, F.w);
+  }
+  const float k = in.link_k;
+  for (int e = e0; e < e1; ++e) {
+    const int2 L = ent[e];
+    if (L.x >= in.n) continue;  // an endpoint beyond the active count
+    const float4 o = xs[inv[in.slot_begin + L.x]];
+    const bool is_a = L.y > 0;
+    // u = p_b - p_a
+    const float u0 = is_a ? o.x - p.x : p.x - o.x;
+    const float u1 = is_a ? o.y - p.y : p.y - o.y;
+    const float u2 = is_a ? o.z - p.z : p.z - o.z;
+    const float dist = sqrtf(len2(u0, u1, u2));
+    const float kk = is_a ? k : -k;
+    F.x = F.x + (0.0f + (kk * u0) / dist);
+    F.y = F.y + (0.0f + (kk * u1) / dist);
+    F.z = F.z + (0.0f + (kk * u2) / dist);
+  }
+  d[s] = F;
 }
 
 // inv[slot_begin + agent] = slot, over every slot of the batch (the ids of a padded range are a
@@ -49,30 +88,6 @@ __global__ __launch_bounds__(kBlock) void k_inverse_ids(const InstanceDesc* inst
   const int s = lb * kBlock + (int)threadIdx.x;
   const int a = id[s];
   if ((unsigned)a < (unsigned)in.padded) inv[in.slot_begin + a] = s;
-}
-
-// One thread per link: (instance, a, b, strength bits). u = p_b - p_a, d = |u|;
-// delta_a += 0 + (k u) / d, delta_b += 0 + (-k u) / d (a zeroed pack per endpoint).
-__global__ __launch_bounds__(kBlock) void k_links(const InstanceDesc* inst, const int4* links,
-                                                  int n_links, const int* inv,
-                                                  const float4* xs, float4* d) {
-  const int q = (int)blockIdx.x * kBlock + (int)threadIdx.x;
-  if (q >= n_links) return;
-  const int4 L = links[q];
-  const InstanceDesc& in = inst[L.w];
-  if (L.x == L.y || L.x >= in.n || L.y >= in.n) return;
-  const int sa = inv[in.slot_begin + L.x], sb = inv[in.slot_begin + L.y];
-  const float k = __int_as_float(L.z);
-  const float4 pa = xs[sa], pb = xs[sb];
-  const float u[3] = {pb.x - pa.x, pb.y - pa.y, pb.z - pa.z};
-  const float dist = sqrtf(len2(u[0], u[1], u[2]));
-  float* da = &d[sa].x;
-  float* db = &d[sb].x;
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    unsafeAtomicAdd(da + c, 0.0f + (k * u[c]) / dist);
-    unsafeAtomicAdd(db + c, 0.0f + (-k * u[c]) / dist);
-  }
 }
 
 // Daughter q of a division: parent agent index parents[q], direction dirs[3q..3q+2]; the
@@ -95,22 +110,21 @@ __global__ __launch_bounds__(kBlock) void k_divide(const InstanceDesc* inst, int
 
 }  // namespace
 
-void launch_walls(const InstanceDesc* inst, const int* block_inst, const float4* xs, float4* d,
-                  int n_blocks, hipStream_t stream) {
-  hipLaunchKernelGGL(k_walls, dim3(n_blocks), dim3(kBlock), 0, stream, inst, block_inst, xs, d);
-}
-
 void launch_inverse_ids(const InstanceDesc* inst, const int* block_inst, const int* id, int* inv,
                         int n_blocks, hipStream_t stream) {
   hipLaunchKernelGGL(k_inverse_ids, dim3(n_blocks), dim3(kBlock), 0, stream, inst, block_inst, id,
                      inv);
 }
 
-void launch_links(const InstanceDesc* inst, const int4* links, int n_links, const int* inv,
-                  const float4* xs, float4* d, hipStream_t stream) {
-  if (n_links <= 0) return;
-  hipLaunchKernelGGL(k_links, dim3((n_links + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, inst,
-                     links, n_links, inv, xs, d);
+void launch_extra(const InstanceDesc* inst, const int* block_inst, const int* id, const int* inv,
+                  const int* rows, const int2* ent, const float4* xs, float4* d, int n_blocks,
+                  hipStream_t stream) {
+  if (rows)
+    hipLaunchKernelGGL(k_extra<true>, dim3(n_blocks), dim3(kBlock), 0, stream, inst, block_inst,
+                       id, inv, rows, ent, xs, d);
+  else
+    hipLaunchKernelGGL(k_extra<false>, dim3(n_blocks), dim3(kBlock), 0, stream, inst, block_inst,
+                       id, inv, rows, ent, xs, d);
 }
 
 void launch_divide(const InstanceDesc* inst, int ii, const int* inv, const int* parents,

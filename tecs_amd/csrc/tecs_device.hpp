@@ -60,6 +60,7 @@ struct InstanceDesc {
   ForceParams force;
   int n_walls;           // generic wall forces: normalised (a, b, c, d) and strength
   float wall[kMaxWalls][5];
+  float link_k;          // link force strength
 };
 
 // Logical workgroup index with an XCD-aware remap: the hardware hands consecutive workgroup

@@ -238,15 +238,15 @@ struct QueryArgs {
 void launch_query(int kind, const QueryArgs& a, hipStream_t stream);
 
 // Generic wall forces, link forces and division (kernels_extra.hip).
-// d += the walls' sum W (as W + d) for the active cells of instances with walls
-void launch_walls(const InstanceDesc* inst, const int* block_inst, const float4* xs, float4* d,
-                  int n_blocks, hipStream_t stream);
 // inv[slot_begin + agent] = slot over all slots
 void launch_inverse_ids(const InstanceDesc* inst, const int* block_inst, const int* id, int* inv,
                         int n_blocks, hipStream_t stream);
-// links: (a, b, strength bits, instance), agent indices of the instance
-void launch_links(const InstanceDesc* inst, const int4* links, int n_links, const int* inv,
-                  const float4* xs, float4* d, hipStream_t stream);
+// per active cell: d = W + d for the instance's walls, then its link contributions (rows over
+// agent space slot_begin + agent, ent = (other endpoint, +1 for the link's a end, -1 for its b
+// end) in link order; rows null: no links)
+void launch_extra(const InstanceDesc* inst, const int* block_inst, const int* id, const int* inv,
+                  const int* rows, const int2* ent, const float4* xs, float4* d, int n_blocks,
+                  hipStream_t stream);
 void launch_divide(const InstanceDesc* inst, int ii, const int* inv, const int* parents,
                    const float* dirs, int count, int first, float offset, float4* x_cur,
                    float4* x_other, hipStream_t stream);
