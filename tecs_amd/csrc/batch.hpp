@@ -775,6 +775,9 @@ struct Batch {
       k.skey = skey[ic];
       k.head = head[ic];
       k.nxt = nxt;
+      // the scan's tile sums move with the counts when the scan takes the fused path, which
+      // leaves them in bsum (with more tiles the top-level scan overwrites them)
+      if (scan_tiles_for(total_cbins) <= kScanFusedTileLimit) k.tsum = bsum;
     }
     return k;
   }

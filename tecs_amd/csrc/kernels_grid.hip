@@ -27,6 +27,7 @@ namespace {
 
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;
+static_assert(kScanTile == kScanTileSegments, "count_key keeps the sums of these tiles");
 
 __global__ __launch_bounds__(kBlock) void k_bin_keys(RebuildArgs a) {
   const int lb = logical_block();
@@ -302,7 +303,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_top(int* bsum, int nt, int* tot
 
 // Scans of at most this many tiles skip k_scan_top: every k_scan_apply_fused workgroup sums
 // the tile sums before its own tile (at most 4096 reads from L2 per workgroup).
-constexpr int kScanFusedTiles = 4096;
+constexpr int kScanFusedTiles = kScanFusedTileLimit;
 
 __global__ __launch_bounds__(kBlock) void k_scan_apply_fused(const int* in, int n, const int* bsum,
                                                              int nt, int* out, int* head_clear) {
@@ -479,23 +480,29 @@ void launch_copy_probe(const float4* src, float4* dst, int64_t n, hipStream_t st
 int scan_tiles_for(int n) { return (n + kScanTile - 1) / kScanTile; }
 
 // out[0..n] = exclusive scan of in[0..n), out[n] = total; head_clear (may be null) is set to -1
-// over the same range. bsum needs scan_tiles_for(n) ints.
+// over the same range. bsum needs scan_tiles_for(n) ints and holds the tiles' sums afterwards
+// when the scan fits the fused path; tile_sums_ready: it holds them already (count_key kept
+// them current), so the reduce pass is skipped.
 void launch_exclusive_scan(const int* in, int n, int* out, int* bsum, int* head_clear,
-                           hipStream_t stream) {
+                           hipStream_t stream, bool tile_sums_ready) {
   const int nt = scan_tiles_for(n);
-  hipLaunchKernelGGL(k_scan_reduce, dim3(nt), dim3(kBlock), 0, stream, in, n, bsum);
   if (nt <= kScanFusedTiles) {
+    if (!tile_sums_ready)
+      hipLaunchKernelGGL(k_scan_reduce, dim3(nt), dim3(kBlock), 0, stream, in, n, bsum);
     hipLaunchKernelGGL(k_scan_apply_fused, dim3(nt), dim3(kBlock), 0, stream, in, n, bsum, nt, out,
                        head_clear);
     return;
   }
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nt), dim3(kBlock), 0, stream, in, n, bsum);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kBlock), 0, stream, bsum, nt, out + n);
   hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(kBlock), 0, stream, in, n, bsum, out, head_clear);
 }
 
 void launch_rebuild(const RebuildArgs& a, int n_blocks, hipStream_t stream) {
   if (!a.keys_ready) hipLaunchKernelGGL(k_bin_keys, dim3(n_blocks), dim3(kBlock), 0, stream, a);
-  launch_exclusive_scan(a.kc.cnt, a.total_cbins, a.coff, a.scan_bsum, a.head_clear, stream);
+  // an incremental rebuild's counts moved with their tiles' sums (count_key)
+  launch_exclusive_scan(a.kc.cnt, a.total_cbins, a.coff, a.scan_bsum, a.head_clear, stream,
+                        a.incremental && a.kc.tsum != nullptr);
   const dim3 gf(n_blocks + (a.fill ? a.fill_blocks : 0));
   if (a.incremental) {
     hipLaunchKernelGGL(k_gather_fill<true>, gf, dim3(kBlock), 0, stream, a, n_blocks);

@@ -150,19 +150,24 @@ __device__ __forceinline__ int bin_key(const InstanceDesc& in, float4 p) {
 // rebuild: cnt starts at zero and every cell counts. Incremental rebuild: cnt holds the counts
 // of the last rebuild and only a cell whose segment changed moves its count (old segment - 1,
 // new segment + 1) and joins the new segment's arrival list head[] / nxt[].
+// Segments per tile of the rebuild's exclusive scan (256 threads x 8 items).
+constexpr int kScanTileSegments = 2048;
+
 struct KeyCount {
   int* keys;        // key per slot (current order)
   int* cnt;         // per-segment counts
   const int* skey;  // incremental: key per slot at the last rebuild (null = full)
   int* head;
   int* nxt;
+  int* tsum;        // incremental: the scan's per-tile sums of cnt, kept current with it
 };
 
 // Step 1 of the rebuild (kernels_grid.hip) for one slot: store the key of the cell's position
 // and count it. Full rebuild: every cell counts in its segment (wavefront-aggregated atomics)
 // and the slot just past the instance's cells sets the pad segment's count. Incremental
 // rebuild: only a cell whose segment differs from its segment at the last rebuild moves its
-// count and joins its new segment's arrival list. Called by every lane of the wavefront;
+// count (and, between scan tiles, the tiles' sums, so the scan need not sum the tiles again)
+// and joins its new segment's arrival list. Called by every lane of the wavefront;
 // `cell` marks lanes that hold a cell, `pad` the lane of the slot just past the cells. No
 // atomic here returns a value the wavefront waits for, except the arrival-list exchange.
 __device__ __forceinline__ void count_key(const InstanceDesc& in, const KeyCount& k, int s,
@@ -176,6 +181,11 @@ __device__ __forceinline__ void count_key(const InstanceDesc& in, const KeyCount
     if (gn != go) {
       __hip_atomic_fetch_add(k.cnt + go, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_fetch_add(k.cnt + gn, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const int to = go / kScanTileSegments, tn = gn / kScanTileSegments;
+      if (k.tsum && to != tn) {
+        __hip_atomic_fetch_add(k.tsum + to, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(k.tsum + tn, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
       k.nxt[s] = atomicExch(k.head + gn, s);
     }
     return;

@@ -166,8 +166,10 @@ struct RebuildArgs {
 };
 void launch_rebuild(const RebuildArgs& a, int n_blocks, hipStream_t stream);
 int scan_tiles_for(int n);
+// Scans of at most this many tiles take the fused path (and leave the tiles' sums in bsum).
+constexpr int kScanFusedTileLimit = 4096;
 void launch_exclusive_scan(const int* in, int n, int* out, int* bsum, int* head_clear,
-                           hipStream_t stream);
+                           hipStream_t stream, bool tile_sums_ready = false);
 
 void launch_init_ids(const InstanceDesc* inst, const int* block_inst, int* id, int n_blocks,
                      hipStream_t stream);
