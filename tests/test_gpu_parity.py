@@ -538,3 +538,37 @@ def test_tiny_populations(gpu, oracle_lib, finder, n):
     xg, _, _ = g.download()
     xo, _, _ = o.download()
     assert np.array_equal(xg.view(np.uint32), xo.view(np.uint32))
+
+
+def _outside_cloud(finder):
+    """A 3000-cell ball centred at (16, 16, -17) under the default bounds +-20: about 1300 cells
+    lie outside the grid and collapse into its edge bins (grid.hpp:222-230 clamps the bin
+    coordinates), whose rows then hold ~20 cells a bin."""
+    w = W.relaxation(3000, finder=finder)
+    pos = (w.positions + np.float32([16.0, 16.0, -17.0])).astype(np.float32)
+    d = abi.copy_desc(w.desc)
+    d.pair_finder_settings.min_bounds = (-20.0, -20.0, -20.0)
+    d.pair_finder_settings.max_bounds = (20.0, 20.0, 20.0)
+    return w, d, pos
+
+
+@pytest.mark.parametrize("variant", VARIANTS, ids=VARIANT_IDS)
+@pytest.mark.parametrize("finder", [abi.BINNED_ALL_PAIRS, abi.BINNED_GABRIEL], ids=lambda f: abi.FINDER_NAMES[f])
+def test_cells_outside_the_grid_bounds(gpu, oracle_lib, finder, variant):
+    """Cells beyond the bounds: pairs on the fresh grid, forces, a 3-step Euler trajectory and
+    the pairs on the grid of its last rebuild, all bitwise the oracle's."""
+    w, d, pos = _outside_cloud(finder)
+    print(f"case outside bounds finder={abi.FINDER_NAMES[finder]} variant={variant}", flush=True)
+    g, o = _pair(oracle_lib, d, w, positions=pos, variant=variant)
+    pg = g.pairs(True)
+    assert pg[1].size > 0
+    _assert_same_csr(pg, o.pairs(True), "outside bounds, fresh grid")
+    fg, _ = g.evaluate_force(True)
+    fo, _ = o.evaluate_force(True)
+    assert np.array_equal(fg.view(np.uint32), fo.view(np.uint32)), "forces differ bitwise"
+    for s in (g, o):
+        s.take_step(w.dt, 3)
+    xg, _, _ = g.download()
+    xo, _, _ = o.download()
+    assert np.array_equal(xg.view(np.uint32), xo.view(np.uint32)), "trajectory differs"
+    _assert_same_csr(g.pairs(False), o.pairs(False), "outside bounds, stale grid")

@@ -286,3 +286,41 @@ def test_tiny_populations(gpu, oracle_lib, finder, n):
     xg, _, _ = g.download()
     xo, _, _ = o.download()
     assert np.array_equal(xg.view(np.uint32), xo.view(np.uint32))
+
+
+@pytest.mark.parametrize("finder", BINNED_SYM, ids=NAME)
+def test_cells_outside_the_grid_bounds(gpu, oracle_lib, finder):
+    """A 3000-cell ball centred at (16, 16, -17) under the default bounds +-20 (about 1300 cells
+    collapse into the edge bins, grid.hpp:222-230): pairs on the fresh grid bit-exact, forces
+    within 1e-5 of FP64, 3 Euler steps within the trajectory bar, then the pairs on the grid of
+    the last rebuild bit-exact."""
+    import oracle
+
+    w = W.relaxation(3000, finder=finder)
+    pos = (w.positions + np.float32([16.0, 16.0, -17.0])).astype(np.float32)
+    d = abi.copy_desc(w.desc)
+    d.pair_finder_settings.min_bounds = (-20.0, -20.0, -20.0)
+    d.pair_finder_settings.max_bounds = (20.0, 20.0, 20.0)
+    print(f"case sym outside bounds finder={NAME(finder)}", flush=True)
+    g, o = _pair(oracle_lib, d, w, positions=pos)
+    rg, cg = g.pairs(True)
+    ro, co = o.pairs(True)
+    assert cg.size > 0 and np.array_equal(rg, ro) and np.array_equal(cg, co)
+    fg, _ = g.evaluate_force(True)
+    f64, _, s64, _ = o.forces_f64(ro, co)
+    assert oracle.force_excess(fg, f64, s64, 1e-5) <= 0.0
+    for s in (g, o):
+        s.take_step(w.dt, 3)
+    xg, _, _ = g.download()
+    xo, _, _ = o.download()
+    _assert_close_states(xg, xo, "outside bounds, 3 steps")
+    # the stale grid is the same grid when the positions are the same; compare on the oracle's
+    for s in (g, o):
+        s.upload(xo, None, None)
+        s.rebuild()
+    moved = (xo + W.rng(78).uniform(-0.3, 0.3, xo.shape)).astype(np.float32)
+    for s in (g, o):
+        s.upload(moved, None, None)
+    rg, cg = g.pairs(False)
+    ro, co = o.pairs(False)
+    assert np.array_equal(rg, ro) and np.array_equal(cg, co)

@@ -194,3 +194,35 @@ def test_main_cpp_eleven_point_fixture(oracle_lib):
         counts = np.diff(rp)
         assert counts.tolist() == g["neighbour_counts"]
         assert len(got) // 2 == g["total_links"] == 16
+
+
+def _outside_cloud(finder):
+    """A 3000-cell ball centred at (16, 16, -17) under the default bounds +-20: about a third of
+    the cells lie outside the grid and collapse into its edge bins (grid.hpp:222-230 clamps the
+    bin coordinates)."""
+    w = W.relaxation(3000, finder=finder)
+    pos = (w.positions + np.float32([16.0, 16.0, -17.0])).astype(np.float32)
+    d = abi.copy_desc(w.desc)
+    d.pair_finder_settings.min_bounds = (-20.0, -20.0, -20.0)
+    d.pair_finder_settings.max_bounds = (20.0, 20.0, 20.0)
+    return w, d, pos
+
+
+def test_cells_outside_the_grid_collapse_into_edge_bins(oracle_lib):
+    """Pair set with cells beyond the bounds: (i, j) in P  <=>  d2 < r^2 and the clamped bins
+    of i (multiplication) and j (division) differ by at most 1 in every dimension."""
+    w, d, pos = _outside_cloud(abi.BINNED_ALL_PAIRS)
+    bi, ext = _bins_mul(pos, d)
+    bj, _ = _bins_div(pos, d)
+    outside = (np.abs(pos) > 20.0).any(axis=1)
+    assert outside.sum() > 500 and (bi == ext - 1).any() and (bi == 0).any()
+    rows = _rows(*_oracle_pairs(oracle_lib, d, w, positions=pos))
+    near = _brute_all_pairs(pos, 1.0)
+    for i in range(w.n):
+        ok = np.abs(bj[near[i]] - bi[i]).max(axis=1) <= 1
+        assert np.array_equal(np.sort(rows[i]), near[i][ok]), i
+    # clamping only draws bins together, so no pair within the cutoff is lost; the edge bins
+    # hold many cells (long candidate rows)
+    key = bj[:, 0] + ext[0] * (bj[:, 1] + ext[1] * bj[:, 2])
+    assert np.bincount(key).max() > 20
+    assert sum(len(r) for r in rows) == sum(len(r) for r in near)
