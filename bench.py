@@ -114,12 +114,14 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the streaming copy kernel used to calibrate HBM byte counters")
     ap.add_argument("--workload", default="epithelium",
-                    choices=["epithelium", "relaxation", "sorting", "sweep", "serve", "spheroid", "queries"],
+                    choices=["epithelium", "relaxation", "sorting", "sweep", "serve", "spheroid", "queries",
+                             "snapshots"],
                     help="epithelium (default): 200k cells, BinnedGabriel + polarity, Heun; "
                     "relaxation: 10k cells, BinnedAllPairs, Euler; sorting: 4M cells, "
                     "BinnedAllPairs + type table, Heun; sweep: 256 mixed-size instances, batched "
                     "vs one simulation at a time; spheroid: NaiveDelaunay spheroids of 100 cells, "
-                    "batched; queries: grid queries on the epithelium's grid")
+                    "batched; queries: grid queries on the epithelium's grid; snapshots: the sorting "
+                    "tissue stepped with HDF5 snapshots, asynchronous hand-off vs download + write")
     ap.add_argument("--instances", type=int, default=256, help="spheroid workload: batch size")
     ap.add_argument("--walls", action="store_true",
                     help="epithelium / relaxation / sorting with a generic wall force (a floor just "
@@ -132,6 +134,8 @@ def main():
     ap.add_argument("--requests", type=int, default=48, help="serve workload: number of requests")
     ap.add_argument("--rate", type=float, default=8.0, help="serve workload: arrivals per second")
     ap.add_argument("--no-plain", action="store_true", help="serve workload: skip the one-at-a-time run")
+    ap.add_argument("--snapshot-every", type=int, default=50,
+                    help="snapshots workload: steps between snapshots")
     ap.add_argument("--symmetric", action="store_true",
                     help="epithelium / relaxation / sorting with the symmetric variant of their pair "
                     "finder (one functor call per unordered pair)")
@@ -166,6 +170,8 @@ def main():
         return bench_spheroid(args)
     if args.workload == "queries":
         return bench_queries(args)
+    if args.workload == "snapshots":
+        return bench_snapshots(args)
     sym = {"finder": abi.BINNED_GABRIEL_SYMMETRIC if args.workload == "epithelium"
            else abi.BINNED_ALL_PAIRS_SYMMETRIC} if args.symmetric else {}
     if args.workload == "relaxation":
@@ -185,6 +191,92 @@ WORKLOAD_TEXT = {
     "sorting": "large single tissue: {n} cells uniform in a cube at 1.5 cells/unit^3, types A/B, "
     "BinnedAllPairs r_max=1, PiecewiseLinear(0.8,2,tau) tau_AB=0.3, Heun dt=0.05",
 }
+
+
+def bench_snapshots(args):
+    """Simulation::write on the 4M-cell sorting tissue (48 MB of positions a snapshot): K Heun
+    steps with no snapshot, with a snapshot every E steps through the asynchronous hand-off
+    (gather kernel, pinned copy on a side stream, writer thread; timed until the last file is
+    written), and with the synchronous pattern of the reference (download to the host, then
+    write the datasets, simulation.hpp:607-632)."""
+    import shutil
+    import tempfile
+
+    from tecs_amd import GpuSimulation, workloads
+    from tecs_amd.io import Writer
+
+    n = 4_000_000 if args.n == 200_000 else args.n
+    w = workloads.sorting(n)
+    K = args.steps if args.steps != 2000 else 500
+    E = max(1, args.snapshot_every)
+    tmp = tempfile.mkdtemp(prefix="tecs_snap_")
+    sim = GpuSimulation(w.desc)
+    sim.upload(w.positions, w.polarities, w.types)
+    sim.take_step(w.dt, max(1, args.warmup))
+    sim.synchronize()
+    out = {}
+    try:
+        for mode in ("none", "async", "sync", "none"):
+            print(f"bench: snapshots mode {mode}, {K} steps, every {E}", file=sys.stderr, flush=True)
+            writer = Writer(f"{tmp}/{mode}") if mode != "none" else None
+            n_snap = 0
+            t0 = time.perf_counter()
+            done = 0
+            while done < K:
+                k = min(E, K - done)
+                sim.take_step(w.dt, k)
+                done += k
+                if mode == "async":
+                    sim.write(writer, done * w.dt)
+                    n_snap += 1
+                elif mode == "sync":
+                    pos, pol, _ = sim.download()
+                    writer.write(done * w.dt, n_snap, pos, pol)
+                    n_snap += 1
+            sim.synchronize()
+            if writer is not None:
+                writer.flush()
+                writer.close()
+            t = time.perf_counter() - t0
+            out.setdefault(mode, []).append({"seconds": t, "snapshots": n_snap})
+            for f in os.listdir(tmp):
+                os.remove(os.path.join(tmp, f))
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    t_none = min(r["seconds"] for r in out["none"])
+    t_async = out["async"][0]["seconds"]
+    t_sync = out["sync"][0]["seconds"]
+    snaps = out["async"][0]["snapshots"]
+    mb = n * 12 / 1e6
+    print(f"bench: snapshots none {t_none * 1e3 / K:.3f} ms/step, async {t_async * 1e3 / K:.3f}, "
+          f"sync {t_sync * 1e3 / K:.3f}", file=sys.stderr, flush=True)
+    res = {
+        "metric": "ms per Heun step with a snapshot every E steps (asynchronous hand-off)",
+        "value": t_async * 1e3 / K,
+        "unit": "ms/step",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": t_async * 1e3 / K,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Philox, seed 2807)",
+        "config": {
+            "workload": f"large single tissue: {n} cells, BinnedAllPairs + type table, Heun dt=0.05, "
+            f"an HDF5 snapshot (positions {mb:.0f} MB) every {E} steps",
+            "snapshots": snaps,
+            "ms_per_step_no_snapshots": t_none * 1e3 / K,
+            "ms_per_step_async": t_async * 1e3 / K,
+            "ms_per_step_sync_download_write": t_sync * 1e3 / K,
+            "ms_per_snapshot_async": (t_async - t_none) * 1e3 / snaps,
+            "ms_per_snapshot_sync": (t_sync - t_none) * 1e3 / snaps,
+            "parallelism": "single GPU",
+        },
+        "roofline": None,
+    }
+    print(json.dumps(res), flush=True)
 
 
 def bench_single(args, w):
