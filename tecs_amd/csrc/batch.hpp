@@ -907,8 +907,13 @@ struct Batch {
     u.old_vel = track_ov ? ov[xc] : nullptr;
     u.q_pos = with_q ? q : nullptr;
     u.q_pol = with_q ? qp : nullptr;
+    // the update that ends a step counts the keys of the new positions when the next step
+    // rebuilds, as the fused sweep epilogues do
+    const bool emit = kind != kUpdatePred && emit_keys();
+    if (emit) u.kc = key_count();
     launch_update(kind, has_pol, u, n_blocks, stream);
     HIP_TRY(hipGetLastError());
+    if (kind != kUpdatePred) keys_ready = emit;
     return TECS_OK;
   }
 

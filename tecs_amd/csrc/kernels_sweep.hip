@@ -711,13 +711,10 @@ __global__ __launch_bounds__(kBlock) void k_com_finalize(const InstanceDesc* ins
   }
 }
 
+// The update of slot s (a cell); xnew: its new position (corrector, Euler).
 template <int KIND, bool HAS_POL>
-__global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
-  const int lb = logical_block();
-  const int ii = a.block_inst[lb];
-  const InstanceDesc& in = a.inst[ii];
-  const int s = lb * kBlock + (int)threadIdx.x;
-  if ((s - in.slot_begin) >= in.n) return;
+__device__ __forceinline__ void update_cell(const UpdateArgs& a, const InstanceDesc& in, int ii,
+                                            int s, float4& xnew) {
   const float4 com = a.com[ii];
   const float dt = in.dt;
   // a symmetric evaluation on a stale grid: the movers' contributions in q join the evaluated
@@ -759,7 +756,8 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
     const float s0 = d1f.x + d3.x, s1 = d1f.y + d3.y, s2 = d1f.z + d3.z;
     if (a.old_vel) a.old_vel[s] = make_float4(s0 * 0.5f, s1 * 0.5f, s2 * 0.5f, 0.0f);
     const float4 x = a.x[s];
-    a.x[s] = make_float4(x.x + s0 * 0.5f * dt, x.y + s1 * 0.5f * dt, x.z + s2 * 0.5f * dt, x.w);
+    xnew = make_float4(x.x + s0 * 0.5f * dt, x.y + s1 * 0.5f * dt, x.z + s2 * 0.5f * dt, x.w);
+    a.x[s] = xnew;
     if (HAS_POL) {
       float2 e3 = a.d3_pol[s];
       e3 = make_float2(e3.x + ql.x, e3.y + ql.y);  // ql = 0 without q
@@ -772,7 +770,8 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
     // euler.hpp:56-66
     if (a.old_vel) a.old_vel[s] = d1;
     const float4 x = a.x[s];
-    a.x[s] = make_float4(x.x + d1.x * dt, x.y + d1.y * dt, x.z + d1.z * dt, x.w);
+    xnew = make_float4(x.x + d1.x * dt, x.y + d1.y * dt, x.z + d1.z * dt, x.w);
+    a.x[s] = xnew;
     if (HAS_POL) {
       float2 e = a.d1_pol[s];
       e = make_float2(e.x + ql.x, e.y + ql.y);
@@ -782,6 +781,23 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
       a.trig[s] = polar_trig(np);
     }
   }
+}
+
+
+template <int KIND, bool HAS_POL>
+__global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
+  const int lb = logical_block();
+  const int ii = a.block_inst[lb];
+  const InstanceDesc& in = a.inst[ii];
+  const int s = lb * kBlock + (int)threadIdx.x;
+  const bool cell = (s - in.slot_begin) < in.n;
+  // the corrector and Euler updates count the next rebuild's keys when asked to, which every
+  // lane of the wavefront takes part in
+  const bool keys = KIND != kUpdatePred && a.kc.keys != nullptr;
+  if (!cell && !keys) return;
+  float4 xnew = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (cell) update_cell<KIND, HAS_POL>(a, in, ii, s, xnew);
+  if (keys) count_key(in, a.kc, s, cell, s - in.slot_begin == in.n, xnew);
 }
 
 }  // namespace

@@ -113,6 +113,8 @@ struct UpdateArgs {
   float4* old_vel;
   float4* q_pos;         // symmetric evaluation on a stale grid: movers' contributions to add to
   float2* q_pol;         // the evaluated deltas (then cleared); null otherwise
+  KeyCount kc;           // corrector / Euler: the next rebuild's keys of the new positions
+                         // (kc.keys null: not counted here)
 };
 enum UpdateKind { kUpdatePred = 0, kUpdateCorr = 1, kUpdateEuler = 2 };
 void launch_update(int kind, bool has_pol, const UpdateArgs& a, int n_blocks, hipStream_t stream);
