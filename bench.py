@@ -1,16 +1,20 @@
 #!/usr/bin/env python3
-"""Benchmark: pair-interactions per second per Heun step on one MI355X.
+"""Benchmark: pair-interactions per second per Heun step on one MI355X, batched vs plain.
 
-Default workload (the one-GPU polar-epithelium model): 200,000 cells on a jittered sphere of
-radius sqrt(N/4pi)*0.8 with polarities, BinnedGabriel (c = 0.8, r_max = 1), PiecewiseLinear
-(0.8, 2, 1) + bending + bidirectional polarity alignment, Heun dt = 0.05.
+Default workload (the batched parameter sweep): 256 independent simulations, cell counts
+log-uniform in [1k, 100k], relaxation (ball) or sorting (cube, A/B types) geometry, tau_AB
+linear in [0.1, 1], BinnedAllPairs r_max = 1, Heun dt = 0.05. A step is one Heun step of every
+instance: one launch per kernel for all of them, replayed from captured hipGraphs.
+value = pair interactions (force-functor invocations on ordered pairs, both evaluations) of
+all instances in the timed steps / timed wall time, state resident in HBM. The same instances
+are then run one simulation at a time ("plain"), each warmed past its graph capture, and the
+ratio is reported. A short served run (Poisson arrivals of the three models, continuous
+batching under an HBM budget) adds p50/p99 request latencies.
 
-A step is one Heun step (grid rebuild + two pair sweeps with the fused predictor/corrector).
-value = pair interactions (force-functor invocations on ordered pairs, both evaluations) in the
-timed steps / timed wall time, state resident in HBM. The roofline entry is for the dominant
-kernel class (a pair sweep), timed with HIP events on the engine's stream in a separate pass.
-cpu_baseline is the C++/OpenMP oracle on the host cores, on a bounded sample of the same
-workload.
+The roofline entry is for the dominant kernel class of the batched step (a pair sweep), timed
+with HIP events on the batch's stream in a pass right after the timed steps. cpu_baseline is
+the C++/OpenMP oracle on the host cores allowed to this process, on a bounded stratified sample
+of the same instances.
 """
 from __future__ import annotations
 
@@ -71,41 +75,77 @@ def algorithmic_bytes_per_cell(has_pol: bool, has_type: bool):
     return {"full": 3 * s + a, "stale": 4 * s + a}
 
 
-def cpu_baseline(w, seconds_target: float = 15.0):
-    import numpy as np  # noqa: F401
-
+def _cpu_setup():
+    """Thread count and binding of the CPU baseline: every core this process may run on
+    (OMP_NUM_THREADS when the environment sets it), OMP_PROC_BIND=close, OMP_PLACES=cores,
+    set before the OpenMP runtime of the oracle library starts. Returns (threads, host info)."""
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    threads = int(os.environ.get("TECS_CPU_THREADS", os.environ.get("OMP_NUM_THREADS", allowed)))
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     import oracle
 
-    threads = int(os.environ.get("TECS_CPU_THREADS", min(16, os.cpu_count() or 1)))
     oracle.set_threads(threads)
+    host = {"cpu_model": model, "cpus_allowed": allowed, "omp_proc_bind": os.environ["OMP_PROC_BIND"],
+            "omp_places": os.environ["OMP_PLACES"]}
+    return threads, host
+
+
+def _median(xs):
+    a = sorted(xs)
+    m = len(a) // 2
+    return a[m] if len(a) % 2 else 0.5 * (a[m - 1] + a[m])
+
+
+def cpu_baseline(w, seconds_target: float = 15.0, warmup: int = 10, min_steps: int = 100):
+    """The oracle on one workload: `warmup` untimed steps, then timed steps until
+    seconds_target has passed and at least min_steps ran (or 1000 steps); value = the median of
+    the per-step rates (pair interactions of the step / its time)."""
+    threads, host = _cpu_setup()
+    import oracle
+
     o = oracle.OracleSim(w.desc)
     o.upload(w.positions, w.polarities, w.types)
-    o.take_step(w.dt, 1)  # warm-up
-    s0 = o.stats()["pair_interactions"]
-    steps = 0
+    o.take_step(w.dt, warmup)
+    rates = []
     t0 = time.perf_counter()
     while True:
+        before = o.stats()["pair_interactions"]
+        t1 = time.perf_counter()
         o.take_step(w.dt, 1)
-        steps += 1
+        dt_s = time.perf_counter() - t1
+        rates.append((o.stats()["pair_interactions"] - before) / dt_s)
         el = time.perf_counter() - t0
-        if el >= seconds_target or steps >= 1000:
+        if (el >= seconds_target and len(rates) >= min_steps) or len(rates) >= 1000 or el >= 4 * seconds_target:
             break
-    pairs = o.stats()["pair_interactions"] - s0
     return {
-        "value": pairs / el,
+        "value": _median(rates),
         "unit": "pair-interactions/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{steps} Heun steps of the same {w.n}-cell workload after 1 warm-up step "
-        f"({el:.1f} s, C++/OpenMP oracle, {threads} threads)",
+        "sample": f"median over {len(rates)} timed Heun steps of the same {w.n}-cell workload after "
+        f"{warmup} warm-up steps ({el:.1f} s, C++/OpenMP oracle, {threads} threads)",
+        **host,
     }
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=200_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--sweep-variant", default="auto", choices=["auto", "thread", "team4", "team8", "team16"],
@@ -113,13 +153,13 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the streaming copy kernel used to calibrate HBM byte counters")
-    ap.add_argument("--workload", default="epithelium",
+    ap.add_argument("--workload", default="sweep",
                     choices=["epithelium", "relaxation", "sorting", "sweep", "serve", "spheroid", "queries",
                              "snapshots"],
-                    help="epithelium (default): 200k cells, BinnedGabriel + polarity, Heun; "
-                    "relaxation: 10k cells, BinnedAllPairs, Euler; sorting: 4M cells, "
-                    "BinnedAllPairs + type table, Heun; sweep: 256 mixed-size instances, batched "
-                    "vs one simulation at a time; spheroid: NaiveDelaunay spheroids of 100 cells, "
+                    help="sweep (default): 256 mixed-size instances, batched vs one simulation at a "
+                    "time, plus a short served run; epithelium: 200k cells, BinnedGabriel + polarity, "
+                    "Heun; relaxation: 10k cells, BinnedAllPairs, Euler; sorting: 4M cells, "
+                    "BinnedAllPairs + type table, Heun; serve: served requests only; spheroid: NaiveDelaunay spheroids of 100 cells, "
                     "batched; queries: grid queries on the epithelium's grid; snapshots: the sorting "
                     "tissue stepped with HDF5 snapshots, asynchronous hand-off vs download + write")
     ap.add_argument("--instances", type=int, default=256, help="spheroid workload: batch size")
@@ -131,9 +171,11 @@ def main():
                     "and a random neighbour within distance 1 (strength 0.1)")
     ap.add_argument("--plain-steps", type=int, default=20,
                     help="sweep workload: steps per instance for the one-at-a-time comparison")
-    ap.add_argument("--requests", type=int, default=48, help="serve workload: number of requests")
-    ap.add_argument("--rate", type=float, default=8.0, help="serve workload: arrivals per second")
-    ap.add_argument("--no-plain", action="store_true", help="serve workload: skip the one-at-a-time run")
+    ap.add_argument("--requests", type=int, default=48, help="served requests: number of requests")
+    ap.add_argument("--rate", type=float, default=8.0, help="served requests: arrivals per second")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="sweep / serve: skip the one-simulation-at-a-time runs")
+    ap.add_argument("--no-serve", action="store_true", help="sweep: skip the served run")
     ap.add_argument("--snapshot-every", type=int, default=50,
                     help="snapshots workload: steps between snapshots")
     ap.add_argument("--symmetric", action="store_true",
@@ -303,8 +345,9 @@ def bench_single(args, w):
         b_idx = sim.random_neighbour(x[a_idx], 1.0, g.random(len(a_idx)).astype(np.float32))
         keep = b_idx >= 0
         sim.set_links(np.stack([a_idx[keep], b_idx[keep]], axis=1), 0.1)
-    print(f"bench: {w.name} n={w.n} warmup={args.warmup} steps={args.steps}", file=sys.stderr, flush=True)
-    sim.take_step(w.dt, args.warmup)
+    warm = max(args.warmup, MIN_WARMUP)
+    print(f"bench: {w.name} n={w.n} warmup={warm} steps={args.steps}", file=sys.stderr, flush=True)
+    sim.take_step(w.dt, warm)
     sim.synchronize()
     sim.reset_stats()
     t0 = time.perf_counter()
@@ -374,7 +417,7 @@ def bench_single(args, w):
         "unit": "pair-interactions/s",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": args.warmup,
+        "warmup": warm,
         "ms_per_step": ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",
@@ -504,37 +547,41 @@ def bench_queries(args):
     print(json.dumps(out), flush=True)
 
 
-def cpu_baseline_instances(ws, seconds_target: float):
-    """The oracle stepping the instances one after another (one step each, round robin) until
-    seconds_target has passed."""
+def cpu_baseline_instances(ws, seconds_target: float, warmup: int = 10, min_rounds: int = 100):
+    """The oracle stepping a set of instances, one Heun step of each per round (round robin):
+    `warmup` untimed rounds, then timed rounds until seconds_target has passed and at least
+    min_rounds ran; value = the median over rounds of (pair interactions of the round / its
+    time)."""
+    threads, host = _cpu_setup()
     import oracle
 
-    threads = int(os.environ.get("TECS_CPU_THREADS", min(16, os.cpu_count() or 1)))
-    oracle.set_threads(threads)
     sims = []
-    pairs = 0
-    steps = 0
+    for w in ws:
+        o = oracle.OracleSim(w.desc)
+        o.upload(w.positions, w.polarities, w.types)
+        o.take_step(w.dt, warmup)
+        sims.append((o, w))
+    rates = []
     t0 = time.perf_counter()
     while True:
-        k = steps % len(ws)
-        if k == len(sims):
-            o = oracle.OracleSim(ws[k].desc)
-            o.upload(ws[k].positions, ws[k].polarities, ws[k].types)
-            sims.append(o)
-        before = sims[k].stats()["pair_interactions"]
-        sims[k].take_step(ws[k].dt, 1)
-        pairs += sims[k].stats()["pair_interactions"] - before
-        steps += 1
+        before = sum(o.stats()["pair_interactions"] for o, _ in sims)
+        t1 = time.perf_counter()
+        for o, w in sims:
+            o.take_step(w.dt, 1)
+        dt_s = time.perf_counter() - t1
+        rates.append((sum(o.stats()["pair_interactions"] for o, _ in sims) - before) / dt_s)
         el = time.perf_counter() - t0
-        if el >= seconds_target or steps >= 100000:
+        if (el >= seconds_target and len(rates) >= min_rounds) or len(rates) >= 100000 or el >= 4 * seconds_target:
             break
     return {
-        "value": pairs / el,
+        "value": _median(rates),
         "unit": "pair-interactions/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{steps} Heun steps over the first {len(sims)} instances of the same batch, "
-        f"from their initial state ({el:.1f} s, C++/OpenMP oracle, {threads} threads)",
+        "sample": f"median over {len(rates)} timed rounds (one Heun step of each of {len(sims)} "
+        f"instances, {sum(w.n for _, w in sims)} cells) after {warmup} warm-up rounds ({el:.1f} s, "
+        f"C++/OpenMP oracle, {threads} threads)",
+        **host,
     }
 
 
@@ -634,54 +681,94 @@ def bench_spheroid(args):
     print(json.dumps(out), flush=True)
 
 
+# untimed steps that always cover the capture of the step graphs (one period is 2 steps when
+# the grid is rebuilt every step; the first step builds the grid and runs the autotuner)
+MIN_WARMUP = 5
+
+
 def bench_sweep(args):
     """256 independent instances (sizes log-uniform in [1k, 100k], relaxation or sorting
     geometry, tau_AB swept over [0.1, 1]), Heun dt=0.05: batched (every kernel launched once per
-    substep for all instances) vs plain (each instance stepped alone, one after another)."""
-    from tecs_amd import GpuBatch, GpuSimulation, workloads
+    substep for all instances, hipGraph replay) vs plain (each instance stepped alone, one after
+    another, each warmed past its own graph capture)."""
+    from tecs_amd import GpuBatch, GpuSimulation, abi, workloads
 
     ws = workloads.sweep(256)
     total_cells = sum(w.n for w in ws)
-    print(f"bench: sweep of {len(ws)} instances, {total_cells} cells", file=sys.stderr, flush=True)
+    warm = max(args.warmup, MIN_WARMUP)
+    print(f"bench: sweep of {len(ws)} instances, {total_cells} cells, warmup {warm}, steps {args.steps}",
+          file=sys.stderr, flush=True)
     b = GpuBatch([w.desc for w in ws])
+    if args.sweep_variant != "auto":
+        b.set_sweep_variant({v: k for k, v in abi.SWEEP_NAMES.items()}[args.sweep_variant])
     for k, w in enumerate(ws):
         b.upload(k, w.positions, w.polarities, w.types)
     dts = [w.dt for w in ws]
-    b.take_step(dts, args.warmup)
+    b.take_step(dts, warm)
     b.synchronize()
     b.reset_stats()
+    g0 = b.graph_launches()
     t0 = time.perf_counter()
     b.take_step(dts, args.steps)
     b.synchronize()
     el = time.perf_counter() - t0
-    pairs = b.stats()["pair_interactions"]
+    replays = b.graph_launches() - g0
+    st = b.stats()
+    pairs = st["pair_interactions"]
     batched = pairs / el
-    print(f"bench: batched {args.steps} steps in {el:.3f} s", file=sys.stderr, flush=True)
-    # plain: one simulation on the GPU at a time, the same instances and state
-    plain_pairs = 0
+    print(f"bench: batched {args.steps} steps in {el:.4f} s ({replays} graph replays)", file=sys.stderr,
+          flush=True)
+    # kernel timing pass: HIP events around every sweep and rebuild on the batch's stream
+    k_steps = min(max(args.steps, 20), 200)
+    b.set_kernel_timing(True)
+    b.take_step(dts, k_steps)
+    b.synchronize()
+    full_ms, full_n = b.kernel_timing(abi.KT_SWEEP_FULL)
+    stale_ms, stale_n = b.kernel_timing(abi.KT_SWEEP_STALE)
+    reb_ms, reb_n = b.kernel_timing(abi.KT_REBUILD)
+    b.set_kernel_timing(False)
+    bpc = algorithmic_bytes_per_cell(False, True)
+    per = {"full": (full_ms / max(full_n, 1), bpc["full"]), "stale": (stale_ms / max(stale_n, 1), bpc["stale"])}
+    dom = max(per, key=lambda k: per[k][0])
+    avg_ms, b_cell = per[dom]
+    achieved = (b_cell * total_cells) / (avg_ms * 1e-3) / 1e9
+    variant = abi.SWEEP_NAMES.get(b.sweep_variant(), "?")
+    kname = "k_sweep_binned" if variant == "thread" else f"k_sweep_team<{variant[4:]}>"
+    kernel_names = {"full": f"{kname} BinnedAllPairs mode Pred (Heun substep 1), all instances",
+                    "stale": f"{kname} BinnedAllPairs mode Corr (Heun substep 2, stale grid), all instances"}
+    traffic = measured_traffic("sweep", variant, dom)
+    autotune = b.autotune_ms()
+    b.close()
+    print(f"bench: sweep full {per['full'][0]*1e3:.1f} us, stale {per['stale'][0]*1e3:.1f} us, "
+          f"rebuild {reb_ms/max(reb_n,1)*1e3:.1f} us per launch", file=sys.stderr, flush=True)
+
+    plain = None
     plain_time = 0.0
-    for w in ws:
-        s = GpuSimulation(w.desc)
-        s.upload(w.positions, w.polarities, w.types)
-        s.take_step(w.dt, 2)
-        s.synchronize()
-        s.reset_stats()
-        t1 = time.perf_counter()
-        s.take_step(w.dt, args.plain_steps)
-        s.synchronize()
-        plain_time += time.perf_counter() - t1
-        plain_pairs += s.stats()["pair_interactions"]
-        s.close()
-    plain = plain_pairs / plain_time
-    print(f"bench: plain {args.plain_steps} steps per instance in {plain_time:.3f} s",
-          file=sys.stderr, flush=True)
+    if not args.no_plain:
+        # plain: one simulation on the GPU at a time, the same instances and initial state
+        plain_pairs = 0
+        for w in ws:
+            s = GpuSimulation(w.desc)
+            s.upload(w.positions, w.polarities, w.types)
+            s.take_step(w.dt, warm)
+            s.synchronize()
+            s.reset_stats()
+            t1 = time.perf_counter()
+            s.take_step(w.dt, args.plain_steps)
+            s.synchronize()
+            plain_time += time.perf_counter() - t1
+            plain_pairs += s.stats()["pair_interactions"]
+            s.close()
+        plain = plain_pairs / plain_time
+        print(f"bench: plain {args.plain_steps} steps per instance in {plain_time:.3f} s", file=sys.stderr,
+              flush=True)
     out = {
-        "metric": "pair-interactions/sec per Heun step, batched vs plain GPU",
+        "metric": "pair-interactions/sec per Heun step, batched vs plain GPU; p50/p99 request latency",
         "value": batched,
         "unit": "pair-interactions/s",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": args.warmup,
+        "warmup": warm,
         "ms_per_step": 1e3 * el / max(args.steps, 1),
         "higher_is_better": True,
         "scaling": "weak",
@@ -689,20 +776,53 @@ def bench_sweep(args):
         "dtype": "f32",
         "data": "synthetic (seeded Philox, seed 2807)",
         "config": {
-            "workload": "batched parameter sweep: 256 instances, cells log-uniform in [1k,100k], "
-            "relaxation (ball) or sorting (cube, A/B) geometry, tau_AB linear in [0.1,1], "
-            "BinnedAllPairs, Heun dt=0.05",
+            "workload": "batched parameter sweep: 256 instances in one launch per kernel per substep, "
+            "cells log-uniform in [1k,100k], relaxation (ball, radius 0.5*cbrt(N)) or sorting (cube, "
+            "1.5 cells/unit^3, A/B types) geometry, tau_AB linear in [0.1,1], BinnedAllPairs r_max=1, "
+            "PiecewiseLinear(0.8,2,tau), Heun dt=0.05, hipGraph-captured steps",
             "cells": total_cells,
             "instances": len(ws),
+            "pair_interactions_per_step": pairs / max(args.steps, 1),
+            "graph_replays_timed": replays,
             "plain_value": plain,
-            "batched_over_plain": batched / plain,
+            "batched_over_plain": None if plain is None else batched / plain,
             "plain_steps_per_instance": args.plain_steps,
+            "plain_timed_s": plain_time,
             "parallelism": "single GPU",
         },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": kernel_names[dom],
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None if traffic is None else traffic["fetch"] + traffic["write"],
+            "traffic_fetch": None if traffic is None else traffic["fetch"],
+            "traffic_write": None if traffic is None else traffic["write"],
+            "traffic_over_algorithmic": None if traffic is None
+            else (traffic["fetch"] + traffic["write"]) / (b_cell * total_cells),
+            "avg_launch_us": avg_ms * 1e3,
+            "algorithmic_bytes_per_cell": b_cell,
+            "cells_per_launch": total_cells,
+            "sweep_full_us": per["full"][0] * 1e3,
+            "sweep_stale_us": per["stale"][0] * 1e3,
+            "rebuild_us": reb_ms / max(reb_n, 1) * 1e3,
+            "timing_launches": {"full": full_n, "stale": stale_n, "rebuild": reb_n},
+            "sweep_variant": variant,
+            "autotune_ms": autotune,
+        },
     }
+    if not args.no_serve:
+        print("bench: served requests ...", file=sys.stderr, flush=True)
+        sv = serve_run(workloads.serve_requests(args.requests, args.rate), args.rate, with_plain=not args.no_plain)
+        out["serve"] = sv
+        out["latency_p50_s"] = sv["latency_p50_s"]
+        out["latency_p99_s"] = sv["latency_p99_s"]
     if not args.no_cpu_baseline:
         print("bench: cpu baseline ...", file=sys.stderr, flush=True)
-        cb = cpu_baseline(ws[0] if ws[0].n > 20000 else max(ws, key=lambda w: w.n), args.cpu_seconds)
+        sample = ws[0::16] + ws[1::16]  # every 16th instance of each geometry
+        cb = cpu_baseline_instances(sample, args.cpu_seconds)
         out["cpu_baseline"] = cb
         out["gpu_over_cpu"] = batched / cb["value"]
     print(json.dumps(out), flush=True)
@@ -715,17 +835,14 @@ def _percentiles(xs):
     return float(np.percentile(a, 50)), float(np.percentile(a, 99))
 
 
-def bench_serve(args):
-    """Served simulation requests: Poisson arrivals, models relaxation / epithelium / sorting
-    drawn uniformly, cells log-uniform in [2k, 500k], steps uniform in [100, 5000], admission by
-    an HBM budget of 200 GB, continuous batching; compared with serving the same requests one
-    at a time on the GPU (each runs alone, in arrival order)."""
-    from tecs_amd import GpuSimulation, workloads
+def serve_run(reqs, rate: float, with_plain: bool = True, chunk: int = 32) -> dict:
+    """Serve the requests through the continuous-batching server (arrivals replayed in real
+    time, HBM budget 200 GB) and, with_plain, the same requests one at a time in arrival order
+    (each alone on the GPU). Latency = completion time - arrival time."""
+    from tecs_amd import GpuSimulation
     from tecs_amd.serving import Server
 
-    reqs = workloads.serve_requests(args.requests, args.rate)
-    print(f"bench: serve {len(reqs)} requests at {args.rate}/s", file=sys.stderr, flush=True)
-    srv = Server(200 * 10**9, max_chunk_steps=32)
+    srv = Server(200 * 10**9, max_chunk_steps=chunk)
     t0 = time.perf_counter()
     ids = []
     nxt = 0
@@ -750,37 +867,25 @@ def bench_serve(args):
     makespan = time.perf_counter() - t0
     st = srv.stats()
     lat = [srv.info(rid)["t_done"] - srv.info(rid)["t_submit"] for rid in ids]
+    srv.close()
     p50, p99 = _percentiles(lat)
     served = st["pair_interactions"] / makespan
-    print(f"bench: served makespan {makespan:.2f} s p50 {p50:.2f} s p99 {p99:.2f} s",
-          file=sys.stderr, flush=True)
+    print(f"bench: served makespan {makespan:.2f} s p50 {p50:.2f} s p99 {p99:.2f} s", file=sys.stderr, flush=True)
     out = {
-        "metric": "pair-interactions/sec per Heun step, served; p50/p99 request latency",
+        "workload": f"served requests: {len(reqs)} Poisson arrivals at {rate}/s, model "
+        "relaxation/epithelium/sorting uniform, cells log-uniform [2k,500k], steps uniform "
+        f"[100,5000], HBM budget 200 GB, continuous batching (chunks of <= {chunk} steps)",
         "value": served,
         "unit": "pair-interactions/s",
-        "n_gpus": 1,
-        "steps": int(sum(r.n_steps for r in reqs)),
-        "warmup": 0,
-        "ms_per_step": None,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded Philox, seed 2807)",
-        "config": {
-            "workload": f"served requests: {len(reqs)} Poisson arrivals at {args.rate}/s, model "
-            "relaxation/epithelium/sorting uniform, cells log-uniform [2k,500k], steps uniform "
-            "[100,5000], HBM budget 200 GB, continuous batching (chunks of <= 32 steps)",
-            "requests": len(reqs),
-            "cells_total": int(sum(r.workload.n for r in reqs)),
-            "makespan_s": makespan,
-            "latency_p50_s": p50,
-            "latency_p99_s": p99,
-            "regroups": st["regroups"],
-            "parallelism": "single GPU",
-        },
+        "requests": len(reqs),
+        "cells_total": int(sum(r.workload.n for r in reqs)),
+        "steps_total": int(sum(r.n_steps for r in reqs)),
+        "makespan_s": makespan,
+        "latency_p50_s": p50,
+        "latency_p99_s": p99,
+        "regroups": st["regroups"],
     }
-    if not args.no_plain:
+    if with_plain:
         plain_pairs = 0
         t1 = time.perf_counter()
         lat_plain = []
@@ -805,12 +910,39 @@ def bench_serve(args):
             m["pair_interactions"] += pairs_r
         plain_span = time.perf_counter() - t1
         q50, q99 = _percentiles(lat_plain)
-        out["config"].update({"plain_value": plain_pairs / plain_span, "plain_makespan_s": plain_span,
-                              "plain_latency_p50_s": q50, "plain_latency_p99_s": q99,
-                              "served_over_plain": served / (plain_pairs / plain_span),
-                              "plain_by_model": by_model})
-        print(f"bench: plain makespan {plain_span:.2f} s p50 {q50:.2f} s p99 {q99:.2f} s",
-              file=sys.stderr, flush=True)
+        out.update({"plain_value": plain_pairs / plain_span, "plain_makespan_s": plain_span,
+                    "plain_latency_p50_s": q50, "plain_latency_p99_s": q99,
+                    "served_over_plain": served / (plain_pairs / plain_span), "plain_by_model": by_model})
+        print(f"bench: plain makespan {plain_span:.2f} s p50 {q50:.2f} s p99 {q99:.2f} s", file=sys.stderr,
+              flush=True)
+    return out
+
+
+def bench_serve(args):
+    """Served simulation requests: Poisson arrivals, models relaxation / epithelium / sorting
+    drawn uniformly, cells log-uniform in [2k, 500k], steps uniform in [100, 5000], admission by
+    an HBM budget of 200 GB, continuous batching; compared with serving the same requests one
+    at a time on the GPU (each runs alone, in arrival order)."""
+    from tecs_amd import workloads
+
+    reqs = workloads.serve_requests(args.requests, args.rate)
+    print(f"bench: serve {len(reqs)} requests at {args.rate}/s", file=sys.stderr, flush=True)
+    sv = serve_run(reqs, args.rate, with_plain=not args.no_plain)
+    out = {
+        "metric": "pair-interactions/sec per Heun step, served; p50/p99 request latency",
+        "value": sv["value"],
+        "unit": "pair-interactions/s",
+        "n_gpus": 1,
+        "steps": sv["steps_total"],
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Philox, seed 2807)",
+        "config": {**{k: v for k, v in sv.items() if k not in ("value", "unit")}, "parallelism": "single GPU"},
+    }
     print(json.dumps(out), flush=True)
 
 

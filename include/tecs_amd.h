@@ -291,10 +291,11 @@ int tecs_sim_get_sweep_variant(tecs_sim* sim, int32_t* variant);
 /* The autotuner's best time in ms per variant, indexed by TECS_SWEEP_* (0 = not timed). */
 int tecs_sim_get_autotune(tecs_sim* sim, double* ms_per_variant /* [6] */);
 
-/* hipGraph replay of step sequences inside tecs_sim_take_step. AUTO (default) captures and
- * replays when a call runs at least 4 graph periods of steps (a period is
- * 2 * max(1, rebuild_every_n) steps) and kernel timing is off; ON replays whenever a whole
- * period fits; OFF always launches kernel by kernel. Results are identical in every mode.
+/* hipGraph replay of step sequences inside tecs_sim_take_step. A period is
+ * 2 * max(1, rebuild_every_n) steps; a sequence of one period is captured once per starting
+ * state and replayed by every later call that starts there. AUTO (default) and ON replay
+ * whenever a whole period fits (AUTO only while kernel timing is off); OFF always launches
+ * kernel by kernel. Results are identical in every mode.
  * No counterpart in the reference. */
 enum { TECS_GRAPHS_AUTO = 0, TECS_GRAPHS_OFF = 1, TECS_GRAPHS_ON = 2 };
 int tecs_sim_set_graph_mode(tecs_sim* sim, int32_t mode);

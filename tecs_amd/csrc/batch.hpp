@@ -160,6 +160,9 @@ struct Batch {
   std::vector<InstanceDesc> h_inst;
   std::vector<int> h_block_inst;
   int n_inst = 0, n_blocks = 0, total_slots = 0, total_bins = 0, max_n = 0;
+  // cells the host staging buffers, the pair-export rows and the snapshot slots hold: the
+  // largest member slot range, which bounds every agent count set_capacity can allow
+  int stage_n = 0;
   int total_cbins = 0;         // counting-sort segments of all instances
   bool fill_bins = false;      // some instance counts multi-bin segments (sparse grid)
   std::vector<int> h_fill_inst, h_fill_base;
@@ -284,9 +287,18 @@ struct Batch {
   }
   int graph_period() const { return 2 * std::max(1, is_binned(finder) ? rebuild_every_n : 1); }
 
+  // Executable graphs that ran once and are not kept (a non-periodic capture); destroyed with
+  // the others once the stream is idle.
+  std::vector<hipGraphExec_t> retired;
+
+  // Destroys every executable graph. A launched graph may still be running on the stream, so
+  // the stream is drained first.
   void drop_graphs() {
+    if ((!graphs.empty() || !retired.empty()) && stream) hipStreamSynchronize(stream);
     for (auto& g : graphs) hipGraphExecDestroy(g.exec);
     graphs.clear();
+    for (hipGraphExec_t e : retired) hipGraphExecDestroy(e);
+    retired.clear();
   }
 
   ~Batch() { release(); }
@@ -432,6 +444,7 @@ struct Batch {
       // leave room for the fill's last 4-bin quad, which may reach bin_count + 3
       bins += (((int64_t)in.bin_count + 4 + 63) / 64) * 64;
       max_n = std::max(max_n, in.cap);
+      stage_n = std::max(stage_n, in.padded);
       if (slot > (int64_t)1 << 30 || bins > (int64_t)2000000000)
         return fail(TECS_ERR_INVALID, "batch too large for 32-bit slot/bin indices");
     }
@@ -560,7 +573,7 @@ struct Batch {
       if (has_pol) launch_polar_trig(pol[b], trig[b], total_slots, stream);
     }
     HIP_TRY(hipGetLastError());
-    const size_t M = (size_t)std::max(max_n, 1);
+    const size_t M = (size_t)std::max(stage_n, 1);
     TRY(alloc(&st_pos, 3 * M));
     TRY(alloc(&st_ov, 3 * M));
     if (has_pol) TRY(alloc(&st_pol, 2 * M));
@@ -1073,8 +1086,9 @@ struct Batch {
     out->delta.rebuilds = stats.rebuilds - before.rebuilds;
     out->delta.steps = stats.steps - before.steps;
     if (!(host_state() == start)) {  // not periodic from here: run it once, do not keep it
+      // (destroyed by drop_graphs once the stream is idle, not while the launch is in flight)
+      retired.push_back(exec);
       HIP_TRY(hipGraphLaunch(exec, stream));
-      hipGraphExecDestroy(exec);
       out->exec = nullptr;
     }
     return TECS_OK;
@@ -1093,8 +1107,10 @@ struct Batch {
       TRY(autotune());
     }
     const int P = graph_period();
+    // AUTO captures as soon as a call covers one period: a captured sequence is kept per
+    // starting state and replayed by every later call, so the capture cost is paid once
     const bool graphs_on = graph_mode == TECS_GRAPHS_ON ||
-                           (graph_mode == TECS_GRAPHS_AUTO && n_steps >= 4 * P);
+                           (graph_mode == TECS_GRAPHS_AUTO && n_steps >= P);
     if (graphs_on && !timing) {
       // the first step after creation or a manual rebuild may start a new rebuild phase
       while (done < n_steps && (step_count == 0 && is_binned(finder))) {
@@ -1147,6 +1163,7 @@ struct Batch {
     if ((uint64_t)first + count > d.capacity)
       return fail(TECS_ERR_INVALID, "cells beyond the capacity");
     const size_t n = count;
+    if (n > (size_t)stage_n) return fail(TECS_ERR_INVALID, "upload larger than the staging buffers");
     if (has_type && type && d.force.n_types > 0)
       for (size_t i = 0; i < n; ++i)
         if (type[i] >= (unsigned)d.force.n_types)
@@ -1452,7 +1469,7 @@ struct Batch {
     if (new_count) *new_count = (uint32_t)total;
     if (parents.empty()) return TECS_OK;
     if (!inv) TRY(alloc(&inv, (size_t)total_slots));
-    if (!st_idx) TRY(alloc(&st_idx, (size_t)std::max(max_n, 1)));
+    if (!st_idx) TRY(alloc(&st_idx, (size_t)std::max(stage_n, 1)));
     TRY(sync_inst());
     HIP_TRY(hipMemcpyAsync(st_idx, parents.data(), sizeof(int) * parents.size(), hipMemcpyHostToDevice, stream));
     HIP_TRY(hipMemcpyAsync(st_pos, dirs.data(), sizeof(float) * dirs.size(), hipMemcpyHostToDevice, stream));
@@ -1513,6 +1530,7 @@ struct Batch {
   int download(int k, float* pos, float* p, float* old_vel) {
     TRY(check_index(k));
     const size_t n = descs[k].agent_count;
+    if (n > (size_t)stage_n) return fail(TECS_ERR_STATE, "agent count beyond the staging buffers");
     const InstanceDesc& in = h_inst[k];
     const bool want_pol = p && has_pol;
     const bool want_ov = old_vel != nullptr;
@@ -1558,6 +1576,7 @@ struct Batch {
   int pairs(bool is_full_step, int64_t* row_ptr, int32_t* cols, int64_t cols_capacity,
             int64_t* n_pairs) {
     const size_t n = descs[0].agent_count;
+    if (n > (size_t)stage_n) return fail(TECS_ERR_STATE, "agent count beyond the staging buffers");
     if (!cols) {
       // size query: build the grid a due full evaluation would see, advance nothing
       if (is_binned(finder) && is_full_step && (step_count == 0 || step_count >= rebuild_every_n)) {
@@ -1615,7 +1634,8 @@ struct Batch {
     TRY(check_index(k));
     const InstanceDesc& in = h_inst[k];
     const size_t n = descs[k].agent_count;
-    const size_t cap = (size_t)std::max(max_n, 1);
+    const size_t cap = (size_t)std::max(stage_n, 1);
+    if (n > cap) return fail(TECS_ERR_STATE, "agent count beyond the snapshot buffers");
     if (!side) HIP_TRY(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
     SnapshotSlot* sl = nullptr;
     {

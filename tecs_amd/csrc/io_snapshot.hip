@@ -321,10 +321,12 @@ int Writer::flush(std::string* err) {
 // ----------------------------------------------------------------------------- SnapshotSlot
 void SnapshotSlot::release() {
   {
+    // notify while holding the owner's mutex: once it is released the owner may see every
+    // slot free, return from its teardown wait and destroy the condition variable
     std::lock_guard<std::mutex> g(*mu);
     busy = false;
+    cv->notify_all();
   }
-  cv->notify_all();
 }
 
 // --------------------------------------------------------------------------------- reading

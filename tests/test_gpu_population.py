@@ -158,3 +158,25 @@ def test_batch_refuses_shared_rebuild_period(gpu):
     with pytest.raises(TecsError) as e:
         b.set_agent_count(0, 500)
     assert e.value.code == abi.ERR_UNSUPPORTED
+
+
+def test_capacity_within_the_allocation_uses_every_slot(gpu, oracle_lib):
+    """A capacity raised to the member's whole slot range (2000 -> 2048, no reallocation) and
+    an agent count at that capacity: uploads, steps, downloads and the pair export cover all
+    2048 cells (the host staging buffers hold the slot range, not the creation capacity)."""
+    w = W.relaxation(2048)
+    d = abi.copy_desc(w.desc)
+    d.agent_count, d.capacity = 2000, 2000
+    print("case capacity 2000 -> 2048 within the allocation", flush=True)
+    g, o = _both(oracle_lib, d, w, 2000)
+    for s in (g, o):
+        s.take_step(w.dt, 3)
+        s.set_capacity(2048)
+        s.set_agent_count(2048)
+        s.upload_range(2000, w.positions[2000:])
+        s.take_step(w.dt, 3)
+    _same(g, o, "capacity 2048")
+    rg, cg = g.pairs(True)
+    ro, co = o.pairs(True)
+    assert rg.shape == (2049,)
+    assert np.array_equal(rg, ro) and np.array_equal(cg, co)
