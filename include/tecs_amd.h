@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define TECS_AMD_ABI_VERSION 2
+#define TECS_AMD_ABI_VERSION 3
 
 /* ---- status codes ---------------------------------------------------------------------- */
 enum {
@@ -88,19 +88,39 @@ typedef struct tecs_pair_finder_settings {
 
 #define TECS_MAX_TYPES 4
 
-/* The fused pairwise force, evaluated per ordered pair (i, j) in this order:
- *   position.delta += PiecewiseLinear(disp, d, homeostatic_radius, repulsion_scale, k_adh)
- *        with k_adh = adhesion_table[type_i * n_types + type_j] when n_types > 0,
- *        else adhesion_scale                                   (predefined.hpp:125-144)
- *   if bending_strength != 0:   b = polarity_i.bending_force(disp, polarity_j, d)
+/* Terms of the fused pairwise force (the forces given to Simulation::take_step). */
+enum {
+  TECS_TERM_END = 0,
+  TECS_TERM_PIECEWISE_LINEAR = 1, /* position.delta += PiecewiseLinear(...) */
+  TECS_TERM_BENDING = 2,          /* position.delta and polarity.delta += bending terms */
+  TECS_TERM_ALIGNMENT = 3         /* polarity.delta += alignment term */
+};
+
+/* The fused pairwise force, evaluated per ordered pair (i, j). Its terms:
+ *   PiecewiseLinear: position.delta += PiecewiseLinear(disp, d, homeostatic_radius,
+ *        repulsion_scale, k_adh) with k_adh = adhesion_table[type_i * n_types + type_j] when
+ *        n_types > 0, else adhesion_scale                        (predefined.hpp:125-144)
+ *   bending (bending_strength != 0): b = polarity_i.bending_force(disp, polarity_j, d)
  *        position.delta += bending_strength * b.vector
- *        polarity.delta += bending_strength * b.polarity        (polarity.hpp:181-198)
- *   if alignment_strength != 0:
+ *        polarity.delta += bending_strength * b.polarity         (polarity.hpp:181-198)
+ *   alignment (alignment_strength != 0):
  *        polarity.delta += alignment_strength *
  *             polarity_i.bidirectional_polarization_force(polarity_j)   (polarity.hpp:165-168)
- * With a strength of exactly 1 the product is exact, so the term equals the reference's plain
- * `delta += term`. Polarity terms need TECS_FIELD_POLARITY, a type table needs TECS_FIELD_TYPE.
- * No term writes drag, so the drag/velocity correction of the reference adds +0. */
+ * term_order lists the terms that are applied, in the order they add to the deltas, as the
+ * fused forces run in argument order (KernelFuser, kernel_fuser.hpp:40-55): TECS_TERM_* ids,
+ * each at most once, the list ending at the first TECS_TERM_END. All zeros (the default)
+ * means PiecewiseLinear, bending, alignment. A bending or alignment term with strength 0 is
+ * not applied. With a strength of exactly 1 the product is exact, so the term equals the
+ * reference's plain `delta += term`. Polarity terms need TECS_FIELD_POLARITY, a type table
+ * needs TECS_FIELD_TYPE.
+ *
+ * Drag (the functor's `drag` output, binned_all_pairs.hpp:155-180): each pair's drag is
+ * `drag`, or drag_table[type_i * n_types + type_j] when drag_by_type is 1 (the type-dependent
+ * `if (...) drag += 1.0` of examples/tribolium/main.cpp:113-115). The finder sums the drags
+ * and drag * old_velocity_j over the cell's pairs and adds
+ * total_velocity / fmax(total_drag, FLT_EPSILON) to the position delta. With drag 0
+ * everywhere that is +0. Drag needs track_old_velocities and an ordered finder (not the
+ * symmetric ones: TECS_ERR_UNSUPPORTED). */
 typedef struct tecs_force_desc {
   float homeostatic_radius;
   float repulsion_scale;
@@ -109,6 +129,10 @@ typedef struct tecs_force_desc {
   float adhesion_table[TECS_MAX_TYPES * TECS_MAX_TYPES];
   float bending_strength;
   float alignment_strength;
+  int32_t term_order[3];
+  float drag;
+  int32_t drag_by_type;
+  float drag_table[TECS_MAX_TYPES * TECS_MAX_TYPES];
 } tecs_force_desc;
 
 /* Everything Simulation<Config>(Settings) needs for this path (simulation.hpp:76-98, 133-182). */

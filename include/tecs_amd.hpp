@@ -166,33 +166,84 @@ struct DefaultSimulationConfig {
 /* ------------------------------------------------------------------ forces */
 namespace forces {
 
-// A pairwise force of the engine: the terms of tecs_force_desc, applied per ordered pair in the
-// order PiecewiseLinear, bending, alignment. `a + b` combines the terms of both (each term
-// may come from one side only).
+// A pairwise force of the engine: the terms of tecs_force_desc. `a + b` applies a's terms, then
+// b's, per ordered pair, as take_step(dt, a, b) runs its fused forces in argument order
+// (kernel_fuser.hpp:40-55); the order is kept in tecs_force_desc::term_order, so the sums are
+// the reference's for every composition. Each term may appear once.
 struct PairwiseForce {
   tecs_force_desc desc{};
-  bool has_linear = false;
+  int terms[3] = {TECS_TERM_END, TECS_TERM_END, TECS_TERM_END};
+  int n_terms = 0;
+  bool has_drag = false;
 
+  bool has(int t) const {
+    for (int k = 0; k < n_terms; ++k)
+      if (terms[k] == t) return true;
+    return false;
+  }
   PairwiseForce operator+(const PairwiseForce& o) const {
-    if (has_linear && o.has_linear) throw Error(TECS_ERR_INVALID, "two PiecewiseLinear terms");
-    if (desc.bending_strength != 0.0f && o.desc.bending_strength != 0.0f)
-      throw Error(TECS_ERR_INVALID, "two bending terms");
-    if (desc.alignment_strength != 0.0f && o.desc.alignment_strength != 0.0f)
-      throw Error(TECS_ERR_INVALID, "two alignment terms");
-    PairwiseForce r = has_linear ? *this : o;
-    r.has_linear = has_linear || o.has_linear;
-    r.desc.bending_strength = desc.bending_strength != 0.0f ? desc.bending_strength : o.desc.bending_strength;
-    r.desc.alignment_strength =
-        desc.alignment_strength != 0.0f ? desc.alignment_strength : o.desc.alignment_strength;
+    PairwiseForce r = *this;
+    for (int k = 0; k < o.n_terms; ++k) {
+      const int t = o.terms[k];
+      if (r.has(t)) throw Error(TECS_ERR_INVALID, "a force term appears twice in the composition");
+      r.terms[r.n_terms++] = t;
+      if (t == TECS_TERM_PIECEWISE_LINEAR) {
+        r.desc.homeostatic_radius = o.desc.homeostatic_radius;
+        r.desc.repulsion_scale = o.desc.repulsion_scale;
+        r.desc.adhesion_scale = o.desc.adhesion_scale;
+        if (o.desc.n_types > 0) {
+          if (r.desc.drag_by_type && r.desc.n_types != o.desc.n_types)
+            throw Error(TECS_ERR_INVALID, "drag and adhesion tables of different sizes");
+          r.desc.n_types = o.desc.n_types;
+          std::memcpy(r.desc.adhesion_table, o.desc.adhesion_table, sizeof(r.desc.adhesion_table));
+        }
+      } else if (t == TECS_TERM_BENDING) {
+        r.desc.bending_strength = o.desc.bending_strength;
+      } else {
+        r.desc.alignment_strength = o.desc.alignment_strength;
+      }
+    }
+    if (o.has_drag) {
+      if (r.has_drag) throw Error(TECS_ERR_INVALID, "two drag terms");
+      r.has_drag = true;
+      r.desc.drag = o.desc.drag;
+      r.desc.drag_by_type = o.desc.drag_by_type;
+      std::memcpy(r.desc.drag_table, o.desc.drag_table, sizeof(r.desc.drag_table));
+      if (o.desc.drag_by_type) {
+        if (r.desc.n_types > 0 && r.desc.n_types != o.desc.n_types)
+          throw Error(TECS_ERR_INVALID, "drag and adhesion tables of different sizes");
+        r.desc.n_types = o.desc.n_types;
+      }
+    }
+    for (int k = 0; k < 3; ++k) r.desc.term_order[k] = k < r.n_terms ? r.terms[k] : TECS_TERM_END;
     return r;
   }
+  // the descriptor as tecs_sim_create / tecs_sim_set_force take it
+  tecs_force_desc descriptor() const {
+    tecs_force_desc d = desc;
+    for (int k = 0; k < 3; ++k) d.term_order[k] = k < n_terms ? terms[k] : TECS_TERM_END;
+    if (n_terms == 0) {
+      // an all-zero term_order means the default terms; a force without terms (drag alone) is
+      // a PiecewiseLinear with zero scales, whose +-0 contributions leave every sum unchanged
+      d.term_order[0] = TECS_TERM_PIECEWISE_LINEAR;
+      d.homeostatic_radius = d.repulsion_scale = d.adhesion_scale = 0.0f;
+    }
+    return d;
+  }
 };
+
+inline PairwiseForce single_term(int t) {
+  PairwiseForce f;
+  f.terms[0] = t;
+  f.n_terms = 1;
+  f.desc.term_order[0] = t;
+  return f;
+}
 
 // position.delta += PiecewiseLinear(disp, d, homeostatic_radius, repulsion, adhesion)
 inline PairwiseForce PiecewiseLinear(float homeostatic_radius, float repulsion_scale,
                                      float adhesion_scale) {
-  PairwiseForce f;
-  f.has_linear = true;
+  PairwiseForce f = single_term(TECS_TERM_PIECEWISE_LINEAR);
   f.desc.homeostatic_radius = homeostatic_radius;
   f.desc.repulsion_scale = repulsion_scale;
   f.desc.adhesion_scale = adhesion_scale;
@@ -222,15 +273,41 @@ inline PairwiseForce DifferentialAdhesion(float homeostatic_radius, float repuls
 // position.delta += s * b.vector, polarity.delta += s * b.polarity with
 // b = polarity_i.bending_force(disp, polarity_j, d); needs fields::polarity.
 inline PairwiseForce Bending(float strength = 1.0f) {
-  PairwiseForce f;
+  PairwiseForce f = single_term(TECS_TERM_BENDING);
   f.desc.bending_strength = strength;
   return f;
 }
 
 // polarity.delta += s * polarity_i.bidirectional_polarization_force(polarity_j).
 inline PairwiseForce Alignment(float strength = 1.0f) {
-  PairwiseForce f;
+  PairwiseForce f = single_term(TECS_TERM_ALIGNMENT);
   f.desc.alignment_strength = strength;
+  return f;
+}
+
+// drag += value for every pair (examples/intercalation.cpp:62, turing.cpp:68): the finder adds
+// sum(drag * old_velocity_j) / fmax(sum(drag), FLT_EPSILON) to the position delta
+// (binned_all_pairs.hpp:168-180). Needs an ordered pair finder.
+inline PairwiseForce Drag(float value = 1.0f) {
+  PairwiseForce f;
+  f.has_drag = true;
+  f.desc.drag = value;
+  return f;
+}
+
+// drag += table[type_i][type_j] (the type-dependent drag of examples/tribolium/main.cpp:113-115);
+// needs fields::type and, with an adhesion table, one of the same size.
+inline PairwiseForce DragByType(const std::vector<std::vector<float>>& table) {
+  const size_t n = table.size();
+  if (n == 0 || n > TECS_MAX_TYPES) throw Error(TECS_ERR_INVALID, "drag table size");
+  PairwiseForce f;
+  f.has_drag = true;
+  f.desc.drag_by_type = 1;
+  f.desc.n_types = (int32_t)n;
+  for (size_t i = 0; i < n; ++i) {
+    if (table[i].size() != n) throw Error(TECS_ERR_INVALID, "drag table must be square");
+    for (size_t j = 0; j < n; ++j) f.desc.drag_table[i * n + j] = table[i][j];
+  }
   return f;
 }
 
@@ -377,9 +454,10 @@ class Simulation {
 
   // Simulation::take_step(dt, force) (simulation.hpp:548-556). Asynchronous.
   void take_step(double dt, const forces::PairwiseForce& force, int n_steps = 1) {
-    if (std::memcmp(&force.desc, &force_, sizeof(force_)) != 0) {
-      check(tecs_sim_set_force(h_, &force.desc));
-      force_ = force.desc;
+    const tecs_force_desc d = force.descriptor();
+    if (std::memcmp(&d, &force_, sizeof(force_)) != 0) {
+      check(tecs_sim_set_force(h_, &d));
+      force_ = d;
     }
     check(tecs_sim_take_step(h_, dt, n_steps));
   }

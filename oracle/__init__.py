@@ -70,6 +70,7 @@ def load():
         "orc_set_walls": (C.c_int, [P, P, C.c_int]),
         "orc_set_links": (C.c_int, [P, P, C.c_int64, C.c_float]),
         "orc_divide": (C.c_int, [P, C.c_float, C.c_float, P, C.POINTER(C.c_uint)]),
+        "orc_set_force": (C.c_int, [P, C.POINTER(abi.ForceDesc)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -143,6 +144,11 @@ class OracleSim:
 
     def set_capacity(self, capacity: int):
         _check(self._lib.orc_set_capacity(self._h, int(capacity)))
+
+    def set_force(self, force):
+        """The tecs_force_desc (abi.ForceDesc) of the following steps."""
+        _check(self._lib.orc_set_force(self._h, C.byref(force)))
+        C.memmove(C.addressof(self.desc.force), C.addressof(force), C.sizeof(abi.ForceDesc))
 
     def set_walls(self, walls):
         """walls: rows (a, b, c, d, strength)."""

@@ -160,33 +160,63 @@ struct Force {
   int n_types;
   float table[TECS_MAX_TYPES * TECS_MAX_TYPES];
   float bend, align;
+  int order[3];  // TECS_TERM_* in the order the fused forces run, TECS_TERM_END after the last
+  float drag;
+  int drag_by_type;
+  float drag_table[TECS_MAX_TYPES * TECS_MAX_TYPES];
 };
+
+// The force of a descriptor: the terms in term_order (all zeros = PiecewiseLinear, bending,
+// alignment), as take_step(dt, f1, f2, ...) fuses f1, f2, ... in argument order
+// (kernel_fuser.hpp:40-55).
+void load_force(Force& o, const tecs_force_desc& f) {
+  o.h = f.homeostatic_radius;
+  o.krep = f.repulsion_scale;
+  o.kadh = f.adhesion_scale;
+  o.n_types = f.n_types;
+  std::memcpy(o.table, f.adhesion_table, sizeof(o.table));
+  o.bend = f.bending_strength;
+  o.align = f.alignment_strength;
+  const bool dflt = f.term_order[0] == 0 && f.term_order[1] == 0 && f.term_order[2] == 0;
+  for (int k = 0; k < 3; ++k) o.order[k] = dflt ? k + 1 : f.term_order[k];
+  o.drag = f.drag;
+  o.drag_by_type = f.drag_by_type;
+  std::memcpy(o.drag_table, f.drag_table, sizeof(o.drag_table));
+}
 
 struct Acc {
   float pos[3];
   float pol[2];
 };
 
-// One invocation of the fused pairwise functor (kernel_fuser.hpp:40-55 applies the fused
-// forces in argument order; the config forces are listed in tecs_amd.h).
-inline void apply_force(const Force& f, const float* disp, float d, const float* pol_i,
-                        const float* pol_j, int type_i, int type_j, Acc& acc) {
-  float kadh = f.kadh;
-  if (f.n_types > 0) kadh = f.table[type_i * f.n_types + type_j];
-  float v[3];
-  piecewise_linear(disp, d, f.h, f.krep, kadh, v);
-  for (int c = 0; c < 3; ++c) acc.pos[c] += v[c];
-  if (f.bend != 0.0f) {
-    float bv[3], bp[2];
-    pol_bending(pol_i, disp, pol_j, d, bv, bp);
-    for (int c = 0; c < 3; ++c) acc.pos[c] += f.bend * bv[c];
-    for (int c = 0; c < 2; ++c) acc.pol[c] += f.bend * bp[c];
+// One invocation of the fused pairwise functor: the forces in argument order, each adding its
+// terms to the pack (kernel_fuser.hpp:40-55); the config forces are listed in tecs_amd.h.
+// Returns the pair's drag (each force adds to the zeroed pairwise drag,
+// binned_all_pairs.hpp:155-166; here one drag value per pair).
+inline float apply_force(const Force& f, const float* disp, float d, const float* pol_i,
+                         const float* pol_j, int type_i, int type_j, Acc& acc) {
+  for (int k = 0; k < 3 && f.order[k] != TECS_TERM_END; ++k) {
+    const int term = f.order[k];
+    if (term == TECS_TERM_PIECEWISE_LINEAR) {
+      float kadh = f.kadh;
+      if (f.n_types > 0) kadh = f.table[type_i * f.n_types + type_j];
+      float v[3];
+      piecewise_linear(disp, d, f.h, f.krep, kadh, v);
+      for (int c = 0; c < 3; ++c) acc.pos[c] += v[c];
+    } else if (term == TECS_TERM_BENDING && f.bend != 0.0f) {
+      float bv[3], bp[2];
+      pol_bending(pol_i, disp, pol_j, d, bv, bp);
+      for (int c = 0; c < 3; ++c) acc.pos[c] += f.bend * bv[c];
+      for (int c = 0; c < 2; ++c) acc.pol[c] += f.bend * bp[c];
+    } else if (term == TECS_TERM_ALIGNMENT && f.align != 0.0f) {
+      float ap[2];
+      pol_bidirectional(pol_i, pol_j, ap);
+      for (int c = 0; c < 2; ++c) acc.pol[c] += f.align * ap[c];
+    }
   }
-  if (f.align != 0.0f) {
-    float ap[2];
-    pol_bidirectional(pol_i, pol_j, ap);
-    for (int c = 0; c < 2; ++c) acc.pol[c] += f.align * ap[c];
-  }
+  float drag = 0.0f;
+  drag += f.drag_by_type ? f.drag_table[type_i * f.n_types + type_j] : f.drag;
+  return drag;
 }
 
 // ------------------------------------------------------------------------------- the grid
@@ -280,13 +310,12 @@ inline void visit_pair(const Ctx& x, unsigned i, unsigned j, const float* disp, 
     row->push_back((int)j);
     return;
   }
-  float pairwise_drag = 0.0f;
   static const float zero2[2] = {0.0f, 0.0f};
   const float* pi = x.pol ? &x.pol[2 * (size_t)i] : zero2;
   const float* pj = x.pol ? &x.pol[2 * (size_t)j] : zero2;
   const int ti = x.type ? x.type[i] : 0;
   const int tj = x.type ? x.type[j] : 0;
-  apply_force(x.s->force, disp, d, pi, pj, ti, tj, acc);
+  const float pairwise_drag = apply_force(x.s->force, disp, d, pi, pj, ti, tj, acc);
   // binned_all_pairs.hpp:168-169
   total_drag += pairwise_drag;
   for (int c = 0; c < 3; ++c) total_vel[c] += pairwise_drag * x.old_vel[3 * (size_t)j + c];
@@ -812,14 +841,32 @@ int validate(const tecs_sim_desc* d) {
       !(d->fields & TECS_FIELD_POLARITY))
     return fail(TECS_ERR_INVALID, "polarity forces need TECS_FIELD_POLARITY");
   if (!(d->cutoff_distance > 0.0f)) return fail(TECS_ERR_INVALID, "cutoff must be > 0");
+  for (int k = 0; k < 3; ++k)
+    if (f.term_order[k] < TECS_TERM_END || f.term_order[k] > TECS_TERM_ALIGNMENT)
+      return fail(TECS_ERR_INVALID, "bad term in term_order");
+  if (f.drag_by_type && f.n_types == 0) return fail(TECS_ERR_INVALID, "drag by type needs a type table");
+  bool drag = f.drag != 0.0f;
+  for (int k = 0; f.drag_by_type && k < f.n_types * f.n_types; ++k) drag = drag || f.drag_table[k] != 0.0f;
+  if (drag && (d->pair_finder >= TECS_NAIVE_ALL_PAIRS_SYMMETRIC && d->pair_finder <= TECS_BINNED_GABRIEL_SYMMETRIC))
+    return fail(TECS_ERR_UNSUPPORTED, "drag with a symmetric pair finder is not restated");
   return TECS_OK;
 }
 
+bool term_on(const Force& f, int term) {
+  for (int k = 0; k < 3 && f.order[k] != TECS_TERM_END; ++k)
+    if (f.order[k] == term) return true;
+  return false;
+}
+
 // Double-precision value of the fused force of one pair (i as self, j as other) at the current
-// stage-0 state, and per component the sum of the absolute values of its terms.
+// stage-0 state, and per component the sum of the absolute values of its terms (the terms the
+// force applies; their order does not matter in this evaluation).
 void pair_force_f64(const Sim& s, long i, long j, double* vpos, double* vpol, double* apos,
                     double* apol) {
   const Force& f = s.force;
+  const bool use_linear = term_on(f, TECS_TERM_PIECEWISE_LINEAR);
+  const bool use_bend = term_on(f, TECS_TERM_BENDING) && f.bend != 0.0f;
+  const bool use_align = term_on(f, TECS_TERM_ALIGNMENT) && f.align != 0.0f;
   for (int c = 0; c < 3; ++c) vpos[c] = apos[c] = 0.0;
   for (int c = 0; c < 2; ++c) vpol[c] = apol[c] = 0.0;
   const double pi[3] = {s.pos[3 * i], s.pos[3 * i + 1], s.pos[3 * i + 2]};
@@ -828,7 +875,7 @@ void pair_force_f64(const Sim& s, long i, long j, double* vpos, double* vpol, do
   const double d = std::sqrt(disp[0] * disp[0] + disp[1] * disp[1] + disp[2] * disp[2]);
   double kadh = f.kadh;
   if (f.n_types > 0) kadh = f.table[s.type[i] * f.n_types + s.type[j]];
-  if (d != 0.0) {
+  if (d != 0.0 && use_linear) {
     const double F = (double)f.krep * std::fmax((double)f.h - d, 0.0) -
                      kadh * std::fmax(d - (double)f.h, 0.0);
     for (int c = 0; c < 3; ++c) {
@@ -853,7 +900,7 @@ void pair_force_f64(const Sim& s, long i, long j, double* vpos, double* vpol, do
     if (std::fabs(std::sin((float)a[0])) > FLT_EPSILON)
       r[1] = -std::sin(b[0]) * std::sin(a[1] - b[1]) / st;
   };
-  if (f.bend != 0.0f) {
+  if (use_bend) {
     double tv[3], ov[3];
     to_vec(li, tv);
     to_vec(lj, ov);
@@ -876,7 +923,7 @@ void pair_force_f64(const Sim& s, long i, long j, double* vpos, double* vpol, do
       apol[c] += std::fabs(v);
     }
   }
-  if (f.align != 0.0f) {
+  if (use_align) {
     double u[2];
     uni(li, lj, u);
     const double dd =
@@ -917,14 +964,7 @@ int orc_create(const tecs_sim_desc* desc, orc_sim** out) {
   s->cap = desc->capacity;
   s->has_pol = (desc->fields & TECS_FIELD_POLARITY) != 0;
   s->has_type = (desc->fields & TECS_FIELD_TYPE) != 0;
-  const tecs_force_desc& f = desc->force;
-  s->force.h = f.homeostatic_radius;
-  s->force.krep = f.repulsion_scale;
-  s->force.kadh = f.adhesion_scale;
-  s->force.n_types = f.n_types;
-  std::memcpy(s->force.table, f.adhesion_table, sizeof(s->force.table));
-  s->force.bend = f.bending_strength;
-  s->force.align = f.alignment_strength;
+  load_force(s->force, desc->force);
   // pair finder constructors, binned_all_pairs.hpp:27-39, binned_gabriel.hpp:28-42,
   // naive_gabriel.hpp:386-392
   const tecs_pair_finder_settings& ps = desc->pair_finder_settings;
@@ -965,6 +1005,20 @@ int orc_upload(orc_sim* h, const float* pos, const float* pol, const uint8_t* ty
   if (s.has_type && type && s.force.n_types > 0)
     for (unsigned i = 0; i < s.n; ++i)
       if (type[i] >= (unsigned)s.force.n_types) return fail(TECS_ERR_INVALID, "type out of range");
+  return TECS_OK;
+}
+
+// Simulation::take_step(dt, forces...) takes its forces with every call: the force of the
+// following steps.
+int orc_set_force(orc_sim* h, const tecs_force_desc* f) {
+  if (!h || !f) return fail(TECS_ERR_INVALID, "null argument");
+  Sim& s = *reinterpret_cast<Sim*>(h);
+  tecs_sim_desc d = s.desc;
+  d.force = *f;
+  const int rc = validate(&d);
+  if (rc != TECS_OK) return rc;
+  s.desc.force = *f;
+  load_force(s.force, *f);
   return TECS_OK;
 }
 

@@ -6,7 +6,7 @@ from __future__ import annotations
 
 import ctypes as C
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 # status codes
 OK = 0
@@ -69,6 +69,13 @@ KT_SWEEP_STALE = 1
 KT_REBUILD = 2
 
 
+# terms of the fused pairwise force (tecs_force_desc.term_order)
+TERM_END = 0
+TERM_PIECEWISE_LINEAR = 1
+TERM_BENDING = 2
+TERM_ALIGNMENT = 3
+
+
 class PairFinderSettings(C.Structure):
     _fields_ = [
         ("min_bounds", C.c_float * 3),
@@ -88,6 +95,10 @@ class ForceDesc(C.Structure):
         ("adhesion_table", C.c_float * (MAX_TYPES * MAX_TYPES)),
         ("bending_strength", C.c_float),
         ("alignment_strength", C.c_float),
+        ("term_order", C.c_int32 * 3),
+        ("drag", C.c_float),
+        ("drag_by_type", C.c_int32),
+        ("drag_table", C.c_float * (MAX_TYPES * MAX_TYPES)),
     ]
 
 
@@ -151,6 +162,9 @@ def make_desc(
     adhesion_table=None,
     bending_strength: float = 0.0,
     alignment_strength: float = 0.0,
+    term_order=None,
+    drag: float = 0.0,
+    drag_table=None,
     track_old_velocities: bool = True,
 ) -> SimDesc:
     """A descriptor with the reference defaults (Settings structs of the finders, RelaxForce's
@@ -187,8 +201,35 @@ def make_desc(
                 f.adhesion_table[a * n + b] = float(t[a, b])
     f.bending_strength = float(bending_strength)
     f.alignment_strength = float(alignment_strength)
+    set_terms(f, term_order=term_order, drag=drag, drag_table=drag_table)
     d.track_old_velocities = 1 if track_old_velocities else 0
     return d
+
+
+def set_terms(f: ForceDesc, term_order=None, drag: float = 0.0, drag_table=None) -> ForceDesc:
+    """The order of the force terms (a sequence of TERM_* ids, None = the default
+    PiecewiseLinear, bending, alignment) and the drag (a constant, or a type-pair table)."""
+    order = list(term_order or [])
+    if len(order) > 3:
+        raise ValueError("at most three terms")
+    for k in range(3):
+        f.term_order[k] = int(order[k]) if k < len(order) else TERM_END
+    f.drag = float(drag)
+    f.drag_by_type = 0
+    for k in range(MAX_TYPES * MAX_TYPES):
+        f.drag_table[k] = 0.0
+    if drag_table is not None:
+        import numpy as np
+
+        t = np.asarray(drag_table, dtype=np.float32)
+        n = t.shape[0]
+        if t.shape != (n, n) or n != f.n_types:
+            raise ValueError("drag_table must be square with one row per cell type of the adhesion table")
+        f.drag_by_type = 1
+        for a in range(n):
+            for b in range(n):
+                f.drag_table[a * n + b] = float(t[a, b])
+    return f
 
 
 def copy_desc(d: SimDesc) -> SimDesc:

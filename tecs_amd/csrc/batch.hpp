@@ -54,12 +54,63 @@ inline int fail(int code, const std::string& msg) {
 enum TimingClass { kTimeSweepFull = 0, kTimeSweepStale = 1, kTimeRebuild = 2, kTimeClasses = 3 };
 
 
-inline int validate_force(const tecs_force_desc& f, uint32_t fields) {
+// The terms of a force in the order they apply: term_order up to its first TECS_TERM_END, or
+// PiecewiseLinear, bending, alignment when it is all zeros. pos[t] = place of term t, -1 if
+// it is not applied.
+inline void term_places(const tecs_force_desc& f, int pos[4]) {
+  pos[0] = pos[1] = pos[2] = pos[3] = -1;
+  if (f.term_order[0] == TECS_TERM_END && f.term_order[1] == TECS_TERM_END &&
+      f.term_order[2] == TECS_TERM_END) {
+    for (int t = 1; t <= 3; ++t) pos[t] = t - 1;
+    return;
+  }
+  for (int k = 0; k < 3 && f.term_order[k] != TECS_TERM_END; ++k) pos[f.term_order[k]] = k;
+}
+
+inline bool force_has_drag(const tecs_force_desc& f) {
+  if (f.drag_by_type) {
+    for (int k = 0; k < f.n_types * f.n_types; ++k)
+      if (f.drag_table[k] != 0.0f) return true;
+    return false;
+  }
+  return f.drag != 0.0f;
+}
+
+inline int validate_force(const tecs_force_desc& f, const tecs_sim_desc& d) {
+  const uint32_t fields = d.fields;
   if (f.n_types < 0 || f.n_types > TECS_MAX_TYPES) return fail(TECS_ERR_INVALID, "bad n_types");
   if (f.n_types > 0 && !(fields & TECS_FIELD_TYPE))
     return fail(TECS_ERR_INVALID, "type table needs TECS_FIELD_TYPE");
-  if ((f.bending_strength != 0.0f || f.alignment_strength != 0.0f) && !(fields & TECS_FIELD_POLARITY))
+  bool seen[4] = {false, false, false, false};
+  bool ended = false;
+  for (int k = 0; k < 3; ++k) {
+    const int t = f.term_order[k];
+    if (t < TECS_TERM_END || t > TECS_TERM_ALIGNMENT) return fail(TECS_ERR_INVALID, "bad term in term_order");
+    if (t == TECS_TERM_END) {
+      ended = true;
+      continue;
+    }
+    if (ended) return fail(TECS_ERR_INVALID, "term_order continues after TECS_TERM_END");
+    if (seen[t]) return fail(TECS_ERR_INVALID, "a term appears twice in term_order");
+    seen[t] = true;
+  }
+  int pos[4];
+  term_places(f, pos);
+  if (((pos[TECS_TERM_BENDING] >= 0 && f.bending_strength != 0.0f) ||
+       (pos[TECS_TERM_ALIGNMENT] >= 0 && f.alignment_strength != 0.0f)) &&
+      !(fields & TECS_FIELD_POLARITY))
     return fail(TECS_ERR_INVALID, "polarity forces need TECS_FIELD_POLARITY");
+  if (f.drag_by_type != 0 && f.drag_by_type != 1) return fail(TECS_ERR_INVALID, "drag_by_type must be 0 or 1");
+  if (f.drag_by_type && f.n_types == 0) return fail(TECS_ERR_INVALID, "drag by type needs a type table");
+  if (!std::isfinite(f.drag)) return fail(TECS_ERR_INVALID, "drag must be finite");
+  for (int k = 0; k < TECS_MAX_TYPES * TECS_MAX_TYPES; ++k)
+    if (!std::isfinite(f.drag_table[k])) return fail(TECS_ERR_INVALID, "drag table entries must be finite");
+  if (force_has_drag(f)) {
+    if (!d.track_old_velocities)
+      return fail(TECS_ERR_INVALID, "drag needs the old velocities (track_old_velocities = 1)");
+    if (is_symmetric(d.pair_finder))
+      return fail(TECS_ERR_UNSUPPORTED, "drag with a symmetric pair finder is not implemented");
+  }
   return TECS_OK;
 }
 
@@ -75,7 +126,7 @@ inline int validate_desc(const tecs_sim_desc* d) {
     return fail(TECS_ERR_INVALID, "bad integrator");
   if (d->com_fixer != TECS_NO_COM_FIXER && d->com_fixer != TECS_GLOBAL_COM_FIXER)
     return fail(TECS_ERR_INVALID, "bad com_fixer");
-  TRY(validate_force(d->force, d->fields));
+  TRY(validate_force(d->force, *d));
   if (!(d->cutoff_distance > 0.0f) || !std::isfinite(d->cutoff_distance))
     return fail(TECS_ERR_INVALID, "cutoff_distance must be finite and > 0");
   if (is_binned(d->pair_finder)) {
@@ -500,7 +551,11 @@ struct Batch {
       TRY(alloc(&xp, S));
       if (has_pol) TRY(alloc(&polp, S));
     }
-    if (integrator == TECS_HEUN && (com_fixer == TECS_GLOBAL_COM_FIXER || delta_only(finder))) {
+    bool any_drag = false;
+    for (int k = 0; k < count; ++k) any_drag = any_drag || force_has_drag(ds[k].force);
+    // the Heun corrector's own delta buffer: every step path that evaluates into deltas before
+    // the update (COM fixer, delta-only finders, drag; walls and links allocate it when set)
+    if (integrator == TECS_HEUN && (com_fixer == TECS_GLOBAL_COM_FIXER || delta_only(finder) || any_drag)) {
       TRY(alloc(&d3, S));
       if (has_pol) TRY(alloc(&d3p, S));
     }
@@ -639,23 +694,44 @@ struct Batch {
     return TECS_OK;
   }
   static void load_force(ForceParams& f, const tecs_force_desc& d) {
+    int pos[4];
+    term_places(d, pos);
     f.h = d.homeostatic_radius;
     f.krep = d.repulsion_scale;
     f.kadh = d.adhesion_scale;
     f.n_types = d.n_types;
     std::memcpy(f.table, d.adhesion_table, sizeof(f.table));
-    f.bend = d.bending_strength;
-    f.align = d.alignment_strength;
+    // a term that is not listed is not applied
+    f.bend = pos[TECS_TERM_BENDING] >= 0 ? d.bending_strength : 0.0f;
+    f.align = pos[TECS_TERM_ALIGNMENT] >= 0 ? d.alignment_strength : 0.0f;
+    const bool linear = pos[TECS_TERM_PIECEWISE_LINEAR] >= 0;
+    f.flags = linear ? kForceLinear : 0;
+    if (linear && f.bend != 0.0f && pos[TECS_TERM_BENDING] < pos[TECS_TERM_PIECEWISE_LINEAR])
+      f.flags |= kForceBendFirst;
+    if (f.bend != 0.0f && f.align != 0.0f && pos[TECS_TERM_ALIGNMENT] < pos[TECS_TERM_BENDING])
+      f.flags |= kForceAlignFirst;
+    if (force_has_drag(d)) f.flags |= kForceDrag;
+    f.drag = d.drag;
+    f.drag_by_type = d.drag_by_type;
+    std::memcpy(f.drag_table, d.drag_table, sizeof(f.drag_table));
   }
   // The force of member k for the following steps. The kernels read it from the device
   // instance table, which is refreshed before the next launch (captured graphs stay valid).
   int set_force(int k, const tecs_force_desc* f) {
     TRY(check_index(k));
     if (!f) return fail(TECS_ERR_INVALID, "null force");
-    TRY(validate_force(*f, descs[k].fields));
+    TRY(validate_force(*f, descs[k]));
+    // drag takes the separate-update path, whose Heun corrector evaluates into d3
+    if (force_has_drag(*f)) TRY(ensure_d3());
+    const bool drag_before = has_drag();
     descs[k].force = *f;
     load_force(h_inst[k].force, *f);
     inst_dirty = true;
+    if (has_drag() != drag_before) {
+      // the step takes another path (drag needs the separate update): captured graphs are
+      // stale; the stream is drained first, a graph may still be running
+      drop_graphs();
+    }
     return TECS_OK;
   }
   void set_dt(int k, double dt) {
@@ -881,11 +957,44 @@ struct Batch {
     return TECS_OK;
   }
 
-  int sweep(int mode, const SweepArgs& a, int cls) {
+  // The buffers a sweep of this mode reads and writes besides the state: a missing one is a
+  // host-side error, not a launch.
+  int check_sweep_args(int mode, const SweepArgs& a) const {
+    bool ok = a.inst && a.block_inst && a.xs && a.id && a.block_pairs && (!has_pol || a.ps);
+    if (is_binned(finder)) ok = ok && a.off;
+    switch (mode) {
+      case kModeDeltaDrag: ok = ok && a.ov_in; [[fallthrough]];
+      case kModeDelta: ok = ok && a.o_pos && (!has_pol || a.o_pol); break;
+      case kModePred:
+        ok = ok && a.o_pos && a.o2_pos && (!has_pol || (a.o_pol && a.o2_pol && a.o_trig));
+        break;
+      case kModeCorr:
+        ok = ok && a.o2_pos && a.d1_pos && (!has_pol || (a.o2_pol && a.d1_pol && a.o_trig));
+        break;
+      case kModeEuler: ok = ok && a.o_pos && (!has_pol || (a.o_pol && a.o_trig)); break;
+      case kModeCount: ok = ok && a.row_count; break;
+      case kModeEmit: ok = ok && a.row_ptr && a.cols; break;
+      default: ok = false;
+    }
+    if (finder == kNaiveDelaunay) ok = ok && a.dl_cnt;
+    if (!ok) return fail(TECS_ERR_STATE, "sweep launched without one of its buffers");
+    return TECS_OK;
+  }
+
+  int sweep(int mode, const SweepArgs& a_in, int cls) {
     Pending p{};
     TRY(time_begin(&p, cls));
     // the symmetric team sweeps evaluate forces only; the pair export walks one thread per cell
-    const int v = is_symmetric(finder) && mode != kModeDelta ? TECS_SWEEP_THREAD : concrete_variant();
+    int v = is_symmetric(finder) && mode != kModeDelta ? TECS_SWEEP_THREAD : concrete_variant();
+    SweepArgs a = a_in;
+    if (mode == kModeDelta && has_drag()) {
+      // the drag terms: one thread per cell sums them in visiting order, reading the old
+      // velocities of the current slot order
+      mode = kModeDeltaDrag;
+      v = TECS_SWEEP_THREAD;
+      a.ov_in = ov[xc];
+    }
+    TRY(check_sweep_args(mode, a));
     TRY(launch_variant(v, mode, a));
     HIP_TRY(hipGetLastError());
     TRY(time_end(&p));
@@ -946,8 +1055,16 @@ struct Batch {
       if (in.n_walls > 0) return true;
     return false;
   }
+  // Some member's force has drag: its evaluations read the old velocities of every partner,
+  // so none may update them in place (the fused Heun corrector and Euler epilogues do), and the
+  // thread sweep, which sums the drag terms in visiting order, evaluates them.
+  bool has_drag() const {
+    for (const auto& in : h_inst)
+      if (in.force.flags & kForceDrag) return true;
+    return false;
+  }
   // The steps take the separate-update path (below).
-  bool separate_update() const { return delta_only(finder) || has_extra(); }
+  bool separate_update() const { return delta_only(finder) || has_extra() || has_drag(); }
 
   // The generic wall forces (before the pair forces in the reference's order, so W + F) and the
   // link forces (after them) of an evaluation at stage positions xs into its deltas F. The
@@ -967,6 +1084,8 @@ struct Batch {
   // forces join them, then the COM fix and the update run as their own kernels.
   // Heun: evaluate -> predictor -> evaluate (stale grid) -> corrector.
   int step_delta_only() {
+    if (integrator == TECS_HEUN && (!d3 || (has_pol && !d3p)))
+      return fail(TECS_ERR_STATE, "separate-update step without the corrector's delta buffer");
     const bool sym_binned = is_symmetric(finder) && is_binned(finder);
     const bool stale1 = sym_binned && !rebuilt_now;
     SweepArgs a = base_args(x[xc], pol[xc]);
@@ -1602,6 +1721,7 @@ struct Batch {
     if (row_ptr) std::memcpy(row_ptr, rp.data(), sizeof(int64_t) * (n + 1));
     if (!cols) return TECS_OK;
     if (cols_capacity < total) return fail(TECS_ERR_CAPACITY, "cols buffer smaller than n_pairs");
+    if (total == 0) return TECS_OK;  // nothing to emit (the counting sweep left every counter clear)
     if (total > cols_cap) {
       void* v = nullptr;
       HIP_TRY(hipMalloc(&v, sizeof(int32_t) * (size_t)std::max<int64_t>(total, 1)));

@@ -172,8 +172,10 @@ __global__ __launch_bounds__(kBlock) void k_dl_cells(SweepArgs a) {
   const int s = lb * kBlock + (int)threadIdx.x;
   const int i = s - in.slot_begin;
   const bool active = i < in.n;
+  constexpr bool kForce = MODE == kModeDelta || MODE == kModeDeltaDrag;
   unsigned pairs = 0;
   float acc_pos[3] = {0.0f, 0.0f, 0.0f}, acc_pol[2] = {0.0f, 0.0f};
+  float tdrag = 0.0f, tvel[3] = {0.0f, 0.0f, 0.0f};
   if (active) {
     uint32_t* row = a.dl_cnt + ((size_t)ii * a.dl_row + i) * a.dl_row;
     const float4 me = a.xs[s];
@@ -190,7 +192,7 @@ __global__ __launch_bounds__(kBlock) void k_dl_cells(SweepArgs a) {
         const int32_t jid = a.id[m];
         for (unsigned r = 0; r < c; ++r) a.cols[at++] = jid;
       }
-      if (MODE != kModeDelta) continue;
+      if (!kForce) continue;
       const float4 pj = a.xs[m];
       const float disp[3] = {me.x - pj.x, me.y - pj.y, me.z - pj.z};
       const float d = sqrtf(len2(disp[0], disp[1], disp[2]));
@@ -205,6 +207,17 @@ __global__ __launch_bounds__(kBlock) void k_dl_cells(SweepArgs a) {
         acc_pol[0] += vl[0];
         acc_pol[1] += vl[1];
       }
+      if (MODE == kModeDeltaDrag) {
+        // naive_delaunay.hpp:256-276: each item adds its drag and drag * old_vel(j)
+        const float g = pair_drag<HAS_TYPE>(in.force, __float_as_int(me.w), __float_as_int(pj.w));
+        const float4 ov = a.ov_in[m];
+        for (unsigned r = 0; r < c; ++r) {
+          tdrag = tdrag + g;
+          tvel[0] = tvel[0] + g * ov.x;
+          tvel[1] = tvel[1] + g * ov.y;
+          tvel[2] = tvel[2] + g * ov.z;
+        }
+      }
     }
   }
   if (MODE == kModeCount) {
@@ -213,10 +226,14 @@ __global__ __launch_bounds__(kBlock) void k_dl_cells(SweepArgs a) {
   }
   if (MODE == kModeEmit) return;
   if (active) {
-    // out(i) holds the items' packs; the drag pass adds total_velocity / fmax(total_drag, eps),
-    // which is +0 because no force term writes drag (naive_delaunay.hpp:282-289)
-    a.o_pos[s] = make_float4((0.0f + acc_pos[0]) + 0.0f, (0.0f + acc_pos[1]) + 0.0f,
-                             (0.0f + acc_pos[2]) + 0.0f, 0.0f);
+    // out(i) holds the items' packs; the drag pass adds total_velocity / fmax(total_drag, eps)
+    // (naive_delaunay.hpp:282-289), +0 when no pair has drag
+    const float den = fmaxf(tdrag, 1.1920928955078125e-07f);
+    const float e0 = MODE == kModeDeltaDrag ? tvel[0] / den : 0.0f;
+    const float e1 = MODE == kModeDeltaDrag ? tvel[1] / den : 0.0f;
+    const float e2 = MODE == kModeDeltaDrag ? tvel[2] / den : 0.0f;
+    a.o_pos[s] = make_float4((0.0f + acc_pos[0]) + e0, (0.0f + acc_pos[1]) + e1,
+                             (0.0f + acc_pos[2]) + e2, 0.0f);
     if (HAS_POL) a.o_pol[s] = make_float2(0.0f + acc_pol[0], 0.0f + acc_pol[1]);
   }
   const unsigned tp = block_sum_u(active ? pairs : 0u, lds_u);
@@ -226,7 +243,7 @@ __global__ __launch_bounds__(kBlock) void k_dl_cells(SweepArgs a) {
 template <int MODE>
 void launch_cells(bool has_type, bool has_pol, const SweepArgs& a, int n_blocks, hipStream_t stream) {
   const dim3 g(n_blocks), b(kBlock);
-  if (MODE != kModeDelta)
+  if (MODE != kModeDelta && MODE != kModeDeltaDrag)
     hipLaunchKernelGGL((k_dl_cells<false, false, MODE>), g, b, 0, stream, a);
   else if (has_type && has_pol)
     hipLaunchKernelGGL((k_dl_cells<true, true, MODE>), g, b, 0, stream, a);
@@ -242,11 +259,12 @@ void launch_cells(bool has_type, bool has_pol, const SweepArgs& a, int n_blocks,
 
 int launch_delaunay(bool has_type, bool has_pol, int mode, const SweepArgs& a, int n_blocks,
                     hipStream_t stream) {
-  if (mode != kModeDelta && mode != kModeCount && mode != kModeEmit) return -1;
+  if (mode != kModeDelta && mode != kModeDeltaDrag && mode != kModeCount && mode != kModeEmit) return -1;
   const int64_t groups = (int64_t)a.n_inst * a.dl_row * a.dl_splits;
   if (groups > 0)
     hipLaunchKernelGGL(k_dl_count, dim3((unsigned)groups), dim3(kBlock), 0, stream, a);
   if (mode == kModeDelta) launch_cells<kModeDelta>(has_type, has_pol, a, n_blocks, stream);
+  else if (mode == kModeDeltaDrag) launch_cells<kModeDeltaDrag>(has_type, has_pol, a, n_blocks, stream);
   else if (mode == kModeCount) launch_cells<kModeCount>(has_type, has_pol, a, n_blocks, stream);
   else launch_cells<kModeEmit>(has_type, has_pol, a, n_blocks, stream);
   return 0;

@@ -31,6 +31,8 @@ struct CellState {
   float acc_pol[2];
   unsigned pairs;
   int64_t emit_at;
+  float drag;    // DeltaDrag: sum of the pairs' drag (binned_all_pairs.hpp:113, 168)
+  float vel[3];  //            sum of drag * old_velocity_j (:114, 169)
 };
 
 template <bool HAS_TYPE, bool HAS_POL, int MODE, bool SYM>
@@ -49,6 +51,15 @@ __device__ __forceinline__ void visit(const SweepArgs& a, const InstanceDesc& in
   static_assert(!SYM || MODE == kModeCount || MODE == kModeEmit, "symmetric walks export pairs only");
   apply_force<HAS_TYPE, HAS_POL>(in.force, disp, d, __float_as_int(me.w), __float_as_int(pj.w),
                                  pme, po, cs.acc_pos, cs.acc_pol);
+  if (MODE == kModeDeltaDrag) {
+    // binned_all_pairs.hpp:168-169: total_drag += drag; total_velocity += drag * old_vel(j)
+    const float g = pair_drag<HAS_TYPE>(in.force, __float_as_int(me.w), __float_as_int(pj.w));
+    const float4 ov = a.ov_in[m];
+    cs.drag = cs.drag + g;
+    cs.vel[0] = cs.vel[0] + g * ov.x;
+    cs.vel[1] = cs.vel[1] + g * ov.y;
+    cs.vel[2] = cs.vel[2] + g * ov.z;
+  }
 }
 
 template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE, bool SYM>
@@ -224,17 +235,23 @@ __device__ __forceinline__ void finish(const SweepArgs& a, const InstanceDesc& i
   }
   if (MODE == kModeEmit) return;
   // out(i) += total (the delta buffer is zero before an evaluation), then
-  // out.pos(i) += total_velocity / fmax(total_drag, eps) which is +0 because no force term
-  // writes drag (binned_all_pairs.hpp:174-180).
+  // out.pos(i) += total_velocity / fmax(total_drag, eps) (binned_all_pairs.hpp:174-180), which
+  // is +0 when no pair has drag (the modes other than DeltaDrag).
   float F[3], G[2];
+  if (MODE == kModeDeltaDrag) {
+    const float den = fmaxf(cs.drag, 1.1920928955078125e-07f);
 #pragma unroll
-  for (int c = 0; c < 3; ++c) F[c] = (0.0f + cs.acc_pos[c]) + 0.0f;
+    for (int c = 0; c < 3; ++c) F[c] = (0.0f + cs.acc_pos[c]) + cs.vel[c] / den;
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) F[c] = (0.0f + cs.acc_pos[c]) + 0.0f;
+  }
   G[0] = 0.0f + cs.acc_pol[0];
   G[1] = 0.0f + cs.acc_pol[1];
   const float dt = in.dt;
   float4 xnew = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // Corr / Euler: the new position
   if (active) {
-    if (MODE == kModeDelta) {
+    if (MODE == kModeDelta || MODE == kModeDeltaDrag) {
       a.o_pos[s] = make_float4(F[0], F[1], F[2], 0.0f);
       if (HAS_POL) a.o_pol[s] = make_float2(G[0], G[1]);
     } else if (MODE == kModePred) {
@@ -296,7 +313,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_binned(SweepArgs a) {
   const InstanceDesc& in = a.inst[ii];
   const int s = lb * kBlock + (int)threadIdx.x;
   const bool active = (s - in.slot_begin) < in.n;
-  CellState cs{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f}, 0u, 0};
+  CellState cs{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f}, 0u, 0, 0.0f, {0.0f, 0.0f, 0.0f}};
   float4 me = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   float2 mp = make_float2(0.0f, 0.0f);
   Pol pme{};
@@ -312,8 +329,9 @@ __global__ __launch_bounds__(kBlock) void k_sweep_binned(SweepArgs a) {
     if (MODE == kModeEmit) cs.emit_at = a.row_ptr[a.id[s]];
     if (MODE == kModeCorr) ci = load_corr<HAS_POL>(a, s);
   }
-  constexpr bool kForce = MODE != kModeCount && MODE != kModeEmit;
-  if (kForce && in.search_radius == 1) {
+  // the drag mode walks inline: its drag sums run in the same visiting order either way
+  constexpr bool kDefer = MODE != kModeCount && MODE != kModeEmit && MODE != kModeDeltaDrag;
+  if (kDefer && in.search_radius == 1) {
     // the instance decides the path for the whole workgroup (all its slots are one instance)
     __shared__ int list[kDeferMax * kBlock];
     walk_binned_deferred<GABRIEL, HAS_TYPE, HAS_POL, MODE, SYM>(a, in, s, me_id, active, me, pme,
@@ -336,7 +354,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_naive(SweepArgs a) {
   const InstanceDesc& in = a.inst[ii];
   const int s = lb * kBlock + (int)threadIdx.x;
   const bool active = (s - in.slot_begin) < in.n;
-  CellState cs{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f}, 0u, 0};
+  CellState cs{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f}, 0u, 0, 0.0f, {0.0f, 0.0f, 0.0f}};
   float4 me = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   float2 mp = make_float2(0.0f, 0.0f);
   Pol pme{};
@@ -599,17 +617,21 @@ void launch_force(bool has_type, bool has_pol, const SweepArgs& a, int n_blocks,
   }
 }
 
+// Returns nonzero for an unknown mode.
 template <bool GABRIEL, bool BINNED>
-void launch_mode(int mode, bool has_type, bool has_pol, const SweepArgs& a, int n_blocks,
-                 hipStream_t stream) {
+int launch_mode(int mode, bool has_type, bool has_pol, const SweepArgs& a, int n_blocks,
+                hipStream_t stream) {
   switch (mode) {
     case kModeDelta: launch_force<GABRIEL, BINNED, kModeDelta, false>(has_type, has_pol, a, n_blocks, stream); break;
     case kModePred: launch_force<GABRIEL, BINNED, kModePred, false>(has_type, has_pol, a, n_blocks, stream); break;
     case kModeCorr: launch_force<GABRIEL, BINNED, kModeCorr, false>(has_type, has_pol, a, n_blocks, stream); break;
     case kModeEuler: launch_force<GABRIEL, BINNED, kModeEuler, false>(has_type, has_pol, a, n_blocks, stream); break;
+    case kModeDeltaDrag: launch_force<GABRIEL, BINNED, kModeDeltaDrag, false>(has_type, has_pol, a, n_blocks, stream); break;
     case kModeCount: launch_force<GABRIEL, BINNED, kModeCount, false>(has_type, has_pol, a, n_blocks, stream); break;
-    default: launch_force<GABRIEL, BINNED, kModeEmit, false>(has_type, has_pol, a, n_blocks, stream); break;
+    case kModeEmit: launch_force<GABRIEL, BINNED, kModeEmit, false>(has_type, has_pol, a, n_blocks, stream); break;
+    default: return -1;
   }
+  return 0;
 }
 
 template <bool GABRIEL, bool BINNED, bool HAS_TYPE, bool HAS_POL>
@@ -805,10 +827,10 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
 int launch_sweep(int finder, bool has_type, bool has_pol, int mode, const SweepArgs& a,
                  int n_blocks, hipStream_t stream) {
   switch (finder) {
-    case kBinnedAllPairs: launch_mode<false, true>(mode, has_type, has_pol, a, n_blocks, stream); return 0;
-    case kBinnedGabriel: launch_mode<true, true>(mode, has_type, has_pol, a, n_blocks, stream); return 0;
-    case kNaiveAllPairs: launch_mode<false, false>(mode, has_type, has_pol, a, n_blocks, stream); return 0;
-    case kNaiveGabriel: launch_mode<true, false>(mode, has_type, has_pol, a, n_blocks, stream); return 0;
+    case kBinnedAllPairs: return launch_mode<false, true>(mode, has_type, has_pol, a, n_blocks, stream);
+    case kBinnedGabriel: return launch_mode<true, true>(mode, has_type, has_pol, a, n_blocks, stream);
+    case kNaiveAllPairs: return launch_mode<false, false>(mode, has_type, has_pol, a, n_blocks, stream);
+    case kNaiveGabriel: return launch_mode<true, false>(mode, has_type, has_pol, a, n_blocks, stream);
     case kBinnedAllPairsSym: return launch_mode_sym<false, true>(mode, has_type, has_pol, a, n_blocks, stream);
     case kBinnedGabrielSym: return launch_mode_sym<true, true>(mode, has_type, has_pol, a, n_blocks, stream);
     case kNaiveAllPairsSym: return launch_mode_sym<false, false>(mode, has_type, has_pol, a, n_blocks, stream);

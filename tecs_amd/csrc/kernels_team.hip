@@ -30,6 +30,8 @@
 // that the other cell delivers). The ball of a gather pair (m, i) is the one m's walk tests;
 // it lies within (c/2 + 1/2) d(m, i) of i as well, so its blockers are accepted candidates of
 // i too.
+#include <type_traits>
+
 #include "tecs_internal.hpp"
 #include "walks.hpp"
 
@@ -46,12 +48,20 @@ struct TeamCap {
   static constexpr int kAMax = GABRIEL ? 16 : 24;
 };
 
-template <int T, bool GABRIEL, bool HAS_POL>
+// SPLIT: an ordered sweep whose force can have two terms per field (polarity models): each
+// pair's terms are kept apart, so the leader adds them to the cell's running sums one by one in
+// the composition's order, as the reference's fused forces add to one pack
+// (kernel_fuser.hpp:40-55). The symmetric finders evaluate a pair on a zeroed pack and add the
+// pack (binned_all_pairs_symmetric.hpp:176-200), so they keep one value per pair.
+template <int T, bool GABRIEL, bool HAS_POL, bool SPLIT = false>
 struct TeamLds {
   static constexpr int TB = kBlock / T;  // teams (cells) per workgroup
   static constexpr int AM = TeamCap<GABRIEL>::kAMax;
   static constexpr int TG = GABRIEL ? TB : 1;
   static constexpr int TP = HAS_POL ? TB : 1;
+  // all-pairs computes forces while the rows are still read, so its second terms need arrays
+  // of their own; Gabriel computes them after the rows are dead and keeps them there
+  static constexpr int TS = SPLIT && !GABRIEL ? TB : 1;
   // per accepted neighbour, in visiting order:
   //   ent: Gabriel - its position (w = type bits) until the Gabriel tests are done, then
   //        the pair's position contribution; all-pairs - the contribution. w = 1 when the
@@ -63,13 +73,32 @@ struct TeamLds {
   int aslot[TB][AM + 1];    // its slot
   int abit[TG][AM + 1];     // one-hot bit rx + 3 ry + 9 rz of its (stale) bin relative to
                             // i's centre bin - 1 (r* in 0..2)
-  float2 cpol[TP][AM + 1];  // polarity contribution
-  int row_start[TB][9];
-  int row_len[TB][9];
-  int row_base[TB][9];  // row_start - the row's first candidate index (slot = base + c)
-  int row_b1[TG][9];    // first slot of the row's second bin (Gabriel)
-  int row_b2[TG][9];    // first slot of the row's third bin (Gabriel)
-  int row_xlo[TG][9];
+  float2 cpol[TP][AM + 1];  // polarity contribution (SPLIT: its first term)
+  float4 ent2[TS][AM + 1];  // SPLIT all-pairs: the second position term (xyz)
+  float2 cpol2[TS][AM + 1]; // SPLIT all-pairs: the second polarity term
+  // The 9 neighbour rows of each team's cell. A Gabriel SPLIT force phase, which runs after the
+  // last read of the rows, keeps each entry's second position term in their place (and the
+  // second polarity term in the entry's aslot / abit, which are dead by then as well).
+  struct RowsG {
+    int start[9];
+    int len[9];
+    int base[9];  // start - the row's first candidate index (slot = base + c)
+    int b1[9];    // first slot of the row's second bin
+    int b2[9];    // first slot of the row's third bin
+    int xlo[9];
+  };
+  struct RowsA {
+    int start[9];
+    int len[9];
+    int base[9];
+    int b1[1], b2[1], xlo[1];  // unused without Gabriel
+  };
+  union TeamRows {
+    typename std::conditional<GABRIEL, RowsG, RowsA>::type r;
+    float p2[GABRIEL ? 3 * AM : 1];
+  };
+  static_assert(!GABRIEL || sizeof(RowsG) >= sizeof(float) * 3 * AM, "second terms fit the rows");
+  TeamRows rows[TB];
   int unsafe[TB];
 };
 
@@ -179,9 +208,10 @@ constexpr int kRoleShift = 28;
 template <int T, bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE, bool SYM>
 __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
   static_assert(!SYM || MODE == kModeDelta, "symmetric team sweeps evaluate forces only");
-  constexpr int TB = TeamLds<T, GABRIEL, HAS_POL>::TB;
-  constexpr int AM = TeamLds<T, GABRIEL, HAS_POL>::AM;
-  __shared__ TeamLds<T, GABRIEL, HAS_POL> L;
+  constexpr bool SPLIT = HAS_POL && !SYM;
+  constexpr int TB = TeamLds<T, GABRIEL, HAS_POL, SPLIT>::TB;
+  constexpr int AM = TeamLds<T, GABRIEL, HAS_POL, SPLIT>::AM;
+  __shared__ TeamLds<T, GABRIEL, HAS_POL, SPLIT> L;
   __shared__ float lds_f[4];
 
   // team-block -> slots: each 256-slot chunk (one instance) holds T team-blocks;
@@ -234,12 +264,12 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
           if (x_lo + 2 <= x_hi) b2 = offb[b_lo + 2];
         }
       }
-      L.row_start[team][rr] = start;
-      L.row_len[team][rr] = len;
+      L.rows[team].r.start[rr] = start;
+      L.rows[team].r.len[rr] = len;
       if (GABRIEL) {
-        L.row_b1[team][rr] = b1;
-        L.row_b2[team][rr] = b2;
-        L.row_xlo[team][rr] = x_lo;
+        L.rows[team].r.b1[rr] = b1;
+        L.rows[team].r.b2[rr] = b2;
+        L.rows[team].r.xlo[rr] = x_lo;
       }
     }
   }
@@ -251,13 +281,13 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
 #pragma unroll
   for (int rr = 0; rr < 9; ++rr) {
     row_pre[rr] = total;
-    row_start[rr] = L.row_start[team][rr];
-    total += L.row_len[team][rr];
+    row_start[rr] = L.rows[team].r.start[rr];
+    total += L.rows[team].r.len[rr];
   }
   if (!active) total = 0;
   if (lane == 0) {
 #pragma unroll
-    for (int rr = 0; rr < 9; ++rr) L.row_base[team][rr] = row_start[rr] - row_pre[rr];
+    for (int rr = 0; rr < 9; ++rr) L.rows[team].r.base[rr] = row_start[rr] - row_pre[rr];
   }
   team_sync();
   const float cut2 = in.cutoff2;
@@ -280,7 +310,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
       // prefix with the next one)
 #pragma unroll
       for (int rr = 1; rr < 9; ++rr) rr_sel += c >= row_pre[rr] ? 1 : 0;
-      m = L.row_base[team][rr_sel] + c;
+      m = L.rows[team].r.base[rr_sel] + c;
       if (m != s) {
         pj = a.xs[m];
         disp[0] = me.x - pj.x;
@@ -309,8 +339,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
       if (GABRIEL) {
         // stale bin of m relative to (bc - 1): the row gives y and z, the row's bin
         // boundaries give x
-        const int bx = L.row_xlo[team][rr_sel] + (m >= L.row_b1[team][rr_sel] ? 1 : 0) +
-                       (m >= L.row_b2[team][rr_sel] ? 1 : 0);
+        const int bx = L.rows[team].r.xlo[rr_sel] + (m >= L.rows[team].r.b1[rr_sel] ? 1 : 0) +
+                       (m >= L.rows[team].r.b2[rr_sel] ? 1 : 0);
         L.ent[team][pos] = pj;
         // row rr_sel = 3 rz + ry, so 3 ry + 9 rz = 3 rr_sel
         L.abit[team][pos] = (1 << ((bx - bc[0] + 1) + 3 * rr_sel)) | (SYM ? role << kRoleShift : 0);
@@ -323,15 +353,24 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
       } else {
         // all-pairs: the force needs nothing else, compute it here
         float cp[3] = {0.0f, 0.0f, 0.0f}, cl[2] = {0.0f, 0.0f};
+        float cp2[3] = {0.0f, 0.0f, 0.0f}, cl2[2] = {0.0f, 0.0f};
         if (MODE != kModeCount && MODE != kModeEmit) {
           const float d = sqrtf(d2);
           Pol po{};
           if (HAS_POL) po = make_pol(a.ps[m], a.trig[m]);
-          apply_force<HAS_TYPE, HAS_POL>(in.force, disp, d, __float_as_int(me.w),
-                                         __float_as_int(pj.w), pme, po, cp, cl);
+          if (SPLIT)
+            apply_force_terms<HAS_TYPE, HAS_POL>(in.force, disp, d, __float_as_int(me.w),
+                                                 __float_as_int(pj.w), pme, po, cp, cp2, cl, cl2);
+          else
+            apply_force<HAS_TYPE, HAS_POL>(in.force, disp, d, __float_as_int(me.w),
+                                           __float_as_int(pj.w), pme, po, cp, cl);
         }
         L.ent[team][pos] = make_float4(cp[0], cp[1], cp[2], 1.0f);
         if (HAS_POL) L.cpol[team][pos] = make_float2(cl[0], cl[1]);
+        if (SPLIT) {
+          L.ent2[team][pos] = make_float4(cp2[0], cp2[1], cp2[2], 0.0f);
+          L.cpol2[team][pos] = make_float2(cl2[0], cl2[1]);
+        }
       }
     }
     base += __popc(tm);
@@ -405,6 +444,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
       const float4 pj = L.ent[team][e];
       const bool blocked = (blocked_bits >> r) & 1u;
       float cp[3] = {0.0f, 0.0f, 0.0f}, cl[2] = {0.0f, 0.0f};
+      float cp2[3] = {0.0f, 0.0f, 0.0f}, cl2[2] = {0.0f, 0.0f};
       const int erole = SYM ? (L.abit[team][e] >> kRoleShift) : kRoleOwn;
       if (SYM && !blocked) {
         sym_eval<HAS_TYPE, HAS_POL>(a, in, me, pme, L.aslot[team][e], erole, cp, cl);
@@ -414,13 +454,26 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
         const float d = sqrtf(len2(disp[0], disp[1], disp[2]));
         Pol po{};
         if (HAS_POL) po = make_pol(a.ps[m], a.trig[m]);
-        apply_force<HAS_TYPE, HAS_POL>(in.force, disp, d, __float_as_int(me.w),
-                                       __float_as_int(pj.w), pme, po, cp, cl);
+        if (SPLIT)
+          apply_force_terms<HAS_TYPE, HAS_POL>(in.force, disp, d, __float_as_int(me.w),
+                                               __float_as_int(pj.w), pme, po, cp, cp2, cl, cl2);
+        else
+          apply_force<HAS_TYPE, HAS_POL>(in.force, disp, d, __float_as_int(me.w),
+                                         __float_as_int(pj.w), pme, po, cp, cl);
       }
       // w: 0 no pair, 1 a pair the reference counts (ordered, or own), 2 a gathered pair
       L.ent[team][e] = make_float4(cp[0], cp[1], cp[2],
                                    blocked ? 0.0f : (erole == kRoleGather ? 2.0f : 1.0f));
       if (HAS_POL) L.cpol[team][e] = make_float2(cl[0], cl[1]);
+      if (SPLIT) {
+        // the rows were last read by the candidate phase and aslot / abit of this entry by
+        // this lane above and by the Gabriel tests before the last team_sync
+        L.rows[team].p2[3 * e + 0] = cp2[0];
+        L.rows[team].p2[3 * e + 1] = cp2[1];
+        L.rows[team].p2[3 * e + 2] = cp2[2];
+        L.aslot[team][e] = __float_as_int(cl2[0]);
+        L.abit[team][e] = __float_as_int(cl2[1]);
+      }
     }
   }
   team_sync();
@@ -449,10 +502,32 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
         acc_pos[0] = acc_pos[0] + c.x;
         acc_pos[1] = acc_pos[1] + c.y;
         acc_pos[2] = acc_pos[2] + c.z;
+        if (SPLIT) {
+          float c2[3];
+          if (GABRIEL) {
+            c2[0] = L.rows[team].p2[3 * e + 0];
+            c2[1] = L.rows[team].p2[3 * e + 1];
+            c2[2] = L.rows[team].p2[3 * e + 2];
+          } else {
+            const float4 v = L.ent2[team][e];
+            c2[0] = v.x;
+            c2[1] = v.y;
+            c2[2] = v.z;
+          }
+          acc_pos[0] = acc_pos[0] + c2[0];
+          acc_pos[1] = acc_pos[1] + c2[1];
+          acc_pos[2] = acc_pos[2] + c2[2];
+        }
         if (HAS_POL) {
           const float2 g = L.cpol[team][e];
           acc_pol[0] = acc_pol[0] + g.x;
           acc_pol[1] = acc_pol[1] + g.y;
+        }
+        if (SPLIT) {
+          const float2 g2 = GABRIEL ? make_float2(__int_as_float(L.aslot[team][e]), __int_as_float(L.abit[team][e]))
+                                    : L.cpol2[team][e];
+          acc_pol[0] = acc_pol[0] + g2.x;
+          acc_pol[1] = acc_pol[1] + g2.y;
         }
       }
     }
@@ -557,32 +632,34 @@ void launch_force_t(bool has_type, bool has_pol, const SweepArgs& a, int nb, hip
   }
 }
 
+// Returns nonzero for a mode the team sweep does not have (the drag mode runs on the thread
+// sweep).
 template <int T, bool GABRIEL>
-void launch_mode_t(int mode, bool has_type, bool has_pol, const SweepArgs& a, int nb,
-                   hipStream_t st) {
+int launch_mode_t(int mode, bool has_type, bool has_pol, const SweepArgs& a, int nb,
+                  hipStream_t st) {
   switch (mode) {
-    case kModeDelta: launch_force_t<T, GABRIEL, kModeDelta>(has_type, has_pol, a, nb, st); break;
-    case kModePred: launch_force_t<T, GABRIEL, kModePred>(has_type, has_pol, a, nb, st); break;
-    case kModeCorr: launch_force_t<T, GABRIEL, kModeCorr>(has_type, has_pol, a, nb, st); break;
-    case kModeEuler: launch_force_t<T, GABRIEL, kModeEuler>(has_type, has_pol, a, nb, st); break;
-    case kModeCount: launch_force_t<T, GABRIEL, kModeCount>(has_type, has_pol, a, nb, st); break;
-    default: launch_force_t<T, GABRIEL, kModeEmit>(has_type, has_pol, a, nb, st); break;
+    case kModeDelta: launch_force_t<T, GABRIEL, kModeDelta>(has_type, has_pol, a, nb, st); return 0;
+    case kModePred: launch_force_t<T, GABRIEL, kModePred>(has_type, has_pol, a, nb, st); return 0;
+    case kModeCorr: launch_force_t<T, GABRIEL, kModeCorr>(has_type, has_pol, a, nb, st); return 0;
+    case kModeEuler: launch_force_t<T, GABRIEL, kModeEuler>(has_type, has_pol, a, nb, st); return 0;
+    case kModeCount: launch_force_t<T, GABRIEL, kModeCount>(has_type, has_pol, a, nb, st); return 0;
+    case kModeEmit: launch_force_t<T, GABRIEL, kModeEmit>(has_type, has_pol, a, nb, st); return 0;
+    default: return -1;
   }
 }
 
 template <int T>
-void launch_team_width(bool gabriel, bool sym, bool has_type, bool has_pol, int mode,
-                       const SweepArgs& a, int n_blocks, hipStream_t stream) {
+int launch_team_width(bool gabriel, bool sym, bool has_type, bool has_pol, int mode,
+                      const SweepArgs& a, int n_blocks, hipStream_t stream) {
   if (sym) {  // Delta only (checked by the caller)
     if (gabriel)
       launch_force_t<T, true, kModeDelta, true>(has_type, has_pol, a, n_blocks, stream);
     else
       launch_force_t<T, false, kModeDelta, true>(has_type, has_pol, a, n_blocks, stream);
-  } else if (gabriel) {
-    launch_mode_t<T, true>(mode, has_type, has_pol, a, n_blocks, stream);
-  } else {
-    launch_mode_t<T, false>(mode, has_type, has_pol, a, n_blocks, stream);
+    return 0;
   }
+  if (gabriel) return launch_mode_t<T, true>(mode, has_type, has_pol, a, n_blocks, stream);
+  return launch_mode_t<T, false>(mode, has_type, has_pol, a, n_blocks, stream);
 }
 
 }  // namespace
@@ -591,11 +668,11 @@ int launch_sweep_team(int team_lanes, bool gabriel, bool sym, bool has_type, boo
                       int mode, const SweepArgs& a, int n_blocks, hipStream_t stream) {
   if (sym && mode != kModeDelta) return -1;
   switch (team_lanes) {
-    case 4: launch_team_width<4>(gabriel, sym, has_type, has_pol, mode, a, n_blocks, stream); break;
-    case 8: launch_team_width<8>(gabriel, sym, has_type, has_pol, mode, a, n_blocks, stream); break;
-    default: launch_team_width<16>(gabriel, sym, has_type, has_pol, mode, a, n_blocks, stream); break;
+    case 4: return launch_team_width<4>(gabriel, sym, has_type, has_pol, mode, a, n_blocks, stream);
+    case 8: return launch_team_width<8>(gabriel, sym, has_type, has_pol, mode, a, n_blocks, stream);
+    case 16: return launch_team_width<16>(gabriel, sym, has_type, has_pol, mode, a, n_blocks, stream);
+    default: return -1;
   }
-  return 0;
 }
 
 }  // namespace tecs_amd

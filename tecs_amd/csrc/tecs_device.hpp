@@ -29,11 +29,23 @@ constexpr int kBlock = 256;  // slots per workgroup (4 wavefronts of 64)
 constexpr int kMaxTypes = 4;
 constexpr int kMaxWalls = 8;  // TECS_MAX_WALLS
 
+// Term order flags of the fused force: only the relative order of terms that write the same
+// field changes a sum (PiecewiseLinear and bending both write the position delta, bending and
+// alignment both write the polarity delta).
+constexpr int kForceLinear = 1;      // the PiecewiseLinear term is applied
+constexpr int kForceBendFirst = 2;   // bending's position term is added before PiecewiseLinear's
+constexpr int kForceAlignFirst = 4;  // alignment is added before bending's polarity term
+constexpr int kForceDrag = 8;        // some pair has nonzero drag (the drag correction runs)
+
 struct ForceParams {
   float h, krep, kadh;
   int n_types;
   float table[kMaxTypes * kMaxTypes];
   float bend, align;
+  int flags;
+  float drag;
+  int drag_by_type;
+  float drag_table[kMaxTypes * kMaxTypes];
 };
 
 struct InstanceDesc {
@@ -264,8 +276,7 @@ __device__ __forceinline__ float4 polar_trig(float2 q) {
   return make_float4(st, ct, sp, cp);
 }
 
-// One invocation of the fused pairwise force (see tecs_force_desc in tecs_amd.h), adding the
-// contributions to the accumulators in the reference's statement order.
+// The arithmetic of the fused pairwise force's terms (see tecs_force_desc in tecs_amd.h).
 //
 // The PiecewiseLinear part is evaluated with the reference's operations in its order, so it is
 // bitwise the reference's FP32 result. The polarity terms are evaluated from the same
@@ -283,56 +294,115 @@ __device__ __forceinline__ float4 polar_trig(float2 q) {
 //     cos(p_i - p_j) = cos p_i cos p_j + sin p_i sin p_j and
 //     sin(p_i - p_j) = sin p_i cos p_j - cos p_i sin p_j.
 // The guard |sin theta_i| > FLT_EPSILON (polarity.hpp:154) is kept on the same value.
+// The values of the terms of one pair, each on its own (zero when the term is not applied):
+// PiecewiseLinear's position term v, bending's position term bpos = bend * b.vector and
+// polarity term bpol = bend * b.polarity, alignment's polarity term al.
+struct PairTerms {
+  float v[3], bpos[3], bpol[2], al[2];
+};
+
 template <bool HAS_TYPE, bool HAS_POL>
-__device__ __forceinline__ void apply_force(const ForceParams& f, const float* disp, float d,
-                                            int type_i, int type_j, const Pol& pi, const Pol& pj,
-                                            float* acc_pos, float* acc_pol) {
+__device__ __forceinline__ void pair_terms(const ForceParams& f, const float* disp, float d,
+                                           int type_i, int type_j, const Pol& pi, const Pol& pj,
+                                           PairTerms& t) {
   float kadh = f.kadh;
   if (HAS_TYPE) {
     if (f.n_types > 0) kadh = f.table[type_i * f.n_types + type_j];
   }
-  float v[3];
-  piecewise_linear(disp, d, f.h, f.krep, kadh, v);
-  acc_pos[0] = acc_pos[0] + v[0];
-  acc_pos[1] = acc_pos[1] + v[1];
-  acc_pos[2] = acc_pos[2] + v[2];
-  if (HAS_POL) {
-    const float sti = pi.trig.x, cti = pi.trig.y, spi = pi.trig.z, cpi = pi.trig.w;
-    const float stj = pj.trig.x, ctj = pj.trig.y, spj = pj.trig.z, cpj = pj.trig.w;
-    const bool pole = !(fabsf(sti) > 1.1920928955078125e-07f);
-    const float inv_sti = pole ? 0.0f : 1.0f / sti;
-    if (f.bend != 0.0f) {
-      const float inv_d = 1.0f / d;
-      const float tv[3] = {sti * cpi, sti * spi, cti};  // to_vector3, polarity.hpp:111-118
-      const float ov[3] = {stj * cpj, stj * spj, ctj};
-      const float prod_i = dot3(tv, disp) * inv_d;
-      const float prod_j = dot3(ov, disp) * inv_d;
-      const float ri = prod_i * inv_d, rj = prod_j * inv_d;
-      const float a = -ri, b = ri * ri, c = -rj, e = rj * rj;
-      float bv[3];
+  t.v[0] = t.v[1] = t.v[2] = 0.0f;
+  t.bpos[0] = t.bpos[1] = t.bpos[2] = 0.0f;
+  t.bpol[0] = t.bpol[1] = t.al[0] = t.al[1] = 0.0f;
+  if (f.flags & kForceLinear) piecewise_linear(disp, d, f.h, f.krep, kadh, t.v);
+  if (!HAS_POL) return;
+  const float sti = pi.trig.x, cti = pi.trig.y, spi = pi.trig.z, cpi = pi.trig.w;
+  const float stj = pj.trig.x, ctj = pj.trig.y, spj = pj.trig.z, cpj = pj.trig.w;
+  const bool pole = !(fabsf(sti) > 1.1920928955078125e-07f);
+  const float inv_sti = pole ? 0.0f : 1.0f / sti;
+  if (f.bend != 0.0f) {
+    const float inv_d = 1.0f / d;
+    const float tv[3] = {sti * cpi, sti * spi, cti};  // to_vector3, polarity.hpp:111-118
+    const float ov[3] = {stj * cpj, stj * spj, ctj};
+    const float prod_i = dot3(tv, disp) * inv_d;
+    const float prod_j = dot3(ov, disp) * inv_d;
+    const float ri = prod_i * inv_d, rj = prod_j * inv_d;
+    const float a = -ri, b = ri * ri, c = -rj, e = rj * rj;
 #pragma unroll
-      for (int k = 0; k < 3; ++k) bv[k] = ((a * tv[k] + b * disp[k]) + c * ov[k]) + e * disp[k];
-      const float sc = (cpi * disp[0] + spi * disp[1]) * inv_d;  // sin q0 cos(p_i - q1)
-      const float ss = (spi * disp[0] - cpi * disp[1]) * inv_d;  // sin q0 sin(p_i - q1)
-      const float cq0 = disp[2] * inv_d;
-      const float u0 = cti * sc - sti * cq0;
-      const float u1 = pole ? 0.0f : -ss * inv_sti;
-      acc_pos[0] = acc_pos[0] + f.bend * bv[0];
-      acc_pos[1] = acc_pos[1] + f.bend * bv[1];
-      acc_pos[2] = acc_pos[2] + f.bend * bv[2];
-      acc_pol[0] = acc_pol[0] + f.bend * (-prod_i * u0);
-      acc_pol[1] = acc_pol[1] + f.bend * (-prod_i * u1);
-    }
-    if (f.align != 0.0f) {
-      const float cdp = cpi * cpj + spi * spj;  // cos(p_i - p_j)
-      const float sdp = spi * cpj - cpi * spj;  // sin(p_i - p_j)
-      const float u0 = cti * stj * cdp - sti * ctj;
-      const float u1 = pole ? 0.0f : -stj * sdp * inv_sti;
-      const float sdot = sti * stj * cdp + cti * ctj;
-      acc_pol[0] = acc_pol[0] + f.align * (sdot * u0);
-      acc_pol[1] = acc_pol[1] + f.align * (sdot * u1);
-    }
+    for (int k = 0; k < 3; ++k) t.bpos[k] = f.bend * (((a * tv[k] + b * disp[k]) + c * ov[k]) + e * disp[k]);
+    const float sc = (cpi * disp[0] + spi * disp[1]) * inv_d;  // sin q0 cos(p_i - q1)
+    const float ss = (spi * disp[0] - cpi * disp[1]) * inv_d;  // sin q0 sin(p_i - q1)
+    const float cq0 = disp[2] * inv_d;
+    const float u0 = cti * sc - sti * cq0;
+    const float u1 = pole ? 0.0f : -ss * inv_sti;
+    t.bpol[0] = f.bend * (-prod_i * u0);
+    t.bpol[1] = f.bend * (-prod_i * u1);
   }
+  if (f.align != 0.0f) {
+    const float cdp = cpi * cpj + spi * spj;  // cos(p_i - p_j)
+    const float sdp = spi * cpj - cpi * spj;  // sin(p_i - p_j)
+    const float u0 = cti * stj * cdp - sti * ctj;
+    const float u1 = pole ? 0.0f : -stj * sdp * inv_sti;
+    const float sdot = sti * stj * cdp + cti * ctj;
+    t.al[0] = f.align * (sdot * u0);
+    t.al[1] = f.align * (sdot * u1);
+  }
+}
+
+// The terms of one pair in the order they add to a field: p1 then p2 for the position, q1 then
+// q2 for the polarity. A term that is not applied is +0, and adding it changes nothing: a sum
+// that starts at +0 never becomes -0 under round-to-nearest, and adding +0 to any other value
+// leaves it unchanged. So acc = (acc + p1) + p2, pair after pair, is the reference's running
+// sum with only the applied terms, in the composition's order. The order flags are set on the
+// host only when both terms of a field are applied.
+template <bool HAS_TYPE, bool HAS_POL>
+__device__ __forceinline__ void apply_force_terms(const ForceParams& f, const float* disp, float d,
+                                                  int type_i, int type_j, const Pol& pi,
+                                                  const Pol& pj, float* p1, float* p2, float* q1,
+                                                  float* q2) {
+  PairTerms t;
+  pair_terms<HAS_TYPE, HAS_POL>(f, disp, d, type_i, type_j, pi, pj, t);
+  const bool bend_first = (f.flags & kForceBendFirst) != 0;
+  const bool align_first = (f.flags & kForceAlignFirst) != 0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    p1[k] = bend_first ? t.bpos[k] : t.v[k];
+    p2[k] = bend_first ? t.v[k] : t.bpos[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    q1[k] = align_first ? t.al[k] : t.bpol[k];
+    q2[k] = align_first ? t.bpol[k] : t.al[k];
+  }
+}
+
+// One invocation of the fused force, added term by term to the cell's running accumulators in
+// the composition's order, as the reference's fused forces add to the same pack
+// (kernel_fuser.hpp:40-55). Without polarity the only term is PiecewiseLinear (+0 when it is
+// not applied).
+template <bool HAS_TYPE, bool HAS_POL>
+__device__ __forceinline__ void apply_force(const ForceParams& f, const float* disp, float d,
+                                            int type_i, int type_j, const Pol& pi, const Pol& pj,
+                                            float* acc_pos, float* acc_pol) {
+  if (!HAS_POL) {
+    PairTerms t;
+    pair_terms<HAS_TYPE, HAS_POL>(f, disp, d, type_i, type_j, pi, pj, t);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) acc_pos[k] = acc_pos[k] + t.v[k];
+    return;
+  }
+  float p1[3], p2[3], q1[2], q2[2];
+  apply_force_terms<HAS_TYPE, HAS_POL>(f, disp, d, type_i, type_j, pi, pj, p1, p2, q1, q2);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) acc_pos[k] = (acc_pos[k] + p1[k]) + p2[k];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) acc_pol[k] = (acc_pol[k] + q1[k]) + q2[k];
+}
+
+// The drag output of the fused force for pair (i, j) (binned_all_pairs.hpp:155-169: the
+// functor adds to a zeroed pairwise drag).
+template <bool HAS_TYPE>
+__device__ __forceinline__ float pair_drag(const ForceParams& f, int type_i, int type_j) {
+  if (HAS_TYPE && f.drag_by_type) return 0.0f + f.drag_table[type_i * f.n_types + type_j];
+  return 0.0f + f.drag;
 }
 
 // Deterministic block sum of one float per thread (fixed butterfly + fixed wave order).

@@ -42,7 +42,8 @@ enum Mode {
   kModeCorr = 2,   // Heun substep 2 without COM fix: x += ((d1 + F)*0.5)*dt, old_vel
   kModeEuler = 3,  // Euler without COM fix: old_vel = F, x' = x + F*dt (ping-pong buffer)
   kModeCount = 4,  // pair export: number of pairs per agent
-  kModeEmit = 5    // pair export: the pairs, in visiting order
+  kModeEmit = 5,   // pair export: the pairs, in visiting order
+  kModeDeltaDrag = 6  // Delta with the drag correction (reads SweepArgs::ov_in)
 };
 
 struct SweepArgs {
@@ -67,6 +68,7 @@ struct SweepArgs {
   KeyCount kc;           // Corr / Euler: count_key of the new positions for the next rebuild
                          // (kc.keys null = not this step)
   float4* old_vel;       // Corr / Euler, may be null
+  const float4* ov_in;   // DeltaDrag: the old velocities of the evaluated slots
   float4* com_partial;   // Delta: per-block sums of F (may be null)
   unsigned long long* block_pairs;  // per-block pair-interaction counters (accumulated)
   int* row_count;        // Count

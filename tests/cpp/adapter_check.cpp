@@ -73,13 +73,29 @@ static int host_checks() {
   EXPECT(f.desc.homeostatic_radius == 0.7f && f.desc.repulsion_scale == 0.8f);
   EXPECT(f.desc.adhesion_scale == 1.0f && f.desc.n_types == 0);
   EXPECT(f.desc.bending_strength == 0.5f && f.desc.alignment_strength == 1.0f);
-  const auto g2 = forces::Bending() + forces::RelaxForce(0.8f);  // order of composition is free
+  EXPECT(f.descriptor().term_order[0] == TECS_TERM_PIECEWISE_LINEAR &&
+         f.descriptor().term_order[1] == TECS_TERM_BENDING && f.descriptor().term_order[2] == TECS_TERM_ALIGNMENT);
+  // the composition order is kept: bending first, then PiecewiseLinear (kernel_fuser.hpp:40-55)
+  const auto g2 = forces::Bending() + forces::RelaxForce(0.8f);
   EXPECT(g2.desc.homeostatic_radius == 0.8f && g2.desc.repulsion_scale == 2.0f && g2.desc.bending_strength == 1.0f);
+  EXPECT(g2.descriptor().term_order[0] == TECS_TERM_BENDING &&
+         g2.descriptor().term_order[1] == TECS_TERM_PIECEWISE_LINEAR && g2.descriptor().term_order[2] == TECS_TERM_END);
+  const auto g3 = forces::Alignment(0.25f) + forces::Drag(1.0f) + forces::RelaxForce(0.8f);
+  EXPECT(g3.descriptor().term_order[0] == TECS_TERM_ALIGNMENT &&
+         g3.descriptor().term_order[1] == TECS_TERM_PIECEWISE_LINEAR && g3.desc.drag == 1.0f &&
+         g3.desc.drag_by_type == 0 && g3.desc.alignment_strength == 0.25f);
+  const auto g4 = forces::Drag(2.0f);  // drag alone: a zero-scale PiecewiseLinear carries it
+  EXPECT(g4.descriptor().term_order[0] == TECS_TERM_PIECEWISE_LINEAR && g4.descriptor().repulsion_scale == 0.0f &&
+         g4.descriptor().drag == 2.0f);
+  const auto g5 = forces::DifferentialAdhesion(0.8f, 2.0f, {{1.0f, 0.3f}, {0.3f, 1.0f}}) +
+                  forces::DragByType({{1.0f, 0.0f}, {0.0f, 1.0f}});
+  EXPECT(g5.desc.drag_by_type == 1 && g5.desc.n_types == 2 && g5.desc.drag_table[1] == 0.0f &&
+         g5.desc.drag_table[3] == 1.0f);
   const auto t = forces::DifferentialAdhesion(0.8f, 2.0f, {{1.0f, 0.3f}, {0.3f, 1.0f}});
   EXPECT(t.desc.n_types == 2 && t.desc.adhesion_table[1] == 0.3f && t.desc.adhesion_table[3] == 1.0f);
   bool threw = false;
   try {
-    (void)(forces::PiecewiseLinear(1, 1, 1) + forces::RelaxForce(0.8f));
+    (void)(forces::PiecewiseLinear(1, 1, 1) + forces::RelaxForce(0.8f));  // the same term twice
   } catch (const Error& e) {
     threw = e.code() == TECS_ERR_INVALID;
   }
