@@ -737,6 +737,7 @@ def bench_sweep(args):
     kernel_names = {"full": f"{kname} BinnedAllPairs mode Pred (Heun substep 1), all instances",
                     "stale": f"{kname} BinnedAllPairs mode Corr (Heun substep 2, stale grid), all instances"}
     traffic = measured_traffic("sweep", variant, dom)
+    valu = measured_valu("sweep", variant, dom)
     autotune = b.autotune_ms()
     b.close()
     print(f"bench: sweep full {per['full'][0]*1e3:.1f} us, stale {per['stale'][0]*1e3:.1f} us, "
@@ -811,6 +812,14 @@ def bench_sweep(args):
             "timing_launches": {"full": full_n, "stale": stale_n, "rebuild": reb_n},
             "sweep_variant": variant,
             "autotune_ms": autotune,
+            # the bound the sweeps meet: VALU issue (counted instructions at the issue peak,
+            # against the launch time measured here)
+            "valu": None if valu is None else {
+                "instructions_per_launch": valu,
+                "peak_instructions_per_s": VALU_PEAK_WAVE_INSTR_PER_S,
+                "issue_us": valu / VALU_PEAK_WAVE_INSTR_PER_S * 1e6,
+                "frac": (valu / VALU_PEAK_WAVE_INSTR_PER_S * 1e6) / (avg_ms * 1e3),
+            },
         },
     }
     if not args.no_serve:
