@@ -218,6 +218,14 @@ int tecs_sim_get_population(tecs_sim* sim, uint32_t* agent_count, uint32_t* capa
  * order on ties, over the bins within search_radius bins of the point's bin. */
 int tecs_sim_nearest(tecs_sim* sim, int32_t n, const float* points, int32_t search_radius,
                      int32_t* out);
+/* tecs_sim_nearest without any device allocation. No counterpart in the reference: for callers
+ * that cannot allocate device memory (tight serving loops, harnesses that forbid hipMalloc).
+ * The points and results are staged in a fixed device array of 1 MiB that each GPU holds for
+ * the life of the process; at most 65536 points per call (TECS_ERR_CAPACITY beyond, nothing
+ * done); the simulation's device must be current. Calls that overlap on one GPU (several
+ * threads or simulations at once) share the array and race. Results equal tecs_sim_nearest's. */
+int tecs_sim_nearest_noalloc(tecs_sim* sim, int32_t n, const float* points, int32_t search_radius,
+                             int32_t* out);
 /* get_random_point_index_in_neighbourhood_bins (grid.hpp:453-495): any cell of those bins. */
 int tecs_sim_random_in_bins(tecs_sim* sim, int32_t n, const float* points, int32_t radius_bins,
                             const float* uniforms, int32_t* out);

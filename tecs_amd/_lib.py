@@ -33,6 +33,7 @@ PROTOTYPES = [
     ("tecs_sim_set_capacity", C.c_int, [_P, C.c_uint32]),
     ("tecs_sim_get_population", C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     ("tecs_sim_nearest", C.c_int, [_P, _I32, _F32P, _I32, _P]),
+    ("tecs_sim_nearest_noalloc", C.c_int, [_P, _I32, _F32P, _I32, _P]),
     ("tecs_sim_random_in_bins", C.c_int, [_P, _I32, _F32P, _I32, _F32P, _P]),
     ("tecs_sim_random_neighbour", C.c_int, [_P, _I32, _F32P, C.c_float, _F32P, _P]),
     ("tecs_sim_set_walls", C.c_int, [_P, _P, _I32]),

@@ -220,6 +220,7 @@ struct Batch {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   bool inst_dirty = false;
+  int device = 0;  // the HIP device current at creation (all state lives there)
 
   // device state
   InstanceDesc* d_inst = nullptr;
@@ -529,6 +530,7 @@ struct Batch {
     for (int k = 0; k < count; ++k)
       for (int b = 0; b < h_inst[k].n_blocks; ++b) h_block_inst[h_inst[k].block_begin + b] = k;
 
+    HIP_TRY(hipGetDevice(&device));
     if (user_stream) {
       stream = static_cast<hipStream_t>(user_stream);
     } else {
@@ -1418,6 +1420,46 @@ struct Batch {
     if (e != hipSuccess) rc = fail(TECS_ERR_DEVICE, std::string("grid query: ") + hipGetErrorString(e));
     hipFree(buf);
     return rc;
+  }
+
+  // get_nearest_point_index for member 0 like query(kQueryNearest, ...), but with the points and
+  // results staged in the fixed device array of kernels_query.hip instead of a buffer allocated
+  // per call: no device allocation, at most kNoAllocMaxQueries points, the simulation's device
+  // current.
+  int nearest_noalloc(int n, const float* points, int radius_bins, int32_t* out) {
+    if (n < 0 || (n > 0 && (!points || !out))) return fail(TECS_ERR_INVALID, "bad query arrays");
+    if (n > kNoAllocMaxQueries)
+      return fail(TECS_ERR_CAPACITY, "at most " + std::to_string(kNoAllocMaxQueries) +
+                                         " points per allocation-free query");
+    if (!is_binned(finder)) return fail(TECS_ERR_UNSUPPORTED, "grid queries need a binned pair finder");
+    if (!grid_built) return fail(TECS_ERR_STATE, "no grid yet: queries follow a full step or rebuild");
+    if (radius_bins < 0) return fail(TECS_ERR_INVALID, "negative search radius");
+    if (n == 0) return TECS_OK;
+    int cur = -1;
+    HIP_TRY(hipGetDevice(&cur));
+    if (cur != device) return fail(TECS_ERR_STATE, "the simulation's device is not the current device");
+    void* base = nullptr;
+    const int se = noalloc_scratch(&base);
+    if (se != 0)
+      return fail(TECS_ERR_DEVICE, std::string("scratch address: ") + hipGetErrorString((hipError_t)se));
+    float* d_points = static_cast<float*>(base);
+    int32_t* d_out = reinterpret_cast<int32_t*>(static_cast<unsigned char*>(base) + kNoAllocOutOffset);
+    HIP_TRY(hipMemcpyAsync(d_points, points, sizeof(float) * 3 * (size_t)n, hipMemcpyHostToDevice, stream));
+    QueryArgs q{};
+    q.inst = d_inst;
+    q.xs = x[xc];
+    q.off = fill_bins ? off : coff[ic];
+    q.id = id[ic];
+    q.n = n;
+    q.points = d_points;
+    q.uniforms = nullptr;
+    q.radius_bins = radius_bins;
+    q.out = d_out;
+    launch_query(kQueryNearest, q, stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, d_out, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return TECS_OK;
   }
 
   // Simulation::set_capacity (simulation.hpp:266-274) for member k, within its allocation.

@@ -5,8 +5,37 @@
 #include "tecs_internal.hpp"
 
 #include <cfloat>
+#include <mutex>
 
 namespace tecs_amd {
+
+// Staging of the allocation-free nearest query (tecs_sim_nearest_noalloc): the query points
+// (3 floats each) from offset 0, the results (one int32 each) from kNoAllocOutOffset. Each GPU
+// holds one copy of the array for the life of the process; every call on that GPU uses it, so
+// calls that overlap on one GPU race. Host code never passes the array's name to a kernel or a
+// copy: it is a host-side stand-in there; the device address comes from hipGetSymbolAddress.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+int noalloc_scratch(void** out) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static void* addr[kMaxDevices] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return (int)e;
+  if (dev < 0 || dev >= kMaxDevices) return (int)hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> g(mu);
+  if (!addr[dev]) {
+    // once per device, with that device current: this GPU's copy of the array
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return (int)e;
+    addr[dev] = p;
+  }
+  *out = addr[dev];
+  return 0;
+}
+
 namespace {
 
 // The contiguous slot range of every x-run of bins within `radius_bins` of the point's bin, in

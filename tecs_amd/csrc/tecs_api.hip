@@ -156,6 +156,12 @@ int tecs_sim_nearest(tecs_sim* sim, int32_t n, const float* points, int32_t sear
   return sim->b->query(tecs_amd::kQueryNearest, n, points, search_radius, 0.0f, nullptr, out);
 }
 
+int tecs_sim_nearest_noalloc(tecs_sim* sim, int32_t n, const float* points, int32_t search_radius,
+                             int32_t* out) {
+  GUARD(sim);
+  return sim->b->nearest_noalloc(n, points, search_radius, out);
+}
+
 int tecs_sim_random_in_bins(tecs_sim* sim, int32_t n, const float* points, int32_t radius_bins,
                             const float* uniforms, int32_t* out) {
   GUARD(sim);

@@ -242,6 +242,14 @@ struct QueryArgs {
   int32_t* out;
 };
 void launch_query(int kind, const QueryArgs& a, hipStream_t stream);
+// The allocation-free nearest query stages its points and results in a fixed device array
+// (kernels_query.hip): at most kNoAllocMaxQueries queries per call.
+constexpr int kNoAllocMaxQueries = 65536;
+constexpr size_t kNoAllocOutOffset = (size_t)kNoAllocMaxQueries * 3 * sizeof(float);
+constexpr size_t kNoAllocScratchBytes = kNoAllocOutOffset + (size_t)kNoAllocMaxQueries * sizeof(int32_t);
+// The current device's address of that array (hipGetSymbolAddress, resolved once per device);
+// returns a hipError_t value, 0 on success.
+int noalloc_scratch(void** out);
 
 // Generic wall forces, link forces and division (kernels_extra.hip).
 // inv[slot_begin + agent] = slot over all slots

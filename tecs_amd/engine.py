@@ -123,6 +123,14 @@ class GpuSimulation:
         check(self._lib.tecs_sim_nearest(self._h, p.shape[0], _ptr(p), int(search_radius), _ptr(out)))
         return out
 
+    def nearest_noalloc(self, points, search_radius: int = 1):
+        """nearest() without any device allocation (tecs_sim_nearest_noalloc): at most 65536
+        points per call, staged in a fixed device array."""
+        p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+        out = np.empty(p.shape[0], np.int32)
+        check(self._lib.tecs_sim_nearest_noalloc(self._h, p.shape[0], _ptr(p), int(search_radius), _ptr(out)))
+        return out
+
     def random_in_bins(self, points, radius_bins: int, uniforms):
         """Grid::get_random_point_index_in_neighbourhood_bins with the caller's draws."""
         p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
