@@ -301,6 +301,10 @@ int tecs_sim_pairs(tecs_sim* sim, int32_t is_full_step, int64_t* row_ptr, int32_
 int tecs_sim_rebuild(tecs_sim* sim);
 
 int tecs_sim_synchronize(tecs_sim* sim);
+/* Whether a step ended with a non-finite position or polarity (NaN or infinity, e.g. an
+ * overflow or a NaN upload) since creation: *flag = 1, else 0. The kernels that end a step set
+ * it; it stays set. Synchronous. No counterpart in the reference (it does not check). */
+int tecs_sim_get_nonfinite(tecs_sim* sim, int32_t* flag);
 int tecs_sim_get_stats(tecs_sim* sim, tecs_stats* stats);
 int tecs_sim_reset_stats(tecs_sim* sim);
 
@@ -380,6 +384,8 @@ int tecs_batch_divide(tecs_batch* batch, int32_t index, float rate, float offset
                       const double* uniforms, uint32_t* new_count);
 int tecs_batch_set_capacity(tecs_batch* batch, int32_t index, uint32_t capacity);
 int tecs_batch_synchronize(tecs_batch* batch);
+/* tecs_sim_get_nonfinite per member: flags[count]. */
+int tecs_batch_get_nonfinite(tecs_batch* batch, int32_t* flags);
 int tecs_batch_get_stats(tecs_batch* batch, tecs_stats* stats);
 int tecs_batch_reset_stats(tecs_batch* batch);
 int tecs_batch_set_sweep_variant(tecs_batch* batch, int32_t variant);
@@ -424,14 +430,23 @@ int tecs_read_dataset(const char* h5_path, const char* dataset, float* out, int6
 
 /* =========================================================================================
  * Serving: simulation requests admitted by an HBM budget as they arrive and advanced together
- * (continuous batching). Requests that share pair finder, integrator, COM fixer, fields and
- * old-velocity tracking run as members of one batch; a batch is re-formed when members join or
- * leave, with the running members' state copied on the device. Every request keeps its own
- * arithmetic: its result is bitwise that of the same simulation run alone. Served requests must
- * rebuild the grid every step (rebuild_every_n = 0). No counterpart in the reference.
+ * (continuous batching). Requests that share pair finder, integrator, COM fixer, fields,
+ * old-velocity tracking and (binned finders) rebuild period run as members of one batch; a batch
+ * is re-formed when members join or leave, with the running members' state copied on the
+ * device, and with a rebuild period R > 0 only where the continuing members are at a rebuild
+ * point of their own schedule. Every request keeps its own arithmetic: its result is bitwise
+ * that of the same simulation run alone. A request whose state becomes non-finite (NaN or
+ * infinity at the end of a step) fails without disturbing the others; a request can be
+ * cancelled. No counterpart in the reference.
  * ========================================================================================= */
 typedef struct tecs_server tecs_server;
-enum { TECS_REQUEST_PENDING = 0, TECS_REQUEST_RUNNING = 1, TECS_REQUEST_DONE = 2 };
+enum {
+  TECS_REQUEST_PENDING = 0,
+  TECS_REQUEST_RUNNING = 1,
+  TECS_REQUEST_DONE = 2,
+  TECS_REQUEST_FAILED = 3,    /* a step ended with a non-finite position or polarity */
+  TECS_REQUEST_CANCELLED = 4  /* tecs_server_cancel */
+};
 typedef struct tecs_request_info {
   int32_t state;        /* TECS_REQUEST_* */
   int32_t steps_done;
@@ -441,7 +456,7 @@ typedef struct tecs_request_info {
   uint64_t hbm_bytes;   /* the device bytes the request is charged */
 } tecs_request_info;
 typedef struct tecs_server_stats {
-  uint64_t completed, pending, running;
+  uint64_t completed, pending, running, failed, cancelled;
   uint64_t pair_interactions;  /* over all requests, all evaluations */
   uint64_t member_steps;       /* sum over requests of the steps run */
   uint64_t regroups;           /* batches re-formed */
@@ -462,7 +477,11 @@ int tecs_server_submit(tecs_server* server, const tecs_sim_desc* desc, const flo
  * at least one group was serviced, or at once when nothing is running. */
 int tecs_server_advance(tecs_server* server, int32_t* running, int32_t* pending);
 int tecs_server_request_info(tecs_server* server, int64_t request_id, tecs_request_info* info);
-/* The final state of a finished request (positions n*3, polarities n*2 or NULL). */
+/* Cancel a request: a pending one leaves at once, a running one at the end of its group's
+ * chunk in flight (its state stays RUNNING until then). TECS_ERR_STATE when it already ended. */
+int tecs_server_cancel(tecs_server* server, int64_t request_id);
+/* The final state of a finished request (positions n*3, polarities n*2 or NULL);
+ * TECS_ERR_STATE for a request that is not done, failed or was cancelled. */
 int tecs_server_result(tecs_server* server, int64_t request_id, float* positions,
                        float* polarities);
 int tecs_server_get_stats(tecs_server* server, tecs_server_stats* stats);

@@ -45,6 +45,7 @@ PROTOTYPES = [
     ("tecs_sim_pairs", C.c_int, [_P, _I32, _P, _P, C.c_int64, _I64P]),
     ("tecs_sim_rebuild", C.c_int, [_P]),
     ("tecs_sim_synchronize", C.c_int, [_P]),
+    ("tecs_sim_get_nonfinite", C.c_int, [_P, C.POINTER(C.c_int32)]),
     ("tecs_sim_get_stats", C.c_int, [_P, C.POINTER(abi.Stats)]),
     ("tecs_sim_reset_stats", C.c_int, [_P]),
     ("tecs_sim_set_sweep_variant", C.c_int, [_P, _I32]),
@@ -67,6 +68,7 @@ PROTOTYPES = [
     ("tecs_batch_divide", C.c_int, [_P, _I32, C.c_float, C.c_float, _P, C.POINTER(C.c_uint32)]),
     ("tecs_batch_set_capacity", C.c_int, [_P, _I32, C.c_uint32]),
     ("tecs_batch_synchronize", C.c_int, [_P]),
+    ("tecs_batch_get_nonfinite", C.c_int, [_P, _P]),
     ("tecs_batch_get_stats", C.c_int, [_P, C.POINTER(abi.Stats)]),
     ("tecs_batch_reset_stats", C.c_int, [_P]),
     ("tecs_batch_set_sweep_variant", C.c_int, [_P, _I32]),
@@ -89,6 +91,7 @@ PROTOTYPES = [
     ("tecs_server_submit", C.c_int, [_P, C.POINTER(abi.SimDesc), _F32P, _F32P, _P, C.c_double, _I32, _I64P]),
     ("tecs_server_advance", C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     ("tecs_server_request_info", C.c_int, [_P, C.c_int64, C.POINTER(abi.RequestInfo)]),
+    ("tecs_server_cancel", C.c_int, [_P, C.c_int64]),
     ("tecs_server_result", C.c_int, [_P, C.c_int64, _F32P, _F32P]),
     ("tecs_server_get_stats", C.c_int, [_P, C.POINTER(abi.ServerStats)]),
     ("tecs_copy_probe", C.c_int, [C.c_int64, _I32, C.POINTER(C.c_double)]),

@@ -241,6 +241,8 @@ def copy_desc(d: SimDesc) -> SimDesc:
 REQUEST_PENDING = 0
 REQUEST_RUNNING = 1
 REQUEST_DONE = 2
+REQUEST_FAILED = 3
+REQUEST_CANCELLED = 4
 
 
 class RequestInfo(C.Structure):
@@ -259,6 +261,8 @@ class ServerStats(C.Structure):
         ("completed", C.c_uint64),
         ("pending", C.c_uint64),
         ("running", C.c_uint64),
+        ("failed", C.c_uint64),
+        ("cancelled", C.c_uint64),
         ("pair_interactions", C.c_uint64),
         ("member_steps", C.c_uint64),
         ("regroups", C.c_uint64),

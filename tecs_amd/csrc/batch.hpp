@@ -253,6 +253,8 @@ struct Batch {
   unsigned long long* block_pairs = nullptr;
   float4* com_partial = nullptr;
   float4* com = nullptr;
+  int* nonfinite = nullptr;  // per member: 1 once a step ended with a non-finite position or
+                             // polarity (set by the corrector / Euler kernels, never cleared by them)
   float* st_pos = nullptr;
   float* st_pol = nullptr;
   float* st_ov = nullptr;
@@ -621,6 +623,8 @@ struct Batch {
     }
     TRY(alloc(&com_partial, (size_t)n_blocks * kMaxTeamLanes));
     TRY(alloc(&com, n_inst));
+    TRY(alloc(&nonfinite, n_inst));
+    HIP_TRY(hipMemsetAsync(nonfinite, 0, sizeof(int) * n_inst, stream));
     HIP_TRY(hipMemsetAsync(block_pairs, 0, sizeof(unsigned long long) * n_blocks * kMaxTeamLanes, stream));
     HIP_TRY(hipMemsetAsync(com, 0, sizeof(float4) * n_inst, stream));
     for (int b = 0; b < 2; ++b) {
@@ -1035,6 +1039,7 @@ struct Batch {
     // rebuilds, as the fused sweep epilogues do
     const bool emit = kind != kUpdatePred && emit_keys();
     if (emit) u.kc = key_count();
+    if (kind != kUpdatePred) u.nonfinite = nonfinite;
     launch_update(kind, has_pol, u, n_blocks, stream);
     HIP_TRY(hipGetLastError());
     if (kind != kUpdatePred) keys_ready = emit;
@@ -1137,6 +1142,7 @@ struct Batch {
         b.d1_pos = d1;
         b.d1_pol = d1p;
         b.old_vel = track_ov ? ov[xc] : nullptr;
+        b.nonfinite = nonfinite;
         TRY(sweep(kModeCorr, b, kTimeSweepStale));
         keys_ready = emit;
       } else {
@@ -1164,6 +1170,7 @@ struct Batch {
           a.kc = key_count();
         }
         a.old_vel = track_ov ? ov[xc ^ 1] : nullptr;
+        a.nonfinite = nonfinite;
         TRY(sweep(kModeEuler, a, kTimeSweepFull));
         xc ^= 1;
         keys_ready = emit;
@@ -1671,6 +1678,7 @@ struct Batch {
     if (!old.member_links.empty() && !old.member_links[0].empty())
       TRY(set_links(0, old.member_links[0].data(), (int64_t)(old.member_links[0].size() / 2),
                     old.member_link_k[0]));
+    HIP_TRY(hipMemcpyAsync(nonfinite, old.nonfinite, sizeof(int), hipMemcpyDeviceToDevice, stream));
     variant = old.variant;
     graph_mode = old.graph_mode;
     timing = old.timing;
@@ -1857,8 +1865,9 @@ struct Batch {
 
   // Move member ks of src (its current cell state: positions, types, polarities, ids, old
   // velocities) into member kd of this batch, which must not have stepped yet. The grid is not
-  // carried: this batch rebuilds it at its first full step, which every member's schedule does
-  // anyway when rebuild_every_n is 0 (the server only groups such requests).
+  // carried: this batch rebuilds it at its first full step, which the member's own schedule does
+  // at that step when rebuild_every_n is 0, or when its steps so far are a multiple of a period
+  // > 0 (the server re-forms such groups only then).
   int copy_member_from(Batch& src, int ks, int kd) {
     TRY(check_index(kd));
     TRY(src.check_index(ks));
@@ -1906,6 +1915,20 @@ struct Batch {
     size_t dl = 0;  // NaiveDelaunay counters, one per (cell, partner) slot pair
     if (d.pair_finder == kNaiveDelaunay) dl = (slots - kBlock) * (slots - kBlock) * 4;
     return slots * per_slot + bins * 28 + dl + 2 * 4096;
+  }
+
+  // Which members' state became non-finite at the end of some step since creation (or the last
+  // clear): flags[k] = 1 for member k. Synchronous.
+  int get_nonfinite(int32_t* flags) {
+    if (!flags) return fail(TECS_ERR_INVALID, "null flags");
+    HIP_TRY(hipMemcpyAsync(flags, nonfinite, sizeof(int) * n_inst, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return TECS_OK;
+  }
+  int clear_nonfinite() {
+    HIP_TRY(hipMemsetAsync(nonfinite, 0, sizeof(int) * n_inst, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return TECS_OK;
   }
 
   int get_stats(tecs_stats* out) {

@@ -224,8 +224,8 @@ __device__ __forceinline__ CorrIn load_corr(const SweepArgs& a, int s) {
 // Epilogue shared by the binned and naive sweeps: turn the accumulators into the evaluation
 // result and apply the mode's integrator update.
 template <bool HAS_POL, int MODE, bool SYM>
-__device__ __forceinline__ void finish(const SweepArgs& a, const InstanceDesc& in, int lb, int s,
-                                       bool active, float4 me, float2 mp, CellState& cs,
+__device__ __forceinline__ void finish(const SweepArgs& a, const InstanceDesc& in, int ii, int lb,
+                                       int s, bool active, float4 me, float2 mp, CellState& cs,
                                        const CorrIn& ci) {
   __shared__ float lds_f[4];
   __shared__ unsigned lds_u[4];
@@ -250,6 +250,7 @@ __device__ __forceinline__ void finish(const SweepArgs& a, const InstanceDesc& i
   G[1] = 0.0f + cs.acc_pol[1];
   const float dt = in.dt;
   float4 xnew = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // Corr / Euler: the new position
+  float2 pnew = make_float2(0.0f, 0.0f);               // Corr / Euler: the new polarity
   if (active) {
     if (MODE == kModeDelta || MODE == kModeDeltaDrag) {
       a.o_pos[s] = make_float4(F[0], F[1], F[2], 0.0f);
@@ -279,6 +280,7 @@ __device__ __forceinline__ void finish(const SweepArgs& a, const InstanceDesc& i
                                       p0.y + (e1.y + G[1]) * 0.5f * dt);
         a.o2_pol[s] = np;
         a.o_trig[s] = polar_trig(np);
+        pnew = np;
       }
     } else if (MODE == kModeEuler) {
       // apply_euler, euler.hpp:52-68
@@ -289,8 +291,11 @@ __device__ __forceinline__ void finish(const SweepArgs& a, const InstanceDesc& i
         const float2 np = make_float2(mp.x + G[0] * dt, mp.y + G[1] * dt);
         a.o_pol[s] = np;
         a.o_trig[s] = polar_trig(np);
+        pnew = np;
       }
     }
+    if ((MODE == kModeCorr || MODE == kModeEuler) && a.nonfinite && not_finite(xnew, pnew, HAS_POL))
+      a.nonfinite[ii] = 1;
   }
   // keys of the new positions for the rebuild that starts the next step
   if ((MODE == kModeCorr || MODE == kModeEuler) && a.kc.keys)
@@ -339,7 +344,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_binned(SweepArgs a) {
   } else if (active) {
     walk_binned<GABRIEL, HAS_TYPE, HAS_POL, MODE, SYM>(a, in, s, me_id, me, pme, cs);
   }
-  finish<HAS_POL, MODE, SYM>(a, in, lb, s, active, me, mp, cs, ci);
+  finish<HAS_POL, MODE, SYM>(a, in, ii, lb, s, active, me, mp, cs, ci);
 }
 
 // Naive finders (naive_all_pairs.hpp:41-103, naive_gabriel.hpp:47-119): j runs over all agents
@@ -403,7 +408,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_naive(SweepArgs a) {
       visit<HAS_TYPE, HAS_POL, MODE, SYM>(a, in, cs, me, pme, m, pj, disp, d2);
     }
   }
-  finish<HAS_POL, MODE, SYM>(a, in, lb, s, active, me, mp, cs, ci);
+  finish<HAS_POL, MODE, SYM>(a, in, ii, lb, s, active, me, mp, cs, ci);
 }
 
 // ------------------------------------------------------------------ symmetric finders
@@ -818,7 +823,13 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
   const bool keys = KIND != kUpdatePred && a.kc.keys != nullptr;
   if (!cell && !keys) return;
   float4 xnew = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  if (cell) update_cell<KIND, HAS_POL>(a, in, ii, s, xnew);
+  if (cell) {
+    update_cell<KIND, HAS_POL>(a, in, ii, s, xnew);
+    if (KIND != kUpdatePred && a.nonfinite) {
+      const float2 pn = HAS_POL ? a.pol[s] : make_float2(0.0f, 0.0f);
+      if (not_finite(xnew, pn, HAS_POL)) a.nonfinite[ii] = 1;
+    }
+  }
   if (keys) count_key(in, a.kc, s, cell, s - in.slot_begin == in.n, xnew);
 }
 

@@ -589,6 +589,9 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
       }
     }
   }
+  if ((MODE == kModeCorr || MODE == kModeEuler) && leader && a.nonfinite &&
+      not_finite(xnew, npol, HAS_POL))
+    a.nonfinite[ii] = 1;
   if (HAS_POL && (MODE == kModePred || MODE == kModeCorr || MODE == kModeEuler)) {
     // polar_trig of the new polarity, split over the team: lane 0 takes theta, lane 1 phi
     const int base_lane = (int)(threadIdx.x & 63) - lane;

@@ -18,9 +18,19 @@
 // running request stays on the device between chunks, and batch buffers come from a device
 // pool, so a re-form costs no hipMalloc/hipFree in the steady state. The autotuner's choice is
 // kept per group and re-used while the group's cell count stays within a factor of two.
-// Every request rebuilds its grid at every full step (rebuild_every_n = 0 is required), so
-// regrouping changes nothing in its arithmetic: a request's result is bitwise the result of the
-// same simulation run alone. This serving layer has no counterpart in the reference.
+// A re-formed batch rebuilds its grid at its first full step. Requests that rebuild every step
+// (rebuild_every_n = 0) do so anyway; requests with a period R > 0 are grouped by R, and their
+// group is re-formed only where every continuing member is at a rebuild point of its own
+// schedule (its steps so far a multiple of R: the reference rebuilds at steps 1, R + 1, ...,
+// binned_all_pairs.hpp:77-84). A member that finishes between two such points stays in the batch
+// (its result already taken) until the next one, and joiners wait for it. So regrouping changes
+// nothing in a request's arithmetic: its result is bitwise the result of the same simulation run
+// alone.
+// Failure handling: the kernels that end a step flag members whose new state is not finite;
+// after each chunk such requests fail (state TECS_REQUEST_FAILED) and leave their group at its
+// next re-form, and the others go on unaffected. tecs_server_cancel drops a pending request at
+// once and a running one at the end of its group's chunk. This serving layer has no
+// counterpart in the reference.
 #include "server.hpp"
 
 #include <chrono>
@@ -33,10 +43,15 @@ double now_s(const std::chrono::steady_clock::time_point& t0) {
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+int period_of(const tecs_sim_desc& d) {
+  return is_binned(d.pair_finder) ? d.pair_finder_settings.rebuild_every_n : 0;
+}
+
 bool same_group(const tecs_sim_desc& a, const tecs_sim_desc& b) {
   return a.pair_finder == b.pair_finder && a.integrator == b.integrator &&
          a.com_fixer == b.com_fixer && a.fields == b.fields &&
-         (a.track_old_velocities != 0) == (b.track_old_velocities != 0);
+         (a.track_old_velocities != 0) == (b.track_old_velocities != 0) &&
+         period_of(a) == period_of(b);
 }
 }  // namespace
 
@@ -57,8 +72,6 @@ int Server::submit(const tecs_sim_desc* d, const float* pos, const float* pol, c
                    double dt, int32_t n_steps, int64_t* id_out) {
   int rc = validate_desc(d);
   if (rc) return rc;
-  if (is_binned(d->pair_finder) && d->pair_finder_settings.rebuild_every_n != 0)
-    return fail(TECS_ERR_UNSUPPORTED, "served requests must rebuild the grid every step");
   if (n_steps < 0) return fail(TECS_ERR_INVALID, "n_steps must be >= 0");
   if (!pos && d->agent_count) return fail(TECS_ERR_INVALID, "positions required");
   auto r = std::make_unique<Request>();
@@ -118,6 +131,7 @@ int Server::regroup(Group& g, std::vector<Request*> members) {
   g.batch = nb.release();
   g.members = std::move(members);
   g.retired = false;
+  g.phase = 0;
   regroup_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
   return TECS_OK;
 }
@@ -125,6 +139,9 @@ int Server::regroup(Group& g, std::vector<Request*> members) {
 // The chunk in flight on the group's (now idle) stream has ended: account it, retire members.
 int Server::finish_chunk(Group& g) {
   g.busy = false;
+  if (g.period > 0) g.phase = (g.phase + g.chunk_steps) % g.period;
+  std::vector<int32_t> bad(g.members.size(), 0);
+  if (g.batch && !g.members.empty()) TRY(g.batch->get_nonfinite(bad.data()));
   if (g.batch && g.batch->tuned >= 0 && g.tuned != g.batch->tuned) {
     g.tuned = g.batch->tuned;
     g.tuned_cells = 0;
@@ -134,6 +151,17 @@ int Server::finish_chunk(Group& g) {
     if (r->done) continue;
     r->steps_done += g.chunk_steps;
     steps_run += (uint64_t)g.chunk_steps;
+    if (bad[(size_t)r->slot] || r->cancel) {
+      // leaves without a result: a step ended with a non-finite state, or it was cancelled
+      r->failed = bad[(size_t)r->slot] != 0;
+      r->cancelled = !r->failed;
+      r->t_done = now_s(t0);
+      r->done = true;
+      used -= r->bytes;
+      (r->failed ? failed : cancelled)++;
+      g.retired = true;
+      continue;
+    }
     if (r->steps_done < r->n_steps) continue;
     const size_t n = r->desc.agent_count;
     r->pos.assign(3 * n, 0.0f);
@@ -152,7 +180,13 @@ int Server::finish_chunk(Group& g) {
 
 // Re-form the (idle) group if its membership changed, then enqueue its next chunk.
 int Server::start_chunk(Group& g) {
-  if (g.retired || !g.joining.empty()) {
+  bool live = false;  // members that continue through a re-form
+  for (Request* r : g.members) live = live || !r->done;
+  // a re-form rebuilds the grid: with a rebuild period, continuing members must be at a
+  // rebuild point of their schedule (every member of a batch has the same phase)
+  const bool can_reform = g.period == 0 || g.phase == 0 || !live;
+  const bool want_reform = g.retired || !g.joining.empty();
+  if (want_reform && can_reform) {
     std::vector<Request*> members;
     for (Request* r : g.members) {
       if (r->done)
@@ -165,8 +199,14 @@ int Server::start_chunk(Group& g) {
     TRY(regroup(g, members));
   }
   if (g.members.empty()) return TECS_OK;
+  live = false;
+  for (Request* r : g.members) live = live || !r->done;
+  if (!live) return TECS_OK;  // only finished members, waiting for a rebuild point: none left to run
   int steps = chunk;
-  for (Request* r : g.members) steps = std::min(steps, r->n_steps - r->steps_done);
+  for (Request* r : g.members)
+    if (!r->done) steps = std::min(steps, r->n_steps - r->steps_done);
+  // a re-form is waiting: stop at the next rebuild point
+  if (want_reform && !can_reform) steps = std::min(steps, g.period - g.phase);
   std::vector<double> dts;
   for (Request* r : g.members) dts.push_back(r->dt);
   if (steps > 0) TRY(g.batch->take_step(dts.data(), steps));
@@ -191,6 +231,7 @@ int Server::advance(int32_t* running, int32_t* waiting) {
       groups.emplace_back();
       g = &groups.back();
       g->key = r->desc;
+      g->period = period_of(r->desc);
       HIP_TRY(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
     }
     g->joining.push_back(r);
@@ -231,7 +272,10 @@ int Server::info(int64_t id, tecs_request_info* out) const {
   auto it = requests.find(id);
   if (it == requests.end()) return fail(TECS_ERR_INVALID, "unknown request");
   const Request& r = *it->second;
-  out->state = r.done ? TECS_REQUEST_DONE : (r.slot >= 0 ? TECS_REQUEST_RUNNING : TECS_REQUEST_PENDING);
+  out->state = r.failed      ? TECS_REQUEST_FAILED
+               : r.cancelled ? TECS_REQUEST_CANCELLED
+               : r.done      ? TECS_REQUEST_DONE
+               : (r.slot >= 0 ? TECS_REQUEST_RUNNING : TECS_REQUEST_PENDING);
   out->steps_done = r.steps_done;
   out->t_submit = r.t_submit;
   out->t_admit = r.t_admit;
@@ -244,14 +288,47 @@ int Server::result(int64_t id, float* pos, float* pol) {
   auto it = requests.find(id);
   if (it == requests.end()) return fail(TECS_ERR_INVALID, "unknown request");
   Request& r = *it->second;
+  if (r.failed) return fail(TECS_ERR_STATE, "request failed: a step ended with a non-finite state");
+  if (r.cancelled) return fail(TECS_ERR_STATE, "request was cancelled");
   if (!r.done) return fail(TECS_ERR_STATE, "request not finished");
   if (pos) std::memcpy(pos, r.pos.data(), sizeof(float) * r.pos.size());
   if (pol && !r.pol.empty()) std::memcpy(pol, r.pol.data(), sizeof(float) * r.pol.size());
   return TECS_OK;
 }
 
+int Server::cancel(int64_t id) {
+  auto it = requests.find(id);
+  if (it == requests.end()) return fail(TECS_ERR_INVALID, "unknown request");
+  Request* r = it->second.get();
+  if (r->done) return fail(TECS_ERR_STATE, "request already finished");
+  auto drop = [&](bool charged) {
+    r->cancelled = true;
+    r->done = true;
+    r->t_done = now_s(t0);
+    if (charged) used -= r->bytes;
+    cancelled++;
+  };
+  for (auto p = pending.begin(); p != pending.end(); ++p)
+    if (*p == r) {
+      pending.erase(p);
+      drop(false);
+      return TECS_OK;
+    }
+  for (auto& g : groups)
+    for (auto j = g.joining.begin(); j != g.joining.end(); ++j)
+      if (*j == r) {
+        g.joining.erase(j);
+        drop(true);
+        return TECS_OK;
+      }
+  r->cancel = true;  // running: leaves at the end of its group's chunk
+  return TECS_OK;
+}
+
 int Server::stats(tecs_server_stats* out) {
   out->completed = completed;
+  out->failed = failed;
+  out->cancelled = cancelled;
   out->pending = pending.size();
   out->running = 0;
   uint64_t pairs = retired_pairs;

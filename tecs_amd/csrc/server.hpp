@@ -21,7 +21,10 @@ struct Request {
   std::vector<float> pos, pol;  // initial state, then the result
   std::vector<uint8_t> types;
   int slot = -1;  // member index in its group's batch while running
-  bool done = false;
+  bool done = false;       // finished, failed or cancelled: no longer advanced
+  bool failed = false;     // a step ended with a non-finite state
+  bool cancelled = false;
+  bool cancel = false;     // cancellation asked while running: leaves at the chunk's end
   double t_submit = 0.0, t_admit = 0.0, t_done = 0.0;
 };
 
@@ -33,6 +36,7 @@ class Server {
              double dt, int32_t n_steps, int64_t* id_out);
   int advance(int32_t* running, int32_t* waiting);
   int info(int64_t id, tecs_request_info* out) const;
+  int cancel(int64_t id);
   int result(int64_t id, float* pos, float* pol);
   int stats(tecs_server_stats* out);
 
@@ -48,6 +52,8 @@ class Server {
     int chunk_steps = 0;              // steps of the chunk in flight
     int tuned = -1;                   // sweep variant the autotuner picked for this group
     int64_t tuned_cells = 0;          // the group's cell count when it was tuned
+    int period = 0;                   // rebuild_every_n of the group's binned members (0: every step)
+    int phase = 0;                    // steps since the batch was formed, modulo period
   };
   int regroup(Group& g, std::vector<Request*> members);
   int finish_chunk(Group& g);
@@ -62,7 +68,7 @@ class Server {
   std::map<int64_t, std::unique_ptr<Request>> requests;
   std::list<Request*> pending;
   std::list<Group> groups;  // stable addresses
-  uint64_t retired_pairs = 0, steps_run = 0, rebuilds = 0, completed = 0;
+  uint64_t retired_pairs = 0, steps_run = 0, rebuilds = 0, completed = 0, failed = 0, cancelled = 0;
   double regroup_s = 0.0;
 };
 

@@ -229,6 +229,16 @@ int tecs_sim_synchronize(tecs_sim* sim) {
   return TECS_OK;
 }
 
+int tecs_sim_get_nonfinite(tecs_sim* sim, int32_t* flag) {
+  GUARD(sim);
+  return sim->b->get_nonfinite(flag);
+}
+
+int tecs_batch_get_nonfinite(tecs_batch* batch, int32_t* flags) {
+  GUARD(batch);
+  return batch->b.get_nonfinite(flags);
+}
+
 int tecs_sim_get_stats(tecs_sim* sim, tecs_stats* stats) {
   GUARD(sim);
   if (!stats) return fail(TECS_ERR_INVALID, "null stats");
@@ -549,6 +559,11 @@ int tecs_server_request_info(tecs_server* server, int64_t request_id, tecs_reque
 int tecs_server_result(tecs_server* server, int64_t request_id, float* positions, float* polarities) {
   GUARD(server);
   return server->s.result(request_id, positions, polarities);
+}
+
+int tecs_server_cancel(tecs_server* server, int64_t request_id) {
+  GUARD(server);
+  return server->s.cancel(request_id);
 }
 
 int tecs_server_get_stats(tecs_server* server, tecs_server_stats* stats) {

@@ -405,6 +405,13 @@ __device__ __forceinline__ float pair_drag(const ForceParams& f, int type_i, int
   return 0.0f + f.drag;
 }
 
+// A step's new state of a cell is not finite (a NaN or an infinity: overflow, or a NaN upload).
+__device__ __forceinline__ bool not_finite(float4 x, float2 p, bool has_pol) {
+  bool ok = isfinite(x.x) && isfinite(x.y) && isfinite(x.z);
+  if (has_pol) ok = ok && isfinite(p.x) && isfinite(p.y);
+  return !ok;
+}
+
 // Deterministic block sum of one float per thread (fixed butterfly + fixed wave order).
 __device__ __forceinline__ float block_sum_f(float v, float* lds) {
 #pragma unroll

@@ -69,6 +69,8 @@ struct SweepArgs {
                          // (kc.keys null = not this step)
   float4* old_vel;       // Corr / Euler, may be null
   const float4* ov_in;   // DeltaDrag: the old velocities of the evaluated slots
+  int* nonfinite;        // Corr / Euler: set to 1 for an instance whose new state is not
+                         // finite (per instance; may be null)
   float4* com_partial;   // Delta: per-block sums of F (may be null)
   unsigned long long* block_pairs;  // per-block pair-interaction counters (accumulated)
   int* row_count;        // Count
@@ -117,6 +119,7 @@ struct UpdateArgs {
   float2* q_pol;         // the evaluated deltas (then cleared); null otherwise
   KeyCount kc;           // corrector / Euler: the next rebuild's keys of the new positions
                          // (kc.keys null: not counted here)
+  int* nonfinite;        // corrector / Euler: 1 for an instance whose new state is not finite
 };
 enum UpdateKind { kUpdatePred = 0, kUpdateCorr = 1, kUpdateEuler = 2 };
 void launch_update(int kind, bool has_pol, const UpdateArgs& a, int n_blocks, hipStream_t stream);

@@ -185,6 +185,12 @@ class GpuSimulation:
     def rebuild(self):
         check(self._lib.tecs_sim_rebuild(self._h))
 
+    def nonfinite(self) -> bool:
+        """A step ended with a non-finite position or polarity since creation."""
+        f = C.c_int32(0)
+        check(self._lib.tecs_sim_get_nonfinite(self._h, C.byref(f)))
+        return bool(f.value)
+
     def stats(self) -> dict:
         s = abi.Stats()
         check(self._lib.tecs_sim_get_stats(self._h, C.byref(s)))
@@ -311,6 +317,12 @@ class GpuBatch:
 
     def synchronize(self):
         check(self._lib.tecs_batch_synchronize(self._h))
+
+    def nonfinite(self):
+        """Per member: a step ended with a non-finite position or polarity."""
+        f = np.zeros(len(self.descs), np.int32)
+        check(self._lib.tecs_batch_get_nonfinite(self._h, _ptr(f)))
+        return f.astype(bool)
 
     def stats(self) -> dict:
         s = abi.Stats()

@@ -41,6 +41,10 @@ class Server:
         check(self._lib.tecs_server_request_info(self._h, int(rid), C.byref(i)))
         return {k: getattr(i, k) for k, _ in abi.RequestInfo._fields_}
 
+    def cancel(self, rid: int):
+        """A pending request leaves at once, a running one at the end of its group's chunk."""
+        check(self._lib.tecs_server_cancel(self._h, int(rid)))
+
     def result(self, rid: int):
         d = self._descs[rid]
         n = int(d.agent_count)
