@@ -423,10 +423,27 @@ int tecs_writer_write_static(tecs_writer* writer, const char* label, uint32_t n,
  * steps overlap with the copy and the file write. Blocks only when 3 snapshots are in flight. */
 int tecs_sim_write(tecs_sim* sim, tecs_writer* writer, double time);
 int tecs_batch_write(tecs_batch* batch, int32_t index, tecs_writer* writer, double time);
-/* HDF5_Reader::read_dataset (include/io/hdf5_reader.hpp:66-71): a float dataset of rank 1 or
- * 2 ("t3/position"); dims[0..1] receive its shape; out may be NULL to query the shape. */
+/* HDF5_Reader::read_dataset (include/io/hdf5_reader.hpp:66-71): a dataset of rank 1 or 2
+ * ("t3/position") into float32; dims[0..1] receive its shape (rank 1: dims[1] = 1); out may be
+ * NULL to query the shape (HDF5_Reader::get_dataset_dimensions, :43-53). capacity counts
+ * elements. */
 int tecs_read_dataset(const char* h5_path, const char* dataset, float* out, int64_t capacity,
                       int64_t* dims);
+/* Element types of tecs_read_dataset_typed. */
+enum {
+  TECS_DTYPE_FLOAT32 = 0,
+  TECS_DTYPE_FLOAT64 = 1,
+  TECS_DTYPE_INT32 = 2,
+  TECS_DTYPE_UINT32 = 3,
+  TECS_DTYPE_UINT8 = 4,
+  TECS_DTYPE_INT64 = 5
+};
+/* read_dataset into a View<T> of any element type (the View<int> of cell types in
+ * examples/tribolium/main.cpp:71-72): HDF5 converts from the file's type to `dtype` on read.
+ * Same shapes and errors as tecs_read_dataset; TECS_ERR_INVALID for an unknown dtype or a
+ * rank above 2. */
+int tecs_read_dataset_typed(const char* h5_path, const char* dataset, int32_t dtype, void* out,
+                            int64_t capacity, int64_t* dims);
 
 /* =========================================================================================
  * Serving: simulation requests admitted by an HBM budget as they arrive and advanced together

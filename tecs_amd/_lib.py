@@ -86,6 +86,7 @@ PROTOTYPES = [
     ("tecs_sim_write", C.c_int, [_P, _P, C.c_double]),
     ("tecs_batch_write", C.c_int, [_P, _I32, _P, C.c_double]),
     ("tecs_read_dataset", C.c_int, [C.c_char_p, C.c_char_p, _F32P, C.c_int64, _I64P]),
+    ("tecs_read_dataset_typed", C.c_int, [C.c_char_p, C.c_char_p, _I32, _P, C.c_int64, _I64P]),
     ("tecs_server_create", C.c_int, [C.c_uint64, _I32, C.POINTER(_P)]),
     ("tecs_server_destroy", C.c_int, [_P]),
     ("tecs_server_submit", C.c_int, [_P, C.POINTER(abi.SimDesc), _F32P, _F32P, _P, C.c_double, _I32, _I64P]),

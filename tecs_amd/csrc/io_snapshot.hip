@@ -44,6 +44,7 @@ struct H5Api {
   hid_t (*screate_simple)(int, const hsize_t*, const hsize_t*) = nullptr;
   herr_t (*sclose)(hid_t) = nullptr;
   int (*sget_dims)(hid_t, hsize_t*, hsize_t*) = nullptr;
+  int (*sget_ndims)(hid_t) = nullptr;
   hid_t (*dcreate)(hid_t, const char*, hid_t, hid_t, hid_t, hid_t, hid_t) = nullptr;
   hid_t (*dopen)(hid_t, const char*, hid_t) = nullptr;
   hid_t (*dget_space)(hid_t) = nullptr;
@@ -53,6 +54,9 @@ struct H5Api {
   hid_t* native_float = nullptr;
   hid_t* native_uchar = nullptr;
   hid_t* native_uint = nullptr;
+  hid_t* native_double = nullptr;
+  hid_t* native_int = nullptr;
+  hid_t* native_llong = nullptr;
   std::string error;
 
   bool load() {
@@ -86,6 +90,7 @@ struct H5Api {
     sym(screate_simple, "H5Screate_simple");
     sym(sclose, "H5Sclose");
     sym(sget_dims, "H5Sget_simple_extent_dims");
+    sym(sget_ndims, "H5Sget_simple_extent_ndims");
     sym(dcreate, "H5Dcreate2");
     sym(dopen, "H5Dopen2");
     sym(dget_space, "H5Dget_space");
@@ -95,6 +100,9 @@ struct H5Api {
     sym(native_float, "H5T_NATIVE_FLOAT_g");
     sym(native_uchar, "H5T_NATIVE_UCHAR_g");
     sym(native_uint, "H5T_NATIVE_UINT_g");
+    sym(native_double, "H5T_NATIVE_DOUBLE_g");
+    sym(native_int, "H5T_NATIVE_INT_g");
+    sym(native_llong, "H5T_NATIVE_LLONG_g");
     if (ok && open() < 0) {
       ok = false;
       error = "H5open failed";
@@ -330,13 +338,27 @@ void SnapshotSlot::release() {
 }
 
 // --------------------------------------------------------------------------------- reading
-// HDF5_Reader::read_dataset, include/io/hdf5_reader.hpp:66-71: a float dataset of rank 1 or 2.
-int read_dataset(const std::string& h5_path, const std::string& name, float* out, int64_t capacity,
-                 int64_t* dims, std::string* err) {
+// HDF5_Reader::read_dataset, include/io/hdf5_reader.hpp:66-71 (get_dataset_dimensions, :43-53,
+// with out null): a dataset of rank 1 or 2 read into the memory type `dtype` (HDF5 converts
+// from the file type on read, as HighFive does for the View's element type).
+int read_dataset_typed(const std::string& h5_path, const std::string& name, int dtype, void* out,
+                       int64_t capacity, int64_t* dims, std::string* err) {
   std::lock_guard<std::mutex> g(h5_mutex());
   if (!h5().load()) {
     *err = h5().error;
     return -1;
+  }
+  hid_t mem = -1;
+  switch (dtype) {
+    case TECS_DTYPE_FLOAT32: mem = *h5().native_float; break;
+    case TECS_DTYPE_FLOAT64: mem = *h5().native_double; break;
+    case TECS_DTYPE_INT32: mem = *h5().native_int; break;
+    case TECS_DTYPE_UINT32: mem = *h5().native_uint; break;
+    case TECS_DTYPE_UINT8: mem = *h5().native_uchar; break;
+    case TECS_DTYPE_INT64: mem = *h5().native_llong; break;
+    default:
+      *err = "unknown element type";
+      return -3;
   }
   const hid_t f = h5().fopen(h5_path.c_str(), kH5FAccRdonly, kH5PDefault);
   if (f < 0) {
@@ -350,18 +372,23 @@ int read_dataset(const std::string& h5_path, const std::string& name, float* out
     rc = -1;
   } else {
     const hid_t sp = h5().dget_space(ds);
-    hsize_t shape[2] = {0, 1};
-    const int rank = h5().sget_dims(sp, shape, nullptr);
-    dims[0] = (int64_t)shape[0];
-    dims[1] = rank >= 2 ? (int64_t)shape[1] : 1;
+    const int rank = h5().sget_ndims(sp);
+    if (rank < 0 || rank > 2) {
+      *err = "dataset " + name + " has rank " + std::to_string(rank) + " (1 or 2 supported)";
+      rc = -3;
+    } else {
+      hsize_t shape[2] = {0, 1};
+      h5().sget_dims(sp, shape, nullptr);
+      dims[0] = rank >= 1 ? (int64_t)shape[0] : 1;
+      dims[1] = rank >= 2 ? (int64_t)shape[1] : 1;
+    }
     h5().sclose(sp);
-    const int64_t total = dims[0] * dims[1];
-    if (out) {
+    const int64_t total = rc == 0 ? dims[0] * dims[1] : 0;
+    if (rc == 0 && out) {
       if (total > capacity) {
         *err = "buffer too small";
         rc = -2;
-      } else if (total > 0 &&
-                 h5().dread(ds, *h5().native_float, kH5SAll, kH5SAll, kH5PDefault, out) < 0) {
+      } else if (total > 0 && h5().dread(ds, mem, kH5SAll, kH5SAll, kH5PDefault, out) < 0) {
         *err = "read failed";
         rc = -1;
       }
@@ -370,6 +397,11 @@ int read_dataset(const std::string& h5_path, const std::string& name, float* out
   }
   h5().fclose(f);
   return rc;
+}
+
+int read_dataset(const std::string& h5_path, const std::string& name, float* out, int64_t capacity,
+                 int64_t* dims, std::string* err) {
+  return read_dataset_typed(h5_path, name, TECS_DTYPE_FLOAT32, out, capacity, dims, err);
 }
 
 }  // namespace tecs_amd

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "../../include/tecs_amd.h"
+
 #include <condition_variable>
 #include <cstdint>
 #include <deque>
@@ -68,5 +70,7 @@ class Writer {
 
 int read_dataset(const std::string& h5_path, const std::string& name, float* out, int64_t capacity,
                  int64_t* dims, std::string* err);
+int read_dataset_typed(const std::string& h5_path, const std::string& name, int dtype, void* out,
+                       int64_t capacity, int64_t* dims, std::string* err);
 
 }  // namespace tecs_amd

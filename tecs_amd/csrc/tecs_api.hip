@@ -524,6 +524,16 @@ int tecs_read_dataset(const char* h5_path, const char* dataset, float* out, int6
   return TECS_OK;
 }
 
+int tecs_read_dataset_typed(const char* h5_path, const char* dataset, int32_t dtype, void* out,
+                            int64_t capacity, int64_t* dims) {
+  if (!h5_path || !dataset || !dims) return fail(TECS_ERR_INVALID, "null argument");
+  std::string err;
+  const int rc = tecs_amd::read_dataset_typed(h5_path, dataset, dtype, out, capacity, dims, &err);
+  if (rc == -2) return fail(TECS_ERR_CAPACITY, err);
+  if (rc != 0) return fail(err.find("libhdf5") != std::string::npos ? TECS_ERR_UNSUPPORTED : TECS_ERR_INVALID, err);
+  return TECS_OK;
+}
+
 // ------------------------------------------------------------------------------ server
 int tecs_server_create(uint64_t hbm_budget_bytes, int32_t max_chunk_steps, tecs_server** out) {
   if (!out) return fail(TECS_ERR_INVALID, "null output handle");

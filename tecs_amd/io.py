@@ -52,11 +52,36 @@ class Writer:
             pass
 
 
-def read_dataset(h5_path: str, name: str) -> np.ndarray:
-    """HDF5_Reader::read_dataset (include/io/hdf5_reader.hpp:66-71) for float datasets."""
+_DTYPES = {
+    np.dtype(np.float32): 0,
+    np.dtype(np.float64): 1,
+    np.dtype(np.int32): 2,
+    np.dtype(np.uint32): 3,
+    np.dtype(np.uint8): 4,
+    np.dtype(np.int64): 5,
+}
+
+
+def dataset_shape(h5_path: str, name: str):
+    """HDF5_Reader::get_dataset_dimensions (include/io/hdf5_reader.hpp:43-53): (rows, columns),
+    columns 1 for a rank-1 dataset."""
     lib = load()
     dims = (C.c_int64 * 2)()
-    check(lib.tecs_read_dataset(str(h5_path).encode(), name.encode(), None, 0, dims))
-    out = np.empty((dims[0], dims[1]), np.float32)
-    check(lib.tecs_read_dataset(str(h5_path).encode(), name.encode(), out.ctypes.data, out.size, dims))
+    check(lib.tecs_read_dataset_typed(str(h5_path).encode(), name.encode(), 0, None, 0, dims))
+    return int(dims[0]), int(dims[1])
+
+
+def read_dataset(h5_path: str, name: str, dtype=np.float32) -> np.ndarray:
+    """HDF5_Reader::read_dataset (include/io/hdf5_reader.hpp:66-71) into an array of `dtype`
+    (float32, float64, int32, uint32, uint8 or int64; HDF5 converts from the file's type), shape
+    (rows, columns)."""
+    lib = load()
+    dt = np.dtype(dtype)
+    if dt not in _DTYPES:
+        raise ValueError(f"unsupported element type {dt}")
+    rows, cols = dataset_shape(h5_path, name)
+    out = np.empty((rows, cols), dt)
+    dims = (C.c_int64 * 2)()
+    check(lib.tecs_read_dataset_typed(str(h5_path).encode(), name.encode(), _DTYPES[dt], out.ctypes.data,
+                                      out.size, dims))
     return out

@@ -572,3 +572,49 @@ def test_cells_outside_the_grid_bounds(gpu, oracle_lib, finder, variant):
     xo, _, _ = o.download()
     assert np.array_equal(xg.view(np.uint32), xo.view(np.uint32)), "trajectory differs"
     _assert_same_csr(g.pairs(False), o.pairs(False), "outside bounds, stale grid")
+
+
+@pytest.mark.parametrize("name", ["sorting", "epithelium"])
+def test_snapshots_equal_the_oracle_state(gpu, oracle_lib, tmp_path, name):
+    """Simulation::write through the asynchronous hand-off against the oracle's state at the same
+    step (simulation.hpp:607-611: t<k> holds the state at the k-th write, agent order, active
+    cells): bitwise for the type-table tissue, the trajectory bar for the polarity model:
+    median |d| <= 1e-5, p99 <= 1e-3, at most 0.2% of cells beyond 1e-3, and a maximum of 0.05
+    for positions and 0.25 for the polarities' unit vectors (a Gabriel test within an ulp of its
+    threshold can flip between GPU and host libm and turn a cell's polarity by up to
+    ~2 dt per step while it lasts)."""
+    from tecs_amd import io
+
+    w = _workload(name, 20000, abi.BINNED_GABRIEL if name == "epithelium" else abi.BINNED_ALL_PAIRS)
+    s = GpuSimulation(w.desc)
+    o = oracle_lib.OracleSim(w.desc)
+    for sim in (s, o):
+        sim.upload(w.positions, w.polarities, w.types)
+    print(f"case snapshots vs oracle {name} n=20000", flush=True)
+    expected = []
+    with io.Writer(str(tmp_path / "snap" / name)) as writer:
+        for k in range(4):
+            s.write(writer, k * w.dt * 7)
+            expected.append(o.download()[:2])
+            s.take_step(w.dt, 7)
+            o.take_step(w.dt, 7)
+    h5 = str(tmp_path / "snap" / f"{name}.h5")
+    for k, (xo, po) in enumerate(expected):
+        xs = io.read_dataset(h5, f"t{k}/position")
+        assert xs.shape == (w.n, 3)
+        if name == "sorting":
+            assert np.array_equal(xs.view(np.uint32), xo.view(np.uint32)), k
+            continue
+        ps = io.read_dataset(h5, f"t{k}/polarity")
+        for what, a, b, bound in (("positions", xs, xo, 0.05), ("polarity vectors", _unit(ps), _unit(po), 0.25)):
+            d = np.abs(a.astype(np.float64) - b.astype(np.float64)).max(axis=1)
+            stats = (f"t{k} {what}: median {np.median(d):.2e} p99 {np.quantile(d, 0.99):.2e} "
+                     f"max {d.max():.2e} beyond 1e-3: {(d > 1e-3).sum()}")
+            print(stats, flush=True)
+            assert np.median(d) <= 1e-5 and np.quantile(d, 0.99) <= 1e-3, stats
+            assert (d > 1e-3).mean() <= 0.002 and d.max() <= bound, stats
+
+
+def _unit(p):
+    t, f = p[:, 0].astype(np.float64), p[:, 1].astype(np.float64)
+    return np.stack([np.sin(t) * np.cos(f), np.sin(t) * np.sin(f), np.cos(t)], axis=1)
