@@ -247,6 +247,7 @@ struct Batch {
   int* skey[2] = {nullptr, nullptr};   // key of every slot of the order held in id[b]
   int* head[2] = {nullptr, nullptr};   // incremental rebuild: arrival lists per segment
   int* nxt = nullptr;
+  unsigned long long* movers = nullptr;  // incremental rebuild: marks of cells that changed segment
   int* d_fill_inst = nullptr;
   int* d_fill_base = nullptr;
   int* bsum = nullptr;
@@ -543,8 +544,10 @@ struct Batch {
     const size_t S = (size_t)total_slots;
     TRY(alloc(&d_inst, n_inst));
     TRY(alloc(&d_block_inst, n_blocks));
+    // positions carry one slot past the last: the all-pairs walk loads two candidates at a
+    // time and may read that slot (its result unused)
     for (int b = 0; b < 2; ++b) {
-      TRY(alloc(&x[b], S));
+      TRY(alloc(&x[b], S + 1));
       TRY(alloc(&id[b], S));
       if (has_pol) TRY(alloc(&pol[b], S));
       if (track_ov) TRY(alloc(&ov[b], S));
@@ -552,7 +555,7 @@ struct Batch {
     TRY(alloc(&d1, S));
     if (has_pol) TRY(alloc(&d1p, S));
     if (integrator == TECS_HEUN) {
-      TRY(alloc(&xp, S));
+      TRY(alloc(&xp, S + 1));
       if (has_pol) TRY(alloc(&polp, S));
     }
     bool any_drag = false;
@@ -578,6 +581,8 @@ struct Batch {
       TRY(alloc(&tidb, S));
       TRY(alloc(&cnt, (size_t)total_cbins));
       TRY(alloc(&nxt, S));
+      TRY(alloc(&movers, S / 64));
+      HIP_TRY(hipMemsetAsync(movers, 0, sizeof(unsigned long long) * (S / 64), stream));
       for (int b = 0; b < 2; ++b) {
         TRY(alloc(&coff[b], (size_t)total_cbins + 1));
         TRY(alloc(&skey[b], S));
@@ -870,6 +875,7 @@ struct Batch {
       k.skey = skey[ic];
       k.head = head[ic];
       k.nxt = nxt;
+      k.movers = movers;
       // the scan's tile sums move with the counts when the scan takes the fused path, which
       // leaves them in bsum (with more tiles the top-level scan overwrites them)
       if (scan_tiles_for(total_cbins) <= kScanFusedTileLimit) k.tsum = bsum;
@@ -884,6 +890,8 @@ struct Batch {
     if (is_binned(finder) && (keys_ready || inc_ok)) {
       HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(int) * (size_t)total_cbins, stream));
       if (keys_ready) HIP_TRY(hipMemsetAsync(head[ic], 0xff, sizeof(int) * (size_t)total_cbins, stream));
+      // marks of an incremental count that no rebuild will link
+      if (inc_ok) HIP_TRY(hipMemsetAsync(movers, 0, sizeof(unsigned long long) * (size_t)(total_slots / 64), stream));
     }
     keys_ready = false;
     inc_ok = false;

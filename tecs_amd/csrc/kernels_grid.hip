@@ -209,6 +209,25 @@ __global__ __launch_bounds__(kBlock) void k_counts_from_offsets(const int* coff,
   if (g < n) cnt[g] = coff[g + 1] - coff[g];
 }
 
+// Incremental rebuild, before the scan: every cell marked as a mover (count_key) joins its new
+// segment's arrival list. One thread per 64-slot word of marks (the 64 slots are in one
+// workgroup, so one instance); the marks are cleared for the next rebuild. The list order is
+// free: the gather and the fill only count a segment's members.
+__global__ __launch_bounds__(kBlock) void k_link_movers(RebuildArgs a, int n_words) {
+  const int w = (int)blockIdx.x * kBlock + (int)threadIdx.x;
+  if (w >= n_words) return;
+  unsigned long long m = a.kc.movers[w];
+  if (m == 0ull) return;
+  a.kc.movers[w] = 0ull;
+  const InstanceDesc& in = a.inst[a.block_inst[(w * 64) / kBlock]];
+  while (m != 0ull) {
+    const int s = w * 64 + (__ffsll((long long)m) - 1);
+    m &= m - 1ull;
+    const int g = in.cbin_begin + (a.kc.keys[s] >> in.shift);
+    a.kc.nxt[s] = atomicExch(a.kc.head + g, s);
+  }
+}
+
 // Steps 4 and 5 in one launch: workgroups [0, n_gather) gather (XCD-aware order), the rest fill.
 template <bool INC>
 __global__ __launch_bounds__(kBlock) void k_gather_fill(RebuildArgs a, int n_gather) {
@@ -500,6 +519,11 @@ void launch_exclusive_scan(const int* in, int n, int* out, int* bsum, int* head_
 
 void launch_rebuild(const RebuildArgs& a, int n_blocks, hipStream_t stream) {
   if (!a.keys_ready) hipLaunchKernelGGL(k_bin_keys, dim3(n_blocks), dim3(kBlock), 0, stream, a);
+  if (a.incremental) {
+    const int n_words = n_blocks * (kBlock / 64);
+    hipLaunchKernelGGL(k_link_movers, dim3((n_words + kBlock - 1) / kBlock), dim3(kBlock), 0, stream,
+                       a, n_words);
+  }
   // an incremental rebuild's counts moved with their tiles' sums (count_key)
   launch_exclusive_scan(a.kc.cnt, a.total_cbins, a.coff, a.scan_bsum, a.head_clear, stream,
                         a.incremental && a.kc.tsum != nullptr);

@@ -17,6 +17,8 @@
 // Symmetric finders (binned_all_pairs_symmetric.hpp:148-219 and its three siblings): the pair
 // export walks keep only partners with a smaller agent index (SYM); the force evaluation runs
 // in the k_sym_* kernels below.
+#include <type_traits>
+
 #include "tecs_internal.hpp"
 #include "walks.hpp"
 
@@ -35,10 +37,11 @@ struct CellState {
   float vel[3];  //            sum of drag * old_velocity_j (:114, 169)
 };
 
+// `tab` is the instance's adhesion table (the InstanceDesc's, or a copy in LDS).
 template <bool HAS_TYPE, bool HAS_POL, int MODE, bool SYM>
 __device__ __forceinline__ void visit(const SweepArgs& a, const InstanceDesc& in, CellState& cs,
                                       float4 me, const Pol& pme, int m, float4 pj,
-                                      const float* disp, float d2) {
+                                      const float* disp, float d2, const float* tab) {
   cs.pairs++;
   if (MODE == kModeCount) return;
   if (MODE == kModeEmit) {
@@ -49,8 +52,9 @@ __device__ __forceinline__ void visit(const SweepArgs& a, const InstanceDesc& in
   Pol po{};
   if (HAS_POL) po = make_pol(a.ps[m], a.trig[m]);
   static_assert(!SYM || MODE == kModeCount || MODE == kModeEmit, "symmetric walks export pairs only");
-  apply_force<HAS_TYPE, HAS_POL>(in.force, disp, d, __float_as_int(me.w), __float_as_int(pj.w),
-                                 pme, po, cs.acc_pos, cs.acc_pol);
+  apply_force_k<HAS_POL>(in.force, disp, d,
+                         pair_kadh<HAS_TYPE>(in.force, tab, __float_as_int(me.w), __float_as_int(pj.w)),
+                         pme, po, cs.acc_pos, cs.acc_pol);
   if (MODE == kModeDeltaDrag) {
     // binned_all_pairs.hpp:168-169: total_drag += drag; total_velocity += drag * old_vel(j)
     const float g = pair_drag<HAS_TYPE>(in.force, __float_as_int(me.w), __float_as_int(pj.w));
@@ -94,7 +98,7 @@ __device__ __forceinline__ void walk_binned(const SweepArgs& a, const InstanceDe
         if (GABRIEL) {
           if (gabriel_blocked(in, xs, offb, s, m, me, disp, d2)) continue;
         }
-        visit<HAS_TYPE, HAS_POL, MODE, SYM>(a, in, cs, me, pme, m, pj, disp, d2);
+        visit<HAS_TYPE, HAS_POL, MODE, SYM>(a, in, cs, me, pme, m, pj, disp, d2, in.force.table);
       }
     }
   }
@@ -189,7 +193,7 @@ __device__ __forceinline__ void walk_binned_deferred(const SweepArgs& a, const I
       const float4 pj = xs[m];
       const float disp[3] = {me.x - pj.x, me.y - pj.y, me.z - pj.z};
       const float d2 = len2(disp[0], disp[1], disp[2]);
-      visit<HAS_TYPE, HAS_POL, MODE, SYM>(a, in, cs, me, pme, m, pj, disp, d2);
+      visit<HAS_TYPE, HAS_POL, MODE, SYM>(a, in, cs, me, pme, m, pj, disp, d2, in.force.table);
     }
   }
   // 3. the accepted candidates after the first kDeferMax, inline and in order
@@ -203,7 +207,176 @@ __device__ __forceinline__ void walk_binned_deferred(const SweepArgs& a, const I
         if (k++ < kDeferMax) continue;
         const float disp[3] = {me.x - pj.x, me.y - pj.y, me.z - pj.z};
         const float d2 = len2(disp[0], disp[1], disp[2]);
-        visit<HAS_TYPE, HAS_POL, MODE, SYM>(a, in, cs, me, pme, m, pj, disp, d2);
+        visit<HAS_TYPE, HAS_POL, MODE, SYM>(a, in, cs, me, pme, m, pj, disp, d2, in.force.table);
+      }
+    }
+  }
+}
+
+// The all-pairs walk with one candidate stream per lane (search radius 1, force modes).
+// walk_binned_deferred runs its candidate loop row by row, so a wavefront spends on each of the
+// 9 rows as many iterations as its longest lane there; here every lane walks the slot ranges
+// of its rows back to back, and the wavefront runs as many iterations as its longest lane's
+// whole candidate set. Each lane's ranges (its 9 rows, empty ones dropped) are kept in LDS:
+//   1. each lane walks its ranges in the reference's order, two candidates per iteration, and
+//      appends the accepted ones to its LDS list (the first kDeferMax; it counts all of them)
+//      as (range << 8) | offset in the range. The cell's own slot is walked like the others
+//      and is accepted (distance 0, or NaN, neither >= r2); it takes a list entry;
+//   2. the wavefront evaluates the forces list entry by list entry, skipping the own slot;
+//   3. a lane that accepted more than kDeferMax walks its ranges again and evaluates the ones
+//      after the first kDeferMax inline.
+// Each lane adds its contributions in its own visiting order without its own slot, so the
+// sums are bitwise those of walk_binned. A range must hold fewer than 256 slots (flat_ranges
+// reports it); the LDS (about 19.3 KiB per workgroup) keeps 8 workgroups on a CU.
+constexpr int kFlatRanges = 9;
+
+// LDS of the flattened walk, lane-interleaved ([k * kBlock + lane]).
+struct FlatLds {
+  int start[kFlatRanges * kBlock];
+  unsigned short list[kDeferMax * kBlock];
+  unsigned char len[kFlatRanges * kBlock];
+  float tab[kMaxTypes * kMaxTypes];  // adhesion scale by (type_i * n + type_j), or [0] untyped
+};
+// the row-by-row walk's list fits in it (its fallback)
+static_assert(sizeof(FlatLds) >= sizeof(int) * kDeferMax * kBlock, "FlatLds too small");
+
+// Step 0: the lane's ranges into LDS; returns their count. `fits` is cleared when a range holds
+// 256 slots or more; `self` is set when a range holds the lane's own slot s and the walk will
+// accept it (its distance to itself, 0 or NaN, is not >= r2).
+__device__ __forceinline__ int flat_ranges(const SweepArgs& a, const InstanceDesc& in, int s,
+                                           float4 me, FlatLds& L, bool& fits, bool& self) {
+  const int* __restrict__ offb = a.off + in.bin_begin;
+  const int lane = (int)threadIdx.x;
+  int bc[3];
+  bin_coords_mul(in, me.x, me.y, me.z, bc);  // binned_all_pairs.hpp:103-109
+  const int x_lo = max(bc[0] - 1, 0);
+  const int x_hi = min(bc[0] + 1, in.ext[0] - 1);
+  int rs[9], re[9];
+#pragma unroll
+  for (int r = 0; r < 9; ++r) {
+    const int z = bc[2] + r / 3 - 1, y = bc[1] + r % 3 - 1;
+    const bool ok = z >= 0 && z < in.ext[2] && y >= 0 && y < in.ext[1];
+    rs[r] = ok ? offb[flatten(in, x_lo, y, z)] : 0;
+    re[r] = ok ? offb[flatten(in, x_hi, y, z) + 1] : 0;
+  }
+  int nr = 0;
+#pragma unroll
+  for (int r = 0; r < 9; ++r) {
+    const int b = rs[r], e = re[r];
+    if (e - b >= 256) fits = false;
+    if (b <= s && s < e) self = !(len2(me.x - me.x, me.y - me.y, me.z - me.z) >= in.cutoff2);
+    if (b < e) {
+      L.start[nr * kBlock + lane] = b;
+      L.len[(nr++) * kBlock + lane] = (unsigned char)(e - b);
+    }
+  }
+  return nr;
+}
+
+template <bool HAS_TYPE, bool HAS_POL, int MODE>
+__device__ __forceinline__ void walk_binned_flat(const SweepArgs& a, const InstanceDesc& in, int s,
+                                                 int nr, bool self, float4 me, const Pol& pme,
+                                                 CellState& cs, FlatLds& L) {
+  const float4* __restrict__ xs = a.xs;
+  const int lane = (int)threadIdx.x;
+  const float r2 = in.cutoff2;
+  int na = 0;  // accepted candidates (the own slot among them), also those beyond the list
+  if (nr > 0) {
+    // one loop over the lane's candidates: q walks range k, o is the offset in it, n its
+    // length; the second load of an iteration may read one slot past the range (positions
+    // carry a slot past the last one), its result used only when o + 1 < n
+    const int* sp = L.start + lane;
+    const unsigned char* lp = L.len + lane;
+    const int* const sp_end = sp + nr * kBlock;
+    const float4* q = xs + *sp;
+    int n = (int)*lp, o = 0, tag = 0;
+    while (o < n) {
+      const float4 q0 = q[0];
+      const float4 q1 = q[1];
+      // binned_all_pairs.hpp:140-151: a NaN distance is accepted, as `d2 >= r2` is false
+      const float d0 = len2(me.x - q0.x, me.y - q0.y, me.z - q0.z);
+      const float d1 = len2(me.x - q1.x, me.y - q1.y, me.z - q1.z);
+      if (!(d0 >= r2)) {
+        if (na < kDeferMax) L.list[na * kBlock + lane] = (unsigned short)(tag | o);
+        ++na;
+      }
+      if (o + 1 < n && !(d1 >= r2)) {
+        if (na < kDeferMax) L.list[na * kBlock + lane] = (unsigned short)(tag | (o + 1));
+        ++na;
+      }
+      o += 2;
+      q += 2;
+      if (o >= n) {
+        sp += kBlock;
+        lp += kBlock;
+        tag += 1 << 8;
+        if (sp < sp_end) {
+          q = xs + *sp;
+          n = (int)*lp;
+          o = 0;
+        }
+      }
+    }
+  }
+  // 2. forces, one list entry per lane per round; the rounds run to the wavefront's longest
+  // list, a lane past its own list evaluates its own slot and adds nothing
+  const int nl = min(na, kDeferMax);
+  int rounds = nl;
+#pragma unroll
+  for (int w = 32; w > 0; w >>= 1) rounds = max(rounds, __shfl_xor(rounds, w, 64));
+  if (!HAS_POL) {
+    // PiecewiseLinear alone (predefined.hpp:107-144), its parameters in registers, the
+    // adhesion table in LDS, a loop with no branch in its body
+    cs.pairs += (unsigned)(na - (self ? 1 : 0));  // the own slot's acceptance is not a pair
+    const ForceParams& f = in.force;
+    const float h = f.h, krep = f.krep;
+    const int tn = (HAS_TYPE && f.n_types > 0) ? f.n_types : 0;
+    const int row = tn * __float_as_int(me.w);
+    const int tmask = tn > 0 ? kMaxTypes * kMaxTypes - 1 : 0;
+    if ((f.flags & kForceLinear) != 0) {
+      for (int e = 0; e < rounds; ++e) {
+        const int en = (int)L.list[min(e, kDeferMax - 1) * kBlock + lane];
+        const int m = e < nl ? L.start[(en >> 8) * kBlock + lane] + (en & 255) : s;
+        const float4 pj = xs[m];
+        const float kadh = L.tab[(row + (__float_as_int(pj.w) & tmask)) & (kMaxTypes * kMaxTypes - 1)];
+        const float disp[3] = {me.x - pj.x, me.y - pj.y, me.z - pj.z};
+        const float d = sqrtf(len2(disp[0], disp[1], disp[2]));
+        const float F = krep * fmaxf(h - d, 0.0f) - kadh * fmaxf(d - h, 0.0f);
+        // the own slot, or d == 0 (a zero force): adding +0 changes no sum
+        const bool z = m == s || d == 0.0f;
+        cs.acc_pos[0] = cs.acc_pos[0] + (z ? 0.0f : disp[0] * F / d);
+        cs.acc_pos[1] = cs.acc_pos[1] + (z ? 0.0f : disp[1] * F / d);
+        cs.acc_pos[2] = cs.acc_pos[2] + (z ? 0.0f : disp[2] * F / d);
+      }
+    }
+  } else {
+    for (int e = 0; e < rounds; ++e) {
+      if (e < nl) {
+        const int en = (int)L.list[e * kBlock + lane];
+        const int m = L.start[(en >> 8) * kBlock + lane] + (en & 255);
+        if (m != s) {
+          const float4 pj = xs[m];
+          const float disp[3] = {me.x - pj.x, me.y - pj.y, me.z - pj.z};
+          const float d2 = len2(disp[0], disp[1], disp[2]);
+          visit<HAS_TYPE, HAS_POL, MODE, false>(a, in, cs, me, pme, m, pj, disp, d2, in.force.table);
+        }
+      }
+    }
+  }
+  // 3. the accepted candidates after the first kDeferMax, inline and in order
+  if (na > kDeferMax) {
+    int k = 0;
+    for (int r = 0; r < nr; ++r) {
+      const int b = L.start[r * kBlock + lane];
+      const int end = b + (int)L.len[r * kBlock + lane];
+      for (int m = b; m < end; ++m) {
+        const float4 pj = xs[m];
+        const float disp[3] = {me.x - pj.x, me.y - pj.y, me.z - pj.z};
+        const float d2 = len2(disp[0], disp[1], disp[2]);
+        if (d2 >= r2) continue;
+        if (k++ < kDeferMax || m == s) continue;
+        if (!HAS_POL) cs.pairs--;  // counted above; visit counts it again
+        visit<HAS_TYPE, HAS_POL, MODE, false>(a, in, cs, me, pme, m, pj, disp, d2, L.tab);
       }
     }
   }
@@ -223,10 +396,19 @@ __device__ __forceinline__ CorrIn load_corr(const SweepArgs& a, int s) {
 
 // Epilogue shared by the binned and naive sweeps: turn the accumulators into the evaluation
 // result and apply the mode's integrator update.
+// The key at the last rebuild of the cell in slot s, loaded with the cell's other inputs before
+// the walk when the step's final sweep counts the next rebuild's keys incrementally (else -1).
+template <int MODE>
+__device__ __forceinline__ int load_old_key(const SweepArgs& a, int s, bool active) {
+  if ((MODE == kModeCorr || MODE == kModeEuler) && a.kc.keys && a.kc.skey && active)
+    return a.kc.skey[s];
+  return -1;
+}
+
 template <bool HAS_POL, int MODE, bool SYM>
 __device__ __forceinline__ void finish(const SweepArgs& a, const InstanceDesc& in, int ii, int lb,
                                        int s, bool active, float4 me, float2 mp, CellState& cs,
-                                       const CorrIn& ci) {
+                                       const CorrIn& ci, int old_key = -1) {
   __shared__ float lds_f[4];
   __shared__ unsigned lds_u[4];
   if (MODE == kModeCount) {
@@ -299,7 +481,7 @@ __device__ __forceinline__ void finish(const SweepArgs& a, const InstanceDesc& i
   }
   // keys of the new positions for the rebuild that starts the next step
   if ((MODE == kModeCorr || MODE == kModeEuler) && a.kc.keys)
-    count_key(in, a.kc, s, active, s - in.slot_begin == in.n, xnew);
+    count_key(in, a.kc, s, active, s - in.slot_begin == in.n, xnew, old_key);
   if (MODE == kModeDelta && a.com_partial) {
     const float f0 = active ? F[0] : 0.0f, f1 = active ? F[1] : 0.0f, f2 = active ? F[2] : 0.0f;
     const float t0 = block_sum_f(f0, lds_f);
@@ -334,17 +516,36 @@ __global__ __launch_bounds__(kBlock) void k_sweep_binned(SweepArgs a) {
     if (MODE == kModeEmit) cs.emit_at = a.row_ptr[a.id[s]];
     if (MODE == kModeCorr) ci = load_corr<HAS_POL>(a, s);
   }
+  const int old_key = load_old_key<MODE>(a, s, active);
   // the drag mode walks inline: its drag sums run in the same visiting order either way
   constexpr bool kDefer = MODE != kModeCount && MODE != kModeEmit && MODE != kModeDeltaDrag;
-  if (kDefer && in.search_radius == 1) {
+  constexpr bool kFlat = kDefer && !GABRIEL && !SYM;
+  if (kFlat && in.search_radius == 1) {
     // the instance decides the path for the whole workgroup (all its slots are one instance)
+    __shared__ FlatLds flat;
+    if (threadIdx.x < kMaxTypes * kMaxTypes) {
+      const ForceParams& f = in.force;
+      flat.tab[threadIdx.x] = (HAS_TYPE && f.n_types > 0) ? f.table[threadIdx.x] : f.kadh;
+    }
+    bool fits = true, self = false;
+    const int nr = active ? flat_ranges(a, in, s, me, flat, fits, self) : 0;
+    // the table, and whether every range fits an 8-bit length (a workgroup barrier)
+    if (__syncthreads_and(fits)) {
+      walk_binned_flat<HAS_TYPE, HAS_POL, MODE>(a, in, s, nr, self, me, pme, cs, flat);
+    } else {
+      // a range of 256 slots or more: the row-by-row walk, its list in the ranges' LDS (the
+      // workgroup has passed the barrier, so no lane reads the ranges any more)
+      walk_binned_deferred<GABRIEL, HAS_TYPE, HAS_POL, MODE, SYM>(a, in, s, me_id, active, me, pme,
+                                                                  cs, reinterpret_cast<int*>(&flat) + threadIdx.x);
+    }
+  } else if (kDefer && in.search_radius == 1) {
     __shared__ int list[kDeferMax * kBlock];
     walk_binned_deferred<GABRIEL, HAS_TYPE, HAS_POL, MODE, SYM>(a, in, s, me_id, active, me, pme,
                                                                 cs, list + threadIdx.x);
   } else if (active) {
     walk_binned<GABRIEL, HAS_TYPE, HAS_POL, MODE, SYM>(a, in, s, me_id, me, pme, cs);
   }
-  finish<HAS_POL, MODE, SYM>(a, in, ii, lb, s, active, me, mp, cs, ci);
+  finish<HAS_POL, MODE, SYM>(a, in, ii, lb, s, active, me, mp, cs, ci, old_key);
 }
 
 // Naive finders (naive_all_pairs.hpp:41-103, naive_gabriel.hpp:47-119): j runs over all agents
@@ -405,7 +606,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_naive(SweepArgs a) {
         }
         if (blocked) continue;
       }
-      visit<HAS_TYPE, HAS_POL, MODE, SYM>(a, in, cs, me, pme, m, pj, disp, d2);
+      visit<HAS_TYPE, HAS_POL, MODE, SYM>(a, in, cs, me, pme, m, pj, disp, d2, in.force.table);
     }
   }
   finish<HAS_POL, MODE, SYM>(a, in, ii, lb, s, active, me, mp, cs, ci);

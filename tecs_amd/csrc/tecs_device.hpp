@@ -169,9 +169,11 @@ struct KeyCount {
   int* keys;        // key per slot (current order)
   int* cnt;         // per-segment counts
   const int* skey;  // incremental: key per slot at the last rebuild (null = full)
-  int* head;
-  int* nxt;
+  int* head;        // incremental: arrival list heads per segment (-1 = empty) ...
+  int* nxt;         //              ... and links per slot, built by k_link_movers
   int* tsum;        // incremental: the scan's per-tile sums of cnt, kept current with it
+  unsigned long long* movers;  // incremental: bit (s & 63) of word s >> 6 marks a cell that
+                               // changed segment; k_link_movers puts it on its arrival list
 };
 
 // Step 1 of the rebuild (kernels_grid.hip) for one slot: store the key of the cell's position
@@ -179,17 +181,20 @@ struct KeyCount {
 // and the slot just past the instance's cells sets the pad segment's count. Incremental
 // rebuild: only a cell whose segment differs from its segment at the last rebuild moves its
 // count (and, between scan tiles, the tiles' sums, so the scan need not sum the tiles again)
-// and joins its new segment's arrival list. Called by every lane of the wavefront;
-// `cell` marks lanes that hold a cell, `pad` the lane of the slot just past the cells. No
-// atomic here returns a value the wavefront waits for, except the arrival-list exchange.
+// and marks itself as a mover; the rebuild links the movers into their new segments' arrival
+// lists (k_link_movers), so no wavefront here waits for an atomic's result. Called by every
+// lane of the wavefront; `cell` marks lanes that hold a cell, `pad` the lane of the slot just
+// past the cells.
+// `old_key` is k.skey[s] when the caller has loaded it already (a sweep loads it with the
+// cell's other inputs), else -1.
 __device__ __forceinline__ void count_key(const InstanceDesc& in, const KeyCount& k, int s,
-                                          bool cell, bool pad, float4 p) {
+                                          bool cell, bool pad, float4 p, int old_key = -1) {
   if (k.skey) {
     if (!cell) return;
     const int key = bin_key(in, p);
     k.keys[s] = key;
     const int gn = in.cbin_begin + (key >> in.shift);
-    const int go = in.cbin_begin + (k.skey[s] >> in.shift);
+    const int go = in.cbin_begin + ((old_key >= 0 ? old_key : k.skey[s]) >> in.shift);
     if (gn != go) {
       __hip_atomic_fetch_add(k.cnt + go, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_fetch_add(k.cnt + gn, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -198,7 +203,8 @@ __device__ __forceinline__ void count_key(const InstanceDesc& in, const KeyCount
         __hip_atomic_fetch_add(k.tsum + to, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_fetch_add(k.tsum + tn, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
-      k.nxt[s] = atomicExch(k.head + gn, s);
+      __hip_atomic_fetch_or(k.movers + (s >> 6), 1ull << (s & 63), __ATOMIC_RELAXED,
+                            __HIP_MEMORY_SCOPE_AGENT);
     }
     return;
   }
@@ -301,14 +307,19 @@ struct PairTerms {
   float v[3], bpos[3], bpol[2], al[2];
 };
 
-template <bool HAS_TYPE, bool HAS_POL>
-__device__ __forceinline__ void pair_terms(const ForceParams& f, const float* disp, float d,
-                                           int type_i, int type_j, const Pol& pi, const Pol& pj,
-                                           PairTerms& t) {
-  float kadh = f.kadh;
-  if (HAS_TYPE) {
-    if (f.n_types > 0) kadh = f.table[type_i * f.n_types + type_j];
-  }
+// The adhesion scale of pair (i, j): the type table's entry, or the scalar without a table.
+template <bool HAS_TYPE>
+__device__ __forceinline__ float pair_kadh(const ForceParams& f, const float* table, int type_i,
+                                           int type_j) {
+  if (HAS_TYPE && f.n_types > 0) return table[type_i * f.n_types + type_j];
+  return f.kadh;
+}
+
+// `kadh` is the pair's adhesion scale (pair_kadh).
+template <bool HAS_POL>
+__device__ __forceinline__ void pair_terms_k(const ForceParams& f, const float* disp, float d,
+                                             float kadh, const Pol& pi, const Pol& pj,
+                                             PairTerms& t) {
   t.v[0] = t.v[1] = t.v[2] = 0.0f;
   t.bpos[0] = t.bpos[1] = t.bpos[2] = 0.0f;
   t.bpol[0] = t.bpol[1] = t.al[0] = t.al[1] = 0.0f;
@@ -347,19 +358,25 @@ __device__ __forceinline__ void pair_terms(const ForceParams& f, const float* di
   }
 }
 
+template <bool HAS_TYPE, bool HAS_POL>
+__device__ __forceinline__ void pair_terms(const ForceParams& f, const float* disp, float d,
+                                           int type_i, int type_j, const Pol& pi, const Pol& pj,
+                                           PairTerms& t) {
+  pair_terms_k<HAS_POL>(f, disp, d, pair_kadh<HAS_TYPE>(f, f.table, type_i, type_j), pi, pj, t);
+}
+
 // The terms of one pair in the order they add to a field: p1 then p2 for the position, q1 then
 // q2 for the polarity. A term that is not applied is +0, and adding it changes nothing: a sum
 // that starts at +0 never becomes -0 under round-to-nearest, and adding +0 to any other value
 // leaves it unchanged. So acc = (acc + p1) + p2, pair after pair, is the reference's running
 // sum with only the applied terms, in the composition's order. The order flags are set on the
 // host only when both terms of a field are applied.
-template <bool HAS_TYPE, bool HAS_POL>
-__device__ __forceinline__ void apply_force_terms(const ForceParams& f, const float* disp, float d,
-                                                  int type_i, int type_j, const Pol& pi,
-                                                  const Pol& pj, float* p1, float* p2, float* q1,
-                                                  float* q2) {
+template <bool HAS_POL>
+__device__ __forceinline__ void apply_force_terms_k(const ForceParams& f, const float* disp, float d,
+                                                    float kadh, const Pol& pi, const Pol& pj,
+                                                    float* p1, float* p2, float* q1, float* q2) {
   PairTerms t;
-  pair_terms<HAS_TYPE, HAS_POL>(f, disp, d, type_i, type_j, pi, pj, t);
+  pair_terms_k<HAS_POL>(f, disp, d, kadh, pi, pj, t);
   const bool bend_first = (f.flags & kForceBendFirst) != 0;
   const bool align_first = (f.flags & kForceAlignFirst) != 0;
 #pragma unroll
@@ -374,27 +391,45 @@ __device__ __forceinline__ void apply_force_terms(const ForceParams& f, const fl
   }
 }
 
+// apply_force_terms_k with the adhesion scale from the type table.
+template <bool HAS_TYPE, bool HAS_POL>
+__device__ __forceinline__ void apply_force_terms(const ForceParams& f, const float* disp, float d,
+                                                  int type_i, int type_j, const Pol& pi,
+                                                  const Pol& pj, float* p1, float* p2, float* q1,
+                                                  float* q2) {
+  apply_force_terms_k<HAS_POL>(f, disp, d, pair_kadh<HAS_TYPE>(f, f.table, type_i, type_j), pi, pj,
+                               p1, p2, q1, q2);
+}
+
 // One invocation of the fused force, added term by term to the cell's running accumulators in
 // the composition's order, as the reference's fused forces add to the same pack
 // (kernel_fuser.hpp:40-55). Without polarity the only term is PiecewiseLinear (+0 when it is
-// not applied).
-template <bool HAS_TYPE, bool HAS_POL>
-__device__ __forceinline__ void apply_force(const ForceParams& f, const float* disp, float d,
-                                            int type_i, int type_j, const Pol& pi, const Pol& pj,
-                                            float* acc_pos, float* acc_pol) {
+// not applied). `kadh` is the pair's adhesion scale (pair_kadh).
+template <bool HAS_POL>
+__device__ __forceinline__ void apply_force_k(const ForceParams& f, const float* disp, float d,
+                                              float kadh, const Pol& pi, const Pol& pj,
+                                              float* acc_pos, float* acc_pol) {
   if (!HAS_POL) {
     PairTerms t;
-    pair_terms<HAS_TYPE, HAS_POL>(f, disp, d, type_i, type_j, pi, pj, t);
+    pair_terms_k<HAS_POL>(f, disp, d, kadh, pi, pj, t);
 #pragma unroll
     for (int k = 0; k < 3; ++k) acc_pos[k] = acc_pos[k] + t.v[k];
     return;
   }
   float p1[3], p2[3], q1[2], q2[2];
-  apply_force_terms<HAS_TYPE, HAS_POL>(f, disp, d, type_i, type_j, pi, pj, p1, p2, q1, q2);
+  apply_force_terms_k<HAS_POL>(f, disp, d, kadh, pi, pj, p1, p2, q1, q2);
 #pragma unroll
   for (int k = 0; k < 3; ++k) acc_pos[k] = (acc_pos[k] + p1[k]) + p2[k];
 #pragma unroll
   for (int k = 0; k < 2; ++k) acc_pol[k] = (acc_pol[k] + q1[k]) + q2[k];
+}
+
+template <bool HAS_TYPE, bool HAS_POL>
+__device__ __forceinline__ void apply_force(const ForceParams& f, const float* disp, float d,
+                                            int type_i, int type_j, const Pol& pi, const Pol& pj,
+                                            float* acc_pos, float* acc_pol) {
+  apply_force_k<HAS_POL>(f, disp, d, pair_kadh<HAS_TYPE>(f, f.table, type_i, type_j), pi, pj,
+                         acc_pos, acc_pol);
 }
 
 // The drag output of the fused force for pair (i, j) (binned_all_pairs.hpp:155-169: the
