@@ -88,6 +88,23 @@ ADAPTER_SRC = PKG.parent / "tests" / "cpp" / "adapter_check.cpp"
 ADAPTER_BIN = PKG.parent / "tests" / "cpp" / "_build" / "adapter_check"
 
 
+DIV_CHECK_SRC = PKG.parent / "tests" / "cpp" / "div_check.hip"
+DIV_CHECK_LIB = PKG.parent / "tests" / "cpp" / "_build" / "libdiv_check.so"
+
+
+def build_div_check() -> Path:
+    """The GPU check of the engine's division against IEEE division (tests/test_gpu_fast_div.py),
+    compiled with the engine's flags."""
+    DIV_CHECK_LIB.parent.mkdir(parents=True, exist_ok=True)
+    tmp = DIV_CHECK_LIB.with_suffix(".so.tmp")
+    cmd = [_hipcc(), *HIPCC_FLAGS, "-shared", str(DIV_CHECK_SRC), "-o", str(tmp)]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    if r.returncode != 0:
+        raise RuntimeError("division check build failed:\n" + r.stdout.decode(errors="replace"))
+    os.replace(tmp, DIV_CHECK_LIB)
+    return DIV_CHECK_LIB
+
+
 def build_adapter_check() -> Path:
     """The C++ adapter test driver (include/tecs_amd.hpp over the C ABI), linked against the
     engine library with a relative run path so the tree can move."""

@@ -342,11 +342,15 @@ __device__ __forceinline__ void walk_binned_flat(const SweepArgs& a, const Insta
         const float disp[3] = {me.x - pj.x, me.y - pj.y, me.z - pj.z};
         const float d = sqrtf(len2(disp[0], disp[1], disp[2]));
         const float F = krep * fmaxf(h - d, 0.0f) - kadh * fmaxf(d - h, 0.0f);
-        // the own slot, or d == 0 (a zero force): adding +0 changes no sum
+        // the own slot, or d == 0 (a zero force): adding +0 changes no sum; such a lane divides
+        // by 1 (its quotient is not used, its displacement is 0), so it never sends the
+        // wavefront into the IEEE path
         const bool z = m == s || d == 0.0f;
-        cs.acc_pos[0] = cs.acc_pos[0] + (z ? 0.0f : disp[0] * F / d);
-        cs.acc_pos[1] = cs.acc_pos[1] + (z ? 0.0f : disp[1] * F / d);
-        cs.acc_pos[2] = cs.acc_pos[2] + (z ? 0.0f : disp[2] * F / d);
+        float v[3];
+        pl_div3_rn(disp, F, z ? 1.0f : d, v);  // (disp * F) / d
+        cs.acc_pos[0] = cs.acc_pos[0] + (z ? 0.0f : v[0]);
+        cs.acc_pos[1] = cs.acc_pos[1] + (z ? 0.0f : v[1]);
+        cs.acc_pos[2] = cs.acc_pos[2] + (z ? 0.0f : v[2]);
       }
     }
   } else {
@@ -494,7 +498,7 @@ __device__ __forceinline__ void finish(const SweepArgs& a, const InstanceDesc& i
 }
 
 template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE, bool SYM>
-__global__ __launch_bounds__(kBlock) void k_sweep_binned(SweepArgs a) {
+__device__ __forceinline__ void sweep_binned(const SweepArgs& a) {
   const int lb = logical_block();
   const int ii = a.block_inst[lb];
   const InstanceDesc& in = a.inst[ii];
@@ -546,6 +550,20 @@ __global__ __launch_bounds__(kBlock) void k_sweep_binned(SweepArgs a) {
     walk_binned<GABRIEL, HAS_TYPE, HAS_POL, MODE, SYM>(a, in, s, me_id, me, pme, cs);
   }
   finish<HAS_POL, MODE, SYM>(a, in, ii, lb, s, active, me, mp, cs, ci, old_key);
+}
+
+template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE, bool SYM>
+__global__ __launch_bounds__(kBlock) void k_sweep_binned(SweepArgs a) {
+  sweep_binned<GABRIEL, HAS_TYPE, HAS_POL, MODE, SYM>(a);
+}
+
+// The all-pairs sweep of models without polarity, held to 80 scalar registers: with 84 or more
+// the stale-grid sweep kept fewer wavefronts resident on gfx950 (measured with SQ_WAVE_CYCLES /
+// SQ_BUSY_CYCLES, although the compiler reports 8 wavefronts per SIMD either way) and ran
+// slower; at 80 it needs no spills. The other variants need more and keep the default budget.
+template <bool HAS_TYPE, int MODE>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(80))) void k_sweep_binned_ap(SweepArgs a) {
+  sweep_binned<false, HAS_TYPE, false, MODE, false>(a);
 }
 
 // Naive finders (naive_all_pairs.hpp:41-103, naive_gabriel.hpp:47-119): j runs over all agents
@@ -795,7 +813,9 @@ __global__ __launch_bounds__(kBlock) void k_sym_naive(SweepArgs a) {
 
 template <bool GABRIEL, bool BINNED, bool HAS_TYPE, bool HAS_POL, int MODE, bool SYM>
 void launch_one(const SweepArgs& a, int n_blocks, hipStream_t stream) {
-  if (BINNED)
+  if (BINNED && !GABRIEL && !HAS_POL && !SYM)
+    hipLaunchKernelGGL((k_sweep_binned_ap<HAS_TYPE, MODE>), dim3(n_blocks), dim3(kBlock), 0, stream, a);
+  else if (BINNED)
     hipLaunchKernelGGL((k_sweep_binned<GABRIEL, HAS_TYPE, HAS_POL, MODE, SYM>), dim3(n_blocks),
                        dim3(kBlock), 0, stream, a);
   else

@@ -239,6 +239,82 @@ __device__ __forceinline__ float dot3(const float* a, const float* b) {
   return r;
 }
 
+// ------------------------------------------- correctly rounded division by a positive value
+// RN(x / d) and RN(1 / d) from the hardware reciprocal estimate (v_rcp_f32) by Markstein's
+// method (P. Markstein, "Computation of elementary functions on the IBM RISC System/6000
+// processor", IBM J. Res. Develop. 34(1), 1990): one Newton-Raphson step y = y0 + y0 (1 - d y0)
+// on the estimate gives y = RN(1 / d); with q0 = RN(x y) the remainder r = d q0 - x is exact in
+// one FMA, and q = RN(q0 - r y) = RN(x / d). Checked against IEEE division on gfx950
+// (tests/test_gpu_fast_div.py): the reciprocal for every d with |d| in [2^-16, 2^16], the
+// quotient for every pair of significands (x, d in [1, 2)). Scaling x and d by powers of two,
+// or flipping the sign of x, maps every step of the sequence exactly while all its values stay
+// normal, which holds on the quotient's domain d in [2^-16, 2^16], |x| in [2^-96, 2^96] or
+// x = +-0 (then q = x: the remainder is +0 and q0 - (+0) y keeps q0's sign, which needs y > 0,
+// so the quotient's domain has d > 0). Outside its domain each function divides as IEEE.
+constexpr float kDivDMin = 0x1p-16f, kDivDMax = 0x1p16f;
+constexpr float kDivXMin = 0x1p-96f, kDivXMax = 0x1p96f;
+
+__device__ __forceinline__ float rcp_refined(float d) {
+  const float y0 = __builtin_amdgcn_rcpf(d);
+  return __builtin_fmaf(__builtin_fmaf(-d, y0, 1.0f), y0, y0);
+}
+// the reciprocal's domain (either sign) and the quotient's (d > 0)
+// (bitwise operators on ints: no short-circuit branches)
+__device__ __forceinline__ bool recip_in_domain(float d) {
+  const float a = fabsf(d);
+  return (int)(a >= kDivDMin) & (int)(a <= kDivDMax);
+}
+__device__ __forceinline__ bool div_d_in_domain(float d) {
+  return (int)(d >= kDivDMin) & (int)(d <= kDivDMax);
+}
+__device__ __forceinline__ bool div_x_in_domain(float x) {
+  const float a = fabsf(x);
+  return ((int)(a >= kDivXMin) | (int)(a == 0.0f)) & (int)(a <= kDivXMax);
+}
+// RN(x / d) for x, d in the domain, y = rcp_refined(d).
+__device__ __forceinline__ float div_by_rcp(float x, float d, float y) {
+  const float q0 = x * y;
+  const float r = __builtin_fmaf(d, q0, -x);
+  return __builtin_fmaf(-r, y, q0);
+}
+// RN(1 / d).
+__device__ __forceinline__ float recip_rn(float d) {
+  const float y = rcp_refined(d);
+  return recip_in_domain(d) ? y : 1.0f / d;
+}
+// out[c] = RN(x[c] / d), c < 3.
+__device__ __forceinline__ void div3_rn(const float* x, float d, float* out) {
+  const float y = rcp_refined(d);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) out[c] = div_by_rcp(x[c], d, y);
+  const bool in = (int)div_d_in_domain(d) & (int)div_x_in_domain(x[0]) &
+                  (int)div_x_in_domain(x[1]) & (int)div_x_in_domain(x[2]);
+  if (__builtin_expect(!in, 0)) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[c] = x[c] / d;
+  }
+}
+
+// out[c] = RN(v[c] * F / d), c < 3, the quotients of PiecewiseLinear, for d > 0 and
+// |v[c]| <= d (v a displacement and d its length). The domain test is cheaper than div3_rn's:
+// with d <= 2^16 and |F| <= 2^64 every |v[c] * F| is below 2^81, so only the smallest
+// numerator needs a test, through its binary exponent (frexp: 0 for a zero numerator, which
+// is in the domain, at most -96 below 2^-96, subnormals included).
+__device__ __forceinline__ void pl_div3_rn(const float* v, float F, float d, float* out) {
+  const float x[3] = {v[0] * F, v[1] * F, v[2] * F};
+  const float y = rcp_refined(d);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) out[c] = div_by_rcp(x[c], d, y);
+  const int emin = min(min(__builtin_amdgcn_frexp_expf(x[0]), __builtin_amdgcn_frexp_expf(x[1])),
+                       __builtin_amdgcn_frexp_expf(x[2]));
+  const bool in = (int)(d >= kDivDMin) & (int)(d <= kDivDMax) & (int)(fabsf(F) <= 0x1p64f) &
+                  (int)(emin >= -95);
+  if (__builtin_expect(!in, 0)) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[c] = x[c] / d;
+  }
+}
+
 // ---------------------------------------------------------- force laws (predefined.hpp)
 // PiecewiseLinear, predefined.hpp:107-144.
 __device__ __forceinline__ void piecewise_linear(const float* disp, float d, float h, float krep,
@@ -248,9 +324,7 @@ __device__ __forceinline__ void piecewise_linear(const float* disp, float d, flo
     return;
   }
   const float F = krep * fmaxf(h - d, 0.0f) - kadh * fmaxf(d - h, 0.0f);
-  out[0] = disp[0] * F / d;
-  out[1] = disp[1] * F / d;
-  out[2] = disp[2] * F / d;
+  pl_div3_rn(disp, F, d, out);  // (disp * F) / d
 }
 
 // ---------------------------------------------------------------- polarity.hpp:111-198
@@ -328,9 +402,9 @@ __device__ __forceinline__ void pair_terms_k(const ForceParams& f, const float* 
   const float sti = pi.trig.x, cti = pi.trig.y, spi = pi.trig.z, cpi = pi.trig.w;
   const float stj = pj.trig.x, ctj = pj.trig.y, spj = pj.trig.z, cpj = pj.trig.w;
   const bool pole = !(fabsf(sti) > 1.1920928955078125e-07f);
-  const float inv_sti = pole ? 0.0f : 1.0f / sti;
+  const float inv_sti = pole ? 0.0f : recip_rn(sti);
   if (f.bend != 0.0f) {
-    const float inv_d = 1.0f / d;
+    const float inv_d = recip_rn(d);
     const float tv[3] = {sti * cpi, sti * spi, cti};  // to_vector3, polarity.hpp:111-118
     const float ov[3] = {stj * cpj, stj * spj, ctj};
     const float prod_i = dot3(tv, disp) * inv_d;
