@@ -210,13 +210,14 @@ struct Batch {
   std::vector<tecs_sim_desc> descs;
   std::vector<InstanceDesc> h_inst;
   std::vector<int> h_block_inst;
-  int n_inst = 0, n_blocks = 0, total_slots = 0, total_bins = 0, max_n = 0;
+  int n_inst = 0, n_blocks = 0, total_slots = 0, max_n = 0;
   // cells the host staging buffers, the pair-export rows and the snapshot slots hold: the
   // largest member slot range, which bounds every agent count set_capacity can allow
   int stage_n = 0;
   int total_cbins = 0;         // counting-sort segments of all instances
-  bool fill_bins = false;      // some instance counts multi-bin segments (sparse grid)
-  std::vector<int> h_fill_inst, h_fill_base;
+  bool sparse_rows = false;    // some instance has a sparse grid (counts per row of bins)
+  int total_rel = 0;           // row-relative bin offsets of all sparse instances
+  std::vector<int> h_fill_inst, h_fill_base;  // row workgroups: instance, first row
   hipStream_t stream = nullptr;
   bool own_stream = false;
   bool inst_dirty = false;
@@ -243,7 +244,8 @@ struct Batch {
   int* tidb = nullptr;
   int* cnt = nullptr;
   int* coff[2] = {nullptr, nullptr};   // segment offsets of the order held in id[b]
-  int* off = nullptr;                  // per-bin offsets (sparse grids only)
+  int* rel = nullptr;                  // row-relative bin offsets (sparse grids only)
+  int* dirty = nullptr;                // per segment: a sparse row to rewrite at the next rebuild
   int* skey[2] = {nullptr, nullptr};   // key of every slot of the order held in id[b]
   int* head[2] = {nullptr, nullptr};   // incremental rebuild: arrival lists per segment
   int* nxt = nullptr;
@@ -437,8 +439,8 @@ struct Batch {
     n_inst = count;
     write_step_count.assign(count, 0u);
 
-    // layout: slots padded per instance to whole workgroups; bins concatenated (+1 pad bin)
-    int64_t slot = 0, blocks = 0, bins = 0;
+    // layout: slots padded per instance to whole workgroups; segments concatenated (+1 pad segment)
+    int64_t slot = 0, blocks = 0;
     h_inst.resize(count);
     for (int k = 0; k < count; ++k) {
       const tecs_sim_desc& d = ds[k];
@@ -479,55 +481,56 @@ struct Batch {
         in.inv_bin = 1.0f;
         in.bin_count = 0;
       }
-      in.bin_begin = (int)bins;
-      // sparse grids (many bins per cell, e.g. a sheet of cells on a sphere) count segments of
-      // 2^shift bins so the scan does not run over every empty bin: about 4 segments per cell,
-      // which keeps the segments that run along a sheet short (the gather ranks a cell among
-      // its segment's members); on the 200k-cell epithelium 4 per cell beat 8 (rebuild 38.3
-      // vs 40.7 us) and 2 (41.7 us)
-      in.shift = 0;
-      if (is_binned(finder) && in.n > 0) {
-        const double bpc = (double)in.bin_count / (double)in.n;
-        if (bpc >= 8.0) in.shift = std::min(10, std::max(2, (int)std::floor(std::log2(bpc)) - 2));
+      // sparse grids (many bins per cell, e.g. a sheet of cells on a sphere) count per row of
+      // bins, so the scan does not run over every empty bin and a rebuild rewrites the bin
+      // offsets of the rows whose members changed bins only (seg_of); dense grids count per bin
+      in.rel_begin = -1;
+      if (is_binned(finder) && in.n > 0 && (double)in.bin_count / (double)in.n >= 8.0) {
+        in.rel_begin = 0;
+        // division of a key (< 2^31) by ext[0] as a multiply: with l = ceil(log2 e), the
+        // multiplier ceil(2^(31 + l) / e) fits 32 bits and the quotient is exact
+        const uint32_t e = (uint32_t)in.ext[0];
+        int l = 0;
+        while ((1u << l) < e) ++l;
+        in.row_mul = e == 1u ? 0u : (uint32_t)((((uint64_t)1 << (31 + l)) + e - 1) / e);
+        in.row_shift = e == 1u ? 0 : l - 1;
       }
-      in.cbin_count = ((in.bin_count + (1 << in.shift) - 1) >> in.shift) + 1;
+      in.cbin_count = (in.rel_begin >= 0 ? in.ext[1] * in.ext[2] : in.bin_count) + 1;
       in.dt = 0.0f;
       load_force(in.force, d.force);
       slot += in.padded;
       blocks += nb;
-      // bin ranges start on multiples of 64 entries (aligned 16-byte stores in the fill) and
-      // leave room for the fill's last 4-bin quad, which may reach bin_count + 3
-      bins += (((int64_t)in.bin_count + 4 + 63) / 64) * 64;
       max_n = std::max(max_n, in.cap);
       stage_n = std::max(stage_n, in.padded);
-      if (slot > (int64_t)1 << 30 || bins > (int64_t)2000000000)
+      if (slot > (int64_t)1 << 30)
         return fail(TECS_ERR_INVALID, "batch too large for 32-bit slot/bin indices");
     }
     total_slots = (int)slot;
     n_blocks = (int)blocks;
-    total_bins = (int)bins;
-    fill_bins = false;
-    for (int k = 0; k < count; ++k) fill_bins = fill_bins || h_inst[k].shift > 0;
+    sparse_rows = false;
+    for (int k = 0; k < count; ++k) sparse_rows = sparse_rows || h_inst[k].rel_begin >= 0;
     {
-      int64_t cb = 0;
+      // segment and row-relative offset ranges of the instances, each starting on a multiple of
+      // 64 entries (gap entries keep count 0, so the scan carries the next instance's start)
+      int64_t cb = 0, rb = 0;
       for (int k = 0; k < count; ++k) {
-        // without any sparse instance the segments are the bins (shared arrays)
-        h_inst[k].cbin_begin = fill_bins ? (int)cb : h_inst[k].bin_begin;
-        cb += h_inst[k].cbin_count;
-      }
-      // shared arrays cover the padded bin ranges (gap entries keep count 0)
-      total_cbins = fill_bins ? (int)cb : total_bins;
-    }
-    if (fill_bins) {
-      // one fill thread per kFillQuads quads of 4 bins (the padded range has room for the
-      // last quad)
-      for (int k = 0; k < count; ++k) {
-        const int quads = (h_inst[k].bin_count + 1 + 3) / 4;
-        for (int q = 0; q < quads; q += kBlock * kFillQuads) {
-          h_fill_inst.push_back(k);
-          h_fill_base.push_back(q);
+        InstanceDesc& in = h_inst[k];
+        in.cbin_begin = (int)cb;
+        cb += ((in.cbin_count + 63) / 64) * 64;
+        if (in.rel_begin >= 0) {
+          in.rel_begin = (int)rb;
+          rb += (((int64_t)(in.cbin_count - 1) * (in.ext[0] + 1) + 63) / 64) * 64;
+          // row workgroups of kBlock rows each
+          for (int q = 0; q < in.cbin_count - 1; q += kBlock) {
+            h_fill_inst.push_back(k);
+            h_fill_base.push_back(q);
+          }
         }
       }
+      if (cb > (int64_t)2000000000 || rb > (int64_t)2000000000)
+        return fail(TECS_ERR_INVALID, "batch too large for 32-bit bin indices");
+      total_cbins = (int)cb;
+      total_rel = (int)rb;
     }
     h_block_inst.resize(n_blocks);
     for (int k = 0; k < count; ++k)
@@ -590,15 +593,17 @@ struct Batch {
         HIP_TRY(hipMemsetAsync(coff[b], 0, sizeof(int) * ((size_t)total_cbins + 1), stream));
         HIP_TRY(hipMemsetAsync(head[b], 0xff, sizeof(int) * (size_t)total_cbins, stream));
       }
-      if (fill_bins) {
-        TRY(alloc(&off, (size_t)total_bins + 1));
+      if (sparse_rows) {
+        TRY(alloc(&rel, (size_t)total_rel));
+        TRY(alloc(&dirty, (size_t)total_cbins));
         TRY(alloc(&d_fill_inst, h_fill_inst.size()));
         TRY(alloc(&d_fill_base, h_fill_base.size()));
         HIP_TRY(hipMemcpyAsync(d_fill_inst, h_fill_inst.data(), sizeof(int) * h_fill_inst.size(),
                                hipMemcpyHostToDevice, stream));
         HIP_TRY(hipMemcpyAsync(d_fill_base, h_fill_base.data(), sizeof(int) * h_fill_base.size(),
                                hipMemcpyHostToDevice, stream));
-        HIP_TRY(hipMemsetAsync(off, 0, sizeof(int) * ((size_t)total_bins + 1), stream));
+        HIP_TRY(hipMemsetAsync(rel, 0, sizeof(int) * (size_t)total_rel, stream));
+        HIP_TRY(hipMemsetAsync(dirty, 0, sizeof(int) * (size_t)total_cbins, stream));
       }
       TRY(alloc(&bsum, (size_t)scan_tiles_for(total_cbins)));
       HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(int) * (size_t)total_cbins, stream));
@@ -779,11 +784,12 @@ struct Batch {
     r.coff = coff[ic ^ 1];
     r.skey_new = skey[ic ^ 1];
     r.head_clear = head[ic ^ 1];
-    r.off = off;
+    r.rel = rel;
+    r.dirty = dirty;
     r.keys_ready = keys_ready;
     r.scan_bsum = bsum;
     r.total_cbins = total_cbins;
-    r.fill = fill_bins;
+    r.fill = sparse_rows;
     r.fill_inst = d_fill_inst;
     r.fill_base = d_fill_base;
     r.fill_blocks = (int)h_fill_inst.size();
@@ -823,7 +829,8 @@ struct Batch {
     a.xs = xs;
     a.ps = ps;
     a.trig = trig_of(ps);
-    a.off = fill_bins ? off : coff[ic];
+    a.off = coff[ic];
+    a.rel = rel;
     a.id = id[ic];
     a.block_pairs = block_pairs;
     a.dl_cnt = dl_cnt;
@@ -876,6 +883,7 @@ struct Batch {
       k.head = head[ic];
       k.nxt = nxt;
       k.movers = movers;
+      k.dirty = dirty;
       // the scan's tile sums move with the counts when the scan takes the fused path, which
       // leaves them in bsum (with more tiles the top-level scan overwrites them)
       if (scan_tiles_for(total_cbins) <= kScanFusedTileLimit) k.tsum = bsum;
@@ -1418,7 +1426,8 @@ struct Batch {
       QueryArgs q{};
       q.inst = d_inst;
       q.xs = x[xc];
-      q.off = fill_bins ? off : coff[ic];
+      q.off = coff[ic];
+      q.rel = rel;
       q.id = id[ic];
       q.n = n;
       q.points = d_points;
@@ -1463,7 +1472,8 @@ struct Batch {
     QueryArgs q{};
     q.inst = d_inst;
     q.xs = x[xc];
-    q.off = fill_bins ? off : coff[ic];
+    q.off = coff[ic];
+      q.rel = rel;
     q.id = id[ic];
     q.n = n;
     q.points = d_points;

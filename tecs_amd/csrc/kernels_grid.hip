@@ -3,10 +3,10 @@
 // Grid::rebuild (grid.hpp:389-401) computes one bin key per cell and Kokkos::BinSort orders the
 // cells by key; on the Serial backend the members of a bin end up in ascending agent index.
 // The device version reproduces that order exactly and moves the cell state into it. Cells are
-// counted per segment of 2^shift consecutive bins (shift 0 = per bin for dense grids; a sparse
-// grid, e.g. a cell sheet on a sphere with ~40 bins per cell, counts segments of 4 bins, about
-// 8 segments per cell, so the scan runs over segments, not bins):
-//   1. keys:    key = flatten(clamp(int((p - min) / bin))), cnt[key >> shift] += 1 (atomics);
+// counted per segment: per bin for dense grids, per row of bins (fixed y, z) for sparse grids
+// (e.g. a cell sheet on a sphere with ~40 bins per cell, where a scan over every bin would cost
+// more than the cells), see seg_of:
+//   1. keys:    key = flatten(clamp(int((p - min) / bin))), cnt[seg(key)] += 1 (atomics);
 //               done by the epilogue of the sweep that wrote the positions when the host knows
 //               the next step rebuilds (count_key), else by k_bin_keys
 //   2. scan:    coff = exclusive_scan(cnt) over all instances' segments (+ pad segments)
@@ -15,9 +15,10 @@
 //   4. gather:  each cell's final slot is coff[seg] + #(segment members before it in (key,
 //               agent index) order); its state (position, polarity and its sines/cosines, id,
 //               old velocity) moves
-//   5. fill:    (shift > 0 only) per-bin offsets off[b] = coff[seg] + #(members with key < b),
-//               one coalesced write per bin; it reads the members as step 3 left them, so it
-//               runs in the same launch as step 4.
+//   5. rows:    (sparse grids) each row's bin offsets relative to its first slot,
+//               rel[x] = #(members with key < row key + x), for every row after a full rebuild
+//               and for the rows whose members changed bins after an incremental one; it reads
+//               the members as step 3 left them, so it runs in the same launch as step 4.
 // Step 3's order within a segment is arbitrary, step 4 makes it canonical, so the result does
 // not depend on atomic arrival order.
 #include "tecs_internal.hpp"
@@ -48,7 +49,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(RebuildArgs a) {
   int g = 0, key = 0;
   if (valid) {
     key = a.kc.keys[s];
-    g = in.cbin_begin + (key >> in.shift);
+    g = in.cbin_begin + seg_of(in, key);
   } else if (local == in.n) {
     a.kc.cnt[in.cbin_begin + in.cbin_count - 1] = 0;
   }
@@ -84,7 +85,7 @@ __device__ __forceinline__ void gather_block(const RebuildArgs& a, int lb) {
   if ((t - in.slot_begin) >= in.n) return;
   const int key = a.tkey[t];
   const int my_id = a.tid[t];
-  const int g = in.cbin_begin + (key >> in.shift);
+  const int g = in.cbin_begin + seg_of(in, key);
   const int start = a.coff[g], end = a.coff[g + 1];
   int rank = 0;
   for (int u = start; u < end; ++u) rank += before(a.tkey[u], a.tid[u], key, my_id) ? 1 : 0;
@@ -97,9 +98,11 @@ __device__ __forceinline__ void gather_block(const RebuildArgs& a, int lb) {
 template <class F>
 __device__ __forceinline__ void for_members(const RebuildArgs& a, const InstanceDesc& in, int g,
                                             int s0, int s1, int head, F f) {
+  const int nb = seg_bins(in);
+  const int k0 = (g - in.cbin_begin) * nb;  // the segment's keys are [k0, k0 + nb)
   for (int u = s0; u < s1; ++u) {
     const int ku = a.kc.keys[u];
-    if (in.cbin_begin + (ku >> in.shift) == g) f(ku, a.id_old[u]);
+    if ((unsigned)(ku - k0) < (unsigned)nb) f(ku, a.id_old[u]);
   }
   for (int u = head; u >= 0; u = a.kc.nxt[u]) f(a.kc.keys[u], a.id_old[u]);
 }
@@ -111,94 +114,82 @@ __device__ __forceinline__ void gather_block_inc(const RebuildArgs& a, int lb) {
   if ((t - in.slot_begin) >= in.n) return;
   const int key = a.kc.keys[t];
   const int my_id = a.id_old[t];
-  const int g = in.cbin_begin + (key >> in.shift);
+  const int g = in.cbin_begin + seg_of(in, key);
   const int s0 = a.coff_old[g], s1 = a.coff_old[g + 1], head = a.kc.head[g], dst0 = a.coff[g];
   int rank = 0;
   for_members(a, in, g, s0, s1, head, [&](int ku, int iu) { rank += before(ku, iu, key, my_id) ? 1 : 0; });
   move_cell(a, t, dst0 + rank, my_id, key);
 }
 
-// Per-bin offsets from the segment offsets: off[b] = segment start + #(members with key < b).
-// Member order does not matter, so the fill reads the members as the scatter left them (full)
-// or as for_members lists them (incremental) and does not wait for the gather.
-// A thread fills kFillQuads quads of 4 consecutive bins, kBlock quads apart (instance bin
-// ranges start on multiples of 64 entries, so the 16-byte stores of a wavefront cover 1 KiB of
-// consecutive, aligned offsets); all segment-offset loads are issued before the first store,
-// and only non-empty segments read member keys.
+// Step 5, one sparse row per wavefront pass: rel[x] = #(members with key < row key + x) for
+// x = 0..ext[0] (the last entry is the row's count). Member order does not matter, so the rows
+// read the members as the scatter left them (full) or as for_members lists them (incremental:
+// the old range, lanes in parallel, then the arrival list) and do not wait for the gather. The
+// members are counted into an LDS histogram of the row's bins (hist[x + 1] for a member in bin
+// x), whose inclusive scan is rel. A fill workgroup covers kBlock consecutive rows of one
+// instance, 64 per wavefront; after an incremental rebuild only the rows marked dirty by
+// count_key are written (and their marks cleared).
+constexpr int kRowHist = 1024;  // longest row (ext[0] + 1 entries) the histogram holds
+
 template <bool INC>
-__device__ __forceinline__ void fill_block(const RebuildArgs& a, int blk) {
+__device__ __forceinline__ void fill_rows(const RebuildArgs& a, int blk, int* hist_all) {
   const InstanceDesc& in = a.inst[a.fill_inst[blk]];
-  int* out = a.off + in.bin_begin;
-  const bool seg_quads = in.shift >= 2;  // a quad never straddles two segments
-  int b0[kFillQuads], start[kFillQuads], end[kFillQuads];
-#pragma unroll
-  for (int j = 0; j < kFillQuads; ++j) {
-    b0[j] = (a.fill_base[blk] + (int)threadIdx.x + j * kBlock) * 4;
-    start[j] = end[j] = 0;
-    if (seg_quads && b0[j] + 3 < in.bin_count) {
-      const int g = in.cbin_begin + (b0[j] >> in.shift);
-      start[j] = a.coff[g];
-      end[j] = a.coff[g + 1];
-    }
-  }
-  // incremental: the old range and arrivals, loaded only for occupied segments
-  int os0[kFillQuads], os1[kFillQuads], hd[kFillQuads];
-#pragma unroll
-  for (int j = 0; j < kFillQuads; ++j) {
-    os0[j] = os1[j] = 0;
-    hd[j] = -1;
-    if (INC && end[j] > start[j]) {
-      const int g = in.cbin_begin + (b0[j] >> in.shift);
-      os0[j] = a.coff_old[g];
-      os1[j] = a.coff_old[g + 1];
-      hd[j] = a.kc.head[g];
-    }
-  }
-#pragma unroll
-  for (int j = 0; j < kFillQuads; ++j) {
-    if (b0[j] > in.bin_count) continue;
-    int v[4];
-    if (seg_quads && b0[j] + 3 < in.bin_count) {
-      int c0 = 0, c1 = 0, c2 = 0, c3 = 0;
-      const int bq = b0[j];
-      auto count = [&](int k, int) {
-        c0 += k < bq ? 1 : 0;
-        c1 += k < bq + 1 ? 1 : 0;
-        c2 += k < bq + 2 ? 1 : 0;
-        c3 += k < bq + 3 ? 1 : 0;
-      };
-      if (end[j] > start[j]) {
-        if (INC)
-          for_members(a, in, in.cbin_begin + (bq >> in.shift), os0[j], os1[j], hd[j], count);
-        else
-          for (int u = start[j]; u < end[j]; ++u) count(a.tkey[u], 0);
+  const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+  int* hist = hist_all + w * kRowHist;
+  const int rows = in.cbin_count - 1;
+  const int e0 = in.ext[0];
+  const int r0 = a.fill_base[blk] + w * 64;
+  bool need = r0 + lane < rows;
+  if (INC && need) need = a.dirty[in.cbin_begin + r0 + lane] != 0;
+  unsigned long long todo = __ballot(need);
+  while (todo != 0ull) {
+    const int row = r0 + __ffsll((long long)todo) - 1;
+    todo &= todo - 1ull;
+    const int g = in.cbin_begin + row;
+    const int k0 = row * e0;
+    int* out = a.rel + in.rel_begin + row * (e0 + 1);
+    if (e0 + 1 <= kRowHist) {
+      for (int x = lane; x <= e0; x += 64) hist[x] = 0;
+      __builtin_amdgcn_wave_barrier();
+      if (INC) {
+        const int s1 = a.coff_old[g + 1];
+        for (int u = a.coff_old[g] + lane; u < s1; u += 64) {
+          const int ku = a.kc.keys[u] - k0;
+          if ((unsigned)ku < (unsigned)e0) atomicAdd(hist + ku + 1, 1);
+        }
+        if (lane == 0)
+          for (int u = a.kc.head[g]; u >= 0; u = a.kc.nxt[u]) atomicAdd(hist + (a.kc.keys[u] - k0) + 1, 1);
+      } else {
+        const int s1 = a.coff[g + 1];
+        for (int u = a.coff[g] + lane; u < s1; u += 64) atomicAdd(hist + (a.tkey[u] - k0) + 1, 1);
       }
-      v[0] = start[j] + c0;
-      v[1] = start[j] + c1;
-      v[2] = start[j] + c2;
-      v[3] = start[j] + c3;
+      __builtin_amdgcn_wave_barrier();
+      int carry = 0;
+      for (int x0 = 0; x0 <= e0; x0 += 64) {
+        const int x = x0 + lane;
+        int v = x <= e0 ? hist[x] : 0;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+          const int y = __shfl_up(v, o, 64);
+          if (lane >= o) v += y;
+        }
+        if (x <= e0) out[x] = carry + v;
+        carry += __shfl(v, 63, 64);
+      }
+      __builtin_amdgcn_wave_barrier();
     } else {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int b = b0[j] + q;
-        if (b >= in.bin_count) {
-          v[q] = in.slot_begin + in.n;  // the pad bin (and the alignment gap after it)
-          continue;
-        }
-        const int g = in.cbin_begin + (b >> in.shift);
-        const int st = a.coff[g], en = a.coff[g + 1];
+      // a row too long for the histogram: every lane counts the members below its bins
+      for (int x = lane; x <= e0; x += 64) {
         int less = 0;
-        if (en > st) {
-          if (INC)
-            for_members(a, in, g, a.coff_old[g], a.coff_old[g + 1], a.kc.head[g],
-                        [&](int k, int) { less += k < b ? 1 : 0; });
-          else
-            for (int u = st; u < en; ++u) less += a.tkey[u] < b ? 1 : 0;
-        }
-        v[q] = st + less;
+        if (INC)
+          for_members(a, in, g, a.coff_old[g], a.coff_old[g + 1], a.kc.head[g],
+                      [&](int k, int) { less += k < k0 + x ? 1 : 0; });
+        else
+          for (int u = a.coff[g]; u < a.coff[g + 1]; ++u) less += a.tkey[u] < k0 + x ? 1 : 0;
+        out[x] = less;
       }
     }
-    *reinterpret_cast<int4*>(out + b0[j]) = make_int4(v[0], v[1], v[2], v[3]);
+    if (lane == 0) a.dirty[g] = 0;
   }
 }
 
@@ -223,7 +214,7 @@ __global__ __launch_bounds__(kBlock) void k_link_movers(RebuildArgs a, int n_wor
   while (m != 0ull) {
     const int s = w * 64 + (__ffsll((long long)m) - 1);
     m &= m - 1ull;
-    const int g = in.cbin_begin + (a.kc.keys[s] >> in.shift);
+    const int g = in.cbin_begin + seg_of(in, a.kc.keys[s]);
     a.kc.nxt[s] = atomicExch(a.kc.head + g, s);
   }
 }
@@ -231,6 +222,7 @@ __global__ __launch_bounds__(kBlock) void k_link_movers(RebuildArgs a, int n_wor
 // Steps 4 and 5 in one launch: workgroups [0, n_gather) gather (XCD-aware order), the rest fill.
 template <bool INC>
 __global__ __launch_bounds__(kBlock) void k_gather_fill(RebuildArgs a, int n_gather) {
+  __shared__ int hist[(kBlock / 64) * kRowHist];
   const int p = (int)blockIdx.x;
   if (p < n_gather) {
     if (INC)
@@ -238,7 +230,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_fill(RebuildArgs a, int n_gat
     else
       gather_block(a, remap_block(p, n_gather));
   } else {
-    fill_block<INC>(a, p - n_gather);
+    fill_rows<INC>(a, p - n_gather, hist);
   }
 }
 

@@ -43,7 +43,7 @@ namespace {
 template <typename F>
 __device__ __forceinline__ void for_rows(const QueryArgs& a, const InstanceDesc& in, float4 p,
                                          int radius_bins, F&& f) {
-  const int* __restrict__ offb = a.off + in.bin_begin;
+  const BinRanges<true> offb = bin_ranges<true>(in, a.off, a.rel);
   int bc[3];
   bin_coords_div(in, p.x, p.y, p.z, bc);  // calc_bin_coords_from_point, grid.hpp:222-230
   const int x_lo = max(bc[0] - radius_bins, 0);
@@ -54,7 +54,7 @@ __device__ __forceinline__ void for_rows(const QueryArgs& a, const InstanceDesc&
     for (int dy = -radius_bins; dy <= radius_bins; ++dy) {
       const int y = bc[1] + dy;
       if (y < 0 || y >= in.ext[1]) continue;
-      f(offb[flatten(in, x_lo, y, z)], offb[flatten(in, x_hi, y, z) + 1]);
+      f(offb.at(x_lo, y, z), offb.at(x_hi + 1, y, z));
     }
   }
 }

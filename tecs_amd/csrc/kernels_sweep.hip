@@ -66,11 +66,11 @@ __device__ __forceinline__ void visit(const SweepArgs& a, const InstanceDesc& in
   }
 }
 
-template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE, bool SYM>
+template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE, bool SYM, bool ROWS>
 __device__ __forceinline__ void walk_binned(const SweepArgs& a, const InstanceDesc& in, int s,
                                             int me_id, float4 me, const Pol& pme, CellState& cs) {
   const float4* __restrict__ xs = a.xs;
-  const int* __restrict__ offb = a.off + in.bin_begin;
+  const BinRanges<ROWS> offb = bin_ranges<ROWS>(in, a.off, a.rel);
   int bc[3];
   if (GABRIEL || SYM)
     bin_coords_div(in, me.x, me.y, me.z, bc);  // binned_gabriel.hpp:101, *_symmetric.hpp
@@ -86,8 +86,8 @@ __device__ __forceinline__ void walk_binned(const SweepArgs& a, const InstanceDe
     for (int dy = -R; dy <= R; ++dy) {
       const int y = bc[1] + dy;
       if (y < 0 || y >= in.ext[1]) continue;
-      const int start = offb[flatten(in, x_lo, y, z)];
-      const int end = offb[flatten(in, x_hi, y, z) + 1];
+      const int start = offb.at(x_lo, y, z);
+      const int end = offb.at(x_hi + 1, y, z);
       for (int m = start; m < end; ++m) {
         if (m == s) continue;
         const float4 pj = xs[m];
@@ -110,10 +110,10 @@ constexpr int kDeferMax = 16;
 // One candidate m of cell s: the reference's tests in its order (distance, for the symmetric
 // finders the partner index, then the Gabriel ball; binned_all_pairs.hpp:140-151,
 // binned_gabriel.hpp:123-209). A NaN distance is accepted, as `d2 >= r2` is false for it.
-template <bool GABRIEL, bool SYM>
+template <bool GABRIEL, bool SYM, class BR>
 __device__ __forceinline__ bool accepted(const SweepArgs& a, const InstanceDesc& in,
                                          const float4* __restrict__ xs,
-                                         const int* __restrict__ offb, int s, int me_id,
+                                         const BR& offb, int s, int me_id,
                                          float4 me, int m, float4 pj) {
   const float disp[3] = {me.x - pj.x, me.y - pj.y, me.z - pj.z};
   const float d2 = len2(disp[0], disp[1], disp[2]);
@@ -136,12 +136,12 @@ __device__ __forceinline__ bool accepted(const SweepArgs& a, const InstanceDesc&
 //      first kDeferMax inline.
 // Each lane still adds its contributions in its own visiting order, so the sums are bitwise
 // those of walk_binned. Called by every lane of the wavefront (inactive lanes have no cell).
-template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE, bool SYM>
+template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE, bool SYM, bool ROWS>
 __device__ __forceinline__ void walk_binned_deferred(const SweepArgs& a, const InstanceDesc& in,
                                                      int s, int me_id, bool active, float4 me,
                                                      const Pol& pme, CellState& cs, int* list) {
   const float4* __restrict__ xs = a.xs;
-  const int* __restrict__ offb = a.off + in.bin_begin;
+  const BinRanges<ROWS> offb = bin_ranges<ROWS>(in, a.off, a.rel);
   int na = 0;  // accepted candidates, also those beyond the list
   int rs[9], re[9];
   if (active) {
@@ -156,8 +156,8 @@ __device__ __forceinline__ void walk_binned_deferred(const SweepArgs& a, const I
     for (int r = 0; r < 9; ++r) {
       const int z = bc[2] + r / 3 - 1, y = bc[1] + r % 3 - 1;
       const bool ok = z >= 0 && z < in.ext[2] && y >= 0 && y < in.ext[1];
-      rs[r] = ok ? offb[flatten(in, x_lo, y, z)] : 0;
-      re[r] = ok ? offb[flatten(in, x_hi, y, z) + 1] : 0;
+      rs[r] = ok ? offb.at(x_lo, y, z) : 0;
+      re[r] = ok ? offb.at(x_hi + 1, y, z) : 0;
     }
     for (int r = 0; r < 9; ++r) {
       const int end = re[r];
@@ -243,9 +243,10 @@ static_assert(sizeof(FlatLds) >= sizeof(int) * kDeferMax * kBlock, "FlatLds too 
 // Step 0: the lane's ranges into LDS; returns their count. `fits` is cleared when a range holds
 // 256 slots or more; `self` is set when a range holds the lane's own slot s and the walk will
 // accept it (its distance to itself, 0 or NaN, is not >= r2).
+template <bool ROWS>
 __device__ __forceinline__ int flat_ranges(const SweepArgs& a, const InstanceDesc& in, int s,
                                            float4 me, FlatLds& L, bool& fits, bool& self) {
-  const int* __restrict__ offb = a.off + in.bin_begin;
+  const BinRanges<ROWS> offb = bin_ranges<ROWS>(in, a.off, a.rel);
   const int lane = (int)threadIdx.x;
   int bc[3];
   bin_coords_mul(in, me.x, me.y, me.z, bc);  // binned_all_pairs.hpp:103-109
@@ -256,8 +257,8 @@ __device__ __forceinline__ int flat_ranges(const SweepArgs& a, const InstanceDes
   for (int r = 0; r < 9; ++r) {
     const int z = bc[2] + r / 3 - 1, y = bc[1] + r % 3 - 1;
     const bool ok = z >= 0 && z < in.ext[2] && y >= 0 && y < in.ext[1];
-    rs[r] = ok ? offb[flatten(in, x_lo, y, z)] : 0;
-    re[r] = ok ? offb[flatten(in, x_hi, y, z) + 1] : 0;
+    rs[r] = ok ? offb.at(x_lo, y, z) : 0;
+    re[r] = ok ? offb.at(x_hi + 1, y, z) : 0;
   }
   int nr = 0;
 #pragma unroll
@@ -497,7 +498,7 @@ __device__ __forceinline__ void finish(const SweepArgs& a, const InstanceDesc& i
   if (threadIdx.x == 0) a.block_pairs[lb] += tp;
 }
 
-template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE, bool SYM>
+template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE, bool SYM, bool ROWS>
 __device__ __forceinline__ void sweep_binned(const SweepArgs& a) {
   const int lb = logical_block();
   const int ii = a.block_inst[lb];
@@ -532,38 +533,38 @@ __device__ __forceinline__ void sweep_binned(const SweepArgs& a) {
       flat.tab[threadIdx.x] = (HAS_TYPE && f.n_types > 0) ? f.table[threadIdx.x] : f.kadh;
     }
     bool fits = true, self = false;
-    const int nr = active ? flat_ranges(a, in, s, me, flat, fits, self) : 0;
+    const int nr = active ? flat_ranges<ROWS>(a, in, s, me, flat, fits, self) : 0;
     // the table, and whether every range fits an 8-bit length (a workgroup barrier)
     if (__syncthreads_and(fits)) {
       walk_binned_flat<HAS_TYPE, HAS_POL, MODE>(a, in, s, nr, self, me, pme, cs, flat);
     } else {
       // a range of 256 slots or more: the row-by-row walk, its list in the ranges' LDS (the
       // workgroup has passed the barrier, so no lane reads the ranges any more)
-      walk_binned_deferred<GABRIEL, HAS_TYPE, HAS_POL, MODE, SYM>(a, in, s, me_id, active, me, pme,
+      walk_binned_deferred<GABRIEL, HAS_TYPE, HAS_POL, MODE, SYM, ROWS>(a, in, s, me_id, active, me, pme,
                                                                   cs, reinterpret_cast<int*>(&flat) + threadIdx.x);
     }
   } else if (kDefer && in.search_radius == 1) {
     __shared__ int list[kDeferMax * kBlock];
-    walk_binned_deferred<GABRIEL, HAS_TYPE, HAS_POL, MODE, SYM>(a, in, s, me_id, active, me, pme,
+    walk_binned_deferred<GABRIEL, HAS_TYPE, HAS_POL, MODE, SYM, ROWS>(a, in, s, me_id, active, me, pme,
                                                                 cs, list + threadIdx.x);
   } else if (active) {
-    walk_binned<GABRIEL, HAS_TYPE, HAS_POL, MODE, SYM>(a, in, s, me_id, me, pme, cs);
+    walk_binned<GABRIEL, HAS_TYPE, HAS_POL, MODE, SYM, ROWS>(a, in, s, me_id, me, pme, cs);
   }
   finish<HAS_POL, MODE, SYM>(a, in, ii, lb, s, active, me, mp, cs, ci, old_key);
 }
 
-template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE, bool SYM>
+template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE, bool SYM, bool ROWS>
 __global__ __launch_bounds__(kBlock) void k_sweep_binned(SweepArgs a) {
-  sweep_binned<GABRIEL, HAS_TYPE, HAS_POL, MODE, SYM>(a);
+  sweep_binned<GABRIEL, HAS_TYPE, HAS_POL, MODE, SYM, ROWS>(a);
 }
 
 // The all-pairs sweep of models without polarity, held to 80 scalar registers: with 84 or more
 // the stale-grid sweep kept fewer wavefronts resident on gfx950 (measured with SQ_WAVE_CYCLES /
 // SQ_BUSY_CYCLES, although the compiler reports 8 wavefronts per SIMD either way) and ran
 // slower; at 80 it needs no spills. The other variants need more and keep the default budget.
-template <bool HAS_TYPE, int MODE>
+template <bool HAS_TYPE, int MODE, bool ROWS>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(80))) void k_sweep_binned_ap(SweepArgs a) {
-  sweep_binned<false, HAS_TYPE, false, MODE, false>(a);
+  sweep_binned<false, HAS_TYPE, false, MODE, false, ROWS>(a);
 }
 
 // Naive finders (naive_all_pairs.hpp:41-103, naive_gabriel.hpp:47-119): j runs over all agents
@@ -636,7 +637,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_naive(SweepArgs a) {
 // STALE: a.skey is set (a stale grid, movers possible). Search radius 1: the stayers defer
 // their forces as in walk_binned_deferred, list entries carrying the role in bits 30-31; the
 // rare movers walk serially.
-template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, bool STALE>
+template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, bool STALE, bool ROWS>
 __global__ __launch_bounds__(kBlock) void k_sym_binned(SweepArgs a) {
   __shared__ int list_all[kDeferMax * kBlock];
   __shared__ unsigned lds_u[4];
@@ -645,7 +646,7 @@ __global__ __launch_bounds__(kBlock) void k_sym_binned(SweepArgs a) {
   const int s = lb * kBlock + (int)threadIdx.x;
   const bool active = (s - in.slot_begin) < in.n;
   const float4* __restrict__ xs = a.xs;
-  const int* __restrict__ offb = a.off + in.bin_begin;
+  const BinRanges<ROWS> offb = bin_ranges<ROWS>(in, a.off, a.rel);
   const int* skey = STALE ? a.skey : nullptr;
   int* list = list_all + threadIdx.x;
   float4 me = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -673,8 +674,8 @@ __global__ __launch_bounds__(kBlock) void k_sym_binned(SweepArgs a) {
       for (int r = 0; r < 9; ++r) {
         const int z = bc[2] + r / 3 - 1, y = bc[1] + r % 3 - 1;
         const bool ok = z >= 0 && z < in.ext[2] && y >= 0 && y < in.ext[1];
-        rs[r] = ok ? offb[flatten(in, x_lo, y, z)] : 0;
-        re[r] = ok ? offb[flatten(in, x_hi, y, z) + 1] : 0;
+        rs[r] = ok ? offb.at(x_lo, y, z) : 0;
+        re[r] = ok ? offb.at(x_hi + 1, y, z) : 0;
       }
       for (int r = 0; r < 9; ++r) {
         const int end = re[r];
@@ -730,11 +731,11 @@ __global__ __launch_bounds__(kBlock) void k_sym_binned(SweepArgs a) {
       }
     }
   } else if (stayer) {
-    sym_cell_serial<GABRIEL, HAS_TYPE, HAS_POL, false>(a, in, s, me_id, false, me, pme, bc,
+    sym_cell_serial<GABRIEL, HAS_TYPE, HAS_POL, false, ROWS>(a, in, s, me_id, false, me, pme, bc,
                                                        acc_pos, acc_pol, pairs);
   }
   if (STALE && mover)
-    sym_cell_serial<GABRIEL, HAS_TYPE, HAS_POL, false>(a, in, s, me_id, true, me, pme, bc, acc_pos,
+    sym_cell_serial<GABRIEL, HAS_TYPE, HAS_POL, false, ROWS>(a, in, s, me_id, true, me, pme, bc, acc_pos,
                                                        acc_pol, pairs);
   if (active) {
     a.o_pos[s] = make_float4(acc_pos[0], acc_pos[1], acc_pos[2], 0.0f);
@@ -813,11 +814,19 @@ __global__ __launch_bounds__(kBlock) void k_sym_naive(SweepArgs a) {
 
 template <bool GABRIEL, bool BINNED, bool HAS_TYPE, bool HAS_POL, int MODE, bool SYM>
 void launch_one(const SweepArgs& a, int n_blocks, hipStream_t stream) {
-  if (BINNED && !GABRIEL && !HAS_POL && !SYM)
-    hipLaunchKernelGGL((k_sweep_binned_ap<HAS_TYPE, MODE>), dim3(n_blocks), dim3(kBlock), 0, stream, a);
-  else if (BINNED)
-    hipLaunchKernelGGL((k_sweep_binned<GABRIEL, HAS_TYPE, HAS_POL, MODE, SYM>), dim3(n_blocks),
+  // a.rel: the batch has sparse instances (their bin lookups need the ROWS kernels)
+  if (BINNED && !GABRIEL && !HAS_POL && !SYM) {
+    if (a.rel)
+      hipLaunchKernelGGL((k_sweep_binned_ap<HAS_TYPE, MODE, true>), dim3(n_blocks), dim3(kBlock), 0, stream, a);
+    else
+      hipLaunchKernelGGL((k_sweep_binned_ap<HAS_TYPE, MODE, false>), dim3(n_blocks), dim3(kBlock), 0, stream, a);
+  } else if (BINNED && a.rel) {
+    hipLaunchKernelGGL((k_sweep_binned<GABRIEL, HAS_TYPE, HAS_POL, MODE, SYM, true>), dim3(n_blocks),
                        dim3(kBlock), 0, stream, a);
+  } else if (BINNED) {
+    hipLaunchKernelGGL((k_sweep_binned<GABRIEL, HAS_TYPE, HAS_POL, MODE, SYM, false>), dim3(n_blocks),
+                       dim3(kBlock), 0, stream, a);
+  }
   else
     hipLaunchKernelGGL((k_sweep_naive<GABRIEL, HAS_TYPE, HAS_POL, MODE, SYM>), dim3(n_blocks),
                        dim3(kBlock), 0, stream, a);
@@ -862,11 +871,17 @@ int launch_mode(int mode, bool has_type, bool has_pol, const SweepArgs& a, int n
 
 template <bool GABRIEL, bool BINNED, bool HAS_TYPE, bool HAS_POL>
 void launch_sym(const SweepArgs& a, int n_blocks, hipStream_t stream) {
-  if (BINNED && a.skey)
-    hipLaunchKernelGGL((k_sym_binned<GABRIEL, HAS_TYPE, HAS_POL, true>), dim3(n_blocks),
+  if (BINNED && a.skey && a.rel)
+    hipLaunchKernelGGL((k_sym_binned<GABRIEL, HAS_TYPE, HAS_POL, true, true>), dim3(n_blocks),
+                       dim3(kBlock), 0, stream, a);
+  else if (BINNED && a.skey)
+    hipLaunchKernelGGL((k_sym_binned<GABRIEL, HAS_TYPE, HAS_POL, true, false>), dim3(n_blocks),
+                       dim3(kBlock), 0, stream, a);
+  else if (BINNED && a.rel)
+    hipLaunchKernelGGL((k_sym_binned<GABRIEL, HAS_TYPE, HAS_POL, false, true>), dim3(n_blocks),
                        dim3(kBlock), 0, stream, a);
   else if (BINNED)
-    hipLaunchKernelGGL((k_sym_binned<GABRIEL, HAS_TYPE, HAS_POL, false>), dim3(n_blocks),
+    hipLaunchKernelGGL((k_sym_binned<GABRIEL, HAS_TYPE, HAS_POL, false, false>), dim3(n_blocks),
                        dim3(kBlock), 0, stream, a);
   else
     hipLaunchKernelGGL((k_sym_naive<GABRIEL, HAS_TYPE, HAS_POL>), dim3(n_blocks), dim3(kBlock), 0,

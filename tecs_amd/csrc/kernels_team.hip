@@ -103,13 +103,13 @@ struct TeamLds {
 };
 
 // ---------------------------------------------------------------- serial fallback (leader)
-template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE>
+template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE, bool ROWS>
 __device__ void serial_cell(const SweepArgs& a, const InstanceDesc& in, int s, float4 me,
                             const Pol& pme, float* acc_pos, float* acc_pol, unsigned& pairs) {
   int64_t emit_at = 0;
   if (MODE == kModeEmit) emit_at = a.row_ptr[a.id[s]];
   const float4* __restrict__ xs = a.xs;
-  const int* __restrict__ offb = a.off + in.bin_begin;
+  const BinRanges<ROWS> offb = bin_ranges<ROWS>(in, a.off, a.rel);
   int bc[3];
   if (GABRIEL)
     bin_coords_div(in, me.x, me.y, me.z, bc);
@@ -122,8 +122,8 @@ __device__ void serial_cell(const SweepArgs& a, const InstanceDesc& in, int s, f
     for (int dy = -1; dy <= 1; ++dy) {
       const int y = bc[1] + dy;
       if (y < 0 || y >= in.ext[1]) continue;
-      const int start = offb[flatten(in, x_lo, y, z)];
-      const int end = offb[flatten(in, x_hi, y, z) + 1];
+      int start, end;
+      offb.row(x_lo, x_hi, y, z, start, end);
       for (int m = start; m < end; ++m) {
         if (m == s) continue;
         const float4 pj = xs[m];
@@ -145,8 +145,8 @@ __device__ void serial_cell(const SweepArgs& a, const InstanceDesc& in, int s, f
           bool blocked = false;
           for (int b2 = mn[2]; b2 <= mx[2] && !blocked; ++b2)
             for (int b1 = mn[1]; b1 <= mx[1] && !blocked; ++b1) {
-              const int s2 = offb[flatten(in, mn[0], b1, b2)];
-              const int e2 = offb[flatten(in, mx[0], b1, b2) + 1];
+              int s2, e2;
+              offb.row(mn[0], mx[0], b1, b2, s2, e2);
               for (int k = s2; k < e2; ++k) {
                 if (k == s || k == m) continue;
                 const float4 pk = xs[k];
@@ -205,8 +205,8 @@ __device__ __forceinline__ void team_sync() {
 // abit bits 28-29: the entry's role in a symmetric sweep (SymRole; 0 = blocker only)
 constexpr int kRoleShift = 28;
 
-template <int T, bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE, bool SYM>
-__global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
+template <int T, bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE, bool SYM, bool ROWS>
+__global__ __launch_bounds__(kBlock, 6) void k_sweep_team(SweepArgs a) {
   static_assert(!SYM || MODE == kModeDelta, "symmetric team sweeps evaluate forces only");
   constexpr bool SPLIT = HAS_POL && !SYM;
   constexpr int TB = TeamLds<T, GABRIEL, HAS_POL, SPLIT>::TB;
@@ -223,7 +223,11 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
   const int team = (int)threadIdx.x / T;
   const int lane = (int)threadIdx.x % T;
   const int s = tb * TB + team;
-  const int chunk = s / kBlock;  // the 256-slot chunk, one instance
+  // the 256-slot chunk, one instance; TB divides kBlock, so every cell of the workgroup is in
+  // the chunk of its first one (computed from that, the instance and its descriptor stay in
+  // scalar registers)
+  static_assert(kBlock % TB == 0, "a workgroup's cells lie in one chunk");
+  const int chunk = (tb * TB) / kBlock;
   const int ii = a.block_inst[chunk];
   const InstanceDesc& in = a.inst[ii];
   const bool active = (s - in.slot_begin) < in.n;
@@ -250,18 +254,17 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
     }
     // row rr = (dz + 1) * 3 + (dy + 1), looked up by lane rr mod T
     const int x_lo = max(bc[0] - 1, 0), x_hi = min(bc[0] + 1, in.ext[0] - 1);
-    const int* offb = a.off + in.bin_begin;
+    const BinRanges<ROWS> offb = bin_ranges<ROWS>(in, a.off, a.rel);
     for (int rr = lane; rr < 9; rr += T) {
       const int z = bc[2] + rr / 3 - 1, y = bc[1] + rr % 3 - 1;
       int start = 0, len = 0, b1 = 0x7fffffff, b2 = 0x7fffffff;
       if (z >= 0 && z < in.ext[2] && y >= 0 && y < in.ext[1]) {
-        const int b_lo = flatten(in, x_lo, y, z);
-        const int b_hi = flatten(in, x_hi, y, z);
-        start = offb[b_lo];
-        len = offb[b_hi + 1] - start;
+        int end = 0;
+        offb.row(x_lo, x_hi, y, z, start, end);
+        len = end - start;
         if (GABRIEL) {
-          if (x_lo + 1 <= x_hi) b1 = offb[b_lo + 1];
-          if (x_lo + 2 <= x_hi) b2 = offb[b_lo + 2];
+          if (x_lo + 1 <= x_hi) b1 = offb.at(x_lo + 1, y, z);
+          if (x_lo + 2 <= x_hi) b2 = offb.at(x_lo + 2, y, z);
         }
       }
       L.rows[team].r.start[rr] = start;
@@ -484,10 +487,10 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
   const bool leader = lane == 0 && active;
   if (leader) {
     if (SYM && (overflow || L.unsafe[team])) {
-      sym_cell_serial<GABRIEL, HAS_TYPE, HAS_POL, true>(a, in, s, me_id, mover, me, pme, bc,
+      sym_cell_serial<GABRIEL, HAS_TYPE, HAS_POL, true, ROWS>(a, in, s, me_id, mover, me, pme, bc,
                                                         acc_pos, acc_pol, pairs);
     } else if (overflow || L.unsafe[team]) {
-      serial_cell<GABRIEL, HAS_TYPE, HAS_POL, MODE>(a, in, s, me, pme, acc_pos, acc_pol, pairs);
+      serial_cell<GABRIEL, HAS_TYPE, HAS_POL, MODE, ROWS>(a, in, s, me, pme, acc_pos, acc_pol, pairs);
     } else {
       int64_t emit_at = 0;
       if (MODE == kModeEmit) emit_at = a.row_ptr[a.id[s]];
@@ -618,8 +621,13 @@ __global__ __launch_bounds__(kBlock) void k_sweep_team(SweepArgs a) {
 
 template <int T, bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE, bool SYM = false>
 void launch_t(const SweepArgs& a, int n_blocks, hipStream_t stream) {
-  hipLaunchKernelGGL((k_sweep_team<T, GABRIEL, HAS_TYPE, HAS_POL, MODE, SYM>), dim3(n_blocks * T),
-                     dim3(kBlock), 0, stream, a);
+  // a.rel: the batch has sparse instances (their bin lookups need the ROWS kernels)
+  if (a.rel)
+    hipLaunchKernelGGL((k_sweep_team<T, GABRIEL, HAS_TYPE, HAS_POL, MODE, SYM, true>), dim3(n_blocks * T),
+                       dim3(kBlock), 0, stream, a);
+  else
+    hipLaunchKernelGGL((k_sweep_team<T, GABRIEL, HAS_TYPE, HAS_POL, MODE, SYM, false>), dim3(n_blocks * T),
+                       dim3(kBlock), 0, stream, a);
 }
 
 template <int T, bool GABRIEL, int MODE, bool SYM = false>

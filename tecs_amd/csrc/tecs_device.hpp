@@ -55,11 +55,13 @@ struct InstanceDesc {
   int padded;      // slot range length, multiple of kBlock
   int block_begin; // first logical workgroup of the instance
   int n_blocks;    // padded / kBlock
-  int bin_begin;   // first entry of the instance in the concatenated bin arrays
   int bin_count;   // real bins; the pad bin has local index bin_count
-  int shift;       // the counting sort counts segments of 2^shift consecutive bins
   int cbin_begin;  // first entry of the instance in the concatenated segment arrays
   int cbin_count;  // segments including the pad segment (the last one)
+  int rel_begin;   // sparse grid: first entry of its row-relative bin offsets (-1: dense grid,
+                   // one counting segment per bin)
+  unsigned row_mul;  // sparse grid: key / ext[0] == umulhi(key, row_mul) >> row_shift for every
+  int row_shift;     // key < 2^31 (row_mul 0: ext[0] == 1)
   int ext[3];
   float minb[3];
   float bin_size, inv_bin;
@@ -151,6 +153,68 @@ __device__ __forceinline__ int aggregated_atomic(int* counter_base, int idx, boo
   return old - (lane - my_head);
 }
 
+// Counting segments and bin offsets. A dense grid counts its cells per bin, so the scanned
+// segment offsets are the bin offsets. A sparse grid (many bins per cell, e.g. a cell sheet on
+// a sphere) counts per row of bins (fixed y and z) and keeps, per row, the offset of every bin
+// relative to the row's first slot (ext[0] + 1 entries; the last one is the row's count). A
+// rebuild then rewrites the relative offsets of the rows whose members changed bins only,
+// instead of one offset per bin of the whole grid.
+__device__ __forceinline__ int seg_of(const InstanceDesc& in, int key) {
+  if (in.rel_begin < 0 || in.row_mul == 0u) return key;
+  return (int)(__umulhi((unsigned)key, in.row_mul) >> in.row_shift);
+}
+// keys of the bins of local segment g: [g * seg_bins, (g + 1) * seg_bins)
+__device__ __forceinline__ int seg_bins(const InstanceDesc& in) {
+  return in.rel_begin >= 0 ? in.ext[0] : 1;
+}
+
+// The slot ranges of an instance's bins in the current order (see seg_of): the first slot of
+// bin (x, y, z) is base(row) + tab[row * stride + x], row = y + z * ext[1], where a dense grid
+// reads its segment (bin) offsets directly (base 0, stride ext[0]) and a sparse one its
+// row-relative offsets (base = the row's segment offset, stride ext[0] + 1). ROWS: the kernel
+// may meet sparse instances (a batch without any is swept by kernels built with ROWS false,
+// whose lookups are the dense ones only).
+template <bool ROWS>
+struct BinRanges {
+  const int* seg;  // segment offsets of the instance (global slots)
+  const int* tab;  // dense: seg; sparse: the instance's row-relative offsets
+  int stride, e1;
+  bool sparse;
+  __device__ __forceinline__ const int* row_of(int y, int z, int& base) const {
+    const int r = y + z * e1;
+    base = 0;
+    if (ROWS) {
+      // the row's segment offset is a valid entry for a dense instance too (row < bins), so
+      // it is loaded unconditionally and the lookups of several rows can be issued together
+      const int b = seg[r];
+      base = sparse ? b : 0;
+    }
+    return tab + r * stride;
+  }
+  // first slot of bin (x, y, z); x == ext[0] is the end of the row
+  __device__ __forceinline__ int at(int x, int y, int z) const {
+    int base;
+    const int* p = row_of(y, z, base);
+    return base + p[x];
+  }
+  // slots [start, end) of bins x_lo..x_hi of row (y, z)
+  __device__ __forceinline__ void row(int x_lo, int x_hi, int y, int z, int& start, int& end) const {
+    int base;
+    const int* p = row_of(y, z, base);
+    start = base + p[x_lo];
+    end = base + p[x_hi + 1];
+  }
+};
+// seg_off: the segment offsets of all instances, rel: the row-relative offsets of all sparse
+// instances (null when there are none; then ROWS may be false)
+template <bool ROWS>
+__device__ __forceinline__ BinRanges<ROWS> bin_ranges(const InstanceDesc& in, const int* seg_off, const int* rel) {
+  const bool sparse = ROWS && in.rel_begin >= 0;
+  const int* seg = seg_off + in.cbin_begin;
+  return BinRanges<ROWS>{seg, sparse ? rel + in.rel_begin : seg, sparse ? in.ext[0] + 1 : in.ext[0],
+                         in.ext[1], sparse};
+}
+
 // Grid::rebuild's key of a position (grid.hpp:394-395) and its counting-sort segment.
 __device__ __forceinline__ int bin_key(const InstanceDesc& in, float4 p) {
   int c[3];
@@ -174,6 +238,8 @@ struct KeyCount {
   int* tsum;        // incremental: the scan's per-tile sums of cnt, kept current with it
   unsigned long long* movers;  // incremental: bit (s & 63) of word s >> 6 marks a cell that
                                // changed segment; k_link_movers puts it on its arrival list
+  int* dirty;       // incremental, sparse grids: 1 for a row whose members changed bins (its
+                    // relative offsets are rewritten by the rebuild, which clears the flag)
 };
 
 // Step 1 of the rebuild (kernels_grid.hip) for one slot: store the key of the cell's position
@@ -193,8 +259,13 @@ __device__ __forceinline__ void count_key(const InstanceDesc& in, const KeyCount
     if (!cell) return;
     const int key = bin_key(in, p);
     k.keys[s] = key;
-    const int gn = in.cbin_begin + (key >> in.shift);
-    const int go = in.cbin_begin + ((old_key >= 0 ? old_key : k.skey[s]) >> in.shift);
+    const int ko = old_key >= 0 ? old_key : k.skey[s];
+    const int gn = in.cbin_begin + seg_of(in, key);
+    const int go = in.cbin_begin + seg_of(in, ko);
+    if (in.rel_begin >= 0 && key != ko) {
+      k.dirty[gn] = 1;
+      k.dirty[go] = 1;
+    }
     if (gn != go) {
       __hip_atomic_fetch_add(k.cnt + go, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_fetch_add(k.cnt + gn, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -212,7 +283,7 @@ __device__ __forceinline__ void count_key(const InstanceDesc& in, const KeyCount
   if (cell) {
     const int key = bin_key(in, p);
     k.keys[s] = key;
-    g = in.cbin_begin + (key >> in.shift);
+    g = in.cbin_begin + seg_of(in, key);
   } else if (pad) {
     // the pad segment holds the instance's pad slots, so the global scan lands on
     // slot_begin of the next instance

@@ -52,7 +52,8 @@ struct SweepArgs {
   const float4* xs;  // positions of the evaluated stage
   const float2* ps;  // polarities of the evaluated stage (may be null)
   const float4* trig;  // polar_trig of ps (kept current by every writer of a polarity buffer)
-  const int* off;    // concatenated bin offsets (global slots)
+  const int* off;    // segment offsets of all instances (global slots; bin_ranges)
+  const int* rel;    // row-relative bin offsets of the sparse instances (bin_ranges)
   const int* id;     // slot -> agent index
   float4* o_pos;     // Delta: F; Pred: d1; Euler: new positions
   float2* o_pol;
@@ -132,8 +133,6 @@ void launch_com_finalize(const InstanceDesc* inst, int n_inst, const float4* par
 // as n_blocks * team_lanes workgroups; block_pairs and com_partial need
 // n_blocks * kMaxTeamLanes entries.
 constexpr int kMaxTeamLanes = 16;
-// Quads of 4 bins per thread of the per-bin offset fill.
-constexpr int kFillQuads = 4;
 // sym: a symmetric finder (Delta mode only; returns nonzero for another mode)
 int launch_sweep_team(int team_lanes, bool gabriel, bool sym, bool has_type, bool has_pol,
                       int mode, const SweepArgs& a, int n_blocks, hipStream_t stream);
@@ -159,16 +158,17 @@ struct RebuildArgs {
   bool incremental;     // derive the new order from the last one (the moved counts and the
                         // arrival lists of kc) instead of counting and scattering every cell
   const int* coff_old;  // segment offsets of the last rebuild (global slots)
-  int* coff;            // segment offsets being built; the per-bin offsets when every shift is 0
+  int* coff;            // segment offsets being built (the bin offsets of dense grids)
   int* skey_new;        // key of every slot in the new order (the next rebuild's kc.skey)
   int* head_clear;      // arrival heads of the following rebuild, reset to -1 by the scan
-  int* off;             // per-bin offsets (global slots) when some shift > 0
+  int* rel;             // row-relative bin offsets of the sparse instances (seg_of)
+  int* dirty;           // per segment: a sparse row whose relative offsets must be rewritten
   int* scan_bsum;
   int total_cbins;  // segments of all instances (scan length)
-  // per-bin fill (only when some instance counts segments of several bins)
+  // row pass (only when some instance has a sparse grid)
   bool fill;
-  const int* fill_inst;  // fill workgroup -> instance
-  const int* fill_base;  // fill workgroup -> first local 4-bin quad (kFillQuads per thread)
+  const int* fill_inst;  // row workgroup -> instance
+  const int* fill_base;  // row workgroup -> first local row (kBlock rows per workgroup)
   int fill_blocks;
 };
 void launch_rebuild(const RebuildArgs& a, int n_blocks, hipStream_t stream);
@@ -235,7 +235,8 @@ enum QueryKind { kQueryNearest = 0, kQueryRandomInBins = 1, kQueryRandomNeighbou
 struct QueryArgs {
   const InstanceDesc* inst;
   const float4* xs;     // current positions (slot order)
-  const int* off;       // per-bin offsets of the grid
+  const int* off;       // segment offsets of the grid (bin_ranges)
+  const int* rel;       // row-relative bin offsets of a sparse grid (bin_ranges)
   const int* id;        // slot -> agent index
   int n;                // queries
   const float* points;  // n * 3

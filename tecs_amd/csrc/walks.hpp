@@ -9,8 +9,9 @@ namespace walks {
 
 // Gabriel emptiness test of the ball with diameter c*|p_i - p_j| around the midpoint over the
 // members of the (stale) grid bins overlapping its bounding box, binned_gabriel.hpp:133-209.
+template <class BR>
 __device__ __forceinline__ bool gabriel_blocked(const InstanceDesc& in, const float4* __restrict__ xs,
-                                                const int* __restrict__ offb, int s, int m,
+                                                const BR& offb, int s, int m,
                                                 float4 me, const float* disp, float d2) {
   const float mid0 = me.x - disp[0] * 0.5f;
   const float mid1 = me.y - disp[1] * 0.5f;
@@ -26,8 +27,8 @@ __device__ __forceinline__ bool gabriel_blocked(const InstanceDesc& in, const fl
   }
   for (int b2 = mn[2]; b2 <= mx[2]; ++b2) {
     for (int b1 = mn[1]; b1 <= mx[1]; ++b1) {
-      const int start = offb[flatten(in, mn[0], b1, b2)];
-      const int end = offb[flatten(in, mx[0], b1, b2) + 1];
+      const int start = offb.at(mn[0], b1, b2);
+      const int end = offb.at(mx[0] + 1, b1, b2);
       for (int k = start; k < end; ++k) {
         if (k == s || k == m) continue;
         const float4 pk = xs[k];
@@ -73,10 +74,10 @@ __device__ __forceinline__ bool is_mover(const InstanceDesc& in, const int* skey
 // The role candidate m plays in a walk of cell s (skey: the rebuild keys on a stale grid, else
 // null; me_mover: s itself is a mover; SCATTER: the scatter rule, else the two-walk rule). The
 // distance test is shared: d2 of (m, s) is bitwise d2 of (s, m).
-template <bool GABRIEL, bool SCATTER>
+template <bool GABRIEL, bool SCATTER, class BR>
 __device__ __forceinline__ int sym_role(const InstanceDesc& in, const int* skey,
                                         const float4* __restrict__ xs,
-                                        const int* __restrict__ offb, int s, int me_id,
+                                        const BR& offb, int s, int me_id,
                                         bool me_mover, float4 me, int m, float4 pm, int m_id,
                                         int walk = kWalkBoth) {
   const float disp[3] = {me.x - pm.x, me.y - pm.y, me.z - pm.z};
@@ -141,12 +142,12 @@ __device__ __forceinline__ void sym_eval(const SweepArgs& a, const InstanceDesc&
 
 // A serial walk of cell s over the bins within the search radius of bin bc (z, y, x order),
 // evaluating the roles `walk` collects.
-template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, bool SCATTER>
+template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, bool SCATTER, bool ROWS>
 __device__ void sym_walk_serial(const SweepArgs& a, const InstanceDesc& in, int s, int me_id,
                                 bool me_mover, float4 me, const Pol& pme, const int* bc, int walk,
                                 float* acc_pos, float* acc_pol, unsigned& pairs) {
   const float4* __restrict__ xs = a.xs;
-  const int* __restrict__ offb = a.off + in.bin_begin;
+  const BinRanges<ROWS> offb = bin_ranges<ROWS>(in, a.off, a.rel);
   const int R = in.search_radius;
   const int x_lo = max(bc[0] - R, 0);
   const int x_hi = min(bc[0] + R, in.ext[0] - 1);
@@ -156,8 +157,8 @@ __device__ void sym_walk_serial(const SweepArgs& a, const InstanceDesc& in, int 
     for (int dy = -R; dy <= R; ++dy) {
       const int y = bc[1] + dy;
       if (y < 0 || y >= in.ext[1]) continue;
-      const int start = offb[flatten(in, x_lo, y, z)];
-      const int end = offb[flatten(in, x_hi, y, z) + 1];
+      const int start = offb.at(x_lo, y, z);
+      const int end = offb.at(x_hi + 1, y, z);
       for (int m = start; m < end; ++m) {
         const int role = sym_role<GABRIEL, SCATTER>(in, a.skey, xs, offb, s, me_id, me_mover, me,
                                                     m, xs[m], a.id[m], walk);
@@ -172,20 +173,20 @@ __device__ void sym_walk_serial(const SweepArgs& a, const InstanceDesc& in, int 
 // One cell of a symmetric sweep walked serially under the sweep's rule: one walk around its
 // bin bc, or, for a mover under the two-walk rule, the own walk around bc and the gather walk
 // around its rebuild bin.
-template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, bool SCATTER>
+template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, bool SCATTER, bool ROWS>
 __device__ void sym_cell_serial(const SweepArgs& a, const InstanceDesc& in, int s, int me_id,
                                 bool me_mover, float4 me, const Pol& pme, const int* bc,
                                 float* acc_pos, float* acc_pol, unsigned& pairs) {
   if (SCATTER || !me_mover) {
-    sym_walk_serial<GABRIEL, HAS_TYPE, HAS_POL, SCATTER>(a, in, s, me_id, me_mover, me, pme, bc,
+    sym_walk_serial<GABRIEL, HAS_TYPE, HAS_POL, SCATTER, ROWS>(a, in, s, me_id, me_mover, me, pme, bc,
                                                          kWalkBoth, acc_pos, acc_pol, pairs);
     return;
   }
-  sym_walk_serial<GABRIEL, HAS_TYPE, HAS_POL, SCATTER>(a, in, s, me_id, true, me, pme, bc,
+  sym_walk_serial<GABRIEL, HAS_TYPE, HAS_POL, SCATTER, ROWS>(a, in, s, me_id, true, me, pme, bc,
                                                        kWalkOwn, acc_pos, acc_pol, pairs);
   const int key = a.skey[s];
   const int sc[3] = {key % in.ext[0], (key / in.ext[0]) % in.ext[1], key / (in.ext[0] * in.ext[1])};
-  sym_walk_serial<GABRIEL, HAS_TYPE, HAS_POL, SCATTER>(a, in, s, me_id, true, me, pme, sc,
+  sym_walk_serial<GABRIEL, HAS_TYPE, HAS_POL, SCATTER, ROWS>(a, in, s, me_id, true, me, pme, sc,
                                                        kWalkGather, acc_pos, acc_pol, pairs);
 }
 
