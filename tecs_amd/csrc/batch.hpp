@@ -520,8 +520,8 @@ struct Batch {
         if (in.rel_begin >= 0) {
           in.rel_begin = (int)rb;
           rb += (((int64_t)(in.cbin_count - 1) * (in.ext[0] + 1) + 63) / 64) * 64;
-          // row workgroups of kBlock rows each
-          for (int q = 0; q < in.cbin_count - 1; q += kBlock) {
+          // row workgroups of kFillRows rows each
+          for (int q = 0; q < in.cbin_count - 1; q += kFillRows) {
             h_fill_inst.push_back(k);
             h_fill_base.push_back(q);
           }

@@ -126,8 +126,8 @@ __device__ __forceinline__ void gather_block_inc(const RebuildArgs& a, int lb) {
 // read the members as the scatter left them (full) or as for_members lists them (incremental:
 // the old range, lanes in parallel, then the arrival list) and do not wait for the gather. The
 // members are counted into an LDS histogram of the row's bins (hist[x + 1] for a member in bin
-// x), whose inclusive scan is rel. A fill workgroup covers kBlock consecutive rows of one
-// instance, 64 per wavefront; after an incremental rebuild only the rows marked dirty by
+// x), whose inclusive scan is rel. A fill workgroup covers kFillRows consecutive rows of one
+// instance, a quarter per wavefront; after an incremental rebuild only the rows marked dirty by
 // count_key are written (and their marks cleared).
 constexpr int kRowHist = 1024;  // longest row (ext[0] + 1 entries) the histogram holds
 
@@ -138,8 +138,9 @@ __device__ __forceinline__ void fill_rows(const RebuildArgs& a, int blk, int* hi
   int* hist = hist_all + w * kRowHist;
   const int rows = in.cbin_count - 1;
   const int e0 = in.ext[0];
-  const int r0 = a.fill_base[blk] + w * 64;
-  bool need = r0 + lane < rows;
+  constexpr int kWaveRows = kFillRows / (kBlock / 64);
+  const int r0 = a.fill_base[blk] + w * kWaveRows;
+  bool need = lane < kWaveRows && r0 + lane < rows;
   if (INC && need) need = a.dirty[in.cbin_begin + r0 + lane] != 0;
   unsigned long long todo = __ballot(need);
   while (todo != 0ull) {
@@ -204,8 +205,8 @@ __global__ __launch_bounds__(kBlock) void k_counts_from_offsets(const int* coff,
 // segment's arrival list. One thread per 64-slot word of marks (the 64 slots are in one
 // workgroup, so one instance); the marks are cleared for the next rebuild. The list order is
 // free: the gather and the fill only count a segment's members.
-__global__ __launch_bounds__(kBlock) void k_link_movers(RebuildArgs a, int n_words) {
-  const int w = (int)blockIdx.x * kBlock + (int)threadIdx.x;
+__device__ __forceinline__ void link_movers_block(const RebuildArgs& a, int n_words, int blk) {
+  const int w = blk * kBlock + (int)threadIdx.x;
   if (w >= n_words) return;
   unsigned long long m = a.kc.movers[w];
   if (m == 0ull) return;
@@ -217,6 +218,10 @@ __global__ __launch_bounds__(kBlock) void k_link_movers(RebuildArgs a, int n_wor
     const int g = in.cbin_begin + seg_of(in, a.kc.keys[s]);
     a.kc.nxt[s] = atomicExch(a.kc.head + g, s);
   }
+}
+
+__global__ __launch_bounds__(kBlock) void k_link_movers(RebuildArgs a, int n_words) {
+  link_movers_block(a, n_words, (int)blockIdx.x);
 }
 
 // Steps 4 and 5 in one launch: workgroups [0, n_gather) gather (XCD-aware order), the rest fill.
@@ -316,11 +321,10 @@ __global__ __launch_bounds__(kBlock) void k_scan_top(int* bsum, int nt, int* tot
 // the tile sums before its own tile (at most 4096 reads from L2 per workgroup).
 constexpr int kScanFusedTiles = kScanFusedTileLimit;
 
-__global__ __launch_bounds__(kBlock) void k_scan_apply_fused(const int* in, int n, const int* bsum,
-                                                             int nt, int* out, int* head_clear) {
+__device__ __forceinline__ void scan_apply_fused_tile(const int* in, int n, const int* bsum, int nt,
+                                                      int* out, int* head_clear, int tile) {
   __shared__ int lds[4];
   __shared__ int lds_p[4];
-  const int tile = (int)blockIdx.x;
   int part = 0;
   for (int i = (int)threadIdx.x; i < tile; i += kBlock) part += bsum[i];
   int prefix;
@@ -340,6 +344,22 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply_fused(const int* in, int 
     run += v[k];
   }
   if (tile == nt - 1 && threadIdx.x == 0) out[n] = prefix + total;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_apply_fused(const int* in, int n, const int* bsum,
+                                                             int nt, int* out, int* head_clear) {
+  scan_apply_fused_tile(in, n, bsum, nt, out, head_clear, (int)blockIdx.x);
+}
+
+// An incremental rebuild's scan (its tiles' sums are current) and the linking of its movers in
+// one launch: the two touch different arrays (the scan writes the new segment offsets and
+// clears the next rebuild's arrival heads, the movers join this rebuild's lists).
+__global__ __launch_bounds__(kBlock) void k_scan_link(RebuildArgs a, int nt, int n_words) {
+  const int p = (int)blockIdx.x;
+  if (p < nt)
+    scan_apply_fused_tile(a.kc.cnt, a.total_cbins, a.scan_bsum, nt, a.coff, a.head_clear, p);
+  else
+    link_movers_block(a, n_words, p - nt);
 }
 
 __global__ __launch_bounds__(kBlock) void k_scan_apply(const int* in, int n, const int* bsum, int* out,
@@ -511,14 +531,21 @@ void launch_exclusive_scan(const int* in, int n, int* out, int* bsum, int* head_
 
 void launch_rebuild(const RebuildArgs& a, int n_blocks, hipStream_t stream) {
   if (!a.keys_ready) hipLaunchKernelGGL(k_bin_keys, dim3(n_blocks), dim3(kBlock), 0, stream, a);
-  if (a.incremental) {
+  const int nt = scan_tiles_for(a.total_cbins);
+  if (a.incremental && a.kc.tsum != nullptr && nt <= kScanFusedTiles) {
+    // the counts moved with their tiles' sums (count_key): scan and movers in one launch
     const int n_words = n_blocks * (kBlock / 64);
-    hipLaunchKernelGGL(k_link_movers, dim3((n_words + kBlock - 1) / kBlock), dim3(kBlock), 0, stream,
-                       a, n_words);
+    hipLaunchKernelGGL(k_scan_link, dim3(nt + (n_words + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, a,
+                       nt, n_words);
+  } else {
+    if (a.incremental) {
+      const int n_words = n_blocks * (kBlock / 64);
+      hipLaunchKernelGGL(k_link_movers, dim3((n_words + kBlock - 1) / kBlock), dim3(kBlock), 0, stream,
+                         a, n_words);
+    }
+    launch_exclusive_scan(a.kc.cnt, a.total_cbins, a.coff, a.scan_bsum, a.head_clear, stream,
+                          a.incremental && a.kc.tsum != nullptr);
   }
-  // an incremental rebuild's counts moved with their tiles' sums (count_key)
-  launch_exclusive_scan(a.kc.cnt, a.total_cbins, a.coff, a.scan_bsum, a.head_clear, stream,
-                        a.incremental && a.kc.tsum != nullptr);
   const dim3 gf(n_blocks + (a.fill ? a.fill_blocks : 0));
   if (a.incremental) {
     hipLaunchKernelGGL(k_gather_fill<true>, gf, dim3(kBlock), 0, stream, a, n_blocks);

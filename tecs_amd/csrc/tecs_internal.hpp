@@ -168,10 +168,13 @@ struct RebuildArgs {
   // row pass (only when some instance has a sparse grid)
   bool fill;
   const int* fill_inst;  // row workgroup -> instance
-  const int* fill_base;  // row workgroup -> first local row (kBlock rows per workgroup)
+  const int* fill_base;  // row workgroup -> first local row (kFillRows rows per workgroup)
   int fill_blocks;
 };
 void launch_rebuild(const RebuildArgs& a, int n_blocks, hipStream_t stream);
+// Rows of a sparse grid per row workgroup of the rebuild (kFillRows / 4 per wavefront: a
+// wavefront writes its rows one after another, so it gets few of them).
+constexpr int kFillRows = 64;
 int scan_tiles_for(int n);
 // Scans of at most this many tiles take the fused path (and leave the tiles' sums in bsum).
 constexpr int kScanFusedTileLimit = 4096;
