@@ -66,15 +66,52 @@ __device__ __forceinline__ bool before(int ka, int ia, int kb, int ib) {
   return ka < kb || (ka == kb && ia < ib);
 }
 
-__device__ __forceinline__ void move_cell(const RebuildArgs& a, int o, int dst, int id, int key) {
+// The state of one cell, loaded before its new slot is known (the loads overlap the rank
+// count). The sines and cosines of the polarity are not moved but computed again from it
+// (polar_trig, the function every writer of a polarity uses, so they are the same values).
+struct CellMove {
+  float4 x, ov;
+  float2 pol;
+};
+__device__ __forceinline__ CellMove load_cell(const RebuildArgs& a, int o) {
+  CellMove c{};
+  c.x = a.x_old[o];
+  if (a.pol_new) c.pol = a.pol_old[o];
+  if (a.ov_new) c.ov = a.ov_old[o];
+  return c;
+}
+__device__ __forceinline__ void store_cell(const RebuildArgs& a, const CellMove& c, int dst, int id, int key) {
   a.id_new[dst] = id;
   a.skey_new[dst] = key;
-  a.x_new[dst] = a.x_old[o];
+  a.x_new[dst] = c.x;
   if (a.pol_new) {
-    a.pol_new[dst] = a.pol_old[o];
-    a.trig_new[dst] = a.trig_old[o];
+    a.pol_new[dst] = c.pol;
+    a.trig_new[dst] = polar_trig(c.pol);
   }
-  if (a.ov_new) a.ov_new[dst] = a.ov_old[o];
+  if (a.ov_new) a.ov_new[dst] = c.ov;
+}
+
+// #(members u in [s0, s1) with key in [k0, k0 + nb) before (key, id)); four members per
+// iteration, their loads issued together.
+__device__ __forceinline__ int rank_in_range(const int* __restrict__ keys, const int* __restrict__ ids,
+                                             int s0, int s1, int k0, int nb, int key, int id) {
+  int rank = 0;
+  int u = s0;
+  for (; u + 3 < s1; u += 4) {
+    int k[4], i[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      k[q] = keys[u + q];
+      i[q] = ids[u + q];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) rank += ((unsigned)(k[q] - k0) < (unsigned)nb && before(k[q], i[q], key, id)) ? 1 : 0;
+  }
+  for (; u < s1; ++u) {
+    const int ku = keys[u];
+    rank += ((unsigned)(ku - k0) < (unsigned)nb && before(ku, ids[u], key, id)) ? 1 : 0;
+  }
+  return rank;
 }
 
 // Full rebuild, step 4: final slot = segment start + #(segment members before it in (key,
@@ -85,11 +122,12 @@ __device__ __forceinline__ void gather_block(const RebuildArgs& a, int lb) {
   if ((t - in.slot_begin) >= in.n) return;
   const int key = a.tkey[t];
   const int my_id = a.tid[t];
+  const CellMove c = load_cell(a, a.tmp[t]);
   const int g = in.cbin_begin + seg_of(in, key);
   const int start = a.coff[g], end = a.coff[g + 1];
-  int rank = 0;
-  for (int u = start; u < end; ++u) rank += before(a.tkey[u], a.tid[u], key, my_id) ? 1 : 0;
-  move_cell(a, a.tmp[t], start + rank, my_id, key);
+  // every cell of the range is a member: the key filter passes all of them
+  const int rank = rank_in_range(a.tkey, a.tid, start, end, 0, 0x7fffffff, key, my_id);
+  store_cell(a, c, start + rank, my_id, key);
 }
 
 // Incremental rebuild: the members of segment g are the cells of its range at the last
@@ -114,11 +152,14 @@ __device__ __forceinline__ void gather_block_inc(const RebuildArgs& a, int lb) {
   if ((t - in.slot_begin) >= in.n) return;
   const int key = a.kc.keys[t];
   const int my_id = a.id_old[t];
+  const CellMove c = load_cell(a, t);
   const int g = in.cbin_begin + seg_of(in, key);
   const int s0 = a.coff_old[g], s1 = a.coff_old[g + 1], head = a.kc.head[g], dst0 = a.coff[g];
-  int rank = 0;
-  for_members(a, in, g, s0, s1, head, [&](int ku, int iu) { rank += before(ku, iu, key, my_id) ? 1 : 0; });
-  move_cell(a, t, dst0 + rank, my_id, key);
+  // for_members, with the old range counted four members at a time
+  const int nb = seg_bins(in);
+  int rank = rank_in_range(a.kc.keys, a.id_old, s0, s1, (g - in.cbin_begin) * nb, nb, key, my_id);
+  for (int u = head; u >= 0; u = a.kc.nxt[u]) rank += before(a.kc.keys[u], a.id_old[u], key, my_id) ? 1 : 0;
+  store_cell(a, c, dst0 + rank, my_id, key);
 }
 
 // Step 5, one sparse row per wavefront pass: rel[x] = #(members with key < row key + x) for
