@@ -618,3 +618,45 @@ def test_snapshots_equal_the_oracle_state(gpu, oracle_lib, tmp_path, name):
 def _unit(p):
     t, f = p[:, 0].astype(np.float64), p[:, 1].astype(np.float64)
     return np.stack([np.sin(t) * np.cos(f), np.sin(t) * np.sin(f), np.cos(t)], axis=1)
+
+
+def _sparse_row_case(kind, finder):
+    """Sparse grids (8 or more bins per cell) count per row of bins and keep row-relative bin
+    offsets. "long": a slab 1200 bins long in x, so a row holds more entries than the rebuild's
+    row histogram (1024) and takes the counting path; "narrow": one bin in x, so every row is a
+    single bin (ext[0] = 1)."""
+    g = W.rng(7)
+    if kind == "long":
+        n = 2000
+        pos = np.stack([g.uniform(-600.0, 600.0, n), g.uniform(0.0, 2.0, n), g.uniform(0.0, 2.0, n)], 1)
+        lo, hi = (-602.0, -2.0, -2.0), (602.0, 4.0, 4.0)
+    else:
+        n = 1500
+        pos = np.stack([g.uniform(0.2, 0.8, n), g.uniform(0.0, 40.0, n), g.uniform(0.0, 40.0, n)], 1)
+        lo, hi = (0.0, -60.0, -60.0), (1.0, 100.0, 100.0)
+    pos = pos.astype(np.float32)
+    d = abi.make_desc(n, pair_finder=finder, min_bounds=lo, max_bounds=hi, integrator=abi.HEUN)
+    return W.Workload("sparse-" + kind, d, pos, None, None, 0.05, 30)
+
+
+@pytest.mark.parametrize("variant", VARIANTS, ids=VARIANT_IDS)
+@pytest.mark.parametrize("finder", [abi.BINNED_ALL_PAIRS, abi.BINNED_GABRIEL], ids=lambda f: abi.FINDER_NAMES[f])
+@pytest.mark.parametrize("kind", ["long", "narrow"])
+def test_sparse_grid_rows(gpu, oracle_lib, kind, finder, variant):
+    """Pairs on the fresh grid, 30 Heun steps (incremental rebuilds rewriting the rows whose
+    members moved) and the pairs on the last rebuild's grid, bitwise the oracle's."""
+    w = _sparse_row_case(kind, finder)
+    e = [int(np.ceil((w.desc.pair_finder_settings.max_bounds[k] - w.desc.pair_finder_settings.min_bounds[k])))
+         for k in range(3)]
+    assert e[0] * e[1] * e[2] >= 8 * w.n, "the case must be a sparse grid"
+    print(f"case sparse rows {kind} finder={abi.FINDER_NAMES[finder]} variant={variant} ext={e}", flush=True)
+    g, o = _pair(oracle_lib, w.desc, w, variant=variant)
+    pg = g.pairs(True)
+    assert pg[1].size > 0
+    _assert_same_csr(pg, o.pairs(True), f"sparse rows {kind}, fresh grid")
+    for s in (g, o):
+        s.take_step(w.dt, w.steps)
+    xg, _, _ = g.download()
+    xo, _, _ = o.download()
+    assert np.array_equal(xg.view(np.uint32), xo.view(np.uint32)), "trajectory differs"
+    _assert_same_csr(g.pairs(False), o.pairs(False), f"sparse rows {kind}, stale grid")
