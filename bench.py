@@ -739,7 +739,10 @@ def bench_sweep(args):
     traffic = measured_traffic("sweep", variant, dom)
     valu = measured_valu("sweep", variant, dom)
     autotune = b.autotune_ms()
+    member_class, class_variants = b.size_classes()
     b.close()
+    print(f"bench: size classes {dict(sorted((c, member_class.count(c)) for c in set(member_class)))} "
+          f"variants {class_variants}", file=sys.stderr, flush=True)
     print(f"bench: sweep full {per['full'][0]*1e3:.1f} us, stale {per['stale'][0]*1e3:.1f} us, "
           f"rebuild {reb_ms/max(reb_n,1)*1e3:.1f} us per launch", file=sys.stderr, flush=True)
 
@@ -812,6 +815,9 @@ def bench_sweep(args):
             "timing_launches": {"full": full_n, "stale": stale_n, "rebuild": reb_n},
             "sweep_variant": variant,
             "autotune_ms": autotune,
+            # the autotuner's variant per size class (cells in [4^c, 4^(c+1))), members per class
+            "size_class_variants": {str(c): v for c, v in sorted(class_variants.items())},
+            "size_class_members": {str(c): member_class.count(c) for c in sorted(set(member_class))},
             # the bound the sweeps meet: VALU issue (counted instructions at the issue peak,
             # against the launch time measured here)
             "valu": None if valu is None else {

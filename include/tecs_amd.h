@@ -390,7 +390,22 @@ int tecs_batch_get_stats(tecs_batch* batch, tecs_stats* stats);
 int tecs_batch_reset_stats(tecs_batch* batch);
 int tecs_batch_set_sweep_variant(tecs_batch* batch, int32_t variant);
 int tecs_batch_get_sweep_variant(tecs_batch* batch, int32_t* variant);
+/* With AUTO, the autotuner of a batch whose members span several size classes (cell counts
+ * in powers of four: class c holds 4^c <= n < 4^(c+1)), a binned finder with exact team
+ * sweeps, ordered pairs and no COM fixer, times every variant on each class's cells alone and
+ * gives each class its fastest; the classes that picked the same variant are swept by one
+ * launch. tecs_batch_get_autotune then reports, per variant, the sum of its class times. */
 int tecs_batch_get_autotune(tecs_batch* batch, double* ms_per_variant /* [6] */);
+/* After the autotuner ran: the size class of every member (member_class[count]) and the
+ * variant each class picked (class_variant[TECS_MAX_SIZE_CLASSES], -1 for a class without
+ * members). No counterpart in the reference. */
+#define TECS_MAX_SIZE_CLASSES 16
+int tecs_batch_get_size_classes(tecs_batch* batch, int32_t* member_class, int32_t* class_variant);
+/* Chooses the variant of every size class instead of the autotuner (and selects AUTO):
+ * class_variant[TECS_MAX_SIZE_CLASSES], read for the classes that have members (THREAD, or
+ * TEAM4/8/16 where the team sweep is exact). Ordered binned finders without a COM fixer. No
+ * counterpart in the reference. */
+int tecs_batch_set_size_class_variants(tecs_batch* batch, const int32_t* class_variant);
 int tecs_batch_set_graph_mode(tecs_batch* batch, int32_t mode);
 int tecs_batch_get_graph_launches(tecs_batch* batch, int64_t* launches);
 int tecs_batch_set_kernel_timing(tecs_batch* batch, int32_t enabled);

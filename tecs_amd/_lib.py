@@ -74,6 +74,8 @@ PROTOTYPES = [
     ("tecs_batch_set_sweep_variant", C.c_int, [_P, _I32]),
     ("tecs_batch_get_sweep_variant", C.c_int, [_P, C.POINTER(C.c_int32)]),
     ("tecs_batch_get_autotune", C.c_int, [_P, C.POINTER(C.c_double)]),
+    ("tecs_batch_get_size_classes", C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    ("tecs_batch_set_size_class_variants", C.c_int, [_P, C.POINTER(C.c_int32)]),
     ("tecs_batch_set_graph_mode", C.c_int, [_P, _I32]),
     ("tecs_batch_get_graph_launches", C.c_int, [_P, _I64P]),
     ("tecs_batch_set_kernel_timing", C.c_int, [_P, _I32]),

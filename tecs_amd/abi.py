@@ -59,6 +59,7 @@ SWEEP_TEAM4 = 3
 SWEEP_TEAM8 = 4
 SWEEP_TEAM16 = 5
 SWEEP_NAMES = {1: "thread", 3: "team4", 4: "team8", 5: "team16"}
+MAX_SIZE_CLASSES = 16  # TECS_MAX_SIZE_CLASSES
 
 GRAPHS_AUTO = 0
 GRAPHS_OFF = 1

@@ -282,6 +282,20 @@ struct Batch {
 
   int variant = TECS_SWEEP_AUTO;  // requested sweep variant
   int tuned = -1;                 // concrete variant picked by the autotuner (AUTO)
+  // Autotuning per size class (AUTO, a binned finder whose team sweep is exact, no COM fixer,
+  // ordered pairs): the members are grouped by cell count in powers of four and each class
+  // gets the variant that sweeps its chunks fastest; the classes that picked the same variant
+  // form a group swept by one launch over its chunks. One group: `tuned`, no chunk lists.
+  static constexpr int kMaxClasses = 16;
+  std::vector<int> member_class;  // size class of every member (filled by the autotuner)
+  int class_variant[kMaxClasses];  // the variant a class picked (-1: no member in it)
+  double class_tune_ms[kMaxClasses][6];
+  struct SweepGroup {
+    int variant;
+    int n_chunks;
+    int* chunks;  // device list of the group's 256-slot chunks
+  };
+  std::vector<SweepGroup> groups;
   bool team_ok = false;           // the team sweep reproduces this batch exactly
   float4* trig[2] = {nullptr, nullptr};  // polar_trig of pol[b] (kept current by every writer)
   float4* trigp = nullptr;                // polar_trig of polp
@@ -857,15 +871,24 @@ struct Batch {
   }
   int partials_per_block() const { return lanes_of(concrete_variant()); }
 
-  int launch_variant(int v, int mode, const SweepArgs& a) {
+  // nb: the chunks the launch covers (a.chunks lists them when nb < n_blocks)
+  int launch_variant(int v, int mode, const SweepArgs& a, int nb = -1) {
+    if (nb < 0) nb = n_blocks;
     if (v == TECS_SWEEP_THREAD) {
-      if (launch_sweep(finder, has_type, has_pol, mode, a, n_blocks, stream) != 0)
+      if (launch_sweep(finder, has_type, has_pol, mode, a, nb, stream) != 0)
         return fail(TECS_ERR_STATE, "sweep mode not available for this pair finder");
     } else if (launch_sweep_team(lanes_of(v), is_gabriel(finder), is_symmetric(finder), has_type,
-                                 has_pol, mode, a, n_blocks, stream) != 0) {
+                                 has_pol, mode, a, nb, stream) != 0) {
       return fail(TECS_ERR_STATE, "sweep mode not available for this pair finder");
     }
     return TECS_OK;
+  }
+
+  // Size class of a member: its cell count in powers of four (1k-4k, 4k-16k, ...).
+  static int size_class_of(int n) {
+    int c = 0;
+    for (int v = std::max(n, 1); v >= 4 && c < kMaxClasses - 1; v >>= 2) ++c;
+    return c;
   }
 
   // The next full evaluation rebuilds (maybe_rebuild), so the sweep that ends this step can
@@ -921,13 +944,22 @@ struct Batch {
   int autotune() {
     if (tuned >= 0) return TECS_OK;
     if (!team_ok) {
+      const std::vector<std::vector<int>> class_chunks = chunks_by_class();
+      for (int c = 0; c < kMaxClasses; ++c) class_variant[c] = class_chunks[c].empty() ? -1 : TECS_SWEEP_THREAD;
       tuned = TECS_SWEEP_THREAD;
       return TECS_OK;
     }
     const int cands[4] = {TECS_SWEEP_TEAM4, TECS_SWEEP_TEAM8, TECS_SWEEP_TEAM16, TECS_SWEEP_THREAD};
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    struct Events {  // destroyed on every return path
+      hipEvent_t e0 = nullptr, e1 = nullptr;
+      ~Events() {
+        if (e0) hipEventDestroy(e0);
+        if (e1) hipEventDestroy(e1);
+      }
+    } ev;
+    HIP_TRY(hipEventCreate(&ev.e0));
+    HIP_TRY(hipEventCreate(&ev.e1));
+    hipEvent_t e0 = ev.e0, e1 = ev.e1;
     double best = 1e30;
     int best_v = TECS_SWEEP_TEAM8;
     int rc = TECS_OK;
@@ -953,29 +985,181 @@ struct Batch {
       t.o_trig = trig_of(pol[xc ^ 1]);
       t.old_vel = track_ov ? ov[xc ^ 1] : nullptr;
     }
-    for (int v : cands) {
+    // the fastest of 5 timed launches of variant v over nb chunks (the first launch warms
+    // the caches)
+    auto time_variant = [&](int v, const SweepArgs& args, int nb) -> float {
       float ms_min = 1e30f;
       for (int rep = 0; rep < 6 && rc == TECS_OK; ++rep) {
         hipEventRecord(e0, stream);
-        rc = launch_variant(v, mode, t);
+        rc = launch_variant(v, mode, args, nb);
         if (rc != TECS_OK) break;
         hipEventRecord(e1, stream);
         hipError_t e = hipEventSynchronize(e1);
         float ms = 0.0f;
         if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
         if (e != hipSuccess) rc = fail(TECS_ERR_DEVICE, std::string("autotune: ") + hipGetErrorString(e));
-        if (rep > 0) ms_min = std::min(ms_min, ms);  // first launch warms caches
+        if (rep > 0) ms_min = std::min(ms_min, ms);
       }
-      tune_ms[v] = ms_min;
-      if (ms_min < best) {
-        best = ms_min;
+      return ms_min;
+    };
+    // size classes of the members, and the chunks of each class
+    const std::vector<std::vector<int>> class_chunks = chunks_by_class();
+    int n_classes = 0;
+    for (int c = 0; c < kMaxClasses; ++c) {
+      class_variant[c] = -1;
+      for (int v = 0; v < 6; ++v) class_tune_ms[c][v] = 0.0;
+      n_classes += class_chunks[c].empty() ? 0 : 1;
+    }
+    const bool per_class = n_classes > 1 && !com_on && !is_symmetric(finder);
+    for (int v = 0; v < 6; ++v) tune_ms[v] = 0.0;
+    // one variant for the whole batch: each timed over all chunks in one launch
+    for (int v : cands) {
+      const float ms = time_variant(v, t, n_blocks);
+      tune_ms[v] = ms;
+      if (ms < best) {
+        best = ms;
         best_v = v;
       }
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    for (int c = 0; c < kMaxClasses; ++c) class_variant[c] = class_chunks[c].empty() ? -1 : best_v;
+    if (per_class && rc == TECS_OK) {
+      // Then class by class (largest first), another variant for the class is kept when the
+      // whole sweep - one launch per group of classes, one after another as the steps run
+      // them - gets faster by more than 2%. Timing a class alone would favour variants that
+      // fill an otherwise idle GPU with its few cells, which they do not in the full sweep.
+      int* d_lists = nullptr;
+      TRY(alloc(&d_lists, (size_t)n_blocks));
+      auto time_assignment = [&](const int* cv) -> float {
+        std::vector<int> all;
+        std::vector<std::pair<int, int>> grp;  // (variant, chunks)
+        for (int v : cands) {
+          const size_t before = all.size();
+          for (int c = 0; c < kMaxClasses; ++c)
+            if (cv[c] == v) all.insert(all.end(), class_chunks[c].begin(), class_chunks[c].end());
+          if (all.size() > before) grp.push_back({v, (int)(all.size() - before)});
+        }
+        if (hipMemcpyAsync(d_lists, all.data(), sizeof(int) * all.size(), hipMemcpyHostToDevice, stream) != hipSuccess) {
+          rc = fail(TECS_ERR_DEVICE, "autotune: list upload");
+          return 1e30f;
+        }
+        float ms_min = 1e30f;
+        for (int rep = 0; rep < 6 && rc == TECS_OK; ++rep) {
+          hipEventRecord(e0, stream);
+          int off = 0;
+          for (const auto& gv : grp) {
+            SweepArgs tg = t;
+            tg.chunks = d_lists + off;
+            rc = launch_variant(gv.first, mode, tg, gv.second);
+            off += gv.second;
+            if (rc != TECS_OK) break;
+          }
+          hipEventRecord(e1, stream);
+          hipError_t e = hipEventSynchronize(e1);
+          float ms = 0.0f;
+          if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+          if (e != hipSuccess) rc = fail(TECS_ERR_DEVICE, std::string("autotune: ") + hipGetErrorString(e));
+          if (rep > 0) ms_min = std::min(ms_min, ms);
+        }
+        return ms_min;
+      };
+      std::vector<int> by_size;
+      for (int c = 0; c < kMaxClasses; ++c)
+        if (!class_chunks[c].empty()) by_size.push_back(c);
+      std::sort(by_size.begin(), by_size.end(),
+                [&](int a, int b) { return class_chunks[a].size() > class_chunks[b].size(); });
+      double current = best;
+      for (int c : by_size) {
+        for (int v : cands) {
+          if (rc != TECS_OK) break;
+          if (v == class_variant[c]) {
+            class_tune_ms[c][v] = current;
+            continue;
+          }
+          int cv[kMaxClasses];
+          for (int q = 0; q < kMaxClasses; ++q) cv[q] = class_variant[q];
+          cv[c] = v;
+          const float ms = time_assignment(cv);
+          class_tune_ms[c][v] = ms;
+          if (ms < current * 0.98) {
+            current = ms;
+            class_variant[c] = v;
+          }
+        }
+      }
+      free_one(d_lists);
+    }
     if (rc != TECS_OK) return rc;
+    TRY(build_groups(class_chunks));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return TECS_OK;
+  }
+
+  // Members by size class: their classes (member_class) and each class's chunks.
+  std::vector<std::vector<int>> chunks_by_class() {
+    member_class.assign(n_inst, 0);
+    std::vector<std::vector<int>> class_chunks(kMaxClasses);
+    for (int k = 0; k < n_inst; ++k) {
+      member_class[k] = size_class_of(h_inst[k].n);
+      for (int b = 0; b < h_inst[k].n_blocks; ++b) class_chunks[member_class[k]].push_back(h_inst[k].block_begin + b);
+    }
+    return class_chunks;
+  }
+
+  // The launch groups of class_variant: the classes that use the same variant, swept by one
+  // launch over a device list of their chunks (one group: no lists). `tuned` becomes the
+  // variant of the largest group.
+  int build_groups(const std::vector<std::vector<int>>& class_chunks) {
+    const int order[4] = {TECS_SWEEP_TEAM4, TECS_SWEEP_TEAM8, TECS_SWEEP_TEAM16, TECS_SWEEP_THREAD};
+    groups.clear();
+    std::vector<std::vector<int>> lists;
+    int largest = -1, best_v = TECS_SWEEP_THREAD;
+    for (int v : order) {
+      std::vector<int> ch;
+      for (int c = 0; c < kMaxClasses; ++c)
+        if (class_variant[c] == v) ch.insert(ch.end(), class_chunks[c].begin(), class_chunks[c].end());
+      if (ch.empty()) continue;
+      if ((int)ch.size() > largest) {
+        largest = (int)ch.size();
+        best_v = v;
+      }
+      groups.push_back(SweepGroup{v, (int)ch.size(), nullptr});
+      lists.push_back(std::move(ch));
+    }
+    if (groups.size() > 1) {
+      for (size_t g = 0; g < groups.size(); ++g) {
+        TRY(alloc(&groups[g].chunks, lists[g].size()));
+        HIP_TRY(hipMemcpyAsync(groups[g].chunks, lists[g].data(), sizeof(int) * lists[g].size(),
+                               hipMemcpyHostToDevice, stream));
+      }
+    } else {
+      groups.clear();
+    }
     tuned = best_v;
+    return TECS_OK;
+  }
+
+  // A variant per size class chosen by the caller (AUTO only): class_variant_in[c] for every
+  // class that has members (THREAD or, when the team sweep is exact here, TEAM4/8/16).
+  int set_class_variants(const int32_t* class_variant_in) {
+    const std::vector<std::vector<int>> class_chunks = chunks_by_class();
+    for (int c = 0; c < kMaxClasses; ++c) {
+      if (class_chunks[c].empty()) continue;
+      const int v = class_variant_in[c];
+      const bool team = v == TECS_SWEEP_TEAM4 || v == TECS_SWEEP_TEAM8 || v == TECS_SWEEP_TEAM16;
+      if (v != TECS_SWEEP_THREAD && !(team && team_ok))
+        return fail(TECS_ERR_INVALID, "class variant must be THREAD or, where exact, TEAM4/8/16");
+    }
+    if (com_fixer == TECS_GLOBAL_COM_FIXER || is_symmetric(finder))
+      return fail(TECS_ERR_UNSUPPORTED, "variants per size class need ordered pairs and no COM fixer");
+    HIP_TRY(hipStreamSynchronize(stream));
+    drop_graphs();
+    for (int c = 0; c < kMaxClasses; ++c) {
+      class_variant[c] = class_chunks[c].empty() ? -1 : class_variant_in[c];
+      for (int v = 0; v < 6; ++v) class_tune_ms[c][v] = 0.0;
+    }
+    variant = TECS_SWEEP_AUTO;
+    TRY(build_groups(class_chunks));
+    HIP_TRY(hipStreamSynchronize(stream));
     return TECS_OK;
   }
 
@@ -1017,7 +1201,17 @@ struct Batch {
       a.ov_in = ov[xc];
     }
     TRY(check_sweep_args(mode, a));
-    TRY(launch_variant(v, mode, a));
+    const bool force_mode = mode == kModeDelta || mode == kModePred || mode == kModeCorr || mode == kModeEuler;
+    if (!groups.empty() && variant == TECS_SWEEP_AUTO && force_mode && v == tuned) {
+      // per size class: one launch per group of classes, each over its chunks
+      for (const SweepGroup& g : groups) {
+        SweepArgs ag = a;
+        ag.chunks = g.chunks;
+        TRY(launch_variant(g.variant, mode, ag, g.n_chunks));
+      }
+    } else {
+      TRY(launch_variant(v, mode, a));
+    }
     HIP_TRY(hipGetLastError());
     TRY(time_end(&p));
     if (mode != kModeCount && mode != kModeEmit) stats.evaluations++;

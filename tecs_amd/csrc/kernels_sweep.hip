@@ -500,7 +500,7 @@ __device__ __forceinline__ void finish(const SweepArgs& a, const InstanceDesc& i
 
 template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE, bool SYM, bool ROWS>
 __device__ __forceinline__ void sweep_binned(const SweepArgs& a) {
-  const int lb = logical_block();
+  const int lb = a.chunks ? a.chunks[logical_block()] : logical_block();
   const int ii = a.block_inst[lb];
   const InstanceDesc& in = a.inst[ii];
   const int s = lb * kBlock + (int)threadIdx.x;
@@ -641,7 +641,7 @@ template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, bool STALE, bool ROWS>
 __global__ __launch_bounds__(kBlock) void k_sym_binned(SweepArgs a) {
   __shared__ int list_all[kDeferMax * kBlock];
   __shared__ unsigned lds_u[4];
-  const int lb = logical_block();
+  const int lb = a.chunks ? a.chunks[logical_block()] : logical_block();
   const InstanceDesc& in = a.inst[a.block_inst[lb]];
   const int s = lb * kBlock + (int)threadIdx.x;
   const bool active = (s - in.slot_begin) < in.n;

@@ -219,7 +219,9 @@ __global__ __launch_bounds__(kBlock, 6) void k_sweep_team(SweepArgs a) {
   const int p = (int)blockIdx.x;
   const int nb = (int)gridDim.x;
   const int q = nb >> 3, r = nb & 7, xcd = p & 7, idx = p >> 3;
-  const int tb = (xcd < r) ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
+  const int tl = (xcd < r) ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
+  // a launch over part of the batch: team-block tl is team-block (tl mod T) of listed chunk tl / T
+  const int tb = a.chunks ? a.chunks[tl / T] * T + tl % T : tl;
   const int team = (int)threadIdx.x / T;
   const int lane = (int)threadIdx.x % T;
   const int s = tb * TB + team;

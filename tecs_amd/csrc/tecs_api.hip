@@ -422,6 +422,24 @@ int tecs_batch_get_autotune(tecs_batch* batch, double* ms_per_variant) {
   return TECS_OK;
 }
 
+int tecs_batch_get_size_classes(tecs_batch* batch, int32_t* member_class, int32_t* class_variant) {
+  GUARD(batch);
+  if (!member_class || !class_variant) return fail(TECS_ERR_INVALID, "null output");
+  Batch& b = batch->b;
+  if (b.member_class.empty()) return fail(TECS_ERR_STATE, "the autotuner has not run");
+  static_assert(Batch::kMaxClasses == TECS_MAX_SIZE_CLASSES, "class count of the ABI");
+  for (int k = 0; k < b.n_inst; ++k) member_class[k] = b.member_class[k];
+  for (int c = 0; c < Batch::kMaxClasses; ++c) class_variant[c] = b.class_variant[c];
+  return TECS_OK;
+}
+
+int tecs_batch_set_size_class_variants(tecs_batch* batch, const int32_t* class_variant) {
+  GUARD(batch);
+  if (!class_variant) return fail(TECS_ERR_INVALID, "null class variants");
+  if (!tecs_amd::is_binned(batch->b.finder)) return fail(TECS_ERR_UNSUPPORTED, "variants per size class need a binned finder");
+  return batch->b.set_class_variants(class_variant);
+}
+
 int tecs_batch_set_graph_mode(tecs_batch* batch, int32_t mode) {
   GUARD(batch);
   if (mode < TECS_GRAPHS_AUTO || mode > TECS_GRAPHS_ON) return fail(TECS_ERR_INVALID, "bad graph mode");

@@ -49,6 +49,9 @@ enum Mode {
 struct SweepArgs {
   const InstanceDesc* inst;
   const int* block_inst;
+  // binned sweeps: the 256-slot chunks this launch covers, one per workgroup (team sweeps: per T
+  // workgroups), when it covers part of the batch (a size class); null: all chunks in order
+  const int* chunks;
   const float4* xs;  // positions of the evaluated stage
   const float2* ps;  // polarities of the evaluated stage (may be null)
   const float4* trig;  // polar_trig of ps (kept current by every writer of a polarity buffer)

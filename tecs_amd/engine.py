@@ -345,6 +345,21 @@ class GpuBatch:
         check(self._lib.tecs_batch_get_autotune(self._h, arr))
         return {abi.SWEEP_NAMES[v]: arr[v] for v in abi.SWEEP_NAMES if arr[v] > 0}
 
+    def size_classes(self):
+        """(size class of every member, {class: variant name}) as the autotuner chose them."""
+        mc = (C.c_int32 * max(1, len(self.descs)))()
+        cv = (C.c_int32 * abi.MAX_SIZE_CLASSES)()
+        check(self._lib.tecs_batch_get_size_classes(self._h, mc, cv))
+        return [int(mc[k]) for k in range(len(self.descs))], {
+            c: abi.SWEEP_NAMES[cv[c]] for c in range(abi.MAX_SIZE_CLASSES) if cv[c] >= 0}
+
+    def set_size_class_variants(self, class_variant: dict):
+        """{size class: variant} for the classes with members (tecs_batch_set_size_class_variants)."""
+        cv = (C.c_int32 * abi.MAX_SIZE_CLASSES)(*([-1] * abi.MAX_SIZE_CLASSES))
+        for c, v in class_variant.items():
+            cv[int(c)] = int(v)
+        check(self._lib.tecs_batch_set_size_class_variants(self._h, cv))
+
     def set_graph_mode(self, mode: int):
         check(self._lib.tecs_batch_set_graph_mode(self._h, int(mode)))
 
