@@ -368,18 +368,19 @@ __device__ __forceinline__ void walk_binned_flat(const SweepArgs& a, const Insta
       }
     }
   }
-  // 3. the accepted candidates after the first kDeferMax, inline and in order
+  // 3. the accepted candidates after the first kDeferMax, inline and in order, resuming the
+  // walk right after the last listed one (its range and offset are in its list entry)
   if (na > kDeferMax) {
-    int k = 0;
-    for (int r = 0; r < nr; ++r) {
+    const int last = (int)L.list[(kDeferMax - 1) * kBlock + lane];
+    int o0 = (last & 255) + 1;
+    for (int r = last >> 8; r < nr; ++r, o0 = 0) {
       const int b = L.start[r * kBlock + lane];
       const int end = b + (int)L.len[r * kBlock + lane];
-      for (int m = b; m < end; ++m) {
+      for (int m = b + o0; m < end; ++m) {
         const float4 pj = xs[m];
         const float disp[3] = {me.x - pj.x, me.y - pj.y, me.z - pj.z};
         const float d2 = len2(disp[0], disp[1], disp[2]);
-        if (d2 >= r2) continue;
-        if (k++ < kDeferMax || m == s) continue;
+        if (d2 >= r2 || m == s) continue;
         if (!HAS_POL) cs.pairs--;  // counted above; visit counts it again
         visit<HAS_TYPE, HAS_POL, MODE, false>(a, in, cs, me, pme, m, pj, disp, d2, L.tab);
       }
