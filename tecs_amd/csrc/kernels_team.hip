@@ -240,6 +240,8 @@ __global__ __launch_bounds__(kBlock, 6) void k_sweep_team(SweepArgs a) {
   int bc[3] = {0, 0, 0};
   int me_id = 0;
   bool mover = false;  // symmetric: this cell left its rebuild bin
+  float4 cin_d1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), cin_x0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float2 cin_e1 = make_float2(0.0f, 0.0f), cin_p0 = make_float2(0.0f, 0.0f);
   if (active) {
     me = a.xs[s];
     if (HAS_POL) {
@@ -253,6 +255,16 @@ __global__ __launch_bounds__(kBlock, 6) void k_sweep_team(SweepArgs a) {
     if (SYM) {
       me_id = a.id[s];
       mover = a.skey != nullptr && flatten(in, bc[0], bc[1], bc[2]) != a.skey[s];
+    }
+    // the corrector's inputs, loaded by the leader now so that their latency hides behind the
+    // sweep rather than stalling the epilogue
+    if (MODE == kModeCorr && lane == 0) {
+      cin_d1 = a.d1_pos[s];
+      cin_x0 = a.o2_pos[s];
+      if (HAS_POL) {
+        cin_e1 = a.d1_pol[s];
+        cin_p0 = a.o2_pol[s];
+      }
     }
     // row rr = (dz + 1) * 3 + (dy + 1), looked up by lane rr mod T
     const int x_lo = max(bc[0] - 1, 0), x_hi = min(bc[0] + 1, in.ext[0] - 1);
@@ -568,15 +580,15 @@ __global__ __launch_bounds__(kBlock, 6) void k_sweep_team(SweepArgs a) {
       }
     } else if (MODE == kModeCorr) {
       // heun.hpp:170-186
-      const float4 d1 = a.d1_pos[s];
+      const float4 d1 = cin_d1;
       const float s0 = d1.x + F[0], s1 = d1.y + F[1], s2 = d1.z + F[2];
       if (a.old_vel) a.old_vel[s] = make_float4(s0 * 0.5f, s1 * 0.5f, s2 * 0.5f, 0.0f);
-      const float4 x0 = a.o2_pos[s];
+      const float4 x0 = cin_x0;
       xnew = make_float4(x0.x + s0 * 0.5f * dt, x0.y + s1 * 0.5f * dt, x0.z + s2 * 0.5f * dt, x0.w);
       a.o2_pos[s] = xnew;
       if (HAS_POL) {
-        const float2 e1 = a.d1_pol[s];
-        const float2 p0 = a.o2_pol[s];
+        const float2 e1 = cin_e1;
+        const float2 p0 = cin_p0;
         const float2 np = make_float2(p0.x + (e1.x + G[0]) * 0.5f * dt,
                                       p0.y + (e1.y + G[1]) * 0.5f * dt);
         a.o2_pol[s] = np;
