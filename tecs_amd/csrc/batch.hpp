@@ -236,6 +236,7 @@ struct Batch {
   float2* d1p = nullptr;
   float4* d3 = nullptr;
   float2* d3p = nullptr;
+  unsigned char* mover8 = nullptr;  // binned symmetric finders: the predictor's mover flags
   float4* q = nullptr;   // binned symmetric finders: contributions scattered by movers (zero
   float2* qp = nullptr;  // between evaluations; the consuming update clears what it adds)
   int* keys = nullptr;
@@ -586,6 +587,7 @@ struct Batch {
     if (is_symmetric(finder) && is_binned(finder)) {
       TRY(alloc(&q, S));
       HIP_TRY(hipMemsetAsync(q, 0, sizeof(float4) * S, stream));
+      TRY(alloc(&mover8, S));
       if (has_pol) {
         TRY(alloc(&qp, S));
         HIP_TRY(hipMemsetAsync(qp, 0, sizeof(float2) * S, stream));
@@ -1222,7 +1224,9 @@ struct Batch {
   // deltas are F. The ordered sweeps wrote per-workgroup COM partials themselves; for the
   // symmetric ones a pass computes them from F (+ q, the movers' contributions, when the
   // evaluation saw a stale grid: with_q), and the update adds q and clears it.
-  int com_and_update(int kind, const float4* F = nullptr, bool with_q = false) {
+  // movers: the predictor flags the cells outside their bin of the last rebuild for the
+  // symmetric evaluation that follows on that stale grid (mover8)
+  int com_and_update(int kind, const float4* F = nullptr, bool with_q = false, bool movers = false) {
     const bool sym = separate_update();
     if (com_fixer == TECS_GLOBAL_COM_FIXER) {
       if (sym) launch_com_partials(d_inst, d_block_inst, F, with_q ? q : nullptr, com_partial, n_blocks, stream);
@@ -1250,6 +1254,10 @@ struct Batch {
     const bool emit = kind != kUpdatePred && emit_keys();
     if (emit) u.kc = key_count();
     if (kind != kUpdatePred) u.nonfinite = nonfinite;
+    if (kind == kUpdatePred && movers && mover8) {
+      u.mover_key = skey[ic];
+      u.mover8 = mover8;
+    }
     launch_update(kind, has_pol, u, n_blocks, stream);
     HIP_TRY(hipGetLastError());
     if (kind != kUpdatePred) keys_ready = emit;
@@ -1312,11 +1320,12 @@ struct Batch {
     TRY(sweep(kModeDelta, a, kTimeSweepFull));
     if (has_extra()) TRY(apply_extra(x[xc], d1, true));
     if (integrator == TECS_HEUN) {
-      TRY(com_and_update(kUpdatePred, d1, stale1));
+      TRY(com_and_update(kUpdatePred, d1, stale1, sym_binned));
       SweepArgs b = base_args(xp, polp);
       b.o_pos = d3;
       b.o_pol = d3p;
       sym_grid_args(&b, sym_binned);
+      if (sym_binned) b.mover8 = mover8;
       TRY(sweep(kModeDelta, b, kTimeSweepStale));
       if (has_extra()) TRY(apply_extra(xp, d3, false));
       TRY(com_and_update(kUpdateCorr, d3, sym_binned));

@@ -686,8 +686,8 @@ __global__ __launch_bounds__(kBlock) void k_sym_binned(SweepArgs a) {
           const float4 p0 = xs[m];
           const float4 p1 = xs[m + 1];
           const int i0 = a.id[m], i1 = a.id[m + 1];
-          const int r0 = sym_role<GABRIEL, false>(in, skey, xs, offb, s, me_id, false, me, m, p0, i0);
-          const int r1 = sym_role<GABRIEL, false>(in, skey, xs, offb, s, me_id, false, me, m + 1, p1, i1);
+          const int r0 = sym_role<GABRIEL, false>(in, skey, a.mover8, xs, offb, s, me_id, false, me, m, p0, i0);
+          const int r1 = sym_role<GABRIEL, false>(in, skey, a.mover8, xs, offb, s, me_id, false, me, m + 1, p1, i1);
           if (r0 != kRoleNone) {
             if (na < kDeferMax) list[na * kBlock] = (int)((unsigned)m | ((unsigned)r0 << 30));
             ++na;
@@ -699,7 +699,7 @@ __global__ __launch_bounds__(kBlock) void k_sym_binned(SweepArgs a) {
         }
         if (m < end) {
           const int r0 =
-              sym_role<GABRIEL, false>(in, skey, xs, offb, s, me_id, false, me, m, xs[m], a.id[m]);
+              sym_role<GABRIEL, false>(in, skey, a.mover8, xs, offb, s, me_id, false, me, m, xs[m], a.id[m]);
           if (r0 != kRoleNone) {
             if (na < kDeferMax) list[na * kBlock] = (int)((unsigned)m | ((unsigned)r0 << 30));
             ++na;
@@ -724,7 +724,7 @@ __global__ __launch_bounds__(kBlock) void k_sym_binned(SweepArgs a) {
         const int end = re[r];
         for (int m = rs[r]; m < end; ++m) {
           const int role =
-              sym_role<GABRIEL, false>(in, skey, xs, offb, s, me_id, false, me, m, xs[m], a.id[m]);
+              sym_role<GABRIEL, false>(in, skey, a.mover8, xs, offb, s, me_id, false, me, m, xs[m], a.id[m]);
           if (role == kRoleNone || k++ < kDeferMax) continue;
           sym_eval<HAS_TYPE, HAS_POL, false>(a, in, me, pme, m, role, acc_pos, acc_pol);
           pairs += role != kRoleGather;
@@ -1004,7 +1004,9 @@ __device__ __forceinline__ void update_cell(const UpdateArgs& a, const InstanceD
       a.d1_pol[s] = make_float2(e.x + ql.x, e.y + ql.y);
     }
     const float4 x = a.x[s];
-    a.xp[s] = make_float4(x.x + d1.x * dt, x.y + d1.y * dt, x.z + d1.z * dt, x.w);
+    const float4 xp = make_float4(x.x + d1.x * dt, x.y + d1.y * dt, x.z + d1.z * dt, x.w);
+    a.xp[s] = xp;
+    if (a.mover8) a.mover8[s] = bin_key(in, xp) != a.mover_key[s] ? 1 : 0;
     if (HAS_POL) {
       const float2 p = a.pol[s], e = a.d1_pol[s];
       const float2 np = make_float2(p.x + e.x * dt, p.y + e.y * dt);

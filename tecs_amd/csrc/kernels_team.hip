@@ -340,7 +340,7 @@ __global__ __launch_bounds__(kBlock, 6) void k_sweep_team(SweepArgs a) {
     int role = kRoleOwn;
     if (SYM && acc) {
       // the scatter rule: a pair with a mover goes through its larger-index cell
-      const bool other_mover = is_mover(in, a.skey, m, pj);
+      const bool other_mover = is_mover(in, a.skey, a.mover8, m, pj);
       if (a.id[m] < me_id)
         role = mover || other_mover ? kRoleOwnScatter : kRoleOwn;
       else

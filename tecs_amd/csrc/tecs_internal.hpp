@@ -67,6 +67,8 @@ struct SweepArgs {
   float4* o_trig;        // Pred / Corr / Euler: polar_trig of the polarity written
   const int* skey;       // symmetric Delta: bin key of every slot at the last rebuild (null: the
                          // grid was built from these positions)
+  const unsigned char* mover8;  // symmetric Delta on a stale grid: 1 for a slot outside its bin
+                                // of the last rebuild (null: computed from skey per candidate)
   float4* q_pos;         // symmetric Delta on a stale grid: the movers' scattered contributions
   float2* q_pol;
   KeyCount kc;           // Corr / Euler: count_key of the new positions for the next rebuild
@@ -124,6 +126,11 @@ struct UpdateArgs {
   KeyCount kc;           // corrector / Euler: the next rebuild's keys of the new positions
                          // (kc.keys null: not counted here)
   int* nonfinite;        // corrector / Euler: 1 for an instance whose new state is not finite
+  // predictor before a symmetric evaluation on a stale grid: mover8[s] = 1 for a cell whose
+  // predicted position lies outside its bin of the last rebuild (keys mover_key), so the sweep
+  // reads one byte per candidate instead of computing its bin
+  const int* mover_key;
+  unsigned char* mover8;
 };
 enum UpdateKind { kUpdatePred = 0, kUpdateCorr = 1, kUpdateEuler = 2 };
 void launch_update(int kind, bool has_pol, const UpdateArgs& a, int n_blocks, hipStream_t stream);

@@ -67,8 +67,11 @@ enum SymRole { kRoleNone = 0, kRoleOwn = 1, kRoleGather = 2, kRoleOwnScatter = 3
 enum SymWalk { kWalkBoth = 3, kWalkOwn = 1, kWalkGather = 2 };
 
 // Slot k sits outside its rebuild bin (skey null: the grid was built from these positions).
-__device__ __forceinline__ bool is_mover(const InstanceDesc& in, const int* skey, int k, float4 p) {
-  return skey != nullptr && bin_key(in, p) != skey[k];
+// mover8: the predictor's flags of these positions, when it wrote them.
+__device__ __forceinline__ bool is_mover(const InstanceDesc& in, const int* skey,
+                                         const unsigned char* mover8, int k, float4 p) {
+  if (skey == nullptr) return false;
+  return mover8 ? mover8[k] != 0 : bin_key(in, p) != skey[k];
 }
 
 // The role candidate m plays in a walk of cell s (skey: the rebuild keys on a stale grid, else
@@ -76,6 +79,7 @@ __device__ __forceinline__ bool is_mover(const InstanceDesc& in, const int* skey
 // distance test is shared: d2 of (m, s) is bitwise d2 of (s, m).
 template <bool GABRIEL, bool SCATTER, class BR>
 __device__ __forceinline__ int sym_role(const InstanceDesc& in, const int* skey,
+                                        const unsigned char* mover8,
                                         const float4* __restrict__ xs,
                                         const BR& offb, int s, int me_id,
                                         bool me_mover, float4 me, int m, float4 pm, int m_id,
@@ -86,10 +90,10 @@ __device__ __forceinline__ int sym_role(const InstanceDesc& in, const int* skey,
   if (m_id < me_id) {
     if (!(walk & kWalkOwn)) return kRoleNone;
     if (GABRIEL && gabriel_blocked(in, xs, offb, s, m, me, disp, d2)) return kRoleNone;
-    if (me_mover || (SCATTER && is_mover(in, skey, m, pm))) return kRoleOwnScatter;
+    if (me_mover || (SCATTER && is_mover(in, skey, mover8, m, pm))) return kRoleOwnScatter;
     return kRoleOwn;
   }
-  if (!(walk & kWalkGather) || (SCATTER && me_mover) || is_mover(in, skey, m, pm))
+  if (!(walk & kWalkGather) || (SCATTER && me_mover) || is_mover(in, skey, mover8, m, pm))
     return kRoleNone;
   if (GABRIEL) {
     const float back[3] = {pm.x - me.x, pm.y - me.y, pm.z - me.z};
@@ -160,7 +164,7 @@ __device__ void sym_walk_serial(const SweepArgs& a, const InstanceDesc& in, int 
       const int start = offb.at(x_lo, y, z);
       const int end = offb.at(x_hi + 1, y, z);
       for (int m = start; m < end; ++m) {
-        const int role = sym_role<GABRIEL, SCATTER>(in, a.skey, xs, offb, s, me_id, me_mover, me,
+        const int role = sym_role<GABRIEL, SCATTER>(in, a.skey, a.mover8, xs, offb, s, me_id, me_mover, me,
                                                     m, xs[m], a.id[m], walk);
         if (role == kRoleNone) continue;
         sym_eval<HAS_TYPE, HAS_POL>(a, in, me, pme, m, role, acc_pos, acc_pol);
