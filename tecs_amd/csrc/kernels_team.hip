@@ -13,7 +13,8 @@
 //      whose bin (of the last rebuild) lies in the ball's bounding box, which is checked with
 //      the candidate's row and x-bin. The result is the reference's blocked flag exactly;
 //   5. forces of the surviving pairs are computed by their lanes and stored in LDS;
-//   6. the team leader sums them in visiting order, so the sum is bitwise the serial one.
+//   6. the cell's force components are summed in visiting order, one component per lane
+//      (x, y, z and the polarity's two), so each sum is bitwise the serial one.
 // A team whose accepted list overflows the LDS capacity (or, defensively, whose Gabriel box
 // leaves the 27 bins) hands the cell to its leader, which walks it serially like the
 // one-thread-per-cell kernel. Requires search_radius == 1 and, for Gabriel,
@@ -311,6 +312,9 @@ __global__ __launch_bounds__(kBlock, 6) void k_sweep_team(SweepArgs a) {
 
   // candidates -> accepted list in LDS, in visiting order
   int base = 0;
+  // pairs this lane's entries count for the reference (ordered, or own), kept when no serial
+  // walk replaces the team's list
+  unsigned lane_pairs = 0;
   const int team_shift = ((int)(threadIdx.x & 63) / T) * T;
   const unsigned team_bits = (T == 32) ? 0xffffffffu : ((1u << T) - 1u);
   const int n_iter = (total + T - 1) / T;
@@ -353,6 +357,7 @@ __global__ __launch_bounds__(kBlock, 6) void k_sweep_team(SweepArgs a) {
     const int pos = base + __popc(tm & ((1u << lane) - 1u));
     if (acc && pos < AM) {
       L.aslot[team][pos] = m;
+      if (!GABRIEL) lane_pairs += role != kRoleGather;
       if (GABRIEL) {
         // stale bin of m relative to (bc - 1): the row gives y and z, the row's bin
         // boundaries give x
@@ -478,7 +483,9 @@ __global__ __launch_bounds__(kBlock, 6) void k_sweep_team(SweepArgs a) {
           apply_force<HAS_TYPE, HAS_POL>(in.force, disp, d, __float_as_int(me.w),
                                          __float_as_int(pj.w), pme, po, cp, cl);
       }
-      // w: 0 no pair, 1 a pair the reference counts (ordered, or own), 2 a gathered pair
+      // w: 0 no pair, 1 a pair the reference counts (ordered, or own), 2 a gathered pair;
+      // a blocked entry keeps zero terms, which the ordered sums add as the identity
+      lane_pairs += !blocked && erole != kRoleGather;
       L.ent[team][e] = make_float4(cp[0], cp[1], cp[2],
                                    blocked ? 0.0f : (erole == kRoleGather ? 2.0f : 1.0f));
       if (HAS_POL) L.cpol[team][e] = make_float2(cl[0], cl[1]);
@@ -495,65 +502,82 @@ __global__ __launch_bounds__(kBlock, 6) void k_sweep_team(SweepArgs a) {
   }
   team_sync();
 
-  // leader: ordered sum (or the serial walk after an overflow / unsafe box)
+  // Ordered sums. The cell's force is NC independent sums (x, y, z and, with polarity, the
+  // two polarity components), each over the list in visiting order with a pair's first term
+  // before its second (kernel_fuser.hpp:40-55). Lane q of the team runs sum q (and q + T when
+  // T < NC), so each add is one of the reference's adds in its order, and the leader collects
+  // the results. Entries without a pair hold zero terms: a running sum that starts at +0 is
+  // never -0, so adding +0 leaves it unchanged, as skipping the entry would. After an
+  // overflow or an unsafe Gabriel box the leader walks the cell serially instead.
+  constexpr int NC = HAS_POL ? 5 : 3;
+  constexpr int NQ = (NC + T - 1) / T;  // sums per lane
   float acc_pos[3] = {0.0f, 0.0f, 0.0f}, acc_pol[2] = {0.0f, 0.0f};
-  unsigned pairs = 0;
   const bool leader = lane == 0 && active;
-  if (leader) {
-    if (SYM && (overflow || L.unsafe[team])) {
+  const bool serial = active && (overflow || L.unsafe[team]);
+  unsigned pairs = active && !serial ? lane_pairs : 0u;
+  if (leader && serial) {
+    if (SYM)
       sym_cell_serial<GABRIEL, HAS_TYPE, HAS_POL, true, ROWS>(a, in, s, me_id, mover, me, pme, bc,
                                                         acc_pos, acc_pol, pairs);
-    } else if (overflow || L.unsafe[team]) {
+    else
       serial_cell<GABRIEL, HAS_TYPE, HAS_POL, MODE, ROWS>(a, in, s, me, pme, acc_pos, acc_pol, pairs);
-    } else {
-      int64_t emit_at = 0;
-      if (MODE == kModeEmit) emit_at = a.row_ptr[a.id[s]];
-      for (int e = 0; e < na; ++e) {
-        const float4 c = L.ent[team][e];
-        if (c.w == 0.0f) continue;
-        pairs += c.w == 1.0f;
-        if (MODE == kModeEmit) {
-          a.cols[emit_at++] = a.id[L.aslot[team][e]];
-          continue;
-        }
-        acc_pos[0] = acc_pos[0] + c.x;
-        acc_pos[1] = acc_pos[1] + c.y;
-        acc_pos[2] = acc_pos[2] + c.z;
-        if (SPLIT) {
-          float c2[3];
-          if (GABRIEL) {
-            c2[0] = L.rows[team].p2[3 * e + 0];
-            c2[1] = L.rows[team].p2[3 * e + 1];
-            c2[2] = L.rows[team].p2[3 * e + 2];
-          } else {
-            const float4 v = L.ent2[team][e];
-            c2[0] = v.x;
-            c2[1] = v.y;
-            c2[2] = v.z;
-          }
-          acc_pos[0] = acc_pos[0] + c2[0];
-          acc_pos[1] = acc_pos[1] + c2[1];
-          acc_pos[2] = acc_pos[2] + c2[2];
-        }
-        if (HAS_POL) {
-          const float2 g = L.cpol[team][e];
-          acc_pol[0] = acc_pol[0] + g.x;
-          acc_pol[1] = acc_pol[1] + g.y;
-        }
-        if (SPLIT) {
-          const float2 g2 = GABRIEL ? make_float2(__int_as_float(L.aslot[team][e]), __int_as_float(L.abit[team][e]))
-                                    : L.cpol2[team][e];
-          acc_pol[0] = acc_pol[0] + g2.x;
-          acc_pol[1] = acc_pol[1] + g2.y;
-        }
-      }
+  }
+  if (MODE == kModeEmit) {
+    if (leader && !serial) {
+      int64_t emit_at = a.row_ptr[a.id[s]];
+      for (int e = 0; e < na; ++e)
+        if (L.ent[team][e].w != 0.0f) a.cols[emit_at++] = a.id[L.aslot[team][e]];
     }
+    return;
   }
   if (MODE == kModeCount) {
+#pragma unroll
+    for (int w = 1; w < T; w <<= 1) pairs += __shfl_xor(pairs, w, 64);
     if (leader) a.row_count[a.id[s]] = (int)pairs;
     return;
   }
-  if (MODE == kModeEmit) return;
+  float lane_sum[NQ];
+#pragma unroll
+  for (int j = 0; j < NQ; ++j) {
+    const int q = lane + j * T;
+    lane_sum[j] = 0.0f;
+    if (active && !serial && q < NC) {
+      // the first term of sum q at entry e is t1[e * s1], the second t2[e * s2]
+      const float* t1 = !HAS_POL || q < 3 ? &L.ent[team][0].x + q : &L.cpol[team][0].x + (q - 3);
+      const int s1 = q < 3 ? 4 : 2;
+      const float* t2 = nullptr;
+      int s2 = 0;
+      if (SPLIT && GABRIEL) {
+        t2 = q < 3 ? &L.rows[team].p2[q]
+                   : reinterpret_cast<const float*>(q == 3 ? &L.aslot[team][0] : &L.abit[team][0]);
+        s2 = q < 3 ? 3 : 1;
+      } else if (SPLIT) {
+        t2 = q < 3 ? &L.ent2[team][0].x + q : &L.cpol2[team][0].x + (q - 3);
+        s2 = q < 3 ? 4 : 2;
+      }
+      float acc = 0.0f;
+      for (int e = 0; e < na; ++e) {
+        acc = acc + t1[e * s1];
+        if (SPLIT) acc = acc + t2[e * s2];
+      }
+      lane_sum[j] = acc;
+    }
+  }
+  {
+    const int base_lane = (int)(threadIdx.x & 63) - lane;
+    float got[NC];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) got[q] = __shfl(lane_sum[q / T], base_lane + q % T, 64);
+    if (leader && !serial) {
+      acc_pos[0] = got[0];
+      acc_pos[1] = got[1];
+      acc_pos[2] = got[2];
+      if (HAS_POL) {
+        acc_pol[0] = got[NC - 2];
+        acc_pol[1] = got[NC - 1];
+      }
+    }
+  }
 
   // out(i) += total, then the +0 drag correction (binned_all_pairs.hpp:174-180)
   float F[3], G[2];
