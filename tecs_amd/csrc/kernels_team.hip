@@ -210,6 +210,11 @@ template <int T, bool GABRIEL, bool HAS_TYPE, bool HAS_POL, int MODE, bool SYM, 
 __global__ __launch_bounds__(kBlock, 6) void k_sweep_team(SweepArgs a) {
   static_assert(!SYM || MODE == kModeDelta, "symmetric team sweeps evaluate forces only");
   constexpr bool SPLIT = HAS_POL && !SYM;
+  // the cell's force components: x, y, z and, with polarity, the polarity's two
+  constexpr int NC = HAS_POL ? 5 : 3;
+  // the epilogue by component (each lane updates the component it summed) where it replaces
+  // the most leader work: the updates that end the step and bin the new position
+  constexpr bool LANE_EPI = T >= NC && (MODE == kModeCorr || MODE == kModeEuler);
   constexpr int TB = TeamLds<T, GABRIEL, HAS_POL, SPLIT>::TB;
   constexpr int AM = TeamLds<T, GABRIEL, HAS_POL, SPLIT>::AM;
   __shared__ TeamLds<T, GABRIEL, HAS_POL, SPLIT> L;
@@ -241,8 +246,11 @@ __global__ __launch_bounds__(kBlock, 6) void k_sweep_team(SweepArgs a) {
   int bc[3] = {0, 0, 0};
   int me_id = 0;
   bool mover = false;  // symmetric: this cell left its rebuild bin
+  // corrector inputs: the leader's, or with the epilogue by component the lane's (d1 / e1 of
+  // its force component, x0 of lanes 0-3 (lane 3: w), p0 of the polarity lanes)
   float4 cin_d1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), cin_x0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   float2 cin_e1 = make_float2(0.0f, 0.0f), cin_p0 = make_float2(0.0f, 0.0f);
+  float cin_q = 0.0f, cin_x = 0.0f, cin_p = 0.0f;
   if (active) {
     me = a.xs[s];
     if (HAS_POL) {
@@ -259,12 +267,22 @@ __global__ __launch_bounds__(kBlock, 6) void k_sweep_team(SweepArgs a) {
     }
     // the corrector's inputs, loaded by the leader now so that their latency hides behind the
     // sweep rather than stalling the epilogue
-    if (MODE == kModeCorr && lane == 0) {
-      cin_d1 = a.d1_pos[s];
-      cin_x0 = a.o2_pos[s];
-      if (HAS_POL) {
-        cin_e1 = a.d1_pol[s];
-        cin_p0 = a.o2_pol[s];
+    if (MODE == kModeCorr) {
+      if constexpr (LANE_EPI) {
+        if (lane < 3)
+          cin_q = reinterpret_cast<const float*>(a.d1_pos + s)[lane];
+        else if (HAS_POL && lane < 5)
+          cin_q = reinterpret_cast<const float*>(a.d1_pol + s)[lane - 3];
+        if (lane < 4) cin_x = reinterpret_cast<const float*>(a.o2_pos + s)[lane];
+        if (HAS_POL && lane >= 3 && lane < 5)
+          cin_p = reinterpret_cast<const float*>(a.o2_pol + s)[lane - 3];
+      } else if (lane == 0) {
+        cin_d1 = a.d1_pos[s];
+        cin_x0 = a.o2_pos[s];
+        if (HAS_POL) {
+          cin_e1 = a.d1_pol[s];
+          cin_p0 = a.o2_pol[s];
+        }
       }
     }
     // row rr = (dz + 1) * 3 + (dy + 1), looked up by lane rr mod T
@@ -509,7 +527,6 @@ __global__ __launch_bounds__(kBlock, 6) void k_sweep_team(SweepArgs a) {
   // the results. Entries without a pair hold zero terms: a running sum that starts at +0 is
   // never -0, so adding +0 leaves it unchanged, as skipping the entry would. After an
   // overflow or an unsafe Gabriel box the leader walks the cell serially instead.
-  constexpr int NC = HAS_POL ? 5 : 3;
   constexpr int NQ = (NC + T - 1) / T;  // sums per lane
   float acc_pos[3] = {0.0f, 0.0f, 0.0f}, acc_pol[2] = {0.0f, 0.0f};
   const bool leader = lane == 0 && active;
@@ -563,8 +580,75 @@ __global__ __launch_bounds__(kBlock, 6) void k_sweep_team(SweepArgs a) {
       lane_sum[j] = acc;
     }
   }
-  {
-    const int base_lane = (int)(threadIdx.x & 63) - lane;
+  const int base_lane = (int)(threadIdx.x & 63) - lane;
+  float F[3] = {0.0f, 0.0f, 0.0f};  // the leader's force sums where the epilogue needs them
+  if constexpr (LANE_EPI) {
+    // Epilogue by component (Corr / Euler): lane q < NC holds sum q and updates that component
+    // of the cell (lanes 0-2 the position, 3-4 the polarity; lane 3 also writes the position's
+    // w), with the same arithmetic the integrators apply per component.
+    float fq = lane_sum[0];
+    if (__ballot(serial) != 0ull) {
+      const float la[5] = {acc_pos[0], acc_pos[1], acc_pos[2], acc_pol[0], acc_pol[1]};
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        const float v = __shfl(la[c], base_lane, 64);
+        if (serial && lane == c) fq = v;
+      }
+    }
+    // out(i) += total, then the +0 drag correction (binned_all_pairs.hpp:174-180)
+    fq = lane < 3 ? (0.0f + fq) + 0.0f : 0.0f + fq;
+    const bool pos_lane = active && lane < 3;
+    const bool w_lane = active && lane == 3;
+    const bool pol_lane = HAS_POL && active && lane >= 3 && lane < 5;
+    const int pc = lane - 3;  // a polarity lane's component
+    const float me_q = lane == 0 ? me.x : lane == 1 ? me.y : lane == 2 ? me.z : me.w;
+    const float mp_c = lane == 3 ? mp.x : mp.y;
+    const float dt = in.dt;
+    float* const pos_out = reinterpret_cast<float*>(MODE == kModeCorr ? a.o2_pos + s : a.o_pos + s);
+    float xq = 0.0f;   // the new position's component (lanes 0-2)
+    float npc = 0.0f;  // the new polarity's component (lanes 3-4)
+    if (MODE == kModeCorr) {
+      // heun.hpp:170-186
+      const float sq = cin_q + fq;
+      if (a.old_vel && (pos_lane || w_lane))
+        reinterpret_cast<float*>(a.old_vel + s)[lane] = pos_lane ? sq * 0.5f : 0.0f;
+      xq = cin_x + sq * 0.5f * dt;
+      if (pos_lane || w_lane) pos_out[lane] = pos_lane ? xq : cin_x;
+      if (pol_lane) {
+        npc = cin_p + (cin_q + fq) * 0.5f * dt;
+        reinterpret_cast<float*>(a.o2_pol + s)[pc] = npc;
+      }
+    } else if (MODE == kModeEuler) {
+      // euler.hpp:52-68
+      if (a.old_vel && (pos_lane || w_lane))
+        reinterpret_cast<float*>(a.old_vel + s)[lane] = pos_lane ? fq : 0.0f;
+      xq = me_q + fq * dt;
+      if (pos_lane || w_lane) pos_out[lane] = pos_lane ? xq : me.w;
+      if (pol_lane) {
+        npc = mp_c + fq * dt;
+        reinterpret_cast<float*>(a.o_pol + s)[pc] = npc;
+      }
+    }
+    if (a.nonfinite && ((pos_lane && !isfinite(xq)) || (pol_lane && !isfinite(npc))))
+      a.nonfinite[ii] = 1;
+    if (HAS_POL && pol_lane) {
+      // polar_trig of the new polarity: each polarity lane its angle's sine and cosine
+      float sv, cv;
+      sincosf(npc, &sv, &cv);
+      reinterpret_cast<float2*>(a.o_trig + s)[pc] = make_float2(sv, cv);
+    }
+    // keys of the new positions for the rebuild that starts the next step: lane d < 3 bins
+    // dimension d (Grid::rebuild's key, grid.hpp:394-395)
+    if (a.kc.keys) {
+      const int d = min(lane, 2);
+      const int cd = bin_coord_div(in, xq, d == 0 ? in.minb[0] : d == 1 ? in.minb[1] : in.minb[2],
+                                   d == 0 ? in.ext[0] : d == 1 ? in.ext[1] : in.ext[2]);
+      const int c1 = __shfl(cd, base_lane + 1, 64), c2 = __shfl(cd, base_lane + 2, 64);
+      count_key_of(in, a.kc, s, leader, lane == 0 && s - in.slot_begin == in.n,
+                   flatten(in, cd, c1, c2));
+    }
+  } else {
+    // the leader collects the sums and updates the cell
     float got[NC];
 #pragma unroll
     for (int q = 0; q < NC; ++q) got[q] = __shfl(lane_sum[q / T], base_lane + q % T, 64);
@@ -577,76 +661,74 @@ __global__ __launch_bounds__(kBlock, 6) void k_sweep_team(SweepArgs a) {
         acc_pol[1] = got[NC - 1];
       }
     }
-  }
-
-  // out(i) += total, then the +0 drag correction (binned_all_pairs.hpp:174-180)
-  float F[3], G[2];
+    // out(i) += total, then the +0 drag correction (binned_all_pairs.hpp:174-180)
+    float G[2];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) F[c] = (0.0f + acc_pos[c]) + 0.0f;
-  G[0] = 0.0f + acc_pol[0];
-  G[1] = 0.0f + acc_pol[1];
-  const float dt = in.dt;
-  float4 xnew = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // Corr / Euler: the new position
-  float2 npol = make_float2(0.0f, 0.0f);               // Pred / Corr / Euler: the new polarity
-  if (leader) {
-    if (MODE == kModeDelta) {
-      a.o_pos[s] = make_float4(F[0], F[1], F[2], 0.0f);
-      if (HAS_POL) a.o_pol[s] = make_float2(G[0], G[1]);
-    } else if (MODE == kModePred) {
-      // heun.hpp:146-156
-      a.o_pos[s] = make_float4(F[0], F[1], F[2], 0.0f);
-      a.o2_pos[s] = make_float4(me.x + F[0] * dt, me.y + F[1] * dt, me.z + F[2] * dt, me.w);
-      if (HAS_POL) {
-        a.o_pol[s] = make_float2(G[0], G[1]);
-        const float2 np = make_float2(mp.x + G[0] * dt, mp.y + G[1] * dt);
-        a.o2_pol[s] = np;
-        npol = np;
-      }
-    } else if (MODE == kModeCorr) {
-      // heun.hpp:170-186
-      const float4 d1 = cin_d1;
-      const float s0 = d1.x + F[0], s1 = d1.y + F[1], s2 = d1.z + F[2];
-      if (a.old_vel) a.old_vel[s] = make_float4(s0 * 0.5f, s1 * 0.5f, s2 * 0.5f, 0.0f);
-      const float4 x0 = cin_x0;
-      xnew = make_float4(x0.x + s0 * 0.5f * dt, x0.y + s1 * 0.5f * dt, x0.z + s2 * 0.5f * dt, x0.w);
-      a.o2_pos[s] = xnew;
-      if (HAS_POL) {
-        const float2 e1 = cin_e1;
-        const float2 p0 = cin_p0;
-        const float2 np = make_float2(p0.x + (e1.x + G[0]) * 0.5f * dt,
-                                      p0.y + (e1.y + G[1]) * 0.5f * dt);
-        a.o2_pol[s] = np;
-        npol = np;
-      }
-    } else if (MODE == kModeEuler) {
-      // euler.hpp:52-68
-      if (a.old_vel) a.old_vel[s] = make_float4(F[0], F[1], F[2], 0.0f);
-      xnew = make_float4(me.x + F[0] * dt, me.y + F[1] * dt, me.z + F[2] * dt, me.w);
-      a.o_pos[s] = xnew;
-      if (HAS_POL) {
-        const float2 np = make_float2(mp.x + G[0] * dt, mp.y + G[1] * dt);
-        a.o_pol[s] = np;
-        npol = np;
+    for (int c = 0; c < 3; ++c) F[c] = (0.0f + acc_pos[c]) + 0.0f;
+    G[0] = 0.0f + acc_pol[0];
+    G[1] = 0.0f + acc_pol[1];
+    const float dt = in.dt;
+    float4 xnew = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // Corr / Euler: the new position
+    float2 npol = make_float2(0.0f, 0.0f);               // Pred / Corr / Euler: the new polarity
+    if (leader) {
+      if (MODE == kModeDelta) {
+        a.o_pos[s] = make_float4(F[0], F[1], F[2], 0.0f);
+        if (HAS_POL) a.o_pol[s] = make_float2(G[0], G[1]);
+      } else if (MODE == kModePred) {
+        // heun.hpp:146-156
+        a.o_pos[s] = make_float4(F[0], F[1], F[2], 0.0f);
+        a.o2_pos[s] = make_float4(me.x + F[0] * dt, me.y + F[1] * dt, me.z + F[2] * dt, me.w);
+        if (HAS_POL) {
+          a.o_pol[s] = make_float2(G[0], G[1]);
+          const float2 np = make_float2(mp.x + G[0] * dt, mp.y + G[1] * dt);
+          a.o2_pol[s] = np;
+          npol = np;
+        }
+      } else if (MODE == kModeCorr) {
+        // heun.hpp:170-186
+        const float4 d1 = cin_d1;
+        const float s0 = d1.x + F[0], s1 = d1.y + F[1], s2 = d1.z + F[2];
+        if (a.old_vel) a.old_vel[s] = make_float4(s0 * 0.5f, s1 * 0.5f, s2 * 0.5f, 0.0f);
+        const float4 x0 = cin_x0;
+        xnew = make_float4(x0.x + s0 * 0.5f * dt, x0.y + s1 * 0.5f * dt, x0.z + s2 * 0.5f * dt, x0.w);
+        a.o2_pos[s] = xnew;
+        if (HAS_POL) {
+          const float2 e1 = cin_e1;
+          const float2 p0 = cin_p0;
+          const float2 np = make_float2(p0.x + (e1.x + G[0]) * 0.5f * dt,
+                                        p0.y + (e1.y + G[1]) * 0.5f * dt);
+          a.o2_pol[s] = np;
+          npol = np;
+        }
+      } else if (MODE == kModeEuler) {
+        // euler.hpp:52-68
+        if (a.old_vel) a.old_vel[s] = make_float4(F[0], F[1], F[2], 0.0f);
+        xnew = make_float4(me.x + F[0] * dt, me.y + F[1] * dt, me.z + F[2] * dt, me.w);
+        a.o_pos[s] = xnew;
+        if (HAS_POL) {
+          const float2 np = make_float2(mp.x + G[0] * dt, mp.y + G[1] * dt);
+          a.o_pol[s] = np;
+          npol = np;
+        }
       }
     }
+    if ((MODE == kModeCorr || MODE == kModeEuler) && leader && a.nonfinite &&
+        not_finite(xnew, npol, HAS_POL))
+      a.nonfinite[ii] = 1;
+    if (HAS_POL && (MODE == kModePred || MODE == kModeCorr || MODE == kModeEuler)) {
+      // polar_trig of the new polarity, split over the team: lane 0 takes theta, lane 1 phi
+      const float th = __shfl(npol.x, base_lane, 64);
+      const float ph = __shfl(npol.y, base_lane, 64);
+      float sv, cv;
+      sincosf(lane == 1 ? ph : th, &sv, &cv);
+      const float sp = __shfl(sv, base_lane + 1, 64);
+      const float cp = __shfl(cv, base_lane + 1, 64);
+      if (leader) a.o_trig[s] = make_float4(sv, cv, sp, cp);
+    }
+    // keys of the new positions for the rebuild that starts the next step
+    if ((MODE == kModeCorr || MODE == kModeEuler) && a.kc.keys)
+      count_key(in, a.kc, s, leader, lane == 0 && s - in.slot_begin == in.n, xnew);
   }
-  if ((MODE == kModeCorr || MODE == kModeEuler) && leader && a.nonfinite &&
-      not_finite(xnew, npol, HAS_POL))
-    a.nonfinite[ii] = 1;
-  if (HAS_POL && (MODE == kModePred || MODE == kModeCorr || MODE == kModeEuler)) {
-    // polar_trig of the new polarity, split over the team: lane 0 takes theta, lane 1 phi
-    const int base_lane = (int)(threadIdx.x & 63) - lane;
-    const float th = __shfl(npol.x, base_lane, 64);
-    const float ph = __shfl(npol.y, base_lane, 64);
-    float sv, cv;
-    sincosf(lane == 1 ? ph : th, &sv, &cv);
-    const float sp = __shfl(sv, base_lane + 1, 64);
-    const float cp = __shfl(cv, base_lane + 1, 64);
-    if (leader) a.o_trig[s] = make_float4(sv, cv, sp, cp);
-  }
-  // keys of the new positions for the rebuild that starts the next step
-  if ((MODE == kModeCorr || MODE == kModeEuler) && a.kc.keys)
-    count_key(in, a.kc, s, leader, lane == 0 && s - in.slot_begin == in.n, xnew);
   if (MODE == kModeDelta && a.com_partial) {
     const float t0 = tblock_sum_f(leader ? F[0] : 0.0f, lds_f);
     const float t1 = tblock_sum_f(leader ? F[1] : 0.0f, lds_f);

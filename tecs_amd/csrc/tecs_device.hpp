@@ -106,6 +106,10 @@ __device__ __forceinline__ void bin_coords_div(const InstanceDesc& in, float x, 
   c[1] = clampi((int)((y - in.minb[1]) / in.bin_size), in.ext[1] - 1);
   c[2] = clampi((int)((z - in.minb[2]) / in.bin_size), in.ext[2] - 1);
 }
+// One coordinate of bin_coords_div: dimension d's position v, minimum mn and extent ext.
+__device__ __forceinline__ int bin_coord_div(const InstanceDesc& in, float v, float mn, int ext) {
+  return clampi((int)(in.bin_pow2 ? (v - mn) * in.inv_bin : (v - mn) / in.bin_size), ext - 1);
+}
 // binned_all_pairs.hpp:103-109: (p - min) * inv_bin, truncation, clamp.
 __device__ __forceinline__ void bin_coords_mul(const InstanceDesc& in, float x, float y, float z,
                                                int* c) {
@@ -253,11 +257,11 @@ struct KeyCount {
 // past the cells.
 // `old_key` is k.skey[s] when the caller has loaded it already (a sweep loads it with the
 // cell's other inputs), else -1.
-__device__ __forceinline__ void count_key(const InstanceDesc& in, const KeyCount& k, int s,
-                                          bool cell, bool pad, float4 p, int old_key = -1) {
+// count_key_of: the same with the cell's key computed by the caller (ignored where !cell).
+__device__ __forceinline__ void count_key_of(const InstanceDesc& in, const KeyCount& k, int s,
+                                             bool cell, bool pad, int key, int old_key = -1) {
   if (k.skey) {
     if (!cell) return;
-    const int key = bin_key(in, p);
     k.keys[s] = key;
     const int ko = old_key >= 0 ? old_key : k.skey[s];
     const int gn = in.cbin_begin + seg_of(in, key);
@@ -281,7 +285,6 @@ __device__ __forceinline__ void count_key(const InstanceDesc& in, const KeyCount
   }
   int g = 0;
   if (cell) {
-    const int key = bin_key(in, p);
     k.keys[s] = key;
     g = in.cbin_begin + seg_of(in, key);
   } else if (pad) {
@@ -291,6 +294,10 @@ __device__ __forceinline__ void count_key(const InstanceDesc& in, const KeyCount
   }
   const LaneRun lr = lane_run(g, cell);
   if (lr.head) __hip_atomic_fetch_add(k.cnt + g, lr.run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void count_key(const InstanceDesc& in, const KeyCount& k, int s,
+                                          bool cell, bool pad, float4 p, int old_key = -1) {
+  count_key_of(in, k, s, cell, pad, cell ? bin_key(in, p) : 0, old_key);
 }
 
 // ------------------------------------------------------------------- vector.hpp:312-343
