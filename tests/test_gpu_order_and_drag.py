@@ -191,23 +191,29 @@ def test_batch_members_with_and_without_drag(gpu, oracle_lib):
         assert np.array_equal(_bits(b.download(k)[0]), _bits(o.download()[0])), k
 
 
+@pytest.mark.parametrize("integrator", [abi.HEUN, abi.EULER], ids=["Heun", "Euler"])
 @pytest.mark.parametrize("variant", [abi.SWEEP_TEAM4, abi.SWEEP_TEAM8, abi.SWEEP_TEAM16], ids=["team4", "team8", "team16"])
-def test_team_sweeps_sum_like_the_thread_sweep(gpu, variant):
+def test_team_sweeps_sum_like_the_thread_sweep(gpu, variant, integrator):
     """The team sweeps add each pair's terms to the running sums in the composition's order,
-    like the thread sweep, so their polarity-model forces and trajectories are bitwise the
-    thread sweep's (both read the same sines and cosines)."""
+    like the thread sweep, and update a cell with the integrator's arithmetic (by component
+    where the team has a lane per component), so their polarity-model forces, trajectories and
+    old velocities are bitwise the thread sweep's (both read the same sines and cosines)."""
     w = W.epithelium(4000)
-    print(f"case epithelium team variant {variant} vs thread", flush=True)
+    d = abi.copy_desc(w.desc)
+    d.integrator = integrator
+    print(f"case epithelium team variant {variant} vs thread, integrator {integrator}", flush=True)
     out = {}
     for v in (abi.SWEEP_THREAD, variant):
-        g = GpuSimulation(w.desc)
+        g = GpuSimulation(d)
         g.set_sweep_variant(v)
         g.upload(w.positions, w.polarities)
         f = g.evaluate_force(True)
         g.take_step(w.dt, 10)
         out[v] = (f, g.download())
         g.close()
-    (fa, pa), (xa, qa, _) = out[abi.SWEEP_THREAD]
-    (fb, pb), (xb, qb, _) = out[variant]
+    (fa, pa), (xa, qa, va) = out[abi.SWEEP_THREAD]
+    (fb, pb), (xb, qb, vb) = out[variant]
     assert np.array_equal(_bits(fa), _bits(fb)) and np.array_equal(_bits(pa), _bits(pb))
     assert np.array_equal(_bits(xa), _bits(xb)) and np.array_equal(_bits(qa), _bits(qb))
+    assert np.array_equal(_bits(va), _bits(vb)), "old velocities"
+    assert np.isfinite(xb).all()

@@ -143,7 +143,8 @@ def test_cancel_a_running_request(gpu):
     assert srv.stats()["cancelled"] == 1
 
 
-@pytest.mark.parametrize("name,variant", [("relaxation", abi.SWEEP_THREAD), ("sorting", abi.SWEEP_TEAM8),
+@pytest.mark.parametrize("name,variant", [("relaxation", abi.SWEEP_THREAD), ("relaxation", abi.SWEEP_TEAM8),
+                                          ("sorting", abi.SWEEP_TEAM8),
                                           ("epithelium", abi.SWEEP_TEAM8), ("epithelium", abi.SWEEP_THREAD)])
 def test_simulation_nonfinite_flag(gpu, name, variant):
     """The flag stays clear on a normal run and is set once an overflow reaches the state, on
