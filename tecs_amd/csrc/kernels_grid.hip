@@ -155,10 +155,17 @@ __device__ __forceinline__ void gather_block_inc(const RebuildArgs& a, int lb) {
   const CellMove c = load_cell(a, t);
   const int g = in.cbin_begin + seg_of(in, key);
   const int s0 = a.coff_old[g], s1 = a.coff_old[g + 1], head = a.kc.head[g], dst0 = a.coff[g];
-  // for_members, with the old range counted four members at a time
-  const int nb = seg_bins(in);
-  int rank = rank_in_range(a.kc.keys, a.id_old, s0, s1, (g - in.cbin_begin) * nb, nb, key, my_id);
-  for (int u = head; u >= 0; u = a.kc.nxt[u]) rank += before(a.kc.keys[u], a.id_old[u], key, my_id) ? 1 : 0;
+  int rank;
+  if (in.rel_begin < 0 && head < 0 && a.coff[g + 1] - dst0 == s1 - s0) {
+    // a bin (dense grid: one segment per bin) with no arrivals and as many members as at the
+    // last rebuild lost none either, and its members' keys are its own: they keep their order
+    rank = t - s0;
+  } else {
+    // for_members, with the old range counted four members at a time
+    const int nb = seg_bins(in);
+    rank = rank_in_range(a.kc.keys, a.id_old, s0, s1, (g - in.cbin_begin) * nb, nb, key, my_id);
+    for (int u = head; u >= 0; u = a.kc.nxt[u]) rank += before(a.kc.keys[u], a.id_old[u], key, my_id) ? 1 : 0;
+  }
   store_cell(a, c, dst0 + rank, my_id, key);
 }
 
