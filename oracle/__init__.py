@@ -252,6 +252,25 @@ def com_fix(deltas, capacity: int) -> np.ndarray:
     return out
 
 
+def force_bar_usage(f32, f64, scale, rel: float = 1e-5) -> dict:
+    """How much of force_excess's bar each term carries: `floor_needed` counts the components
+    whose error exceeds rel * S_i alone (the cell's own contributions), which only the F_rms
+    term admits; `max_own` is the largest |F32 - F64| / (rel * S_i) and `max_floor` the largest
+    |F32 - F64| / (rel * F_rms)."""
+    f64 = np.asarray(f64, np.float64)
+    d = np.abs(np.asarray(f32, np.float64) - f64)
+    own = rel * np.asarray(scale, np.float64)
+    rms = np.sqrt(np.mean(f64 * f64, axis=0, keepdims=True))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        r_own = np.where(own > 0, d / own, np.where(d > 0, np.inf, 0.0))
+        r_floor = np.where(rms > 0, d / (rel * rms), np.where(d > 0, np.inf, 0.0))
+    need = d > own
+    return {"components": int(d.size), "floor_needed": int(need.sum()),
+            "fraction": float(need.mean()) if d.size else 0.0,
+            "max_own": float(r_own.max()) if d.size else 0.0,
+            "max_floor": float(r_floor.max()) if d.size else 0.0}
+
+
 def force_excess(f32, f64, scale, rel: float = 1e-5) -> float:
     """How far FP32 forces exceed the bar |F32 - F64| <= rel * (S_i + F_rms) per component.
 

@@ -142,6 +142,13 @@ def test_forces_polarity_within_1e5_of_fp64(gpu, oracle_lib, finder, variant):
     e2 = _force_scale_check(pg, p64, sp64)
     print(f"excess over 1e-5 bound: pos {e1:.3e} pol {e2:.3e}")
     assert e1 <= 0.0 and e2 <= 0.0
+    # how often the bar's F_rms floor carries it (the cell's own S_i alone covers the rest)
+    import oracle
+
+    for fld, a, b, sc in (("pos", fg, f64, s64), ("pol", pg, p64, sp64)):
+        u = oracle.force_bar_usage(a, b, sc)
+        print(f"bar usage {fld}: {u}")
+        assert u["fraction"] <= 1e-3
 
 
 def _run_both(oracle_lib, w, desc, steps, variant=abi.SWEEP_AUTO):

@@ -134,6 +134,29 @@ def test_fp32_forces_within_1e5_of_fp64(oracle_lib, name):
     assert np.abs(f64.sum(axis=0)).max() <= 1e-9 * s64.sum()
 
 
+@pytest.mark.parametrize("name,max_fraction", [("relaxation", 0.0), ("epithelium", 0.0), ("sorting", 1e-3)])
+def test_force_bar_floor_is_rarely_needed(oracle_lib, name, max_fraction):
+    """The force bar |F32 - F64| <= 1e-5 (S_i + F_rms) has a floor, F_rms, for cells whose
+    pair contributions nearly cancel. This measures how often it carries the bar: never for
+    the relaxation and the epithelium (positions and polarities within 1e-5 of the cell's own
+    S_i), and for at most 0.1% of the sorting tissue's components (cells at the rest length
+    of all their neighbours), which stay within 1e-5 F_rms on their own."""
+    w = {"relaxation": W.relaxation(3000), "epithelium": W.epithelium(4000),
+         "sorting": W.sorting(20000)}[name]
+    s = oracle_lib.OracleSim(w.desc)
+    s.upload(w.positions, w.polarities, w.types)
+    f32, p32 = s.evaluate_force(True)
+    rp, cols = s.pairs(False)
+    f64, p64, s64, sp64 = s.forces_f64(rp, cols)
+    for fld, a, b, sc in (("position", f32, f64, s64), ("polarity", p32, p64, sp64)):
+        if a is None:
+            continue
+        u = oracle_lib.force_bar_usage(a, b, sc)
+        print(f"{name} {fld}: {u}")
+        assert u["fraction"] <= max_fraction
+        assert u["max_floor"] <= 1.0
+
+
 def test_fp64_evaluator_closed_form(oracle_lib):
     """The FP64 evaluator on a two-body system equals the analytic PiecewiseLinear force."""
     u = np.array([0.6, 0.0, 0.8])
