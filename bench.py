@@ -75,14 +75,21 @@ def algorithmic_bytes_per_cell(has_pol: bool, has_type: bool):
     return {"full": 3 * s + a, "stale": 4 * s + a}
 
 
+_ALLOWED_CPUS = []  # the process's CPU count, taken before the OpenMP runtime binds this thread
+
+
 def _cpu_setup():
     """Thread count and binding of the CPU baseline: every core this process may run on
     (OMP_NUM_THREADS when the environment sets it), OMP_PROC_BIND=close, OMP_PLACES=cores,
-    set before the OpenMP runtime of the oracle library starts. Returns (threads, host info)."""
-    try:
-        allowed = len(os.sched_getaffinity(0))
-    except AttributeError:
-        allowed = os.cpu_count() or 1
+    set before the OpenMP runtime of the oracle library starts. Returns (threads, host info).
+    The count is taken once: with OMP_PROC_BIND the runtime pins the calling thread to one core,
+    after which its affinity mask no longer shows the process's CPUs."""
+    if not _ALLOWED_CPUS:
+        try:
+            _ALLOWED_CPUS.append(len(os.sched_getaffinity(0)))
+        except AttributeError:
+            _ALLOWED_CPUS.append(os.cpu_count() or 1)
+    allowed = _ALLOWED_CPUS[0]
     threads = int(os.environ.get("TECS_CPU_THREADS", os.environ.get("OMP_NUM_THREADS", allowed)))
     os.environ.setdefault("OMP_PROC_BIND", "close")
     os.environ.setdefault("OMP_PLACES", "cores")
@@ -115,6 +122,7 @@ def cpu_baseline(w, seconds_target: float = 15.0, warmup: int = 10, min_steps: i
     the per-step rates (pair interactions of the step / its time)."""
     threads, host = _cpu_setup()
     import oracle
+    from tecs_amd import abi
 
     o = oracle.OracleSim(w.desc)
     o.upload(w.positions, w.polarities, w.types)
@@ -135,7 +143,8 @@ def cpu_baseline(w, seconds_target: float = 15.0, warmup: int = 10, min_steps: i
         "unit": "pair-interactions/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"median over {len(rates)} timed Heun steps of the same {w.n}-cell workload after "
+        "sample": f"median over {len(rates)} timed {'Heun' if int(w.desc.integrator) == abi.HEUN else 'Euler'} "
+        f"steps of the same {w.n}-cell workload after "
         f"{warmup} warm-up steps ({el:.1f} s, C++/OpenMP oracle, {threads} threads)",
         **host,
     }
