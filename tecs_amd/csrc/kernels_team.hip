@@ -212,9 +212,9 @@ __global__ __launch_bounds__(kBlock, 6) void k_sweep_team(SweepArgs a) {
   constexpr bool SPLIT = HAS_POL && !SYM;
   // the cell's force components: x, y, z and, with polarity, the polarity's two
   constexpr int NC = HAS_POL ? 5 : 3;
-  // the epilogue by component (each lane updates the component it summed) where it replaces
-  // the most leader work: the updates that end the step and bin the new position
-  constexpr bool LANE_EPI = T >= NC && (MODE == kModeCorr || MODE == kModeEuler);
+  // the epilogue by component (each lane updates the component it summed) for the integrator
+  // modes; Delta, the symmetric sweeps' mode, keeps the leader's (measured faster there)
+  constexpr bool LANE_EPI = T >= NC && (MODE == kModePred || MODE == kModeCorr || MODE == kModeEuler);
   constexpr int TB = TeamLds<T, GABRIEL, HAS_POL, SPLIT>::TB;
   constexpr int AM = TeamLds<T, GABRIEL, HAS_POL, SPLIT>::AM;
   __shared__ TeamLds<T, GABRIEL, HAS_POL, SPLIT> L;
@@ -583,7 +583,7 @@ __global__ __launch_bounds__(kBlock, 6) void k_sweep_team(SweepArgs a) {
   const int base_lane = (int)(threadIdx.x & 63) - lane;
   float F[3] = {0.0f, 0.0f, 0.0f};  // the leader's force sums where the epilogue needs them
   if constexpr (LANE_EPI) {
-    // Epilogue by component (Corr / Euler): lane q < NC holds sum q and updates that component
+    // Epilogue by component (Pred / Corr / Euler): lane q < NC holds sum q and updates that component
     // of the cell (lanes 0-2 the position, 3-4 the polarity; lane 3 also writes the position's
     // w), with the same arithmetic the integrators apply per component.
     float fq = lane_sum[0];
@@ -607,7 +607,18 @@ __global__ __launch_bounds__(kBlock, 6) void k_sweep_team(SweepArgs a) {
     float* const pos_out = reinterpret_cast<float*>(MODE == kModeCorr ? a.o2_pos + s : a.o_pos + s);
     float xq = 0.0f;   // the new position's component (lanes 0-2)
     float npc = 0.0f;  // the new polarity's component (lanes 3-4)
-    if (MODE == kModeCorr) {
+    if (MODE == kModePred) {
+      // heun.hpp:146-156
+      if (pos_lane || w_lane) {
+        pos_out[lane] = pos_lane ? fq : 0.0f;
+        reinterpret_cast<float*>(a.o2_pos + s)[lane] = pos_lane ? me_q + fq * dt : me.w;
+      }
+      if (pol_lane) {
+        reinterpret_cast<float*>(a.o_pol + s)[pc] = fq;
+        npc = mp_c + fq * dt;
+        reinterpret_cast<float*>(a.o2_pol + s)[pc] = npc;
+      }
+    } else if (MODE == kModeCorr) {
       // heun.hpp:170-186
       const float sq = cin_q + fq;
       if (a.old_vel && (pos_lane || w_lane))
@@ -629,7 +640,8 @@ __global__ __launch_bounds__(kBlock, 6) void k_sweep_team(SweepArgs a) {
         reinterpret_cast<float*>(a.o_pol + s)[pc] = npc;
       }
     }
-    if (a.nonfinite && ((pos_lane && !isfinite(xq)) || (pol_lane && !isfinite(npc))))
+    if (MODE != kModePred && a.nonfinite &&
+        ((pos_lane && !isfinite(xq)) || (pol_lane && !isfinite(npc))))
       a.nonfinite[ii] = 1;
     if (HAS_POL && pol_lane) {
       // polar_trig of the new polarity: each polarity lane its angle's sine and cosine
@@ -639,7 +651,7 @@ __global__ __launch_bounds__(kBlock, 6) void k_sweep_team(SweepArgs a) {
     }
     // keys of the new positions for the rebuild that starts the next step: lane d < 3 bins
     // dimension d (Grid::rebuild's key, grid.hpp:394-395)
-    if (a.kc.keys) {
+    if (MODE != kModePred && a.kc.keys) {
       const int d = min(lane, 2);
       const int cd = bin_coord_div(in, xq, d == 0 ? in.minb[0] : d == 1 ? in.minb[1] : in.minb[2],
                                    d == 0 ? in.ext[0] : d == 1 ? in.ext[1] : in.ext[2]);
