@@ -265,8 +265,8 @@ __global__ __launch_bounds__(kBlock, 6) void k_sweep_team(SweepArgs a) {
       me_id = a.id[s];
       mover = a.skey != nullptr && flatten(in, bc[0], bc[1], bc[2]) != a.skey[s];
     }
-    // the corrector's inputs, loaded by the leader now so that their latency hides behind the
-    // sweep rather than stalling the epilogue
+    // the corrector's inputs, loaded now so that their latency hides behind the sweep rather
+    // than stalling the epilogue (by component with LANE_EPI, else by the leader)
     if (MODE == kModeCorr) {
       if constexpr (LANE_EPI) {
         if (lane < 3)
@@ -523,8 +523,9 @@ __global__ __launch_bounds__(kBlock, 6) void k_sweep_team(SweepArgs a) {
   // Ordered sums. The cell's force is NC independent sums (x, y, z and, with polarity, the
   // two polarity components), each over the list in visiting order with a pair's first term
   // before its second (kernel_fuser.hpp:40-55). Lane q of the team runs sum q (and q + T when
-  // T < NC), so each add is one of the reference's adds in its order, and the leader collects
-  // the results. Entries without a pair hold zero terms: a running sum that starts at +0 is
+  // T < NC), so each add is one of the reference's adds in its order; the lane then updates
+  // that component of the cell (LANE_EPI), or the leader collects the sums. Entries without a
+  // pair hold zero terms: a running sum that starts at +0 is
   // never -0, so adding +0 leaves it unchanged, as skipping the entry would. After an
   // overflow or an unsafe Gabriel box the leader walks the cell serially instead.
   constexpr int NQ = (NC + T - 1) / T;  // sums per lane
@@ -583,9 +584,9 @@ __global__ __launch_bounds__(kBlock, 6) void k_sweep_team(SweepArgs a) {
   const int base_lane = (int)(threadIdx.x & 63) - lane;
   float F[3] = {0.0f, 0.0f, 0.0f};  // the leader's force sums where the epilogue needs them
   if constexpr (LANE_EPI) {
-    // Epilogue by component (Pred / Corr / Euler): lane q < NC holds sum q and updates that component
-    // of the cell (lanes 0-2 the position, 3-4 the polarity; lane 3 also writes the position's
-    // w), with the same arithmetic the integrators apply per component.
+    // Epilogue by component (Pred / Corr / Euler): lane q < NC holds sum q and updates that
+    // component of the cell (lanes 0-2 the position, 3-4 the polarity; lane 3 also writes the
+    // position's w), with the same arithmetic the integrators apply per component.
     float fq = lane_sum[0];
     if (__ballot(serial) != 0ull) {
       const float la[5] = {acc_pos[0], acc_pos[1], acc_pos[2], acc_pol[0], acc_pol[1]};
