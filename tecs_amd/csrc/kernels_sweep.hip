@@ -637,7 +637,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_naive(SweepArgs a) {
 // own and gathered contributions (plain stores), q_pos / q_pol the movers' scattered ones.
 // STALE: a.skey is set (a stale grid, movers possible). Search radius 1: the stayers defer
 // their forces as in walk_binned_deferred, list entries carrying the role in bits 30-31; the
-// rare movers walk serially.
+// rare movers are walked by the whole wavefront afterwards (sym_movers_wave).
 template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, bool STALE, bool ROWS>
 __global__ __launch_bounds__(kBlock) void k_sym_binned(SweepArgs a) {
   __shared__ int list_all[kDeferMax * kBlock];
@@ -735,14 +735,21 @@ __global__ __launch_bounds__(kBlock) void k_sym_binned(SweepArgs a) {
     sym_cell_serial<GABRIEL, HAS_TYPE, HAS_POL, false, ROWS>(a, in, s, me_id, false, me, pme, bc,
                                                        acc_pos, acc_pol, pairs);
   }
-  if (STALE && mover)
-    sym_cell_serial<GABRIEL, HAS_TYPE, HAS_POL, false, ROWS>(a, in, s, me_id, true, me, pme, bc, acc_pos,
-                                                       acc_pol, pairs);
+  // the movers: walked by the whole wavefront, one after another (every lane takes part, so
+  // the pairs it counts for them are summed whether or not it holds a cell)
+  unsigned wave_pairs = 0;
+  if (STALE) {
+    if (in.search_radius == 1)
+      wave_pairs = sym_movers_wave<GABRIEL, HAS_TYPE, HAS_POL, ROWS>(a, in, s, mover, acc_pos, acc_pol);
+    else if (mover)
+      sym_cell_serial<GABRIEL, HAS_TYPE, HAS_POL, false, ROWS>(a, in, s, me_id, true, me, pme, bc,
+                                                         acc_pos, acc_pol, pairs);
+  }
   if (active) {
     a.o_pos[s] = make_float4(acc_pos[0], acc_pos[1], acc_pos[2], 0.0f);
     if (HAS_POL) a.o_pol[s] = make_float2(acc_pol[0], acc_pol[1]);
   }
-  const unsigned tp = block_sum_u(active ? pairs : 0u, lds_u);
+  const unsigned tp = block_sum_u((active ? pairs : 0u) + wave_pairs, lds_u);
   if (threadIdx.x == 0) a.block_pairs[lb] += tp;
 }
 

@@ -194,5 +194,81 @@ __device__ void sym_cell_serial(const SweepArgs& a, const InstanceDesc& in, int 
                                                        kWalkGather, acc_pos, acc_pol, pairs);
 }
 
+// The movers of a wavefront on a stale grid, under the two-walk rule (the own walk around the
+// cell's bin, the gather walk around its rebuild bin), one mover after another over all 64
+// lanes instead of each serially in its own lane while the rest of the wavefront waits: a
+// walk's candidates (the 9 rows around the bin) are spread over the lanes, each lane adds the
+// contributions of its candidates (scattering as the serial walk does), and a butterfly over
+// the wavefront sums them into the mover's lane. The symmetric finders are held to the FP64
+// bar, not to an order of summation, and the butterfly's order is fixed, so the result is
+// deterministic. Requires search_radius == 1; every lane of the wavefront calls it (the
+// lanes without a cell as well). Returns the pairs this lane counted.
+template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, bool ROWS>
+__device__ unsigned sym_movers_wave(const SweepArgs& a, const InstanceDesc& in, int s, bool mover,
+                                    float* acc_pos, float* acc_pol) {
+  const int lane = (int)(threadIdx.x & 63);
+  const float4* __restrict__ xs = a.xs;
+  const BinRanges<ROWS> offb = bin_ranges<ROWS>(in, a.off, a.rel);
+  unsigned pairs = 0;
+  unsigned long long mv = __ballot(mover);
+  while (mv != 0ull) {
+    const int src = __ffsll((long long)mv) - 1;
+    mv &= mv - 1ull;
+    const int ms = s - lane + src;  // the mover's slot
+    const float4 mme = xs[ms];
+    Pol mpol{};
+    if (HAS_POL) mpol = make_pol(a.ps[ms], a.trig[ms]);
+    const int mid = a.id[ms];
+    int bc[3];
+    bin_coords_div(in, mme.x, mme.y, mme.z, bc);
+    const int key = a.skey[ms];
+    const int sc[3] = {key % in.ext[0], (key / in.ext[0]) % in.ext[1], key / (in.ext[0] * in.ext[1])};
+    float p[3] = {0.0f, 0.0f, 0.0f}, l[2] = {0.0f, 0.0f};
+#pragma unroll
+    for (int w = 0; w < 2; ++w) {
+      const int cx = w == 0 ? bc[0] : sc[0], cy = w == 0 ? bc[1] : sc[1], cz = w == 0 ? bc[2] : sc[2];
+      const int walk = w == 0 ? kWalkOwn : kWalkGather;
+      const int x_lo = max(cx - 1, 0), x_hi = min(cx + 1, in.ext[0] - 1);
+      int rs[9], pre[10];
+      pre[0] = 0;
+#pragma unroll
+      for (int r = 0; r < 9; ++r) {
+        const int z = cz + r / 3 - 1, y = cy + r % 3 - 1;
+        const bool ok = z >= 0 && z < in.ext[2] && y >= 0 && y < in.ext[1];
+        rs[r] = ok ? offb.at(x_lo, y, z) : 0;
+        pre[r + 1] = pre[r] + (ok ? offb.at(x_hi + 1, y, z) - rs[r] : 0);
+      }
+      for (int k = lane; k < pre[9]; k += 64) {
+        // candidate k of the walk: row r holds candidates pre[r] .. pre[r + 1] - 1
+        int m = 0;
+#pragma unroll
+        for (int r = 0; r < 9; ++r)
+          if (k >= pre[r] && k < pre[r + 1]) m = rs[r] + (k - pre[r]);
+        const int role =
+            sym_role<GABRIEL, false>(in, a.skey, a.mover8, xs, offb, ms, mid, true, mme, m, xs[m], a.id[m], walk);
+        if (role == kRoleNone) continue;
+        sym_eval<HAS_TYPE, HAS_POL>(a, in, mme, mpol, m, role, p, l);
+        pairs += role != kRoleGather;
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) p[c] = p[c] + __shfl_xor(p[c], o, 64);
+      if (HAS_POL) {
+        l[0] = l[0] + __shfl_xor(l[0], o, 64);
+        l[1] = l[1] + __shfl_xor(l[1], o, 64);
+      }
+    }
+    if (lane == src) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) acc_pos[c] += p[c];
+      acc_pol[0] += l[0];
+      acc_pol[1] += l[1];
+    }
+  }
+  return pairs;
+}
+
 }  // namespace walks
 }  // namespace tecs_amd
