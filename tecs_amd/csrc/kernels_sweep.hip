@@ -637,7 +637,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_naive(SweepArgs a) {
 // own and gathered contributions (plain stores), q_pos / q_pol the movers' scattered ones.
 // STALE: a.skey is set (a stale grid, movers possible). Search radius 1: the stayers defer
 // their forces as in walk_binned_deferred, list entries carrying the role in bits 30-31; the
-// rare movers are walked by the whole wavefront afterwards (sym_movers_wave).
+// rare movers are walked by k_sym_movers, a wavefront per mover, after the launch.
 template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, bool STALE, bool ROWS>
 __global__ __launch_bounds__(kBlock) void k_sym_binned(SweepArgs a) {
   __shared__ int list_all[kDeferMax * kBlock];
@@ -735,22 +735,48 @@ __global__ __launch_bounds__(kBlock) void k_sym_binned(SweepArgs a) {
     sym_cell_serial<GABRIEL, HAS_TYPE, HAS_POL, false, ROWS>(a, in, s, me_id, false, me, pme, bc,
                                                        acc_pos, acc_pol, pairs);
   }
-  // the movers: walked by the whole wavefront, one after another (every lane takes part, so
-  // the pairs it counts for them are summed whether or not it holds a cell)
-  unsigned wave_pairs = 0;
-  if (STALE) {
-    if (in.search_radius == 1)
-      wave_pairs = sym_movers_wave<GABRIEL, HAS_TYPE, HAS_POL, ROWS>(a, in, s, mover, acc_pos, acc_pol);
-    else if (mover)
-      sym_cell_serial<GABRIEL, HAS_TYPE, HAS_POL, false, ROWS>(a, in, s, me_id, true, me, pme, bc,
-                                                         acc_pos, acc_pol, pairs);
-  }
+  // the movers: with search radius 1, k_sym_movers walks them after this launch (a mover's
+  // delta is written as zero here and replaced there); otherwise serially in their lanes
+  if (STALE && mover && in.search_radius != 1)
+    sym_cell_serial<GABRIEL, HAS_TYPE, HAS_POL, false, ROWS>(a, in, s, me_id, true, me, pme, bc,
+                                                       acc_pos, acc_pol, pairs);
   if (active) {
     a.o_pos[s] = make_float4(acc_pos[0], acc_pos[1], acc_pos[2], 0.0f);
     if (HAS_POL) a.o_pol[s] = make_float2(acc_pol[0], acc_pol[1]);
   }
-  const unsigned tp = block_sum_u((active ? pairs : 0u) + wave_pairs, lds_u);
+  const unsigned tp = block_sum_u(active ? pairs : 0u, lds_u);
   if (threadIdx.x == 0) a.block_pairs[lb] += tp;
+}
+
+// The movers of a stale-grid symmetric sweep (search radius 1), after k_sym_binned: each
+// wavefront takes the movers among its 64 slots, one after another over all its lanes
+// (sym_movers_wave), and writes their deltas; a wavefront without a mover ends at once. Its
+// own register budget keeps the stayers' sweep at eight wavefronts per SIMD.
+template <bool GABRIEL, bool HAS_TYPE, bool HAS_POL, bool ROWS>
+__global__ __launch_bounds__(kBlock) void k_sym_movers(SweepArgs a) {
+  const int lb = a.chunks ? a.chunks[logical_block()] : logical_block();
+  const InstanceDesc& in = a.inst[a.block_inst[lb]];
+  if (in.search_radius != 1) return;  // k_sym_binned walked them
+  const int s = lb * kBlock + (int)threadIdx.x;
+  const bool active = (s - in.slot_begin) < in.n;
+  bool mover = false;
+  if (active) {
+    const float4 me = a.xs[s];
+    int bc[3];
+    bin_coords_div(in, me.x, me.y, me.z, bc);  // as k_sym_binned decides it
+    mover = flatten(in, bc[0], bc[1], bc[2]) != a.skey[s];
+  }
+  if (__ballot(mover) == 0ull) return;
+  float acc_pos[3] = {0.0f, 0.0f, 0.0f}, acc_pol[2] = {0.0f, 0.0f};
+  const unsigned pairs = sym_movers_wave<GABRIEL, HAS_TYPE, HAS_POL, ROWS>(a, in, s, mover, acc_pos, acc_pol);
+  if (mover) {
+    a.o_pos[s] = make_float4(acc_pos[0], acc_pos[1], acc_pos[2], 0.0f);
+    if (HAS_POL) a.o_pol[s] = make_float2(acc_pol[0], acc_pol[1]);
+  }
+  unsigned tp = pairs;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) tp += __shfl_xor(tp, o, 64);
+  if ((threadIdx.x & 63) == 0 && tp != 0u) atomicAdd(&a.block_pairs[lb], (unsigned long long)tp);
 }
 
 // Naive symmetric finders, Delta mode (naive_all_pairs_symmetric.hpp:100-143,
@@ -879,12 +905,17 @@ int launch_mode(int mode, bool has_type, bool has_pol, const SweepArgs& a, int n
 
 template <bool GABRIEL, bool BINNED, bool HAS_TYPE, bool HAS_POL>
 void launch_sym(const SweepArgs& a, int n_blocks, hipStream_t stream) {
-  if (BINNED && a.skey && a.rel)
+  if (BINNED && a.skey && a.rel) {
     hipLaunchKernelGGL((k_sym_binned<GABRIEL, HAS_TYPE, HAS_POL, true, true>), dim3(n_blocks),
                        dim3(kBlock), 0, stream, a);
-  else if (BINNED && a.skey)
+    hipLaunchKernelGGL((k_sym_movers<GABRIEL, HAS_TYPE, HAS_POL, true>), dim3(n_blocks),
+                       dim3(kBlock), 0, stream, a);
+  } else if (BINNED && a.skey) {
     hipLaunchKernelGGL((k_sym_binned<GABRIEL, HAS_TYPE, HAS_POL, true, false>), dim3(n_blocks),
                        dim3(kBlock), 0, stream, a);
+    hipLaunchKernelGGL((k_sym_movers<GABRIEL, HAS_TYPE, HAS_POL, false>), dim3(n_blocks),
+                       dim3(kBlock), 0, stream, a);
+  }
   else if (BINNED && a.rel)
     hipLaunchKernelGGL((k_sym_binned<GABRIEL, HAS_TYPE, HAS_POL, false, true>), dim3(n_blocks),
                        dim3(kBlock), 0, stream, a);
