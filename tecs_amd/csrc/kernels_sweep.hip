@@ -682,12 +682,17 @@ __global__ __launch_bounds__(kBlock) void k_sym_binned(SweepArgs a) {
         const int end = re[r];
         int m = rs[r];
         for (; m + 1 < end; m += 2) {
-          // positions and agent indices of both candidates are loaded before any test
+          // positions, agent indices and (stale grid) mover flags of both candidates are loaded
+          // before any test
           const float4 p0 = xs[m];
           const float4 p1 = xs[m + 1];
           const int i0 = a.id[m], i1 = a.id[m + 1];
-          const int r0 = sym_role<GABRIEL, false>(in, skey, a.mover8, xs, offb, s, me_id, false, me, m, p0, i0);
-          const int r1 = sym_role<GABRIEL, false>(in, skey, a.mover8, xs, offb, s, me_id, false, me, m + 1, p1, i1);
+          const int f0 = STALE && a.mover8 ? (int)a.mover8[m] : -1;
+          const int f1 = STALE && a.mover8 ? (int)a.mover8[m + 1] : -1;
+          const int r0 = sym_role<GABRIEL, false>(in, skey, a.mover8, xs, offb, s, me_id, false, me, m, p0, i0,
+                                                  kWalkBoth, f0);
+          const int r1 = sym_role<GABRIEL, false>(in, skey, a.mover8, xs, offb, s, me_id, false, me, m + 1, p1, i1,
+                                                  kWalkBoth, f1);
           if (r0 != kRoleNone) {
             if (na < kDeferMax) list[na * kBlock] = (int)((unsigned)m | ((unsigned)r0 << 30));
             ++na;

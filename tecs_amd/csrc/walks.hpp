@@ -76,24 +76,27 @@ __device__ __forceinline__ bool is_mover(const InstanceDesc& in, const int* skey
 
 // The role candidate m plays in a walk of cell s (skey: the rebuild keys on a stale grid, else
 // null; me_mover: s itself is a mover; SCATTER: the scatter rule, else the two-walk rule). The
-// distance test is shared: d2 of (m, s) is bitwise d2 of (s, m).
+// distance test is shared: d2 of (m, s) is bitwise d2 of (s, m). m_mover: whether m is a mover
+// when the caller loaded its flag already (0 or 1; -1: looked up here).
 template <bool GABRIEL, bool SCATTER, class BR>
 __device__ __forceinline__ int sym_role(const InstanceDesc& in, const int* skey,
                                         const unsigned char* mover8,
                                         const float4* __restrict__ xs,
                                         const BR& offb, int s, int me_id,
                                         bool me_mover, float4 me, int m, float4 pm, int m_id,
-                                        int walk = kWalkBoth) {
+                                        int walk = kWalkBoth, int m_mover = -1) {
   const float disp[3] = {me.x - pm.x, me.y - pm.y, me.z - pm.z};
   const float d2 = len2(disp[0], disp[1], disp[2]);
   if (d2 >= in.cutoff2 || m == s) return kRoleNone;
   if (m_id < me_id) {
     if (!(walk & kWalkOwn)) return kRoleNone;
     if (GABRIEL && gabriel_blocked(in, xs, offb, s, m, me, disp, d2)) return kRoleNone;
-    if (me_mover || (SCATTER && is_mover(in, skey, mover8, m, pm))) return kRoleOwnScatter;
+    if (me_mover || (SCATTER && (m_mover >= 0 ? m_mover != 0 : is_mover(in, skey, mover8, m, pm))))
+      return kRoleOwnScatter;
     return kRoleOwn;
   }
-  if (!(walk & kWalkGather) || (SCATTER && me_mover) || is_mover(in, skey, mover8, m, pm))
+  if (!(walk & kWalkGather) || (SCATTER && me_mover) ||
+      (m_mover >= 0 ? m_mover != 0 : is_mover(in, skey, mover8, m, pm)))
     return kRoleNone;
   if (GABRIEL) {
     const float back[3] = {pm.x - me.x, pm.y - me.y, pm.z - me.z};
