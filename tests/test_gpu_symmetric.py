@@ -137,6 +137,31 @@ def test_forces_every_variant(gpu, oracle_lib, finder, name, n, variant, stale):
     assert g.stats()["pair_interactions"] == len(cols)
 
 
+@pytest.mark.parametrize("stale", [False, True], ids=["fresh", "stale"])
+def test_search_radius_two(gpu, oracle_lib, stale):
+    """Bins of half the cutoff (search radius 2): the thread sweep walks every cell serially,
+    the movers of a stale grid included (the wavefront walk of the movers takes radius 1 only
+    and leaves these instances alone)."""
+    w = W.sorting(20000, finder=abi.BINNED_ALL_PAIRS_SYMMETRIC)
+    d = abi.copy_desc(w.desc)
+    d.pair_finder_settings.bin_size_scale = 0.5
+    print(f"case sym search radius 2 stale={stale}", flush=True)
+    g = GpuSimulation(d)
+    o = oracle_lib.OracleSim(d)
+    for s in (g, o):
+        s.upload(w.positions, w.polarities, w.types)
+    if stale:
+        moved = (w.positions + W.rng(79).uniform(-0.3, 0.3, w.positions.shape)).astype(np.float32)
+        for s in (g, o):
+            s.rebuild()
+            s.upload(moved, w.polarities, w.types)
+    fg, _ = g.evaluate_force(not stale)
+    rp, cols = o.pairs(not stale)
+    f64, _, s64, _ = o.forces_f64(rp, cols)
+    assert oracle_lib.force_excess(fg, f64, s64) <= 0.0
+    assert g.stats()["pair_interactions"] == len(cols)
+
+
 def test_half_the_functor_calls(gpu):
     """Fresh grid, all-pairs: the symmetric finder calls the functor once per unordered pair,
     the ordered one twice."""
