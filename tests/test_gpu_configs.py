@@ -9,7 +9,8 @@
     (median |dx| <= 1e-5, p99 <= 1e-3, max <= 0.05, for positions and for the polarities'
     unit vectors);
   * full sizes: the 10,000-cell relaxation over all 1,000 Euler steps bitwise; the 200,000-cell
-    epithelium's Gabriel pair set bit-exact and its forces within the 1e-5 bar of FP64; the
+    epithelium's Gabriel pair set bit-exact and its forces within the 1e-5 bar of FP64, and 3
+    Heun steps of its team sweep bitwise the thread sweep's; the
     4,000,000-cell sorting tissue's pair CSR bit-exact on a fresh and on a stale grid and one
     Heun step bitwise (int32 slot and bin arithmetic at 4M cells and 2.7M bins).
 """
@@ -115,6 +116,28 @@ def test_epithelium_200k_pairs_and_forces(gpu, oracle_lib):
     f64, p64, s64, sp64 = o.forces_f64(ro, co)
     assert oracle_lib.force_excess(fg, f64, s64) <= 0.0
     assert oracle_lib.force_excess(pg, p64, sp64) <= 0.0
+
+
+def test_epithelium_200k_team_steps_equal_the_thread_sweep(gpu):
+    """The 200,000-cell epithelium (a sparse grid) over 3 Heun steps with the 8-lane team
+    sweep, whose sums and integrator updates run one force component per lane, against the
+    thread sweep: positions, polarities and old velocities bitwise (both add each pair's terms
+    in the composition's order and read the same sines and cosines)."""
+    w = W.epithelium(200_000)
+    print("case epithelium 200000 cells: 3 Heun steps, team8 vs thread", flush=True)
+    out = {}
+    for v in (abi.SWEEP_THREAD, abi.SWEEP_TEAM8):
+        g = GpuSimulation(w.desc)
+        g.set_sweep_variant(v)
+        g.upload(w.positions, w.polarities)
+        g.take_step(w.dt, 3)
+        out[v] = g.download()
+        g.close()
+    (xa, pa, va), (xb, pb, vb) = out[abi.SWEEP_THREAD], out[abi.SWEEP_TEAM8]
+    assert np.isfinite(xb).all()
+    assert np.array_equal(_bits(xa), _bits(xb)), "positions"
+    assert np.array_equal(_bits(pa), _bits(pb)), "polarities"
+    assert np.array_equal(_bits(va), _bits(vb)), "old velocities"
 
 
 def test_sorting_4m_pairs_fresh_and_stale_and_one_step(gpu, oracle_lib):
