@@ -335,10 +335,21 @@ __device__ __forceinline__ void walk_binned_flat(const SweepArgs& a, const Insta
     const int row = tn * __float_as_int(me.w);
     const int tmask = tn > 0 ? kMaxTypes * kMaxTypes - 1 : 0;
     if ((f.flags & kForceLinear) != 0) {
-      for (int e = 0; e < rounds; ++e) {
+      // round e's partner slot (a lane past its list: its own slot)
+      auto partner = [&](int e) {
         const int en = (int)L.list[min(e, kDeferMax - 1) * kBlock + lane];
-        const int m = e < nl ? L.start[(en >> 8) * kBlock + lane] + (en & 255) : s;
-        const float4 pj = xs[m];
+        return e < nl ? L.start[(en >> 8) * kBlock + lane] + (en & 255) : s;
+      };
+      // the rounds are software-pipelined: the next round's partner is looked up and its
+      // position loaded while this round's force is computed (the loads, not the arithmetic,
+      // bound the rounds)
+      int m_next = partner(0);
+      float4 pj_next = xs[m_next];
+      for (int e = 0; e < rounds; ++e) {
+        const int m = m_next;
+        const float4 pj = pj_next;
+        m_next = partner(e + 1);
+        pj_next = xs[m_next];
         const float kadh = L.tab[(row + (__float_as_int(pj.w) & tmask)) & (kMaxTypes * kMaxTypes - 1)];
         const float disp[3] = {me.x - pj.x, me.y - pj.y, me.z - pj.z};
         const float d = sqrtf(len2(disp[0], disp[1], disp[2]));
