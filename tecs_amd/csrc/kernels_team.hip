@@ -336,22 +336,33 @@ __global__ __launch_bounds__(kBlock, 6) void k_sweep_team(SweepArgs a) {
   const int team_shift = ((int)(threadIdx.x & 63) / T) * T;
   const unsigned team_bits = (T == 32) ? 0xffffffffu : ((1u << T) - 1u);
   const int n_iter = (total + T - 1) / T;
-  for (int it = 0; it < n_iter; ++it) {
+  // candidate c of iteration it: its row and slot (-1 past the lane's last candidate); the
+  // row holding c is the last row whose prefix is <= c, i.e. the number of rows 1..8 whose
+  // prefix is <= c (prefixes never decrease; an empty row shares its prefix with the next one)
+  auto candidate = [&](int it, int& rsel) {
     const int c = it * T + lane;
+    rsel = 0;
+    if (c >= total) return -1;
+#pragma unroll
+    for (int rr = 1; rr < 9; ++rr) rsel += c >= row_pre[rr] ? 1 : 0;
+    return L.rows[team].r.base[rsel] + c;
+  };
+  // each iteration loads the next one's candidate position before it works on its own, so
+  // that the load's latency hides behind this iteration's tests and LDS stores
+  int rsel_next = 0;
+  int m_next = candidate(0, rsel_next);
+  float4 pj_next = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (m_next >= 0 && m_next != s) pj_next = a.xs[m_next];
+  for (int it = 0; it < n_iter; ++it) {
     bool acc = false;
-    int m = -1, rr_sel = 0;
-    float4 pj = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const int m = m_next, rr_sel = rsel_next;
+    float4 pj = pj_next;
+    m_next = candidate(it + 1, rsel_next);
+    if (m_next >= 0 && m_next != s) pj_next = a.xs[m_next];
     float disp[3] = {0.0f, 0.0f, 0.0f};
     float d2 = 0.0f;
-    if (c < total) {
-      // the row holding candidate c: the last row whose prefix is <= c, i.e. the number of
-      // rows 1..8 whose prefix is <= c (prefixes never decrease; an empty row shares its
-      // prefix with the next one)
-#pragma unroll
-      for (int rr = 1; rr < 9; ++rr) rr_sel += c >= row_pre[rr] ? 1 : 0;
-      m = L.rows[team].r.base[rr_sel] + c;
+    if (m >= 0) {
       if (m != s) {
-        pj = a.xs[m];
         disp[0] = me.x - pj.x;
         disp[1] = me.y - pj.y;
         disp[2] = me.z - pj.z;
