@@ -291,20 +291,13 @@ __device__ __forceinline__ void walk_binned_flat(const SweepArgs& a, const Insta
     const int* const sp_end = sp + nr * kBlock;
     const float4* q = xs + *sp;
     int n = (int)*lp, o = 0, tag = 0;
+    // each iteration's pair is loaded in the iteration before (the first before the loop), so
+    // its loads are in flight while that iteration tests its own pair
+    float4 q0 = q[0], q1 = q[1];
     while (o < n) {
-      const float4 q0 = q[0];
-      const float4 q1 = q[1];
-      // binned_all_pairs.hpp:140-151: a NaN distance is accepted, as `d2 >= r2` is false
-      const float d0 = len2(me.x - q0.x, me.y - q0.y, me.z - q0.z);
-      const float d1 = len2(me.x - q1.x, me.y - q1.y, me.z - q1.z);
-      if (!(d0 >= r2)) {
-        if (na < kDeferMax) L.list[na * kBlock + lane] = (unsigned short)(tag | o);
-        ++na;
-      }
-      if (o + 1 < n && !(d1 >= r2)) {
-        if (na < kDeferMax) L.list[na * kBlock + lane] = (unsigned short)(tag | (o + 1));
-        ++na;
-      }
+      const float4 c0 = q0, c1 = q1;
+      const int entry = tag | o;      // the list entry of c0 (c1's: entry + 1)
+      const bool has1 = o + 1 < n;
       o += 2;
       q += 2;
       if (o >= n) {
@@ -316,6 +309,21 @@ __device__ __forceinline__ void walk_binned_flat(const SweepArgs& a, const Insta
           n = (int)*lp;
           o = 0;
         }
+      }
+      // past the lane's last candidate it loads its own slot, unused
+      const float4* qn = o < n ? q : xs + s;
+      q0 = qn[0];
+      q1 = qn[1];
+      // binned_all_pairs.hpp:140-151: a NaN distance is accepted, as `d2 >= r2` is false
+      const float d0 = len2(me.x - c0.x, me.y - c0.y, me.z - c0.z);
+      const float d1 = len2(me.x - c1.x, me.y - c1.y, me.z - c1.z);
+      if (!(d0 >= r2)) {
+        if (na < kDeferMax) L.list[na * kBlock + lane] = (unsigned short)entry;
+        ++na;
+      }
+      if (has1 && !(d1 >= r2)) {
+        if (na < kDeferMax) L.list[na * kBlock + lane] = (unsigned short)(entry + 1);
+        ++na;
       }
     }
   }
