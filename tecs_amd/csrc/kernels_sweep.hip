@@ -731,14 +731,26 @@ __global__ __launch_bounds__(kBlock) void k_sym_binned(SweepArgs a) {
         }
       }
     }
-    // stayers have no role that scatters
+    // stayers have no role that scatters; each round loads the next round's entry and partner
+    // position before evaluating its own
     const int nl = min(na, kDeferMax);
+    unsigned ent_next = 0u;
+    float4 pm_next = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (nl > 0) {
+      ent_next = (unsigned)list[0];
+      pm_next = xs[ent_next & 0x3fffffffu];
+    }
     for (int e = 0; __ballot(e < nl) != 0ull; ++e) {
+      const unsigned ent = ent_next;
+      const float4 pm = pm_next;
+      if (e + 1 < nl) {
+        ent_next = (unsigned)list[(e + 1) * kBlock];
+        pm_next = xs[ent_next & 0x3fffffffu];
+      }
       if (e < nl) {
-        const unsigned ent = (unsigned)list[e * kBlock];
         const int role = (int)(ent >> 30);
-        sym_eval<HAS_TYPE, HAS_POL, false>(a, in, me, pme, (int)(ent & 0x3fffffffu), role, acc_pos,
-                                           acc_pol);
+        sym_eval_at<HAS_TYPE, HAS_POL, false>(a, in, me, pme, (int)(ent & 0x3fffffffu), pm, role,
+                                              acc_pos, acc_pol);
         pairs += role != kRoleGather;
       }
     }

@@ -105,46 +105,51 @@ __device__ __forceinline__ int sym_role(const InstanceDesc& in, const int* skey,
   return kRoleGather;
 }
 
-// The functor on a zeroed accumulator pack for the pair of the role: own v(s, m) added (and
-// -v scattered to m's q), gather -v(m, s) added.
+// The functor on a zeroed accumulator pack for the pair of the role, m's position pm already
+// loaded: own v(s, m) added (and -v scattered to m's q), gather -v(m, s) added. A gathered pair
+// is m's own evaluation with the two cells swapped, so both roles run one evaluation on
+// operands selected by the role (a wavefront whose lanes hold both roles does not run the
+// force twice) and the gathered result is negated, exactly, before it is added.
+template <bool HAS_TYPE, bool HAS_POL, bool MAY_SCATTER = true>
+__device__ __forceinline__ void sym_eval_at(const SweepArgs& a, const InstanceDesc& in, float4 me,
+                                            const Pol& pme, int m, float4 pm, int role,
+                                            float* acc_pos, float* acc_pol) {
+  Pol po{};
+  if (HAS_POL) po = make_pol(a.ps[m], a.trig[m]);
+  const bool g = role == kRoleGather;
+  const float4 pa = g ? pm : me, pb = g ? me : pm;
+  Pol qa{}, qb{};
+  if (HAS_POL) {
+    qa = g ? po : pme;
+    qb = g ? pme : po;
+  }
+  float vp[3] = {0.0f, 0.0f, 0.0f}, vl[2] = {0.0f, 0.0f};
+  const float disp[3] = {pa.x - pb.x, pa.y - pb.y, pa.z - pb.z};
+  const float d = sqrtf(len2(disp[0], disp[1], disp[2]));
+  apply_force<HAS_TYPE, HAS_POL>(in.force, disp, d, __float_as_int(pa.w), __float_as_int(pb.w),
+                                 qa, qb, vp, vl);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) acc_pos[c] += g ? -vp[c] : vp[c];
+  acc_pol[0] += g ? -vl[0] : vl[0];
+  acc_pol[1] += g ? -vl[1] : vl[1];
+  if (MAY_SCATTER && role == kRoleOwnScatter) {
+    float* q = &a.q_pos[m].x;
+    unsafeAtomicAdd(q + 0, -vp[0]);
+    unsafeAtomicAdd(q + 1, -vp[1]);
+    unsafeAtomicAdd(q + 2, -vp[2]);
+    if (HAS_POL) {
+      float* r = &a.q_pol[m].x;
+      unsafeAtomicAdd(r + 0, -vl[0]);
+      unsafeAtomicAdd(r + 1, -vl[1]);
+    }
+  }
+}
+
 template <bool HAS_TYPE, bool HAS_POL, bool MAY_SCATTER = true>
 __device__ __forceinline__ void sym_eval(const SweepArgs& a, const InstanceDesc& in, float4 me,
                                          const Pol& pme, int m, int role, float* acc_pos,
                                          float* acc_pol) {
-  const float4 pm = a.xs[m];
-  Pol po{};
-  if (HAS_POL) po = make_pol(a.ps[m], a.trig[m]);
-  float vp[3] = {0.0f, 0.0f, 0.0f}, vl[2] = {0.0f, 0.0f};
-  if (role != kRoleGather) {
-    const float disp[3] = {me.x - pm.x, me.y - pm.y, me.z - pm.z};
-    const float d = sqrtf(len2(disp[0], disp[1], disp[2]));
-    apply_force<HAS_TYPE, HAS_POL>(in.force, disp, d, __float_as_int(me.w), __float_as_int(pm.w),
-                                   pme, po, vp, vl);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) acc_pos[c] += vp[c];
-    acc_pol[0] += vl[0];
-    acc_pol[1] += vl[1];
-    if (MAY_SCATTER && role == kRoleOwnScatter) {
-      float* q = &a.q_pos[m].x;
-      unsafeAtomicAdd(q + 0, -vp[0]);
-      unsafeAtomicAdd(q + 1, -vp[1]);
-      unsafeAtomicAdd(q + 2, -vp[2]);
-      if (HAS_POL) {
-        float* r = &a.q_pol[m].x;
-        unsafeAtomicAdd(r + 0, -vl[0]);
-        unsafeAtomicAdd(r + 1, -vl[1]);
-      }
-    }
-  } else {
-    const float disp[3] = {pm.x - me.x, pm.y - me.y, pm.z - me.z};
-    const float d = sqrtf(len2(disp[0], disp[1], disp[2]));
-    apply_force<HAS_TYPE, HAS_POL>(in.force, disp, d, __float_as_int(pm.w), __float_as_int(me.w),
-                                   po, pme, vp, vl);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) acc_pos[c] += -vp[c];
-    acc_pol[0] += -vl[0];
-    acc_pol[1] += -vl[1];
-  }
+  sym_eval_at<HAS_TYPE, HAS_POL, MAY_SCATTER>(a, in, me, pme, m, a.xs[m], role, acc_pos, acc_pol);
 }
 
 // A serial walk of cell s over the bins within the search radius of bin bc (z, y, x order),
