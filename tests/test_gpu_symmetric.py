@@ -137,6 +137,37 @@ def test_forces_every_variant(gpu, oracle_lib, finder, name, n, variant, stale):
     assert g.stats()["pair_interactions"] == len(cols)
 
 
+@pytest.mark.parametrize("radius", [3.0, 1.5], ids=["110-neighbours", "900-neighbours"])
+@pytest.mark.parametrize("stale", [False, True], ids=["fresh", "stale"])
+def test_dense_cluster_thread_sweep(gpu, oracle_lib, radius, stale):
+    """Dense clusters on the all-pairs symmetric thread sweep: ~110 and ~900 neighbours per
+    cell, far more accepted candidates than the deferred list holds (the rest are evaluated in
+    a second walk), on the grid built from the positions and on a stale grid whose cells moved
+    by up to 0.3 (most of them movers)."""
+    n = 3000
+    pos = W.ball(n, radius, W.rng(22))
+    lo, hi = W.bounds(pos)
+    d = abi.make_desc(n, pair_finder=abi.BINNED_ALL_PAIRS_SYMMETRIC, min_bounds=lo, max_bounds=hi,
+                      integrator=abi.HEUN)
+    print(f"case sym dense cluster radius={radius} stale={stale}", flush=True)
+    g = GpuSimulation(d)
+    g.set_sweep_variant(abi.SWEEP_THREAD)
+    o = oracle_lib.OracleSim(d)
+    for s in (g, o):
+        s.upload(pos, None, None)
+    if stale:
+        moved = (pos + W.rng(23).uniform(-0.3, 0.3, pos.shape)).astype(np.float32)
+        for s in (g, o):
+            s.rebuild()
+            s.upload(moved, None, None)
+    fg, _ = g.evaluate_force(not stale)
+    rp, cols = o.pairs(not stale)
+    f64, _, s64, _ = o.forces_f64(rp, cols)
+    assert oracle_lib.force_excess(fg, f64, s64) <= 0.0
+    assert g.stats()["pair_interactions"] == len(cols)
+    assert abs(fg.astype(np.float64).sum(axis=0)).max() <= 1e-5 * np.abs(fg).sum()
+
+
 @pytest.mark.parametrize("stale", [False, True], ids=["fresh", "stale"])
 def test_search_radius_two(gpu, oracle_lib, stale):
     """Bins of half the cutoff (search radius 2): the thread sweep walks every cell serially,
