@@ -46,9 +46,10 @@ def measured_traffic(workload: str, variant: str, phase: str):
     return None
 
 
-# VALU issue peak: 256 CUs x 4 SIMDs, a wavefront64 FP32 instruction (unpacked) occupies its
-# SIMD for 4 cycles, 2.4 GHz engine clock
-VALU_PEAK_WAVE_INSTR_PER_S = 256 * 4 * 2.4e9 / 4
+# VALU issue peak from the published FP32 vector peak (157.3 TFLOP/s = 256 CUs x 128 FMA lanes
+# x 2 x 2.4 GHz): 4 SIMDs of 32 lanes per CU, a wavefront64 instruction in 2 cycles. (At 4 cycles
+# the sweeps' counted instructions would need more than their measured launch times.)
+VALU_PEAK_WAVE_INSTR_PER_S = 256 * 4 * 2.4e9 / 2
 
 
 def measured_valu(workload: str, variant: str, phase: str):
@@ -461,8 +462,8 @@ def bench_single(args, w):
             "rebuild_us": reb_ms / max(reb_n, 1) * 1e3,
             "sweep_variant": variant,
             "autotune_ms": sim.autotune_ms(),
-            # the bound the sweeps actually meet: VALU issue (counted instructions at the issue
-            # peak, against the launch time measured here)
+            # VALU issue: counted instructions at the issue peak, against the launch time
+            # measured here
             "valu": None if valu is None else {
                 "instructions_per_launch": valu,
                 "peak_instructions_per_s": VALU_PEAK_WAVE_INSTR_PER_S,
@@ -827,8 +828,8 @@ def bench_sweep(args):
             # the autotuner's variant per size class (cells in [4^c, 4^(c+1))), members per class
             "size_class_variants": {str(c): v for c, v in sorted(class_variants.items())},
             "size_class_members": {str(c): member_class.count(c) for c in sorted(set(member_class))},
-            # the bound the sweeps meet: VALU issue (counted instructions at the issue peak,
-            # against the launch time measured here)
+            # VALU issue: counted instructions at the issue peak, against the launch time
+            # measured here
             "valu": None if valu is None else {
                 "instructions_per_launch": valu,
                 "peak_instructions_per_s": VALU_PEAK_WAVE_INSTR_PER_S,
