@@ -869,8 +869,8 @@ __global__ __launch_bounds__(kBlock) void k_sym_naive(SweepArgs a) {
         }
         if (blocked) continue;
       }
-      sym_eval<HAS_TYPE, HAS_POL, false>(a, in, me, pme, m, own ? kRoleOwn : kRoleGather, acc_pos,
-                                         acc_pol);
+      sym_eval_at<HAS_TYPE, HAS_POL, false>(a, in, me, pme, m, pm, own ? kRoleOwn : kRoleGather,
+                                            acc_pos, acc_pol);
       pairs += own;
     }
   }

@@ -398,7 +398,7 @@ __global__ __launch_bounds__(kBlock, 6) void k_sweep_team(SweepArgs a) {
       } else if (SYM) {
         // all-pairs, symmetric: own v(i, m) or gather -v(m, i), computed here
         float cp[3] = {0.0f, 0.0f, 0.0f}, cl[2] = {0.0f, 0.0f};
-        sym_eval<HAS_TYPE, HAS_POL>(a, in, me, pme, m, role, cp, cl);
+        sym_eval_at<HAS_TYPE, HAS_POL>(a, in, me, pme, m, pj, role, cp, cl);
         L.ent[team][pos] = make_float4(cp[0], cp[1], cp[2], role == kRoleGather ? 2.0f : 1.0f);
         if (HAS_POL) L.cpol[team][pos] = make_float2(cl[0], cl[1]);
       } else {
@@ -498,7 +498,8 @@ __global__ __launch_bounds__(kBlock, 6) void k_sweep_team(SweepArgs a) {
       float cp2[3] = {0.0f, 0.0f, 0.0f}, cl2[2] = {0.0f, 0.0f};
       const int erole = SYM ? (L.abit[team][e] >> kRoleShift) : kRoleOwn;
       if (SYM && !blocked) {
-        sym_eval<HAS_TYPE, HAS_POL>(a, in, me, pme, L.aslot[team][e], erole, cp, cl);
+        // the entry holds m's position (written by the candidate phase)
+        sym_eval_at<HAS_TYPE, HAS_POL>(a, in, me, pme, L.aslot[team][e], pj, erole, cp, cl);
       } else if (!blocked && MODE != kModeCount && MODE != kModeEmit) {
         const int m = L.aslot[team][e];
         const float disp[3] = {me.x - pj.x, me.y - pj.y, me.z - pj.z};

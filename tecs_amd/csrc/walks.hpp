@@ -172,10 +172,11 @@ __device__ void sym_walk_serial(const SweepArgs& a, const InstanceDesc& in, int 
       const int start = offb.at(x_lo, y, z);
       const int end = offb.at(x_hi + 1, y, z);
       for (int m = start; m < end; ++m) {
+        const float4 pm = xs[m];
         const int role = sym_role<GABRIEL, SCATTER>(in, a.skey, a.mover8, xs, offb, s, me_id, me_mover, me,
-                                                    m, xs[m], a.id[m], walk);
+                                                    m, pm, a.id[m], walk);
         if (role == kRoleNone) continue;
-        sym_eval<HAS_TYPE, HAS_POL>(a, in, me, pme, m, role, acc_pos, acc_pol);
+        sym_eval_at<HAS_TYPE, HAS_POL>(a, in, me, pme, m, pm, role, acc_pos, acc_pol);
         pairs += role != kRoleGather;
       }
     }
@@ -252,10 +253,11 @@ __device__ unsigned sym_movers_wave(const SweepArgs& a, const InstanceDesc& in, 
 #pragma unroll
         for (int r = 0; r < 9; ++r)
           if (k >= pre[r] && k < pre[r + 1]) m = rs[r] + (k - pre[r]);
+        const float4 pm = xs[m];
         const int role =
-            sym_role<GABRIEL, false>(in, a.skey, a.mover8, xs, offb, ms, mid, true, mme, m, xs[m], a.id[m], walk);
+            sym_role<GABRIEL, false>(in, a.skey, a.mover8, xs, offb, ms, mid, true, mme, m, pm, a.id[m], walk);
         if (role == kRoleNone) continue;
-        sym_eval<HAS_TYPE, HAS_POL>(a, in, mme, mpol, m, role, p, l);
+        sym_eval_at<HAS_TYPE, HAS_POL>(a, in, mme, mpol, m, pm, role, p, l);
         pairs += role != kRoleGather;
       }
     }
