@@ -774,7 +774,11 @@ struct Batch {
     }
   }
 
-  int rebuild() {
+  // keep_ov false: the step that follows writes every active cell's old velocity before any
+  // kernel reads one (no member has drag), so the old velocities are not moved: the new
+  // generation's active slots hold stale values until that write, inactive slots keep theirs
+  // in both generations.
+  int rebuild(bool keep_ov = true) {
     if (!is_binned(finder)) return TECS_OK;
     Pending p{};
     TRY(time_begin(&p, kTimeRebuild));
@@ -785,12 +789,12 @@ struct Batch {
     r.pol_old = pol[xc];
     r.trig_old = trig_of(pol[xc]);
     r.id_old = id[ic];
-    r.ov_old = ov[xc];
+    r.ov_old = keep_ov ? ov[xc] : nullptr;
     r.x_new = x[xc ^ 1];
     r.pol_new = pol[xc ^ 1];
     r.trig_new = trig_of(pol[xc ^ 1]);
     r.id_new = id[ic ^ 1];
-    r.ov_new = ov[xc ^ 1];
+    r.ov_new = keep_ov ? ov[xc ^ 1] : nullptr;
     r.kc = key_count();
     r.tmp = tmp;
     r.tkey = tkey;
@@ -822,12 +826,12 @@ struct Batch {
   }
 
   // The reference rebuild schedule for a full evaluation (binned_all_pairs.hpp:77-84).
-  int maybe_rebuild(bool is_full_step) {
+  int maybe_rebuild(bool is_full_step, bool keep_ov = true) {
     rebuilt_now = false;
     if (!is_binned(finder)) return TECS_OK;
     if (is_full_step) {
       if (step_count == 0 || step_count >= rebuild_every_n) {
-        TRY(rebuild());
+        TRY(rebuild(keep_ov));
         step_count = 0;
         rebuilt_now = true;
       }
@@ -1338,7 +1342,9 @@ struct Batch {
 
   // One integrator step (Euler::integrate euler.hpp:81-85, Heun::integrate heun.hpp:201-208).
   int step_once() {
-    TRY(maybe_rebuild(true));
+    // every path below ends in a write of each active cell's old velocity (the fused Heun
+    // corrector or Euler epilogue, or the update kernel); only drag reads them before that
+    TRY(maybe_rebuild(true, has_drag()));
     if (separate_update()) return step_delta_only();
     const bool com_on = com_fixer == TECS_GLOBAL_COM_FIXER;
     if (integrator == TECS_HEUN) {
