@@ -230,6 +230,36 @@ def test_rebuild_every_n_bit_exact(gpu, oracle_lib):
     assert g.stats()["rebuilds"] == o.stats()["rebuilds"] == 4
 
 
+@pytest.mark.parametrize("integrator", [abi.EULER, abi.HEUN], ids=["Euler", "Heun"])
+def test_old_velocities_across_rebuilds_outside_steps(gpu, oracle_lib, integrator):
+    """A step's own rebuild leaves the old velocities to the step's final write; rebuilds
+    outside a step (a full evaluation, a pair export) between steps move them with their
+    cells. Positions and old velocities stay bitwise the oracle's through both."""
+    w = _workload("sorting", 20000, abi.BINNED_ALL_PAIRS)
+    d = _with(w.desc, integrator=integrator)
+    print(f"case old velocities across rebuilds integrator={integrator}", flush=True)
+    g, o = _pair(oracle_lib, d, w)
+
+    def same(what):
+        xg, _, vg = g.download()
+        xo, _, vo = o.download()
+        assert np.array_equal(xg.view(np.uint32), xo.view(np.uint32)), f"positions differ {what}"
+        assert np.array_equal(vg.view(np.uint32), vo.view(np.uint32)), f"old velocities differ {what}"
+
+    for s in (g, o):
+        s.take_step(w.dt, 4)
+    same("after 4 steps")
+    for s in (g, o):
+        s.evaluate_force(True)
+    same("after an evaluation that rebuilt the grid")
+    for s in (g, o):
+        s.pairs(True)
+    same("after a pair export that rebuilt the grid")
+    for s in (g, o):
+        s.take_step(w.dt, 3)
+    same("after 3 more steps")
+
+
 def test_batch_member_equals_alone(gpu):
     """Every member of a batch ends bitwise where it ends when run alone."""
     ws = W.sweep(count=6, n_min=500, n_max=20000)
