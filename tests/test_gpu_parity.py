@@ -298,6 +298,41 @@ def test_dense_cluster_team_overflow_path(gpu, oracle_lib, finder, variant):
     assert np.array_equal(fg.view(np.uint32), fo.view(np.uint32))
 
 
+@pytest.mark.parametrize("shape", ["slab", "dense"])
+def test_thread_sweep_wide_and_dense_grids(gpu, oracle_lib, shape):
+    """The all-pairs thread sweep keeps its row ranges in LDS with 8-bit lengths; a workgroup
+    with a range of 256 slots or more ("dense": ~900 neighbours per cell, far more than the
+    deferred list holds) takes the row-by-row walk. "slab": a thin sheet of ~100k cells per
+    layer of bins, whose neighbour rows lie up to ~100k slots apart. Pairs and forces stay
+    bit-exact, and so do a few Heun steps."""
+    g0 = W.rng(24)
+    if shape == "slab":
+        n = 200_000
+        pos = np.stack([g0.uniform(-200.0, 200.0, n), g0.uniform(-200.0, 200.0, n),
+                        g0.uniform(-0.6, 0.6, n)], axis=1).astype(np.float32)
+    else:
+        n = 3000
+        pos = W.ball(n, 1.5, g0)
+    types = (g0.random(n) < 0.5).astype(np.uint8)
+    lo, hi = W.bounds(pos)
+    d = abi.make_desc(n, pair_finder=abi.BINNED_ALL_PAIRS, min_bounds=lo, max_bounds=hi,
+                      integrator=abi.HEUN, types=True, adhesion_table=[[1.0, 0.3], [0.3, 1.0]])
+    w = W.Workload(shape, d, pos, None, types, 0.01, 1)
+    print(f"case thread sweep row-walk fallback shape={shape} n={n}", flush=True)
+    g, o = _pair(oracle_lib, d, w, variant=abi.SWEEP_THREAD)
+    assert g.sweep_variant() == abi.SWEEP_THREAD
+    _assert_same_csr(g.pairs(True), o.pairs(True), shape)
+    fg, _ = g.evaluate_force(True)
+    fo, _ = o.evaluate_force(True)
+    assert np.array_equal(fg.view(np.uint32), fo.view(np.uint32))
+    for sim in (g, o):
+        sim.take_step(w.dt, 3)
+    xg, _, vg = g.download()
+    xo, _, vo = o.download()
+    assert np.array_equal(xg.view(np.uint32), xo.view(np.uint32))
+    assert np.array_equal(vg.view(np.uint32), vo.view(np.uint32))
+
+
 def test_team_variant_requires_search_radius_one(gpu):
     w = _workload("sorting", 5000, abi.BINNED_ALL_PAIRS)
     d = _with(w.desc, bin_size_scale=0.5)
