@@ -410,8 +410,8 @@ __device__ __forceinline__ void walk_binned_flat(const SweepArgs& a, const Insta
 template <bool HAS_POL>
 __device__ __forceinline__ CorrIn load_corr(const SweepArgs& a, int s) {
   CorrIn c{};
-  c.d1 = a.d1_pos[s];
-  c.x0 = a.o2_pos[s];
+  c.d1 = load_stream(a.d1_pos + s);
+  c.x0 = load_stream(a.o2_pos + s);
   if (HAS_POL) {
     c.e1 = a.d1_pol[s];
     c.p0 = a.o2_pol[s];
@@ -463,8 +463,9 @@ __device__ __forceinline__ void finish(const SweepArgs& a, const InstanceDesc& i
       a.o_pos[s] = make_float4(F[0], F[1], F[2], 0.0f);
       if (HAS_POL) a.o_pol[s] = make_float2(G[0], G[1]);
     } else if (MODE == kModePred) {
-      // apply_euler_predictor, heun.hpp:146-156: predicted = current + delta * dt
-      a.o_pos[s] = make_float4(F[0], F[1], F[2], 0.0f);
+      // apply_euler_predictor, heun.hpp:146-156: predicted = current + delta * dt (the
+      // delta is read once, by the corrector; the predicted positions by every walk)
+      store_stream(a.o_pos + s, make_float4(F[0], F[1], F[2], 0.0f));
       a.o2_pos[s] = make_float4(me.x + F[0] * dt, me.y + F[1] * dt, me.z + F[2] * dt, me.w);
       if (HAS_POL) {
         a.o_pol[s] = make_float2(G[0], G[1]);
@@ -476,10 +477,10 @@ __device__ __forceinline__ void finish(const SweepArgs& a, const InstanceDesc& i
       // apply_heun_corrector, heun.hpp:170-186
       const float4 d1 = ci.d1;
       const float s0 = d1.x + F[0], s1 = d1.y + F[1], s2 = d1.z + F[2];
-      if (a.old_vel) a.old_vel[s] = make_float4(s0 * 0.5f, s1 * 0.5f, s2 * 0.5f, 0.0f);
+      if (a.old_vel) store_stream(a.old_vel + s, make_float4(s0 * 0.5f, s1 * 0.5f, s2 * 0.5f, 0.0f));
       const float4 x0 = ci.x0;
       xnew = make_float4(x0.x + s0 * 0.5f * dt, x0.y + s1 * 0.5f * dt, x0.z + s2 * 0.5f * dt, x0.w);
-      a.o2_pos[s] = xnew;
+      store_stream(a.o2_pos + s, xnew);  // read once more, by the next rebuild
       if (HAS_POL) {
         const float2 e1 = ci.e1;
         const float2 p0 = ci.p0;

@@ -300,6 +300,19 @@ __device__ __forceinline__ void count_key(const InstanceDesc& in, const KeyCount
   count_key_of(in, k, s, cell, pad, cell ? bin_key(in, p) : 0, old_key);
 }
 
+// Streaming accesses: data a kernel touches once (the corrector's inputs, the step's outputs
+// that the next kernel reads once) is loaded and stored with the non-temporal hint, so that it
+// does not push out of the caches the positions the pair walks read many times over.
+typedef float StreamF4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 load_stream(const float4* p) {
+  const StreamF4 v = __builtin_nontemporal_load(reinterpret_cast<const StreamF4*>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void store_stream(float4* p, float4 v) {
+  const StreamF4 w = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(w, reinterpret_cast<StreamF4*>(p));
+}
+
 // ------------------------------------------------------------------- vector.hpp:312-343
 __device__ __forceinline__ float len2(float a, float b, float c) {
   float r = 0.0f;
