@@ -75,8 +75,10 @@ struct CellMove {
 };
 __device__ __forceinline__ CellMove load_cell(const RebuildArgs& a, int o) {
   CellMove c{};
-  c.x = a.x_old[o];
-  if (a.pol_new) c.pol = a.pol_old[o];
+  // read once here (streaming); the new slots are written with ordinary stores, as the sweeps
+  // read them next
+  c.x = load_stream(a.x_old + o);
+  if (a.pol_new) c.pol = load_stream(a.pol_old + o);
   if (a.ov_new) c.ov = a.ov_old[o];
   return c;
 }

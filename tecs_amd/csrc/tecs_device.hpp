@@ -308,6 +308,11 @@ __device__ __forceinline__ float4 load_stream(const float4* p) {
   const StreamF4 v = __builtin_nontemporal_load(reinterpret_cast<const StreamF4*>(p));
   return make_float4(v.x, v.y, v.z, v.w);
 }
+typedef float StreamF2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 load_stream(const float2* p) {
+  const StreamF2 v = __builtin_nontemporal_load(reinterpret_cast<const StreamF2*>(p));
+  return make_float2(v.x, v.y);
+}
 __device__ __forceinline__ void store_stream(float4* p, float4 v) {
   const StreamF4 w = {v.x, v.y, v.z, v.w};
   __builtin_nontemporal_store(w, reinterpret_cast<StreamF4*>(p));
