@@ -778,7 +778,8 @@ __global__ __launch_bounds__(kBlock) void k_sym_binned(SweepArgs a) {
     sym_cell_serial<GABRIEL, HAS_TYPE, HAS_POL, false, ROWS>(a, in, s, me_id, true, me, pme, bc,
                                                        acc_pos, acc_pol, pairs);
   if (active) {
-    a.o_pos[s] = make_float4(acc_pos[0], acc_pos[1], acc_pos[2], 0.0f);
+    // read once, by the update (streaming hint)
+    store_stream(a.o_pos + s, make_float4(acc_pos[0], acc_pos[1], acc_pos[2], 0.0f));
     if (HAS_POL) a.o_pol[s] = make_float2(acc_pol[0], acc_pol[1]);
   }
   const unsigned tp = block_sum_u(active ? pairs : 0u, lds_u);
@@ -876,7 +877,8 @@ __global__ __launch_bounds__(kBlock) void k_sym_naive(SweepArgs a) {
     }
   }
   if (active) {
-    a.o_pos[s] = make_float4(acc_pos[0], acc_pos[1], acc_pos[2], 0.0f);
+    // read once, by the update (streaming hint)
+    store_stream(a.o_pos + s, make_float4(acc_pos[0], acc_pos[1], acc_pos[2], 0.0f));
     if (HAS_POL) a.o_pol[s] = make_float2(acc_pol[0], acc_pol[1]);
   }
   const unsigned tp = block_sum_u(active ? pairs : 0u, lds_u);
@@ -1068,17 +1070,20 @@ __device__ __forceinline__ void update_cell(const UpdateArgs& a, const InstanceD
       if (ql.x != 0.0f || ql.y != 0.0f) a.q_pol[s] = make_float2(0.0f, 0.0f);
     }
   }
-  float4 d1 = a.d1_pos[s];
+  // every cell's state is read and written once here (streaming hints), except the predicted
+  // positions and mover flags, which the next sweep's walks read many times
+  const float4 d1_in = load_stream(a.d1_pos + s);
+  float4 d1 = d1_in;
   if (KIND != kUpdateCorr && a.q_pos) d1 = make_float4(d1.x + qv.x, d1.y + qv.y, d1.z + qv.z, 0.0f);
   d1 = make_float4(d1.x - com.x, d1.y - com.y, d1.z - com.z, 0.0f);
   if (KIND == kUpdatePred) {
     // heun.hpp:150-154
-    a.d1_pos[s] = d1;
+    store_stream(a.d1_pos + s, d1);
     if (HAS_POL && a.q_pos) {
       const float2 e = a.d1_pol[s];
       a.d1_pol[s] = make_float2(e.x + ql.x, e.y + ql.y);
     }
-    const float4 x = a.x[s];
+    const float4 x = load_stream(a.x + s);
     const float4 xp = make_float4(x.x + d1.x * dt, x.y + d1.y * dt, x.z + d1.z * dt, x.w);
     a.xp[s] = xp;
     if (a.mover8) a.mover8[s] = bin_key(in, xp) != a.mover_key[s] ? 1 : 0;
@@ -1090,15 +1095,15 @@ __device__ __forceinline__ void update_cell(const UpdateArgs& a, const InstanceD
     }
   } else if (KIND == kUpdateCorr) {
     // heun.hpp:174-183 (d1 was COM-fixed by the predictor; com here is the corrector's)
-    const float4 d1f = a.d1_pos[s];
-    float4 d3 = a.d3_pos[s];
+    const float4 d1f = d1_in;
+    float4 d3 = load_stream(a.d3_pos + s);
     if (a.q_pos) d3 = make_float4(d3.x + qv.x, d3.y + qv.y, d3.z + qv.z, 0.0f);
     d3 = make_float4(d3.x - com.x, d3.y - com.y, d3.z - com.z, 0.0f);
     const float s0 = d1f.x + d3.x, s1 = d1f.y + d3.y, s2 = d1f.z + d3.z;
-    if (a.old_vel) a.old_vel[s] = make_float4(s0 * 0.5f, s1 * 0.5f, s2 * 0.5f, 0.0f);
-    const float4 x = a.x[s];
+    if (a.old_vel) store_stream(a.old_vel + s, make_float4(s0 * 0.5f, s1 * 0.5f, s2 * 0.5f, 0.0f));
+    const float4 x = load_stream(a.x + s);
     xnew = make_float4(x.x + s0 * 0.5f * dt, x.y + s1 * 0.5f * dt, x.z + s2 * 0.5f * dt, x.w);
-    a.x[s] = xnew;
+    store_stream(a.x + s, xnew);
     if (HAS_POL) {
       float2 e3 = a.d3_pol[s];
       e3 = make_float2(e3.x + ql.x, e3.y + ql.y);  // ql = 0 without q
@@ -1109,10 +1114,10 @@ __device__ __forceinline__ void update_cell(const UpdateArgs& a, const InstanceD
     }
   } else {
     // euler.hpp:56-66
-    if (a.old_vel) a.old_vel[s] = d1;
-    const float4 x = a.x[s];
+    if (a.old_vel) store_stream(a.old_vel + s, d1);
+    const float4 x = load_stream(a.x + s);
     xnew = make_float4(x.x + d1.x * dt, x.y + d1.y * dt, x.z + d1.z * dt, x.w);
-    a.x[s] = xnew;
+    store_stream(a.x + s, xnew);
     if (HAS_POL) {
       float2 e = a.d1_pol[s];
       e = make_float2(e.x + ql.x, e.y + ql.y);
