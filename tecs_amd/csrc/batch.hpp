@@ -861,6 +861,16 @@ struct Batch {
   }
 
   // The concrete sweep: TECS_SWEEP_THREAD or TECS_SWEEP_TEAM4/8/16.
+  // Every launch of a force-mode sweep runs the thread variant (per size class when tuned so).
+  bool thread_sweeps_only() const {
+    const int v = concrete_variant();
+    if (!groups.empty() && variant == TECS_SWEEP_AUTO && v == tuned) {
+      for (const SweepGroup& g : groups)
+        if (g.variant != TECS_SWEEP_THREAD) return false;
+      return true;
+    }
+    return v == TECS_SWEEP_THREAD;
+  }
   int concrete_variant() const {
     if (!team_ok) return TECS_SWEEP_THREAD;
     switch (variant) {
@@ -1207,8 +1217,12 @@ struct Batch {
       a.ov_in = ov[xc];
     }
     TRY(check_sweep_args(mode, a));
+    // a symmetric Delta sweep with the predictor fused (o2_pos set): only the thread sweep
+    // fuses it, so every class takes that variant
+    if (mode == kModeDelta && is_symmetric(finder) && a.o2_pos) v = TECS_SWEEP_THREAD;
     const bool force_mode = mode == kModeDelta || mode == kModePred || mode == kModeCorr || mode == kModeEuler;
-    if (!groups.empty() && variant == TECS_SWEEP_AUTO && force_mode && v == tuned) {
+    if (!groups.empty() && variant == TECS_SWEEP_AUTO && force_mode && v == tuned &&
+        !(mode == kModeDelta && is_symmetric(finder) && a.o2_pos)) {
       // per size class: one launch per group of classes, each over its chunks
       for (const SweepGroup& g : groups) {
         SweepArgs ag = a;
@@ -1321,10 +1335,21 @@ struct Batch {
     a.o_pos = d1;
     a.o_pol = d1p;
     sym_grid_args(&a, stale1);
+    // the predictor fused into a fresh symmetric thread sweep: nothing else adds to the delta
+    // (no movers on a fresh grid, no COM fix, no walls or links)
+    const bool fuse_pred = integrator == TECS_HEUN && sym_binned && !stale1 &&
+                           com_fixer != TECS_GLOBAL_COM_FIXER && !has_extra() && !has_drag() &&
+                           thread_sweeps_only();
+    if (fuse_pred) {
+      a.o2_pos = xp;
+      a.o2_pol = polp;
+      a.o_trig = trigp;
+      a.mover8_out = mover8;
+    }
     TRY(sweep(kModeDelta, a, kTimeSweepFull));
     if (has_extra()) TRY(apply_extra(x[xc], d1, true));
     if (integrator == TECS_HEUN) {
-      TRY(com_and_update(kUpdatePred, d1, stale1, sym_binned));
+      if (!fuse_pred) TRY(com_and_update(kUpdatePred, d1, stale1, sym_binned));
       SweepArgs b = base_args(xp, polp);
       b.o_pos = d3;
       b.o_pol = d3p;

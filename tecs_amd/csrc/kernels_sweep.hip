@@ -781,6 +781,23 @@ __global__ __launch_bounds__(kBlock) void k_sym_binned(SweepArgs a) {
     // read once, by the update (streaming hint)
     store_stream(a.o_pos + s, make_float4(acc_pos[0], acc_pos[1], acc_pos[2], 0.0f));
     if (HAS_POL) a.o_pol[s] = make_float2(acc_pol[0], acc_pol[1]);
+    if (!STALE && a.o2_pos) {
+      // the Heun predictor fused (heun.hpp:146-156), asked for on a fresh grid without the COM
+      // fix: no cell moved out of its bin, so no contribution arrives after this walk and the
+      // delta is final. The same arithmetic as the separate update, and the flags of the cells
+      // the predicted positions take out of their rebuild bin (flatten(bc), the key the grid
+      // was just built with).
+      const float dt = in.dt;
+      const float4 xp = make_float4(me.x + acc_pos[0] * dt, me.y + acc_pos[1] * dt,
+                                    me.z + acc_pos[2] * dt, me.w);
+      a.o2_pos[s] = xp;
+      if (a.mover8_out) a.mover8_out[s] = bin_key(in, xp) != flatten(in, bc[0], bc[1], bc[2]) ? 1 : 0;
+      if (HAS_POL) {
+        const float2 np = make_float2(pme.t + acc_pol[0] * dt, pme.p + acc_pol[1] * dt);
+        a.o2_pol[s] = np;
+        a.o_trig[s] = polar_trig(np);
+      }
+    }
   }
   const unsigned tp = block_sum_u(active ? pairs : 0u, lds_u);
   if (threadIdx.x == 0) a.block_pairs[lb] += tp;

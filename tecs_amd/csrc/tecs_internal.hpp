@@ -68,6 +68,7 @@ struct SweepArgs {
   const int* skey;       // symmetric Delta: bin key of every slot at the last rebuild (null: the
                          // grid was built from these positions)
   const unsigned char* mover8;  // symmetric Delta on a stale grid: 1 for a slot outside its bin
+  unsigned char* mover8_out = nullptr;  // the fused symmetric predictor's flags of these positions
                                 // of the last rebuild (null: computed from skey per candidate)
   float4* q_pos;         // symmetric Delta on a stale grid: the movers' scattered contributions
   float2* q_pol;

@@ -240,6 +240,29 @@ def test_trajectory_tolerance(gpu, oracle_lib, finder, name, integrator, com):
     assert g.stats()["evaluations"] == o.stats()["evaluations"]
 
 
+@pytest.mark.parametrize("name", ["sorting", "epithelium"])
+def test_fused_predictor_thread_sweep(gpu, oracle_lib, name):
+    """Heun without the COM fix on the binned all-pairs symmetric thread sweep: the predictor
+    runs in the fresh sweep's epilogue (positions, polarities and the flags of the cells it
+    takes out of their rebuild bin), the stale sweep then routes the movers' pairs by those
+    flags. 20 steps against the oracle at the trajectory bar of this file."""
+    w = _workload(name, {"sorting": 20000, "epithelium": 4000}[name], abi.BINNED_ALL_PAIRS_SYMMETRIC)
+    d = abi.copy_desc(w.desc)
+    d.integrator = abi.HEUN
+    print(f"case sym fused predictor {name} n={w.n}", flush=True)
+    g, o = _pair(oracle_lib, d, w)
+    g.set_sweep_variant(abi.SWEEP_THREAD)
+    for s in (g, o):
+        s.take_step(w.dt, 20)
+    xg, pg, vg = g.download()
+    xo, po, vo = o.download()
+    _assert_close_states(xg, xo, "positions")
+    _assert_close_states(vg, vo, "old velocities")
+    if pg is not None:
+        _assert_close_states(pg, po, "polarities")
+    assert g.stats()["steps"] == o.stats()["steps"] == 20
+
+
 def test_graph_replay_matches_eager(gpu):
     """Steps replayed from captured graphs end where kernel-by-kernel launches end, to the
     summation-order tolerance; the state that zeroes the deltas before the first step is
